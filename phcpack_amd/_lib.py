@@ -1,0 +1,130 @@
+"""ctypes binding of libphcmd.so (the HIP library) with declared signatures.
+
+The library must have been built (``python -m phcpack_amd.build``); there is no
+CPU fallback: every compute entry point runs on the GPU or raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libphcmd.so")
+
+c_int = ctypes.c_int
+c_double = ctypes.c_double
+c_size_t = ctypes.c_size_t
+c_void_p = ctypes.c_void_p
+c_char_p = ctypes.c_char_p
+P_int = ctypes.POINTER(ctypes.c_int)
+P_double = ctypes.POINTER(ctypes.c_double)
+P_void_p = ctypes.POINTER(ctypes.c_void_p)
+P_float = ctypes.POINTER(ctypes.c_float)
+
+# name -> (restype, argtypes); every symbol of include/phcmd.h
+SIGNATURES = {
+    "phc_last_error": (c_char_p, []),
+    "phc_version": (c_int, []),
+    "phc_device_count": (c_int, [P_int]),
+    "phc_malloc": (c_int, [P_void_p, c_size_t]),
+    "phc_free": (c_int, [c_void_p]),
+    "phc_memcpy_htod": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "phc_memcpy_dtoh": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "phc_memset": (c_int, [c_void_p, c_int, c_size_t]),
+    "phc_synchronize": (c_int, []),
+    "phc_stream_synchronize": (c_int, [c_void_p]),
+    "phc_event_create": (c_int, [P_void_p]),
+    "phc_event_destroy": (c_int, [c_void_p]),
+    "phc_event_record": (c_int, [c_void_p, c_void_p]),
+    "phc_event_elapsed_ms": (c_int, [c_void_p, c_void_p, P_float]),
+    "phc_md_batch": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "phc_series_product": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                   c_void_p]),
+    "phc_poly_create": (c_int, [P_void_p, c_int, c_int, c_int, c_int, c_int, P_int, P_int,
+                                P_double, P_double]),
+    "phc_poly_destroy": (c_int, [c_void_p]),
+    "phc_poly_evaldiff": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+}
+
+_lib = None
+
+
+class PhcError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libphcmd.so once; raise if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PhcError(f"{LIB_PATH} is missing: build it with `python -m phcpack_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status):
+    if status != 0:
+        raise PhcError(load().phc_last_error().decode())
+    return status
+
+
+def ptr_i(a):
+    return a.ctypes.data_as(P_int)
+
+
+def ptr_d(a):
+    return a.ctypes.data_as(P_double)
+
+
+class DeviceArray:
+    """A float64/int32 device buffer owned by the HIP library."""
+
+    def __init__(self, shape, dtype=np.float64):
+        lib = load()
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = c_void_p()
+        check(lib.phc_malloc(ctypes.byref(p), max(self.nbytes, 8)))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        check(load().phc_memcpy_htod(d.ptr, a.ctypes.data_as(c_void_p), d.nbytes))
+        return d
+
+    @classmethod
+    def zeros(cls, shape, dtype=np.float64):
+        d = cls(shape, dtype)
+        check(load().phc_memset(d.ptr, 0, d.nbytes))
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        check(load().phc_memcpy_htod(self.ptr, a.ctypes.data_as(c_void_p), self.nbytes))
+
+    def to_host(self):
+        out = np.empty(self.shape, self.dtype)
+        check(load().phc_memcpy_dtoh(out.ctypes.data_as(c_void_p), self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            load().phc_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
