@@ -78,6 +78,81 @@ int phc_poly_destroy(phc_poly *plan);
 int phc_poly_evaldiff(phc_poly *plan, int nbatch, const double *input_dev, double *output_dev,
                       void *stream);
 
+/* --------------------------------------------------------------- tracking */
+
+/* Compiled homotopy H_i(x,t) = sum_r w_r(t) sum_m c[r,i,m] x^m (see
+ * phcpack_amd/homotopy.py for the construction).  Nodes 0..n-1 are the
+ * variables, node n the constant 1, node n+1+i = node left[i] * variable
+ * right[i]; product nodes are grouped by DAG level (lev_ptr[nlev+1]).
+ * off[], slen[]: storage of every node's series coefficients (complex units);
+ * variables and factor nodes keep deg+1 coefficients, others a ring of
+ * wdeg+1.  Pairs (component, node) carry weighted values u = sum_l W_l
+ * node_{k-l}; residual terms (pair, coefficient) are grouped in segments
+ * (seg_ptr) and equations (eq_seg); Jacobian entry i*n+j sums jterms
+ * (jpair, coefficient*exponent) with jpair values W_0 * node_0.
+ * Coefficient arrays are complex octo-double planes [2][8][count] (the
+ * leading K limbs are used at precision K); weights [2][8][3*ncomp] hold
+ * the coefficients of 1, t, t^2 of every w_r.
+ * Replaces the homotopy set-up of gpumanypaths_dd
+ * (Ada/PHCtoC/gpumanypaths_dd.cpp:18-57, which reads PHCpack's containers)
+ * and the per-call job planning of GPU_dbl2_poly_evaldiff. */
+typedef struct {
+  int n, deg, nprod, nlev, nslots, ncomp, wdeg;
+  const int *left, *right, *lev_ptr, *off, *slen;
+  int npair;
+  const int *pair_comp, *pair_node;
+  int nseg, nterm;
+  const int *seg_ptr, *eq_seg, *term_pair;
+  const double *term_coef;
+  int njpair, njterm;
+  const int *jpair_comp, *jpair_node, *jent_ptr, *jterm_pair;
+  const double *jterm_coef;
+  const double *weights;
+} phc_program;
+
+/* Step control of the tracker; defaults of the reference
+ * (Deformations/Curves/homotopy_continuation_parameters.adb:11-22):
+ * maxstep 0.1, minstep 1e-6, pbeta 0.5, alpha 1e-3, corsteps 4, maxsteps 1000. */
+typedef struct {
+  int maxsteps;     /* steps per path before giving up */
+  int corsteps;     /* Newton iterations per corrector */
+  int endsteps;     /* Newton iterations at t = 1 */
+  int fabry_offset; /* pole radius |c_{d-1-o}/c_{d-o}| */
+  double maxstep, minstep, pbeta, alpha;
+  double tolcor;    /* corrector: max|dx| <= tolcor*(1+max|x|) */
+  double tolend;    /* endpoint Newton tolerance */
+  double divergence;/* path declared diverged when max|x| exceeds this */
+} phc_track_params;
+
+typedef struct phc_tracker phc_tracker;
+int phc_tracker_create(phc_tracker **tracker, const phc_program *program);
+int phc_tracker_destroy(phc_tracker *tracker);
+/* LDS bytes per path at precision K (1, 2, 4, 8) */
+int phc_tracker_lds_bytes(const phc_tracker *tracker, int K, size_t *bytes);
+
+/* Track npaths paths from t = 0 to t = 1 in precision K (1 = double,
+ * 2 = double-double) with a persistent kernel (one workgroup of `block`
+ * threads per path, paths taken from an atomic queue).
+ * start [2*KS][npaths][n] (KS limbs), endp [2*K][npaths][n]; per path:
+ * tend, status (1 success, 2 step below minstep, 3 maxsteps, 4 singular
+ * Jacobian, 5 diverged), nsteps, nrej (rejected steps), dxlast.
+ * Replaces GPU_Path_mult / path_mult (Path/Path_GPU/path_gpu_mult.cu:84-383)
+ * behind the gpumanypaths_dd entry (Ada/PHCtoC/gpumanypaths_dd.cpp:18). */
+int phc_track(phc_tracker *tracker, int K, int npaths, const double *start_dev, int KS,
+              double *endp_dev, double *tend_dev, int *status_dev, int *nsteps_dev,
+              int *nrej_dev, double *dxlast_dev, const phc_track_params *params, int block,
+              void *stream);
+
+/* Newton refinement at fixed t in precision K (1, 2, 4, 8) of points with KS
+ * limbs; xout [2*K][npaths][n]; res = max_i |H_i(xout, t)| evaluated in
+ * precision K; status 1 converged (max|dx| <= tol*(1+max|x|)), 0 not, -1
+ * singular.  iters = 0 only evaluates the residual.
+ * Plays the role of the Newton drivers at the endpoints
+ * (Newton/dbl2_newton_method.cpp:429, row Newton step) in dd/qd/od. */
+int phc_refine(phc_tracker *tracker, int K, int npaths, double t, const double *xin_dev, int KS,
+               double *xout_dev, int iters, double tol, double *dxlast_dev, double *res_dev,
+               int *status_dev, int block, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,53 @@
+/* CPU oracle: path tracking of compiled homotopies (test infrastructure only).
+ *
+ * The program and parameter structs have the layout of phc_program and
+ * phc_track_params in include/phcmd.h so the same host data feed both. */
+#ifndef PHC_ORACLE_TRACK_H
+#define PHC_ORACLE_TRACK_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int n, deg, nprod, nlev, nslots, ncomp, wdeg;
+  const int *left, *right, *lev_ptr, *off, *slen;
+  int npair;
+  const int *pair_comp, *pair_node;
+  int nseg, nterm;
+  const int *seg_ptr, *eq_seg, *term_pair;
+  const double *term_coef;
+  int njpair, njterm;
+  const int *jpair_comp, *jpair_node, *jent_ptr, *jterm_pair;
+  const double *jterm_coef;
+  const double *weights;
+} otr_program;
+
+typedef struct {
+  int maxsteps, corsteps, endsteps, fabry_offset;
+  double maxstep, minstep, pbeta, alpha, tolcor, tolend, divergence;
+} otr_params;
+
+/* paths p0 <= p < p1 of npaths; arrays as in phc_track */
+void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const double *start,
+               int KS, double *endp, double *tend, int *status, int *nsteps, int *nrej,
+               double *dxlast, const otr_params *prm);
+
+/* same as otr_track over all paths with nthreads POSIX threads */
+void otr_track_mt(const otr_program *P, int K, int npaths, const double *start, int KS,
+                  double *endp, double *tend, int *status, int *nsteps, int *nrej,
+                  double *dxlast, const otr_params *prm, int nthreads);
+
+void otr_refine(const otr_program *P, int K, int npaths, double t, const double *xin, int KS,
+                double *xout, int iters, double tol, double *dxlast, double *res, int *status);
+
+/* value and Jacobian of H at (x, t) for one point: x [2K][n] planar,
+ * val [2K][n], jac [2K][n*n] */
+void otr_eval(const otr_program *P, int K, double t, const double *x, double *val, double *jac);
+
+/* Taylor coefficients of the solution curve at (x, t): series [2K][n][deg+1] */
+int otr_series(const otr_program *P, int K, double t, const double *x, double *series);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,173 @@
+"""Host side of the batched path tracker: compiled homotopies on the device,
+tracking, endpoint refinement and residuals through the C-ABI.
+
+The interface mirrors the reference's batched tracker entry
+gpumanypaths_dd(mode, verbose, regamma, imgamma)
+(Ada/PHCtoC/gpumanypaths_dd.cpp:18-57) and its Path_Parameters
+(Ada/PHCtoC/path_parameters.ads:14-60), reduced to what the series tracker
+uses, with the step-control defaults of
+Deformations/Curves/homotopy_continuation_parameters.adb:11-22.
+"""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import DeviceArray, P_double, P_int, PhcProgram, PhcTrackParams, check, load
+from .homotopy import compile_homotopy
+
+STATUS = {0: "running", 1: "success", 2: "step below minstep", 3: "maxsteps", 4: "singular",
+          5: "diverged"}
+
+
+@dataclass
+class TrackParams:
+    maxsteps: int = 1000
+    corsteps: int = 4
+    endsteps: int = 8
+    fabry_offset: int = 2
+    maxstep: float = 0.1
+    minstep: float = 1e-6
+    pbeta: float = 0.5
+    alpha: float = 1e-3
+    tolcor: float = 1e-8
+    tolend: float = 1e-28
+    divergence: float = 1e8
+
+    def to_c(self):
+        return PhcTrackParams(self.maxsteps, self.corsteps, self.endsteps, self.fabry_offset,
+                              self.maxstep, self.minstep, self.pbeta, self.alpha, self.tolcor,
+                              self.tolend, self.divergence)
+
+
+def program_struct(prog):
+    """ctypes phc_program over the numpy arrays of a Program (the arrays are
+    kept alive on the returned object)."""
+    keep = {}
+
+    def ia(a):
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        if a.size == 0:
+            a = np.zeros(1, np.int32)
+        keep[id(a)] = a
+        return a.ctypes.data_as(P_int)
+
+    def da(a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep[id(a)] = a
+        return a.ctypes.data_as(P_double)
+
+    s = PhcProgram(
+        prog.n, prog.deg, len(prog.left), len(prog.lev_ptr) - 1, prog.nslots, prog.ncomp, prog.wdeg,
+        ia(prog.left), ia(prog.right), ia(prog.lev_ptr), ia(prog.off), ia(prog.slen),
+        len(prog.pair_comp), ia(prog.pair_comp), ia(prog.pair_node),
+        len(prog.seg_ptr) - 1, len(prog.term_pair), ia(prog.seg_ptr), ia(prog.eq_seg),
+        ia(prog.term_pair), da(prog.term_coef),
+        len(prog.jpair_comp), len(prog.jterm_pair), ia(prog.jpair_comp), ia(prog.jpair_node),
+        ia(prog.jent_ptr), ia(prog.jterm_pair), da(prog.jterm_coef), da(prog.weights))
+    s._keep = keep
+    return s
+
+
+def points_to_planes(x, K=1):
+    """complex128 [npaths][n] -> [2K][npaths][n] limb planes (exact)."""
+    x = np.asarray(x, dtype=np.complex128)
+    out = np.zeros((2 * K,) + x.shape)
+    out[0] = x.real
+    out[K] = x.imag
+    return out
+
+
+def planes_to_points(p, K):
+    """[2K][npaths][n] -> complex128 (rounded to double)."""
+    return p[:K].sum(axis=0) + 1j * p[K:].sum(axis=0)
+
+
+class DeviceProgram:
+    """A compiled homotopy uploaded to the device (phc_tracker handle)."""
+
+    def __init__(self, prog):
+        lib = load()
+        self.prog = prog
+        self.cstruct = program_struct(prog)
+        h = ctypes.c_void_p()
+        check(lib.phc_tracker_create(ctypes.byref(h), ctypes.byref(self.cstruct)))
+        self.handle = h
+
+    def lds_bytes(self, K):
+        b = ctypes.c_size_t()
+        check(load().phc_tracker_lds_bytes(self.handle, K, ctypes.byref(b)))
+        return b.value
+
+    def __del__(self):
+        try:
+            if self.handle:
+                load().phc_tracker_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class TrackBuffers:
+    """Device outputs of one tracking run."""
+
+    def __init__(self, npaths, n, K):
+        self.endp = DeviceArray((2 * K, npaths, n))
+        self.tend = DeviceArray((npaths,))
+        self.status = DeviceArray((npaths,), np.int32)
+        self.nsteps = DeviceArray((npaths,), np.int32)
+        self.nrej = DeviceArray((npaths,), np.int32)
+        self.dx = DeviceArray((npaths,))
+
+    def to_host(self):
+        return dict(endp=self.endp.to_host(), tend=self.tend.to_host(),
+                    status=self.status.to_host(), nsteps=self.nsteps.to_host(),
+                    nrej=self.nrej.to_host(), dx=self.dx.to_host())
+
+
+class Tracker:
+    """Batched tracker for one homotopy (predictor series degree deg)."""
+
+    def __init__(self, hom, deg=8):
+        self.hom = hom
+        self.n = hom.n
+        self.deg = deg
+        self.prog = compile_homotopy(hom, deg)
+        self.prog0 = compile_homotopy(hom, 0)
+        self.dev = DeviceProgram(self.prog)
+        self.dev0 = DeviceProgram(self.prog0)
+
+    def track_device(self, d_start, KS, npaths, K, bufs, params=None, block=64, stream=None):
+        p = (params or TrackParams()).to_c()
+        check(load().phc_track(self.dev.handle, K, npaths, d_start.ptr, KS, bufs.endp.ptr,
+                               bufs.tend.ptr, bufs.status.ptr, bufs.nsteps.ptr, bufs.nrej.ptr,
+                               bufs.dx.ptr, ctypes.byref(p), block, stream))
+
+    def track(self, starts, K=2, params=None, block=64):
+        """starts: complex128 [npaths][n] (or [2KS][npaths][n] planes)."""
+        planes = starts if starts.dtype == np.float64 else points_to_planes(starts)
+        KS = planes.shape[0] // 2
+        npaths = planes.shape[1]
+        d_start = DeviceArray.from_host(planes)
+        bufs = TrackBuffers(npaths, self.n, K)
+        self.track_device(d_start, KS, npaths, K, bufs, params, block)
+        check(load().phc_synchronize())
+        return bufs.to_host()
+
+    def refine_device(self, d_x, KS, npaths, K, d_out, d_dx, d_res, d_st, iters, tol, t=1.0,
+                      block=64, stream=None):
+        check(load().phc_refine(self.dev0.handle, K, npaths, t, d_x.ptr, KS, d_out.ptr, iters,
+                                tol, d_dx.ptr, d_res.ptr, d_st.ptr, block, stream))
+
+    def refine(self, planes, K, iters, tol, t=1.0, block=64):
+        """Newton at t in precision K from [2KS][npaths][n] planes."""
+        KS = planes.shape[0] // 2
+        npaths = planes.shape[1]
+        d_x = DeviceArray.from_host(planes)
+        d_out = DeviceArray((2 * K, npaths, self.n))
+        d_dx, d_res = DeviceArray((npaths,)), DeviceArray((npaths,))
+        d_st = DeviceArray((npaths,), np.int32)
+        self.refine_device(d_x, KS, npaths, K, d_out, d_dx, d_res, d_st, iters, tol, t, block)
+        check(load().phc_synchronize())
+        return dict(x=d_out.to_host(), dx=d_dx.to_host(), res=d_res.to_host(),
+                    status=d_st.to_host())

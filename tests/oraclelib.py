@@ -75,3 +75,83 @@ def ulp_err(a, b, K):
     diff = np.abs(np.sum(a - b, axis=0))
     scale = np.maximum(np.abs(a[0]), 1e-300)
     return float(np.max(diff / scale)) / (2.0 ** -52) ** K
+
+
+# ----------------------------------------------------------------- tracker --
+
+def _track_sigs():
+    import ctypes as C
+    from phcpack_amd._lib import PhcProgram, PhcTrackParams
+    o = oracle()
+    o.otr_track.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_int, C.c_int, C.c_int, P_d, C.c_int,
+                            P_d, P_d, P_i, P_i, P_i, P_d, C.POINTER(PhcTrackParams)]
+    o.otr_track_mt.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_int, P_d, C.c_int, P_d, P_d,
+                               P_i, P_i, P_i, P_d, C.POINTER(PhcTrackParams), C.c_int]
+    o.otr_refine.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_int, C.c_double, P_d, C.c_int,
+                             P_d, C.c_int, C.c_double, P_d, P_d, P_i]
+    o.otr_eval.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_double, P_d, P_d, P_d]
+    o.otr_series.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_double, P_d, P_d]
+    o.otr_series.restype = C.c_int
+    return o
+
+
+def oracle_track(prog, starts_planes, K, params, nthreads=1, paths=None):
+    """Track with the C oracle; starts_planes [2KS][npaths][n]."""
+    import ctypes as C
+    from phcpack_amd.tracker import program_struct
+    o = _track_sigs()
+    ps = program_struct(prog)
+    st = np.ascontiguousarray(starts_planes, dtype=np.float64)
+    KS, npaths, n = st.shape[0] // 2, st.shape[1], st.shape[2]
+    out = dict(endp=np.zeros((2 * K, npaths, n)), tend=np.zeros(npaths),
+               status=np.zeros(npaths, np.int32), nsteps=np.zeros(npaths, np.int32),
+               nrej=np.zeros(npaths, np.int32), dx=np.zeros(npaths))
+    p = params.to_c()
+    if nthreads > 1:
+        o.otr_track_mt(C.byref(ps), K, npaths, pd(st), KS, pd(out["endp"]), pd(out["tend"]),
+                       pi(out["status"]), pi(out["nsteps"]), pi(out["nrej"]), pd(out["dx"]),
+                       C.byref(p), nthreads)
+    else:
+        p0, p1 = (0, npaths) if paths is None else paths
+        o.otr_track(C.byref(ps), K, npaths, p0, p1, pd(st), KS, pd(out["endp"]), pd(out["tend"]),
+                    pi(out["status"]), pi(out["nsteps"]), pi(out["nrej"]), pd(out["dx"]),
+                    C.byref(p))
+    return out
+
+
+def oracle_refine(prog, planes, K, iters, tol, t=1.0):
+    import ctypes as C
+    from phcpack_amd.tracker import program_struct
+    o = _track_sigs()
+    ps = program_struct(prog)
+    x = np.ascontiguousarray(planes, dtype=np.float64)
+    KS, npaths, n = x.shape[0] // 2, x.shape[1], x.shape[2]
+    out = dict(x=np.zeros((2 * K, npaths, n)), dx=np.zeros(npaths), res=np.zeros(npaths),
+               status=np.zeros(npaths, np.int32))
+    o.otr_refine(C.byref(ps), K, npaths, t, pd(x), KS, pd(out["x"]), iters, tol, pd(out["dx"]),
+                 pd(out["res"]), pi(out["status"]))
+    return out
+
+
+def oracle_eval(prog, x_planes, K, t):
+    """value [2K][n] and Jacobian [2K][n*n] of the program at one point [2K][n]"""
+    import ctypes as C
+    from phcpack_amd.tracker import program_struct
+    o = _track_sigs()
+    ps = program_struct(prog)
+    n = prog.n
+    x = np.ascontiguousarray(x_planes, dtype=np.float64)
+    val, jac = np.zeros((2 * K, n)), np.zeros((2 * K, n * n))
+    o.otr_eval(C.byref(ps), K, t, pd(x), pd(val), pd(jac))
+    return val, jac
+
+
+def oracle_series(prog, x_planes, K, t):
+    import ctypes as C
+    from phcpack_amd.tracker import program_struct
+    o = _track_sigs()
+    ps = program_struct(prog)
+    x = np.ascontiguousarray(x_planes, dtype=np.float64)
+    s = np.zeros((2 * K, prog.n, prog.deg + 1))
+    ok = o.otr_series(C.byref(ps), K, t, pd(x), pd(s))
+    return s, ok

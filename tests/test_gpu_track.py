@@ -1,0 +1,84 @@
+"""GPU tracker and endpoint refinement against the C oracle.  Kernel and
+oracle evaluate the same compiled program in the same operation order, so
+endpoints, step counts and statuses must agree exactly."""
+import numpy as np
+import pytest
+
+from oraclelib import oracle_refine, oracle_track
+
+pytestmark = pytest.mark.gpu
+
+
+def make(n, degree, seed, rank_deficient=0, k=1):
+    from phcpack_amd.homotopy import dense_system, total_degree_homotopy
+    rng = np.random.default_rng(seed)
+    p = dense_system(n, degree, rng, rank_deficient)
+    hom, sols = total_degree_homotopy(p, rng, k=k)
+    return p, hom, sols
+
+
+@pytest.mark.parametrize("K", [1, 2])
+@pytest.mark.parametrize("n,degree,seed", [(3, 2, 1), (4, 2, 2), (3, 3, 3)])
+def test_track_matches_oracle(K, n, degree, seed):
+    from phcpack_amd.tracker import Tracker, TrackParams, points_to_planes
+    p, hom, sols = make(n, degree, seed)
+    prm = TrackParams(tolend=1e-13 if K == 1 else 1e-28)
+    tr = Tracker(hom, deg=8)
+    print(f"track K={K} n={n} degree={degree} seed={seed} paths={len(sols)}", flush=True)
+    g = tr.track(sols, K=K, params=prm)
+    o = oracle_track(tr.prog, points_to_planes(sols), K, prm)
+    assert np.array_equal(g["status"], o["status"])
+    assert np.array_equal(g["nsteps"], o["nsteps"])
+    assert np.array_equal(g["nrej"], o["nrej"])
+    assert np.array_equal(g["tend"], o["tend"])
+    assert np.array_equal(g["endp"], o["endp"])
+    assert np.all(g["status"] == 1)
+
+
+@pytest.mark.parametrize("block", [64, 128])
+def test_track_block_sizes_agree(block):
+    from phcpack_amd.tracker import Tracker, TrackParams
+    p, hom, sols = make(4, 2, 5)
+    tr = Tracker(hom, deg=8)
+    prm = TrackParams()
+    a = tr.track(sols, K=2, params=prm, block=64)
+    b = tr.track(sols, K=2, params=prm, block=block)
+    assert np.array_equal(a["endp"], b["endp"])
+
+
+@pytest.mark.parametrize("K", [2, 4, 8])
+def test_refine_matches_oracle(K):
+    from phcpack_amd.tracker import Tracker, TrackParams
+    p, hom, sols = make(4, 2, 7)
+    tr = Tracker(hom, deg=8)
+    g = tr.track(sols, K=1, params=TrackParams(tolend=1e-13))
+    x = g["endp"]
+    r = tr.refine(x, K, iters=3, tol=1e-60)
+    o = oracle_refine(tr.prog0, x, K, iters=3, tol=1e-60)
+    for key in ("x", "dx", "res", "status"):
+        assert np.array_equal(r[key], o[key]), key
+    assert r["res"].max() < {2: 1e-28, 4: 1e-55, 8: 1e-100}[K]
+
+
+def test_decoupled_closed_form():
+    """x_i^2 = a_i: every endpoint is a sign pattern of sqrt(a)."""
+    import itertools
+    from phcpack_amd.homotopy import System, total_degree_homotopy
+    from phcpack_amd.tracker import Tracker, TrackParams, planes_to_points
+    rng = np.random.default_rng(11)
+    n = 4
+    a = np.exp(2j * np.pi * rng.uniform(size=n)) * rng.uniform(0.5, 2.0, n)
+    eqs = []
+    for i in range(n):
+        e = [0] * n
+        e[i] = 2
+        eqs.append([(1.0 + 0j, tuple(e)), (complex(-a[i]), (0,) * n)])
+    hom, sols = total_degree_homotopy(System(n, eqs), rng)
+    g = Tracker(hom, deg=8).track(sols, K=2, params=TrackParams())
+    x = planes_to_points(g["endp"], 2)
+    want = np.array([np.array(s) * np.sqrt(a) for s in itertools.product([1, -1], repeat=n)])
+    for xi in x:
+        assert np.min(np.abs(want - xi).max(axis=1)) < 1e-14
+    # all sign patterns reached
+    got = {tuple(np.sign(np.round((xi / np.sqrt(a)).real)).astype(int)) for xi in x}
+    assert len(got) == 2 ** n
