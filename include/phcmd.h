@@ -124,6 +124,10 @@ typedef struct {
   double divergence;/* path declared diverged when max|x| exceeds this */
 } phc_track_params;
 
+/* Checks every index of a program against its sizes (no device work);
+ * phc_tracker_create calls it first. */
+int phc_program_validate(const phc_program *program);
+
 typedef struct phc_tracker phc_tracker;
 int phc_tracker_create(phc_tracker **tracker, const phc_program *program);
 int phc_tracker_destroy(phc_tracker *tracker);

@@ -72,6 +72,7 @@ class PhcTrackParams(ctypes.Structure):
 SIGNATURES.update({
     "phc_tracker_create": (c_int, [ctypes.POINTER(c_void_p), ctypes.POINTER(PhcProgram)]),
     "phc_tracker_destroy": (c_int, [c_void_p]),
+    "phc_program_validate": (c_int, [ctypes.POINTER(PhcProgram)]),
     "phc_tracker_lds_bytes": (c_int, [c_void_p, c_int, ctypes.POINTER(ctypes.c_size_t)]),
     "phc_track": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                           c_void_p, c_void_p, c_void_p, ctypes.POINTER(PhcTrackParams), c_int,

@@ -529,43 +529,70 @@ int occupancy_grid(const void *fn, int block, size_t lds, int npaths) {
 
 extern "C" {
 
+int phc_program_validate(const phc_program *d) {
+  PHC_REQUIRE(d != nullptr, "phc_program_validate: null program");
+  PHC_REQUIRE(d->n >= 1 && d->n <= 64, "phc_program: n must be in 1..64");
+  PHC_REQUIRE(d->deg >= 0 && d->deg <= 32, "phc_program: deg must be in 0..32");
+  PHC_REQUIRE(d->wdeg >= 0 && d->wdeg <= 2, "phc_program: weight degree must be <= 2");
+  PHC_REQUIRE(d->ncomp >= 1 && d->nprod >= 0 && d->nlev >= 0 && d->npair >= 0 && d->njpair >= 0,
+              "phc_program: negative count");
+  PHC_REQUIRE(d->nseg >= d->n && d->nterm >= 0 && d->njterm >= 0, "phc_program: bad counts");
+  const int nn = d->n + 1 + d->nprod;
+  const int need = d->wdeg < d->deg ? d->wdeg : d->deg;  // orders k-l kept besides k
+  for (int i = 0; i < d->nprod; ++i)
+    PHC_REQUIRE(d->left[i] >= 0 && d->left[i] < d->n + 1 + i && d->left[i] != d->n &&
+                d->right[i] >= 0 && d->right[i] < d->n,
+                "phc_program: product node factors must be earlier nodes, right a variable");
+  for (int i = 0; i < nn; ++i)
+    PHC_REQUIRE(d->off[i] >= 0 && d->slen[i] >= 1 && d->off[i] + d->slen[i] <= d->nslots,
+                "phc_program: node storage out of range");
+  for (int i = 0; i <= d->n; ++i)
+    PHC_REQUIRE(d->slen[i] == d->deg + 1, "phc_program: variables need full series storage");
+  for (int i = 0; i < d->nprod; ++i)
+    PHC_REQUIRE(d->slen[d->left[i]] == d->deg + 1, "phc_program: factor nodes need full series storage");
+  for (int i = 0; i < d->npair; ++i)
+    PHC_REQUIRE(d->pair_comp[i] >= 0 && d->pair_comp[i] < d->ncomp && d->pair_node[i] >= 0 &&
+                d->pair_node[i] < nn && d->slen[d->pair_node[i]] > need,
+                "phc_program: bad component pair");
+  for (int i = 0; i < d->njpair; ++i)
+    PHC_REQUIRE(d->jpair_comp[i] >= 0 && d->jpair_comp[i] < d->ncomp && d->jpair_node[i] >= 0 &&
+                d->jpair_node[i] < nn, "phc_program: bad Jacobian pair");
+  PHC_REQUIRE(d->seg_ptr[0] == 0 && d->seg_ptr[d->nseg] == d->nterm && d->eq_seg[0] == 0 &&
+              d->eq_seg[d->n] == d->nseg, "phc_program: bad residual segments");
+  for (int s = 0; s < d->nseg; ++s)
+    PHC_REQUIRE(d->seg_ptr[s + 1] >= d->seg_ptr[s], "phc_program: segments not increasing");
+  for (int i = 0; i < d->n; ++i)
+    PHC_REQUIRE(d->eq_seg[i + 1] > d->eq_seg[i], "phc_program: equation without segments");
+  for (int i = 0; i < d->nterm; ++i)
+    PHC_REQUIRE(d->term_pair[i] >= 0 && d->term_pair[i] < d->npair, "phc_program: bad term");
+  PHC_REQUIRE(d->jent_ptr[0] == 0 && d->jent_ptr[d->n * d->n] == d->njterm,
+              "phc_program: bad Jacobian entries");
+  for (int e = 0; e < d->n * d->n; ++e)
+    PHC_REQUIRE(d->jent_ptr[e + 1] >= d->jent_ptr[e], "phc_program: Jacobian entries not increasing");
+  for (int i = 0; i < d->njterm; ++i)
+    PHC_REQUIRE(d->jterm_pair[i] >= 0 && d->jterm_pair[i] < d->njpair,
+                "phc_program: bad Jacobian term");
+  PHC_REQUIRE(d->lev_ptr[0] == 0 && d->lev_ptr[d->nlev] == d->nprod, "phc_program: bad levels");
+  for (int l = 1; l <= d->nlev; ++l) {
+    PHC_REQUIRE(d->lev_ptr[l] >= d->lev_ptr[l - 1], "phc_program: bad levels");
+    // a product's left factor lies on the previous level (variables on level 0)
+    for (int i = d->lev_ptr[l - 1]; i < d->lev_ptr[l]; ++i) {
+      const int a = d->left[i];
+      const int la = (a < d->n) ? 0 : -1;
+      if (la == 0) PHC_REQUIRE(l == 1, "phc_program: level of a product of two variables must be 1");
+      else
+        PHC_REQUIRE(a - d->n - 1 >= (l >= 2 ? d->lev_ptr[l - 2] : 0) && a - d->n - 1 < d->lev_ptr[l - 1],
+                    "phc_program: left factor not on the previous level");
+    }
+  }
+  return 0;
+}
+
 int phc_tracker_create(phc_tracker **out, const phc_program *d) {
   PHC_REQUIRE(out && d, "phc_tracker_create: null argument");
   *out = nullptr;
-  PHC_REQUIRE(d->n >= 1 && d->n <= 64, "phc_tracker_create: n must be in 1..64");
-  PHC_REQUIRE(d->deg >= 0 && d->deg <= 32, "phc_tracker_create: deg must be in 0..32");
-  PHC_REQUIRE(d->wdeg >= 0 && d->wdeg <= 2, "phc_tracker_create: weight degree must be <= 2");
+  if (phc_program_validate(d) != 0) return -2;
   const int nn = d->n + 1 + d->nprod;
-  for (int i = 0; i < d->nprod; ++i)
-    PHC_REQUIRE(d->left[i] >= 0 && d->left[i] < d->n + 1 + i && d->right[i] >= 0 && d->right[i] < d->n,
-                "phc_tracker_create: product node factors must be earlier nodes, right a variable");
-  for (int i = 0; i < nn; ++i)
-    PHC_REQUIRE(d->off[i] >= 0 && d->off[i] + d->slen[i] <= d->nslots && d->slen[i] >= 1,
-                "phc_tracker_create: node storage out of range");
-  for (int i = 0; i <= d->n; ++i)
-    PHC_REQUIRE(d->slen[i] == d->deg + 1, "phc_tracker_create: variables need full series storage");
-  for (int i = 0; i < d->npair; ++i)
-    PHC_REQUIRE(d->pair_comp[i] >= 0 && d->pair_comp[i] < d->ncomp && d->pair_node[i] >= 0 &&
-                d->pair_node[i] < nn && (d->slen[d->pair_node[i]] > d->wdeg),
-                "phc_tracker_create: bad component pair");
-  for (int i = 0; i < d->njpair; ++i)
-    PHC_REQUIRE(d->jpair_comp[i] >= 0 && d->jpair_comp[i] < d->ncomp && d->jpair_node[i] >= 0 &&
-                d->jpair_node[i] < nn, "phc_tracker_create: bad Jacobian pair");
-  PHC_REQUIRE(d->seg_ptr[0] == 0 && d->seg_ptr[d->nseg] == d->nterm && d->eq_seg[0] == 0 &&
-              d->eq_seg[d->n] == d->nseg, "phc_tracker_create: bad residual segments");
-  for (int i = 0; i < d->n; ++i)
-    PHC_REQUIRE(d->eq_seg[i + 1] > d->eq_seg[i], "phc_tracker_create: equation without terms");
-  for (int i = 0; i < d->nterm; ++i)
-    PHC_REQUIRE(d->term_pair[i] >= 0 && d->term_pair[i] < d->npair, "phc_tracker_create: bad term");
-  PHC_REQUIRE(d->jent_ptr[0] == 0 && d->jent_ptr[d->n * d->n] == d->njterm,
-              "phc_tracker_create: bad Jacobian entries");
-  for (int i = 0; i < d->njterm; ++i)
-    PHC_REQUIRE(d->jterm_pair[i] >= 0 && d->jterm_pair[i] < d->njpair,
-                "phc_tracker_create: bad Jacobian term");
-  for (int l = 1; l <= d->nlev; ++l)
-    PHC_REQUIRE(d->lev_ptr[l] >= d->lev_ptr[l - 1], "phc_tracker_create: bad levels");
-  PHC_REQUIRE(d->lev_ptr[0] == 0 && d->lev_ptr[d->nlev] == d->nprod, "phc_tracker_create: bad levels");
-
   phc_tracker *tr = new phc_tracker();
   DevProgram &P = tr->P;
   P.n = d->n; P.D = d->deg + 1; P.nprod = d->nprod; P.nlev = d->nlev; P.nslots = d->nslots;
@@ -602,8 +629,8 @@ int phc_tracker_create(phc_tracker **out, const phc_program *d) {
 
 int phc_tracker_destroy(phc_tracker *tr) {
   if (!tr) return 0;
-  for (void *p : tr->bufs) hipFree(p);
-  hipFree(tr->counter);
+  for (void *p : tr->bufs) (void)hipFree(p);
+  (void)hipFree(tr->counter);
   delete tr;
   return 0;
 }

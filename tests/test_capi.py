@@ -34,3 +34,27 @@ def test_version_and_error_string():
     lib = _lib.load()
     assert lib.phc_version() >= 100
     assert isinstance(lib.phc_last_error(), bytes)
+
+
+def test_programs_validate_on_cpu():
+    """Compiled homotopies pass the library's index validation; a corrupted
+    one is rejected with an error (no device work involved)."""
+    import ctypes
+
+    import numpy as np
+
+    from phcpack_amd.homotopy import compile_homotopy, dense_system, total_degree_homotopy
+    from phcpack_amd.tracker import program_struct
+    lib = _lib.load()
+    rng = np.random.default_rng(0)
+    for n, d, k in [(3, 2, 1), (4, 3, 1), (3, 2, 2), (12, 2, 1)]:
+        hom, _ = total_degree_homotopy(dense_system(n, d, rng), rng, k=k)
+        for deg in (0, 1, 8):
+            ps = program_struct(compile_homotopy(hom, deg))
+            assert lib.phc_program_validate(ctypes.byref(ps)) == 0, lib.phc_last_error()
+    prog = compile_homotopy(hom, 8)
+    prog.term_pair = prog.term_pair.copy()
+    prog.term_pair[3] = 10 ** 6
+    ps = program_struct(prog)
+    assert lib.phc_program_validate(ctypes.byref(ps)) == -2
+    assert b"bad term" in lib.phc_last_error()

@@ -1,0 +1,192 @@
+"""Pins the tracker oracle.
+
+1. The compiled program's value and Jacobian of H(x,t) agree with the
+   reference-order polynomial evaluator (osr_poly_evaldiff, bit-exact with
+   CPU_*_poly_evaldiff) applied to every component, monomials with exponents
+   written as repeated variables (the product rule then sums the occurrences).
+2. The Taylor series of the solution curve makes H(x(s), t0+s) vanish to
+   order deg when H is evaluated with coefficient series by the same
+   reference-order evaluator.
+3. Closed forms: x_i^2 = a_i (all sign patterns of sqrt(a)), and the Bezout
+   count of distinct endpoints for dense random quadratics.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from oraclelib import oracle, oracle_eval, oracle_series, oracle_track, pd, pi
+from phcpack_amd.homotopy import (System, compile_homotopy, dense_system, total_degree_homotopy)
+from phcpack_amd.tracker import TrackParams, planes_to_points, points_to_planes
+
+
+def cplx_to_md(z, K):
+    """complex128 array -> [2K][...] planes"""
+    z = np.asarray(z, dtype=np.complex128)
+    out = np.zeros((2 * K,) + z.shape)
+    out[0], out[K] = z.real, z.imag
+    return out
+
+
+def supports_of(sysm, i):
+    sup, cf = [], []
+    for c, e in sysm.eqs[i]:
+        if sum(e) == 0:
+            continue
+        sup.append([j for j in range(sysm.n) for _ in range(e[j])])
+        cf.append(c)
+    cst = sum(c for c, e in sysm.eqs[i] if sum(e) == 0)
+    return sup, cf, cst
+
+
+def ref_eval_component(sysm, i, K, x_series, coef_scale_series):
+    """value and gradient of equation i at series x (planes [2K][n][D]),
+    coefficients multiplied by a scalar series [2K][D]; reference order."""
+    sup, cf, cst = supports_of(sysm, i)
+    n, D = sysm.n, x_series.shape[2]
+    o = oracle()
+    P = 2 * K
+    cffs = np.zeros((P, max(len(sup), 1), D))
+    for m, c in enumerate(cf):
+        cm = cplx_to_md(c, K)
+        for k in range(D):
+            prod = np.zeros(P)
+            o.omd_cmul(K, pd(np.ascontiguousarray(cm)), pd(np.ascontiguousarray(coef_scale_series[:, k])), pd(prod))
+            cffs[:, m, k] = prod
+    cstc = np.zeros((P, D))
+    cm = cplx_to_md(cst, K)
+    for k in range(D):
+        prod = np.zeros(P)
+        o.omd_cmul(K, pd(np.ascontiguousarray(cm)), pd(np.ascontiguousarray(coef_scale_series[:, k])), pd(prod))
+        cstc[:, k] = prod
+    nvr = np.array([len(s) for s in sup], np.int32)
+    idx = np.array([j for s in sup for j in s] or [0], np.int32)
+    ptr = np.concatenate([[0], np.cumsum(nvr)]).astype(np.int32)
+    out = np.zeros((P, n + 1, D))
+    o.osr_poly_evaldiff(K, 1, n, len(sup), D - 1, pi(nvr), pi(ptr), pi(idx), pd(cstc),
+                        pd(np.ascontiguousarray(cffs)), pd(np.ascontiguousarray(x_series)), pd(out))
+    return out
+
+
+def weight_series(w, t, D, K):
+    """Taylor coefficients of w(t0+s) as complex K planes [2K][D]"""
+    W = np.zeros(D, np.complex128)
+    W[0] = w[0] + w[1] * t + w[2] * t * t
+    if D > 1:
+        W[1] = w[1] + 2 * w[2] * t
+    if D > 2:
+        W[2] = w[2]
+    return cplx_to_md(W, K)
+
+
+@pytest.mark.parametrize("degree", [2, 3])
+def test_program_eval_matches_reference_evaluator(degree):
+    rng = np.random.default_rng(degree)
+    n, K = 4, 2
+    p = dense_system(n, degree, rng)
+    hom, _ = total_degree_homotopy(p, rng)
+    prog = compile_homotopy(hom, 0)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    t = 0.37
+    val, jac = oracle_eval(prog, cplx_to_md(x, K), K, t)
+    v = val[:K].sum(0) + 1j * val[K:].sum(0)
+    J = (jac[:K].sum(0) + 1j * jac[K:].sum(0)).reshape(n, n)
+    vr = np.zeros(n, np.complex128)
+    Jr = np.zeros((n, n), np.complex128)
+    xs = cplx_to_md(x, K)[:, :, None]
+    for w, sysm in hom.components:
+        ws = weight_series(w, t, 1, K)
+        for i in range(n):
+            out = ref_eval_component(sysm, i, K, xs, ws)
+            z = out[:K].sum(0) + 1j * out[K:].sum(0)
+            vr[i] += z[n, 0]
+            Jr[i] += z[:n, 0]
+    assert np.allclose(v, vr, rtol=0, atol=1e-13 * np.abs(vr).max())
+    assert np.allclose(J, Jr, rtol=0, atol=1e-13 * np.abs(Jr).max())
+    # independent complex128 evaluation of the value
+    assert np.allclose(v, hom.eval(x, t), atol=1e-12)
+
+
+@pytest.mark.parametrize("degree,k", [(2, 1), (3, 1), (2, 2)])
+def test_series_solves_homotopy_to_order_deg(degree, k):
+    """H(x(s), t0+s) = O(s^{deg+1}) with the reference-order evaluator on
+    coefficient series (a planted-solution check as in dbl2_row_setup,
+    Newton/dbl2_newton_method.cpp:823-991)."""
+    from phcpack_amd.tracker import Tracker  # noqa: F401  (import check only)
+    rng = np.random.default_rng(10 + degree + k)
+    n, K, deg = 3, 2, 8
+    p = dense_system(n, degree, rng)
+    hom, sols = total_degree_homotopy(p, rng, k=k)
+    prog = compile_homotopy(hom, deg)
+    # a point on the path at t0: track the first path to t0 with the oracle
+    t0 = 0.0
+    x0 = sols[1]
+    s, ok = oracle_series(prog, cplx_to_md(x0, K), K, t0)
+    assert ok
+    D = deg + 1
+    total = np.zeros((n, D), np.complex128)
+    for w, sysm in hom.components:
+        ws = weight_series(w, t0, D, K)
+        for i in range(n):
+            out = ref_eval_component(sysm, i, K, s, ws)
+            total[i] += out[:K, n].sum(0) + 1j * out[K:, n].sum(0)
+    scale = np.abs(s[:K].sum(0) + 1j * s[K:].sum(0)).max() + 1
+    # order 0 is H at the start root, which is given in double precision
+    assert np.abs(total[:, 0]).max() < 1e-14
+    # orders 1..deg are solved in double-double
+    assert np.abs(total[:, 1:]).max() < 1e-27 * scale ** degree
+    # and the series is not trivially zero
+    assert np.abs(s[0, :, 1:]).max() > 1e-3
+
+
+def test_decoupled_closed_form_oracle():
+    rng = np.random.default_rng(11)
+    n = 3
+    a = np.exp(2j * np.pi * rng.uniform(size=n)) * rng.uniform(0.5, 2.0, n)
+    eqs = []
+    for i in range(n):
+        e = [0] * n
+        e[i] = 2
+        eqs.append([(1.0 + 0j, tuple(e)), (complex(-a[i]), (0,) * n)])
+    hom, sols = total_degree_homotopy(System(n, eqs), rng)
+    prog = compile_homotopy(hom, 8)
+    out = oracle_track(prog, points_to_planes(sols), 2, TrackParams())
+    assert np.all(out["status"] == 1)
+    x = planes_to_points(out["endp"], 2)
+    want = np.array([np.array(s) * np.sqrt(a) for s in itertools.product([1, -1], repeat=n)])
+    for xi in x:
+        assert np.min(np.abs(want - xi).max(axis=1)) < 1e-14
+
+
+@pytest.mark.parametrize("n", [3, 4])
+def test_bezout_count_distinct_endpoints(n):
+    rng = np.random.default_rng(20 + n)
+    p = dense_system(n, 2, rng)
+    hom, sols = total_degree_homotopy(p, rng)
+    prog = compile_homotopy(hom, 8)
+    out = oracle_track(prog, points_to_planes(sols), 2, TrackParams())
+    assert np.all(out["status"] == 1)
+    x = planes_to_points(out["endp"], 2)
+    assert len(x) == 2 ** n
+    dmin = min(np.abs(x[i] - x[j]).max() for i, j in itertools.combinations(range(len(x)), 2))
+    assert dmin > 1e-6
+    assert max(np.abs(p.eval(xi)).max() for xi in x) < 1e-12
+
+
+def test_fabry_pole_step_closed_form():
+    """For x(t) with a pole at distance r the Fabry ratio recovers r: the
+    scalar homotopy (1-t) x - gamma... built as x*(r - t) = r*c gives
+    x = r*c/(r-t), coefficients c*(1/r)^k, ratio r (Track/dbl_fabry_host.cpp:9-64)."""
+    r, c = 1.7, 0.8 + 0.3j
+    # H = w_p(t)*p + w_q(t)*q with p = x - 0 ... encode (r - t) x - r c = 0:
+    # component A: weight (r - t), system x ; component B: weight 1, system -r c
+    from phcpack_amd.homotopy import Homotopy
+    hom = Homotopy(System(1, [[(1.0 + 0j, (1,))]]), System(1, [[(complex(-r * c), (0,))]]), 1.0)
+    hom.components = [(np.array([r, -1.0, 0.0], np.complex128), hom.target),
+                      (np.array([1.0, 0.0, 0.0], np.complex128), hom.start)]
+    prog = compile_homotopy(hom, 8)
+    s, ok = oracle_series(prog, cplx_to_md(np.array([c]), 2), 2, 0.0)
+    assert ok
+    z = s[:2, 0].sum(0) + 1j * s[2:, 0].sum(0)
+    assert np.allclose(z, c * (1.0 / r) ** np.arange(9), rtol=1e-14)
+    assert abs(abs(z[5] / z[6]) - r) < 1e-12
