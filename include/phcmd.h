@@ -38,6 +38,9 @@ int phc_event_create(void **ev);
 int phc_event_destroy(void *ev);
 int phc_event_record(void *ev, void *stream);
 int phc_event_elapsed_ms(void *start, void *stop, float *ms);
+/* device-to-device copy by a streaming kernel (bytes and pointers 16-byte
+ * aligned); used to calibrate HBM byte counters */
+int phc_device_copy(void *dst_dev, const void *src_dev, size_t bytes, void *stream);
 
 /* ------------------------------------------------------------- arithmetic */
 
@@ -139,13 +142,16 @@ int phc_tracker_lds_bytes(const phc_tracker *tracker, int K, size_t *bytes);
  * threads per path, paths taken from an atomic queue).
  * start [2*KS][npaths][n] (KS limbs), endp [2*K][npaths][n]; per path:
  * tend, status (1 success, 2 step below minstep, 3 maxsteps, 4 singular
- * Jacobian, 5 diverged), nsteps, nrej (rejected steps), dxlast.
+ * Jacobian, 5 diverged), nsteps, nrej (rejected steps), dxlast, and when
+ * work is not NULL the work counters work[c*npaths + path] for c = 0
+ * predictor series computed, 1 Newton iterations, 2 predicted-residual
+ * evaluations.
  * Replaces GPU_Path_mult / path_mult (Path/Path_GPU/path_gpu_mult.cu:84-383)
  * behind the gpumanypaths_dd entry (Ada/PHCtoC/gpumanypaths_dd.cpp:18). */
 int phc_track(phc_tracker *tracker, int K, int npaths, const double *start_dev, int KS,
               double *endp_dev, double *tend_dev, int *status_dev, int *nsteps_dev,
-              int *nrej_dev, double *dxlast_dev, const phc_track_params *params, int block,
-              void *stream);
+              int *nrej_dev, double *dxlast_dev, int *work_dev, const phc_track_params *params,
+              int block, void *stream);
 
 /* Newton refinement at fixed t in precision K (1, 2, 4, 8) of points with KS
  * limbs; xout [2*K][npaths][n]; res = max_i |H_i(xout, t)| evaluated in

@@ -235,11 +235,13 @@ static double max_abs_x(const otr_program *P, work_t *w) {
   return m;
 }
 
-static int newton(const otr_program *P, work_t *w, int iters, double tol, double *dxl) {
+static int newton(const otr_program *P, work_t *w, int iters, double tol, double *dxl,
+                  int *count) {
   const int K = w->K;
   int it, i, j;
   for (it = 0; it < iters; it++) {
     double dx;
+    ++*count;
     eval_point(P, w, 1);
     if (!lu_factor(P, w)) return -1;
     for (i = 0; i < P->n; i++) cneg(K, ZP(w->rhs, i));
@@ -331,14 +333,14 @@ static int predictor_series(const otr_program *P, work_t *w) {
 
 void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const double *start,
                int KS, double *endp, double *tend, int *status, int *nsteps, int *nrej,
-               double *dxlast, const otr_params *prm) {
+               double *dxlast, int *work, const otr_params *prm) {
   const int n = P->n, D = P->deg + 1;
   work_t w;
   int path;
   if (!work_alloc(P, K, &w)) { work_free(&w); return; }
   for (path = p0; path < p1; path++) {
     double t = 0.0, h = prm->maxstep, dxl = 0.0;
-    int st = ST_RUNNING, steps = 0, rej = 0, j, k;
+    int st = ST_RUNNING, steps = 0, rej = 0, j, k, npred = 0, nnewt = 0, nfeval = 0;
     load_point(P, &w, start, KS, npaths, path);
     set_one(P, &w);
     while (st == ST_RUNNING && t < 1.0) {
@@ -346,6 +348,7 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
       double rad = -1.0;
       int accepted = 0;
       set_weights(P, &w, t);
+      ++npred;
       if (!predictor_series(P, &w)) { st = ST_SINGULAR; break; }
       if (kd >= 1)
         for (j = 0; j < n; j++) {
@@ -373,11 +376,12 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
           omd_cadd(K, ZP(w.x0, j), tmp, nodec(P, &w, j, 0));
         }
         set_weights(P, &w, tn);
+        ++nfeval;
         eval_point(P, &w, 0);
         res = max_abs(K, w.rhs, n);
         ok = res <= prm->alpha * (1.0 + max_abs_x(P, &w));
         if (ok) {
-          const int nr = newton(P, &w, prm->corsteps, prm->tolcor, &dxl);
+          const int nr = newton(P, &w, prm->corsteps, prm->tolcor, &dxl, &nnewt);
           if (nr < 0) { st = ST_SINGULAR; break; }
           ok = (nr == 1);
         }
@@ -396,7 +400,7 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
     if (st == ST_RUNNING) {
       int nr;
       set_weights(P, &w, 1.0);
-      nr = newton(P, &w, prm->endsteps, prm->tolend, &dxl);
+      nr = newton(P, &w, prm->endsteps, prm->tolend, &dxl, &nnewt);
       st = (nr < 0) ? ST_SINGULAR : ST_SUCCESS;
     }
     store_point(P, &w, endp, npaths, path);
@@ -405,6 +409,11 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
     nsteps[path] = steps;
     nrej[path] = rej;
     dxlast[path] = dxl;
+    if (work) {
+      work[path] = npred;
+      work[npaths + path] = nnewt;
+      work[2 * npaths + path] = nfeval;
+    }
   }
   work_free(&w);
 }
@@ -414,20 +423,20 @@ typedef struct {
   int K, npaths, p0, p1, KS;
   const double *start;
   double *endp, *tend, *dxlast;
-  int *status, *nsteps, *nrej;
+  int *status, *nsteps, *nrej, *work;
   const otr_params *prm;
 } job_t;
 
 static void *run_job(void *arg) {
   job_t *j = (job_t *)arg;
   otr_track(j->P, j->K, j->npaths, j->p0, j->p1, j->start, j->KS, j->endp, j->tend, j->status,
-            j->nsteps, j->nrej, j->dxlast, j->prm);
+            j->nsteps, j->nrej, j->dxlast, j->work, j->prm);
   return NULL;
 }
 
 void otr_track_mt(const otr_program *P, int K, int npaths, const double *start, int KS,
                   double *endp, double *tend, int *status, int *nsteps, int *nrej,
-                  double *dxlast, const otr_params *prm, int nthreads) {
+                  double *dxlast, int *work, const otr_params *prm, int nthreads) {
   pthread_t th[256];
   job_t jobs[256];
   int i;
@@ -439,6 +448,7 @@ void otr_track_mt(const otr_program *P, int K, int npaths, const double *start, 
     jobs[i].p1 = (int)((long)npaths * (i + 1) / nthreads);
     jobs[i].start = start; jobs[i].endp = endp; jobs[i].tend = tend; jobs[i].dxlast = dxlast;
     jobs[i].status = status; jobs[i].nsteps = nsteps; jobs[i].nrej = nrej; jobs[i].prm = prm;
+    jobs[i].work = work;
     pthread_create(&th[i], NULL, run_job, &jobs[i]);
   }
   for (i = 0; i < nthreads; i++) pthread_join(th[i], NULL);
@@ -455,7 +465,8 @@ void otr_refine(const otr_program *P, int K, int npaths, double t, const double 
     load_point(P, &w, xin, KS, npaths, path);
     cset(K, nodec(P, &w, P->n, 0), 1.0);
     set_weights(P, &w, t);
-    if (iters > 0) nr = newton(P, &w, iters, tol, &dxl);
+    int its = 0;
+    if (iters > 0) nr = newton(P, &w, iters, tol, &dxl, &its);
     eval_point(P, &w, 0);
     res[path] = max_abs(K, w.rhs, P->n);
     store_point(P, &w, xout, npaths, path);

@@ -30,12 +30,12 @@ typedef struct {
 /* paths p0 <= p < p1 of npaths; arrays as in phc_track */
 void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const double *start,
                int KS, double *endp, double *tend, int *status, int *nsteps, int *nrej,
-               double *dxlast, const otr_params *prm);
+               double *dxlast, int *work, const otr_params *prm);
 
 /* same as otr_track over all paths with nthreads POSIX threads */
 void otr_track_mt(const otr_program *P, int K, int npaths, const double *start, int KS,
                   double *endp, double *tend, int *status, int *nsteps, int *nrej,
-                  double *dxlast, const otr_params *prm, int nthreads);
+                  double *dxlast, int *work, const otr_params *prm, int nthreads);
 
 void otr_refine(const otr_program *P, int K, int npaths, double t, const double *xin, int KS,
                 double *xout, int iters, double tol, double *dxlast, double *res, int *status);

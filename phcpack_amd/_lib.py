@@ -37,6 +37,7 @@ SIGNATURES = {
     "phc_event_destroy": (c_int, [c_void_p]),
     "phc_event_record": (c_int, [c_void_p, c_void_p]),
     "phc_event_elapsed_ms": (c_int, [c_void_p, c_void_p, P_float]),
+    "phc_device_copy": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
     "phc_md_batch": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "phc_series_product": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                    c_void_p]),
@@ -75,8 +76,8 @@ SIGNATURES.update({
     "phc_program_validate": (c_int, [ctypes.POINTER(PhcProgram)]),
     "phc_tracker_lds_bytes": (c_int, [c_void_p, c_int, ctypes.POINTER(ctypes.c_size_t)]),
     "phc_track": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
-                          c_void_p, c_void_p, c_void_p, ctypes.POINTER(PhcTrackParams), c_int,
-                          c_void_p]),
+                          c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(PhcTrackParams),
+                          c_int, c_void_p]),
     "phc_refine": (c_int, [c_void_p, c_int, c_int, c_double, c_void_p, c_int, c_void_p, c_int,
                            c_double, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
 })

@@ -11,7 +11,26 @@ static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
 }  // namespace phc
 
+namespace {
+// plain streaming copy: 16 bytes per lane per iteration
+__global__ void copy_kernel(double2 *__restrict__ dst, const double2 *__restrict__ src, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace
+
 extern "C" {
+
+int phc_device_copy(void *dst, const void *src, size_t bytes, void *stream) {
+  PHC_REQUIRE(bytes % 16 == 0, "phc_device_copy: bytes must be a multiple of 16");
+  PHC_REQUIRE(((size_t)dst | (size_t)src) % 16 == 0, "phc_device_copy: pointers must be 16-byte aligned");
+  if (bytes == 0) return 0;
+  const size_t n = bytes / 16;
+  hipLaunchKernelGGL(copy_kernel, dim3(phc::grid_for(n, 256) < 8192 ? phc::grid_for(n, 256) : 8192),
+                     dim3(256), 0, (hipStream_t)stream, (double2 *)dst, (const double2 *)src, n);
+  PHC_HIP(hipGetLastError());
+  return 0;
+}
 
 const char *phc_last_error(void) { return phc::g_last_error.c_str(); }
 

@@ -261,9 +261,11 @@ template <int K> __device__ double max_abs_x(const DevProgram &P, Work<K> &w) {
 // Newton steps at fixed t on the order-0 slots; returns status
 // (1 converged, 0 not converged, -1 singular); *dxl = last max |dx|
 template <int K>
-__device__ int newton(const DevProgram &P, Work<K> &w, int iters, double tol, double *dxl) {
+__device__ int newton(const DevProgram &P, Work<K> &w, int iters, double tol, double *dxl,
+                      int *count) {
   typedef C<K> Cx;
   for (int it = 0; it < iters; ++it) {
+    ++*count;
     eval_point<K>(P, w, true);
     if (!lu_factor<K>(P, w)) return -1;
     for (int i = threadIdx.x; i < P.n; i += blockDim.x) w.rhs[i] = Cx::neg(w.rhs[i]);
@@ -312,7 +314,7 @@ __global__ void __launch_bounds__(256) track_kernel(DevProgram P, Params par, in
                                                    const double *__restrict__ start, int KS,
                                                    double *__restrict__ endp, double *tend,
                                                    int *status, int *nsteps_out, int *nrej_out,
-                                                   double *dx_out, int *counter) {
+                                                   double *dx_out, int *work, int *counter) {
   typedef C<K> Cx;
   extern __shared__ double lds[];
   __shared__ int s_path;
@@ -339,11 +341,12 @@ __global__ void __launch_bounds__(256) track_kernel(DevProgram P, Params par, in
       w.slots[P.off[n] + a] = (a == 0) ? Cx::from(1.0, 0.0) : Cx::zero();
     __syncthreads();
     double t = 0.0, h = par.maxstep, dxl = 0.0;
-    int st = ST_RUNNING, steps = 0, rej = 0;
+    int st = ST_RUNNING, steps = 0, rej = 0, npred = 0, nnewt = 0, nfeval = 0;
     while (st == ST_RUNNING && t < 1.0) {
       // ---- predictor: Taylor series of x(t+s) ----
       set_weights<K>(P, w, t);
       __syncthreads();
+      ++npred;
       eval_point<K>(P, w, true);
       if (!lu_factor<K>(P, w)) { st = ST_SINGULAR; break; }
       for (int k = 1; k < D; ++k) {
@@ -385,11 +388,12 @@ __global__ void __launch_bounds__(256) track_kernel(DevProgram P, Params par, in
         }
         set_weights<K>(P, w, tn);
         __syncthreads();
+        ++nfeval;
         eval_point<K>(P, w, false);
         const double res = max_abs<K>(w.rhs, n);
         bool ok = res <= par.alpha * (1.0 + max_abs_x<K>(P, w));
         if (ok) {
-          const int nr = newton<K>(P, w, par.corsteps, par.tolcor, &dxl);
+          const int nr = newton<K>(P, w, par.corsteps, par.tolcor, &dxl, &nnewt);
           if (nr < 0) { st = ST_SINGULAR; break; }
           ok = (nr == 1);
         }
@@ -409,7 +413,7 @@ __global__ void __launch_bounds__(256) track_kernel(DevProgram P, Params par, in
       // endpoint: Newton at t = 1
       set_weights<K>(P, w, 1.0);
       __syncthreads();
-      const int nr = newton<K>(P, w, par.endsteps, par.tolend, &dxl);
+      const int nr = newton<K>(P, w, par.endsteps, par.tolend, &dxl, &nnewt);
       st = (nr < 0) ? ST_SINGULAR : ST_SUCCESS;
     }
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
@@ -425,6 +429,11 @@ __global__ void __launch_bounds__(256) track_kernel(DevProgram P, Params par, in
       nsteps_out[path] = steps;
       nrej_out[path] = rej;
       dx_out[path] = dxl;
+      if (work) {
+        work[path] = npred;
+        work[npaths + path] = nnewt;
+        work[2 * npaths + path] = nfeval;
+      }
     }
     __syncthreads();
   }
@@ -466,7 +475,8 @@ __global__ void __launch_bounds__(256) refine_kernel(DevProgram P, int npaths, d
     __syncthreads();
     double dxl = 0.0;
     int nr = 1;
-    if (iters > 0) nr = newton<K>(P, w, iters, tol, &dxl);
+    int its = 0;
+    if (iters > 0) nr = newton<K>(P, w, iters, tol, &dxl, &its);
     eval_point<K>(P, w, false);
     const double res = max_abs<K>(w.rhs, n);
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
@@ -643,7 +653,7 @@ int phc_tracker_lds_bytes(const phc_tracker *tr, int K, size_t *bytes) {
 }
 
 int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, double *endp,
-              double *tend, int *status, int *nsteps, int *nrej, double *dxlast,
+              double *tend, int *status, int *nsteps, int *nrej, double *dxlast, int *work,
               const phc_track_params *prm, int block, void *stream) {
   PHC_REQUIRE(tr && prm, "phc_track: null argument");
   PHC_REQUIRE(K == 1 || K == 2, "phc_track: tracking precision K must be 1 or 2");
@@ -671,10 +681,10 @@ int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, d
   PHC_REQUIRE(grid > 0, "phc_track: occupancy query failed");
   if (K == 1)
     hipLaunchKernelGGL(track_kernel<1>, dim3(grid), dim3(block), lds, st, tr->P, par, npaths, start,
-                       KS, endp, tend, status, nsteps, nrej, dxlast, tr->counter);
+                       KS, endp, tend, status, nsteps, nrej, dxlast, work, tr->counter);
   else
     hipLaunchKernelGGL(track_kernel<2>, dim3(grid), dim3(block), lds, st, tr->P, par, npaths, start,
-                       KS, endp, tend, status, nsteps, nrej, dxlast, tr->counter);
+                       KS, endp, tend, status, nsteps, nrej, dxlast, work, tr->counter);
   PHC_HIP(hipGetLastError());
   return 0;
 }

@@ -1,0 +1,122 @@
+"""Algorithmic work of the tracker and refinement kernels.
+
+Counts complex multiple-double operations per phase exactly from the
+compiled program and the per-path work counters the tracker returns, then
+converts them to FP64 floating-point operations with two tables:
+
+* build: the FP64 instructions the error-free transformations of
+  phcpack_amd/csrc/md.hpp issue (a fused multiply-add counts 2, a division
+  1); two_prod is a multiply plus an FMA (3), two_sum 6, quick_two_sum 3.
+* reference: the cost factors PHCpack reports with
+  (Matrices/write_dbl2_qrtimeflops.cpp:49-58, write_dbl4_qrtimeflops.cpp:50-59,
+  write_dbl8_qrtimeflops.cpp:50-59): dd add/mul/div 20/23/70,
+  qd 89/336/893, od 270/1742/5126 (Dekker products).
+"""
+
+TWO_SUM, QTS, TWO_PROD = 6, 3, 3
+
+
+def _add_k(K):
+    if K == 1:
+        return 1
+    if K == 2:
+        return 2 * TWO_SUM + 1 + QTS + 1 + QTS
+    ts = K + K * (K - 1) // 2
+    return ts * TWO_SUM + K + K * QTS + (K - 1) * QTS
+
+
+def _mul_k(K):
+    if K == 1:
+        return 1
+    if K == 2:
+        return TWO_PROD + 4 + QTS
+    f = (K - 1) + (K - 2)  # plain products of level K
+    for L in range(K - 1, -1, -1):
+        tail = K - 1 - L
+        f += TWO_PROD + tail * TWO_SUM + 1
+        f += L * (TWO_PROD + tail * TWO_SUM + 1 + TWO_SUM + tail * TWO_SUM + 1)
+    return f + K * QTS + (K - 1) * QTS
+
+
+def _mul_d_k(K):
+    if K == 1:
+        return 1
+    if K == 2:
+        return TWO_PROD + 2 + QTS
+    ts = (K - 1) * K // 2
+    return K * TWO_PROD + ts * TWO_SUM + K + K * QTS + (K - 1) * QTS
+
+
+def _div_k(K):
+    if K == 1:
+        return 1
+    if K == 2:
+        return 3 + 2 * (_mul_d_k(2) + _add_k(2)) + QTS + (TWO_SUM + 1 + QTS)
+    return (K + 1) + K * (_mul_d_k(K) + _add_k(K)) + K * QTS + (K - 1) * QTS
+
+
+BUILD = {K: dict(add=_add_k(K), mul=_mul_k(K), mul_d=_mul_d_k(K), div=_div_k(K))
+         for K in (1, 2, 4, 8)}
+REFERENCE = {1: dict(add=1, mul=1, mul_d=1, div=1),
+             2: dict(add=20, mul=23, mul_d=23, div=70),
+             4: dict(add=89, mul=336, mul_d=336, div=893),
+             8: dict(add=270, mul=1742, mul_d=1742, div=5126)}
+
+
+def complex_costs(K, table=BUILD):
+    t = table[K]
+    return dict(cmul=4 * t["mul"] + 2 * t["add"], cmac=4 * t["mul"] + 4 * t["add"],
+                cadd=2 * t["add"], scale_d=2 * t["mul_d"],
+                cdiv=6 * t["mul"] + 3 * t["add"] + 2 * t["div"])
+
+
+def _phase_ops(prog):
+    """complex operation counts of each phase: dict phase -> dict op -> count"""
+    n, D = prog.n, prog.deg + 1
+    nprod, npair, nterm = len(prog.left), len(prog.pair_comp), len(prog.term_pair)
+    nseg = len(prog.seg_ptr) - 1
+    njpair, njterm = len(prog.jpair_comp), len(prog.jterm_pair)
+    ev = dict(cmul=nprod + npair, cmac=nterm, cadd=nseg - n)
+    jac = dict(cmul=njpair, cmac=njterm)
+    lu = dict(cdiv=n, cmul=sum(n - j - 1 for j in range(n)),
+              cmac=sum((n - j - 1) ** 2 for j in range(n)))
+    solve = dict(cmac=n * (n - 1), cmul=n)
+
+    def add(*ds):
+        out = {}
+        for d in ds:
+            for k, v in d.items():
+                out[k] = out.get(k, 0) + v
+        return out
+
+    newton = add(ev, jac, lu, solve, dict(cadd=n))
+    orders = {}
+    for k in range(1, D):
+        orders = add(orders, dict(cmul=nprod + npair, cmac=nprod * k + npair * min(prog.wdeg, k) + nterm,
+                                  cadd=nseg - n), solve, dict(cmul=nprod, cmac=nprod, cadd=nprod))
+    pred = add(ev, jac, lu, orders)
+    feval = add(ev, dict(scale_d=n * (D - 1), cadd=n * (D - 1)))
+    return dict(predictor=pred, newton=newton, feval=feval, eval=ev)
+
+
+def tracker_flops(prog, K, work, table=BUILD):
+    """FP64 flops of one tracking launch given the work counters [3][npaths]
+    (predictor series, Newton iterations, predicted-residual evaluations)."""
+    ph = _phase_ops(prog)
+    c = complex_costs(K, table)
+
+    def cost(d):
+        return sum(c[k] * v for k, v in d.items())
+
+    npred, nnewt, nfeval = (int(x.sum()) for x in work)
+    return npred * cost(ph["predictor"]) + nnewt * cost(ph["newton"]) + nfeval * cost(ph["feval"])
+
+
+def refine_flops(prog0, K, npaths, iters, table=BUILD):
+    """FP64 flops of a refinement launch of `iters` Newton iterations on
+    every path plus the final residual evaluation (upper bound: converged
+    paths stop early)."""
+    ph = _phase_ops(prog0)
+    c = complex_costs(K, table)
+    cost = lambda d: sum(c[k] * v for k, v in d.items())  # noqa: E731
+    return npaths * (iters * cost(ph["newton"]) + cost(ph["eval"]))

@@ -118,11 +118,12 @@ class TrackBuffers:
         self.nsteps = DeviceArray((npaths,), np.int32)
         self.nrej = DeviceArray((npaths,), np.int32)
         self.dx = DeviceArray((npaths,))
+        self.work = DeviceArray((3, npaths), np.int32)
 
     def to_host(self):
         return dict(endp=self.endp.to_host(), tend=self.tend.to_host(),
                     status=self.status.to_host(), nsteps=self.nsteps.to_host(),
-                    nrej=self.nrej.to_host(), dx=self.dx.to_host())
+                    nrej=self.nrej.to_host(), dx=self.dx.to_host(), work=self.work.to_host())
 
 
 class Tracker:
@@ -141,7 +142,7 @@ class Tracker:
         p = (params or TrackParams()).to_c()
         check(load().phc_track(self.dev.handle, K, npaths, d_start.ptr, KS, bufs.endp.ptr,
                                bufs.tend.ptr, bufs.status.ptr, bufs.nsteps.ptr, bufs.nrej.ptr,
-                               bufs.dx.ptr, ctypes.byref(p), block, stream))
+                               bufs.dx.ptr, bufs.work.ptr, ctypes.byref(p), block, stream))
 
     def track(self, starts, K=2, params=None, block=64):
         """starts: complex128 [npaths][n] (or [2KS][npaths][n] planes)."""

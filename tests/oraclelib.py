@@ -84,9 +84,9 @@ def _track_sigs():
     from phcpack_amd._lib import PhcProgram, PhcTrackParams
     o = oracle()
     o.otr_track.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_int, C.c_int, C.c_int, P_d, C.c_int,
-                            P_d, P_d, P_i, P_i, P_i, P_d, C.POINTER(PhcTrackParams)]
+                            P_d, P_d, P_i, P_i, P_i, P_d, P_i, C.POINTER(PhcTrackParams)]
     o.otr_track_mt.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_int, P_d, C.c_int, P_d, P_d,
-                               P_i, P_i, P_i, P_d, C.POINTER(PhcTrackParams), C.c_int]
+                               P_i, P_i, P_i, P_d, P_i, C.POINTER(PhcTrackParams), C.c_int]
     o.otr_refine.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_int, C.c_double, P_d, C.c_int,
                              P_d, C.c_int, C.c_double, P_d, P_d, P_i]
     o.otr_eval.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_double, P_d, P_d, P_d]
@@ -105,17 +105,18 @@ def oracle_track(prog, starts_planes, K, params, nthreads=1, paths=None):
     KS, npaths, n = st.shape[0] // 2, st.shape[1], st.shape[2]
     out = dict(endp=np.zeros((2 * K, npaths, n)), tend=np.zeros(npaths),
                status=np.zeros(npaths, np.int32), nsteps=np.zeros(npaths, np.int32),
-               nrej=np.zeros(npaths, np.int32), dx=np.zeros(npaths))
+               nrej=np.zeros(npaths, np.int32), dx=np.zeros(npaths),
+               work=np.zeros((3, npaths), np.int32))
     p = params.to_c()
     if nthreads > 1:
         o.otr_track_mt(C.byref(ps), K, npaths, pd(st), KS, pd(out["endp"]), pd(out["tend"]),
                        pi(out["status"]), pi(out["nsteps"]), pi(out["nrej"]), pd(out["dx"]),
-                       C.byref(p), nthreads)
+                       pi(out["work"]), C.byref(p), nthreads)
     else:
         p0, p1 = (0, npaths) if paths is None else paths
         o.otr_track(C.byref(ps), K, npaths, p0, p1, pd(st), KS, pd(out["endp"]), pd(out["tend"]),
                     pi(out["status"]), pi(out["nsteps"]), pi(out["nrej"]), pd(out["dx"]),
-                    C.byref(p))
+                    pi(out["work"]), C.byref(p))
     return out
 
 

@@ -32,6 +32,7 @@ def test_track_matches_oracle(K, n, degree, seed):
     assert np.array_equal(g["nrej"], o["nrej"])
     assert np.array_equal(g["tend"], o["tend"])
     assert np.array_equal(g["endp"], o["endp"])
+    assert np.array_equal(g["work"], o["work"])
     assert np.all(g["status"] == 1)
 
 
