@@ -62,7 +62,7 @@ def build_workload(n=12, seed=SEED):
 
 def run_plain(tr, d_start, npaths, bufs, ev, prm):
     ev.start()
-    tr.track_device(d_start, 1, npaths, 2, bufs, prm, block=64)
+    tr.track_device(d_start, 1, npaths, 2, bufs, prm)
     return ev.stop_ms()
 
 
@@ -71,7 +71,7 @@ def run_mixed(tr, d_start, npaths, bufs, ref, ev, prm, lib):
     converge are refined in qd and then od."""
     from phcpack_amd._lib import check
     ev.start()
-    tr.track_device(d_start, 1, npaths, 1, bufs, prm, block=64)
+    tr.track_device(d_start, 1, npaths, 1, bufs, prm)
     t_track = ev.stop_ms()
     ev.start()
     tr.refine_device(bufs.endp, 1, npaths, 2, ref["x2"], ref["dx"], ref["res"], ref["st"],

@@ -83,33 +83,43 @@ int phc_poly_evaldiff(phc_poly *plan, int nbatch, const double *input_dev, doubl
 
 /* --------------------------------------------------------------- tracking */
 
-/* Compiled homotopy H_i(x,t) = sum_r w_r(t) sum_m c[r,i,m] x^m (see
- * phcpack_amd/homotopy.py for the construction).  Nodes 0..n-1 are the
- * variables, node n the constant 1, node n+1+i = node left[i] * variable
- * right[i]; product nodes are grouped by DAG level (lev_ptr[nlev+1]).
- * off[], slen[]: storage of every node's series coefficients (complex units);
- * variables and factor nodes keep deg+1 coefficients, others a ring of
- * wdeg+1.  Pairs (component, node) carry weighted values u = sum_l W_l
- * node_{k-l}; residual terms (pair, coefficient) are grouped in segments
- * (seg_ptr) and equations (eq_seg); Jacobian entry i*n+j sums jterms
- * (jpair, coefficient*exponent) with jpair values W_0 * node_0.
- * Coefficient arrays are complex octo-double planes [2][8][count] (the
- * leading K limbs are used at precision K); weights [2][8][3*ncomp] hold
- * the coefficients of 1, t, t^2 of every w_r.
+/* Compiled homotopy H_i(x,t) = sum_r w_r(t) sum_m c[r,i,m] x^m (built by
+ * compile_homotopy in phcpack_amd/homotopy.py).  Series coefficients live in
+ * "slots" (complex units): the variables x_j at j*(deg+1), the constant 1 at
+ * n*(deg+1), product nodes where nodes[] says.
+ *   nodes [8*nprod], per product node in DAG-level order:
+ *     aoff, amask (left factor: a variable or a product node, full series,
+ *     amask = -1), boff (right factor: variable j at j*(deg+1)), soff, smask
+ *     (own storage: coefficient a at soff + (a & smask); smask = -1 for full
+ *     series, 2^m - 1 for a ring of the last 2^m orders), adn, sdn (index of
+ *     the left factor's / own order-k correction in the correction buffer,
+ *     -1 for none), 0.
+ *   lev_ptr [nlev+1]: nodes of level l are lev_ptr[l-1]..lev_ptr[l]-1.
+ *   pairs [4*npair]: component r, storage offset, mask, 0 of a node whose
+ *     weighted value u = sum_l W_{r,l} node_{k-l} enters the residual.
+ *   seg_ptr [n*G+1]: terms of segment i*G+g of equation i (G a power of 2,
+ *     n*G <= 64); term_pair [nterm], term_coef [2][nterm] (real plane,
+ *     imaginary plane; coefficients are complex doubles).
+ *   jpairs [2*njpair]: component r, storage offset of node m/x_j.
+ *   jent_ptr [n*n+1], jterms [njterm]: Jacobian entry i*n+j sums
+ *     coef[term] * e * W_{r,0} * node_0 with jterm = term | jpair << 16 | e << 28.
+ *   weights [2][8][3*ncomp]: octo-double coefficients of 1, t, t^2 of w_r.
  * Replaces the homotopy set-up of gpumanypaths_dd
  * (Ada/PHCtoC/gpumanypaths_dd.cpp:18-57, which reads PHCpack's containers)
  * and the per-call job planning of GPU_dbl2_poly_evaldiff. */
 typedef struct {
-  int n, deg, nprod, nlev, nslots, ncomp, wdeg;
-  const int *left, *right, *lev_ptr, *off, *slen;
+  int n, deg, ncomp, wdeg, G;
+  int nprod, nlev, nslots, nfactor;
+  const int *nodes, *lev_ptr;
   int npair;
-  const int *pair_comp, *pair_node;
-  int nseg, nterm;
-  const int *seg_ptr, *eq_seg, *term_pair;
+  const int *pairs;
+  int nterm;
+  const int *seg_ptr, *term_pair;
   const double *term_coef;
-  int njpair, njterm;
-  const int *jpair_comp, *jpair_node, *jent_ptr, *jterm_pair;
-  const double *jterm_coef;
+  int njpair;
+  const int *jpairs;
+  int njterm;
+  const int *jent_ptr, *jterms;
   const double *weights;
 } phc_program;
 
@@ -134,12 +144,15 @@ int phc_program_validate(const phc_program *program);
 typedef struct phc_tracker phc_tracker;
 int phc_tracker_create(phc_tracker **tracker, const phc_program *program);
 int phc_tracker_destroy(phc_tracker *tracker);
-/* LDS bytes per path at precision K (1, 2, 4, 8) */
-int phc_tracker_lds_bytes(const phc_tracker *tracker, int K, size_t *bytes);
+/* LDS bytes of the staged program and of one path at precision K (1, 2, 4, 8) */
+int phc_tracker_lds_bytes(const phc_tracker *tracker, int K, size_t *program_bytes,
+                          size_t *path_bytes);
 
 /* Track npaths paths from t = 0 to t = 1 in precision K (1 = double,
- * 2 = double-double) with a persistent kernel (one workgroup of `block`
- * threads per path, paths taken from an atomic queue).
+ * 2 = double-double) with a persistent kernel: one workgroup per compute
+ * unit stages the program in LDS, each of its (at most `waves`, as LDS
+ * allows) wavefronts tracks one path at a time, paths taken from an atomic
+ * queue.
  * start [2*KS][npaths][n] (KS limbs), endp [2*K][npaths][n]; per path:
  * tend, status (1 success, 2 step below minstep, 3 maxsteps, 4 singular
  * Jacobian, 5 diverged), nsteps, nrej (rejected steps), dxlast, and when
@@ -151,7 +164,7 @@ int phc_tracker_lds_bytes(const phc_tracker *tracker, int K, size_t *bytes);
 int phc_track(phc_tracker *tracker, int K, int npaths, const double *start_dev, int KS,
               double *endp_dev, double *tend_dev, int *status_dev, int *nsteps_dev,
               int *nrej_dev, double *dxlast_dev, int *work_dev, const phc_track_params *params,
-              int block, void *stream);
+              int waves, void *stream);
 
 /* Newton refinement at fixed t in precision K (1, 2, 4, 8) of points with KS
  * limbs; xout [2*K][npaths][n]; res = max_i |H_i(xout, t)| evaluated in
@@ -161,7 +174,7 @@ int phc_track(phc_tracker *tracker, int K, int npaths, const double *start_dev, 
  * (Newton/dbl2_newton_method.cpp:429, row Newton step) in dd/qd/od. */
 int phc_refine(phc_tracker *tracker, int K, int npaths, double t, const double *xin_dev, int KS,
                double *xout_dev, int iters, double tol, double *dxlast_dev, double *res_dev,
-               int *status_dev, int block, void *stream);
+               int *status_dev, int waves, void *stream);
 
 #ifdef __cplusplus
 }

@@ -10,8 +10,13 @@
  *     above alpha (standard_predictor_convolutions.adb:1142-1185, Predictor_Feedback)
  *   Convergence_Radius_Estimates.Fabry (Math_Lib/Circuits/convergence_radius_estimates.adb:138-163,354-390)
  *   Bounded_Update (Deformations/Curves/three_way_minima.adb:96-109)
- * and evaluates the homotopy through the program of phcpack_amd/homotopy.py
- * in the operation order the device kernel uses, value by value.
+ * and evaluates the homotopy through the program of phcpack_amd/homotopy.py.
+ * Every value is computed with the same operations in the same order as the
+ * device kernel: sums over the G segments of an equation and over the G
+ * parts of a matrix-vector product combine pairwise as a butterfly does
+ * ((p0+p1)+(p2+p3) for G = 4), the Jacobian is inverted in place by
+ * Gauss-Jordan with partial pivoting (first maximal |re|+|im| of the leading
+ * limbs, non-finite values counted as infinite).
  *
  * Parity status: the evaluation (value, Jacobian, Taylor series) is pinned
  * against the reference-order polynomial evaluator (series.c, itself
@@ -29,30 +34,19 @@
 enum { ST_RUNNING = 0, ST_SUCCESS = 1, ST_MINSTEP = 2, ST_MAXSTEPS = 3, ST_SINGULAR = 4,
        ST_DIVERGED = 5 };
 
+#define JT_TERM_BITS 16
+#define JT_PAIR_BITS 12
+
 typedef struct {
   int K;
-  double *slots, *u, *uj, *A, *part, *rhs, *x0, *W, *dnode, *invd;
-  int *piv;
+  double *slots, *u, *uj, *A, *rhs, *x0, *col, *W, *dnode, *part;
+  int *perm;
 } work_t;
 
-#define ZS(K) (2 * (K))
-#define ZP(a, i) ((a) + (size_t)(i) * ZS(K))
-
-static double *nodec(const otr_program *P, work_t *w, int m, int a) {
-  const int K = w->K;
-  return ZP(w->slots, P->off[m] + a % P->slen[m]);
-}
-
-static void load_coef(int K, const double *base, int m, int j, double *z) {
-  int i;
-  for (i = 0; i < K; i++) {
-    z[i] = base[(size_t)i * m + j];
-    z[K + i] = base[(size_t)(8 + i) * m + j];
-  }
-}
+#define ZP(a, i) ((a) + (size_t)(i) * 2 * K)
 
 static void czero(int K, double *z) { memset(z, 0, sizeof(double) * 2 * K); }
-static void cset(int K, double *z, double re) { czero(K, z); z[0] = re; }
+static void cset(int K, double *z, double re, double im) { czero(K, z); z[0] = re; z[K] = im; }
 static void ccopy(int K, const double *a, double *z) { memmove(z, a, sizeof(double) * 2 * K); }
 static void cscale_d(int K, const double *a, double s, double *z) {
   omd_mul_d(K, a, s, z);
@@ -76,14 +70,39 @@ static void cmsub(int K, const double *x, const double *y, double *z) {
 }
 static double absz(int K, const double *z) { return fabs(z[0]) + fabs(z[K]); }
 
+/* butterfly sum of G complex values in p (result in p[0]) */
+static void tree_sum(int K, double *p, int G) {
+  int m, g;
+  for (m = 1; m < G; m <<= 1)
+    for (g = 0; g + m < G; g += 2 * m) omd_cadd(K, ZP(p, g), ZP(p, g + m), ZP(p, g));
+}
+
+static double *slot(work_t *w, int off, int mask, int a) {
+  const int K = w->K;
+  return ZP(w->slots, off + (a & mask));
+}
+
+static void coef(const otr_program *P, int K, int t, double *z) {
+  cset(K, z, P->term_coef[t], P->term_coef[P->nterm + t]);
+}
+
+static void load_weight(const otr_program *P, int K, int j, double *z) {
+  const int m = 3 * P->ncomp;
+  int i;
+  for (i = 0; i < K; i++) {
+    z[i] = P->weights[(size_t)i * m + j];
+    z[K + i] = P->weights[(size_t)(8 + i) * m + j];
+  }
+}
+
 static void set_weights(const otr_program *P, work_t *w, double t) {
   const int K = w->K;
   double o0[16], o1[16], o2[16], o2t[16], tmp[16];
   int r;
   for (r = 0; r < P->ncomp; r++) {
-    load_coef(K, P->weights, 3 * P->ncomp, 3 * r, o0);
-    load_coef(K, P->weights, 3 * P->ncomp, 3 * r + 1, o1);
-    load_coef(K, P->weights, 3 * P->ncomp, 3 * r + 2, o2);
+    load_weight(P, K, 3 * r, o0);
+    load_weight(P, K, 3 * r + 1, o1);
+    load_weight(P, K, 3 * r + 2, o2);
     cscale_d(K, o2, t, o2t);
     omd_cadd(K, o1, o2t, tmp);
     cscale_d(K, tmp, t, tmp);
@@ -97,121 +116,137 @@ static void set_weights(const otr_program *P, work_t *w, double t) {
 static void product_pass(const otr_program *P, work_t *w, int k) {
   const int K = w->K;
   double acc[16];
-  int lv, i, q;
-  for (lv = 1; lv <= P->nlev; lv++)
-    for (i = P->lev_ptr[lv - 1]; i < P->lev_ptr[lv]; i++) {
-      const int a = P->left[i], b = P->right[i];
-      omd_cmul(K, nodec(P, w, a, 0), nodec(P, w, b, k), acc);
-      for (q = 1; q <= k; q++) omd_cmac(K, nodec(P, w, a, q), nodec(P, w, b, k - q), acc);
-      ccopy(K, acc, nodec(P, w, P->n + 1 + i, k));
-    }
+  int i, q;
+  for (i = 0; i < P->nprod; i++) {
+    const int *nd = P->nodes + 8 * i;
+    omd_cmul(K, slot(w, nd[0], -1, 0), slot(w, nd[2], -1, k), acc);
+    for (q = 1; q <= k; q++) omd_cmac(K, slot(w, nd[0], -1, q), slot(w, nd[2], -1, k - q), acc);
+    ccopy(K, acc, slot(w, nd[3], nd[4], k));
+  }
 }
 
 static void correction_pass(const otr_program *P, work_t *w, int k) {
   const int K = w->K;
-  double d[16], zero[16], *da;
-  int lv, i;
-  czero(K, zero);
-  for (lv = 1; lv <= P->nlev; lv++)
-    for (i = P->lev_ptr[lv - 1]; i < P->lev_ptr[lv]; i++) {
-      const int a = P->left[i], b = P->right[i];
-      da = (a < P->n) ? nodec(P, w, a, k) : (a == P->n ? zero : ZP(w->dnode, a - P->n - 1));
-      omd_cmul(K, nodec(P, w, a, 0), nodec(P, w, b, k), d);
-      omd_cmac(K, da, nodec(P, w, b, 0), d);
-      ccopy(K, d, ZP(w->dnode, i));
-      omd_cadd(K, nodec(P, w, P->n + 1 + i, k), d, nodec(P, w, P->n + 1 + i, k));
-    }
+  double d[16];
+  const double *da;
+  int i;
+  for (i = 0; i < P->nprod; i++) {
+    const int *nd = P->nodes + 8 * i;
+    da = (nd[5] < 0) ? slot(w, nd[0], -1, k) : ZP(w->dnode, nd[5]);
+    omd_cmul(K, slot(w, nd[0], -1, 0), slot(w, nd[2], -1, k), d);
+    omd_cmac(K, da, slot(w, nd[2], -1, 0), d);
+    if (nd[6] >= 0) ccopy(K, d, ZP(w->dnode, nd[6]));
+    omd_cadd(K, slot(w, nd[3], nd[4], k), d, slot(w, nd[3], nd[4], k));
+  }
 }
 
+/* rhs_i = -[H_i]_k */
 static void residual(const otr_program *P, work_t *w, int k) {
-  const int K = w->K;
+  const int K = w->K, G = P->G;
   double acc[16], c[16];
-  int q, l, s, t, i;
+  int q, l, i, g, t;
   for (q = 0; q < P->npair; q++) {
-    const int r = P->pair_comp[q], m = P->pair_node[q];
-    omd_cmul(K, ZP(w->W, 3 * r), nodec(P, w, m, k), acc);
+    const int *pr = P->pairs + 4 * q;
+    omd_cmul(K, ZP(w->W, 3 * pr[0]), slot(w, pr[1], pr[2], k), acc);
     for (l = 1; l <= P->wdeg && l <= k; l++)
-      omd_cmac(K, ZP(w->W, 3 * r + l), nodec(P, w, m, k - l), acc);
+      omd_cmac(K, ZP(w->W, 3 * pr[0] + l), slot(w, pr[1], pr[2], k - l), acc);
     ccopy(K, acc, ZP(w->u, q));
   }
-  for (s = 0; s < P->nseg; s++) {
-    czero(K, acc);
-    for (t = P->seg_ptr[s]; t < P->seg_ptr[s + 1]; t++) {
-      load_coef(K, P->term_coef, P->nterm, t, c);
-      omd_cmac(K, c, ZP(w->u, P->term_pair[t]), acc);
-    }
-    ccopy(K, acc, ZP(w->part, s));
-  }
   for (i = 0; i < P->n; i++) {
-    ccopy(K, ZP(w->part, P->eq_seg[i]), acc);
-    for (s = P->eq_seg[i] + 1; s < P->eq_seg[i + 1]; s++) omd_cadd(K, acc, ZP(w->part, s), acc);
-    ccopy(K, acc, ZP(w->rhs, i));
+    for (g = 0; g < G; g++) {
+      const int s = i * G + g;
+      czero(K, acc);
+      for (t = P->seg_ptr[s]; t < P->seg_ptr[s + 1]; t++) {
+        coef(P, K, t, c);
+        omd_cmac(K, c, ZP(w->u, P->term_pair[t]), acc);
+      }
+      ccopy(K, acc, ZP(w->part, g));
+    }
+    tree_sum(K, w->part, G);
+    ccopy(K, ZP(w->part, 0), ZP(w->rhs, i));
+    cneg(K, ZP(w->rhs, i));
   }
 }
 
 static void jacobian(const otr_program *P, work_t *w) {
   const int K = w->K;
-  double acc[16], c[16];
+  double acc[16], c[16], v[16];
   int q, e, t;
   for (q = 0; q < P->njpair; q++)
-    omd_cmul(K, ZP(w->W, 3 * P->jpair_comp[q]), nodec(P, w, P->jpair_node[q], 0), ZP(w->uj, q));
+    omd_cmul(K, ZP(w->W, 3 * P->jpairs[2 * q]), ZP(w->slots, P->jpairs[2 * q + 1]), ZP(w->uj, q));
   for (e = 0; e < P->n * P->n; e++) {
     czero(K, acc);
     for (t = P->jent_ptr[e]; t < P->jent_ptr[e + 1]; t++) {
-      load_coef(K, P->jterm_coef, P->njterm, t, c);
-      omd_cmac(K, c, ZP(w->uj, P->jterm_pair[t]), acc);
+      const unsigned jt = (unsigned)P->jterms[t];
+      const int term = jt & ((1u << JT_TERM_BITS) - 1);
+      const int qq = (jt >> JT_TERM_BITS) & ((1u << JT_PAIR_BITS) - 1);
+      const int ex = jt >> (JT_TERM_BITS + JT_PAIR_BITS);
+      if (ex == 1) ccopy(K, ZP(w->uj, qq), v); else cscale_d(K, ZP(w->uj, qq), (double)ex, v);
+      coef(P, K, term, c);
+      omd_cmac(K, c, v, acc);
     }
     ccopy(K, acc, ZP(w->A, e));
   }
 }
 
-static int lu_factor(const otr_program *P, work_t *w) {
+/* A := (P A)^{-1} in place, perm records P; 0 on a zero/non-finite pivot */
+static int gj_invert(const otr_program *P, work_t *w) {
   const int K = w->K, n = P->n;
-  double one[16], inv[16], tmp[16];
-  int j, i, c, p;
-  cset(K, one, 1.0);
+  double one[16], inv[16], tmp[16], *x;
+  int j, i, c, p, e;
+  cset(K, one, 1.0, 0.0);
+  for (i = 0; i < n; i++) w->perm[i] = i;
   for (j = 0; j < n; j++) {
-    double amax = absz(K, ZP(w->A, j * n + j));
+    double amax = -1.0;
     p = j;
-    for (i = j + 1; i < n; i++) {
-      const double v = absz(K, ZP(w->A, i * n + j));
+    for (i = j; i < n; i++) {
+      double v = absz(K, ZP(w->A, i * n + j));
+      if (v != v) v = INFINITY;
       if (v > amax) { amax = v; p = i; }
     }
-    if (!(amax > 0.0)) return 0;
-    if (p != j)
+    if (!(amax > 0.0) || amax == INFINITY) return 0;
+    if (p != j) {
       for (c = 0; c < n; c++) {
         ccopy(K, ZP(w->A, j * n + c), tmp);
         ccopy(K, ZP(w->A, p * n + c), ZP(w->A, j * n + c));
         ccopy(K, tmp, ZP(w->A, p * n + c));
       }
-    w->piv[j] = p;
-    omd_cdiv(K, one, ZP(w->A, j * n + j), inv);
-    for (i = j + 1; i < n; i++) omd_cmul(K, ZP(w->A, i * n + j), inv, ZP(w->A, i * n + j));
-    ccopy(K, inv, ZP(w->invd, j));
-    for (i = j + 1; i < n; i++)
-      for (c = j + 1; c < n; c++)
-        cmsub(K, ZP(w->A, i * n + j), ZP(w->A, j * n + c), ZP(w->A, i * n + c));
+      i = w->perm[j]; w->perm[j] = w->perm[p]; w->perm[p] = i;
+    }
+    for (i = 0; i < n; i++) ccopy(K, ZP(w->A, i * n + j), ZP(w->col, i));
+    omd_cdiv(K, one, ZP(w->col, j), inv);
+    for (c = 0; c < n; c++) {
+      x = ZP(w->A, j * n + c);
+      if (c == j) ccopy(K, inv, x); else omd_cmul(K, x, inv, x);
+    }
+    for (e = 0; e < n * n; e++) {
+      i = e / n;
+      c = e - i * n;
+      if (i == j) continue;
+      if (c == j) {
+        omd_cmul(K, ZP(w->col, i), inv, ZP(w->A, e));
+        cneg(K, ZP(w->A, e));
+      } else {
+        cmsub(K, ZP(w->col, i), ZP(w->A, j * n + c), ZP(w->A, e));
+      }
+    }
   }
   return 1;
 }
 
-static void lu_solve(const otr_program *P, work_t *w) {
-  const int K = w->K, n = P->n;
-  double tmp[16];
-  int j, i;
-  for (j = 0; j < n; j++) {
-    const int p = w->piv[j];
-    if (p != j) {
-      ccopy(K, ZP(w->rhs, j), tmp);
-      ccopy(K, ZP(w->rhs, p), ZP(w->rhs, j));
-      ccopy(K, tmp, ZP(w->rhs, p));
+/* y = (P A)^{-1} (P rhs), G parts per row combined by a butterfly */
+static void apply_inverse(const otr_program *P, work_t *w, double *y) {
+  const int K = w->K, n = P->n, G = P->G;
+  int i, g, j;
+  for (i = 0; i < n; i++) {
+    for (g = 0; g < G; g++) {
+      const int jlo = (g * n) / G, jhi = ((g + 1) * n) / G;
+      double *acc = ZP(w->part, g);
+      czero(K, acc);
+      for (j = jlo; j < jhi; j++) omd_cmac(K, ZP(w->A, i * n + j), ZP(w->rhs, w->perm[j]), acc);
     }
-  }
-  for (j = 0; j < n; j++)
-    for (i = j + 1; i < n; i++) cmsub(K, ZP(w->A, i * n + j), ZP(w->rhs, j), ZP(w->rhs, i));
-  for (j = n - 1; j >= 0; j--) {
-    omd_cmul(K, ZP(w->rhs, j), ZP(w->invd, j), ZP(w->rhs, j));
-    for (i = 0; i < j; i++) cmsub(K, ZP(w->A, i * n + j), ZP(w->rhs, j), ZP(w->rhs, i));
+    tree_sum(K, w->part, G);
+    ccopy(K, ZP(w->part, 0), ZP(y, i));
   }
 }
 
@@ -229,27 +264,27 @@ static double max_abs(int K, const double *v, int n) {
 }
 
 static double max_abs_x(const otr_program *P, work_t *w) {
+  const int K = w->K, D = P->deg + 1;
   double m = 0.0;
   int j;
-  for (j = 0; j < P->n; j++) m = fmax(m, absz(w->K, nodec(P, w, j, 0)));
+  for (j = 0; j < P->n; j++) m = fmax(m, absz(K, ZP(w->slots, j * D)));
   return m;
 }
 
-static int newton(const otr_program *P, work_t *w, int iters, double tol, double *dxl,
-                  int *count) {
-  const int K = w->K;
-  int it, i, j;
+static int newton(const otr_program *P, work_t *w, int iters, double tol, double *dxl, int *count) {
+  const int K = w->K, D = P->deg + 1;
+  double dx[16 * 64];
+  int it, j;
   for (it = 0; it < iters; it++) {
-    double dx;
+    double m;
     ++*count;
     eval_point(P, w, 1);
-    if (!lu_factor(P, w)) return -1;
-    for (i = 0; i < P->n; i++) cneg(K, ZP(w->rhs, i));
-    lu_solve(P, w);
-    dx = max_abs(K, w->rhs, P->n);
-    for (j = 0; j < P->n; j++) omd_cadd(K, nodec(P, w, j, 0), ZP(w->rhs, j), nodec(P, w, j, 0));
-    *dxl = dx;
-    if (dx <= tol * (1.0 + max_abs_x(P, w))) return 1;
+    if (!gj_invert(P, w)) return -1;
+    apply_inverse(P, w, dx);
+    for (j = 0; j < P->n; j++) omd_cadd(K, ZP(w->slots, j * D), ZP(dx, j), ZP(w->slots, j * D));
+    m = max_abs(K, dx, P->n);
+    *dxl = m;
+    if (m <= tol * (1.0 + max_abs_x(P, w))) return 1;
   }
   return 0;
 }
@@ -261,41 +296,45 @@ static int work_alloc(const otr_program *P, int K, work_t *w) {
   w->u = calloc((size_t)(P->npair > 0 ? P->npair : 1), zs);
   w->uj = calloc((size_t)(P->njpair > 0 ? P->njpair : 1), zs);
   w->A = calloc((size_t)P->n * P->n, zs);
-  w->part = calloc((size_t)P->nseg, zs);
   w->rhs = calloc((size_t)P->n, zs);
   w->x0 = calloc((size_t)P->n, zs);
+  w->col = calloc((size_t)P->n, zs);
   w->W = calloc((size_t)3 * P->ncomp, zs);
-  w->dnode = calloc((size_t)(P->nprod > 0 ? P->nprod : 1), zs);
-  w->invd = calloc((size_t)P->n, zs);
-  w->piv = calloc((size_t)P->n, sizeof(int));
-  return w->slots && w->u && w->uj && w->A && w->part && w->rhs && w->x0 && w->W && w->dnode &&
-         w->invd && w->piv;
+  w->dnode = calloc((size_t)(P->nfactor > 0 ? P->nfactor : 1), zs);
+  w->part = calloc((size_t)(P->G > P->n ? P->G : P->n) + 64, zs);
+  w->perm = calloc((size_t)P->n, sizeof(int));
+  return w->slots && w->u && w->uj && w->A && w->rhs && w->x0 && w->col && w->W && w->dnode &&
+         w->part && w->perm;
 }
 
 static void work_free(work_t *w) {
-  free(w->slots); free(w->u); free(w->uj); free(w->A); free(w->part); free(w->rhs);
-  free(w->x0); free(w->W); free(w->dnode); free(w->invd); free(w->piv);
+  free(w->slots); free(w->u); free(w->uj); free(w->A); free(w->rhs); free(w->x0);
+  free(w->col); free(w->W); free(w->dnode); free(w->part); free(w->perm);
 }
 
 static void load_point(const otr_program *P, work_t *w, const double *x, int KS, int npaths,
                        int path) {
-  const int K = w->K, n = P->n;
-  int j, i;
+  const int K = w->K, n = P->n, D = P->deg + 1;
+  int j, i, a;
   for (j = 0; j < n; j++) {
-    double *z = nodec(P, w, j, 0);
+    double *z = ZP(w->slots, j * D);
     czero(K, z);
     for (i = 0; i < K && i < KS; i++) {
       z[i] = x[(size_t)i * npaths * n + (size_t)path * n + j];
       z[K + i] = x[(size_t)(KS + i) * npaths * n + (size_t)path * n + j];
     }
   }
+  for (a = 0; a < D; a++) {
+    if (a == 0) cset(K, ZP(w->slots, n * D), 1.0, 0.0);
+    else czero(K, ZP(w->slots, n * D + a));
+  }
 }
 
 static void store_point(const otr_program *P, work_t *w, double *x, int npaths, int path) {
-  const int K = w->K, n = P->n;
+  const int K = w->K, n = P->n, D = P->deg + 1;
   int j, i;
   for (j = 0; j < n; j++) {
-    const double *z = nodec(P, w, j, 0);
+    const double *z = ZP(w->slots, j * D);
     for (i = 0; i < K; i++) {
       x[(size_t)i * npaths * n + (size_t)path * n + j] = z[i];
       x[(size_t)(K + i) * npaths * n + (size_t)path * n + j] = z[K + i];
@@ -303,29 +342,18 @@ static void store_point(const otr_program *P, work_t *w, double *x, int npaths, 
   }
 }
 
-static void set_one(const otr_program *P, work_t *w) {
-  const int K = w->K, D = P->deg + 1;
-  int a;
-  for (a = 0; a < D; a++) {
-    if (a == 0) cset(K, nodec(P, w, P->n, a), 1.0);
-    else czero(K, nodec(P, w, P->n, a));
-  }
-}
-
-/* Taylor coefficients x_1..x_deg at the point in the order-0 slots;
- * returns 0 on a singular Jacobian */
 static int predictor_series(const otr_program *P, work_t *w) {
   const int K = w->K, n = P->n, D = P->deg + 1;
-  int k, j, i;
+  double xk[16 * 64];
+  int k, j;
   eval_point(P, w, 1);
-  if (!lu_factor(P, w)) return 0;
+  if (!gj_invert(P, w)) return 0;
   for (k = 1; k < D; k++) {
-    for (j = 0; j < n; j++) czero(K, nodec(P, w, j, k));
+    for (j = 0; j < n; j++) czero(K, ZP(w->slots, j * D + k));
     product_pass(P, w, k);
     residual(P, w, k);
-    for (i = 0; i < n; i++) cneg(K, ZP(w->rhs, i));
-    lu_solve(P, w);
-    for (j = 0; j < n; j++) ccopy(K, ZP(w->rhs, j), nodec(P, w, j, k));
+    apply_inverse(P, w, xk);
+    for (j = 0; j < n; j++) ccopy(K, ZP(xk, j), ZP(w->slots, j * D + k));
     correction_pass(P, w, k);
   }
   return 1;
@@ -337,29 +365,25 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
   const int n = P->n, D = P->deg + 1;
   work_t w;
   int path;
-  if (!work_alloc(P, K, &w)) { work_free(&w); return; }
+  if (n > 64 || !work_alloc(P, K, &w)) { work_free(&w); return; }
   for (path = p0; path < p1; path++) {
     double t = 0.0, h = prm->maxstep, dxl = 0.0;
     int st = ST_RUNNING, steps = 0, rej = 0, j, k, npred = 0, nnewt = 0, nfeval = 0;
     load_point(P, &w, start, KS, npaths, path);
-    set_one(P, &w);
     while (st == ST_RUNNING && t < 1.0) {
       const int kd = D - 1 - prm->fabry_offset;
-      double rad = -1.0;
+      double r = INFINITY;
       int accepted = 0;
       set_weights(P, &w, t);
       ++npred;
       if (!predictor_series(P, &w)) { st = ST_SINGULAR; break; }
       if (kd >= 1)
         for (j = 0; j < n; j++) {
-          const double den = absz(K, nodec(P, &w, j, kd));
-          if (den > 0.0) {
-            const double r = absz(K, nodec(P, &w, j, kd - 1)) / den;
-            if (rad < 0.0 || r < rad) rad = r;
-          }
+          const double den = absz(K, ZP(w.slots, j * D + kd));
+          if (den > 0.0) r = fmin(r, absz(K, ZP(w.slots, j * D + kd - 1)) / den);
         }
-      h = (rad > 0.0) ? fmin(prm->pbeta * rad, prm->maxstep) : prm->maxstep;
-      for (j = 0; j < n; j++) ccopy(K, nodec(P, &w, j, 0), ZP(w.x0, j));
+      h = (r < INFINITY && r > 0.0) ? fmin(prm->pbeta * r, prm->maxstep) : prm->maxstep;
+      for (j = 0; j < n; j++) ccopy(K, ZP(w.slots, j * D), ZP(w.x0, j));
       while (!accepted) {
         double tn = t + h, hs, res;
         int ok;
@@ -367,13 +391,13 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
         hs = tn - t;
         for (j = 0; j < n; j++) {
           double acc[16], tmp[16];
-          ccopy(K, nodec(P, &w, j, D - 1), acc);
+          ccopy(K, ZP(w.slots, j * D + D - 1), acc);
           for (k = D - 2; k >= 1; k--) {
             cscale_d(K, acc, hs, tmp);
-            omd_cadd(K, tmp, nodec(P, &w, j, k), acc);
+            omd_cadd(K, tmp, ZP(w.slots, j * D + k), acc);
           }
           cscale_d(K, acc, hs, tmp);
-          omd_cadd(K, ZP(w.x0, j), tmp, nodec(P, &w, j, 0));
+          omd_cadd(K, ZP(w.x0, j), tmp, ZP(w.slots, j * D));
         }
         set_weights(P, &w, tn);
         ++nfeval;
@@ -458,14 +482,12 @@ void otr_refine(const otr_program *P, int K, int npaths, double t, const double 
                 double *xout, int iters, double tol, double *dxlast, double *res, int *status) {
   work_t w;
   int path;
-  if (!work_alloc(P, K, &w)) { work_free(&w); return; }
+  if (P->n > 64 || !work_alloc(P, K, &w)) { work_free(&w); return; }
   for (path = 0; path < npaths; path++) {
     double dxl = 0.0;
-    int nr = 1;
+    int nr = 1, its = 0;
     load_point(P, &w, xin, KS, npaths, path);
-    cset(K, nodec(P, &w, P->n, 0), 1.0);
     set_weights(P, &w, t);
-    int its = 0;
     if (iters > 0) nr = newton(P, &w, iters, tol, &dxl, &its);
     eval_point(P, &w, 0);
     res[path] = max_abs(K, w.rhs, P->n);
@@ -482,14 +504,15 @@ void otr_eval(const otr_program *P, int K, double t, const double *x, double *va
   int i, j, e;
   if (!work_alloc(P, K, &w)) { work_free(&w); return; }
   load_point(P, &w, x, K, 1, 0);
-  set_one(P, &w);
   set_weights(P, &w, t);
   eval_point(P, &w, 1);
-  for (i = 0; i < n; i++)
+  for (i = 0; i < n; i++) {
+    cneg(K, ZP(w.rhs, i));  /* the residual routine stores -H */
     for (j = 0; j < K; j++) {
       val[(size_t)j * n + i] = ZP(w.rhs, i)[j];
       val[(size_t)(K + j) * n + i] = ZP(w.rhs, i)[K + j];
     }
+  }
   for (e = 0; e < n * n; e++)
     for (j = 0; j < K; j++) {
       jac[(size_t)j * n * n + e] = ZP(w.A, e)[j];
@@ -502,16 +525,15 @@ int otr_series(const otr_program *P, int K, double t, const double *x, double *s
   const int n = P->n, D = P->deg + 1;
   work_t w;
   int j, k, i, ok;
-  if (!work_alloc(P, K, &w)) { work_free(&w); return 0; }
+  if (P->n > 64 || !work_alloc(P, K, &w)) { work_free(&w); return 0; }
   load_point(P, &w, x, K, 1, 0);
-  set_one(P, &w);
   set_weights(P, &w, t);
   ok = predictor_series(P, &w);
   for (j = 0; j < n; j++)
     for (k = 0; k < D; k++)
       for (i = 0; i < K; i++) {
-        series[((size_t)i * n + j) * D + k] = nodec(P, &w, j, k)[i];
-        series[((size_t)(K + i) * n + j) * D + k] = nodec(P, &w, j, k)[K + i];
+        series[((size_t)i * n + j) * D + k] = ZP(w.slots, j * D + k)[i];
+        series[((size_t)(K + i) * n + j) * D + k] = ZP(w.slots, j * D + k)[K + i];
       }
   work_free(&w);
   return ok;

@@ -9,16 +9,18 @@ extern "C" {
 #endif
 
 typedef struct {
-  int n, deg, nprod, nlev, nslots, ncomp, wdeg;
-  const int *left, *right, *lev_ptr, *off, *slen;
+  int n, deg, ncomp, wdeg, G;
+  int nprod, nlev, nslots, nfactor;
+  const int *nodes, *lev_ptr;
   int npair;
-  const int *pair_comp, *pair_node;
-  int nseg, nterm;
-  const int *seg_ptr, *eq_seg, *term_pair;
+  const int *pairs;
+  int nterm;
+  const int *seg_ptr, *term_pair;
   const double *term_coef;
-  int njpair, njterm;
-  const int *jpair_comp, *jpair_node, *jent_ptr, *jterm_pair;
-  const double *jterm_coef;
+  int njpair;
+  const int *jpairs;
+  int njterm;
+  const int *jent_ptr, *jterms;
   const double *weights;
 } otr_program;
 

@@ -49,14 +49,13 @@ SIGNATURES = {
 
 class PhcProgram(ctypes.Structure):
     _fields_ = [
-        ("n", c_int), ("deg", c_int), ("nprod", c_int), ("nlev", c_int), ("nslots", c_int),
-        ("ncomp", c_int), ("wdeg", c_int),
-        ("left", P_int), ("right", P_int), ("lev_ptr", P_int), ("off", P_int), ("slen", P_int),
-        ("npair", c_int), ("pair_comp", P_int), ("pair_node", P_int),
-        ("nseg", c_int), ("nterm", c_int), ("seg_ptr", P_int), ("eq_seg", P_int),
-        ("term_pair", P_int), ("term_coef", P_double),
-        ("njpair", c_int), ("njterm", c_int), ("jpair_comp", P_int), ("jpair_node", P_int),
-        ("jent_ptr", P_int), ("jterm_pair", P_int), ("jterm_coef", P_double),
+        ("n", c_int), ("deg", c_int), ("ncomp", c_int), ("wdeg", c_int), ("G", c_int),
+        ("nprod", c_int), ("nlev", c_int), ("nslots", c_int), ("nfactor", c_int),
+        ("nodes", P_int), ("lev_ptr", P_int),
+        ("npair", c_int), ("pairs", P_int),
+        ("nterm", c_int), ("seg_ptr", P_int), ("term_pair", P_int), ("term_coef", P_double),
+        ("njpair", c_int), ("jpairs", P_int),
+        ("njterm", c_int), ("jent_ptr", P_int), ("jterms", P_int),
         ("weights", P_double),
     ]
 
@@ -74,7 +73,8 @@ SIGNATURES.update({
     "phc_tracker_create": (c_int, [ctypes.POINTER(c_void_p), ctypes.POINTER(PhcProgram)]),
     "phc_tracker_destroy": (c_int, [c_void_p]),
     "phc_program_validate": (c_int, [ctypes.POINTER(PhcProgram)]),
-    "phc_tracker_lds_bytes": (c_int, [c_void_p, c_int, ctypes.POINTER(ctypes.c_size_t)]),
+    "phc_tracker_lds_bytes": (c_int, [c_void_p, c_int, ctypes.POINTER(ctypes.c_size_t),
+                                      ctypes.POINTER(ctypes.c_size_t)]),
     "phc_track": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                           c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(PhcTrackParams),
                           c_int, c_void_p]),

@@ -1,5 +1,5 @@
 // Batched predictor-corrector path tracking of homotopies on gfx950, with
-// endpoint Newton refinement and residuals in any of double, double-double,
+// endpoint Newton refinement and residuals in double, double-double,
 // quad-double and octo-double.
 //
 // Reference behaviour (restated, not translated):
@@ -12,28 +12,30 @@
 //                    Bounded_Update (three_way_minima.adb:96-109), and the
 //                    predictor feedback that halves the step while the
 //                    predicted residual exceeds alpha (Predictor_Feedback)
-//   corrector        LU Newton steps, step halved on failure down to minstep
+//   corrector        Newton steps, step halved on failure down to minstep
 //   batching         GPU_Path_mult (Path/Path_GPU/path_gpu_mult.cu:84-308)
 //                    compacts live paths on the host after every step; here
-//                    each workgroup pulls the next path from an atomic queue
-//                    when its path ends, so no path waits and nothing returns
-//                    to the host until all paths are done.
+//                    every wavefront pulls the next path from an atomic queue
+//                    when its path ends, so nothing returns to the host until
+//                    all paths are done.
 //
 // Power series of the solution curve.  At the base point (x0, t0) the
-// Taylor coefficients x_k of x(t0+s) are computed order by order: with the
-// coefficient x_k set to zero, coefficient k of H(x(s), t0+s) is evaluated
-// through the monomial product DAG; it is affine in x_k with the Jacobian
-// A0 = H_x(x0,t0) as linear part, so x_k = -A0^{-1} [H]_k.  This is the lower
-// block-triangular Toeplitz solve of power-series Newton
-// (Newton/dbl2_bals_host.cpp:228-335, x_k = A0^{-1}(b_k - sum_j A_j x_{k-j}))
-// with the tail sum formed implicitly by the convolutions, one LU of A0 per
-// step and no stored Jacobian series.
+// Taylor coefficients x_k of x(t0+s) are computed order by order: with x_k
+// set to zero, coefficient k of H(x(s), t0+s) is evaluated through the
+// monomial product DAG; it is affine in x_k with the Jacobian A0 = H_x(x0,t0)
+// as linear part, so x_k = -A0^{-1} [H]_k.  This is the lower block-triangular
+// Toeplitz solve of power-series Newton (Newton/dbl2_bals_host.cpp:228-335,
+// x_k = A0^{-1}(b_k - sum_j A_j x_{k-j})) with the tail sum formed by the
+// convolutions themselves.  A0 is inverted once per step (Gauss-Jordan with
+// partial pivoting), so every order costs one matrix-vector product.
 //
-// Layout: one workgroup per path; every per-path array lives in LDS
-// (node series, weighted component values, Jacobian LU, vectors).  Lanes
-// split the work of each phase (nodes of a DAG level, residual segments,
-// Jacobian entries, matrix rows); all control decisions are computed by
-// every lane from LDS values, so branches and barriers stay uniform.
+// Layout.  A workgroup of W wavefronts stages the compiled program (node
+// metadata, terms, coefficients) into LDS once; then each wavefront tracks
+// its own paths, one at a time, with all per-path state in its own LDS
+// region.  Wavefronts never wait for each other: lanes of one wavefront
+// exchange data through LDS with wavefront-scope ordering only, and sums
+// across the G lanes that share an equation use butterfly shuffles.  Every
+// control decision is computed identically by all lanes of the wavefront.
 #include <hip/hip_runtime.h>
 #include <vector>
 #include "common.hpp"
@@ -47,20 +49,15 @@ namespace {
 enum { ST_RUNNING = 0, ST_SUCCESS = 1, ST_MINSTEP = 2, ST_MAXSTEPS = 3, ST_SINGULAR = 4,
        ST_DIVERGED = 5 };
 
+constexpr int JT_TERM_BITS = 16, JT_PAIR_BITS = 12;
+constexpr size_t LDS_LIMIT = 160 * 1024;
+
+// device copy of the program (global memory)
 struct DevProgram {
-  int n, D, nprod, nlev, nslots, ring, wdeg, ncomp;
-  const int *left, *right, *lev_ptr, *off, *slen;
-  int npair;
-  const int *pair_comp, *pair_node;
-  int nseg;
-  const int *seg_ptr, *eq_seg, *term_pair;
-  int nterm;
-  const double *term_coef;
-  int njpair;
-  const int *jpair_comp, *jpair_node, *jent_ptr, *jterm_pair;
-  int njterm;
-  const double *jterm_coef;
-  const double *weights;
+  int n, D, ncomp, wdeg, G, nprod, nlev, nslots, nfactor, npair, nterm, njpair, njterm;
+  const int *nodes, *lev_ptr, *pairs, *seg_ptr, *term_pair, *jpairs, *jent_ptr, *jterms;
+  const double *term_coef;  // [2][nterm]
+  const double *weights;    // [2][8][3*ncomp]
 };
 
 struct Params {
@@ -68,329 +65,462 @@ struct Params {
   double maxstep, minstep, pbeta, alpha, tolcor, tolend, divergence;
 };
 
-// complex coefficient j of an array of m coefficients stored as octo-double
-// planes [2][8][m]; the leading K limbs are used
-template <int K>
-__device__ __forceinline__ cplx<K> load_coef(const double *base, int m, int j) {
-  cplx<K> z;
-  z.re = R<K>::load(base, m, j);
-  z.im = R<K>::load(base + 8 * (size_t)m, m, j);
-  return z;
+__host__ __device__ inline size_t al16(size_t b) { return (b + 15) & ~(size_t)15; }
+
+// byte offsets of the program sections in LDS
+struct ProgLayout {
+  size_t nodes, lev, pairs, seg, tpair, tcoef, jpairs, jent, jterms, total;
+  __host__ __device__ explicit ProgLayout(const DevProgram &P) {
+    size_t o = 0;
+    nodes = o; o = al16(o + sizeof(int) * 8 * P.nprod);
+    lev = o; o = al16(o + sizeof(int) * (P.nlev + 1));
+    pairs = o; o = al16(o + sizeof(int) * 4 * P.npair);
+    seg = o; o = al16(o + sizeof(int) * (P.n * P.G + 1));
+    tpair = o; o = al16(o + sizeof(int) * P.nterm);
+    tcoef = o; o = al16(o + sizeof(double) * 2 * P.nterm);
+    jpairs = o; o = al16(o + sizeof(int) * 2 * P.njpair);
+    jent = o; o = al16(o + sizeof(int) * (P.n * P.n + 1));
+    jterms = o; o = al16(o + sizeof(int) * P.njterm);
+    total = o;
+  }
+};
+
+// per-wavefront path region (bytes)
+template <int K> struct PathLayout {
+  size_t slots, u, uj, A, rhs, x0, col, W, dnode, perm, total;
+  __host__ __device__ explicit PathLayout(const DevProgram &P) {
+    const size_t z = sizeof(cplx<K>);
+    size_t o = 0;
+    slots = o; o += z * P.nslots;
+    u = o; o += z * P.npair;
+    uj = o; o += z * P.njpair;
+    A = o; o += z * P.n * P.n;
+    rhs = o; o += z * P.n;
+    x0 = o; o += z * P.n;
+    col = o; o += z * P.n;
+    W = o; o += z * 3 * P.ncomp;
+    dnode = o; o += z * (P.nfactor > 0 ? P.nfactor : 1);
+    perm = o; o = al16(o + sizeof(int) * P.n);
+    total = al16(o);
+  }
+};
+
+// the program as seen from LDS
+struct LProg {
+  int n, D, ncomp, wdeg, G, nprod, nlev, npair, nterm, njpair;
+  const int *nodes, *lev, *pairs, *seg, *tpair, *jpairs, *jent, *jterms;
+  const double2 *tcoef;
+  const double *weights;  // global
+};
+
+template <int K> struct Path {
+  cplx<K> *slots, *u, *uj, *A, *rhs, *x0, *col, *W, *dnode;
+  int *perm;
+};
+
+__device__ __forceinline__ void wsync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int K> struct Work {
-  typedef cplx<K> Z;
-  Z *slots, *u, *uj, *A, *part, *rhs, *x0, *W, *dnode, *invd;
-  int *piv;
-  double *red;  // reduction scratch (one double per lane)
-};
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
 template <int K> __device__ __forceinline__ double absz(const cplx<K> &z) {
   return fabs(R<K>::hi(z.re)) + fabs(R<K>::hi(z.im));
 }
 
-// node m coefficient a (a >= 0)
-template <int K>
-__device__ __forceinline__ cplx<K> &nodec(const DevProgram &P, Work<K> &w, int m, int a) {
-  return w.slots[P.off[m] + (a % P.slen[m])];
+template <int K> __device__ __forceinline__ cplx<K> shfl_xor_z(const cplx<K> &z, int m) {
+  cplx<K> r;
+  const double *a = reinterpret_cast<const double *>(&z);
+  double *b = reinterpret_cast<double *>(&r);
+#pragma unroll
+  for (int i = 0; i < 2 * K; ++i) b[i] = __shfl_xor(a[i], m, 64);
+  return r;
+}
+
+// sum over the aligned group of G lanes (butterfly; identical in every lane)
+template <int K> __device__ __forceinline__ cplx<K> group_sum(cplx<K> v, int G) {
+  for (int m = 1; m < G; m <<= 1) v = C<K>::add(v, shfl_xor_z<K>(v, m));
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int m = 1; m < 64; m <<= 1) v = fmax(v, __shfl_xor(v, m, 64));
+  return v;
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+  for (int m = 1; m < 64; m <<= 1) v = fmin(v, __shfl_xor(v, m, 64));
+  return v;
+}
+
+template <int K> __device__ __forceinline__ cplx<K> coef(const LProg &P, int t) {
+  const double2 c = P.tcoef[t];
+  return C<K>::from(c.x, c.y);
+}
+
+template <int K> __device__ __forceinline__ cplx<K> load_weight(const LProg &P, int j) {
+  cplx<K> z;
+  z.re = R<K>::load(P.weights, 3 * P.ncomp, j);
+  z.im = R<K>::load(P.weights + 8 * 3 * (size_t)P.ncomp, 3 * P.ncomp, j);
+  return z;
 }
 
 // Taylor coefficients W[r][l] at t of w_r(t) = om0 + om1 t + om2 t^2
-template <int K> __device__ void set_weights(const DevProgram &P, Work<K> &w, double t) {
+template <int K> __device__ void set_weights(const LProg &P, Path<K> &w, double t) {
   typedef C<K> Cx;
-  for (int r = threadIdx.x; r < P.ncomp; r += blockDim.x) {
-    cplx<K> o0 = load_coef<K>(P.weights, P.ncomp * 3, 3 * r);
-    cplx<K> o1 = load_coef<K>(P.weights, P.ncomp * 3, 3 * r + 1);
-    cplx<K> o2 = load_coef<K>(P.weights, P.ncomp * 3, 3 * r + 2);
+  const int l = lane_id();
+  if (l < P.ncomp) {
+    cplx<K> o0 = load_weight<K>(P, 3 * l), o1 = load_weight<K>(P, 3 * l + 1),
+            o2 = load_weight<K>(P, 3 * l + 2);
     cplx<K> o2t = Cx::scale_d(o2, t);
-    w.W[3 * r] = Cx::add(o0, Cx::scale_d(Cx::add(o1, o2t), t));
-    w.W[3 * r + 1] = Cx::add(o1, Cx::scale_d(o2t, 2.0));
-    w.W[3 * r + 2] = o2;
+    w.W[3 * l] = Cx::add(o0, Cx::scale_d(Cx::add(o1, o2t), t));
+    w.W[3 * l + 1] = Cx::add(o1, Cx::scale_d(o2t, 2.0));
+    w.W[3 * l + 2] = o2;
   }
+  wsync();
 }
 
-// products of the DAG at order k: node_k = sum_a left_a * right_{k-a}
-template <int K> __device__ void product_pass(const DevProgram &P, Work<K> &w, int k) {
+// node_k = sum_q left_q * right_{k-q} for every product node, level by level
+template <int K> __device__ void product_pass(const LProg &P, Path<K> &w, int k) {
   typedef C<K> Cx;
   for (int lv = 1; lv <= P.nlev; ++lv) {
-    for (int i = P.lev_ptr[lv - 1] + threadIdx.x; i < P.lev_ptr[lv]; i += blockDim.x) {
-      const int a = P.left[i], b = P.right[i];
-      cplx<K> acc = Cx::mul(nodec(P, w, a, 0), nodec(P, w, b, k));
-      for (int q = 1; q <= k; ++q) Cx::mac(acc, nodec(P, w, a, q), nodec(P, w, b, k - q));
-      nodec(P, w, P.n + 1 + i, k) = acc;
+    for (int i = P.lev[lv - 1] + lane_id(); i < P.lev[lv]; i += 64) {
+      const int *nd = P.nodes + 8 * i;
+      const int aoff = nd[0], boff = nd[2], soff = nd[3], smask = nd[4];
+      cplx<K> acc = Cx::mul(w.slots[aoff], w.slots[boff + k]);
+      for (int q = 1; q <= k; ++q) Cx::mac(acc, w.slots[aoff + q], w.slots[boff + k - q]);
+      w.slots[soff + (k & smask)] = acc;
     }
-    __syncthreads();
+    wsync();
   }
 }
 
 // node_k += left_0*dright_k + dleft_k*right_0 once x_k is known
-template <int K> __device__ void correction_pass(const DevProgram &P, Work<K> &w, int k) {
+template <int K> __device__ void correction_pass(const LProg &P, Path<K> &w, int k) {
   typedef C<K> Cx;
   for (int lv = 1; lv <= P.nlev; ++lv) {
-    for (int i = P.lev_ptr[lv - 1] + threadIdx.x; i < P.lev_ptr[lv]; i += blockDim.x) {
-      const int a = P.left[i], b = P.right[i];  // b is a variable
-      cplx<K> db = nodec(P, w, b, k);
-      cplx<K> da = (a < P.n) ? nodec(P, w, a, k) : (a == P.n ? Cx::zero() : w.dnode[a - P.n - 1]);
-      cplx<K> d = Cx::mul(nodec(P, w, a, 0), db);
-      Cx::mac(d, da, nodec(P, w, b, 0));
-      w.dnode[i] = d;
-      cplx<K> &v = nodec(P, w, P.n + 1 + i, k);
+    for (int i = P.lev[lv - 1] + lane_id(); i < P.lev[lv]; i += 64) {
+      const int *nd = P.nodes + 8 * i;
+      const int aoff = nd[0], boff = nd[2], soff = nd[3], smask = nd[4], adn = nd[5], sdn = nd[6];
+      const cplx<K> da = (adn < 0) ? w.slots[aoff + k] : w.dnode[adn];
+      cplx<K> d = Cx::mul(w.slots[aoff], w.slots[boff + k]);
+      Cx::mac(d, da, w.slots[boff]);
+      if (sdn >= 0) w.dnode[sdn] = d;
+      cplx<K> &v = w.slots[soff + (k & smask)];
       v = Cx::add(v, d);
     }
-    __syncthreads();
+    wsync();
   }
 }
 
-// residual rhs_i = [H_i]_k from weighted component values at order k
-template <int K> __device__ void residual(const DevProgram &P, Work<K> &w, int k) {
+// rhs_i = -[H_i]_k from the weighted component values at order k
+template <int K> __device__ void residual(const LProg &P, Path<K> &w, int k) {
   typedef C<K> Cx;
-  for (int q = threadIdx.x; q < P.npair; q += blockDim.x) {
-    const int r = P.pair_comp[q], m = P.pair_node[q];
-    cplx<K> acc = Cx::mul(w.W[3 * r], nodec(P, w, m, k));
-    for (int l = 1; l <= P.wdeg && l <= k; ++l) Cx::mac(acc, w.W[3 * r + l], nodec(P, w, m, k - l));
+  const int l = lane_id();
+  for (int q = l; q < P.npair; q += 64) {
+    const int *pr = P.pairs + 4 * q;
+    const int r = pr[0], off = pr[1], mask = pr[2];
+    cplx<K> acc = Cx::mul(w.W[3 * r], w.slots[off + (k & mask)]);
+    for (int j = 1; j <= P.wdeg && j <= k; ++j)
+      Cx::mac(acc, w.W[3 * r + j], w.slots[off + ((k - j) & mask)]);
     w.u[q] = acc;
   }
-  __syncthreads();
-  for (int s = threadIdx.x; s < P.nseg; s += blockDim.x) {
-    cplx<K> acc = Cx::zero();
-    for (int t = P.seg_ptr[s]; t < P.seg_ptr[s + 1]; ++t)
-      Cx::mac(acc, load_coef<K>(P.term_coef, P.nterm, t), w.u[P.term_pair[t]]);
-    w.part[s] = acc;
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < P.n; i += blockDim.x) {
-    cplx<K> acc = w.part[P.eq_seg[i]];
-    for (int s = P.eq_seg[i] + 1; s < P.eq_seg[i + 1]; ++s) acc = Cx::add(acc, w.part[s]);
-    w.rhs[i] = acc;
-  }
-  __syncthreads();
+  wsync();
+  cplx<K> acc = Cx::zero();
+  if (l < P.n * P.G)
+    for (int t = P.seg[l]; t < P.seg[l + 1]; ++t) Cx::mac(acc, coef<K>(P, t), w.u[P.tpair[t]]);
+  acc = group_sum<K>(acc, P.G);
+  if (l < P.n * P.G && (l % P.G) == 0) w.rhs[l / P.G] = Cx::neg(acc);
+  wsync();
 }
 
-// Jacobian A_ij = sum c * e * W_r0 * node(m/x_j)_0
-template <int K> __device__ void jacobian(const DevProgram &P, Work<K> &w) {
+// Jacobian A_ij = sum coef * e * W_r0 * node(m/x_j)_0
+template <int K> __device__ void jacobian(const LProg &P, Path<K> &w) {
   typedef C<K> Cx;
-  for (int q = threadIdx.x; q < P.njpair; q += blockDim.x)
-    w.uj[q] = Cx::mul(w.W[3 * P.jpair_comp[q]], nodec(P, w, P.jpair_node[q], 0));
-  __syncthreads();
-  const int nn = P.n * P.n;
-  for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+  const int l = lane_id();
+  for (int q = l; q < P.njpair; q += 64) {
+    const int *jp = P.jpairs + 2 * q;
+    w.uj[q] = Cx::mul(w.W[3 * jp[0]], w.slots[jp[1]]);
+  }
+  wsync();
+  for (int e = l; e < P.n * P.n; e += 64) {
     cplx<K> acc = Cx::zero();
-    for (int t = P.jent_ptr[e]; t < P.jent_ptr[e + 1]; ++t)
-      Cx::mac(acc, load_coef<K>(P.jterm_coef, P.njterm, t), w.uj[P.jterm_pair[t]]);
+    for (int t = P.jent[e]; t < P.jent[e + 1]; ++t) {
+      const unsigned jt = (unsigned)P.jterms[t];
+      const int term = jt & ((1u << JT_TERM_BITS) - 1);
+      const int q = (jt >> JT_TERM_BITS) & ((1u << JT_PAIR_BITS) - 1);
+      const int ex = jt >> (JT_TERM_BITS + JT_PAIR_BITS);
+      const cplx<K> v = (ex == 1) ? w.uj[q] : Cx::scale_d(w.uj[q], (double)ex);
+      Cx::mac(acc, coef<K>(P, term), v);
+    }
     w.A[e] = acc;
   }
-  __syncthreads();
+  wsync();
 }
 
-// in-place LU with partial pivoting; returns false if a pivot is zero
-template <int K> __device__ bool lu_factor(const DevProgram &P, Work<K> &w) {
+// In-place Gauss-Jordan inverse with partial pivoting: A := (P A)^{-1}
+// where the row permutation P is recorded in perm ((P r)_i = r[perm[i]]).
+// Returns false on a zero or non-finite pivot.
+template <int K> __device__ bool gj_invert(const LProg &P, Path<K> &w) {
   typedef C<K> Cx;
-  const int n = P.n;
+  const int n = P.n, l = lane_id();
+  if (l < n) w.perm[l] = l;
+  wsync();
   for (int j = 0; j < n; ++j) {
-    int p = j;
-    double amax = absz<K>(w.A[j * n + j]);
-    for (int i = j + 1; i < n; ++i) {
-      double v = absz<K>(w.A[i * n + j]);
-      if (v > amax) { amax = v; p = i; }
+    double v = -1.0;
+    int idx = l;
+    if (l >= j && l < n) {
+      v = absz<K>(w.A[l * n + j]);
+      if (v != v) v = INFINITY;
     }
-    if (!(amax > 0.0)) return false;
-    __syncthreads();
-    if (p != j)
-      for (int c = threadIdx.x; c < n; c += blockDim.x) {
-        cplx<K> tmp = w.A[j * n + c];
+    for (int m = 1; m < 64; m <<= 1) {
+      const double ov = __shfl_xor(v, m, 64);
+      const int oi = __shfl_xor(idx, m, 64);
+      if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+    }
+    if (!(v > 0.0) || v == INFINITY) return false;
+    const int p = idx;
+    if (p != j) {
+      for (int c = l; c < n; c += 64) {
+        const cplx<K> tmp = w.A[j * n + c];
         w.A[j * n + c] = w.A[p * n + c];
         w.A[p * n + c] = tmp;
       }
-    if (threadIdx.x == 0) w.piv[j] = p;
-    __syncthreads();
-    const cplx<K> inv = Cx::div(Cx::from(1.0, 0.0), w.A[j * n + j]);
-    for (int i = j + 1 + threadIdx.x; i < n; i += blockDim.x) w.A[i * n + j] = Cx::mul(w.A[i * n + j], inv);
-    if (threadIdx.x == 0) w.invd[j] = inv;
-    __syncthreads();
-    const int m = n - j - 1;
-    for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
-      const int i = j + 1 + e / m, c = j + 1 + e % m;
-      Cx::msub(w.A[i * n + c], w.A[i * n + j], w.A[j * n + c]);
+      if (l == 0) { const int t = w.perm[j]; w.perm[j] = w.perm[p]; w.perm[p] = t; }
+      wsync();
     }
-    __syncthreads();
+    if (l < n) w.col[l] = w.A[l * n + j];
+    wsync();
+    const cplx<K> inv = Cx::div(Cx::from(1.0, 0.0), w.col[j]);
+    for (int c = l; c < n; c += 64) w.A[j * n + c] = (c == j) ? inv : Cx::mul(w.A[j * n + c], inv);
+    wsync();
+    for (int e = l; e < n * n; e += 64) {
+      const int i = e / n, c = e - i * n;
+      if (i == j) continue;
+      if (c == j) w.A[e] = Cx::neg(Cx::mul(w.col[i], inv));
+      else Cx::msub(w.A[e], w.col[i], w.A[j * n + c]);
+    }
+    wsync();
   }
   return true;
 }
 
-// rhs := A^{-1} rhs with the factors of lu_factor
-template <int K> __device__ void lu_solve(const DevProgram &P, Work<K> &w) {
+// y = (P A)^{-1} (P rhs): lanes i*G+g sum columns jlo..jhi of row i, then
+// the G partial sums meet in a butterfly; row i's result is in the lanes of
+// group i.
+template <int K> __device__ cplx<K> apply_inverse(const LProg &P, Path<K> &w) {
   typedef C<K> Cx;
-  const int n = P.n;
-  if (threadIdx.x == 0)
-    for (int j = 0; j < n; ++j) {
-      const int p = w.piv[j];
-      if (p != j) { cplx<K> t = w.rhs[j]; w.rhs[j] = w.rhs[p]; w.rhs[p] = t; }
-    }
-  __syncthreads();
-  for (int j = 0; j < n; ++j) {
-    for (int i = j + 1 + threadIdx.x; i < n; i += blockDim.x) Cx::msub(w.rhs[i], w.A[i * n + j], w.rhs[j]);
-    __syncthreads();
+  const int n = P.n, G = P.G, l = lane_id();
+  cplx<K> acc = Cx::zero();
+  if (l < n * G) {
+    const int i = l / G, g = l - i * G;
+    const int jlo = (g * n) / G, jhi = ((g + 1) * n) / G;
+    for (int j = jlo; j < jhi; ++j) Cx::mac(acc, w.A[i * n + j], w.rhs[w.perm[j]]);
   }
-  for (int j = n - 1; j >= 0; --j) {
-    if (threadIdx.x == 0) w.rhs[j] = Cx::mul(w.rhs[j], w.invd[j]);
-    __syncthreads();
-    for (int i = threadIdx.x; i < j; i += blockDim.x) Cx::msub(w.rhs[i], w.A[i * n + j], w.rhs[j]);
-    __syncthreads();
-  }
+  return group_sum<K>(acc, G);
 }
 
-// value (and optionally Jacobian) at the point held in the order-0 slots
-template <int K> __device__ void eval_point(const DevProgram &P, Work<K> &w, bool jac) {
+template <int K> __device__ void eval_point(const LProg &P, Path<K> &w, bool jac) {
   product_pass<K>(P, w, 0);
   residual<K>(P, w, 0);
   if (jac) jacobian<K>(P, w);
 }
 
-template <int K> __device__ double max_abs(const cplx<K> *v, int n) {
-  double m = 0.0;
-  for (int i = 0; i < n; ++i) m = fmax(m, absz<K>(v[i]));
-  return m;
+template <int K> __device__ double max_abs_rhs(const LProg &P, Path<K> &w) {
+  const int l = lane_id();
+  return wave_max(l < P.n ? absz<K>(w.rhs[l]) : 0.0);
 }
 
-template <int K> __device__ double max_abs_x(const DevProgram &P, Work<K> &w) {
-  double m = 0.0;
-  for (int j = 0; j < P.n; ++j) m = fmax(m, absz<K>(nodec(P, w, j, 0)));
-  return m;
+template <int K> __device__ double max_abs_x(const LProg &P, Path<K> &w) {
+  const int l = lane_id();
+  return wave_max(l < P.n ? absz<K>(w.slots[l * P.D]) : 0.0);
 }
 
-// Newton steps at fixed t on the order-0 slots; returns status
-// (1 converged, 0 not converged, -1 singular); *dxl = last max |dx|
+// Newton at fixed t on the order-0 slots (weights already set): returns
+// 1 converged, 0 not converged, -1 singular; *dxl = last max |dx|
 template <int K>
-__device__ int newton(const DevProgram &P, Work<K> &w, int iters, double tol, double *dxl,
-                      int *count) {
+__device__ int newton(const LProg &P, Path<K> &w, int iters, double tol, double *dxl, int *count) {
   typedef C<K> Cx;
+  const int l = lane_id();
   for (int it = 0; it < iters; ++it) {
     ++*count;
     eval_point<K>(P, w, true);
-    if (!lu_factor<K>(P, w)) return -1;
-    for (int i = threadIdx.x; i < P.n; i += blockDim.x) w.rhs[i] = Cx::neg(w.rhs[i]);
-    __syncthreads();
-    lu_solve<K>(P, w);
-    const double dx = max_abs<K>(w.rhs, P.n);
-    for (int j = threadIdx.x; j < P.n; j += blockDim.x) {
-      cplx<K> &x = nodec(P, w, j, 0);
-      x = Cx::add(x, w.rhs[j]);
+    if (!gj_invert<K>(P, w)) return -1;
+    const cplx<K> dx = apply_inverse<K>(P, w);
+    double mdx = 0.0;
+    if (l < P.n * P.G && (l % P.G) == 0) {
+      cplx<K> &x = w.slots[(l / P.G) * P.D];
+      x = Cx::add(x, dx);
+      mdx = absz<K>(dx);
     }
-    __syncthreads();
-    *dxl = dx;
-    if (dx <= tol * (1.0 + max_abs_x<K>(P, w))) return 1;
+    wsync();
+    mdx = wave_max(mdx);
+    *dxl = mdx;
+    if (mdx <= tol * (1.0 + max_abs_x<K>(P, w))) return 1;
   }
   return 0;
 }
 
-template <int K>
-__device__ void carve(Work<K> &w, const DevProgram &P, double *lds) {
-  typedef cplx<K> Z;
-  Z *z = reinterpret_cast<Z *>(lds);
-  w.slots = z; z += P.nslots;
-  w.u = z; z += P.npair;
-  w.uj = z; z += P.njpair;
-  w.A = z; z += P.n * P.n;
-  w.part = z; z += P.nseg;
-  w.rhs = z; z += P.n;
-  w.x0 = z; z += P.n;
-  w.W = z; z += 3 * P.ncomp;
-  w.dnode = z; z += (P.nprod > 0 ? P.nprod : 1);
-  w.invd = z; z += P.n;
-  w.red = reinterpret_cast<double *>(z);
-  w.piv = reinterpret_cast<int *>(w.red + 8);
+__device__ void stage_program(const DevProgram &G, const ProgLayout &L, unsigned char *lds) {
+  auto cp32 = [&](size_t off, const void *src, size_t bytes) {
+    const unsigned *s = (const unsigned *)src;
+    unsigned *d = (unsigned *)(lds + off);
+    for (size_t i = threadIdx.x; i < bytes / 4; i += blockDim.x) d[i] = s[i];
+  };
+  cp32(L.nodes, G.nodes, sizeof(int) * 8 * G.nprod);
+  cp32(L.lev, G.lev_ptr, sizeof(int) * (G.nlev + 1));
+  cp32(L.pairs, G.pairs, sizeof(int) * 4 * G.npair);
+  cp32(L.seg, G.seg_ptr, sizeof(int) * (G.n * G.G + 1));
+  cp32(L.tpair, G.term_pair, sizeof(int) * G.nterm);
+  // coefficients interleaved (re, im): one 16-byte LDS load per term
+  double2 *tc = (double2 *)(lds + L.tcoef);
+  for (int t = threadIdx.x; t < G.nterm; t += blockDim.x)
+    tc[t] = make_double2(G.term_coef[t], G.term_coef[G.nterm + t]);
+  cp32(L.jpairs, G.jpairs, sizeof(int) * 2 * G.njpair);
+  cp32(L.jent, G.jent_ptr, sizeof(int) * (G.n * G.n + 1));
+  cp32(L.jterms, G.jterms, sizeof(int) * G.njterm);
 }
 
-template <int K> size_t lds_bytes(const DevProgram &P) {
-  size_t z = (size_t)P.nslots + P.npair + P.njpair + P.n * P.n + P.nseg + 3 * P.n +
-             3 * P.ncomp + (P.nprod > 0 ? P.nprod : 1);
-  return z * sizeof(cplx<K>) + 8 * sizeof(double) + 16 * sizeof(int) + P.n * sizeof(int);
+__device__ LProg lprog(const DevProgram &G, const ProgLayout &L, unsigned char *lds) {
+  LProg P;
+  P.n = G.n; P.D = G.D; P.ncomp = G.ncomp; P.wdeg = G.wdeg; P.G = G.G; P.nprod = G.nprod;
+  P.nlev = G.nlev; P.npair = G.npair; P.nterm = G.nterm; P.njpair = G.njpair;
+  P.nodes = (const int *)(lds + L.nodes);
+  P.lev = (const int *)(lds + L.lev);
+  P.pairs = (const int *)(lds + L.pairs);
+  P.seg = (const int *)(lds + L.seg);
+  P.tpair = (const int *)(lds + L.tpair);
+  P.tcoef = (const double2 *)(lds + L.tcoef);
+  P.jpairs = (const int *)(lds + L.jpairs);
+  P.jent = (const int *)(lds + L.jent);
+  P.jterms = (const int *)(lds + L.jterms);
+  P.weights = G.weights;
+  return P;
 }
 
-// ----------------------------------------------------------- tracking ------
+template <int K> __device__ Path<K> carve(const PathLayout<K> &L, unsigned char *base) {
+  Path<K> w;
+  w.slots = (cplx<K> *)(base + L.slots);
+  w.u = (cplx<K> *)(base + L.u);
+  w.uj = (cplx<K> *)(base + L.uj);
+  w.A = (cplx<K> *)(base + L.A);
+  w.rhs = (cplx<K> *)(base + L.rhs);
+  w.x0 = (cplx<K> *)(base + L.x0);
+  w.col = (cplx<K> *)(base + L.col);
+  w.W = (cplx<K> *)(base + L.W);
+  w.dnode = (cplx<K> *)(base + L.dnode);
+  w.perm = (int *)(base + L.perm);
+  return w;
+}
 
 template <int K>
-__global__ void __launch_bounds__(256) track_kernel(DevProgram P, Params par, int npaths,
+__device__ void load_point(const LProg &P, Path<K> &w, const double *x, int KS, int npaths, int path) {
+  const int l = lane_id(), n = P.n;
+  if (l < n) {
+    cplx<K> z = C<K>::zero();
+    double *re = reinterpret_cast<double *>(&z.re), *im = reinterpret_cast<double *>(&z.im);
+    for (int i = 0; i < K && i < KS; ++i) {
+      re[i] = x[(size_t)i * npaths * n + (size_t)path * n + l];
+      im[i] = x[(size_t)(KS + i) * npaths * n + (size_t)path * n + l];
+    }
+    w.slots[l * P.D] = z;
+  }
+  // the constant node 1 + 0 s + 0 s^2 + ...
+  for (int a = l; a < P.D; a += 64) w.slots[n * P.D + a] = (a == 0) ? C<K>::from(1.0, 0.0) : C<K>::zero();
+  wsync();
+}
+
+template <int K>
+__device__ void store_point(const LProg &P, Path<K> &w, double *x, int npaths, int path) {
+  const int l = lane_id(), n = P.n;
+  if (l < n) {
+    const cplx<K> z = w.slots[l * P.D];
+    const double *re = reinterpret_cast<const double *>(&z.re);
+    const double *im = reinterpret_cast<const double *>(&z.im);
+    for (int i = 0; i < K; ++i) {
+      x[(size_t)i * npaths * n + (size_t)path * n + l] = re[i];
+      x[(size_t)(K + i) * npaths * n + (size_t)path * n + l] = im[i];
+    }
+  }
+}
+
+// Taylor coefficients x_1..x_deg at the point in the order-0 slots (weights
+// set at t0); returns false on a singular Jacobian
+template <int K> __device__ bool predictor_series(const LProg &P, Path<K> &w) {
+  typedef C<K> Cx;
+  const int l = lane_id(), n = P.n, D = P.D;
+  eval_point<K>(P, w, true);
+  if (!gj_invert<K>(P, w)) return false;
+  for (int k = 1; k < D; ++k) {
+    if (l < n) w.slots[l * D + k] = Cx::zero();
+    wsync();
+    product_pass<K>(P, w, k);
+    residual<K>(P, w, k);
+    const cplx<K> xk = apply_inverse<K>(P, w);
+    if (l < n * P.G && (l % P.G) == 0) w.slots[(l / P.G) * D + k] = xk;
+    wsync();
+    correction_pass<K>(P, w, k);
+  }
+  return true;
+}
+
+template <int K>
+__global__ void __launch_bounds__(512) track_kernel(DevProgram G, Params par, int npaths,
                                                    const double *__restrict__ start, int KS,
                                                    double *__restrict__ endp, double *tend,
                                                    int *status, int *nsteps_out, int *nrej_out,
                                                    double *dx_out, int *work, int *counter) {
   typedef C<K> Cx;
-  extern __shared__ double lds[];
-  __shared__ int s_path;
-  Work<K> w;
-  carve<K>(w, P, lds);
-  const int n = P.n, D = P.D;
+  extern __shared__ __align__(16) unsigned char lds[];
+  const ProgLayout PL(G);
+  const PathLayout<K> WL(G);
+  stage_program(G, PL, lds);
+  __syncthreads();
+  const LProg P = lprog(G, PL, lds);
+  Path<K> w = carve<K>(WL, lds + PL.total + (threadIdx.x >> 6) * WL.total);
+  const int l = lane_id(), n = P.n, D = P.D;
   for (;;) {
-    if (threadIdx.x == 0) s_path = atomicAdd(counter, 1);
-    __syncthreads();
-    const int path = s_path;
-    __syncthreads();
+    int path = 0;
+    if (l == 0) path = atomicAdd(counter, 1);
+    path = __shfl(path, 0, 64);
     if (path >= npaths) break;
-    // start point: KS limbs -> K limbs (extra limbs zero)
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-      cplx<K> z = Cx::zero();
-      for (int i = 0; i < K && i < KS; ++i) {
-        reinterpret_cast<double *>(&z.re)[i] = start[(size_t)i * npaths * n + (size_t)path * n + j];
-        reinterpret_cast<double *>(&z.im)[i] = start[(size_t)(KS + i) * npaths * n + (size_t)path * n + j];
-      }
-      nodec(P, w, j, 0) = z;
-    }
-    // the constant node ONE = 1 + 0 s + ...
-    for (int a = threadIdx.x; a < D; a += blockDim.x)
-      w.slots[P.off[n] + a] = (a == 0) ? Cx::from(1.0, 0.0) : Cx::zero();
-    __syncthreads();
+    load_point<K>(P, w, start, KS, npaths, path);
     double t = 0.0, h = par.maxstep, dxl = 0.0;
     int st = ST_RUNNING, steps = 0, rej = 0, npred = 0, nnewt = 0, nfeval = 0;
     while (st == ST_RUNNING && t < 1.0) {
       // ---- predictor: Taylor series of x(t+s) ----
       set_weights<K>(P, w, t);
-      __syncthreads();
       ++npred;
-      eval_point<K>(P, w, true);
-      if (!lu_factor<K>(P, w)) { st = ST_SINGULAR; break; }
-      for (int k = 1; k < D; ++k) {
-        for (int j = threadIdx.x; j < n; j += blockDim.x) nodec(P, w, j, k) = Cx::zero();
-        __syncthreads();
-        product_pass<K>(P, w, k);
-        residual<K>(P, w, k);
-        for (int i = threadIdx.x; i < n; i += blockDim.x) w.rhs[i] = Cx::neg(w.rhs[i]);
-        __syncthreads();
-        lu_solve<K>(P, w);
-        for (int j = threadIdx.x; j < n; j += blockDim.x) nodec(P, w, j, k) = w.rhs[j];
-        __syncthreads();
-        correction_pass<K>(P, w, k);
-      }
-      // ---- pole step from the Fabry ratio (offset par.fabry_offset) ----
+      if (!predictor_series<K>(P, w)) { st = ST_SINGULAR; break; }
+      // ---- pole step from the Fabry ratio ----
       const int kd = D - 1 - par.fabry_offset;
-      double rad = -1.0;
-      if (kd >= 1)
-        for (int j = 0; j < n; ++j) {
-          const double den = absz<K>(nodec(P, w, j, kd));
-          if (den > 0.0) {
-            const double r = absz<K>(nodec(P, w, j, kd - 1)) / den;
-            if (rad < 0.0 || r < rad) rad = r;
-          }
-        }
-      h = (rad > 0.0) ? fmin(par.pbeta * rad, par.maxstep) : par.maxstep;
-      for (int j = threadIdx.x; j < n; j += blockDim.x) w.x0[j] = nodec(P, w, j, 0);
-      __syncthreads();
+      double r = INFINITY;
+      if (kd >= 1 && l < n) {
+        const double den = absz<K>(w.slots[l * D + kd]);
+        if (den > 0.0) r = absz<K>(w.slots[l * D + kd - 1]) / den;
+      }
+      r = wave_min(r);
+      h = (r < INFINITY && r > 0.0) ? fmin(par.pbeta * r, par.maxstep) : par.maxstep;
+      if (l < n) w.x0[l] = w.slots[l * D];
+      wsync();
       // ---- feedback and corrector; halve the step on failure ----
       bool accepted = false;
       while (!accepted) {
         double tn = t + h;
         if (tn >= 1.0 - par.minstep) tn = 1.0;
         const double hs = tn - t;
-        for (int j = threadIdx.x; j < n; j += blockDim.x) {
-          cplx<K> acc = nodec(P, w, j, D - 1);
-          for (int k = D - 2; k >= 1; --k) acc = Cx::add(Cx::scale_d(acc, hs), nodec(P, w, j, k));
-          nodec(P, w, j, 0) = Cx::add(w.x0[j], Cx::scale_d(acc, hs));
+        if (l < n) {
+          cplx<K> acc = w.slots[l * D + D - 1];
+          for (int k = D - 2; k >= 1; --k) acc = Cx::add(Cx::scale_d(acc, hs), w.slots[l * D + k]);
+          w.slots[l * D] = Cx::add(w.x0[l], Cx::scale_d(acc, hs));
         }
+        wsync();
         set_weights<K>(P, w, tn);
-        __syncthreads();
         ++nfeval;
         eval_point<K>(P, w, false);
-        const double res = max_abs<K>(w.rhs, n);
+        const double res = max_abs_rhs<K>(P, w);
         bool ok = res <= par.alpha * (1.0 + max_abs_x<K>(P, w));
         if (ok) {
           const int nr = newton<K>(P, w, par.corsteps, par.tolcor, &dxl, &nnewt);
@@ -410,20 +540,12 @@ __global__ void __launch_bounds__(256) track_kernel(DevProgram P, Params par, in
       if (steps >= par.maxsteps && t < 1.0) { st = ST_MAXSTEPS; break; }
     }
     if (st == ST_RUNNING) {
-      // endpoint: Newton at t = 1
       set_weights<K>(P, w, 1.0);
-      __syncthreads();
       const int nr = newton<K>(P, w, par.endsteps, par.tolend, &dxl, &nnewt);
       st = (nr < 0) ? ST_SINGULAR : ST_SUCCESS;
     }
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-      const cplx<K> x = nodec(P, w, j, 0);
-      for (int i = 0; i < K; ++i) {
-        endp[(size_t)i * npaths * n + (size_t)path * n + j] = reinterpret_cast<const double *>(&x.re)[i];
-        endp[(size_t)(K + i) * npaths * n + (size_t)path * n + j] = reinterpret_cast<const double *>(&x.im)[i];
-      }
-    }
-    if (threadIdx.x == 0) {
+    store_point<K>(P, w, endp, npaths, path);
+    if (l == 0) {
       tend[path] = t;
       status[path] = st;
       nsteps_out[path] = steps;
@@ -435,63 +557,46 @@ __global__ void __launch_bounds__(256) track_kernel(DevProgram P, Params par, in
         work[2 * npaths + path] = nfeval;
       }
     }
-    __syncthreads();
+    wsync();
   }
 }
 
-// ------------------------------------------- endpoint refinement / residual --
-
 // Newton at t = tval in K-limb precision from KS-limb points; writes the
-// refined point, the last |dx| and the max residual |H(x,t)| after the last
-// update (evaluated in the same precision).
+// refined point, the last |dx|, the max residual |H(x,t)| after the last
+// update (same precision) and the status.
 template <int K>
-__global__ void __launch_bounds__(256) refine_kernel(DevProgram P, int npaths, double tval,
+__global__ void __launch_bounds__(512) refine_kernel(DevProgram G, int npaths, double tval,
                                                     const double *__restrict__ xin, int KS,
                                                     double *__restrict__ xout, int iters,
                                                     double tol, double *dx_out, double *res_out,
                                                     int *status, int *counter) {
-  typedef C<K> Cx;
-  extern __shared__ double lds[];
-  __shared__ int s_path;
-  Work<K> w;
-  carve<K>(w, P, lds);
-  const int n = P.n;
+  extern __shared__ __align__(16) unsigned char lds[];
+  const ProgLayout PL(G);
+  const PathLayout<K> WL(G);
+  stage_program(G, PL, lds);
+  __syncthreads();
+  const LProg P = lprog(G, PL, lds);
+  Path<K> w = carve<K>(WL, lds + PL.total + (threadIdx.x >> 6) * WL.total);
+  const int l = lane_id();
   for (;;) {
-    if (threadIdx.x == 0) s_path = atomicAdd(counter, 1);
-    __syncthreads();
-    const int path = s_path;
-    __syncthreads();
+    int path = 0;
+    if (l == 0) path = atomicAdd(counter, 1);
+    path = __shfl(path, 0, 64);
     if (path >= npaths) break;
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-      cplx<K> z = Cx::zero();
-      for (int i = 0; i < K && i < KS; ++i) {
-        reinterpret_cast<double *>(&z.re)[i] = xin[(size_t)i * npaths * n + (size_t)path * n + j];
-        reinterpret_cast<double *>(&z.im)[i] = xin[(size_t)(KS + i) * npaths * n + (size_t)path * n + j];
-      }
-      nodec(P, w, j, 0) = z;
-    }
-    if (threadIdx.x == 0) w.slots[P.off[n]] = Cx::from(1.0, 0.0);
+    load_point<K>(P, w, xin, KS, npaths, path);
     set_weights<K>(P, w, tval);
-    __syncthreads();
     double dxl = 0.0;
-    int nr = 1;
-    int its = 0;
+    int nr = 1, its = 0;
     if (iters > 0) nr = newton<K>(P, w, iters, tol, &dxl, &its);
     eval_point<K>(P, w, false);
-    const double res = max_abs<K>(w.rhs, n);
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-      const cplx<K> x = nodec(P, w, j, 0);
-      for (int i = 0; i < K; ++i) {
-        xout[(size_t)i * npaths * n + (size_t)path * n + j] = reinterpret_cast<const double *>(&x.re)[i];
-        xout[(size_t)(K + i) * npaths * n + (size_t)path * n + j] = reinterpret_cast<const double *>(&x.im)[i];
-      }
-    }
-    if (threadIdx.x == 0) {
+    const double res = max_abs_rhs<K>(P, w);
+    store_point<K>(P, w, xout, npaths, path);
+    if (l == 0) {
       dx_out[path] = dxl;
       res_out[path] = res;
       status[path] = nr;
     }
-    __syncthreads();
+    wsync();
   }
 }
 
@@ -509,30 +614,39 @@ namespace {
 
 template <typename T> int upload(phc_tracker *tr, const T *src, size_t count, const T **dst) {
   void *p = nullptr;
-  size_t bytes = sizeof(T) * (count > 0 ? count : 1);
-  PHC_HIP(hipMalloc(&p, bytes));
+  PHC_HIP(hipMalloc(&p, sizeof(T) * (count > 0 ? count : 1)));
   tr->bufs.push_back(p);
   if (count > 0) PHC_HIP(hipMemcpy(p, src, sizeof(T) * count, hipMemcpyHostToDevice));
   *dst = (const T *)p;
   return 0;
 }
 
-size_t lds_for(int K, const DevProgram &P) {
-  if (K == 1) return lds_bytes<1>(P);
-  if (K == 2) return lds_bytes<2>(P);
-  if (K == 4) return lds_bytes<4>(P);
-  return lds_bytes<8>(P);
+size_t path_bytes(int K, const DevProgram &P) {
+  if (K == 1) return PathLayout<1>(P).total;
+  if (K == 2) return PathLayout<2>(P).total;
+  if (K == 4) return PathLayout<4>(P).total;
+  return PathLayout<8>(P).total;
 }
 
-int occupancy_grid(const void *fn, int block, size_t lds, int npaths) {
-  int dev = 0, ncu = 0, per = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return -1;
-  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, block, lds) != hipSuccess) return -1;
-  if (per < 1) per = 1;
-  long g = (long)ncu * per;
-  if (g > npaths) g = npaths;
-  return (int)(g > 0 ? g : 1);
+// launch geometry: one workgroup per CU with as many wavefronts as LDS holds
+int geometry(const DevProgram &P, int K, int max_waves, int npaths, int *grid, int *waves,
+             size_t *lds) {
+  int dev = 0, ncu = 0, ldsmax = 0;
+  PHC_HIP(hipGetDevice(&dev));
+  PHC_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  PHC_HIP(hipDeviceGetAttribute(&ldsmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  const size_t cap = (size_t)ldsmax < LDS_LIMIT ? (size_t)ldsmax : LDS_LIMIT;
+  const size_t prog = ProgLayout(P).total, pb = path_bytes(K, P);
+  PHC_REQUIRE(prog + pb <= cap, "tracker program and one path do not fit in LDS");
+  int wv = (int)((cap - prog) / pb);
+  if (wv > max_waves) wv = max_waves;
+  if (wv > 8) wv = 8;
+  int g = ncu;
+  if ((long)ncu * wv > npaths) g = (npaths + wv - 1) / wv;
+  *grid = g < 1 ? 1 : g;
+  *waves = wv;
+  *lds = prog + (size_t)wv * pb;
+  return 0;
 }
 
 }  // namespace
@@ -544,56 +658,76 @@ int phc_program_validate(const phc_program *d) {
   PHC_REQUIRE(d->n >= 1 && d->n <= 64, "phc_program: n must be in 1..64");
   PHC_REQUIRE(d->deg >= 0 && d->deg <= 32, "phc_program: deg must be in 0..32");
   PHC_REQUIRE(d->wdeg >= 0 && d->wdeg <= 2, "phc_program: weight degree must be <= 2");
-  PHC_REQUIRE(d->ncomp >= 1 && d->nprod >= 0 && d->nlev >= 0 && d->npair >= 0 && d->njpair >= 0,
+  PHC_REQUIRE(d->G >= 1 && (d->G & (d->G - 1)) == 0 && d->n * d->G <= 64,
+              "phc_program: G must be a power of two with n*G <= 64");
+  PHC_REQUIRE(d->ncomp >= 1 && d->ncomp <= 64, "phc_program: ncomp must be in 1..64");
+  PHC_REQUIRE(d->nprod >= 0 && d->nlev >= 0 && d->npair >= 0 && d->njpair >= 0 &&
+              d->nterm >= 0 && d->njterm >= 0 && d->nfactor >= 0 && d->nslots >= 0,
               "phc_program: negative count");
-  PHC_REQUIRE(d->nseg >= d->n && d->nterm >= 0 && d->njterm >= 0, "phc_program: bad counts");
-  const int nn = d->n + 1 + d->nprod;
-  const int need = d->wdeg < d->deg ? d->wdeg : d->deg;  // orders k-l kept besides k
-  for (int i = 0; i < d->nprod; ++i)
-    PHC_REQUIRE(d->left[i] >= 0 && d->left[i] < d->n + 1 + i && d->left[i] != d->n &&
-                d->right[i] >= 0 && d->right[i] < d->n,
-                "phc_program: product node factors must be earlier nodes, right a variable");
-  for (int i = 0; i < nn; ++i)
-    PHC_REQUIRE(d->off[i] >= 0 && d->slen[i] >= 1 && d->off[i] + d->slen[i] <= d->nslots,
-                "phc_program: node storage out of range");
-  for (int i = 0; i <= d->n; ++i)
-    PHC_REQUIRE(d->slen[i] == d->deg + 1, "phc_program: variables need full series storage");
-  for (int i = 0; i < d->nprod; ++i)
-    PHC_REQUIRE(d->slen[d->left[i]] == d->deg + 1, "phc_program: factor nodes need full series storage");
-  for (int i = 0; i < d->npair; ++i)
-    PHC_REQUIRE(d->pair_comp[i] >= 0 && d->pair_comp[i] < d->ncomp && d->pair_node[i] >= 0 &&
-                d->pair_node[i] < nn && d->slen[d->pair_node[i]] > need,
+  PHC_REQUIRE(d->njpair <= (1 << JT_PAIR_BITS) && d->nterm <= (1 << JT_TERM_BITS),
+              "phc_program: too many terms for the packed Jacobian");
+  const int D = d->deg + 1;
+  const int need = d->wdeg < d->deg ? d->wdeg : d->deg;
+  auto storage_ok = [&](int off, int mask) {
+    if (off < 0) return false;
+    if (mask == -1) return off + D <= d->nslots;
+    return mask >= need && ((mask + 1) & mask) == 0 && off + mask + 1 <= d->nslots;
+  };
+  PHC_REQUIRE((d->n + 1) * D <= d->nslots, "phc_program: no room for the variables and the constant");
+  PHC_REQUIRE(d->lev_ptr[0] == 0 && d->lev_ptr[d->nlev] == d->nprod, "phc_program: bad levels");
+  for (int lv = 1; lv <= d->nlev; ++lv)
+    PHC_REQUIRE(d->lev_ptr[lv] >= d->lev_ptr[lv - 1], "phc_program: bad levels");
+  for (int i = 0; i < d->nprod; ++i) {
+    const int *nd = d->nodes + 8 * i;
+    PHC_REQUIRE(storage_ok(nd[0], -1) && nd[1] == -1, "phc_program: left factor needs full storage");
+    PHC_REQUIRE(nd[2] >= 0 && nd[2] % D == 0 && nd[2] / D < d->n,
+                "phc_program: right factor must be a variable");
+    PHC_REQUIRE(storage_ok(nd[3], nd[4]), "phc_program: node storage out of range");
+    PHC_REQUIRE(nd[5] >= -1 && nd[5] < d->nfactor && nd[6] >= -1 && nd[6] < d->nfactor,
+                "phc_program: bad correction index");
+    PHC_REQUIRE(nd[5] >= 0 || (nd[0] % D == 0 && nd[0] / D < d->n),
+                "phc_program: a left factor without correction index must be a variable");
+    PHC_REQUIRE(nd[6] < 0 || nd[4] == -1, "phc_program: factor nodes need full storage");
+  }
+  // a product's left factor is a variable or a node of the previous level
+  for (int lv = 1; lv <= d->nlev; ++lv)
+    for (int i = d->lev_ptr[lv - 1]; i < d->lev_ptr[lv]; ++i) {
+      const int adn = d->nodes[8 * i + 5];
+      if (adn < 0) {
+        PHC_REQUIRE(lv == 1, "phc_program: a product of two variables must be on level 1");
+        continue;
+      }
+      bool found = false;
+      for (int q = (lv >= 2 ? d->lev_ptr[lv - 2] : 0); q < d->lev_ptr[lv - 1]; ++q)
+        if (d->nodes[8 * q + 6] == adn && d->nodes[8 * q + 3] == d->nodes[8 * i]) found = true;
+      PHC_REQUIRE(found, "phc_program: left factor not on the previous level");
+    }
+  for (int q = 0; q < d->npair; ++q) {
+    const int *pr = d->pairs + 4 * q;
+    PHC_REQUIRE(pr[0] >= 0 && pr[0] < d->ncomp && storage_ok(pr[1], pr[2]),
                 "phc_program: bad component pair");
-  for (int i = 0; i < d->njpair; ++i)
-    PHC_REQUIRE(d->jpair_comp[i] >= 0 && d->jpair_comp[i] < d->ncomp && d->jpair_node[i] >= 0 &&
-                d->jpair_node[i] < nn, "phc_program: bad Jacobian pair");
-  PHC_REQUIRE(d->seg_ptr[0] == 0 && d->seg_ptr[d->nseg] == d->nterm && d->eq_seg[0] == 0 &&
-              d->eq_seg[d->n] == d->nseg, "phc_program: bad residual segments");
-  for (int s = 0; s < d->nseg; ++s)
+  }
+  for (int q = 0; q < d->njpair; ++q) {
+    const int *jp = d->jpairs + 2 * q;
+    PHC_REQUIRE(jp[0] >= 0 && jp[0] < d->ncomp && jp[1] >= 0 && jp[1] < d->nslots,
+                "phc_program: bad Jacobian pair");
+  }
+  const int nseg = d->n * d->G;
+  PHC_REQUIRE(d->seg_ptr[0] == 0 && d->seg_ptr[nseg] == d->nterm, "phc_program: bad residual segments");
+  for (int s = 0; s < nseg; ++s)
     PHC_REQUIRE(d->seg_ptr[s + 1] >= d->seg_ptr[s], "phc_program: segments not increasing");
-  for (int i = 0; i < d->n; ++i)
-    PHC_REQUIRE(d->eq_seg[i + 1] > d->eq_seg[i], "phc_program: equation without segments");
-  for (int i = 0; i < d->nterm; ++i)
-    PHC_REQUIRE(d->term_pair[i] >= 0 && d->term_pair[i] < d->npair, "phc_program: bad term");
+  for (int t = 0; t < d->nterm; ++t)
+    PHC_REQUIRE(d->term_pair[t] >= 0 && d->term_pair[t] < d->npair, "phc_program: bad term");
   PHC_REQUIRE(d->jent_ptr[0] == 0 && d->jent_ptr[d->n * d->n] == d->njterm,
               "phc_program: bad Jacobian entries");
   for (int e = 0; e < d->n * d->n; ++e)
     PHC_REQUIRE(d->jent_ptr[e + 1] >= d->jent_ptr[e], "phc_program: Jacobian entries not increasing");
-  for (int i = 0; i < d->njterm; ++i)
-    PHC_REQUIRE(d->jterm_pair[i] >= 0 && d->jterm_pair[i] < d->njpair,
-                "phc_program: bad Jacobian term");
-  PHC_REQUIRE(d->lev_ptr[0] == 0 && d->lev_ptr[d->nlev] == d->nprod, "phc_program: bad levels");
-  for (int l = 1; l <= d->nlev; ++l) {
-    PHC_REQUIRE(d->lev_ptr[l] >= d->lev_ptr[l - 1], "phc_program: bad levels");
-    // a product's left factor lies on the previous level (variables on level 0)
-    for (int i = d->lev_ptr[l - 1]; i < d->lev_ptr[l]; ++i) {
-      const int a = d->left[i];
-      const int la = (a < d->n) ? 0 : -1;
-      if (la == 0) PHC_REQUIRE(l == 1, "phc_program: level of a product of two variables must be 1");
-      else
-        PHC_REQUIRE(a - d->n - 1 >= (l >= 2 ? d->lev_ptr[l - 2] : 0) && a - d->n - 1 < d->lev_ptr[l - 1],
-                    "phc_program: left factor not on the previous level");
-    }
+  for (int t = 0; t < d->njterm; ++t) {
+    const unsigned jt = (unsigned)d->jterms[t];
+    const int term = jt & ((1u << JT_TERM_BITS) - 1);
+    const int q = (jt >> JT_TERM_BITS) & ((1u << JT_PAIR_BITS) - 1);
+    const int ex = jt >> (JT_TERM_BITS + JT_PAIR_BITS);
+    PHC_REQUIRE(term < d->nterm && q < d->njpair && ex >= 1, "phc_program: bad Jacobian term");
   }
   return 0;
 }
@@ -602,29 +736,21 @@ int phc_tracker_create(phc_tracker **out, const phc_program *d) {
   PHC_REQUIRE(out && d, "phc_tracker_create: null argument");
   *out = nullptr;
   if (phc_program_validate(d) != 0) return -2;
-  const int nn = d->n + 1 + d->nprod;
   phc_tracker *tr = new phc_tracker();
   DevProgram &P = tr->P;
-  P.n = d->n; P.D = d->deg + 1; P.nprod = d->nprod; P.nlev = d->nlev; P.nslots = d->nslots;
-  P.ring = 0; P.wdeg = d->wdeg; P.ncomp = d->ncomp; P.npair = d->npair; P.nseg = d->nseg;
-  P.nterm = d->nterm; P.njpair = d->njpair; P.njterm = d->njterm;
+  P.n = d->n; P.D = d->deg + 1; P.ncomp = d->ncomp; P.wdeg = d->wdeg; P.G = d->G;
+  P.nprod = d->nprod; P.nlev = d->nlev; P.nslots = d->nslots; P.nfactor = d->nfactor;
+  P.npair = d->npair; P.nterm = d->nterm; P.njpair = d->njpair; P.njterm = d->njterm;
   int rc = 0;
-  rc |= upload<int>(tr, d->left, d->nprod, &P.left);
-  rc |= upload<int>(tr, d->right, d->nprod, &P.right);
+  rc |= upload<int>(tr, d->nodes, (size_t)8 * d->nprod, &P.nodes);
   rc |= upload<int>(tr, d->lev_ptr, d->nlev + 1, &P.lev_ptr);
-  rc |= upload<int>(tr, d->off, nn, &P.off);
-  rc |= upload<int>(tr, d->slen, nn, &P.slen);
-  rc |= upload<int>(tr, d->pair_comp, d->npair, &P.pair_comp);
-  rc |= upload<int>(tr, d->pair_node, d->npair, &P.pair_node);
-  rc |= upload<int>(tr, d->seg_ptr, d->nseg + 1, &P.seg_ptr);
-  rc |= upload<int>(tr, d->eq_seg, d->n + 1, &P.eq_seg);
+  rc |= upload<int>(tr, d->pairs, (size_t)4 * d->npair, &P.pairs);
+  rc |= upload<int>(tr, d->seg_ptr, (size_t)d->n * d->G + 1, &P.seg_ptr);
   rc |= upload<int>(tr, d->term_pair, d->nterm, &P.term_pair);
-  rc |= upload<double>(tr, d->term_coef, (size_t)16 * d->nterm, &P.term_coef);
-  rc |= upload<int>(tr, d->jpair_comp, d->njpair, &P.jpair_comp);
-  rc |= upload<int>(tr, d->jpair_node, d->njpair, &P.jpair_node);
+  rc |= upload<double>(tr, d->term_coef, (size_t)2 * d->nterm, &P.term_coef);
+  rc |= upload<int>(tr, d->jpairs, (size_t)2 * d->njpair, &P.jpairs);
   rc |= upload<int>(tr, d->jent_ptr, (size_t)d->n * d->n + 1, &P.jent_ptr);
-  rc |= upload<int>(tr, d->jterm_pair, d->njterm, &P.jterm_pair);
-  rc |= upload<double>(tr, d->jterm_coef, (size_t)16 * d->njterm, &P.jterm_coef);
+  rc |= upload<int>(tr, d->jterms, d->njterm, &P.jterms);
   rc |= upload<double>(tr, d->weights, (size_t)16 * 3 * d->ncomp, &P.weights);
   void *c = nullptr;
   if (rc != 0 || hipMalloc(&c, sizeof(int)) != hipSuccess) {
@@ -645,30 +771,32 @@ int phc_tracker_destroy(phc_tracker *tr) {
   return 0;
 }
 
-int phc_tracker_lds_bytes(const phc_tracker *tr, int K, size_t *bytes) {
-  PHC_REQUIRE(tr && bytes, "phc_tracker_lds_bytes: null argument");
+int phc_tracker_lds_bytes(const phc_tracker *tr, int K, size_t *program_bytes, size_t *path_bytes_out) {
+  PHC_REQUIRE(tr && program_bytes && path_bytes_out, "phc_tracker_lds_bytes: null argument");
   PHC_REQUIRE(K == 1 || K == 2 || K == 4 || K == 8, "phc_tracker_lds_bytes: K must be 1, 2, 4, 8");
-  *bytes = lds_for(K, tr->P);
+  *program_bytes = ProgLayout(tr->P).total;
+  *path_bytes_out = path_bytes(K, tr->P);
   return 0;
 }
 
 int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, double *endp,
               double *tend, int *status, int *nsteps, int *nrej, double *dxlast, int *work,
-              const phc_track_params *prm, int block, void *stream) {
+              const phc_track_params *prm, int waves, void *stream) {
   PHC_REQUIRE(tr && prm, "phc_track: null argument");
   PHC_REQUIRE(K == 1 || K == 2, "phc_track: tracking precision K must be 1 or 2");
   PHC_REQUIRE(KS >= 1 && KS <= 8, "phc_track: start limbs KS must be in 1..8");
   PHC_REQUIRE(npaths >= 0, "phc_track: npaths < 0");
-  PHC_REQUIRE(block == 64 || block == 128 || block == 256, "phc_track: block must be 64, 128 or 256");
+  PHC_REQUIRE(waves >= 1 && waves <= 8, "phc_track: waves per workgroup must be in 1..8");
   PHC_REQUIRE(tr->P.D >= 2, "phc_track: the tracker program needs series degree >= 1");
   PHC_REQUIRE(prm->fabry_offset >= 0 && prm->fabry_offset < tr->P.D - 1,
               "phc_track: fabry_offset must be below the series degree");
-  PHC_REQUIRE(prm->maxstep > 0 && prm->minstep > 0 && prm->maxsteps > 0 && prm->corsteps > 0,
-              "phc_track: step parameters must be positive");
+  PHC_REQUIRE(prm->maxstep > 0 && prm->minstep > 0 && prm->maxsteps > 0 && prm->corsteps > 0 &&
+              prm->endsteps >= 0, "phc_track: step parameters must be positive");
   if (npaths == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
-  const size_t lds = lds_for(K, tr->P);
-  PHC_REQUIRE(lds <= 160 * 1024, "phc_track: program too large for LDS");
+  int grid = 0, wv = 0;
+  size_t lds = 0;
+  if (geometry(tr->P, K, waves, npaths, &grid, &wv, &lds) != 0) return -2;
   Params par;
   par.maxsteps = prm->maxsteps; par.corsteps = prm->corsteps; par.endsteps = prm->endsteps;
   par.fabry_offset = prm->fabry_offset; par.maxstep = prm->maxstep; par.minstep = prm->minstep;
@@ -676,38 +804,35 @@ int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, d
   par.tolend = prm->tolend; par.divergence = prm->divergence;
   PHC_HIP(hipMemsetAsync(tr->counter, 0, sizeof(int), st));
   const void *fn = (K == 1) ? (const void *)track_kernel<1> : (const void *)track_kernel<2>;
-  if (lds > 64 * 1024) PHC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const int grid = occupancy_grid(fn, block, lds, npaths);
-  PHC_REQUIRE(grid > 0, "phc_track: occupancy query failed");
+  PHC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   if (K == 1)
-    hipLaunchKernelGGL(track_kernel<1>, dim3(grid), dim3(block), lds, st, tr->P, par, npaths, start,
-                       KS, endp, tend, status, nsteps, nrej, dxlast, work, tr->counter);
+    hipLaunchKernelGGL(track_kernel<1>, dim3(grid), dim3(64 * wv), lds, st, tr->P, par, npaths,
+                       start, KS, endp, tend, status, nsteps, nrej, dxlast, work, tr->counter);
   else
-    hipLaunchKernelGGL(track_kernel<2>, dim3(grid), dim3(block), lds, st, tr->P, par, npaths, start,
-                       KS, endp, tend, status, nsteps, nrej, dxlast, work, tr->counter);
+    hipLaunchKernelGGL(track_kernel<2>, dim3(grid), dim3(64 * wv), lds, st, tr->P, par, npaths,
+                       start, KS, endp, tend, status, nsteps, nrej, dxlast, work, tr->counter);
   PHC_HIP(hipGetLastError());
   return 0;
 }
 
 int phc_refine(phc_tracker *tr, int K, int npaths, double t, const double *xin, int KS,
                double *xout, int iters, double tol, double *dxlast, double *res, int *status,
-               int block, void *stream) {
+               int waves, void *stream) {
   PHC_REQUIRE(tr, "phc_refine: null tracker");
   PHC_REQUIRE(K == 1 || K == 2 || K == 4 || K == 8, "phc_refine: K must be 1, 2, 4 or 8");
   PHC_REQUIRE(KS >= 1 && KS <= 8, "phc_refine: KS must be in 1..8");
   PHC_REQUIRE(npaths >= 0 && iters >= 0, "phc_refine: negative size");
-  PHC_REQUIRE(block == 64 || block == 128 || block == 256, "phc_refine: block must be 64, 128 or 256");
+  PHC_REQUIRE(waves >= 1 && waves <= 8, "phc_refine: waves per workgroup must be in 1..8");
   if (npaths == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
-  const size_t lds = lds_for(K, tr->P);
-  PHC_REQUIRE(lds <= 160 * 1024, "phc_refine: program too large for LDS");
+  int grid = 0, wv = 0;
+  size_t lds = 0;
+  if (geometry(tr->P, K, waves, npaths, &grid, &wv, &lds) != 0) return -2;
   PHC_HIP(hipMemsetAsync(tr->counter, 0, sizeof(int), st));
   const void *fn = K == 1 ? (const void *)refine_kernel<1> : K == 2 ? (const void *)refine_kernel<2>
                  : K == 4 ? (const void *)refine_kernel<4> : (const void *)refine_kernel<8>;
-  if (lds > 64 * 1024) PHC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const int grid = occupancy_grid(fn, block, lds, npaths);
-  PHC_REQUIRE(grid > 0, "phc_refine: occupancy query failed");
-#define RK(KK) hipLaunchKernelGGL(refine_kernel<KK>, dim3(grid), dim3(block), lds, st, tr->P, npaths, t, xin, KS, xout, iters, tol, dxlast, res, status, tr->counter)
+  PHC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+#define RK(KK) hipLaunchKernelGGL(refine_kernel<KK>, dim3(grid), dim3(64 * wv), lds, st, tr->P, npaths, t, xin, KS, xout, iters, tol, dxlast, res, status, tr->counter)
   if (K == 1) RK(1); else if (K == 2) RK(2); else if (K == 4) RK(4); else RK(8);
 #undef RK
   PHC_HIP(hipGetLastError());

@@ -138,33 +138,54 @@ def total_degree_homotopy(target, rng, k=1, gamma=None):
 
 # ---------------------------------------------------------------- program --
 
+JT_TERM_BITS, JT_PAIR_BITS, JT_EXP_BITS = 16, 12, 4
+
+
 @dataclass
 class Program:
+    """Flat tracker program (layout of phc_program in include/phcmd.h).
+
+    nodes [8*nprod]: per product node (ordered by DAG level)
+        aoff, amask, boff, soff, smask, adn, sdn, 0
+      aoff/amask: storage of the left factor (a variable or a product node);
+      boff: series of the right factor (a variable); soff/smask: own storage
+      (coefficient a at soff + (a & smask)); adn/sdn: index of the left
+      factor's / own order-k correction in the dnode buffer (-1: none).
+    pairs [4*npair]: component r, node storage off, mask, 0
+    seg_ptr [n*G+1]: terms of segment s = i*G + g (G segments per equation)
+    term_pair [nterm], term_coef [2][nterm] (real, imaginary parts)
+    jpairs [2*njpair]: component r, node storage off (order 0)
+    jent_ptr [n*n+1]: Jacobian terms of entry i*n+j
+    jterms [njterm]: term index | jpair << 16 | exponent << 28
+    weights [2][8][3*ncomp]: coefficients of 1, t, t^2 of every w_r
+    """
     n: int
     deg: int
-    left: np.ndarray
-    right: np.ndarray
-    lev_ptr: np.ndarray
-    off: np.ndarray        # storage offset (complex units) of every node
-    slen: np.ndarray       # storage length: D for series-kept nodes, 2 for ring nodes
-    nslots: int
-    pair_comp: np.ndarray
-    pair_node: np.ndarray
-    seg_ptr: np.ndarray    # terms of segment s: term range seg_ptr[s]..seg_ptr[s+1]
-    eq_seg: np.ndarray     # segments of equation i: eq_seg[i]..eq_seg[i+1]
-    term_pair: np.ndarray
-    term_coef: np.ndarray  # [2][8][nterm] complex coefficient, octo-double planes
-    jpair_comp: np.ndarray
-    jpair_node: np.ndarray
-    jent_ptr: np.ndarray   # terms of Jacobian entry i*n+j
-    jterm_pair: np.ndarray
-    jterm_coef: np.ndarray  # [2][8][njterm]
-    weights: np.ndarray    # [2][8][ncomp*3] weight polynomial coefficients
     ncomp: int
-    wdeg: int = 1
-    # for checks and for the oracle's independent evaluation
-    monomials: list = None
+    wdeg: int
+    G: int
+    nodes: np.ndarray
+    lev_ptr: np.ndarray
+    nslots: int
+    nfactor: int
+    pairs: np.ndarray
+    seg_ptr: np.ndarray
+    term_pair: np.ndarray
+    term_coef: np.ndarray
+    jpairs: np.ndarray
+    jent_ptr: np.ndarray
+    jterms: np.ndarray
+    weights: np.ndarray
     node_exps: list = None
+    var_off: np.ndarray = None
+
+    @property
+    def nprod(self):
+        return len(self.nodes) // 8
+
+    @property
+    def nlev(self):
+        return len(self.lev_ptr) - 1
 
 
 def _cplx_planes(vals):
@@ -176,11 +197,18 @@ def _cplx_planes(vals):
     return out
 
 
+def segments_per_equation(n, lanes=64):
+    """largest power of two G with n*G <= lanes (at least 1)"""
+    G = 1
+    while n * G * 2 <= lanes:
+        G *= 2
+    return G
+
+
 def compile_homotopy(hom, deg, lanes=64):
     """Build the tracker program for series of degree deg (D = deg+1)."""
     n = hom.n
     D = deg + 1
-    # distinct monomials over all components, plus the derivative monomials
     node_of = {}
     node_exps = [tuple(1 if j == i else 0 for j in range(n)) for i in range(n)]
     for i, e in enumerate(node_exps):
@@ -206,12 +234,17 @@ def compile_homotopy(hom, deg, lanes=64):
         level[idx] = level[a] + 1
         return idx
 
-    monos = set()
-    for _, sysm in hom.components:
-        for eq in sysm.eqs:
-            for _, e in eq:
-                monos.add(e)
-    for e in sorted(monos, key=lambda e: (sum(e), e)):
+    # merged terms per (component, equation, monomial)
+    comp_terms = []
+    for r, (w, sysm) in enumerate(hom.components):
+        for i, eq in enumerate(sysm.eqs):
+            merged = {}
+            for c, e in eq:
+                merged[e] = merged.get(e, 0) + c
+            for e, c in merged.items():
+                comp_terms.append((r, i, e, complex(c)))
+    monos = sorted({e for _, _, e, _ in comp_terms}, key=lambda e: (sum(e), e))
+    for e in monos:
         ensure(e)
         for j in range(n):
             if e[j] > 0:
@@ -219,14 +252,12 @@ def compile_homotopy(hom, deg, lanes=64):
                 d[j] -= 1
                 ensure(tuple(d))
     nprod = len(left)
-    # order product nodes by level (stable); renumber
-    prod_ids = list(range(n + 1, n + 1 + nprod))
-    order = sorted(prod_ids, key=lambda i: (level[i], i))
+    order = sorted(range(n + 1, n + 1 + nprod), key=lambda i: (level[i], i))
     remap = {i: i for i in range(n + 1)}
     for new, old in enumerate(order):
         remap[old] = n + 1 + new
-    L = np.zeros(nprod, np.int32)
-    Rt = np.zeros(nprod, np.int32)
+    L = np.zeros(nprod, np.int64)
+    Rt = np.zeros(nprod, np.int64)
     levs = []
     new_exps = [None] * (n + 1 + nprod)
     for i in range(n + 1):
@@ -243,92 +274,86 @@ def compile_homotopy(hom, deg, lanes=64):
     lev_ptr = np.zeros(nlev + 1, np.int32)
     for lv in range(1, nlev + 1):
         lev_ptr[lv] = lev_ptr[lv - 1] + sum(1 for x in levs if x == lv)
-    # storage: variables, ONE and every node used as a factor keep D
-    # coefficients; other nodes keep a ring of 2 (orders k and k-1)
+    wdeg = max(int(np.max(np.nonzero(np.abs(w) > 0)[0])) for w, _ in hom.components)
+    ring = 1
+    while ring < min(wdeg + 1, D):
+        ring *= 2
+    # storage: variables, ONE and factor nodes keep D coefficients, the
+    # others a power-of-two ring of the last orders
     nn = n + 1 + nprod
     factor = np.zeros(nn, bool)
     factor[: n + 1] = True
     for a in L:
         factor[a] = True
-    wdeg = max(int(np.max(np.nonzero(np.abs(w) > 0)[0])) for w, _ in hom.components)
-    ring = min(wdeg + 1, D)
-    slen = np.where(factor, D, ring).astype(np.int32)
-    off = np.zeros(nn, np.int32)
+    slen = np.where(factor, D, ring)
+    off = np.zeros(nn, np.int64)
     acc = 0
     for i in range(nn):
         off[i] = acc
         acc += slen[i]
-    nslots = int(acc)
-
-    # weighted component values: pairs (component, node)
-    pair_of = {}
-    pair_comp, pair_node = [], []
-    # residual terms per equation
+    mask = np.where(factor, -1, ring - 1)
+    dn = -np.ones(nn, np.int64)
+    nfactor = 0
+    for i in range(n + 1, nn):
+        if factor[i]:
+            dn[i] = nfactor
+            nfactor += 1
+    nodes = np.zeros((nprod, 8), np.int32)
+    for i in range(nprod):
+        a, b, sidx = int(L[i]), int(Rt[i]), n + 1 + i
+        nodes[i] = (off[a], mask[a], off[b], off[sidx], mask[sidx], dn[a] if a > n else -1,
+                    dn[sidx], 0)
+    # pairs (component, node) and terms, segmented per equation
+    pair_of, pairs = {}, []
     eq_terms = [[] for _ in range(n)]
-    for r, (w, sysm) in enumerate(hom.components):
-        for i, eq in enumerate(sysm.eqs):
-            merged = {}
-            for c, e in eq:
-                merged[e] = merged.get(e, 0) + c
-            for e, c in merged.items():
-                key = (r, node_of[e])
-                if key not in pair_of:
-                    pair_of[key] = len(pair_comp)
-                    pair_comp.append(r)
-                    pair_node.append(node_of[e])
-                eq_terms[i].append((pair_of[key], c))
-    # segments: split every equation's terms into chunks for the lanes
-    nterm_tot = sum(len(t) for t in eq_terms)
-    segs_per_eq = max(1, lanes // n)
-    seg_ptr, eq_seg, term_pair, term_cf = [0], [0], [], []
+    term_key = {}
+    for r, i, e, c in comp_terms:
+        key = (r, node_of[e])
+        if key not in pair_of:
+            pair_of[key] = len(pairs)
+            m = node_of[e]
+            pairs.append((r, off[m], mask[m], 0))
+        eq_terms[i].append((pair_of[key], c, (r, i, e)))
+    G = segments_per_equation(n, lanes)
+    seg_ptr, term_pair, term_cf = [0], [], []
     for i in range(n):
         terms = eq_terms[i]
-        ns = min(segs_per_eq, max(1, len(terms)))
-        chunks = np.array_split(np.arange(len(terms)), ns)
+        chunks = np.array_split(np.arange(len(terms)), G)
         for ch in chunks:
             for t in ch:
-                term_pair.append(terms[t][0])
-                term_cf.append(terms[t][1])
+                q, c, key = terms[t]
+                term_key[key] = len(term_pair)
+                term_pair.append(q)
+                term_cf.append(c)
             seg_ptr.append(len(term_pair))
-        eq_seg.append(len(seg_ptr) - 1)
-    assert len(term_pair) == nterm_tot
-    # Jacobian: entry (i,j) terms c*e_j*W0*node(m/x_j)
-    jpair_of = {}
-    jpair_comp, jpair_node = [], []
-    ent_terms = [[] for _ in range(n * n)]
-    for r, (w, sysm) in enumerate(hom.components):
-        for i, eq in enumerate(sysm.eqs):
-            for c, e in eq:
-                for j in range(n):
-                    if e[j] == 0:
-                        continue
-                    d = list(e)
-                    d[j] -= 1
-                    key = (r, node_of[tuple(d)])
-                    if key not in jpair_of:
-                        jpair_of[key] = len(jpair_comp)
-                        jpair_comp.append(r)
-                        jpair_node.append(node_of[tuple(d)])
-                    ent_terms[i * n + j].append((jpair_of[key], c * e[j]))
-    jent_ptr, jterm_pair, jterm_cf = [0], [], []
-    for ent in ent_terms:
-        for q, c in ent:
-            jterm_pair.append(q)
-            jterm_cf.append(c)
-        jent_ptr.append(len(jterm_pair))
-    weights = _cplx_planes(np.concatenate([w for w, _ in hom.components]))
+    # Jacobian: entry (i,j) sums coef[term] * e_j * W_r0 * node(m/x_j)_0
+    jpair_of, jpairs = {}, []
+    ent = [[] for _ in range(n * n)]
+    for r, i, e, c in comp_terms:
+        for j in range(n):
+            if e[j] == 0:
+                continue
+            d = list(e)
+            d[j] -= 1
+            m = node_of[tuple(d)]
+            key = (r, m)
+            if key not in jpair_of:
+                jpair_of[key] = len(jpairs)
+                jpairs.append((r, off[m]))
+            ent[i * n + j].append((term_key[(r, i, e)], jpair_of[key], e[j]))
+    jent_ptr, jterms = [0], []
+    for lst in ent:
+        for t, q, ex in lst:
+            assert t < (1 << JT_TERM_BITS) and q < (1 << JT_PAIR_BITS) and ex < (1 << JT_EXP_BITS)
+            jterms.append(t | (q << JT_TERM_BITS) | (ex << (JT_TERM_BITS + JT_PAIR_BITS)))
+        jent_ptr.append(len(jterms))
+    cf = np.array(term_cf, np.complex128)
     return Program(
-        n=n, deg=deg, left=L, right=Rt, lev_ptr=lev_ptr, off=off, slen=slen, nslots=nslots,
-        pair_comp=np.array(pair_comp, np.int32), pair_node=np.array(pair_node, np.int32),
-        seg_ptr=np.array(seg_ptr, np.int32), eq_seg=np.array(eq_seg, np.int32),
-        term_pair=np.array(term_pair, np.int32), term_coef=_cplx_planes(term_cf),
-        jpair_comp=np.array(jpair_comp, np.int32), jpair_node=np.array(jpair_node, np.int32),
-        jent_ptr=np.array(jent_ptr, np.int32), jterm_pair=np.array(jterm_pair, np.int32),
-        jterm_coef=_cplx_planes(jterm_cf), weights=weights, ncomp=len(hom.components), wdeg=wdeg,
-        monomials=sorted(monos), node_exps=node_exps)
-
-
-def flops_per_order(prog):
-    """complex multiply-adds per series order (products at order k use k+1)."""
-    return dict(products=len(prog.left), pairs=len(prog.pair_comp),
-                terms=len(prog.term_pair), jterms=len(prog.jterm_pair))
+        n=n, deg=deg, ncomp=len(hom.components), wdeg=wdeg, G=G,
+        nodes=nodes.reshape(-1), lev_ptr=lev_ptr, nslots=int(acc), nfactor=nfactor,
+        pairs=np.array(pairs, np.int32).reshape(-1), seg_ptr=np.array(seg_ptr, np.int32),
+        term_pair=np.array(term_pair, np.int32), term_coef=np.stack([cf.real, cf.imag]),
+        jpairs=np.array(jpairs, np.int32).reshape(-1), jent_ptr=np.array(jent_ptr, np.int32),
+        jterms=np.array(jterms, np.uint32).view(np.int32),
+        weights=_cplx_planes(np.concatenate([w for w, _ in hom.components])),
+        node_exps=node_exps, var_off=off[:n + 1].astype(np.int32))

@@ -58,13 +58,13 @@ def program_struct(prog):
         return a.ctypes.data_as(P_double)
 
     s = PhcProgram(
-        prog.n, prog.deg, len(prog.left), len(prog.lev_ptr) - 1, prog.nslots, prog.ncomp, prog.wdeg,
-        ia(prog.left), ia(prog.right), ia(prog.lev_ptr), ia(prog.off), ia(prog.slen),
-        len(prog.pair_comp), ia(prog.pair_comp), ia(prog.pair_node),
-        len(prog.seg_ptr) - 1, len(prog.term_pair), ia(prog.seg_ptr), ia(prog.eq_seg),
-        ia(prog.term_pair), da(prog.term_coef),
-        len(prog.jpair_comp), len(prog.jterm_pair), ia(prog.jpair_comp), ia(prog.jpair_node),
-        ia(prog.jent_ptr), ia(prog.jterm_pair), da(prog.jterm_coef), da(prog.weights))
+        prog.n, prog.deg, prog.ncomp, prog.wdeg, prog.G,
+        prog.nprod, prog.nlev, prog.nslots, prog.nfactor, ia(prog.nodes), ia(prog.lev_ptr),
+        len(prog.pairs) // 4, ia(prog.pairs),
+        len(prog.term_pair), ia(prog.seg_ptr), ia(prog.term_pair), da(prog.term_coef),
+        len(prog.jpairs) // 2, ia(prog.jpairs),
+        len(prog.jterms), ia(prog.jent_ptr), ia(prog.jterms),
+        da(prog.weights))
     s._keep = keep
     return s
 
@@ -95,9 +95,10 @@ class DeviceProgram:
         self.handle = h
 
     def lds_bytes(self, K):
-        b = ctypes.c_size_t()
-        check(load().phc_tracker_lds_bytes(self.handle, K, ctypes.byref(b)))
-        return b.value
+        """(program bytes, bytes per path) of LDS at precision K"""
+        a, b = ctypes.c_size_t(), ctypes.c_size_t()
+        check(load().phc_tracker_lds_bytes(self.handle, K, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def __del__(self):
         try:
@@ -138,29 +139,29 @@ class Tracker:
         self.dev = DeviceProgram(self.prog)
         self.dev0 = DeviceProgram(self.prog0)
 
-    def track_device(self, d_start, KS, npaths, K, bufs, params=None, block=64, stream=None):
+    def track_device(self, d_start, KS, npaths, K, bufs, params=None, waves=8, stream=None):
         p = (params or TrackParams()).to_c()
         check(load().phc_track(self.dev.handle, K, npaths, d_start.ptr, KS, bufs.endp.ptr,
                                bufs.tend.ptr, bufs.status.ptr, bufs.nsteps.ptr, bufs.nrej.ptr,
-                               bufs.dx.ptr, bufs.work.ptr, ctypes.byref(p), block, stream))
+                               bufs.dx.ptr, bufs.work.ptr, ctypes.byref(p), waves, stream))
 
-    def track(self, starts, K=2, params=None, block=64):
+    def track(self, starts, K=2, params=None, waves=8):
         """starts: complex128 [npaths][n] (or [2KS][npaths][n] planes)."""
         planes = starts if starts.dtype == np.float64 else points_to_planes(starts)
         KS = planes.shape[0] // 2
         npaths = planes.shape[1]
         d_start = DeviceArray.from_host(planes)
         bufs = TrackBuffers(npaths, self.n, K)
-        self.track_device(d_start, KS, npaths, K, bufs, params, block)
+        self.track_device(d_start, KS, npaths, K, bufs, params, waves)
         check(load().phc_synchronize())
         return bufs.to_host()
 
     def refine_device(self, d_x, KS, npaths, K, d_out, d_dx, d_res, d_st, iters, tol, t=1.0,
-                      block=64, stream=None):
+                      waves=8, stream=None):
         check(load().phc_refine(self.dev0.handle, K, npaths, t, d_x.ptr, KS, d_out.ptr, iters,
-                                tol, d_dx.ptr, d_res.ptr, d_st.ptr, block, stream))
+                                tol, d_dx.ptr, d_res.ptr, d_st.ptr, waves, stream))
 
-    def refine(self, planes, K, iters, tol, t=1.0, block=64):
+    def refine(self, planes, K, iters, tol, t=1.0, waves=8):
         """Newton at t in precision K from [2KS][npaths][n] planes."""
         KS = planes.shape[0] // 2
         npaths = planes.shape[1]
@@ -168,7 +169,7 @@ class Tracker:
         d_out = DeviceArray((2 * K, npaths, self.n))
         d_dx, d_res = DeviceArray((npaths,)), DeviceArray((npaths,))
         d_st = DeviceArray((npaths,), np.int32)
-        self.refine_device(d_x, KS, npaths, K, d_out, d_dx, d_res, d_st, iters, tol, t, block)
+        self.refine_device(d_x, KS, npaths, K, d_out, d_dx, d_res, d_st, iters, tol, t, waves)
         check(load().phc_synchronize())
         return dict(x=d_out.to_host(), dx=d_dx.to_host(), res=d_res.to_host(),
                     status=d_st.to_host())

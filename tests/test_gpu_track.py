@@ -36,14 +36,16 @@ def test_track_matches_oracle(K, n, degree, seed):
     assert np.all(g["status"] == 1)
 
 
-@pytest.mark.parametrize("block", [64, 128])
-def test_track_block_sizes_agree(block):
+@pytest.mark.parametrize("waves", [1, 3, 8])
+def test_track_wave_counts_agree(waves):
+    """Wavefronts of one workgroup track their paths independently: the
+    result does not depend on how many share a workgroup."""
     from phcpack_amd.tracker import Tracker, TrackParams
     p, hom, sols = make(4, 2, 5)
     tr = Tracker(hom, deg=8)
     prm = TrackParams()
-    a = tr.track(sols, K=2, params=prm, block=64)
-    b = tr.track(sols, K=2, params=prm, block=block)
+    a = tr.track(sols, K=2, params=prm, waves=8)
+    b = tr.track(sols, K=2, params=prm, waves=waves)
     assert np.array_equal(a["endp"], b["endp"])
 
 
