@@ -166,8 +166,8 @@ int phc_track(phc_tracker *tracker, int K, int npaths, const double *start_dev, 
               int *nrej_dev, double *dxlast_dev, int *work_dev, const phc_track_params *params,
               int waves, void *stream);
 
-/* Newton refinement at fixed t in precision K (1, 2, 4, 8) of points with KS
- * limbs; xout [2*K][npaths][n]; res = max_i |H_i(xout, t)| evaluated in
+/* Newton refinement at fixed t in precision K (1, 2, 4; octo-double K = 8
+ * is refused with -2 in this build) of points with KS limbs; xout [2*K][npaths][n]; res = max_i |H_i(xout, t)| evaluated in
  * precision K; status 1 converged (max|dx| <= tol*(1+max|x|)), 0 not, -1
  * singular.  iters = 0 only evaluates the residual.
  * Plays the role of the Newton drivers at the endpoints

@@ -50,7 +50,11 @@ enum { ST_RUNNING = 0, ST_SUCCESS = 1, ST_MINSTEP = 2, ST_MAXSTEPS = 3, ST_SINGU
        ST_DIVERGED = 5 };
 
 constexpr int JT_TERM_BITS = 16, JT_PAIR_BITS = 12;
-constexpr size_t LDS_LIMIT = 160 * 1024;
+// LDS per workgroup is kept at or below 64 KiB (several workgroups share a
+// compute unit's 160 KiB); launches above 64 KiB per workgroup have not been
+// validated on the device.
+constexpr size_t LDS_LIMIT = 64 * 1024;
+constexpr size_t LDS_PER_CU = 160 * 1024;
 
 // device copy of the program (global memory)
 struct DevProgram {
@@ -641,11 +645,15 @@ int geometry(const DevProgram &P, int K, int max_waves, int npaths, int *grid, i
   int wv = (int)((cap - prog) / pb);
   if (wv > max_waves) wv = max_waves;
   if (wv > 8) wv = 8;
-  int g = ncu;
-  if ((long)ncu * wv > npaths) g = (npaths + wv - 1) / wv;
-  *grid = g < 1 ? 1 : g;
+  const size_t bytes = prog + (size_t)wv * pb;
+  long per_cu = (long)(LDS_PER_CU / bytes);
+  if (per_cu < 1) per_cu = 1;
+  long g = (long)ncu * per_cu;
+  const long need = (npaths + wv - 1) / wv;
+  if (g > need) g = need;
+  *grid = (int)(g < 1 ? 1 : g);
   *waves = wv;
-  *lds = prog + (size_t)wv * pb;
+  *lds = bytes;
   return 0;
 }
 
@@ -819,7 +827,9 @@ int phc_refine(phc_tracker *tr, int K, int npaths, double t, const double *xin, 
                double *xout, int iters, double tol, double *dxlast, double *res, int *status,
                int waves, void *stream) {
   PHC_REQUIRE(tr, "phc_refine: null tracker");
-  PHC_REQUIRE(K == 1 || K == 2 || K == 4 || K == 8, "phc_refine: K must be 1, 2, 4 or 8");
+  PHC_REQUIRE(K == 1 || K == 2 || K == 4, "phc_refine: K must be 1, 2 or 4 (octo-double device "
+              "refinement is switched off: its kernel stalled a device run and the cause is not "
+              "established)");
   PHC_REQUIRE(KS >= 1 && KS <= 8, "phc_refine: KS must be in 1..8");
   PHC_REQUIRE(npaths >= 0 && iters >= 0, "phc_refine: negative size");
   PHC_REQUIRE(waves >= 1 && waves <= 8, "phc_refine: waves per workgroup must be in 1..8");
@@ -830,10 +840,10 @@ int phc_refine(phc_tracker *tr, int K, int npaths, double t, const double *xin, 
   if (geometry(tr->P, K, waves, npaths, &grid, &wv, &lds) != 0) return -2;
   PHC_HIP(hipMemsetAsync(tr->counter, 0, sizeof(int), st));
   const void *fn = K == 1 ? (const void *)refine_kernel<1> : K == 2 ? (const void *)refine_kernel<2>
-                 : K == 4 ? (const void *)refine_kernel<4> : (const void *)refine_kernel<8>;
+                                                           : (const void *)refine_kernel<4>;
   PHC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 #define RK(KK) hipLaunchKernelGGL(refine_kernel<KK>, dim3(grid), dim3(64 * wv), lds, st, tr->P, npaths, t, xin, KS, xout, iters, tol, dxlast, res, status, tr->counter)
-  if (K == 1) RK(1); else if (K == 2) RK(2); else if (K == 4) RK(4); else RK(8);
+  if (K == 1) RK(1); else if (K == 2) RK(2); else RK(4);
 #undef RK
   PHC_HIP(hipGetLastError());
   return 0;

@@ -49,7 +49,9 @@ def test_track_wave_counts_agree(waves):
     assert np.array_equal(a["endp"], b["endp"])
 
 
-@pytest.mark.parametrize("K", [2, 4, 8])
+@pytest.mark.parametrize("K", [2, 4, pytest.param(8, marks=pytest.mark.skip(
+    reason="octo-double device refinement is switched off: its kernel (non-inlined helpers "
+           "reaching LDS through FLAT instructions) stalled a device run; cause not established"))])
 def test_refine_matches_oracle(K):
     from phcpack_amd.tracker import Tracker, TrackParams
     p, hom, sols = make(4, 2, 7)
@@ -61,6 +63,17 @@ def test_refine_matches_oracle(K):
     for key in ("x", "dx", "res", "status"):
         assert np.array_equal(r[key], o[key]), key
     assert r["res"].max() < {2: 1e-28, 4: 1e-55, 8: 1e-100}[K]
+
+
+def test_refine_octo_double_refused():
+    """K = 8 on the device is refused with an error, never silently degraded."""
+    from phcpack_amd._lib import PhcError
+    from phcpack_amd.tracker import Tracker, TrackParams
+    p, hom, sols = make(3, 2, 7)
+    tr = Tracker(hom, deg=8)
+    g = tr.track(sols, K=1, params=TrackParams(tolend=1e-13))
+    with pytest.raises(PhcError, match="octo-double"):
+        tr.refine(g["endp"], 8, iters=1, tol=1e-60)
 
 
 def test_decoupled_closed_form():
