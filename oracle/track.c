@@ -119,8 +119,15 @@ static void product_pass(const otr_program *P, work_t *w, int k) {
   int i, q;
   for (i = 0; i < P->nprod; i++) {
     const int *nd = P->nodes + 8 * i;
+    /* terms q = 0..k alternate between two accumulators (even, odd q) */
     omd_cmul(K, slot(w, nd[0], -1, 0), slot(w, nd[2], -1, k), acc);
-    for (q = 1; q <= k; q++) omd_cmac(K, slot(w, nd[0], -1, q), slot(w, nd[2], -1, k - q), acc);
+    if (k >= 1) {
+      double a1[16];
+      omd_cmul(K, slot(w, nd[0], -1, 1), slot(w, nd[2], -1, k - 1), a1);
+      for (q = 2; q <= k; q++)
+        omd_cmac(K, slot(w, nd[0], -1, q), slot(w, nd[2], -1, k - q), (q & 1) ? a1 : acc);
+      omd_cadd(K, acc, a1, acc);
+    }
     ccopy(K, acc, slot(w, nd[3], nd[4], k));
   }
 }
@@ -155,11 +162,17 @@ static void residual(const otr_program *P, work_t *w, int k) {
   for (i = 0; i < P->n; i++) {
     for (g = 0; g < G; g++) {
       const int s = i * G + g;
-      czero(K, acc);
+      double a4[4][16];
+      int r;
+      for (r = 0; r < 4; r++) czero(K, a4[r]);
+      /* term t goes to accumulator (t - first) mod 4 */
       for (t = P->seg_ptr[s]; t < P->seg_ptr[s + 1]; t++) {
         coef(P, K, t, c);
-        omd_cmac(K, c, ZP(w->u, P->term_pair[t]), acc);
+        omd_cmac(K, c, ZP(w->u, P->term_pair[t]), a4[(t - P->seg_ptr[s]) & 3]);
       }
+      omd_cadd(K, a4[0], a4[1], a4[0]);
+      omd_cadd(K, a4[2], a4[3], a4[2]);
+      omd_cadd(K, a4[0], a4[2], acc);
       ccopy(K, acc, ZP(w->part, g));
     }
     tree_sum(K, w->part, G);
@@ -175,7 +188,9 @@ static void jacobian(const otr_program *P, work_t *w) {
   for (q = 0; q < P->njpair; q++)
     omd_cmul(K, ZP(w->W, 3 * P->jpairs[2 * q]), ZP(w->slots, P->jpairs[2 * q + 1]), ZP(w->uj, q));
   for (e = 0; e < P->n * P->n; e++) {
-    czero(K, acc);
+    double a4[4][16];
+    int r;
+    for (r = 0; r < 4; r++) czero(K, a4[r]);
     for (t = P->jent_ptr[e]; t < P->jent_ptr[e + 1]; t++) {
       const unsigned jt = (unsigned)P->jterms[t];
       const int term = jt & ((1u << JT_TERM_BITS) - 1);
@@ -183,8 +198,11 @@ static void jacobian(const otr_program *P, work_t *w) {
       const int ex = jt >> (JT_TERM_BITS + JT_PAIR_BITS);
       if (ex == 1) ccopy(K, ZP(w->uj, qq), v); else cscale_d(K, ZP(w->uj, qq), (double)ex, v);
       coef(P, K, term, c);
-      omd_cmac(K, c, v, acc);
+      omd_cmac(K, c, v, a4[(t - P->jent_ptr[e]) & 3]);
     }
+    omd_cadd(K, a4[0], a4[1], a4[0]);
+    omd_cadd(K, a4[2], a4[3], a4[2]);
+    omd_cadd(K, a4[0], a4[2], acc);
     ccopy(K, acc, ZP(w->A, e));
   }
 }

@@ -193,9 +193,17 @@ template <int K> __device__ void product_pass(const LProg &P, Path<K> &w, int k)
     for (int i = P.lev[lv - 1] + lane_id(); i < P.lev[lv]; i += 64) {
       const int *nd = P.nodes + 8 * i;
       const int aoff = nd[0], boff = nd[2], soff = nd[3], smask = nd[4];
-      cplx<K> acc = Cx::mul(w.slots[aoff], w.slots[boff + k]);
-      for (int q = 1; q <= k; ++q) Cx::mac(acc, w.slots[aoff + q], w.slots[boff + k - q]);
-      w.slots[soff + (k & smask)] = acc;
+      // terms q = 0..k alternate between two accumulators (even, odd q)
+      cplx<K> a0 = Cx::mul(w.slots[aoff], w.slots[boff + k]);
+      if (k >= 1) {
+        cplx<K> a1 = Cx::mul(w.slots[aoff + 1], w.slots[boff + k - 1]);
+        for (int q = 2; q <= k; q += 2) {
+          Cx::mac(a0, w.slots[aoff + q], w.slots[boff + k - q]);
+          if (q + 1 <= k) Cx::mac(a1, w.slots[aoff + q + 1], w.slots[boff + k - q - 1]);
+        }
+        a0 = Cx::add(a0, a1);
+      }
+      w.slots[soff + (k & smask)] = a0;
     }
     wsync();
   }
@@ -233,8 +241,22 @@ template <int K> __device__ void residual(const LProg &P, Path<K> &w, int k) {
   }
   wsync();
   cplx<K> acc = Cx::zero();
-  if (l < P.n * P.G)
-    for (int t = P.seg[l]; t < P.seg[l + 1]; ++t) Cx::mac(acc, coef<K>(P, t), w.u[P.tpair[t]]);
+  if (l < P.n * P.G) {
+    // term t of the segment goes to accumulator (t - first) mod 4
+    cplx<K> a0 = Cx::zero(), a1 = Cx::zero(), a2 = Cx::zero(), a3 = Cx::zero();
+    const int t1 = P.seg[l + 1];
+    int t = P.seg[l];
+    for (; t + 4 <= t1; t += 4) {
+      Cx::mac(a0, coef<K>(P, t), w.u[P.tpair[t]]);
+      Cx::mac(a1, coef<K>(P, t + 1), w.u[P.tpair[t + 1]]);
+      Cx::mac(a2, coef<K>(P, t + 2), w.u[P.tpair[t + 2]]);
+      Cx::mac(a3, coef<K>(P, t + 3), w.u[P.tpair[t + 3]]);
+    }
+    if (t < t1) Cx::mac(a0, coef<K>(P, t), w.u[P.tpair[t]]);
+    if (t + 1 < t1) Cx::mac(a1, coef<K>(P, t + 1), w.u[P.tpair[t + 1]]);
+    if (t + 2 < t1) Cx::mac(a2, coef<K>(P, t + 2), w.u[P.tpair[t + 2]]);
+    acc = Cx::add(Cx::add(a0, a1), Cx::add(a2, a3));
+  }
   acc = group_sum<K>(acc, P.G);
   if (l < P.n * P.G && (l % P.G) == 0) w.rhs[l / P.G] = Cx::neg(acc);
   wsync();
@@ -250,16 +272,23 @@ template <int K> __device__ void jacobian(const LProg &P, Path<K> &w) {
   }
   wsync();
   for (int e = l; e < P.n * P.n; e += 64) {
-    cplx<K> acc = Cx::zero();
-    for (int t = P.jent[e]; t < P.jent[e + 1]; ++t) {
-      const unsigned jt = (unsigned)P.jterms[t];
-      const int term = jt & ((1u << JT_TERM_BITS) - 1);
-      const int q = (jt >> JT_TERM_BITS) & ((1u << JT_PAIR_BITS) - 1);
-      const int ex = jt >> (JT_TERM_BITS + JT_PAIR_BITS);
-      const cplx<K> v = (ex == 1) ? w.uj[q] : Cx::scale_d(w.uj[q], (double)ex);
-      Cx::mac(acc, coef<K>(P, term), v);
+    // term t of the entry goes to accumulator (t - first) mod 4
+    cplx<K> a[4] = {Cx::zero(), Cx::zero(), Cx::zero(), Cx::zero()};
+    const int t0 = P.jent[e], t1 = P.jent[e + 1];
+    for (int t = t0; t < t1; t += 4) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (t + r < t1) {
+          const unsigned jt = (unsigned)P.jterms[t + r];
+          const int term = jt & ((1u << JT_TERM_BITS) - 1);
+          const int q = (jt >> JT_TERM_BITS) & ((1u << JT_PAIR_BITS) - 1);
+          const int ex = jt >> (JT_TERM_BITS + JT_PAIR_BITS);
+          const cplx<K> v = (ex == 1) ? w.uj[q] : Cx::scale_d(w.uj[q], (double)ex);
+          Cx::mac(a[r], coef<K>(P, term), v);
+        }
+      }
     }
-    w.A[e] = acc;
+    w.A[e] = Cx::add(Cx::add(a[0], a[1]), Cx::add(a[2], a[3]));
   }
   wsync();
 }

@@ -13,6 +13,8 @@ converts them to FP64 floating-point operations with two tables:
   qd 89/336/893, od 270/1742/5126 (Dekker products).
 """
 
+import numpy as np
+
 TWO_SUM, QTS, TWO_PROD = 6, 3, 3
 
 
@@ -71,16 +73,13 @@ def complex_costs(K, table=BUILD):
 
 
 def _phase_ops(prog):
-    """complex operation counts of each phase: dict phase -> dict op -> count"""
-    n, D = prog.n, prog.deg + 1
-    nprod, npair, nterm = len(prog.left), len(prog.pair_comp), len(prog.term_pair)
-    nseg = len(prog.seg_ptr) - 1
-    njpair, njterm = len(prog.jpair_comp), len(prog.jterm_pair)
-    ev = dict(cmul=nprod + npair, cmac=nterm, cadd=nseg - n)
-    jac = dict(cmul=njpair, cmac=njterm)
-    lu = dict(cdiv=n, cmul=sum(n - j - 1 for j in range(n)),
-              cmac=sum((n - j - 1) ** 2 for j in range(n)))
-    solve = dict(cmac=n * (n - 1), cmul=n)
+    """complex operation counts of each phase of the tracker kernel:
+    dict phase -> dict op -> count (cmac also stands for cmsub)"""
+    n, D, G = prog.n, prog.deg + 1, prog.G
+    nprod = prog.nprod
+    npair, nterm = len(prog.pairs) // 4, len(prog.term_pair)
+    njpair, njterm = len(prog.jpairs) // 2, len(prog.jterms)
+    nscale = int(((prog.jterms.view(np.uint32) >> 28) != 1).sum()) if njterm else 0
 
     def add(*ds):
         out = {}
@@ -89,12 +88,20 @@ def _phase_ops(prog):
                 out[k] = out.get(k, 0) + v
         return out
 
-    newton = add(ev, jac, lu, solve, dict(cadd=n))
+    def residual(k):
+        return dict(cmul=npair, cmac=npair * min(prog.wdeg, k) + nterm, cadd=n * (G - 1))
+
+    products0 = dict(cmul=nprod)
+    ev = add(products0, residual(0))
+    jac = dict(cmul=njpair, cmac=njterm, scale_d=nscale)
+    gj = dict(cdiv=n, cmul=2 * n * (n - 1), cmac=n * (n - 1) ** 2)
+    apply = dict(cmac=n * n, cadd=n * (G - 1))
+    newton = add(ev, jac, gj, apply, dict(cadd=n))
     orders = {}
     for k in range(1, D):
-        orders = add(orders, dict(cmul=nprod + npair, cmac=nprod * k + npair * min(prog.wdeg, k) + nterm,
-                                  cadd=nseg - n), solve, dict(cmul=nprod, cmac=nprod, cadd=nprod))
-    pred = add(ev, jac, lu, orders)
+        orders = add(orders, dict(cmul=nprod, cmac=nprod * k), residual(k), apply,
+                     dict(cmul=nprod, cmac=nprod, cadd=nprod))
+    pred = add(ev, jac, gj, orders)
     feval = add(ev, dict(scale_d=n * (D - 1), cadd=n * (D - 1)))
     return dict(predictor=pred, newton=newton, feval=feval, eval=ev)
 
