@@ -141,6 +141,38 @@ typedef struct {
  * phc_tracker_create calls it first. */
 int phc_program_validate(const phc_program *program);
 
+/* Learned step-size control (no counterpart in the reference, whose step
+ * rule - the pole step of LU_Prediction,
+ * Deformations/Curves/standard_predictor_convolutions.adb:1142-1185 - stays
+ * the judge and the fallback: the predicted-residual feedback and the
+ * Newton corrector still accept or reject every step, and a rejected step
+ * is halved as before).  The controller multiplies the pole step h0 =
+ * min(pbeta*r, maxstep) by one of ncand candidates mult[c].  Each step the
+ * tracker forms PHC_CTL_NFEAT features (log2 of the Fabry radius, of the
+ * previous step, t, log2 of the largest series coefficient of orders 1..8,
+ * log2 of the last four Newton corrections, rejections of the previous
+ * step), standardizes them with mu and sd, and evaluates an FP32 MLP
+ * nfeat -> 64 (ReLU) -> 64 (ReLU) -> ncand logits (phc_mlp layout).
+ * mode 0: plain rule (the controller is ignored);
+ * mode 1: learned: the largest c whose logit >= logit_threshold (c = 0 if none);
+ * mode 2: explore: c drawn from a hash of (seed, path, step), and every step
+ *         logged: features log_x[path][i][nfeat], candidate log_c[path][i],
+ *         log_y[path][i] = 1 if the first attempt was accepted, log_n[path]
+ *         records (at most log_capacity per path). */
+#define PHC_CTL_NFEAT 16
+#define PHC_CTL_HIDDEN 64
+#define PHC_CTL_MAXCAND 8
+typedef struct {
+  int mode, ncand;
+  double mult[PHC_CTL_MAXCAND];
+  const float *weights, *mu, *sd; /* device */
+  float logit_threshold;
+  unsigned seed;
+  int log_capacity;
+  float *log_x;                   /* device */
+  int *log_c, *log_y, *log_n;     /* device */
+} phc_controller;
+
 typedef struct phc_tracker phc_tracker;
 int phc_tracker_create(phc_tracker **tracker, const phc_program *program);
 int phc_tracker_destroy(phc_tracker *tracker);
@@ -164,7 +196,26 @@ int phc_tracker_lds_bytes(const phc_tracker *tracker, int K, size_t *program_byt
 int phc_track(phc_tracker *tracker, int K, int npaths, const double *start_dev, int KS,
               double *endp_dev, double *tend_dev, int *status_dev, int *nsteps_dev,
               int *nrej_dev, double *dxlast_dev, int *work_dev, const phc_track_params *params,
-              int waves, void *stream);
+              const phc_controller *controller, int waves, void *stream);
+
+/* ------------------------------------------------------- controller MLP */
+
+/* number of floats of an MLP F -> H -> H -> C (W1[F][H], b1, W2[H][H], b2,
+ * W3[H][C], b3) */
+int phc_mlp_size(int F, int H, int C, int *nparams);
+/* logits [B][C] of inputs X [B][F] (device arrays) */
+int phc_mlp_forward(int F, int H, int C, const float *w_dev, int B, const float *X_dev,
+                    float *logits_dev, void *stream);
+/* gradient of the mean binary cross-entropy of logit cand[i] against y[i]
+ * over B samples (device arrays; loss_dev may be NULL) */
+int phc_mlp_grad(int F, int H, int C, const float *w_dev, int B, const float *X_dev,
+                 const int *cand_dev, const float *y_dev, float *grad_dev, float *loss_dev,
+                 void *stream);
+/* minibatch Adam on N samples for `epochs` passes; w updated in place;
+ * losses (host, may be NULL) receives the mean loss of each epoch */
+int phc_mlp_train(int F, int H, int C, float *w_dev, int N, const float *X_dev,
+                  const int *cand_dev, const float *y_dev, int epochs, int batch, float lr,
+                  unsigned seed, float *losses, void *stream);
 
 /* Newton refinement at fixed t in precision K (1, 2, 4; octo-double K = 8
  * is refused with -2 in this build) of points with KS limbs; xout [2*K][npaths][n]; res = max_i |H_i(xout, t)| evaluated in

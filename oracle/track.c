@@ -289,7 +289,59 @@ static double max_abs_x(const otr_program *P, work_t *w) {
   return m;
 }
 
-static int newton(const otr_program *P, work_t *w, int iters, double tol, double *dxl, int *count) {
+/* log2 from the binary exponent and 2(z + z^3/3 + z^5/5 + z^7/7),
+ * z = (m-1)/(m+1): the same IEEE double operations as the device */
+double otr_lg2(double x) {
+  int e = 0;
+  double m, z, z2, sr;
+  if (!(x > 0.0)) return -1100.0;
+  if (x == INFINITY) return 1100.0;
+  m = 2.0 * frexp(x, &e);
+  z = (m - 1.0) / (m + 1.0);
+  z2 = z * z;
+  sr = z * (1.0 + z2 * (1.0 / 3.0 + z2 * (1.0 / 5.0 + z2 * (1.0 / 7.0))));
+  return (double)(e - 1) + sr * 2.8853900817779268;
+}
+
+static unsigned mix_hash(unsigned seed, unsigned path, unsigned step) {
+  unsigned h = seed ^ (path * 0x9E3779B1u) ^ (step * 0x85EBCA77u);
+  h ^= h >> 16;
+  h *= 0x7FEB352Du;
+  h ^= h >> 15;
+  h *= 0x846CA68Bu;
+  h ^= h >> 16;
+  return h;
+}
+
+/* the device's MLP inference: hidden unit j per lane, output sums as a
+ * 64-lane butterfly */
+static int mlp_choose(const otr_controller *ctl, const double *f) {
+  const int F = OTR_NFEAT, H = OTR_HIDDEN, C = ctl->ncand;
+  const float *W1 = ctl->weights, *b1 = W1 + F * H, *W2 = b1 + H, *b2 = W2 + H * H, *W3 = b2 + H;
+  const float *b3 = W3 + H * C;
+  float h1[OTR_HIDDEN], g[OTR_HIDDEN], v[OTR_HIDDEN];
+  int i, j, c, m, best = 0;
+  for (j = 0; j < H; j++) {
+    float h = b1[j];
+    for (i = 0; i < F; i++) h += W1[i * H + j] * (((float)f[i] - ctl->mu[i]) / ctl->sd[i]);
+    h1[j] = h < 0.0f ? 0.0f : h;
+  }
+  for (j = 0; j < H; j++) {
+    float a = b2[j];
+    for (i = 0; i < H; i++) a += W2[i * H + j] * h1[i];
+    g[j] = a < 0.0f ? 0.0f : a;
+  }
+  for (c = 0; c < C; c++) {
+    for (j = 0; j < H; j++) v[j] = W3[j * C + c] * g[j];
+    for (m = 1; m < H; m <<= 1)
+      for (j = 0; j + m < H; j += 2 * m) v[j] = v[j] + v[j + m];
+    if (v[0] + b3[c] >= ctl->logit_threshold) best = c;
+  }
+  return best;
+}
+
+static int newton(const otr_program *P, work_t *w, int iters, double tol, double *dxl, int *count,
+                  double *dxh) {
   const int K = w->K, D = P->deg + 1;
   double dx[16 * 64];
   int it, j;
@@ -302,6 +354,7 @@ static int newton(const otr_program *P, work_t *w, int iters, double tol, double
     for (j = 0; j < P->n; j++) omd_cadd(K, ZP(w->slots, j * D), ZP(dx, j), ZP(w->slots, j * D));
     m = max_abs(K, dx, P->n);
     *dxl = m;
+    if (dxh) { dxh[3] = dxh[2]; dxh[2] = dxh[1]; dxh[1] = dxh[0]; dxh[0] = m; }
     if (m <= tol * (1.0 + max_abs_x(P, w))) return 1;
   }
   return 0;
@@ -379,14 +432,17 @@ static int predictor_series(const otr_program *P, work_t *w) {
 
 void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const double *start,
                int KS, double *endp, double *tend, int *status, int *nsteps, int *nrej,
-               double *dxlast, int *work, const otr_params *prm) {
+               double *dxlast, int *work, const otr_params *prm, const otr_controller *ctl) {
   const int n = P->n, D = P->deg + 1;
+  const int mode = ctl ? ctl->mode : 0;
   work_t w;
   int path;
   if (n > 64 || !work_alloc(P, K, &w)) { work_free(&w); return; }
   for (path = p0; path < p1; path++) {
-    double t = 0.0, h = prm->maxstep, dxl = 0.0;
+    double t = 0.0, h = prm->maxstep, dxl = 0.0, hprev = prm->maxstep;
+    double dxh[4] = {1.0, 1.0, 1.0, 1.0};
     int st = ST_RUNNING, steps = 0, rej = 0, j, k, npred = 0, nnewt = 0, nfeval = 0;
+    int rejprev = 0, nlog = 0;
     load_point(P, &w, start, KS, npaths, path);
     while (st == ST_RUNNING && t < 1.0) {
       const int kd = D - 1 - prm->fabry_offset;
@@ -401,6 +457,25 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
           if (den > 0.0) r = fmin(r, absz(K, ZP(w.slots, j * D + kd - 1)) / den);
         }
       h = (r < INFINITY && r > 0.0) ? fmin(prm->pbeta * r, prm->maxstep) : prm->maxstep;
+      {
+        int cand = 0, rej0, i;
+        double feat[OTR_NFEAT];
+        if (mode != 0) {
+          feat[0] = otr_lg2((r < INFINITY && r > 0.0) ? r : prm->maxstep);
+          feat[1] = otr_lg2(hprev);
+          feat[2] = t;
+          for (k = 1; k <= 8; k++) {
+            double c = 0.0;
+            for (j = 0; j < n; j++) c = fmax(c, k < D ? absz(K, ZP(w.slots, j * D + k)) : 0.0);
+            feat[2 + k] = otr_lg2(c + 1e-300);
+          }
+          for (i = 0; i < 4; i++) feat[11 + i] = otr_lg2(dxh[i] + 1e-300);
+          feat[15] = (double)rejprev;
+          if (mode == 1) cand = mlp_choose(ctl, feat);
+          else cand = (int)(mix_hash(ctl->seed, (unsigned)path, (unsigned)(npred - 1)) % (unsigned)ctl->ncand);
+          h *= ctl->mult[cand];
+        }
+        rej0 = rej;
       for (j = 0; j < n; j++) ccopy(K, ZP(w.slots, j * D), ZP(w.x0, j));
       while (!accepted) {
         double tn = t + h, hs, res;
@@ -423,15 +498,24 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
         res = max_abs(K, w.rhs, n);
         ok = res <= prm->alpha * (1.0 + max_abs_x(P, &w));
         if (ok) {
-          const int nr = newton(P, &w, prm->corsteps, prm->tolcor, &dxl, &nnewt);
+          const int nr = newton(P, &w, prm->corsteps, prm->tolcor, &dxl, &nnewt, dxh);
           if (nr < 0) { st = ST_SINGULAR; break; }
           ok = (nr == 1);
         }
-        if (ok) { t = tn; accepted = 1; }
+        if (ok) { t = tn; accepted = 1; hprev = hs; }
         else {
           ++rej;
           h *= 0.5;
           if (h < prm->minstep) { st = ST_MINSTEP; break; }
+        }
+      }
+        rejprev = rej - rej0;
+        if (mode == 2 && nlog < ctl->log_capacity) {
+          for (i = 0; i < OTR_NFEAT; i++)
+            ctl->log_x[((size_t)path * ctl->log_capacity + nlog) * OTR_NFEAT + i] = (float)feat[i];
+          ctl->log_c[(size_t)path * ctl->log_capacity + nlog] = cand;
+          ctl->log_y[(size_t)path * ctl->log_capacity + nlog] = (rej == rej0 && st == ST_RUNNING) ? 1 : 0;
+          ++nlog;
         }
       }
       if (st != ST_RUNNING) break;
@@ -442,7 +526,7 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
     if (st == ST_RUNNING) {
       int nr;
       set_weights(P, &w, 1.0);
-      nr = newton(P, &w, prm->endsteps, prm->tolend, &dxl, &nnewt);
+      nr = newton(P, &w, prm->endsteps, prm->tolend, &dxl, &nnewt, NULL);
       st = (nr < 0) ? ST_SINGULAR : ST_SUCCESS;
     }
     store_point(P, &w, endp, npaths, path);
@@ -456,6 +540,7 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
       work[npaths + path] = nnewt;
       work[2 * npaths + path] = nfeval;
     }
+    if (mode == 2) ctl->log_n[path] = nlog;
   }
   work_free(&w);
 }
@@ -467,18 +552,20 @@ typedef struct {
   double *endp, *tend, *dxlast;
   int *status, *nsteps, *nrej, *work;
   const otr_params *prm;
+  const otr_controller *ctl;
 } job_t;
 
 static void *run_job(void *arg) {
   job_t *j = (job_t *)arg;
   otr_track(j->P, j->K, j->npaths, j->p0, j->p1, j->start, j->KS, j->endp, j->tend, j->status,
-            j->nsteps, j->nrej, j->dxlast, j->work, j->prm);
+            j->nsteps, j->nrej, j->dxlast, j->work, j->prm, j->ctl);
   return NULL;
 }
 
 void otr_track_mt(const otr_program *P, int K, int npaths, const double *start, int KS,
                   double *endp, double *tend, int *status, int *nsteps, int *nrej,
-                  double *dxlast, int *work, const otr_params *prm, int nthreads) {
+                  double *dxlast, int *work, const otr_params *prm, const otr_controller *ctl,
+                  int nthreads) {
   pthread_t th[256];
   job_t jobs[256];
   int i;
@@ -491,6 +578,7 @@ void otr_track_mt(const otr_program *P, int K, int npaths, const double *start, 
     jobs[i].start = start; jobs[i].endp = endp; jobs[i].tend = tend; jobs[i].dxlast = dxlast;
     jobs[i].status = status; jobs[i].nsteps = nsteps; jobs[i].nrej = nrej; jobs[i].prm = prm;
     jobs[i].work = work;
+    jobs[i].ctl = ctl;
     pthread_create(&th[i], NULL, run_job, &jobs[i]);
   }
   for (i = 0; i < nthreads; i++) pthread_join(th[i], NULL);
@@ -506,7 +594,7 @@ void otr_refine(const otr_program *P, int K, int npaths, double t, const double 
     int nr = 1, its = 0;
     load_point(P, &w, xin, KS, npaths, path);
     set_weights(P, &w, t);
-    if (iters > 0) nr = newton(P, &w, iters, tol, &dxl, &its);
+    if (iters > 0) nr = newton(P, &w, iters, tol, &dxl, &its, NULL);
     eval_point(P, &w, 0);
     res[path] = max_abs(K, w.rhs, P->n);
     store_point(P, &w, xout, npaths, path);

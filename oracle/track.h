@@ -29,15 +29,33 @@ typedef struct {
   double maxstep, minstep, pbeta, alpha, tolcor, tolend, divergence;
 } otr_params;
 
-/* paths p0 <= p < p1 of npaths; arrays as in phc_track */
+/* step-size controller, layout of phc_controller (pointers are host memory) */
+#define OTR_NFEAT 16
+#define OTR_HIDDEN 64
+typedef struct {
+  int mode, ncand;
+  double mult[8];
+  const float *weights, *mu, *sd;
+  float logit_threshold;
+  unsigned seed;
+  int log_capacity;
+  float *log_x;
+  int *log_c, *log_y, *log_n;
+} otr_controller;
+
+/* paths p0 <= p < p1 of npaths; arrays as in phc_track; ctl may be NULL */
 void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const double *start,
                int KS, double *endp, double *tend, int *status, int *nsteps, int *nrej,
-               double *dxlast, int *work, const otr_params *prm);
+               double *dxlast, int *work, const otr_params *prm, const otr_controller *ctl);
 
 /* same as otr_track over all paths with nthreads POSIX threads */
 void otr_track_mt(const otr_program *P, int K, int npaths, const double *start, int KS,
                   double *endp, double *tend, int *status, int *nsteps, int *nrej,
-                  double *dxlast, int *work, const otr_params *prm, int nthreads);
+                  double *dxlast, int *work, const otr_params *prm, const otr_controller *ctl,
+                  int nthreads);
+
+/* log2 approximation shared with the device (exposed for tests) */
+double otr_lg2(double x);
 
 void otr_refine(const otr_program *P, int K, int npaths, double t, const double *xin, int KS,
                 double *xout, int iters, double tol, double *dxlast, double *res, int *status);

@@ -69,6 +69,15 @@ class PhcTrackParams(ctypes.Structure):
 
 
 
+class PhcController(ctypes.Structure):
+    _fields_ = [
+        ("mode", c_int), ("ncand", c_int), ("mult", c_double * 8),
+        ("weights", c_void_p), ("mu", c_void_p), ("sd", c_void_p),
+        ("logit_threshold", ctypes.c_float), ("seed", ctypes.c_uint), ("log_capacity", c_int),
+        ("log_x", c_void_p), ("log_c", c_void_p), ("log_y", c_void_p), ("log_n", c_void_p),
+    ]
+
+
 SIGNATURES.update({
     "phc_tracker_create": (c_int, [ctypes.POINTER(c_void_p), ctypes.POINTER(PhcProgram)]),
     "phc_tracker_destroy": (c_int, [c_void_p]),
@@ -77,7 +86,14 @@ SIGNATURES.update({
                                       ctypes.POINTER(ctypes.c_size_t)]),
     "phc_track": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                           c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(PhcTrackParams),
-                          c_int, c_void_p]),
+                          ctypes.POINTER(PhcController), c_int, c_void_p]),
+    "phc_mlp_size": (c_int, [c_int, c_int, c_int, P_int]),
+    "phc_mlp_forward": (c_int, [c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "phc_mlp_grad": (c_int, [c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                             c_void_p, c_void_p, c_void_p]),
+    "phc_mlp_train": (c_int, [c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                              c_int, c_int, ctypes.c_float, ctypes.c_uint,
+                              ctypes.POINTER(ctypes.c_float), c_void_p]),
     "phc_refine": (c_int, [c_void_p, c_int, c_int, c_double, c_void_p, c_int, c_void_p, c_int,
                            c_double, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
 })

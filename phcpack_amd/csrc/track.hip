@@ -69,6 +69,40 @@ struct Params {
   double maxstep, minstep, pbeta, alpha, tolcor, tolend, divergence;
 };
 
+// step-size controller (device view of phc_controller)
+struct Ctl {
+  int mode, ncand, cap;
+  double mult[PHC_CTL_MAXCAND];
+  const float *w, *mu, *sd;
+  float thresh;
+  unsigned seed;
+  float *log_x;
+  int *log_c, *log_y, *log_n;
+};
+
+// log2 of x > 0 from the binary exponent and the series
+// ln((1+z)/(1-z)) = 2(z + z^3/3 + z^5/5 + z^7/7), z = (m-1)/(m+1), m in [1,2);
+// only IEEE double operations, so host and device agree bit for bit
+__host__ __device__ inline double lg2(double x) {
+  if (!(x > 0.0)) return -1100.0;
+  if (x == INFINITY) return 1100.0;
+  int e = 0;
+  const double m = 2.0 * frexp(x, &e);  // x = m * 2^(e-1), m in [1,2)
+  const double z = (m - 1.0) / (m + 1.0), z2 = z * z;
+  const double s = z * (1.0 + z2 * (1.0 / 3.0 + z2 * (1.0 / 5.0 + z2 * (1.0 / 7.0))));
+  return (double)(e - 1) + s * 2.8853900817779268;  // 2/ln 2
+}
+
+__host__ __device__ inline unsigned mix_hash(unsigned seed, unsigned path, unsigned step) {
+  unsigned h = seed ^ (path * 0x9E3779B1u) ^ (step * 0x85EBCA77u);
+  h ^= h >> 16;
+  h *= 0x7FEB352Du;
+  h ^= h >> 15;
+  h *= 0x846CA68Bu;
+  h ^= h >> 16;
+  return h;
+}
+
 __host__ __device__ inline size_t al16(size_t b) { return (b + 15) & ~(size_t)15; }
 
 // byte offsets of the program sections in LDS
@@ -91,7 +125,7 @@ struct ProgLayout {
 
 // per-wavefront path region (bytes)
 template <int K> struct PathLayout {
-  size_t slots, u, uj, A, rhs, x0, col, W, dnode, perm, total;
+  size_t slots, u, uj, A, rhs, x0, col, W, dnode, perm, hid, total;
   __host__ __device__ explicit PathLayout(const DevProgram &P) {
     const size_t z = sizeof(cplx<K>);
     size_t o = 0;
@@ -105,6 +139,7 @@ template <int K> struct PathLayout {
     W = o; o += z * 3 * P.ncomp;
     dnode = o; o += z * (P.nfactor > 0 ? P.nfactor : 1);
     perm = o; o = al16(o + sizeof(int) * P.n);
+    hid = o; o += sizeof(float) * PHC_CTL_HIDDEN;
     total = al16(o);
   }
 };
@@ -120,6 +155,7 @@ struct LProg {
 template <int K> struct Path {
   cplx<K> *slots, *u, *uj, *A, *rhs, *x0, *col, *W, *dnode;
   int *perm;
+  float *hid;
 };
 
 __device__ __forceinline__ void wsync() {
@@ -374,7 +410,8 @@ template <int K> __device__ double max_abs_x(const LProg &P, Path<K> &w) {
 // Newton at fixed t on the order-0 slots (weights already set): returns
 // 1 converged, 0 not converged, -1 singular; *dxl = last max |dx|
 template <int K>
-__device__ int newton(const LProg &P, Path<K> &w, int iters, double tol, double *dxl, int *count) {
+__device__ int newton(const LProg &P, Path<K> &w, int iters, double tol, double *dxl, int *count,
+                      double *dxh) {
   typedef C<K> Cx;
   const int l = lane_id();
   for (int it = 0; it < iters; ++it) {
@@ -391,6 +428,7 @@ __device__ int newton(const LProg &P, Path<K> &w, int iters, double tol, double 
     wsync();
     mdx = wave_max(mdx);
     *dxl = mdx;
+    if (dxh) { dxh[3] = dxh[2]; dxh[2] = dxh[1]; dxh[1] = dxh[0]; dxh[0] = mdx; }
     if (mdx <= tol * (1.0 + max_abs_x<K>(P, w))) return 1;
   }
   return 0;
@@ -445,6 +483,7 @@ template <int K> __device__ Path<K> carve(const PathLayout<K> &L, unsigned char 
   w.W = (cplx<K> *)(base + L.W);
   w.dnode = (cplx<K> *)(base + L.dnode);
   w.perm = (int *)(base + L.perm);
+  w.hid = (float *)(base + L.hid);
   return w;
 }
 
@@ -499,12 +538,54 @@ template <int K> __device__ bool predictor_series(const LProg &P, Path<K> &w) {
   return true;
 }
 
+// features of the current step (identical in every lane)
+template <int K>
+__device__ void step_features(const LProg &P, Path<K> &w, double rad, double maxstep, double hprev,
+                              double t, const double *dxh, int rejprev, double *f) {
+  const int l = lane_id(), n = P.n, D = P.D;
+  f[0] = lg2((rad < INFINITY && rad > 0.0) ? rad : maxstep);
+  f[1] = lg2(hprev);
+  f[2] = t;
+  for (int k = 1; k <= 8; ++k) {
+    double c = 0.0;
+    if (k < D && l < n) c = absz<K>(w.slots[l * D + k]);
+    f[2 + k] = lg2(wave_max(c) + 1e-300);
+  }
+  for (int i = 0; i < 4; ++i) f[11 + i] = lg2(dxh[i] + 1e-300);
+  f[15] = (double)rejprev;
+}
+
+// candidate chosen by the MLP: the largest c with logit >= threshold
+template <int K> __device__ int mlp_choose(const Ctl &ctl, Path<K> &w, const double *f) {
+  const int j = lane_id();  // hidden unit j
+  constexpr int F = PHC_CTL_NFEAT, H = PHC_CTL_HIDDEN;
+  const float *W1 = ctl.w, *b1 = W1 + F * H, *W2 = b1 + H, *b2 = W2 + H * H, *W3 = b2 + H;
+  const float *b3 = W3 + H * ctl.ncand;
+  float h = b1[j];
+  for (int i = 0; i < F; ++i) h += W1[i * H + j] * (((float)f[i] - ctl.mu[i]) / ctl.sd[i]);
+  h = h < 0.0f ? 0.0f : h;
+  w.hid[j] = h;
+  wsync();
+  float g = b2[j];
+  for (int i = 0; i < H; ++i) g += W2[i * H + j] * w.hid[i];
+  g = g < 0.0f ? 0.0f : g;
+  wsync();
+  int best = 0;
+  for (int c = 0; c < ctl.ncand; ++c) {
+    float v = W3[j * ctl.ncand + c] * g;
+    for (int m = 1; m < 64; m <<= 1) v += __shfl_xor(v, m, 64);
+    if (v + b3[c] >= ctl.thresh) best = c;
+  }
+  return best;
+}
+
 template <int K>
 __global__ void __launch_bounds__(512) track_kernel(DevProgram G, Params par, int npaths,
                                                    const double *__restrict__ start, int KS,
                                                    double *__restrict__ endp, double *tend,
                                                    int *status, int *nsteps_out, int *nrej_out,
-                                                   double *dx_out, int *work, int *counter) {
+                                                   double *dx_out, int *work, Ctl ctl,
+                                                   int *counter) {
   typedef C<K> Cx;
   extern __shared__ __align__(16) unsigned char lds[];
   const ProgLayout PL(G);
@@ -520,8 +601,10 @@ __global__ void __launch_bounds__(512) track_kernel(DevProgram G, Params par, in
     path = __shfl(path, 0, 64);
     if (path >= npaths) break;
     load_point<K>(P, w, start, KS, npaths, path);
-    double t = 0.0, h = par.maxstep, dxl = 0.0;
-    int st = ST_RUNNING, steps = 0, rej = 0, npred = 0, nnewt = 0, nfeval = 0;
+    double t = 0.0, h = par.maxstep, dxl = 0.0, hprev = par.maxstep;
+    double dxh[4] = {1.0, 1.0, 1.0, 1.0};
+    int st = ST_RUNNING, steps = 0, rej = 0, npred = 0, nnewt = 0, nfeval = 0, rejprev = 0;
+    int nlog = 0;
     while (st == ST_RUNNING && t < 1.0) {
       // ---- predictor: Taylor series of x(t+s) ----
       set_weights<K>(P, w, t);
@@ -536,6 +619,16 @@ __global__ void __launch_bounds__(512) track_kernel(DevProgram G, Params par, in
       }
       r = wave_min(r);
       h = (r < INFINITY && r > 0.0) ? fmin(par.pbeta * r, par.maxstep) : par.maxstep;
+      // ---- step-size controller ----
+      int cand = 0;
+      double feat[PHC_CTL_NFEAT];
+      if (ctl.mode != 0) {
+        step_features<K>(P, w, r, par.maxstep, hprev, t, dxh, rejprev, feat);
+        if (ctl.mode == 1) cand = mlp_choose<K>(ctl, w, feat);
+        else cand = (int)(mix_hash(ctl.seed, (unsigned)path, (unsigned)(npred - 1)) % (unsigned)ctl.ncand);
+        h *= ctl.mult[cand];
+      }
+      const int rej0 = rej;
       if (l < n) w.x0[l] = w.slots[l * D];
       wsync();
       // ---- feedback and corrector; halve the step on failure ----
@@ -556,16 +649,27 @@ __global__ void __launch_bounds__(512) track_kernel(DevProgram G, Params par, in
         const double res = max_abs_rhs<K>(P, w);
         bool ok = res <= par.alpha * (1.0 + max_abs_x<K>(P, w));
         if (ok) {
-          const int nr = newton<K>(P, w, par.corsteps, par.tolcor, &dxl, &nnewt);
+          const int nr = newton<K>(P, w, par.corsteps, par.tolcor, &dxl, &nnewt, dxh);
           if (nr < 0) { st = ST_SINGULAR; break; }
           ok = (nr == 1);
         }
-        if (ok) { t = tn; accepted = true; }
+        if (ok) { t = tn; accepted = true; hprev = hs; }
         else {
           ++rej;
           h *= 0.5;
           if (h < par.minstep) { st = ST_MINSTEP; break; }
         }
+      }
+      rejprev = rej - rej0;
+      if (ctl.mode == 2 && nlog < ctl.cap) {
+        if (l == 0) {
+#pragma unroll
+          for (int i = 0; i < PHC_CTL_NFEAT; ++i)
+            ctl.log_x[((size_t)path * ctl.cap + nlog) * PHC_CTL_NFEAT + i] = (float)feat[i];
+          ctl.log_c[(size_t)path * ctl.cap + nlog] = cand;
+          ctl.log_y[(size_t)path * ctl.cap + nlog] = (rej == rej0 && st == ST_RUNNING) ? 1 : 0;
+        }
+        ++nlog;
       }
       if (st != ST_RUNNING) break;
       ++steps;
@@ -574,7 +678,7 @@ __global__ void __launch_bounds__(512) track_kernel(DevProgram G, Params par, in
     }
     if (st == ST_RUNNING) {
       set_weights<K>(P, w, 1.0);
-      const int nr = newton<K>(P, w, par.endsteps, par.tolend, &dxl, &nnewt);
+      const int nr = newton<K>(P, w, par.endsteps, par.tolend, &dxl, &nnewt, nullptr);
       st = (nr < 0) ? ST_SINGULAR : ST_SUCCESS;
     }
     store_point<K>(P, w, endp, npaths, path);
@@ -589,6 +693,7 @@ __global__ void __launch_bounds__(512) track_kernel(DevProgram G, Params par, in
         work[npaths + path] = nnewt;
         work[2 * npaths + path] = nfeval;
       }
+      if (ctl.mode == 2) ctl.log_n[path] = nlog;
     }
     wsync();
   }
@@ -620,7 +725,7 @@ __global__ void __launch_bounds__(512) refine_kernel(DevProgram G, int npaths, d
     set_weights<K>(P, w, tval);
     double dxl = 0.0;
     int nr = 1, its = 0;
-    if (iters > 0) nr = newton<K>(P, w, iters, tol, &dxl, &its);
+    if (iters > 0) nr = newton<K>(P, w, iters, tol, &dxl, &its, nullptr);
     eval_point<K>(P, w, false);
     const double res = max_abs_rhs<K>(P, w);
     store_point<K>(P, w, xout, npaths, path);
@@ -818,7 +923,7 @@ int phc_tracker_lds_bytes(const phc_tracker *tr, int K, size_t *program_bytes, s
 
 int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, double *endp,
               double *tend, int *status, int *nsteps, int *nrej, double *dxlast, int *work,
-              const phc_track_params *prm, int waves, void *stream) {
+              const phc_track_params *prm, const phc_controller *ctlp, int waves, void *stream) {
   PHC_REQUIRE(tr && prm, "phc_track: null argument");
   PHC_REQUIRE(K == 1 || K == 2, "phc_track: tracking precision K must be 1 or 2");
   PHC_REQUIRE(KS >= 1 && KS <= 8, "phc_track: start limbs KS must be in 1..8");
@@ -839,15 +944,36 @@ int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, d
   par.fabry_offset = prm->fabry_offset; par.maxstep = prm->maxstep; par.minstep = prm->minstep;
   par.pbeta = prm->pbeta; par.alpha = prm->alpha; par.tolcor = prm->tolcor;
   par.tolend = prm->tolend; par.divergence = prm->divergence;
+  Ctl ctl;
+  ctl.mode = 0; ctl.ncand = 1; ctl.cap = 0; ctl.w = ctl.mu = ctl.sd = nullptr; ctl.thresh = 0.0f;
+  ctl.seed = 0; ctl.log_x = nullptr; ctl.log_c = ctl.log_y = ctl.log_n = nullptr;
+  for (int c = 0; c < PHC_CTL_MAXCAND; ++c) ctl.mult[c] = 1.0;
+  if (ctlp && ctlp->mode != 0) {
+    PHC_REQUIRE(ctlp->mode == 1 || ctlp->mode == 2, "phc_track: controller mode must be 0, 1 or 2");
+    PHC_REQUIRE(ctlp->ncand >= 1 && ctlp->ncand <= PHC_CTL_MAXCAND, "phc_track: bad ncand");
+    PHC_REQUIRE(tr->P.D >= 9, "phc_track: the controller features need series degree >= 8");
+    for (int c = 0; c < ctlp->ncand; ++c)
+      PHC_REQUIRE(ctlp->mult[c] > 0.0, "phc_track: candidate multipliers must be positive");
+    PHC_REQUIRE(ctlp->mode != 1 || (ctlp->weights && ctlp->mu && ctlp->sd),
+                "phc_track: learned mode needs weights, mu and sd");
+    PHC_REQUIRE(ctlp->mode != 2 || (ctlp->log_capacity >= 1 && ctlp->log_x && ctlp->log_c &&
+                                    ctlp->log_y && ctlp->log_n),
+                "phc_track: explore mode needs the log buffers");
+    ctl.mode = ctlp->mode; ctl.ncand = ctlp->ncand; ctl.cap = ctlp->log_capacity;
+    for (int c = 0; c < ctlp->ncand; ++c) ctl.mult[c] = ctlp->mult[c];
+    ctl.w = ctlp->weights; ctl.mu = ctlp->mu; ctl.sd = ctlp->sd; ctl.thresh = ctlp->logit_threshold;
+    ctl.seed = ctlp->seed; ctl.log_x = ctlp->log_x; ctl.log_c = ctlp->log_c;
+    ctl.log_y = ctlp->log_y; ctl.log_n = ctlp->log_n;
+  }
   PHC_HIP(hipMemsetAsync(tr->counter, 0, sizeof(int), st));
   const void *fn = (K == 1) ? (const void *)track_kernel<1> : (const void *)track_kernel<2>;
   PHC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   if (K == 1)
     hipLaunchKernelGGL(track_kernel<1>, dim3(grid), dim3(64 * wv), lds, st, tr->P, par, npaths,
-                       start, KS, endp, tend, status, nsteps, nrej, dxlast, work, tr->counter);
+                       start, KS, endp, tend, status, nsteps, nrej, dxlast, work, ctl, tr->counter);
   else
     hipLaunchKernelGGL(track_kernel<2>, dim3(grid), dim3(64 * wv), lds, st, tr->P, par, npaths,
-                       start, KS, endp, tend, status, nsteps, nrej, dxlast, work, tr->counter);
+                       start, KS, endp, tend, status, nsteps, nrej, dxlast, work, ctl, tr->counter);
   PHC_HIP(hipGetLastError());
   return 0;
 }

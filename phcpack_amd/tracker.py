@@ -139,20 +139,23 @@ class Tracker:
         self.dev = DeviceProgram(self.prog)
         self.dev0 = DeviceProgram(self.prog0)
 
-    def track_device(self, d_start, KS, npaths, K, bufs, params=None, waves=8, stream=None):
+    def track_device(self, d_start, KS, npaths, K, bufs, params=None, waves=8, stream=None,
+                     controller=None):
+        """controller: a PhcController (or None for the plain pole-step rule)"""
         p = (params or TrackParams()).to_c()
+        cptr = ctypes.byref(controller) if controller is not None else None
         check(load().phc_track(self.dev.handle, K, npaths, d_start.ptr, KS, bufs.endp.ptr,
                                bufs.tend.ptr, bufs.status.ptr, bufs.nsteps.ptr, bufs.nrej.ptr,
-                               bufs.dx.ptr, bufs.work.ptr, ctypes.byref(p), waves, stream))
+                               bufs.dx.ptr, bufs.work.ptr, ctypes.byref(p), cptr, waves, stream))
 
-    def track(self, starts, K=2, params=None, waves=8):
+    def track(self, starts, K=2, params=None, waves=8, controller=None):
         """starts: complex128 [npaths][n] (or [2KS][npaths][n] planes)."""
         planes = starts if starts.dtype == np.float64 else points_to_planes(starts)
         KS = planes.shape[0] // 2
         npaths = planes.shape[1]
         d_start = DeviceArray.from_host(planes)
         bufs = TrackBuffers(npaths, self.n, K)
-        self.track_device(d_start, KS, npaths, K, bufs, params, waves)
+        self.track_device(d_start, KS, npaths, K, bufs, params, waves, controller=controller)
         check(load().phc_synchronize())
         return bufs.to_host()
 

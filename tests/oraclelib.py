@@ -81,12 +81,16 @@ def ulp_err(a, b, K):
 
 def _track_sigs():
     import ctypes as C
-    from phcpack_amd._lib import PhcProgram, PhcTrackParams
+    from phcpack_amd._lib import PhcController, PhcProgram, PhcTrackParams
     o = oracle()
     o.otr_track.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_int, C.c_int, C.c_int, P_d, C.c_int,
-                            P_d, P_d, P_i, P_i, P_i, P_d, P_i, C.POINTER(PhcTrackParams)]
+                            P_d, P_d, P_i, P_i, P_i, P_d, P_i, C.POINTER(PhcTrackParams),
+                            C.POINTER(PhcController)]
     o.otr_track_mt.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_int, P_d, C.c_int, P_d, P_d,
-                               P_i, P_i, P_i, P_d, P_i, C.POINTER(PhcTrackParams), C.c_int]
+                               P_i, P_i, P_i, P_d, P_i, C.POINTER(PhcTrackParams),
+                               C.POINTER(PhcController), C.c_int]
+    o.otr_lg2.argtypes = [C.c_double]
+    o.otr_lg2.restype = C.c_double
     o.otr_refine.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_int, C.c_double, P_d, C.c_int,
                              P_d, C.c_int, C.c_double, P_d, P_d, P_i]
     o.otr_eval.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_double, P_d, P_d, P_d]
@@ -95,8 +99,43 @@ def _track_sigs():
     return o
 
 
-def oracle_track(prog, starts_planes, K, params, nthreads=1, paths=None):
-    """Track with the C oracle; starts_planes [2KS][npaths][n]."""
+class OracleController:
+    """host-memory controller for the oracle (same layout as phc_controller);
+    mode 1 needs a StepController, mode 2 allocates the step logs"""
+
+    def __init__(self, mode, npaths, ctl=None, seed=0, cap=256):
+        import ctypes as C
+        from phcpack_amd._lib import PhcController
+        from phcpack_amd.controller import NFEAT, StepController
+        ctl = ctl or StepController()
+        self.keep = []
+        c = PhcController()
+        c.mode, c.ncand = mode, ctl.C
+        for i, m in enumerate(ctl.mults):
+            c.mult[i] = m
+        for name, arr in (("weights", ctl.w), ("mu", ctl.mu), ("sd", ctl.sd)):
+            a = np.ascontiguousarray(arr, np.float32)
+            self.keep.append(a)
+            setattr(c, name, a.ctypes.data)
+        c.logit_threshold = ctl.logit_threshold
+        c.seed = seed
+        self.cap = cap
+        self.x = np.zeros((npaths, cap, NFEAT), np.float32)
+        self.c = np.zeros((npaths, cap), np.int32)
+        self.y = np.zeros((npaths, cap), np.int32)
+        self.n = np.zeros(npaths, np.int32)
+        c.log_capacity = cap
+        c.log_x, c.log_c, c.log_y, c.log_n = (a.ctypes.data for a in (self.x, self.c, self.y, self.n))
+        self.struct = c
+
+    def logs(self):
+        keep = np.arange(self.cap)[None, :] < self.n[:, None]
+        return self.x[keep], self.c[keep], self.y[keep]
+
+
+def oracle_track(prog, starts_planes, K, params, nthreads=1, paths=None, controller=None):
+    """Track with the C oracle; starts_planes [2KS][npaths][n];
+    controller: an OracleController or None."""
     import ctypes as C
     from phcpack_amd.tracker import program_struct
     o = _track_sigs()
@@ -108,15 +147,16 @@ def oracle_track(prog, starts_planes, K, params, nthreads=1, paths=None):
                nrej=np.zeros(npaths, np.int32), dx=np.zeros(npaths),
                work=np.zeros((3, npaths), np.int32))
     p = params.to_c()
+    cp = C.byref(controller.struct) if controller is not None else None
     if nthreads > 1:
         o.otr_track_mt(C.byref(ps), K, npaths, pd(st), KS, pd(out["endp"]), pd(out["tend"]),
                        pi(out["status"]), pi(out["nsteps"]), pi(out["nrej"]), pd(out["dx"]),
-                       pi(out["work"]), C.byref(p), nthreads)
+                       pi(out["work"]), C.byref(p), cp, nthreads)
     else:
         p0, p1 = (0, npaths) if paths is None else paths
         o.otr_track(C.byref(ps), K, npaths, p0, p1, pd(st), KS, pd(out["endp"]), pd(out["tend"]),
                     pi(out["status"]), pi(out["nsteps"]), pi(out["nrej"]), pd(out["dx"]),
-                    pi(out["work"]), C.byref(p))
+                    pi(out["work"]), C.byref(p), cp)
     return out
 
 

@@ -1,0 +1,87 @@
+"""GPU parity of the controller: MLP forward/backward kernels against the
+float64 numpy reference, training, and tracking in explore and learned
+mode bit-identical to the oracle."""
+import numpy as np
+import pytest
+
+from oraclelib import OracleController, oracle_track
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mlp_forward_and_grad_match_numpy():
+    from phcpack_amd.controller import MULTS, NFEAT, forward_np, grad_np, init_weights, mlp_forward, mlp_grad
+    rng = np.random.default_rng(0)
+    w = init_weights(rng)
+    B = 301
+    X = rng.standard_normal((B, NFEAT)).astype(np.float32)
+    cand = rng.integers(0, len(MULTS), B).astype(np.int32)
+    y = rng.integers(0, 2, B).astype(np.float32)
+    O = mlp_forward(w, X)
+    O_ref, _ = forward_np(w, X.astype(np.float64))
+    assert np.allclose(O, O_ref, rtol=1e-4, atol=1e-4)
+    g, loss = mlp_grad(w, X, cand, y)
+    g_ref, loss_ref = grad_np(w, X.astype(np.float64), cand, y.astype(np.float64))
+    assert abs(loss - loss_ref) < 1e-4 * max(1.0, abs(loss_ref))
+    assert np.allclose(g, g_ref, rtol=1e-3, atol=1e-5 * np.abs(g_ref).max())
+
+
+def test_mlp_training_reduces_loss():
+    from phcpack_amd.controller import MULTS, NFEAT, StepController
+    rng = np.random.default_rng(1)
+    N = 4096
+    X = rng.standard_normal((N, NFEAT)).astype(np.float32)
+    cand = rng.integers(0, len(MULTS), N)
+    # accepted when the candidate index is below a feature-dependent level
+    y = (cand < 3 + 2 * np.tanh(X[:, 0])).astype(np.float32)
+    ctl = StepController(seed=0)
+    losses = ctl.fit(X, cand, y, epochs=20, batch=512, lr=3e-3)
+    assert losses[-1] < 0.6 * losses[0]
+
+
+def _system(n, degree, seed):
+    from phcpack_amd.homotopy import dense_system, total_degree_homotopy
+    rng = np.random.default_rng(seed)
+    p = dense_system(n, degree, rng)
+    hom, sols = total_degree_homotopy(p, rng)
+    return hom, sols
+
+
+@pytest.mark.parametrize("K", [1, 2])
+def test_explore_mode_matches_oracle(K):
+    from phcpack_amd.controller import StepController, StepLogs
+    from phcpack_amd.tracker import Tracker, TrackParams, TrackBuffers, points_to_planes
+    from phcpack_amd._lib import DeviceArray, check, load
+    hom, sols = _system(3, 2, 9)
+    tr = Tracker(hom, deg=8)
+    prm = TrackParams(tolend=1e-13 if K == 1 else 1e-28)
+    ctl = StepController()
+    logs = StepLogs(len(sols), 128)
+    bufs = TrackBuffers(len(sols), 3, K)
+    d = DeviceArray.from_host(points_to_planes(sols))
+    tr.track_device(d, 1, len(sols), K, bufs, prm, controller=ctl.struct(2, logs, seed=11))
+    check(load().phc_synchronize())
+    g = bufs.to_host()
+    oc = OracleController(2, len(sols), ctl=ctl, seed=11, cap=128)
+    o = oracle_track(tr.prog, points_to_planes(sols), K, prm, controller=oc)
+    for k in ("endp", "status", "nsteps", "nrej", "work"):
+        assert np.array_equal(g[k], o[k]), k
+    gx, gc, gy = logs.to_host()
+    ox, oc_, oy = oc.logs()
+    assert np.array_equal(gx, ox) and np.array_equal(gc, oc_) and np.array_equal(gy, oy)
+
+
+@pytest.mark.parametrize("K", [1, 2])
+def test_learned_mode_matches_oracle(K):
+    from phcpack_amd.controller import StepController
+    from phcpack_amd.tracker import Tracker, TrackParams, points_to_planes
+    hom, sols = _system(4, 2, 10)
+    tr = Tracker(hom, deg=8)
+    prm = TrackParams(tolend=1e-13 if K == 1 else 1e-28)
+    ctl = StepController(seed=3, threshold=0.5)
+    g = tr.track(sols, K=K, params=prm, controller=ctl.struct(1))
+    oc = OracleController(1, len(sols), ctl=ctl)
+    o = oracle_track(tr.prog, points_to_planes(sols), K, prm, controller=oc)
+    for k in ("endp", "status", "nsteps", "nrej", "work"):
+        assert np.array_equal(g[k], o[k]), k
+    assert np.all(g["status"] == 1)
