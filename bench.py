@@ -2,13 +2,16 @@
 """Throughput of the batched path tracker on one MI355X.
 
 Metric: paths/s whose endpoint residual max_i |p_i(x)| is <= 1e-24, for the
-mixed-precision tracker (FP64 tracking, double-double Newton at the
-endpoints, quad-double Newton for endpoints that do not converge in
-double-double), beside the plain double-double tracker on the same system.
-The residual of every endpoint is evaluated at the exact endpoint in
-quad-double arithmetic on the GPU (rounding ~1e-60, far below the 1e-24
-threshold); octo-double residuals of sampled endpoints are checked against
-the CPU oracle by the tests.
+learned + mixed-precision tracker (FP64 tracking with the learned step
+controller, double-double Newton at the endpoints, quad-double Newton for
+endpoints that do not converge in double-double), beside the plain
+double-double tracker and the plain-rule mixed tracker on the same system.
+The controller is trained before the timed region on explore-mode step logs
+of two other systems of the same family (seeds 20261203, 20261204) and its
+acceptance threshold is tuned on a third (20261205); the evaluation system
+is never seen in training.  The residual of every endpoint is evaluated at
+the exact endpoint in quad-double arithmetic on the GPU (rounding ~1e-60,
+far below the 1e-24 threshold).
 
 Workload: total-degree homotopy gamma*(1-t)*q + t*p with p = 12 dense
 quadratics in 12 variables (all 91 monomials of degree <= 2, complex
@@ -16,7 +19,7 @@ coefficients on the unit circle), q_i = x_i^2 - c_i, 4096 paths, predictor
 series of degree 8, seed 20261202.  One step = tracking all 4096 paths from
 t = 0 to t = 1 plus the endpoint refinement, inputs resident on the device.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-learned]
 """
 import argparse
 import ctypes
@@ -34,6 +37,8 @@ METRIC = "paths/sec at ≤1e-24 endpoint residual, learned+mixed vs plain dd on 
 SEED = 20261202
 RES_TOL = 1e-24
 FP64_PEAK_TFLOPS = 78.6
+TRAIN_SEEDS = (SEED + 1, SEED + 2)   # controller training systems (same family)
+TUNE_SEED = SEED + 3                 # threshold tuning system
 
 
 class Events:
@@ -69,32 +74,49 @@ def run_plain(tr, d_start, npaths, bufs, ev, prm):
     return ev.stop_ms()
 
 
-def run_mixed(tr, d_start, npaths, bufs, ref, ev, prm, lib):
-    """FP64 tracking, then dd Newton; paths whose dd Newton does not
-    converge are refined in quad-double."""
+def run_mixed(tr, d_start, starts, npaths, bufs, ref, ev, prm, lib, controller=None):
+    """FP64 tracking (plain pole-step rule, or the learned controller when
+    given), then dd Newton at the endpoints.  Paths whose dd Newton does not
+    converge are refined in quad-double; with the learned controller, paths
+    that still fail are tracked again with the plain rule (a path the
+    controller lost is recovered, not dropped).  Returns (tracking ms,
+    endpoint ms, escalated, re-tracked)."""
     from phcpack_amd._lib import check
     ev.start()
-    tr.track_device(d_start, 1, npaths, 1, bufs, prm)
+    tr.track_device(d_start, 1, npaths, 1, bufs, prm, controller=controller)
     t_track = ev.stop_ms()
     ev.start()
     tr.refine_device(bufs.endp, 1, npaths, 2, ref["x2"], ref["dx"], ref["res"], ref["st"],
                      iters=3, tol=1e-28)
     t_ref = ev.stop_ms()
+    t0 = time.perf_counter()
     st = ref["st"].to_host()
     bad = np.nonzero(st != 1)[0]
-    escal = 0
+    fix = {}
+    nres = 0
     if len(bad):
         # escalate the unconverged paths to quad-double
-        x2 = ref["x2"].to_host()[:, bad]
-        t0 = time.perf_counter()
-        r4 = tr.refine(x2, 4, iters=4, tol=1e-56)
+        r4 = tr.refine(ref["x2"].to_host()[:, bad], 4, iters=4, tol=1e-56)
+        ok4 = r4["status"] == 1
+        for j, p in enumerate(bad):
+            fix[int(p)] = (r4["x"][:, j], bool(ok4[j]))
+        lost = bad[~ok4]
+        if controller is not None and len(lost):
+            o = tr.track(starts[:, lost], K=1, params=prm)
+            r2 = tr.refine(o["endp"], 2, iters=3, tol=1e-28)
+            x4 = to_qd(r2["x"], 2)
+            ok = (o["status"] == 1) & (r2["status"] == 1)
+            if not ok.all():
+                r4b = tr.refine(x4[:, ~ok], 4, iters=4, tol=1e-56)
+                x4[:, ~ok] = r4b["x"]
+                ok[~ok] = (o["status"][~ok] == 1) & (r4b["status"] == 1)
+            for j, p in enumerate(lost):
+                fix[int(p)] = (x4[:, j], bool(ok[j]))
+            nres = len(lost)
         check(lib.phc_synchronize())
-        escal = len(bad)
-        t_ref += (time.perf_counter() - t0) * 1e3
-        ref["escalated"] = (bad, r4["x"])
-    else:
-        ref["escalated"] = None
-    return t_track, t_ref, escal
+    t_ref += (time.perf_counter() - t0) * 1e3
+    ref["fix"] = fix
+    return t_track, t_ref, len(bad), nres
 
 
 def qd_residuals(tr, x_planes):
@@ -176,6 +198,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip controller training; the headline is then the plain-rule mixed tracker")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also launch one 256 MiB device copy kernel (HBM counter scale)")
@@ -188,84 +212,152 @@ def main():
     return run(args)
 
 
+def train_learned(n, prm, ev):
+    """Train the step controller on systems of the same family with other
+    seeds (explore-mode logs of FP64 tracking), then tune its threshold on
+    a further system.  Not part of the timed region."""
+    from phcpack_amd.controller import train_controller, tune_threshold
+    from phcpack_amd.tracker import Tracker
+    t0 = time.perf_counter()
+    trs, sts = [], []
+    for s in TRAIN_SEEDS:
+        _, hom, sols = build_workload(n, s)
+        trs.append(Tracker(hom, deg=8))
+        sts.append(sols)
+    ctl, info = train_controller(trs, sts, prm, K=1, seed=7, epochs=25)
+    _, hom, sols = build_workload(n, TUNE_SEED)
+    tt = Tracker(hom, deg=8)
+
+    def timer(fn):
+        ev.start()
+        fn()
+        return ev.stop_ms()
+
+    best, table = tune_threshold(ctl, tt, sols, prm, K=1, timer=timer)
+    info.update(train_seeds=list(TRAIN_SEEDS), tune_seed=TUNE_SEED, threshold=best,
+                tune_table={str(k): v for k, v in table.items()},
+                seconds=round(time.perf_counter() - t0, 2))
+    return (ctl if best is not None else None), info
+
+
+def verify(tr, out, ref, K_end):
+    """endpoints (qd planes), qd residuals and success mask of one method"""
+    ok_track = out["status"] == 1
+    if ref is None:
+        x = to_qd(out["endp"], K_end)
+    else:
+        x = to_qd(ref["x2"].to_host(), 2)
+        for p, (xq, okp) in ref["fix"].items():
+            x[:, p] = xq
+            ok_track[p] = okp
+    res = qd_residuals(tr, x)
+    ok = ok_track & (res <= RES_TOL)
+    return x, res, ok
+
+
 def run(args):
     from phcpack_amd._lib import DeviceArray, check, load
-    from phcpack_amd.flops import tracker_flops, refine_flops
+    from phcpack_amd.flops import tracker_flops
     from phcpack_amd.tracker import TrackBuffers, Tracker, TrackParams, planes_to_points, points_to_planes
     lib = load()
     p, hom, sols = build_workload()
     npaths, n = len(sols), hom.n
     tr = Tracker(hom, deg=8)
-    d_start = DeviceArray.from_host(points_to_planes(sols))
+    h_start = points_to_planes(sols)
+    d_start = DeviceArray.from_host(h_start)
     ev = Events(lib)
     prm_plain = TrackParams(tolend=1e-28)
     prm_mixed = TrackParams(tolend=1e-13)
     b_plain = TrackBuffers(npaths, n, 2)
     b_mixed = TrackBuffers(npaths, n, 1)
-    ref = dict(x2=DeviceArray((4, npaths, n)), dx=DeviceArray((npaths,)),
-               res=DeviceArray((npaths,)), st=DeviceArray((npaths,), np.int32))
+    b_learn = TrackBuffers(npaths, n, 1)
+
+    def refbufs():
+        return dict(x2=DeviceArray((4, npaths, n)), dx=DeviceArray((npaths,)),
+                    res=DeviceArray((npaths,)), st=DeviceArray((npaths,), np.int32))
+
+    ref_m, ref_l = refbufs(), refbufs()
     if args.calibrate_copy:
         big = DeviceArray((32 * 1024 * 1024,))
         dst = DeviceArray((32 * 1024 * 1024,))
         check(lib.phc_device_copy(dst.ptr, big.ptr, big.nbytes, None))
         check(lib.phc_synchronize())
 
+    ctl, train_info = (None, None) if args.no_learned else train_learned(n, prm_mixed, ev)
+    cstruct = ctl.struct(1) if ctl is not None else None
+
     for _ in range(args.warmup):
         run_plain(tr, d_start, npaths, b_plain, ev, prm_plain)
-        run_mixed(tr, d_start, npaths, b_mixed, ref, ev, prm_mixed, lib)
+        run_mixed(tr, d_start, h_start, npaths, b_mixed, ref_m, ev, prm_mixed, lib)
+        if cstruct is not None:
+            run_mixed(tr, d_start, h_start, npaths, b_learn, ref_l, ev, prm_mixed, lib, cstruct)
     check(lib.phc_synchronize())
 
-    plain_ms, track_ms, refine_ms, mixed_ms = [], [], [], []
-    escal = 0
-    check(lib.phc_synchronize())
-    wall0 = time.perf_counter()
-    for _ in range(args.steps):
-        a, b, e = run_mixed(tr, d_start, npaths, b_mixed, ref, ev, prm_mixed, lib)
-        track_ms.append(a)
-        refine_ms.append(b)
-        mixed_ms.append(a + b)
-        escal = e
-    check(lib.phc_synchronize())
-    wall_mixed = (time.perf_counter() - wall0) / args.steps * 1e3
-    wall0 = time.perf_counter()
-    for _ in range(args.steps):
-        plain_ms.append(run_plain(tr, d_start, npaths, b_plain, ev, prm_plain))
-    check(lib.phc_synchronize())
-    wall_plain = (time.perf_counter() - wall0) / args.steps * 1e3
+    def timed(fn):
+        rows = []
+        check(lib.phc_synchronize())
+        w0 = time.perf_counter()
+        for _ in range(args.steps):
+            rows.append(fn())
+        check(lib.phc_synchronize())
+        return rows, (time.perf_counter() - w0) / args.steps * 1e3
 
-    # ---- verification (not timed): octo-double residuals at t = 1 ----
-    op = b_plain.to_host()
-    om = b_mixed.to_host()
-    xp = op["endp"]
-    res_p = qd_residuals(tr, to_qd(xp, 2))
-    x2 = ref["x2"].to_host()
-    xm = to_qd(x2, 2)
-    if ref.get("escalated") is not None:
-        bad, x4 = ref["escalated"]
-        xm[:, bad] = x4
-    res_m = qd_residuals(tr, xm)
-    ok_p = (op["status"] == 1) & (res_p <= RES_TOL)
-    ok_m = (om["status"] == 1) & (res_m <= RES_TOL)
-    cp, cm = planes_to_points(xp, 2), planes_to_points(xm, 4)
-    lost = count_lost(cp, ok_p, cm, ok_m)
+    rows_l, wall_l = (timed(lambda: run_mixed(tr, d_start, h_start, npaths, b_learn, ref_l, ev,
+                                              prm_mixed, lib, cstruct))
+                      if cstruct is not None else (None, None))
+    rows_m, wall_m = timed(lambda: run_mixed(tr, d_start, h_start, npaths, b_mixed, ref_m, ev,
+                                             prm_mixed, lib))
+    rows_p, wall_p = timed(lambda: run_plain(tr, d_start, npaths, b_plain, ev, prm_plain))
 
-    t_mixed = float(np.mean(mixed_ms))
-    t_plain = float(np.mean(plain_ms))
-    v_mixed = ok_m.sum() / (t_mixed / 1e3)
+    # ---- verification (not timed): quad-double residuals at t = 1 ----
+    op, om = b_plain.to_host(), b_mixed.to_host()
+    xp, res_p, ok_p = verify(tr, op, None, 2)
+    xm, res_m, ok_m = verify(tr, om, ref_m, 2)
+    cp, cm = planes_to_points(xp, 4), planes_to_points(xm, 4)
+    t_plain = float(np.mean(rows_p))
     v_plain = ok_p.sum() / (t_plain / 1e3)
-    fl_track = tracker_flops(tr.prog, 1, om["work"])
+
+    def block(out, rows, wall, x, res, ok, c):
+        t_track = float(np.mean([r[0] for r in rows]))
+        t_ref = float(np.mean([r[1] for r in rows]))
+        t = t_track + t_ref
+        v = ok.sum() / (t / 1e3)
+        return v, t, t_track, {
+            "paths_per_s": round(float(v), 1), "ms_per_step": round(t, 3),
+            "good_paths": int(ok.sum()), "max_residual": float(res.max()),
+            "track_ms": round(t_track, 3), "refine_ms": round(t_ref, 3),
+            "escalated_paths": int(rows[-1][2]), "retracked_paths": int(rows[-1][3]),
+            "steps_per_path": float(out["nsteps"].mean()),
+            "rejections_per_path": float(out["nrej"].mean()),
+            "newton_steps_per_s": round(int(out["work"][1].sum()) / (t / 1e3), 1),
+            "distinct_solutions": distinct(c, ok),
+            "lost_paths_vs_plain": int(count_lost(cp, ok_p, c, ok)),
+            "wall_ms_per_step": round(wall, 3)}
+
+    v_m, t_m, tt_m, blk_m = block(om, rows_m, wall_m, xm, res_m, ok_m, cm)
+    head = dict(v=v_m, t=t_m, tt=tt_m, out=om, lost=blk_m["lost_paths_vs_plain"], name="mixed")
+    blk_l = None
+    if rows_l is not None:
+        ol = b_learn.to_host()
+        xl, res_l, ok_l = verify(tr, ol, ref_l, 2)
+        cl = planes_to_points(xl, 4)
+        v_l, t_l, tt_l, blk_l = block(ol, rows_l, wall_l, xl, res_l, ok_l, cl)
+        blk_l["threshold"] = ctl.threshold
+        blk_l["training"] = train_info
+        head = dict(v=v_l, t=t_l, tt=tt_l, out=ol, lost=blk_l["lost_paths_vs_plain"],
+                    name="learned+mixed")
+
+    fl_track = tracker_flops(tr.prog, 1, head["out"]["work"])
     fl_plain = tracker_flops(tr.prog, 2, op["work"])
-    t_track = float(np.mean(track_ms))
-    achieved = fl_track / (t_track / 1e3) / 1e12
-    newton_iters = int(om["work"][1].sum())
+    achieved = fl_track / (head["tt"] / 1e3) / 1e12
     line = {
         "metric": METRIC,
-        "value": round(float(v_mixed), 1),
+        "value": round(float(head["v"]), 1),
         "unit": "paths/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(t_mixed, 3),
+        "ms_per_step": round(head["t"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -274,7 +366,7 @@ def run(args):
         "config": {
             "workload": "total-degree homotopy, 12 dense complex quadratics in 12 variables, "
                         "4096 paths, degree-8 series predictor, seed 20261202",
-            "method": "mixed: FP64 tracking + dd endpoint Newton (qd/od escalation)",
+            "method": head["name"] + ": FP64 tracking + dd endpoint Newton (qd escalation)",
             "paths": npaths, "n": n, "series_degree": 8, "seed": SEED,
             "residual_tol": RES_TOL,
             "residual_check": "max_i |p_i(x)| at the exact endpoint, quad-double on the GPU",
@@ -282,16 +374,13 @@ def run(args):
         "plain_dd": {"paths_per_s": round(float(v_plain), 1), "ms_per_step": round(t_plain, 3),
                      "good_paths": int(ok_p.sum()), "max_residual": float(res_p.max()),
                      "achieved_tflops": round(fl_plain / (t_plain / 1e3) / 1e12, 3),
-                     "steps_per_path": float(op["nsteps"].mean())},
-        "mixed": {"good_paths": int(ok_m.sum()), "max_residual": float(res_m.max()),
-                  "track_ms": round(t_track, 3), "refine_ms": round(float(np.mean(refine_ms)), 3),
-                  "escalated_paths": int(escal), "steps_per_path": float(om["nsteps"].mean()),
-                  "newton_steps_per_s": round(newton_iters / (t_mixed / 1e3), 1),
-                  "distinct_solutions": distinct(cm, ok_m)},
-        "speedup_mixed_vs_plain": round(float(v_mixed / v_plain), 3) if v_plain > 0 else None,
-        "lost_paths_vs_plain": int(lost),
-        "learned_controller": "not yet built",
-        "wall_ms_per_step": {"mixed": round(wall_mixed, 3), "plain": round(wall_plain, 3)},
+                     "steps_per_path": float(op["nsteps"].mean()),
+                     "wall_ms_per_step": round(wall_p, 3)},
+        "mixed": blk_m,
+        "learned_mixed": blk_l,
+        "speedup_vs_plain": round(float(head["v"] / v_plain), 3) if v_plain > 0 else None,
+        "speedup_mixed_vs_plain": round(float(v_m / v_plain), 3) if v_plain > 0 else None,
+        "lost_paths_vs_plain": int(head["lost"]),
         "roofline": {
             "bound": "mfma",
             "achieved": round(achieved, 4),
@@ -300,12 +389,12 @@ def run(args):
             "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
             "traffic": measured_traffic("track_kernel<1>"),
             "traffic_source": "profiles/tracker_c3_hbm_traffic.md",
-            "kernel": "track_kernel<1> (FP64 tracking)",
+            "kernel": "track_kernel<1> (FP64 tracking, " + head["name"] + ")",
             "note": "FP64 vector ALU; MI355X FP64 peak 78.6 TFLOP/s for both VALU and MFMA; "
                     "flops counted from the program and per-path work counters "
                     "(phcpack_amd/flops.py, FMA = 2)",
             "flops_per_launch": float(fl_track),
-            "kernel_ms": round(t_track, 3),
+            "kernel_ms": round(head["tt"], 3),
         },
     }
     if not args.no_cpu:

@@ -294,6 +294,25 @@ void omd_add_d(int K, const double *a, double b, double *c) {
   else k_add_d(K, a, b, c);
 }
 
+/* Norms/double_double_functions.cpp:184-193 (two_sqr) and :206-214 (sqr):
+ * the square adds lo*lo, which the product a*a omits.  The quad and octo
+ * double squares (qdf_sqr, odf_sqr) give the same bits as their products. */
+static void dd_sqr(const double *a, double *b) {
+  double hi, lo, p1, p2;
+  p1 = a[0] * a[0];
+  split(a[0], &hi, &lo);
+  p2 = ((hi * hi - p1) + 2.0 * hi * lo) + lo * lo;
+  p2 += 2.0 * a[0] * a[1];
+  p2 += a[1] * a[1];
+  b[0] = omd_quick_two_sum(p1, p2, &b[1]);
+}
+
+void omd_sqr(int K, const double *a, double *c) {
+  if (K == 1) c[0] = a[0] * a[0];
+  else if (K == 2) dd_sqr(a, c);
+  else k_mul(K, a, a, c);
+}
+
 void omd_neg(int K, const double *a, double *c) {
   int i;
   for (i = 0; i < K; i++) c[i] = -a[i];

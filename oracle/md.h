@@ -33,6 +33,8 @@ void omd_mul(int K, const double *a, const double *b, double *c);
 void omd_div(int K, const double *a, const double *b, double *c);
 void omd_sqrt(int K, const double *a, double *c);
 void omd_mul_d(int K, const double *a, double b, double *c);
+/* c = a*a as the reference squares (ddf_sqr differs from ddf_mul(a,a)) */
+void omd_sqr(int K, const double *a, double *c);
 void omd_add_d(int K, const double *a, double b, double *c);
 void omd_neg(int K, const double *a, double *c);
 void omd_copy(int K, const double *a, double *c);

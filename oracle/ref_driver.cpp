@@ -17,6 +17,7 @@
 #include "dbl2_convolutions_host.h"
 #include "dbl4_convolutions_host.h"
 #include "dbl8_convolutions_host.h"
+#include "dbl_polynomials_host.h"
 #include "dbl2_polynomials_host.h"
 #include "dbl4_polynomials_host.h"
 #include "dbl8_polynomials_host.h"
@@ -117,7 +118,9 @@ void ref_poly_evaldiff(int K, int cplx, int dim, int nbr, int deg, const int *nv
   Rows C(cff, P, nbr, D), X(input, P, dim, D), Y(output, P, dim + 1, D);
   double el = 0.0;
 #define CS(p) (cst + (size_t)(p) * D)
-  if (K == 2 && !cplx)
+  if (K == 1 && !cplx)
+    CPU_dbl_poly_evaldiff(dim, nbr, deg, nv.data(), idx.data(), CS(0), C[0], X[0], Y[0], &el, 0);
+  else if (K == 2 && !cplx)
     CPU_dbl2_poly_evaldiff(dim, nbr, deg, nv.data(), idx.data(), CS(0), CS(1), C[0], C[1],
                            X[0], X[1], Y[0], Y[1], &el, 0);
   else if (K == 2)

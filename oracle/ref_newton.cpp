@@ -1,0 +1,356 @@
+// Flat extern "C" wrappers around the reference's own host code of the
+// linearized power-series Newton method (QR, Toeplitz solve, residual,
+// staggered row Newton) and of its test-system generators, compiled with the
+// reference sources (read in place) into oracle/_ref/libphcref.so.
+//
+// TEST INFRASTRUCTURE ONLY: pins the oracle restatement (oracle/newton.c)
+// and writes golden vectors.  Never linked into the product.
+//
+// Arrays are limb-planar as in oracle/newton.h.
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include "double_double_functions.h"
+#include "quad_double_functions.h"
+#include "octo_double_functions.h"
+#include "random_numbers.h"
+#include "random2_vectors.h"
+#include "dbl_factorizations.h"
+#include "dbl2_factorizations.h"
+#include "dbl4_factorizations.h"
+#include "dbl8_factorizations.h"
+#include "dbl_bals_host.h"
+#include "dbl2_bals_host.h"
+#include "dbl4_bals_host.h"
+#include "dbl8_bals_host.h"
+#include "dbl_polynomials_host.h"
+#include "dbl2_polynomials_host.h"
+#include "dbl4_polynomials_host.h"
+#include "dbl8_polynomials_host.h"
+#include "cyclic_indices.h"
+#include "dbl2_indexed_coefficients.h"
+#include "dbl2_monomial_systems.h"
+
+namespace {
+
+// row pointers of a [P][nmat][rows][cols] array: m[p][k][i] -> row i of matrix k
+struct Mats {
+  std::vector<std::vector<double *>> rows;
+  std::vector<std::vector<double **>> mats;
+  Mats(double *base, int P, int nmat, int nr, int nc) : rows(P * nmat), mats(P) {
+    for (int p = 0; p < P; ++p)
+      for (int k = 0; k < nmat; ++k) {
+        auto &r = rows[p * nmat + k];
+        for (int i = 0; i < nr; ++i)
+          r.push_back(base + (((size_t)p * nmat + k) * nr + i) * nc);
+      }
+    for (int p = 0; p < P; ++p)
+      for (int k = 0; k < nmat; ++k) mats[p].push_back(rows[p * nmat + k].data());
+  }
+  double ***ser(int p) { return mats[p].data(); }   // series of matrices, plane p
+  double **mat(int p) { return mats[p][0]; }        // nmat == 1, plane p
+};
+
+// limb expansions of arrays indexed by plane
+#define L1(a) a[0]
+#define L2(a) a[0], a[1]
+#define L4(a) a[0], a[1], a[2], a[3]
+#define L8(a) a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7]
+
+template <int K> struct Ref;
+
+#define DEFREF(KK, PFX, LX)                                                                   \
+  template <> struct Ref<KK> {                                                                \
+    static void houseqr(int nr, int nc, double **A[], double **Q[], double **R[]) {           \
+      CPU_##PFX##_factors_houseqr(nr, nc, LX(A), LX(Q), LX(R));                               \
+    }                                                                                         \
+    static void solve(int dim, int degp1, int tail, double ***M[], double **B[],             \
+                      double **X[], double **Q[], double **R[], double *W[], bool *zq,        \
+                      bool *nq, int *up, int *bs, int *nt) {                                  \
+      CPU_##PFX##_qrbs_solve(dim, degp1, tail, LX(M), LX(B), LX(X), LX(Q), LX(R), LX(W), zq,  \
+                             nq, up, bs, nt, 0);                                              \
+    }                                                                                         \
+    static void residue(int dim, int degp1, int tail, double ***M[], double **B[],           \
+                        double **X[], double **Rv[], double *rm[]) {                          \
+      CPU_##PFX##_linear_residue(dim, degp1, tail, LX(M), LX(B), LX(X), LX(Rv), LX(rm), 0);   \
+    }                                                                                         \
+    static void evaldiff(int dim, int nbr, int deg, int *nvr, int **idx, double *c[],        \
+                         double **f[], double **in[], double **out[]) {                       \
+      double el = 0.0;                                                                        \
+      CPU_##PFX##_poly_evaldiff(dim, nbr, deg, nvr, idx, LX(c), LX(f), LX(in), LX(out), &el,  \
+                                0);                                                           \
+    }                                                                                         \
+  };
+DEFREF(1, dbl, L1)
+DEFREF(2, dbl2, L2)
+DEFREF(4, dbl4, L4)
+DEFREF(8, dbl8, L8)
+#undef DEFREF
+
+// x += dx with the reference's in-place addition of each precision
+// (dbl2_update_series, Newton/dbl2_newton_testers.cpp:116-149, and its twins)
+template <int K> void inc(double *x[], const double *dx[]);
+template <> void inc<1>(double *x[], const double *dx[]) { *x[0] = *x[0] + *dx[0]; }
+template <> void inc<2>(double *x[], const double *dx[]) { ddf_inc(x[0], x[1], *dx[0], *dx[1]); }
+template <> void inc<4>(double *x[], const double *dx[]) {
+  qdf_inc(x[0], x[1], x[2], x[3], *dx[0], *dx[1], *dx[2], *dx[3]);
+}
+template <> void inc<8>(double *x[], const double *dx[]) {
+  odf_inc(x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], *dx[0], *dx[1], *dx[2], *dx[3],
+          *dx[4], *dx[5], *dx[6], *dx[7]);
+}
+
+template <int K> void houseqr_k(int nr, int nc, double *A, double *Q, double *R) {
+  Mats a(A, K, 1, nr, nc), q(Q, K, 1, nr, nr), r(R, K, 1, nr, nc);
+  double **pa[K], **pq[K], **pr[K];
+  for (int p = 0; p < K; ++p) { pa[p] = a.mat(p); pq[p] = q.mat(p); pr[p] = r.mat(p); }
+  Ref<K>::houseqr(nr, nc, pa, pq, pr);
+}
+
+template <int K>
+void solve_k(int dim, int degp1, int tail, double *mat, double *rhs, double *sol, double *Q,
+             double *R, int *zeroQ, int *noqr, int *up, int *bs, int *nt) {
+  Mats m(mat, K, degp1, dim, dim), b(rhs, K, 1, degp1, dim), x(sol, K, 1, degp1, dim);
+  Mats q(Q, K, 1, dim, dim), r(R, K, 1, dim, dim);
+  std::vector<double> wrk((size_t)K * dim);
+  double ***pm[K], **pb[K], **px[K], **pq[K], **pr[K], *pw[K];
+  for (int p = 0; p < K; ++p) {
+    pm[p] = m.ser(p); pb[p] = b.mat(p); px[p] = x.mat(p); pq[p] = q.mat(p); pr[p] = r.mat(p);
+    pw[p] = wrk.data() + (size_t)p * dim;
+  }
+  bool zq = *zeroQ != 0, nq = *noqr != 0;
+  Ref<K>::solve(dim, degp1, tail, pm, pb, px, pq, pr, pw, &zq, &nq, up, bs, nt);
+  *zeroQ = zq;
+  *noqr = nq;
+}
+
+template <int K>
+void residue_k(int dim, int degp1, int tail, double *mat, double *rhs, double *sol,
+               double *resvec, double *resmax) {
+  Mats m(mat, K, degp1, dim, dim), b(rhs, K, 1, degp1, dim), x(sol, K, 1, degp1, dim);
+  Mats rv(resvec, K, 1, degp1, dim);
+  double ***pm[K], **pb[K], **px[K], **pr[K], *rm[K];
+  for (int p = 0; p < K; ++p) {
+    pm[p] = m.ser(p); pb[p] = b.mat(p); px[p] = x.mat(p); pr[p] = rv.mat(p);
+    rm[p] = resmax + p;
+  }
+  Ref<K>::residue(dim, degp1, tail, pm, pb, px, pr, rm);
+}
+
+// The host branch of the staggered row Newton test (test_dbl2_row_newton,
+// Newton/dbl2_newton_method.cpp:1447-1494, with dbl2_row_newton_qrstep :429-594
+// and dbl2_update_newton_qrstep :134-202), calling the reference's
+// CPU_*_poly_evaldiff and CPU_*_qrbs_solve.
+template <int K>
+int row_newton_k(int dim, int deg, int nbsteps, const int *monoptr, const int *nvr,
+                 const int *idxptr, const int *idxflat, double *cst, double *cff, double *input,
+                 int *wdeg, int *tails, int *ups, int *bss, double *history, double *Qo,
+                 double *Ro) {
+  const int D = deg + 1, nmono = monoptr[dim];
+  std::vector<int> nv(nvr, nvr + (nmono > 0 ? nmono : 1));
+  std::vector<int> idxc(idxflat, idxflat + idxptr[nmono]);
+  std::vector<int *> idx(nmono > 0 ? nmono : 1);
+  for (int m = 0; m < nmono; ++m) idx[m] = idxc.data() + idxptr[m];
+  Mats in(input, K, 1, dim, D);                     // input[p][i] -> series
+  Mats cf(cff, K, 1, nmono > 0 ? nmono : 1, D);     // cff[p][m]
+  std::vector<double> out((size_t)K * dim * (dim + 1) * D);
+  Mats ou(out.data(), K, dim, dim + 1, D);          // out[p][i][j]
+  std::vector<double> jac((size_t)K * D * dim * dim), rhs((size_t)K * D * dim),
+      urhs((size_t)K * D * dim), sol((size_t)K * D * dim), Q((size_t)K * dim * dim),
+      R((size_t)K * dim * dim), wrk((size_t)K * dim);
+  Mats jm(jac.data(), K, D, dim, dim), ub(urhs.data(), K, 1, D, dim), sx(sol.data(), K, 1, D, dim);
+  Mats qm(Q.data(), K, 1, dim, dim), rm(R.data(), K, 1, dim, dim);
+  double ***pm[K], **pb[K], **px[K], **pq[K], **pr[K], *pw[K], **pin[K];
+  for (int p = 0; p < K; ++p) {
+    pm[p] = jm.ser(p); pb[p] = ub.mat(p); px[p] = sx.mat(p); pq[p] = qm.mat(p);
+    pr[p] = rm.mat(p); pw[p] = wrk.data() + (size_t)p * dim; pin[p] = in.mat(p);
+  }
+  bool zeroQ = true, noqr = false;
+  int tailidx = 1, wrkdeg = 0, up = 0, bs = 0, stepcnt = 0;
+  for (int step = 0; step < nbsteps; ++step) {
+    const int degp1 = wrkdeg + 1;
+    for (int i = 0; i < dim; ++i) {
+      double *c[K], **f[K], **o[K];
+      for (int p = 0; p < K; ++p) {
+        c[p] = cst + ((size_t)p * dim + i) * D;
+        f[p] = cf.mat(p) + monoptr[i];
+        o[p] = ou.ser(p)[i];
+      }
+      Ref<K>::evaldiff(dim, monoptr[i + 1] - monoptr[i], wrkdeg, nv.data() + monoptr[i],
+                       idx.data() + monoptr[i], c, f, pin, o);
+    }
+    // dbl2_map_evaldiff_output and rhs = -funval, then the saved copy urhs
+    for (int p = 0; p < K; ++p)
+      for (int i = 0; i < dim; ++i)
+        for (int k = 0; k < degp1; ++k) {
+          rhs[((size_t)p * D + k) * dim + i] = -ou.ser(p)[i][dim][k];
+          for (int j = 0; j < dim; ++j)
+            jac[(((size_t)p * D + k) * dim + i) * dim + j] = ou.ser(p)[i][j][k];
+        }
+    for (int p = 0; p < K; ++p)
+      for (int k = 0; k < degp1; ++k)
+        for (int i = 0; i < dim; ++i) {
+          urhs[((size_t)p * D + k) * dim + i] = rhs[((size_t)p * D + k) * dim + i];
+          sol[((size_t)p * D + k) * dim + i] = 0.0;
+        }
+    int newtail = tailidx;
+    Ref<K>::solve(dim, degp1, tailidx, pm, pb, px, pq, pr, pw, &zeroQ, &noqr, &up, &bs, &newtail);
+    tailidx = newtail;
+    for (int j = tailidx - 1; j < degp1; ++j)
+      for (int i = 0; i < dim; ++i) {
+        double *x[K];
+        const double *dx[K];
+        for (int p = 0; p < K; ++p) {
+          x[p] = &pin[p][i][j];
+          dx[p] = &px[p][j][i];
+        }
+        inc<K>(x, dx);
+      }
+    if (wdeg) wdeg[step] = wrkdeg;
+    if (tails) tails[step] = tailidx;
+    if (ups) ups[step] = up;
+    if (bss) bss[step] = bs;
+    if (history) std::memcpy(history + (size_t)step * K * dim * D, input, sizeof(double) * K * dim * D);
+    ++stepcnt;
+    if (tailidx >= deg) break;
+    wrkdeg = wrkdeg + 1 + wrkdeg / 2;
+    if (wrkdeg > deg) wrkdeg = deg;
+  }
+  if (Qo) std::memcpy(Qo, Q.data(), sizeof(double) * K * dim * dim);
+  if (Ro) std::memcpy(Ro, R.data(), sizeof(double) * K * dim * dim);
+  return stepcnt;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ref_houseqr(int K, int nrows, int ncols, double *A, double *Q, double *R) {
+  switch (K) {
+    case 1: houseqr_k<1>(nrows, ncols, A, Q, R); return 0;
+    case 2: houseqr_k<2>(nrows, ncols, A, Q, R); return 0;
+    case 4: houseqr_k<4>(nrows, ncols, A, Q, R); return 0;
+    case 8: houseqr_k<8>(nrows, ncols, A, Q, R); return 0;
+  }
+  return -1;
+}
+
+int ref_qrbs_solve(int K, int dim, int degp1, int tailidx, double *mat, double *rhs,
+                   double *sol, double *Q, double *R, int *zeroQ, int *noqr, int *upidx,
+                   int *bsidx, int *newtail) {
+  switch (K) {
+#define CASE(KK) \
+    case KK: solve_k<KK>(dim, degp1, tailidx, mat, rhs, sol, Q, R, zeroQ, noqr, upidx, bsidx, newtail); return 0;
+    CASE(1) CASE(2) CASE(4) CASE(8)
+#undef CASE
+  }
+  return -1;
+}
+
+int ref_linear_residue(int K, int dim, int degp1, int tailidx, double *mat, double *rhs,
+                       double *sol, double *resvec, double *resmax) {
+  switch (K) {
+#define CASE(KK) case KK: residue_k<KK>(dim, degp1, tailidx, mat, rhs, sol, resvec, resmax); return 0;
+    CASE(1) CASE(2) CASE(4) CASE(8)
+#undef CASE
+  }
+  return -1;
+}
+
+int ref_row_newton(int K, int dim, int deg, int nbsteps, const int *monoptr, const int *nvr,
+                   const int *idxptr, const int *idx, double *cst, double *cff, double *input,
+                   int *wdeg, int *tail, int *up, int *bs, double *history, double *Q,
+                   double *R) {
+  switch (K) {
+#define CASE(KK)                                                                              \
+    case KK:                                                                                  \
+      return row_newton_k<KK>(dim, deg, nbsteps, monoptr, nvr, idxptr, idx, cst, cff, input,  \
+                              wdeg, tail, up, bs, history, Q, R);
+    CASE(1) CASE(2) CASE(4) CASE(8)
+#undef CASE
+  }
+  return -1;
+}
+
+// The reference's real double-double row Newton test system
+// (dbl2_test_rows, Newton/test_dbl2_newton.cpp:191-212, with
+// test_dbl2_row_newton :1289-1301 and dbl2_row_setup :823-991, mode 1):
+// cyclic supports from make_polynomial_indices, random exponential
+// coefficient series with random leading coefficients, a random exponential
+// test solution with the constants adjusted so that it is a root, and the
+// start series = leading coefficients of the test solution + 1e-16.
+// Sizes: monoptr[dim+1], nvr/idxptr[nmono] (+1 for idxptr), idx[nidx];
+// call with cst == nullptr to get nmono (return) and nidx (*nidx_out).
+int ref_dbl2_cyclic_row_system(unsigned seed, int dim, int deg, int *monoptr, int *nvr,
+                               int *idxptr, int *idxflat, int *nidx_out, double *cst,
+                               double *cff, double *testsol, double *start) {
+  const int D = deg + 1;
+  std::vector<int> nbr(dim);
+  std::vector<int *> nv(dim);
+  std::vector<int **> ix(dim);
+  make_polynomial_indices(dim, nbr.data(), nv.data(), ix.data());
+  int nmono = 0, nidx = 0;
+  for (int i = 0; i < dim; ++i) {
+    nmono += nbr[i];
+    for (int j = 0; j < nbr[i]; ++j) nidx += nv[i][j];
+  }
+  if (nidx_out) *nidx_out = nidx;
+  if (cst == nullptr) return nmono;
+  int m = 0, o = 0;
+  monoptr[0] = 0;
+  for (int i = 0; i < dim; ++i) {
+    for (int j = 0; j < nbr[i]; ++j, ++m) {
+      nvr[m] = nv[i][j];
+      idxptr[m] = o;
+      for (int k = 0; k < nv[i][j]; ++k) idxflat[o++] = ix[i][j][k];
+    }
+    monoptr[i + 1] = m;
+  }
+  idxptr[m] = o;
+  srand(seed);
+  std::vector<double *> csthi(dim), cstlo(dim);
+  std::vector<double **> cffhi(dim), cfflo(dim);
+  dbl2_make_coefficients(dim, deg, nbr.data(), nv.data(), ix.data(), csthi.data(),
+                         cstlo.data(), cffhi.data(), cfflo.data(), 0);
+  for (int i = 0; i < dim; ++i)
+    for (int j = 0; j < nbr[i]; ++j) random_double_double(&cffhi[i][j][0], &cfflo[i][j][0]);
+  std::vector<double *> shi(dim), slo(dim);
+  for (int i = 0; i < dim; ++i) {
+    shi[i] = new double[D];
+    slo[i] = new double[D];
+  }
+  make_real2_exponentials(dim, deg, shi.data(), slo.data());
+  std::vector<double> ob((size_t)2 * (dim + 1) * D);
+  Mats ou(ob.data(), 2, 1, dim + 1, D);
+  for (int i = 0; i < dim; ++i) {
+    double el = 0.0;
+    CPU_dbl2_poly_evaldiff(dim, nbr[i], deg, nv[i], ix[i], csthi[i], cstlo[i], cffhi[i],
+                           cfflo[i], shi.data(), slo.data(), ou.mat(0), ou.mat(1), &el, 0);
+    for (int j = 0; j <= deg; ++j)
+      ddf_dec(&csthi[i][j], &cstlo[i][j], ou.mat(0)[dim][j], ou.mat(1)[dim][j]);
+  }
+  m = 0;
+  for (int i = 0; i < dim; ++i) {
+    for (int k = 0; k < D; ++k) {
+      cst[(size_t)i * D + k] = csthi[i][k];
+      cst[((size_t)dim + i) * D + k] = cstlo[i][k];
+      testsol[(size_t)i * D + k] = shi[i][k];
+      testsol[((size_t)dim + i) * D + k] = slo[i][k];
+      start[(size_t)i * D + k] = 0.0;
+      start[((size_t)dim + i) * D + k] = 0.0;
+    }
+    // real2_start_series_vector (Newton/dbl2_newton_testers.cpp:25-40)
+    double hi = shi[i][0], lo = slo[i][0];
+    ddf_inc(&hi, &lo, 1.0e-16, 0.0);
+    start[(size_t)i * D] = hi;
+    start[((size_t)dim + i) * D] = lo;
+    for (int j = 0; j < nbr[i]; ++j, ++m)
+      for (int k = 0; k < D; ++k) {
+        cff[(size_t)m * D + k] = cffhi[i][j][k];
+        cff[((size_t)nmono + m) * D + k] = cfflo[i][j][k];
+      }
+  }
+  return nmono;
+}
+
+}  // extern "C"

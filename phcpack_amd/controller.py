@@ -197,3 +197,62 @@ def collect_logs(tracker, starts, params, seed=1, cap=256, K=1, controller=None)
     tracker.track_device(d, 1, npaths, K, bufs, params, controller=ctl.struct(2, logs, seed))
     check(load().phc_synchronize())
     return logs.to_host() + (bufs.to_host(),)
+
+
+def train_controller(trackers, starts, params, K=1, seed=1, cap=256, epochs=30, batch=1024,
+                     lr=3e-3, threshold=0.8):
+    """Explore-mode logs from every (tracker, starts) pair, then GPU training.
+
+    Returns the trained StepController and a summary dict (samples,
+    acceptance rate, first and last epoch loss)."""
+    X, C, Y = [], [], []
+    for i, (tr, st) in enumerate(zip(trackers, starts)):
+        x, c, y, _ = collect_logs(tr, st, params, seed=seed + i, cap=cap, K=K)
+        X.append(x)
+        C.append(c)
+        Y.append(y)
+    X, C, Y = np.concatenate(X), np.concatenate(C), np.concatenate(Y)
+    ctl = StepController(threshold=threshold, seed=seed)
+    losses = ctl.fit(X, C, Y, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    return ctl, dict(samples=int(len(X)), accept_rate=float(Y.mean()),
+                     loss_first=float(losses[0]), loss_last=float(losses[-1]))
+
+
+def tune_threshold(ctl, tracker, starts, params, K=1, thresholds=(0.5, 0.6, 0.7, 0.8, 0.9),
+                   timer=None):
+    """Pick the acceptance-probability threshold with the fewest predictor
+    steps (or the shortest time when `timer(fn)` is given) among those that
+    lose no successful path relative to the plain pole-step rule on a
+    tuning system.  Returns (threshold, table)."""
+    from .tracker import TrackBuffers, points_to_planes
+    planes = points_to_planes(starts)
+    npaths = planes.shape[1]
+    d = DeviceArray.from_host(planes)
+    bufs = TrackBuffers(npaths, tracker.n, K)
+
+    def run(c):
+        tracker.track_device(d, 1, npaths, K, bufs, params, controller=c)
+        check(load().phc_synchronize())
+
+    def measure(c):
+        ms = timer(lambda: run(c)) if timer else None
+        if ms is None:
+            run(c)
+        out = bufs.to_host()
+        return dict(ok=int((out["status"] == 1).sum()), pred=int(out["work"][0].sum()),
+                    ms=ms)
+
+    base = measure(None)
+    table = {"plain": base}
+    best, best_key = None, None
+    for th in thresholds:
+        ctl.threshold = th
+        r = measure(ctl.struct(1))
+        table[th] = r
+        if r["ok"] < base["ok"]:
+            continue
+        key = r["ms"] if r["ms"] is not None else r["pred"]
+        if best_key is None or key < best_key:
+            best, best_key = th, key
+    ctl.threshold = best if best is not None else max(thresholds)
+    return best, table

