@@ -227,6 +227,66 @@ int phc_refine(phc_tracker *tracker, int K, int npaths, double t, const double *
                double *xout_dev, int iters, double tol, double *dxlast_dev, double *res_dev,
                int *status_dev, int waves, void *stream);
 
+/* ---------------- linearized power-series Newton (real series) ----------
+ * Layouts are limb-planar with the batch index after the limb:
+ *   matrices [K][nbatch][n][n], matrix series [K][nbatch][degp1][n][n],
+ *   vector series [K][nbatch][degp1][n], series x [K][nsys][dim][deg+1].
+ * K = 1, 2, 4 (double, double-double, quad-double); results are
+ * bit-identical to the reference's host functions named below. */
+
+/* batched Householder QR with explicit Q of n x n matrices (n <= 64):
+ * CPU_dbl2_factors_houseqr (Matrices/dbl2_factorizations.cpp:723) and its
+ * dbl/dbl4 twins, in place of GPU_dbl2_blocked_houseqr for small n. */
+int phc_houseqr_batch(int K, int n, int nbatch, const double *A_dev, double *Q_dev,
+                      double *R_dev, void *stream);
+
+/* batched lower block-triangular Toeplitz solve sum_{j<=k} A_j x_{k-j} = b_k
+ * with the reference's skip rules (threshold 1e-15 / 1e-28 / 1e-56):
+ * CPU_dbl2_qrbs_solve (Newton/dbl2_bals_host.cpp:479), replaces
+ * GPU_dbl2_bals_solve (Newton/dbl2_bals_kernels.cu:554).  rhs is updated
+ * by the tail as in the reference; Q, R are read when the head is skipped
+ * and written when it runs; flags [nbatch][5] = zeroQ, noqr, upidx, bsidx,
+ * newtail, in and out (newtail in = the old tail index). */
+int phc_qrbs_solve_batch(int K, int n, int degp1, int tailidx, int nbatch, const double *mat_dev,
+                         double *rhs_dev, double *sol_dev, double *Q_dev, double *R_dev,
+                         int *flags_dev, void *stream);
+
+/* r_i = b_i - sum_{j<=i-tailidx} A_j x_{i-j}, i = tailidx..degp1-1, and the
+ * limbs of the entry of largest |leading limb| in resmax [K][nbatch]:
+ * CPU_dbl2_linear_residue (Newton/dbl2_bals_host.cpp:548), replaces
+ * GPU_dbl2_linear_residue (Newton/dbl2_tail_kernels.cu:383). */
+int phc_linear_residue_batch(int K, int n, int degp1, int tailidx, int nbatch,
+                             const double *mat_dev, const double *rhs_dev, const double *sol_dev,
+                             double *res_dev, double *resmax_dev, void *stream);
+
+/* the supports of a polynomial system for row Newton: polynomial i has
+ * monomials monoptr[i]..monoptr[i+1]-1, monomial m the nvr[m] distinct
+ * variables idx[idxptr[m]..idxptr[m+1]-1] (the nbr/nvr/idx arguments of
+ * CPU_dbl2_poly_evaldiff, Polynomials/dbl2_polynomials_host.h, flattened);
+ * host arrays, copied to the device. */
+typedef struct phc_rowsys phc_rowsys;
+int phc_rowsys_create(phc_rowsys **sys, int dim, const int *monoptr, const int *nvr,
+                      const int *idxptr, const int *idx);
+int phc_rowsys_destroy(phc_rowsys *sys);
+/* LDS bytes one system occupies at precision K and degree deg */
+int phc_rowsys_lds_bytes(const phc_rowsys *sys, int K, int deg, size_t *bytes);
+
+/* staggered row Newton on nsys systems with the same supports, one
+ * wavefront per system: working degrees 0, 1, 2, 4, 7, ... capped at deg,
+ * at most nbsteps steps, stopping when the tail index reaches deg.  Each
+ * step evaluates and differentiates the system at the series x, solves the
+ * linearized block-Toeplitz system and updates x, as dbl2_row_newton_qrstep
+ * (Newton/dbl2_newton_method.cpp:429) inside the loop of
+ * test_dbl2_row_newton (:1447-1494).  cst [K][nc][dim][deg+1] and
+ * cff [K][nf][nmono][deg+1] with nc (nf) = 1 when cst_shared (cff_shared),
+ * else nsys; x [K][nsys][dim][deg+1] in/out.  Optional outputs (NULL to
+ * skip): steps [nsys]; wdeg, tail, up, bs [nsys][nbsteps]; history
+ * [nbsteps][K][nsys][dim][deg+1]; Q, R [K][nsys][dim][dim]. */
+int phc_row_newton(phc_rowsys *sys, int K, int deg, int nbsteps, int nsys, const double *cst_dev,
+                   int cst_shared, const double *cff_dev, int cff_shared, double *x_dev,
+                   int *steps_dev, int *wdeg_dev, int *tail_dev, int *up_dev, int *bs_dev,
+                   double *history_dev, double *Q_dev, double *R_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

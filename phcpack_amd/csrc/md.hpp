@@ -131,6 +131,18 @@ MD_HD dd div(const dd &a, const dd &b) {
   return add_d(r, q3);
 }
 
+// ddf_sqr (Norms/double_double_functions.cpp:206-214): the square adds
+// lo*lo, which mul(a, a) omits.  two_prod is exact, as ddf_two_sqr is.
+MD_HD dd sqr(const dd &a) {
+  double p1, p2;
+  p1 = two_prod(a.x[0], a.x[0], p2);
+  p2 += 2.0 * a.x[0] * a.x[1];
+  p2 += a.x[1] * a.x[1];
+  dd c;
+  c.x[0] = quick_two_sum(p1, p2, c.x[1]);
+  return c;
+}
+
 // ddf_sqr as used by the Karp square root: (ax)^2 with ax a double
 MD_HD dd sqr_d(double a) {
   double p1, p2;
@@ -329,6 +341,9 @@ MD_HD mdr<8> sqrt(const mdr<8> &a) {
 }
 
 // ------------------------------------------------ uniform generic interface
+
+// qdf_sqr and odf_sqr give the bits of the product a*a
+template <int K> MD_HD mdr<K> sqr(const mdr<K> &a) { return mul<K>(a, a); }
 
 template <int K> MD_HD mdr<K> from_d(double a) {
   mdr<K> c;

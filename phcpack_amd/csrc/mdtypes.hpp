@@ -18,6 +18,7 @@ MD_HD double div(double a, double b) { return a / b; }
 MD_HD double mul_d(double a, double b) { return a * b; }
 MD_HD double add_d(double a, double b) { return a + b; }
 MD_HD double neg(double a) { return -a; }
+MD_HD double sqr(double a) { return a * a; }
 
 template <int K> struct RealT { typedef mdr<K> type; };
 template <> struct RealT<1> { typedef double type; };

@@ -1,0 +1,771 @@
+// Linearized power-series Newton on real multiple-double series, batched:
+// one wavefront runs the whole staggered Newton method of one system in LDS.
+//
+//   reference (host)                          here
+//   CPU_dbl2_poly_evaldiff per polynomial     eval_system (slots of lanes, one
+//     (Polynomials/dbl2_polynomials_host.cpp)   polynomial per slot, lane = coefficient)
+//   dbl2_map_evaldiff_output, rhs = -f        read in place from the output rows
+//     (Newton/dbl2_systems_host.cpp:709)
+//   CPU_dbl2_factors_houseqr (Matrices/       houseqr: lane per column for the
+//     dbl2_factorizations.cpp:723)              reflector products, lane per entry for updates
+//   CPU_dbl2_factors_qrbs / _backward         qrbs: lane per row for Q^T b, column sweep
+//     (:823, :121)                              for the back substitution
+//   CPU_dbl2_qrbs_head/tail/solve             qrbs_solve: same skip rules and flags
+//     (Newton/dbl2_bals_host.cpp:15,228,479)
+//   CPU_dbl2_linear_residue (:548)            linear_residue
+//   dbl2_row_newton_qrstep (Newton/           row_newton_kernel: the staggered loop of
+//     dbl2_newton_method.cpp:429) and the       test_dbl2_row_newton (:1447-1494) in-kernel
+//     staggered loop
+//
+// Every number is computed by the same sequence of multiple-double operations
+// as the reference's host code (each lane keeps the reference's accumulation
+// order for the entries it owns; lanes split only independent entries), so
+// the results are bit-identical to the CPU oracle.  Lanes of one wavefront
+// exchange data through LDS; within a wavefront LDS operations complete in
+// program order, and wsync() keeps the compiler from moving them.
+#include <vector>
+
+#include "common.hpp"
+#include "mdtypes.hpp"
+
+using namespace md;
+using phc::grid_for;
+
+namespace {
+
+__device__ __forceinline__ void wsync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+__device__ __forceinline__ double wave_fmax(double v) {
+  for (int m = 1; m < 64; m <<= 1) v = fmax(v, __shfl_xor(v, m, 64));
+  return v;
+}
+
+constexpr size_t LDS_LIMIT = 160 * 1024;
+
+// skip thresholds of the tail solve (Newton/dbl_bals_host.cpp:199,
+// dbl2_bals_host.cpp:43, dbl4_bals_host.cpp:51, dbl8_bals_host.cpp:70)
+template <int K> __host__ __device__ constexpr double threshold() {
+  return K == 1 ? 1.0e-15 : K == 2 ? 1.0e-28 : K == 4 ? 1.0e-56 : 1.0e-120;
+}
+
+// a planar LDS (or global) array of m numbers with K limbs
+template <int K> struct Arr {
+  typedef typename RealT<K>::type T;
+  double *b;
+  int m;
+  __device__ T ld(int j) const { return R<K>::load(b, m, j); }
+  __device__ void st(int j, const T &v) const { R<K>::store(b, m, j, v); }
+};
+
+__device__ __forceinline__ bool all_zero(double a) { return a == 0.0; }
+template <int K> __device__ __forceinline__ bool all_zero(const mdr<K> &a) {
+  bool z = true;
+#pragma unroll
+  for (int i = 0; i < K; ++i) z = z && (a.x[i] == 0.0);
+  return z;
+}
+__device__ __forceinline__ double limb(double a, int) { return a; }
+template <int K> __device__ __forceinline__ double limb(const mdr<K> &a, int i) { return a.x[i]; }
+
+// carve a planar array of m numbers out of LDS
+template <int K> __device__ Arr<K> carve(double *&p, int m) {
+  Arr<K> a{p, m};
+  p += (size_t)K * m;
+  return a;
+}
+
+// largest |leading limb| of v[0..n-1], as CPU_dbl_onenorm computes it
+// (Newton/dbl_onenorms_host.cpp:9-17): NaN when v[0] is NaN, later NaNs
+// skipped.  Uniform over the wavefront; every lane must call it.
+template <int K> __device__ double maxnorm(const Arr<K> &v, int off, int n) {
+  const int l = lane_id();
+  double r = 0.0, v0 = 0.0;
+  for (int i = l; i < n; i += 64) {
+    const double a = fabs(R<K>::hi(v.ld(off + i)));
+    if (!(a <= r)) r = (a != a) ? r : a;
+  }
+  if (l == 0) v0 = fabs(R<K>::hi(v.ld(off)));
+  r = wave_fmax(r);
+  v0 = __shfl(v0, 0, 64);
+  return (v0 != v0) ? v0 : r;
+}
+
+// ------------------------------------------------ Householder QR (square)
+
+// per-wave work arrays of the linear algebra
+template <int K> struct LinWork {
+  Arr<K> q, r, v, w1, w2, wrk, sc;  // sc[0] = beta, sc[1] = v0, sc[2] = 1 when sigma = 0
+};
+
+// Q R = A for the n x n matrix given by the accessor a(r, c)
+// (CPU_dbl2_factors_houseqr with CPU_dbl2_factors_house, leftRupdate and
+// rightQupdate, Matrices/dbl2_factorizations.cpp:374-440, 534-566, 629-660)
+template <int K, class A> __device__ void houseqr(int n, const A &a, const LinWork<K> &w) {
+  typedef typename RealT<K>::type T;
+  const int l = lane_id();
+  for (int t = l; t < n * n; t += 64) {
+    const int i = t / n, j = t % n;
+    w.q.st(t, R<K>::from(i == j ? 1.0 : 0.0));
+    w.r.st(t, a(i, j));
+  }
+  wsync();
+  for (int k = 0; k < n; ++k) {
+    const int m = n - k;
+    // Householder vector of x = R[k..n-1][k]: sigma, mu, v0, beta in lane 0
+    if (l == 0) {
+      T sigma = R<K>::zero();
+      for (int i = 1; i < m; ++i) sigma = add(sigma, sqr(w.r.ld((k + i) * n + k)));
+      const bool zero = all_zero(sigma);
+      w.sc.st(2, R<K>::from(zero ? 1.0 : 0.0));
+      if (zero) {
+        w.sc.st(0, R<K>::zero());
+        w.sc.st(1, R<K>::from(1.0));
+      } else {
+        const T x0 = w.r.ld(k * n + k);
+        T acc = sqr(x0);
+        acc = add(acc, sigma);
+        const T mu = R<K>::sqrt(acc);
+        T v0;
+        if (R<K>::hi(x0) <= 0.0) {
+          v0 = sub(x0, mu);
+        } else {
+          acc = add(x0, mu);
+          v0 = neg(div(sigma, acc));
+        }
+        const T v0p2 = sqr(v0);
+        acc = add(sigma, v0p2);
+        T beta = div(v0p2, acc);
+        beta = mul_d(beta, 2.0);
+        w.sc.st(0, beta);
+        w.sc.st(1, v0);
+      }
+    }
+    wsync();
+    {
+      const T v0 = w.sc.ld(1);
+      const bool szero = R<K>::hi(w.sc.ld(2)) != 0.0;
+      // v[i] = x[i] / v0 (x[i] unchanged when sigma = 0), v[0] = 1
+      for (int i = l; i < m; i += 64) {
+        if (i == 0) w.v.st(0, R<K>::from(1.0));
+        else {
+          const T xi = w.r.ld((k + i) * n + k);
+          w.v.st(i, szero ? xi : div(xi, v0));
+        }
+      }
+    }
+    wsync();
+    const T beta = w.sc.ld(0);
+    // w1[j-k] = beta * sum_i R[i][j] v[i-k]
+    for (int j = k + l; j < n; j += 64) {
+      T s = R<K>::zero();
+      for (int i = k; i < n; ++i) s = add(s, mul(w.r.ld(i * n + j), w.v.ld(i - k)));
+      w.w1.st(j - k, mul(beta, s));
+    }
+    // w2[i] = beta * sum_{j>=k} Q[i][j] v[j-k]
+    for (int i = l; i < n; i += 64) {
+      T s = R<K>::zero();
+      for (int j = k; j < n; ++j) s = add(s, mul(w.q.ld(i * n + j), w.v.ld(j - k)));
+      w.w2.st(i, mul(beta, s));
+    }
+    wsync();
+    // R[i][j] -= v[i-k] w1[j-k];  Q[i][j] -= w2[i] v[j-k]
+    for (int t = l; t < m * m; t += 64) {
+      const int i = k + t / m, j = k + t % m;
+      w.r.st(i * n + j, sub(w.r.ld(i * n + j), mul(w.v.ld(i - k), w.w1.ld(j - k))));
+    }
+    for (int t = l; t < n * m; t += 64) {
+      const int i = t / m, j = k + t % m;
+      w.q.st(i * n + j, sub(w.q.ld(i * n + j), mul(w.w2.ld(i), w.v.ld(j - k))));
+    }
+    wsync();
+  }
+}
+
+// x = R^{-1} Q^T b (CPU_dbl2_factors_qrbs and _backward): b = bv[bo..],
+// x = xv[xo..]; b and x may be the same row.
+template <int K>
+__device__ void qrbs(int n, const LinWork<K> &w, const Arr<K> &bv, int bo, const Arr<K> &xv,
+                     int xo) {
+  typedef typename RealT<K>::type T;
+  const int l = lane_id();
+  for (int i = l; i < n; i += 64) {
+    T s = R<K>::zero();
+    for (int j = 0; j < n; ++j) s = add(s, mul(w.q.ld(j * n + i), bv.ld(bo + j)));
+    w.wrk.st(i, s);
+  }
+  wsync();
+  for (int i = l; i < n; i += 64) xv.st(xo + i, w.wrk.ld(i));
+  wsync();
+  // column sweep: row i receives the terms j = n-1 .. i+1 in that order
+  for (int j = n - 1; j >= 0; --j) {
+    if (l == (j & 63)) xv.st(xo + j, div(xv.ld(xo + j), w.r.ld(j * n + j)));
+    wsync();
+    const T xj = xv.ld(xo + j);
+    for (int i = l; i < j; i += 64) xv.st(xo + i, sub(xv.ld(xo + i), mul(w.r.ld(i * n + j), xj)));
+    wsync();
+  }
+}
+
+struct Flags {
+  int zeroQ, noqr, up, bs, newtail;
+};
+
+// lower block-triangular Toeplitz solve sum_{j<=k} A_j x_{k-j} = b_k
+// (CPU_dbl2_qrbs_solve with _head and _tail); mat(j, r, c) = A_j[r][c];
+// rhs and sol rows k at offsets k*n.  Uniform control flow.
+template <int K, class M>
+__device__ void qrbs_solve(int n, int degp1, int tailidx, const M &mat, const Arr<K> &rhs,
+                           const Arr<K> &sol, const LinWork<K> &w, Flags &f) {
+  typedef typename RealT<K>::type T;
+  const int l = lane_id();
+  const double thr = threshold<K>();
+  if (!(f.noqr && !f.zeroQ)) {
+    const double nrm = maxnorm<K>(rhs, 0, n);
+    const bool small = (K == 1) ? (nrm + 1.0 == 1.0) : (nrm < thr);
+    if (small) f.noqr = f.zeroQ ? 0 : 1;
+    if (!f.noqr) {
+      houseqr<K>(n, [&](int r, int c) { return mat(0, r, c); }, w);
+      f.zeroQ = 0;
+      qrbs<K>(n, w, rhs, 0, sol, 0);
+    }
+  }
+  if (degp1 > 1) {
+    int skipup = 0, skipbs = 0;
+    bool firstbs = true;
+    f.newtail = degp1;
+    for (int i = tailidx; i < degp1; ++i) {
+      if (maxnorm<K>(sol, (i - 1) * n, n) < thr) {
+        ++skipup;
+      } else {
+        const int npair = (degp1 - i) * n;
+        for (int t = l; t < npair; t += 64) {
+          const int j = i + t / n, k = t % n;
+          T b = rhs.ld(j * n + k);
+          for (int L = 0; L < n; ++L) b = sub(b, mul(mat(j - i + 1, k, L), sol.ld((i - 1) * n + L)));
+          rhs.st(j * n + k, b);
+        }
+      }
+      wsync();
+      const double nrm = maxnorm<K>(rhs, i * n, n);
+      if (nrm < thr || nrm > 1.0e+99) {
+        ++skipbs;
+        for (int j = l; j < n; j += 64) sol.st(i * n + j, R<K>::zero());
+        wsync();
+      } else {
+        if (firstbs) {
+          f.newtail = i;
+          firstbs = false;
+        }
+        qrbs<K>(n, w, rhs, i * n, sol, i * n);
+      }
+    }
+    f.up = skipup;
+    f.bs = skipbs;
+  }
+}
+
+// ---------------------------------------------- polynomial system values
+
+// the supports of a row system (shared by every system of a batch)
+struct RowSys {
+  int dim, nmono, nvmax, nbrmax;
+  const int *monoptr, *nvr, *idxptr, *idx;
+};
+
+// coefficient k of x*y: x_0 y_k, then += x_i y_{k-i} (CPU_dbl2_product,
+// Convolutions/dbl2_convolutions_host.cpp:7-26)
+template <int K, class FX, class FY>
+__device__ __forceinline__ typename RealT<K>::type cprod(const FX &x, const FY &y, int k) {
+  typename RealT<K>::type acc = mul(x(0), y(k));
+  for (int i = 1; i <= k; ++i) acc = add(acc, mul(x(i), y(k - i)));
+  return acc;
+}
+
+// value and gradient series of every polynomial at X (working degree Dw-1):
+// out row (p*(dim+1)+j) = d p / d x_j, row (p*(dim+1)+dim) = p, as
+// CPU_dbl2_poly_evaldiff with CPU_dbl2_poly_speel and CPU_dbl2_speel
+// (Polynomials/dbl2_polynomials_host.cpp, Convolutions/dbl2_monomials_host.cpp:28-87).
+// Slot s (Dp lanes) evaluates polynomials s, s + nslot, ...; lane k of a slot
+// owns coefficient k.  cst/cff planes: cst + p*D, cff + m*D with plane strides.
+template <int K>
+__device__ void eval_system(const RowSys &S, int D, int Dw, int Dp, const double *cst,
+                            size_t cstride, const double *cff, size_t fstride, const Arr<K> &X,
+                            const Arr<K> &OUT, const Arr<K> &FW, const Arr<K> &BW,
+                            const Arr<K> &CR) {
+  typedef typename RealT<K>::type T;
+  const int l = lane_id(), nslot = 64 / Dp, s = l / Dp, k = l % Dp;
+  const int dim = S.dim, D1 = dim + 1;
+  const int eo = s * S.nvmax * D;  // this slot's work rows
+  auto xin = [&](int var) { return [&X, var, D](int j) { return X.ld(var * D + j); }; };
+  auto fw = [&](int r) { return [&FW, eo, r, D](int j) { return FW.ld(eo + r * D + j); }; };
+  auto bw = [&](int r) { return [&BW, eo, r, D](int j) { return BW.ld(eo + r * D + j); }; };
+  for (int r0 = 0; r0 < dim; r0 += nslot) {
+    const int p = r0 + s;
+    const bool on = (p < dim) && (k < Dw);
+    if (on) {
+      OUT.st((p * D1 + dim) * D + k, R<K>::load(cst, cstride, (size_t)p * D + k));
+      for (int j = 0; j < dim; ++j) OUT.st((p * D1 + j) * D + k, R<K>::zero());
+    }
+    const int nb = (p < dim) ? S.monoptr[p + 1] - S.monoptr[p] : 0;
+    for (int mi = 0; mi < S.nbrmax; ++mi) {
+      const bool act = on && mi < nb;
+      const int mono = act ? S.monoptr[p] + mi : 0;
+      const int nv = act ? S.nvr[mono] : 0;
+      const int *ix = S.idx + (act ? S.idxptr[mono] : 0);
+      auto cf = [cff, fstride, mono, D](int j) { return R<K>::load(cff, fstride, (size_t)mono * D + j); };
+      auto addto = [&](int row, const T &v) {
+        const int o = (p * D1 + row) * D + k;
+        OUT.st(o, add(OUT.ld(o), v));
+      };
+      if (nv == 1) {
+        const T f = cprod<K>(xin(ix[0]), cf, k);
+        addto(dim, f);
+        addto(ix[0], cf(k));
+      } else if (nv == 2) {
+        const T f0 = cprod<K>(cf, xin(ix[0]), k);
+        FW.st(eo + k, f0);
+        addto(ix[1], f0);
+        const T b0 = cprod<K>(cf, xin(ix[1]), k);
+        addto(ix[0], b0);
+        wsync();
+        const T f1 = cprod<K>(fw(0), xin(ix[1]), k);
+        addto(dim, f1);
+      } else if (nv > 2) {
+        // forward and backward chains advance together
+        FW.st(eo + k, cprod<K>(cf, xin(ix[0]), k));
+        BW.st(eo + k, cprod<K>(xin(ix[nv - 1]), xin(ix[nv - 2]), k));
+        wsync();
+        for (int j = 1; j < nv; ++j) {
+          FW.st(eo + j * D + k, cprod<K>(fw(j - 1), xin(ix[j]), k));
+          if (j < nv - 2) BW.st(eo + j * D + k, cprod<K>(bw(j - 1), xin(ix[nv - 2 - j]), k));
+          wsync();
+        }
+        const T blast = cprod<K>(bw(nv - 3), cf, k);
+        wsync();
+        BW.st(eo + (nv - 3) * D + k, blast);
+        wsync();
+        if (nv == 3) {
+          CR.st(eo + k, cprod<K>(fw(0), xin(ix[2]), k));
+        } else {
+          for (int j = 0; j < nv - 3; ++j) CR.st(eo + j * D + k, cprod<K>(fw(j), bw(nv - 4 - j), k));
+          CR.st(eo + (nv - 3) * D + k, cprod<K>(fw(nv - 3), xin(ix[nv - 1]), k));
+        }
+        addto(dim, FW.ld(eo + (nv - 1) * D + k));
+        addto(ix[nv - 1], FW.ld(eo + (nv - 2) * D + k));
+        addto(ix[0], BW.ld(eo + (nv - 3) * D + k));
+        for (int j = 1; j < nv - 1; ++j) addto(ix[j], CR.ld(eo + (j - 1) * D + k));
+      }
+      wsync();
+    }
+  }
+  wsync();
+}
+
+// ------------------------------------------------------------ kernels
+
+struct NewtonLayout {
+  int X, OUT, RHS, SOL, Q, R, V, W1, W2, WRK, SC, FW, BW, CR;  // sizes in numbers
+  __host__ __device__ size_t bytes(int K) const {
+    return sizeof(double) * (size_t)K *
+           ((size_t)X + OUT + RHS + SOL + Q + R + V + W1 + W2 + WRK + SC + FW + BW + CR);
+  }
+};
+
+__host__ __device__ inline int pow2_at_least(int d) {
+  int p = 1;
+  while (p < d) p <<= 1;
+  return p;
+}
+
+__host__ __device__ inline NewtonLayout newton_layout(const RowSys &S, int D) {
+  const int n = S.dim, nslot = 64 / pow2_at_least(D), ev = nslot * (S.nvmax > 0 ? S.nvmax : 1) * D;
+  NewtonLayout L;
+  L.X = n * D; L.OUT = n * (n + 1) * D; L.RHS = D * n; L.SOL = D * n;
+  L.Q = n * n; L.R = n * n; L.V = n; L.W1 = n; L.W2 = n; L.WRK = n; L.SC = 3;
+  L.FW = ev; L.BW = ev; L.CR = ev;
+  return L;
+}
+
+template <int K>
+__device__ LinWork<K> carve_lin(double *&p, int n) {
+  LinWork<K> w;
+  w.q = carve<K>(p, n * n); w.r = carve<K>(p, n * n); w.v = carve<K>(p, n);
+  w.w1 = carve<K>(p, n); w.w2 = carve<K>(p, n); w.wrk = carve<K>(p, n); w.sc = carve<K>(p, 3);
+  return w;
+}
+
+// x [K][nsys][dim][D] in/out; cst [K][nc][dim][D], cff [K][nf][nmono][D] with
+// nc, nf = nsys or 1 (shared); logs [nsys][nbsteps]; hist
+// [nbsteps][K][nsys][dim][D]; Q, R [K][nsys][dim][dim].
+template <int K>
+__global__ void __launch_bounds__(512)
+row_newton_kernel(RowSys S, int deg, int nbsteps, int nsys, const double *cst, int cshared,
+                  const double *cff, int fshared, double *x, int *steps_out, int *wdeg_out,
+                  int *tail_out, int *up_out, int *bs_out, double *hist, double *Qo, double *Ro) {
+  typedef typename RealT<K>::type T;
+  extern __shared__ double lds[];
+  const int D = deg + 1, n = S.dim, Dp = pow2_at_least(D);
+  const NewtonLayout L = newton_layout(S, D);
+  const int wave = threadIdx.x >> 6, l = lane_id();
+  const int sysid = blockIdx.x * (blockDim.x >> 6) + wave;
+  if (sysid >= nsys) return;
+  double *p = lds + (size_t)wave * (L.bytes(K) / sizeof(double));
+  const Arr<K> X = carve<K>(p, L.X), OUT = carve<K>(p, L.OUT), RHS = carve<K>(p, L.RHS),
+               SOL = carve<K>(p, L.SOL);
+  const LinWork<K> w = carve_lin<K>(p, n);
+  const Arr<K> FW = carve<K>(p, L.FW), BW = carve<K>(p, L.BW), CR = carve<K>(p, L.CR);
+  const size_t xs = (size_t)nsys * n * D, xb = (size_t)sysid * n * D;
+  const size_t cs = (size_t)(cshared ? 1 : nsys) * n * D;
+  const size_t fs = (size_t)(fshared ? 1 : nsys) * S.nmono * D;
+  const double *cbase = cst + (cshared ? 0 : (size_t)sysid * n * D);
+  const double *fbase = cff + (fshared ? 0 : (size_t)sysid * S.nmono * D);
+  for (int t = l; t < n * D; t += 64) X.st(t, R<K>::load(x, xs, xb + t));
+  wsync();
+  const int D1 = n + 1;
+  auto jac = [&](int j, int r, int c) { return OUT.ld((r * D1 + c) * D + j); };
+  Flags f{1, 0, 0, 0, 1};
+  int tailidx = 1, wrkdeg = 0, stepcnt = 0;
+  for (int step = 0; step < nbsteps; ++step) {
+    const int Dw = wrkdeg + 1;
+    eval_system<K>(S, D, Dw, Dp, cbase, cs, fbase, fs, X, OUT, FW, BW, CR);
+    for (int t = l; t < Dw * n; t += 64) {
+      const int kk = t / n, i = t % n;
+      RHS.st(t, neg(OUT.ld((i * D1 + n) * D + kk)));
+      SOL.st(t, R<K>::zero());
+    }
+    wsync();
+    f.newtail = tailidx;
+    qrbs_solve<K>(n, Dw, tailidx, jac, RHS, SOL, w, f);
+    tailidx = f.newtail;
+    for (int t = l; t < n * Dw; t += 64) {
+      const int i = t / Dw, kk = t % Dw;
+      if (kk >= tailidx - 1) X.st(i * D + kk, add(X.ld(i * D + kk), SOL.ld(kk * n + i)));
+    }
+    wsync();
+    if (l == 0) {
+      const size_t o = (size_t)sysid * nbsteps + step;
+      if (wdeg_out) wdeg_out[o] = wrkdeg;
+      if (tail_out) tail_out[o] = tailidx;
+      if (up_out) up_out[o] = f.up;
+      if (bs_out) bs_out[o] = f.bs;
+    }
+    if (hist) {
+      double *h = hist + (size_t)step * K * xs;
+      for (int t = l; t < n * D; t += 64) R<K>::store(h, xs, xb + t, X.ld(t));
+    }
+    ++stepcnt;
+    if (tailidx >= deg) break;
+    wrkdeg = wrkdeg + 1 + wrkdeg / 2;
+    if (wrkdeg > deg) wrkdeg = deg;
+  }
+  for (int t = l; t < n * D; t += 64) R<K>::store(x, xs, xb + t, X.ld(t));
+  const size_t qs = (size_t)nsys * n * n, qb = (size_t)sysid * n * n;
+  for (int t = l; t < n * n; t += 64) {
+    if (Qo) R<K>::store(Qo, qs, qb + t, w.q.ld(t));
+    if (Ro) R<K>::store(Ro, qs, qb + t, w.r.ld(t));
+  }
+  if (l == 0 && steps_out) steps_out[sysid] = stepcnt;
+}
+
+// batched Householder QR of n x n matrices A [K][nb][n][n] -> Q, R
+template <int K>
+__global__ void __launch_bounds__(512)
+houseqr_kernel(int n, int nb, const double *A, double *Qo, double *Ro) {
+  extern __shared__ double lds[];
+  const int wave = threadIdx.x >> 6, l = lane_id();
+  const int b = blockIdx.x * (blockDim.x >> 6) + wave;
+  if (b >= nb) return;
+  const size_t per = (size_t)K * (2 * n * n + 4 * n + 3);
+  double *p = lds + (size_t)wave * per;
+  const LinWork<K> w = carve_lin<K>(p, n);
+  const size_t s = (size_t)nb * n * n, o = (size_t)b * n * n;
+  houseqr<K>(n, [&](int r, int c) { return R<K>::load(A, s, o + (size_t)r * n + c); }, w);
+  for (int t = l; t < n * n; t += 64) {
+    R<K>::store(Qo, s, o + t, w.q.ld(t));
+    R<K>::store(Ro, s, o + t, w.r.ld(t));
+  }
+}
+
+// batched Toeplitz solve: mat [K][nb][degp1][n][n], rhs/sol [K][nb][degp1][n]
+// (in/out), Q/R [K][nb][n][n] (in/out), flags [nb][5] (zeroQ, noqr, up, bs,
+// newtail; in/out)
+template <int K>
+__global__ void __launch_bounds__(512)
+qrbs_solve_kernel(int n, int degp1, int tailidx, int nb, const double *mat, double *rhs,
+                  double *sol, double *Qio, double *Rio, int *flags) {
+  extern __shared__ double lds[];
+  const int wave = threadIdx.x >> 6, l = lane_id();
+  const int b = blockIdx.x * (blockDim.x >> 6) + wave;
+  if (b >= nb) return;
+  const size_t per = (size_t)K * (2 * n * n + 4 * n + 3 + 2 * degp1 * n);
+  double *p = lds + (size_t)wave * per;
+  const LinWork<K> w = carve_lin<K>(p, n);
+  const Arr<K> RHS = carve<K>(p, degp1 * n), SOL = carve<K>(p, degp1 * n);
+  const size_t vs = (size_t)nb * degp1 * n, vo = (size_t)b * degp1 * n;
+  const size_t qs = (size_t)nb * n * n, qo = (size_t)b * n * n;
+  const size_t ms = (size_t)nb * degp1 * n * n, mo = (size_t)b * degp1 * n * n;
+  for (int t = l; t < degp1 * n; t += 64) {
+    RHS.st(t, R<K>::load(rhs, vs, vo + t));
+    SOL.st(t, R<K>::load(sol, vs, vo + t));
+  }
+  for (int t = l; t < n * n; t += 64) {
+    w.q.st(t, R<K>::load(Qio, qs, qo + t));
+    w.r.st(t, R<K>::load(Rio, qs, qo + t));
+  }
+  wsync();
+  Flags f{flags[b * 5], flags[b * 5 + 1], flags[b * 5 + 2], flags[b * 5 + 3], flags[b * 5 + 4]};
+  auto m = [&](int j, int r, int c) { return R<K>::load(mat, ms, mo + ((size_t)j * n + r) * n + c); };
+  qrbs_solve<K>(n, degp1, tailidx, m, RHS, SOL, w, f);
+  for (int t = l; t < degp1 * n; t += 64) {
+    R<K>::store(rhs, vs, vo + t, RHS.ld(t));
+    R<K>::store(sol, vs, vo + t, SOL.ld(t));
+  }
+  for (int t = l; t < n * n; t += 64) {
+    R<K>::store(Qio, qs, qo + t, w.q.ld(t));
+    R<K>::store(Rio, qs, qo + t, w.r.ld(t));
+  }
+  if (l == 0) {
+    flags[b * 5] = f.zeroQ; flags[b * 5 + 1] = f.noqr; flags[b * 5 + 2] = f.up;
+    flags[b * 5 + 3] = f.bs; flags[b * 5 + 4] = f.newtail;
+  }
+}
+
+// r_i = b_i - sum_{j <= i-tailidx} A_j x_{i-j} for i >= tailidx
+// (CPU_dbl2_linear_residue, Newton/dbl2_bals_host.cpp:548-600); one lane per
+// residual entry; resmax [K][nb] = entry of largest |leading limb|, the first
+// such entry in the reference's scan order.
+template <int K>
+__global__ void __launch_bounds__(512)
+linear_residue_kernel(int n, int degp1, int tailidx, int nb, const double *mat,
+                      const double *rhs, const double *sol, double *res, double *resmax) {
+  typedef typename RealT<K>::type T;
+  const int wave = threadIdx.x >> 6, l = lane_id();
+  const int b = blockIdx.x * (blockDim.x >> 6) + wave;
+  if (b >= nb) return;
+  const size_t vs = (size_t)nb * degp1 * n, vo = (size_t)b * degp1 * n;
+  const size_t ms = (size_t)nb * degp1 * n * n, mo = (size_t)b * degp1 * n * n;
+  double best = 0.0;
+  int bidx = -1;
+  for (int t = tailidx * n + l; t < degp1 * n; t += 64) {
+    const int i = t / n, L = t % n;
+    T r = R<K>::load(rhs, vs, vo + t);
+    for (int j = 0; j <= i - tailidx; ++j)
+      for (int k = 0; k < n; ++k)
+        r = sub(r, mul(R<K>::load(mat, ms, mo + ((size_t)j * n + L) * n + k),
+                       R<K>::load(sol, vs, vo + (size_t)(i - j) * n + k)));
+    R<K>::store(res, vs, vo + t, r);
+    const double a = fabs(R<K>::hi(r));
+    if (a > best) { best = a; bidx = t; }
+  }
+  // the scan keeps the first strictly larger entry: largest value, then the
+  // smallest index among equal values
+  for (int m = 1; m < 64; m <<= 1) {
+    const double ob = __shfl_xor(best, m, 64);
+    const int oi = __shfl_xor(bidx, m, 64);
+    if (ob > best || (ob == best && oi >= 0 && (bidx < 0 || oi < bidx))) { best = ob; bidx = oi; }
+  }
+  if (l == 0) {
+    T r = R<K>::zero();
+    if (bidx >= 0) r = R<K>::load(res, vs, vo + bidx);
+#pragma unroll
+    for (int i = 0; i < K; ++i) resmax[(size_t)i * nb + b] = fabs(limb(r, i));
+  }
+}
+
+int waves_for(size_t per_wave, int *waves) {
+  int dev = 0, ldsmax = 0;
+  PHC_HIP(hipGetDevice(&dev));
+  PHC_HIP(hipDeviceGetAttribute(&ldsmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  const size_t cap = (size_t)ldsmax < LDS_LIMIT ? (size_t)ldsmax : LDS_LIMIT;
+  PHC_REQUIRE(per_wave <= cap, "problem too large for the LDS of one workgroup");
+  int wv = (int)(cap / per_wave);
+  if (wv > 8) wv = 8;
+  *waves = wv;
+  return 0;
+}
+
+}  // namespace
+
+struct phc_rowsys {
+  RowSys S;
+  int *dmem;
+};
+
+extern "C" {
+
+int phc_rowsys_create(phc_rowsys **out, int dim, const int *monoptr, const int *nvr,
+                      const int *idxptr, const int *idx) {
+  PHC_REQUIRE(out && monoptr && nvr && idxptr, "phc_rowsys_create: null argument");
+  PHC_REQUIRE(dim >= 1 && dim <= 64, "phc_rowsys_create: dim must be in 1..64");
+  const int nmono = monoptr[dim];
+  PHC_REQUIRE(monoptr[0] == 0 && nmono >= 0, "phc_rowsys_create: bad monomial pointers");
+  int nvmax = 0, nbrmax = 0;
+  for (int i = 0; i < dim; ++i) {
+    PHC_REQUIRE(monoptr[i + 1] >= monoptr[i], "phc_rowsys_create: bad monomial pointers");
+    if (monoptr[i + 1] - monoptr[i] > nbrmax) nbrmax = monoptr[i + 1] - monoptr[i];
+  }
+  PHC_REQUIRE(idxptr[0] == 0, "phc_rowsys_create: bad index pointers");
+  for (int m = 0; m < nmono; ++m) {
+    PHC_REQUIRE(nvr[m] >= 1 && nvr[m] <= dim && idxptr[m + 1] - idxptr[m] == nvr[m],
+                "phc_rowsys_create: monomial variable counts and index pointers disagree");
+    for (int j = idxptr[m]; j < idxptr[m + 1]; ++j)
+      PHC_REQUIRE(idx && idx[j] >= 0 && idx[j] < dim, "phc_rowsys_create: variable index out of range");
+    if (nvr[m] > nvmax) nvmax = nvr[m];
+  }
+  const int nidx = idxptr[nmono];
+  const size_t ints = (size_t)(dim + 1) + 2 * (size_t)nmono + 1 + nidx;
+  phc_rowsys *s = new phc_rowsys();
+  if (hipMalloc(&s->dmem, sizeof(int) * (ints > 0 ? ints : 1)) != hipSuccess) {
+    delete s;
+    phc::set_error("phc_rowsys_create: hipMalloc failed");
+    return -1;
+  }
+  int *p = s->dmem;
+  std::vector<int> h;
+  h.insert(h.end(), monoptr, monoptr + dim + 1);
+  h.insert(h.end(), nvr, nvr + nmono);
+  h.insert(h.end(), idxptr, idxptr + nmono + 1);
+  if (nidx > 0) h.insert(h.end(), idx, idx + nidx);
+  PHC_HIP(hipMemcpy(p, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
+  s->S.dim = dim; s->S.nmono = nmono; s->S.nvmax = nvmax; s->S.nbrmax = nbrmax;
+  s->S.monoptr = p; s->S.nvr = p + dim + 1; s->S.idxptr = p + dim + 1 + nmono;
+  s->S.idx = p + dim + 1 + 2 * nmono + 1;
+  *out = s;
+  return 0;
+}
+
+int phc_rowsys_destroy(phc_rowsys *s) {
+  if (!s) return 0;
+  hipError_t e = hipFree(s->dmem);
+  delete s;
+  if (e != hipSuccess) {
+    phc::set_error("phc_rowsys_destroy: hipFree failed");
+    return -1;
+  }
+  return 0;
+}
+
+int phc_rowsys_lds_bytes(const phc_rowsys *s, int K, int deg, size_t *bytes) {
+  PHC_REQUIRE(s && bytes, "phc_rowsys_lds_bytes: null argument");
+  PHC_REQUIRE(deg >= 0 && deg <= 63, "phc_rowsys_lds_bytes: deg must be in 0..63");
+  *bytes = newton_layout(s->S, deg + 1).bytes(K);
+  return 0;
+}
+
+int phc_row_newton(phc_rowsys *s, int K, int deg, int nbsteps, int nsys, const double *cst,
+                   int cst_shared, const double *cff, int cff_shared, double *x, int *steps,
+                   int *wdeg, int *tail, int *up, int *bs, double *history, double *Q, double *R,
+                   void *stream) {
+  PHC_REQUIRE(s, "phc_row_newton: null system");
+  PHC_REQUIRE(K == 1 || K == 2 || K == 4, "phc_row_newton: K must be 1, 2 or 4");
+  PHC_REQUIRE(deg >= 0 && deg <= 63, "phc_row_newton: deg must be in 0..63");
+  PHC_REQUIRE(nbsteps >= 0 && nsys >= 0, "phc_row_newton: negative size");
+  PHC_REQUIRE(cst && cff && x, "phc_row_newton: null coefficient or series array");
+  if (nsys == 0 || nbsteps == 0) return 0;
+  const size_t per = newton_layout(s->S, deg + 1).bytes(K);
+  int wv = 0;
+  if (waves_for(per, &wv) != 0) return -2;
+  const int grid = (nsys + wv - 1) / wv;
+  const size_t lds = per * wv;
+  hipStream_t st = (hipStream_t)stream;
+#define LAUNCH(KK)                                                                            \
+  do {                                                                                        \
+    PHC_HIP(hipFuncSetAttribute((const void *)row_newton_kernel<KK>,                         \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));       \
+    hipLaunchKernelGGL(row_newton_kernel<KK>, dim3(grid), dim3(64 * wv), lds, st, s->S, deg,  \
+                       nbsteps, nsys, cst, cst_shared, cff, cff_shared, x, steps, wdeg, tail, \
+                       up, bs, history, Q, R);                                                \
+  } while (0)
+  if (K == 1) LAUNCH(1);
+  else if (K == 2) LAUNCH(2);
+  else LAUNCH(4);
+#undef LAUNCH
+  PHC_HIP(hipGetLastError());
+  return 0;
+}
+
+int phc_houseqr_batch(int K, int n, int nbatch, const double *A, double *Q, double *R,
+                      void *stream) {
+  PHC_REQUIRE(K == 1 || K == 2 || K == 4, "phc_houseqr_batch: K must be 1, 2 or 4");
+  PHC_REQUIRE(n >= 1 && n <= 64 && nbatch >= 0, "phc_houseqr_batch: n must be in 1..64");
+  PHC_REQUIRE(A && Q && R, "phc_houseqr_batch: null array");
+  if (nbatch == 0) return 0;
+  const size_t per = sizeof(double) * K * (2 * (size_t)n * n + 4 * n + 3);
+  int wv = 0;
+  if (waves_for(per, &wv) != 0) return -2;
+  const int grid = (nbatch + wv - 1) / wv;
+  hipStream_t st = (hipStream_t)stream;
+#define LAUNCH(KK)                                                                            \
+  do {                                                                                        \
+    PHC_HIP(hipFuncSetAttribute((const void *)houseqr_kernel<KK>,                            \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)(per * wv))); \
+    hipLaunchKernelGGL(houseqr_kernel<KK>, dim3(grid), dim3(64 * wv), per * wv, st, n, nbatch, \
+                       A, Q, R);                                                              \
+  } while (0)
+  if (K == 1) LAUNCH(1);
+  else if (K == 2) LAUNCH(2);
+  else LAUNCH(4);
+#undef LAUNCH
+  PHC_HIP(hipGetLastError());
+  return 0;
+}
+
+int phc_qrbs_solve_batch(int K, int n, int degp1, int tailidx, int nbatch, const double *mat,
+                         double *rhs, double *sol, double *Q, double *R, int *flags,
+                         void *stream) {
+  PHC_REQUIRE(K == 1 || K == 2 || K == 4, "phc_qrbs_solve_batch: K must be 1, 2 or 4");
+  PHC_REQUIRE(n >= 1 && n <= 64 && degp1 >= 1 && degp1 <= 64 && nbatch >= 0,
+              "phc_qrbs_solve_batch: n and degp1 must be in 1..64");
+  PHC_REQUIRE(tailidx >= 1, "phc_qrbs_solve_batch: tailidx must be at least 1");
+  PHC_REQUIRE(mat && rhs && sol && Q && R && flags, "phc_qrbs_solve_batch: null array");
+  if (nbatch == 0) return 0;
+  const size_t per = sizeof(double) * K * (2 * (size_t)n * n + 4 * n + 3 + 2 * (size_t)degp1 * n);
+  int wv = 0;
+  if (waves_for(per, &wv) != 0) return -2;
+  const int grid = (nbatch + wv - 1) / wv;
+  hipStream_t st = (hipStream_t)stream;
+#define LAUNCH(KK)                                                                            \
+  do {                                                                                        \
+    PHC_HIP(hipFuncSetAttribute((const void *)qrbs_solve_kernel<KK>,                         \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)(per * wv))); \
+    hipLaunchKernelGGL(qrbs_solve_kernel<KK>, dim3(grid), dim3(64 * wv), per * wv, st, n,      \
+                       degp1, tailidx, nbatch, mat, rhs, sol, Q, R, flags);                   \
+  } while (0)
+  if (K == 1) LAUNCH(1);
+  else if (K == 2) LAUNCH(2);
+  else LAUNCH(4);
+#undef LAUNCH
+  PHC_HIP(hipGetLastError());
+  return 0;
+}
+
+int phc_linear_residue_batch(int K, int n, int degp1, int tailidx, int nbatch,
+                             const double *mat, const double *rhs, const double *sol,
+                             double *res, double *resmax, void *stream) {
+  PHC_REQUIRE(K == 1 || K == 2 || K == 4, "phc_linear_residue_batch: K must be 1, 2 or 4");
+  PHC_REQUIRE(n >= 1 && degp1 >= 1 && nbatch >= 0 && tailidx >= 0,
+              "phc_linear_residue_batch: bad sizes");
+  PHC_REQUIRE(mat && rhs && sol && res && resmax, "phc_linear_residue_batch: null array");
+  if (nbatch == 0) return 0;
+  const int wv = 4, grid = (nbatch + wv - 1) / wv;
+  hipStream_t st = (hipStream_t)stream;
+  if (K == 1)
+    hipLaunchKernelGGL(linear_residue_kernel<1>, dim3(grid), dim3(64 * wv), 0, st, n, degp1,
+                       tailidx, nbatch, mat, rhs, sol, res, resmax);
+  else if (K == 2)
+    hipLaunchKernelGGL(linear_residue_kernel<2>, dim3(grid), dim3(64 * wv), 0, st, n, degp1,
+                       tailidx, nbatch, mat, rhs, sol, res, resmax);
+  else
+    hipLaunchKernelGGL(linear_residue_kernel<4>, dim3(grid), dim3(64 * wv), 0, st, n, degp1,
+                       tailidx, nbatch, mat, rhs, sol, res, resmax);
+  PHC_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"

@@ -55,6 +55,9 @@ constexpr int JT_TERM_BITS = 16, JT_PAIR_BITS = 12;
 // intact on an MI355X); the device attribute caps it further if lower.
 constexpr size_t LDS_LIMIT = 160 * 1024;
 constexpr size_t LDS_PER_CU = 160 * 1024;
+// wavefronts per workgroup: FP64 tracking may run 12 (a kernel variant
+// compiled for three waves per SIMD, at most 168 VGPRs); the others 8
+constexpr int TRACK_MAX_WAVES = 12, REFINE_MAX_WAVES = 8;
 
 // device copy of the program (global memory)
 struct DevProgram {
@@ -579,8 +582,8 @@ template <int K> __device__ int mlp_choose(const Ctl &ctl, Path<K> &w, const dou
   return best;
 }
 
-template <int K>
-__global__ void __launch_bounds__(512) track_kernel(DevProgram G, Params par, int npaths,
+template <int K, int MW>
+__global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, Params par, int npaths,
                                                    const double *__restrict__ start, int KS,
                                                    double *__restrict__ endp, double *tend,
                                                    int *status, int *nsteps_out, int *nrej_out,
@@ -703,7 +706,7 @@ __global__ void __launch_bounds__(512) track_kernel(DevProgram G, Params par, in
 // refined point, the last |dx|, the max residual |H(x,t)| after the last
 // update (same precision) and the status.
 template <int K>
-__global__ void __launch_bounds__(512) refine_kernel(DevProgram G, int npaths, double tval,
+__global__ void __launch_bounds__(64 * REFINE_MAX_WAVES) refine_kernel(DevProgram G, int npaths, double tval,
                                                     const double *__restrict__ xin, int KS,
                                                     double *__restrict__ xout, int iters,
                                                     double tol, double *dx_out, double *res_out,
@@ -767,7 +770,7 @@ size_t path_bytes(int K, const DevProgram &P) {
 }
 
 // launch geometry: one workgroup per CU with as many wavefronts as LDS holds
-int geometry(const DevProgram &P, int K, int max_waves, int npaths, int *grid, int *waves,
+int geometry(const DevProgram &P, int K, int max_waves, int kmax, int npaths, int *grid, int *waves,
              size_t *lds) {
   int dev = 0, ncu = 0, ldsmax = 0;
   PHC_HIP(hipGetDevice(&dev));
@@ -778,7 +781,7 @@ int geometry(const DevProgram &P, int K, int max_waves, int npaths, int *grid, i
   PHC_REQUIRE(prog + pb <= cap, "tracker program and one path do not fit in LDS");
   int wv = (int)((cap - prog) / pb);
   if (wv > max_waves) wv = max_waves;
-  if (wv > 8) wv = 8;
+  if (wv > kmax) wv = kmax;
   const size_t bytes = prog + (size_t)wv * pb;
   long per_cu = (long)(LDS_PER_CU / bytes);
   if (per_cu < 1) per_cu = 1;
@@ -928,7 +931,7 @@ int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, d
   PHC_REQUIRE(K == 1 || K == 2, "phc_track: tracking precision K must be 1 or 2");
   PHC_REQUIRE(KS >= 1 && KS <= 8, "phc_track: start limbs KS must be in 1..8");
   PHC_REQUIRE(npaths >= 0, "phc_track: npaths < 0");
-  PHC_REQUIRE(waves >= 1 && waves <= 8, "phc_track: waves per workgroup must be in 1..8");
+  PHC_REQUIRE(waves >= 1 && waves <= TRACK_MAX_WAVES, "phc_track: waves per workgroup must be in 1..12");
   PHC_REQUIRE(tr->P.D >= 2, "phc_track: the tracker program needs series degree >= 1");
   PHC_REQUIRE(prm->fabry_offset >= 0 && prm->fabry_offset < tr->P.D - 1,
               "phc_track: fabry_offset must be below the series degree");
@@ -938,7 +941,8 @@ int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, d
   hipStream_t st = (hipStream_t)stream;
   int grid = 0, wv = 0;
   size_t lds = 0;
-  if (geometry(tr->P, K, waves, npaths, &grid, &wv, &lds) != 0) return -2;
+  if (geometry(tr->P, K, waves, K == 1 ? TRACK_MAX_WAVES : 8, npaths, &grid, &wv, &lds) != 0)
+    return -2;
   Params par;
   par.maxsteps = prm->maxsteps; par.corsteps = prm->corsteps; par.endsteps = prm->endsteps;
   par.fabry_offset = prm->fabry_offset; par.maxstep = prm->maxstep; par.minstep = prm->minstep;
@@ -966,14 +970,20 @@ int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, d
     ctl.log_y = ctlp->log_y; ctl.log_n = ctlp->log_n;
   }
   PHC_HIP(hipMemsetAsync(tr->counter, 0, sizeof(int), st));
-  const void *fn = (K == 1) ? (const void *)track_kernel<1> : (const void *)track_kernel<2>;
-  PHC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  if (K == 1)
-    hipLaunchKernelGGL(track_kernel<1>, dim3(grid), dim3(64 * wv), lds, st, tr->P, par, npaths,
-                       start, KS, endp, tend, status, nsteps, nrej, dxlast, work, ctl, tr->counter);
-  else
-    hipLaunchKernelGGL(track_kernel<2>, dim3(grid), dim3(64 * wv), lds, st, tr->P, par, npaths,
-                       start, KS, endp, tend, status, nsteps, nrej, dxlast, work, ctl, tr->counter);
+  // up to 8 waves: the variant with 256 VGPRs per lane; 9 to 12 waves (FP64
+  // tracking only): the variant compiled for three waves per SIMD
+#define TK(KK, MW)                                                                            \
+  do {                                                                                        \
+    PHC_HIP(hipFuncSetAttribute((const void *)track_kernel<KK, MW>,                          \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));       \
+    hipLaunchKernelGGL((track_kernel<KK, MW>), dim3(grid), dim3(64 * wv), lds, st, tr->P, par, \
+                       npaths, start, KS, endp, tend, status, nsteps, nrej, dxlast, work, ctl, \
+                       tr->counter);                                                          \
+  } while (0)
+  if (K == 1 && wv > 8) TK(1, TRACK_MAX_WAVES);
+  else if (K == 1) TK(1, 8);
+  else TK(2, 8);
+#undef TK
   PHC_HIP(hipGetLastError());
   return 0;
 }
@@ -987,12 +997,12 @@ int phc_refine(phc_tracker *tr, int K, int npaths, double t, const double *xin, 
               "established)");
   PHC_REQUIRE(KS >= 1 && KS <= 8, "phc_refine: KS must be in 1..8");
   PHC_REQUIRE(npaths >= 0 && iters >= 0, "phc_refine: negative size");
-  PHC_REQUIRE(waves >= 1 && waves <= 8, "phc_refine: waves per workgroup must be in 1..8");
+  PHC_REQUIRE(waves >= 1 && waves <= REFINE_MAX_WAVES, "phc_refine: waves per workgroup must be in 1..8");
   if (npaths == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   int grid = 0, wv = 0;
   size_t lds = 0;
-  if (geometry(tr->P, K, waves, npaths, &grid, &wv, &lds) != 0) return -2;
+  if (geometry(tr->P, K, waves, REFINE_MAX_WAVES, npaths, &grid, &wv, &lds) != 0) return -2;
   PHC_HIP(hipMemsetAsync(tr->counter, 0, sizeof(int), st));
   const void *fn = K == 1 ? (const void *)refine_kernel<1> : K == 2 ? (const void *)refine_kernel<2>
                                                            : (const void *)refine_kernel<4>;

@@ -1,0 +1,129 @@
+"""Host side of the linearized power-series Newton method on the GPU.
+
+Mirrors the reference's host interface for this path:
+  houseqr        CPU_dbl2_factors_houseqr (Matrices/dbl2_factorizations.cpp:723)
+  qrbs_solve     CPU_dbl2_qrbs_solve (Newton/dbl2_bals_host.cpp:479) with its
+                 zeroQ / noqr / upidx / bsidx / newtail flags
+  linear_residue CPU_dbl2_linear_residue (Newton/dbl2_bals_host.cpp:548)
+  row_newton     dbl2_row_newton_qrstep (Newton/dbl2_newton_method.cpp:429)
+                 iterated with staggered degrees as test_dbl2_row_newton
+                 (:1447-1494)
+for K = 1, 2, 4 limbs (dbl, dbl2, dbl4), batched: every array carries a batch
+index after the limb index.  Arrays are numpy float64 in the limb-planar
+layout of include/phcmd.h; the work runs on the GPU through libphcmd.so.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import DeviceArray, P_int, check, load
+
+
+def _ip(a):
+    return np.ascontiguousarray(a, np.int32)
+
+
+def houseqr(K, A):
+    """Q, R of every n x n matrix of A [K][nbatch][n][n]"""
+    A = np.ascontiguousarray(A, np.float64)
+    _, nb, n, _ = A.shape
+    dA = DeviceArray.from_host(A)
+    dQ, dR = DeviceArray(A.shape), DeviceArray(A.shape)
+    check(load().phc_houseqr_batch(K, n, nb, dA.ptr, dQ.ptr, dR.ptr, None))
+    return dQ.to_host(), dR.to_host()
+
+
+def qrbs_solve(K, mat, rhs, tailidx, sol=None, Q=None, R=None, zeroQ=1, noqr=0):
+    """block-Toeplitz solve of every batch entry: mat [K][nb][degp1][n][n],
+    rhs [K][nb][degp1][n].  Returns dict(rhs, sol, Q, R, zeroQ, noqr, up, bs,
+    newtail) with the flags as int arrays [nb]."""
+    mat = np.ascontiguousarray(mat, np.float64)
+    _, nb, degp1, n, _ = mat.shape
+    vshape, qshape = (K, nb, degp1, n), (K, nb, n, n)
+    d = {k: DeviceArray.from_host(np.zeros(s) if v is None else np.ascontiguousarray(v, np.float64))
+         for k, v, s in (("rhs", rhs, vshape), ("sol", sol, vshape), ("Q", Q, qshape),
+                         ("R", R, qshape))}
+    flags = np.zeros((nb, 5), np.int32)
+    flags[:, 0], flags[:, 1], flags[:, 4] = zeroQ, noqr, tailidx
+    dm, df = DeviceArray.from_host(mat), DeviceArray.from_host(flags)
+    check(load().phc_qrbs_solve_batch(K, n, degp1, tailidx, nb, dm.ptr, d["rhs"].ptr,
+                                      d["sol"].ptr, d["Q"].ptr, d["R"].ptr, df.ptr, None))
+    f = df.to_host()
+    out = {k: v.to_host() for k, v in d.items()}
+    out.update(zeroQ=f[:, 0], noqr=f[:, 1], up=f[:, 2], bs=f[:, 3], newtail=f[:, 4])
+    return out
+
+
+def linear_residue(K, mat, rhs, sol, tailidx):
+    """residual vectors [K][nb][degp1][n] and the largest entries [K][nb]"""
+    mat = np.ascontiguousarray(mat, np.float64)
+    _, nb, degp1, n, _ = mat.shape
+    dm = DeviceArray.from_host(mat)
+    db, dx = DeviceArray.from_host(np.ascontiguousarray(rhs)), DeviceArray.from_host(np.ascontiguousarray(sol))
+    dr = DeviceArray.zeros((K, nb, degp1, n))
+    dmax = DeviceArray((K, nb))
+    check(load().phc_linear_residue_batch(K, n, degp1, tailidx, nb, dm.ptr, db.ptr, dx.ptr,
+                                          dr.ptr, dmax.ptr, None))
+    return dr.to_host(), dmax.to_host()
+
+
+class RowSystem:
+    """Supports of a polynomial system on the device (phc_rowsys)."""
+
+    def __init__(self, dim, monoptr, nvr, idxptr, idx):
+        self.dim = dim
+        self.monoptr, self.nvr = _ip(monoptr), _ip(nvr)
+        self.idxptr, self.idx = _ip(idxptr), _ip(idx if len(idx) else [0])
+        self.nmono = int(self.monoptr[-1])
+        h = ctypes.c_void_p()
+        check(load().phc_rowsys_create(ctypes.byref(h), dim, self.monoptr.ctypes.data_as(P_int),
+                                       self.nvr.ctypes.data_as(P_int),
+                                       self.idxptr.ctypes.data_as(P_int),
+                                       self.idx.ctypes.data_as(P_int)))
+        self.handle = h
+
+    def lds_bytes(self, K, deg):
+        b = ctypes.c_size_t()
+        check(load().phc_rowsys_lds_bytes(self.handle, K, deg, ctypes.byref(b)))
+        return b.value
+
+    def __del__(self):
+        try:
+            if self.handle:
+                load().phc_rowsys_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    def row_newton_device(self, K, deg, nbsteps, nsys, d_cst, cst_shared, d_cff, cff_shared, d_x,
+                          logs=None, stream=None):
+        """device-resident run; logs: dict of DeviceArrays (steps, wdeg, tail,
+        up, bs, history, Q, R) or None"""
+        g = (lambda k: logs[k].ptr if logs and k in logs else None)
+        check(load().phc_row_newton(self.handle, K, deg, nbsteps, nsys, d_cst.ptr, int(cst_shared),
+                                    d_cff.ptr, int(cff_shared), d_x.ptr, g("steps"), g("wdeg"),
+                                    g("tail"), g("up"), g("bs"), g("history"), g("Q"), g("R"),
+                                    stream))
+
+    def row_newton(self, K, deg, nbsteps, cst, cff, x, history=False):
+        """Staggered row Newton on a batch of systems with these supports.
+
+        cst [K][nc][dim][deg+1], cff [K][nf][nmono][deg+1] (nc, nf = 1 for
+        coefficients shared by the batch), x [K][nsys][dim][deg+1].
+        Returns dict(x, steps, wdeg, tail, up, bs, Q, R[, history])."""
+        x = np.ascontiguousarray(x, np.float64)
+        _, nsys, dim, D = x.shape
+        cst, cff = np.ascontiguousarray(cst, np.float64), np.ascontiguousarray(cff, np.float64)
+        dc, df, dx = DeviceArray.from_host(cst), DeviceArray.from_host(cff), DeviceArray.from_host(x)
+        logs = {k: DeviceArray.zeros((nsys, nbsteps), np.int32) for k in ("wdeg", "tail", "up", "bs")}
+        logs["steps"] = DeviceArray.zeros((nsys,), np.int32)
+        logs["Q"] = DeviceArray.zeros((K, nsys, dim, dim))
+        logs["R"] = DeviceArray.zeros((K, nsys, dim, dim))
+        if history:
+            logs["history"] = DeviceArray.zeros((nbsteps, K, nsys, dim, D))
+        self.row_newton_device(K, D - 1, nbsteps, nsys, dc, cst.shape[1] == 1 and nsys > 1, df,
+                               cff.shape[1] == 1 and nsys > 1, dx, logs)
+        check(load().phc_synchronize())
+        out = {k: v.to_host() for k, v in logs.items()}
+        out["x"] = dx.to_host()
+        return out

@@ -139,7 +139,7 @@ class Tracker:
         self.dev = DeviceProgram(self.prog)
         self.dev0 = DeviceProgram(self.prog0)
 
-    def track_device(self, d_start, KS, npaths, K, bufs, params=None, waves=8, stream=None,
+    def track_device(self, d_start, KS, npaths, K, bufs, params=None, waves=12, stream=None,
                      controller=None):
         """controller: a PhcController (or None for the plain pole-step rule)"""
         p = (params or TrackParams()).to_c()
@@ -148,7 +148,7 @@ class Tracker:
                                bufs.tend.ptr, bufs.status.ptr, bufs.nsteps.ptr, bufs.nrej.ptr,
                                bufs.dx.ptr, bufs.work.ptr, ctypes.byref(p), cptr, waves, stream))
 
-    def track(self, starts, K=2, params=None, waves=8, controller=None):
+    def track(self, starts, K=2, params=None, waves=12, controller=None):
         """starts: complex128 [npaths][n] (or [2KS][npaths][n] planes)."""
         planes = starts if starts.dtype == np.float64 else points_to_planes(starts)
         KS = planes.shape[0] // 2

@@ -1,0 +1,153 @@
+"""GPU parity of the power-series Newton kernels (phcpack_amd/csrc/newton.hip)
+against the CPU oracle (oracle/newton.c, itself pinned bit-exact to the
+reference's host code): bit-identical Q, R, solutions, flags, residuals and
+Newton series at K = 1, 2, 4, on batches of independent systems."""
+import os
+
+import numpy as np
+import pytest
+
+from oraclelib import (GOLDEN, RowSystem as ORowSystem, oracle_houseqr, oracle_linear_residue,
+                       oracle_qrbs_solve, oracle_row_newton, planted_row_system, rand_md,
+                       random_supports)
+
+pytestmark = pytest.mark.gpu
+KS = (1, 2, 4)
+
+
+@pytest.mark.parametrize("K", KS)
+@pytest.mark.parametrize("n", [1, 5, 8, 17])
+def test_houseqr_batch_matches_oracle(K, n):
+    from phcpack_amd.newton import houseqr
+    print(f"houseqr K={K} n={n}", flush=True)
+    rng = np.random.default_rng(10 * K + n)
+    nb = 5
+    A = rand_md(rng, K, (nb, n, n))
+    Q, R = houseqr(K, A)
+    for b in range(nb):
+        q, r = oracle_houseqr(K, np.ascontiguousarray(A[:, b]))
+        assert np.array_equal(Q[:, b], q) and np.array_equal(R[:, b], r), b
+
+
+@pytest.mark.parametrize("K", KS)
+@pytest.mark.parametrize("tailidx", [1, 2])
+def test_qrbs_solve_batch_matches_oracle(K, tailidx):
+    from phcpack_amd.newton import qrbs_solve
+    print(f"qrbs_solve K={K} tail={tailidx}", flush=True)
+    rng = np.random.default_rng(100 + K + tailidx)
+    nb, degp1, n = 4, 7, 6
+    mat = rand_md(rng, K, (nb, degp1, n, n))
+    rhs = rand_md(rng, K, (nb, degp1, n))
+    rhs[:, 1, 3] = 0.0      # a zero stage: back substitution skipped
+    g = qrbs_solve(K, mat, rhs, tailidx)
+    for b in range(nb):
+        o = oracle_qrbs_solve(K, np.ascontiguousarray(mat[:, b]), np.ascontiguousarray(rhs[:, b]),
+                              tailidx)
+        for k in ("rhs", "sol", "Q", "R"):
+            assert np.array_equal(g[k][:, b], o[k]), (b, k)
+        for k in ("zeroQ", "noqr", "up", "bs", "newtail"):
+            assert g[k][b] == o[k], (b, k)
+    # kept factorization, zero leading right-hand side: the head is skipped
+    rhs2 = rhs.copy()
+    rhs2[:, :, 0] = 0.0
+    g2 = qrbs_solve(K, mat, rhs2, tailidx, Q=g["Q"], R=g["R"], zeroQ=0, noqr=0)
+    for b in range(nb):
+        o2 = oracle_qrbs_solve(K, np.ascontiguousarray(mat[:, b]), np.ascontiguousarray(rhs2[:, b]),
+                               tailidx, Q=np.ascontiguousarray(g["Q"][:, b]),
+                               R=np.ascontiguousarray(g["R"][:, b]), zeroQ=0, noqr=0)
+        for k in ("rhs", "sol", "Q", "R"):
+            assert np.array_equal(g2[k][:, b], o2[k]), (b, k)
+        assert g2["noqr"][b] == 1 == o2["noqr"]
+
+
+@pytest.mark.parametrize("K", KS)
+def test_linear_residue_batch_matches_oracle(K):
+    from phcpack_amd.newton import linear_residue
+    print(f"linear_residue K={K}", flush=True)
+    rng = np.random.default_rng(200 + K)
+    nb, degp1, n = 3, 6, 5
+    mat = rand_md(rng, K, (nb, degp1, n, n))
+    rhs = rand_md(rng, K, (nb, degp1, n))
+    sol = rand_md(rng, K, (nb, degp1, n))
+    for tail in (0, 2):
+        r, m = linear_residue(K, mat, rhs, sol, tail)
+        for b in range(nb):
+            ro, mo = oracle_linear_residue(K, np.ascontiguousarray(mat[:, b]),
+                                           np.ascontiguousarray(rhs[:, b]),
+                                           np.ascontiguousarray(sol[:, b]), tail)
+            assert np.array_equal(r[:, b, tail:], ro[:, tail:]) and np.array_equal(m[:, b], mo)
+
+
+def _device_system(sysm):
+    from phcpack_amd.newton import RowSystem
+    return RowSystem(sysm.dim, sysm.monoptr, sysm.nvr, sysm.idxptr, sysm.idx)
+
+
+def test_cyclic_row_newton_matches_fixture():
+    """the reference's own dd test system: every step bit-identical"""
+    print("row newton cyclic fixture", flush=True)
+    g = np.load(os.path.join(GOLDEN, "newton.npz"))
+    sysm = ORowSystem(8, 15, g["monoptr"], g["nvr"], g["idxptr"], g["idx"], g["cst"], g["cff"])
+    dev = _device_system(sysm)
+    out = dev.row_newton(2, 15, 8, g["cst"][:, None], g["cff"][:, None], g["start"][:, None],
+                         history=True)
+    steps = len(g["tail"])
+    assert out["steps"][0] == steps
+    assert np.array_equal(out["tail"][0, :steps], g["tail"])
+    assert np.array_equal(out["up"][0, :steps], g["up"])
+    assert np.array_equal(out["bs"][0, :steps], g["bs"])
+    assert np.array_equal(out["wdeg"][0, :steps], g["wdeg"])
+    for s in range(steps):
+        assert np.array_equal(out["history"][s, :, 0], g["history"][s]), s
+    assert np.array_equal(out["Q"][:, 0], g["Q"]) and np.array_equal(out["R"][:, 0], g["R"])
+
+
+@pytest.mark.parametrize("K", KS)
+@pytest.mark.parametrize("deg", [8, 15])
+def test_row_newton_batch_matches_oracle(K, deg):
+    """a batch of planted systems with the same supports (8 x 8 monomials of
+    3 variables) and their own coefficients, one launch"""
+    print(f"row newton batch K={K} deg={deg}", flush=True)
+    rng = np.random.default_rng(300 + K + deg)
+    sup = random_supports(rng, 8, 8, 3)
+    nsys = 6
+    systems = [planted_row_system(rng, K, 8, deg, supports=sup) for _ in range(nsys)]
+    cst = np.stack([s[0].cst for s in systems], axis=1)
+    cff = np.stack([s[0].cff for s in systems], axis=1)
+    x0 = np.stack([s[2] for s in systems], axis=1)
+    dev = _device_system(systems[0][0])
+    out = dev.row_newton(K, deg, 10, cst, cff, x0)
+    for b, (sysm, root, start) in enumerate(systems):
+        o = oracle_row_newton(K, sysm, start, 10)
+        st = o["steps"]
+        assert out["steps"][b] == st, b
+        for k in ("wdeg", "tail", "up", "bs"):
+            assert np.array_equal(out[k][b, :st], o[k]), (b, k)
+        assert np.array_equal(out["x"][:, b], o["x"]), b
+        assert np.array_equal(out["Q"][:, b], o["Q"]) and np.array_equal(out["R"][:, b], o["R"])
+
+
+@pytest.mark.parametrize("K", (2,))
+def test_row_newton_shared_coefficients(K):
+    """one system, many start series (shared coefficients)"""
+    print("row newton shared", flush=True)
+    rng = np.random.default_rng(400)
+    sysm, root, start = planted_row_system(rng, K, 6, 10, nper=5, nvar=2)
+    nsys = 9
+    starts = np.repeat(start[:, None], nsys, axis=1)
+    starts[0, :, :, 0] += rng.uniform(-1e-6, 1e-6, (nsys, 6))
+    dev = _device_system(sysm)
+    out = dev.row_newton(K, 10, 8, sysm.cst[:, None], sysm.cff[:, None], starts)
+    for b in range(nsys):
+        o = oracle_row_newton(K, sysm, np.ascontiguousarray(starts[:, b]), 8)
+        assert np.array_equal(out["x"][:, b], o["x"]), b
+
+
+def test_row_newton_rejects_bad_arguments():
+    from phcpack_amd._lib import PhcError
+    from phcpack_amd.newton import RowSystem
+    with pytest.raises(PhcError):
+        RowSystem(2, [0, 1, 2], [1, 3], [0, 1, 4], [0, 1, 0, 1])   # nvr 3 > dim
+    dev = RowSystem(2, [0, 1, 2], [1, 1], [0, 1, 2], [0, 1])
+    with pytest.raises(PhcError):
+        dev.row_newton(8, 4, 3, np.zeros((8, 1, 2, 5)), np.zeros((8, 1, 2, 5)), np.zeros((8, 1, 2, 5)))
