@@ -198,6 +198,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("tracker", "qr", "newton"), default="tracker",
+                    help="tracker: the headline line; qr: complex quad-double blocked QR "
+                         "least-squares solve of 1024 x 1024; newton: batched dd row Newton")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip controller training; the headline is then the plain-rule mixed tracker")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -209,6 +212,10 @@ def main():
               "independent and would be split evenly across GPUs, but this build measures "
               "one GPU.", file=sys.stderr)
         return 2
+    if args.workload == "qr":
+        return run_qr(args)
+    if args.workload == "newton":
+        return run_newton(args)
     return run(args)
 
 
@@ -234,7 +241,8 @@ def train_learned(n, prm, ev):
         return ev.stop_ms()
 
     best, table = tune_threshold(ctl, tt, sols, prm, K=1, timer=timer)
-    info.update(train_seeds=list(TRAIN_SEEDS), tune_seed=TUNE_SEED, threshold=best,
+    info.update(train_seeds=list(TRAIN_SEEDS), tune_seed=TUNE_SEED,
+                threshold=best[0] if best else None, pole_cap=best[1] if best else None,
                 tune_table={str(k): v for k, v in table.items()},
                 seconds=round(time.perf_counter() - t0, 2))
     return (ctl if best is not None else None), info
@@ -343,6 +351,7 @@ def run(args):
         cl = planes_to_points(xl, 4)
         v_l, t_l, tt_l, blk_l = block(ol, rows_l, wall_l, xl, res_l, ok_l, cl)
         blk_l["threshold"] = ctl.threshold
+        blk_l["pole_cap"] = ctl.pole_cap
         blk_l["training"] = train_info
         head = dict(v=v_l, t=t_l, tt=tt_l, out=ol, lost=blk_l["lost_paths_vs_plain"],
                     name="learned+mixed")
@@ -401,6 +410,151 @@ def run(args):
         line["cpu_baseline"] = cpu_baseline(hom, sols, args.cpu_seconds)
     else:
         line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def md_lead_diff(lib, K, da, db, nelem):
+    """max over the elements of |leading limb of a - b|, the difference taken
+    in K-limb arithmetic on the GPU, for complex planar [2K][nelem] arrays"""
+    from phcpack_amd._lib import DeviceArray, check
+    dc = DeviceArray((2 * K, nelem))
+    for h in (0, 1):
+        off = h * K * nelem * 8
+        check(lib.phc_md_batch(1, K, nelem, ctypes.c_void_p(da.ptr.value + off),
+                               ctypes.c_void_p(db.ptr.value + off),
+                               ctypes.c_void_p(dc.ptr.value + off), None))
+    c = dc.to_host()
+    return float(max(np.abs(c[0]).max(), np.abs(c[K]).max()))
+
+
+def run_qr(args, n=1024, szt=128, K=4, seed=SEED):
+    """Blocked Householder QR least-squares solve in complex quad-double:
+    A (n x n, re and im uniform in [-1,1]) = QR, y = Q^H b, R x = y by the
+    tiled back substitution, b = A x0 with x0 random; inputs resident on the
+    device.  Checks (outside the timed region, on the GPU in quad-double):
+    max |Q^H Q - I|, max |QR - A|, max |x - x0|."""
+    from phcpack_amd._lib import DeviceArray, check, load
+    from phcpack_amd.baqr import BlockedQR, gemm_device, qh_times, upper_tiled_solver_device
+    from phcpack_amd.flops import BUILD, baqr_canonical, baqr_work, complex_mac
+    lib = load()
+    rng = np.random.default_rng(seed)
+    P = 2 * K
+    A = np.zeros((P, n, n))
+    A[0], A[K] = rng.uniform(-1, 1, (n, n)), rng.uniform(-1, 1, (n, n))
+    x0 = np.zeros((P, n))
+    x0[0], x0[K] = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    dA, dx0 = DeviceArray.from_host(A), DeviceArray.from_host(x0)
+    db = DeviceArray.zeros((P, n))
+    gemm_device(K, False, False, n, 1, n, (dA, 0, n, 1), (dx0, 0, 1, 0), (db, 0, 1, 0))
+    qr = BlockedQR(K, n, n, szt)
+    dQ, dR = DeviceArray((P, n, n)), DeviceArray((P, n, n))
+    dy, dx, dbw = DeviceArray((P, n)), DeviceArray((P, n)), DeviceArray((P, n))
+    ev = Events(lib)
+
+    def step():
+        qr.factor_device(dA, dQ, dR)
+        qh_times(K, dQ, db, n, dy)
+        upper_tiled_solver_device(K, n, szt, n // szt, dR, dy, dx)
+
+    for _ in range(args.warmup):
+        step()
+    check(lib.phc_synchronize())
+    t_qr, t_all = [], []
+    for _ in range(args.steps):
+        ev.start()
+        qr.factor_device(dA, dQ, dR)
+        t_qr.append(ev.stop_ms())
+        ev.start()
+        qh_times(K, dQ, db, n, dy)
+        upper_tiled_solver_device(K, n, szt, n // szt, dR, dy, dx)
+        t_all.append(t_qr[-1] + ev.stop_ms())
+    check(lib.phc_synchronize())
+    # checks in quad-double on the GPU (R's diagonal tiles now hold inverses: refactor)
+    qr.factor_device(dA, dQ, dR)
+    dI = DeviceArray.zeros((P, n, n))
+    gemm_device(K, True, False, n, n, n, (dQ, 0, 1, n), (dQ, 0, n, 1), (dI, 0, n, 1))
+    eye = np.zeros((P, n, n))
+    eye[0] = np.eye(n)
+    qq = md_lead_diff(lib, K, dI, DeviceArray.from_host(eye), n * n)
+    dQR = DeviceArray.zeros((P, n, n))
+    gemm_device(K, False, False, n, n, n, (dQ, 0, n, 1), (dR, 0, n, 1), (dQR, 0, n, 1))
+    qra = md_lead_diff(lib, K, dQR, dA, n * n)
+    xerr = md_lead_diff(lib, K, dx, dx0, n)
+    ms_qr, ms = float(np.mean(t_qr)), float(np.mean(t_all))
+    executed = baqr_work(n, n, szt) * complex_mac(K)
+    cm, ca = baqr_canonical(n)
+    canonical = cm * BUILD[K]["mul"] + ca * BUILD[K]["add"]
+    line = {
+        "metric": "complex quad-double blocked QR least-squares solves/sec (1024x1024)",
+        "value": round(1e3 / ms, 3), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 (quad-double)", "data": "synthetic",
+        "config": {"workload": f"complex quad-double {n}x{n}, {n // szt} tiles of {szt} columns, "
+                               "entries uniform in [-1,1], b = A x0", "seed": seed},
+        "qr_ms": round(ms_qr, 3),
+        "checks": {"max_abs_QHQ_minus_I": qq, "max_abs_QR_minus_A": qra, "max_abs_x_minus_x0": xerr},
+        "roofline": {"bound": "mfma", "achieved": round(executed / (ms_qr / 1e3) / 1e12, 3),
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(executed / (ms_qr / 1e3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+                     "traffic": None, "kernel": "blocked QR (all kernels)",
+                     "flops_executed": executed, "flops_canonical": canonical,
+                     "canonical_tflops": round(canonical / (ms_qr / 1e3) / 1e12, 3)},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def run_newton(args, nsys=8192, seed=SEED):
+    """Batched staggered row Newton in double-double: nsys systems of 8
+    polynomials in 8 variables, 8 monomials of 3 distinct variables each,
+    coefficient series of degree 15 uniform in [-1,1], 8 Newton steps from a
+    random constant start series, one wavefront per system."""
+    from phcpack_amd._lib import DeviceArray, check, load
+    from phcpack_amd.newton import RowSystem
+    lib = load()
+    rng = np.random.default_rng(seed)
+    dim, nper, nvar, deg, K = 8, 8, 3, 15, 2
+    D = deg + 1
+    nmono = dim * nper
+    monoptr = np.arange(0, nmono + 1, nper, dtype=np.int32)
+    nvr = np.full(nmono, nvar, np.int32)
+    idx = np.concatenate([np.sort(rng.choice(dim, nvar, replace=False)) for _ in range(nmono)]).astype(np.int32)
+    idxptr = np.concatenate([[0], np.cumsum(nvr)]).astype(np.int32)
+    sysd = RowSystem(dim, monoptr, nvr, idxptr, idx)
+    cst = np.zeros((K, nsys, dim, D))
+    cff = np.zeros((K, nsys, nmono, D))
+    cst[0] = rng.uniform(-1, 1, (nsys, dim, D))
+    cff[0] = rng.uniform(-1, 1, (nsys, nmono, D))
+    x0 = np.zeros((K, nsys, dim, D))
+    x0[0, :, :, 0] = rng.uniform(-1, 1, (nsys, dim))
+    dc, df = DeviceArray.from_host(cst), DeviceArray.from_host(cff)
+    dx0 = DeviceArray.from_host(x0)
+    dx = DeviceArray((K, nsys, dim, D))
+    steps = DeviceArray((nsys,), np.int32)
+    ev = Events(lib)
+    for _ in range(args.warmup):
+        check(lib.phc_device_copy(dx.ptr, dx0.ptr, dx.nbytes, None))
+        sysd.row_newton_device(K, deg, 8, nsys, dc, False, df, False, dx, {"steps": steps})
+    check(lib.phc_synchronize())
+    ms = []
+    for _ in range(args.steps):
+        check(lib.phc_device_copy(dx.ptr, dx0.ptr, dx.nbytes, None))
+        ev.start()
+        sysd.row_newton_device(K, deg, 8, nsys, dc, False, df, False, dx, {"steps": steps})
+        ms.append(ev.stop_ms())
+    check(lib.phc_synchronize())
+    st = steps.to_host()
+    t = float(np.mean(ms))
+    line = {
+        "metric": "dd row Newton systems/sec (8 polynomials, 8 monomials of 3 variables, degree 15, 8 steps)",
+        "value": round(nsys / (t / 1e3), 1), "unit": "systems/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(t, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 (double-double)", "data": "synthetic",
+        "config": {"workload": f"{nsys} independent systems, one wavefront each", "seed": seed},
+        "newton_steps_per_s": round(float(st.sum()) / (t / 1e3), 1),
+        "roofline": None,
+    }
     print(json.dumps(line), flush=True)
     return 0
 

@@ -158,7 +158,10 @@ int phc_program_validate(const phc_program *program);
  * mode 2: explore: c drawn from a hash of (seed, path, step), and every step
  *         logged: features log_x[path][i][nfeat], candidate log_c[path][i],
  *         log_y[path][i] = 1 if the first attempt was accepted, log_n[path]
- *         records (at most log_capacity per path). */
+ *         records (at most log_capacity per path).
+ * In modes 1 and 2 the multiplied step is capped at pole_cap * r (r the
+ * Fabry radius) when pole_cap > 0, so that the step stays inside the disk
+ * where the series converges; candidate 0 (multiplier 1) is the plain rule. */
 #define PHC_CTL_NFEAT 16
 #define PHC_CTL_HIDDEN 64
 #define PHC_CTL_MAXCAND 8
@@ -171,6 +174,7 @@ typedef struct {
   int log_capacity;
   float *log_x;                   /* device */
   int *log_c, *log_y, *log_n;     /* device */
+  double pole_cap;                /* 0: no cap */
 } phc_controller;
 
 typedef struct phc_tracker phc_tracker;
@@ -286,6 +290,43 @@ int phc_row_newton(phc_rowsys *sys, int K, int deg, int nbsteps, int nsys, const
                    int cst_shared, const double *cff_dev, int cff_shared, double *x_dev,
                    int *steps_dev, int *wdeg_dev, int *tail_dev, int *up_dev, int *bs_dev,
                    double *history_dev, double *Q_dev, double *R_dev, void *stream);
+
+/* ------------- complex blocked Householder QR, tiled back substitution ----
+ * K = 2 or 4.  Matrices are complex limb-planar [2K][rows][cols], row-major.
+ * Results agree with the reference's host code to a few units in the last
+ * limb (long sums are split across lanes). */
+
+/* workspace for A (nrows x ncols, nrows >= ncols, ncols = szt*nbt) */
+typedef struct phc_baqr phc_baqr;
+int phc_baqr_create(phc_baqr **plan, int K, int nrows, int ncols, int szt);
+int phc_baqr_destroy(phc_baqr *plan);
+
+/* A = Q R with Q (nrows x nrows) explicit and R (nrows x ncols), in tiles of
+ * szt columns: CPU_cmplx4_blocked_houseqr (Matrices/dbl4_baqr_host.cpp:1206),
+ * replaces GPU_cmplx4_blocked_houseqr (Matrices/dbl4_baqr_kernels.cu:6197)
+ * and its dd twin GPU_cmplx2_blocked_houseqr (dbl2_baqr_kernels.cu). */
+int phc_cmplx_blocked_houseqr(phc_baqr *plan, const double *A_dev, double *Q_dev, double *R_dev,
+                              void *stream);
+
+/* U x = b for upper triangular U (dim = szt*nbt): the diagonal tiles are
+ * inverted, then the tiles are applied from the last one up.  As in the
+ * reference, U's diagonal tiles are replaced by their inverses and b is
+ * updated.  CPU_cmplx4_upper_tiled_solver (Matrices/dbl4_tabs_host.cpp:345),
+ * replaces GPU_cmplx4_upper_tiled_solver (Matrices/dbl4_tabs_kernels.cu:1802).
+ * Synchronizes the stream. */
+int phc_cmplx_upper_tiled_solver(int K, int dim, int szt, int nbt, double *U_dev, double *b_dev,
+                                 double *x_dev, void *stream);
+
+/* C(i,j) = [C(i,j) +] sum_k op(A(i,k)) op(B(k,j)) for complex
+ * multiple-double operands given by base pointer, plane stride and the two
+ * element strides (element (r,c) of A at A + a0*r + a1*c in every plane);
+ * op conjugates when the flag is set.  The product kernel of the QR (the
+ * trailing updates of dbl4_baqr_host.cpp:357,743), exposed for checks such as
+ * Q^H Q and Q^H b. */
+int phc_cmplx_gemm(int K, int conjA, int conjB, int M, int N, int Kd, const double *A_dev,
+                   size_t a_plane, long a0, long a1, const double *B_dev, size_t b_plane, long b0,
+                   long b1, double *C_dev, size_t c_plane, long c0, long c1, int accumulate,
+                   void *stream);
 
 #ifdef __cplusplus
 }

@@ -474,6 +474,7 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
           if (mode == 1) cand = mlp_choose(ctl, feat);
           else cand = (int)(mix_hash(ctl->seed, (unsigned)path, (unsigned)(npred - 1)) % (unsigned)ctl->ncand);
           h *= ctl->mult[cand];
+          if (ctl->pole_cap > 0.0 && r < INFINITY && r > 0.0) h = fmin(h, ctl->pole_cap * r);
         }
         rej0 = rej;
       for (j = 0; j < n; j++) ccopy(K, ZP(w.slots, j * D), ZP(w.x0, j));

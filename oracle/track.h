@@ -41,6 +41,7 @@ typedef struct {
   int log_capacity;
   float *log_x;
   int *log_c, *log_y, *log_n;
+  double pole_cap;
 } otr_controller;
 
 /* paths p0 <= p < p1 of npaths; arrays as in phc_track; ctl may be NULL */

@@ -75,6 +75,7 @@ class PhcController(ctypes.Structure):
         ("weights", c_void_p), ("mu", c_void_p), ("sd", c_void_p),
         ("logit_threshold", ctypes.c_float), ("seed", ctypes.c_uint), ("log_capacity", c_int),
         ("log_x", c_void_p), ("log_c", c_void_p), ("log_y", c_void_p), ("log_n", c_void_p),
+        ("pole_cap", c_double),
     ]
 
 
@@ -107,6 +108,15 @@ SIGNATURES.update({
     "phc_row_newton": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p,
                                c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_void_p, c_void_p, c_void_p, c_void_p]),
+    "phc_baqr_create": (c_int, [ctypes.POINTER(c_void_p), c_int, c_int, c_int, c_int]),
+    "phc_baqr_destroy": (c_int, [c_void_p]),
+    "phc_cmplx_blocked_houseqr": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "phc_cmplx_upper_tiled_solver": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p,
+                                             c_void_p, c_void_p]),
+    "phc_cmplx_gemm": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int,
+                               c_void_p, c_size_t, ctypes.c_long, ctypes.c_long,
+                               c_void_p, c_size_t, ctypes.c_long, ctypes.c_long,
+                               c_void_p, c_size_t, ctypes.c_long, ctypes.c_long, c_int, c_void_p]),
 })
 
 

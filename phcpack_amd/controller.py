@@ -110,7 +110,8 @@ def mlp_grad(w, X, cand, y):
 class StepController:
     """Weights, input standardization and candidates of the controller."""
 
-    def __init__(self, weights=None, mu=None, sd=None, threshold=0.8, mults=MULTS, seed=0):
+    def __init__(self, weights=None, mu=None, sd=None, threshold=0.8, mults=MULTS, seed=0,
+                 pole_cap=1.0):
         self.mults = tuple(mults)
         self.C = len(self.mults)
         rng = np.random.default_rng(seed)
@@ -118,6 +119,7 @@ class StepController:
         self.mu = np.zeros(NFEAT, np.float32) if mu is None else np.asarray(mu, np.float32)
         self.sd = np.ones(NFEAT, np.float32) if sd is None else np.asarray(sd, np.float32)
         self.threshold = threshold
+        self.pole_cap = pole_cap  # multiplied steps stay <= pole_cap * Fabry radius
         self._dev = None
 
     @property
@@ -141,6 +143,7 @@ class StepController:
             c.weights, c.mu, c.sd = dw.ptr.value, dmu.ptr.value, dsd.ptr.value
         c.logit_threshold = self.logit_threshold
         c.seed = seed
+        c.pole_cap = self.pole_cap
         if logs is not None:
             c.log_capacity = logs.cap
             c.log_x, c.log_c = logs.x.ptr.value, logs.c.ptr.value
@@ -218,12 +221,13 @@ def train_controller(trackers, starts, params, K=1, seed=1, cap=256, epochs=30, 
                      loss_first=float(losses[0]), loss_last=float(losses[-1]))
 
 
-def tune_threshold(ctl, tracker, starts, params, K=1, thresholds=(0.5, 0.6, 0.7, 0.8, 0.9),
+def tune_threshold(ctl, tracker, starts, params, K=1,
+                   grid=((0.7, 1.0), (0.9, 1.0), (0.9, 0.75), (0.9, 1.5), (0.95, 0.75)),
                    timer=None):
-    """Pick the acceptance-probability threshold with the fewest predictor
-    steps (or the shortest time when `timer(fn)` is given) among those that
-    lose no successful path relative to the plain pole-step rule on a
-    tuning system.  Returns (threshold, table)."""
+    """Pick (acceptance-probability threshold, pole cap) with the fewest
+    predictor steps (or the shortest time when `timer(fn)` is given) among
+    those that lose no successful path relative to the plain pole-step rule
+    on a tuning system.  Sets them on ctl; returns (choice or None, table)."""
     from .tracker import TrackBuffers, points_to_planes
     planes = points_to_planes(starts)
     npaths = planes.shape[1]
@@ -245,14 +249,15 @@ def tune_threshold(ctl, tracker, starts, params, K=1, thresholds=(0.5, 0.6, 0.7,
     base = measure(None)
     table = {"plain": base}
     best, best_key = None, None
-    for th in thresholds:
-        ctl.threshold = th
+    for th, cap in grid:
+        ctl.threshold, ctl.pole_cap = th, cap
         r = measure(ctl.struct(1))
-        table[th] = r
+        table[f"{th}/{cap}"] = r
         if r["ok"] < base["ok"]:
             continue
         key = r["ms"] if r["ms"] is not None else r["pred"]
         if best_key is None or key < best_key:
-            best, best_key = th, key
-    ctl.threshold = best if best is not None else max(thresholds)
+            best, best_key = (th, cap), key
+    if best is not None:
+        ctl.threshold, ctl.pole_cap = best
     return best, table

@@ -81,6 +81,7 @@ struct Ctl {
   unsigned seed;
   float *log_x;
   int *log_c, *log_y, *log_n;
+  double pcap;  // step cap as a fraction of the Fabry radius (0: none)
 };
 
 // log2 of x > 0 from the binary exponent and the series
@@ -630,6 +631,7 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, Params par
         if (ctl.mode == 1) cand = mlp_choose<K>(ctl, w, feat);
         else cand = (int)(mix_hash(ctl.seed, (unsigned)path, (unsigned)(npred - 1)) % (unsigned)ctl.ncand);
         h *= ctl.mult[cand];
+        if (ctl.pcap > 0.0 && r < INFINITY && r > 0.0) h = fmin(h, ctl.pcap * r);
       }
       const int rej0 = rej;
       if (l < n) w.x0[l] = w.slots[l * D];
@@ -950,7 +952,7 @@ int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, d
   par.tolend = prm->tolend; par.divergence = prm->divergence;
   Ctl ctl;
   ctl.mode = 0; ctl.ncand = 1; ctl.cap = 0; ctl.w = ctl.mu = ctl.sd = nullptr; ctl.thresh = 0.0f;
-  ctl.seed = 0; ctl.log_x = nullptr; ctl.log_c = ctl.log_y = ctl.log_n = nullptr;
+  ctl.seed = 0; ctl.log_x = nullptr; ctl.log_c = ctl.log_y = ctl.log_n = nullptr; ctl.pcap = 0.0;
   for (int c = 0; c < PHC_CTL_MAXCAND; ++c) ctl.mult[c] = 1.0;
   if (ctlp && ctlp->mode != 0) {
     PHC_REQUIRE(ctlp->mode == 1 || ctlp->mode == 2, "phc_track: controller mode must be 0, 1 or 2");
@@ -968,6 +970,8 @@ int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, d
     ctl.w = ctlp->weights; ctl.mu = ctlp->mu; ctl.sd = ctlp->sd; ctl.thresh = ctlp->logit_threshold;
     ctl.seed = ctlp->seed; ctl.log_x = ctlp->log_x; ctl.log_c = ctlp->log_c;
     ctl.log_y = ctlp->log_y; ctl.log_n = ctlp->log_n;
+    PHC_REQUIRE(ctlp->pole_cap >= 0.0, "phc_track: pole_cap must be >= 0");
+    ctl.pcap = ctlp->pole_cap;
   }
   PHC_HIP(hipMemsetAsync(tr->counter, 0, sizeof(int), st));
   // up to 8 waves: the variant with 256 VGPRs per lane; 9 to 12 waves (FP64

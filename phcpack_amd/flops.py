@@ -127,3 +127,39 @@ def refine_flops(prog0, K, npaths, iters, table=BUILD):
     c = complex_costs(K, table)
     cost = lambda d: sum(c[k] * v for k, v in d.items())  # noqa: E731
     return npaths * (iters * cost(ph["newton"]) + cost(ph["eval"]))
+
+
+def complex_mac(K, table=BUILD):
+    """FP64 flops of one complex multiply-add (4 real products, 4 real sums)"""
+    t = table[K]
+    return 4 * t["mul"] + 4 * t["add"]
+
+
+def baqr_work(m, n, szt):
+    """complex multiply-adds executed by the GPU blocked QR of an m x n matrix
+    in tiles of szt columns (panel dots and updates, Gram matrix, W = Y S,
+    the two thin products of each trailing R update and of each Q update)"""
+    macs = 0
+    nbt = n // szt
+    for k in range(nbt):
+        rowoff = k * szt
+        rowdim = m - rowoff
+        for L in range(szt):
+            colidx = rowoff + L
+            nr, nc = m - colidx, szt - L
+            macs += 2 * nr * nc + nr          # dots, update, |x|^2
+        macs += szt * szt * rowdim            # G = Y^H Y
+        macs += szt * szt * szt // 6          # S recurrence
+        macs += rowdim * szt * szt            # W = Y S
+        if k < nbt - 1:
+            coldim = n - (k + 1) * szt
+            macs += 2 * szt * coldim * rowdim  # W^H C and Y T1
+        macs += 2 * m * szt * rowdim          # Q W and T2 Y^H
+    return macs
+
+
+def baqr_canonical(n):
+    """(real multiplications, real additions) of Householder QR of a complex
+    n x n matrix with Q formed explicitly, LAPACK's count: (16/3) n^3 each"""
+    c = 16.0 / 3.0 * n ** 3
+    return c, c

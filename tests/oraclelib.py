@@ -119,6 +119,7 @@ class OracleController:
             setattr(c, name, a.ctypes.data)
         c.logit_threshold = ctl.logit_threshold
         c.seed = seed
+        c.pole_cap = ctl.pole_cap
         self.cap = cap
         self.x = np.zeros((npaths, cap, NFEAT), np.float32)
         self.c = np.zeros((npaths, cap), np.int32)
@@ -367,3 +368,65 @@ def planted_row_system(rng, K, dim, deg, nper=8, nvar=3, supports=None):
     start = np.zeros((K, dim, D))
     start[0, :, 0] = root[0, :, 0] + 1e-10
     return sysm, root, start
+
+
+# ------------------------------------------------------------ blocked QR --
+
+def _cx_qr(lib, fn, K, A, szt, nbt):
+    P, m, n = A.shape
+    Q, R = np.zeros((P, m, m)), np.zeros((P, m, n))
+    getattr(lib, fn)(K, m, n, szt, nbt, pd(np.ascontiguousarray(A).copy()), pd(Q), pd(R))
+    return Q, R
+
+
+def oracle_blocked_houseqr(K, A, szt, nbt):
+    return _cx_qr(oracle(), "obq_blocked_houseqr", K, A, szt, nbt)
+
+
+def ref_blocked_houseqr(K, A, szt, nbt):
+    return _cx_qr(ref(), "ref_cmplx_blocked_houseqr", K, A, szt, nbt)
+
+
+def _cx_tiled(lib, fn, K, U, b, szt, nbt):
+    U, b = np.ascontiguousarray(U).copy(), np.ascontiguousarray(b).copy()
+    x = np.zeros_like(b)
+    getattr(lib, fn)(K, b.shape[1], szt, nbt, pd(U), pd(b), pd(x))
+    return x, U, b
+
+
+def oracle_upper_tiled_solver(K, U, b, szt, nbt):
+    return _cx_tiled(oracle(), "obq_upper_tiled_solver", K, U, b, szt, nbt)
+
+
+def ref_upper_tiled_solver(K, U, b, szt, nbt):
+    return _cx_tiled(ref(), "ref_cmplx_upper_tiled_solver", K, U, b, szt, nbt)
+
+
+def oracle_qhb(K, Q, b):
+    y = np.zeros_like(b)
+    oracle().obq_qhb(K, b.shape[1], pd(np.ascontiguousarray(Q)), pd(np.ascontiguousarray(b)), pd(y))
+    return y
+
+
+def oracle_matvec(K, A, x):
+    y = np.zeros((A.shape[0], A.shape[1]))
+    oracle().obq_matvec(K, A.shape[1], A.shape[2], pd(np.ascontiguousarray(A)),
+                        pd(np.ascontiguousarray(x)), pd(y))
+    return y
+
+
+def md_diff(K, a, b):
+    """leading limb of a - b computed in K-limb arithmetic, elementwise, for
+    real [K][...] or complex [2K][...] planar arrays"""
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    P = a.shape[0]
+    n = int(np.prod(a.shape[1:]))
+    out = np.zeros((P // K,) + a.shape[1:])
+    for h in range(P // K):
+        aa = np.ascontiguousarray(a[h * K:(h + 1) * K].reshape(K, n))
+        bb = np.ascontiguousarray(b[h * K:(h + 1) * K].reshape(K, n))
+        c = np.zeros((K, n))
+        oracle().omd_batch(1, K, n, pd(aa), pd(bb), pd(c))
+        out[h] = c[0].reshape(a.shape[1:])
+    return out

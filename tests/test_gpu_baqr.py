@@ -1,0 +1,78 @@
+"""GPU complex blocked Householder QR and tiled back substitution
+(phcpack_amd/csrc/baqr.hip) against the CPU oracle (oracle/baqr.c, itself
+bit-exact with the reference's host code).  The GPU splits the long sums
+across lanes, so the results agree to a few units in the last limb, not
+bit for bit: the tolerances below are in units of eps^K (eps = 2^-52)
+times the dimension."""
+import numpy as np
+import pytest
+
+from oraclelib import (md_diff, oracle_blocked_houseqr, oracle_matvec, oracle_upper_tiled_solver,
+                       rand_cmd)
+
+pytestmark = pytest.mark.gpu
+
+
+def ulps(K):
+    return (2.0 ** -52) ** K
+
+
+@pytest.mark.parametrize("K", (2, 4))
+@pytest.mark.parametrize("shape", [(32, 32, 8), (48, 32, 8), (64, 64, 16), (40, 40, 40)])
+def test_blocked_qr_matches_oracle(K, shape):
+    from phcpack_amd.baqr import BlockedQR
+    m, n, szt = shape
+    print(f"blocked qr K={K} {shape}", flush=True)
+    rng = np.random.default_rng(m + n + szt + K)
+    A = rand_cmd(rng, K, (m, n))
+    Q, R = BlockedQR(K, m, n, szt).factor(A)
+    Qo, Ro = oracle_blocked_houseqr(K, A, szt, n // szt)
+    tol = 64 * m * ulps(K)
+    dq, dr = np.abs(md_diff(K, Q, Qo)).max(), np.abs(md_diff(K, R, Ro)).max()
+    print(f"  max |Q - Qo| {dq:.2e}  |R - Ro| {dr:.2e}  tol {tol:.2e}", flush=True)
+    assert dq < tol and dr < tol * np.sqrt(m)
+
+
+@pytest.mark.parametrize("K", (2, 4))
+def test_upper_tiled_solver_matches_oracle(K):
+    from phcpack_amd.baqr import upper_tiled_solver
+    print(f"tiled solver K={K}", flush=True)
+    rng = np.random.default_rng(9 + K)
+    szt, nbt = 8, 4
+    dim = szt * nbt
+    U = rand_cmd(rng, K, (dim, dim))
+    for p in range(2 * K):
+        U[p] = np.triu(U[p])
+    for i in range(dim):
+        U[0, i, i] += 4.0
+    b = rand_cmd(rng, K, (dim,))
+    x, U1, b1 = upper_tiled_solver(K, U, b, szt, nbt)
+    xo, Uo, bo = oracle_upper_tiled_solver(K, U, b, szt, nbt)
+    tol = 64 * dim * ulps(K)
+    assert np.abs(md_diff(K, x, xo)).max() < tol
+    assert np.abs(md_diff(K, U1, Uo)).max() < tol
+    assert np.abs(md_diff(K, b1, bo)).max() < tol
+
+
+@pytest.mark.parametrize("K", (2, 4))
+def test_least_squares_recovers_planted_solution(K):
+    """b = A x0 in the oracle; QR, Q^H b and the tiled solve on the GPU give x0"""
+    from phcpack_amd._lib import DeviceArray, check, load
+    from phcpack_amd.baqr import BlockedQR, qh_times, upper_tiled_solver_device
+    print(f"least squares K={K}", flush=True)
+    rng = np.random.default_rng(21 + K)
+    n, szt = 96, 32
+    A = rand_cmd(rng, K, (n, n))
+    x0 = rand_cmd(rng, K, (n,))
+    b = oracle_matvec(K, A, x0)
+    qr = BlockedQR(K, n, n, szt)
+    dA = DeviceArray.from_host(A)
+    dQ, dR = DeviceArray((2 * K, n, n)), DeviceArray((2 * K, n, n))
+    qr.factor_device(dA, dQ, dR)
+    db, dy, dx = DeviceArray.from_host(b), DeviceArray.zeros((2 * K, n)), DeviceArray.zeros((2 * K, n))
+    qh_times(K, dQ, db, n, dy)
+    upper_tiled_solver_device(K, n, szt, n // szt, dR, dy, dx)
+    check(load().phc_synchronize())
+    err = np.abs(md_diff(K, dx.to_host(), x0)).max()
+    print(f"  error {err:.2e}", flush=True)
+    assert err < 1e6 * ulps(K)

@@ -85,3 +85,34 @@ def test_learned_mode_matches_oracle(K):
     for k in ("endp", "status", "nsteps", "nrej", "work"):
         assert np.array_equal(g[k], o[k]), k
     assert np.all(g["status"] == 1)
+
+
+def test_trained_controller_gains_over_plain_rule():
+    """A/B on held-out systems: the controller trained (on the GPU, by the
+    library's own kernels) on explore logs of other systems of the family
+    needs fewer predictor steps than the plain pole-step rule and loses no
+    path that the plain rule tracks."""
+    from phcpack_amd.controller import train_controller
+    from phcpack_amd.tracker import Tracker, TrackParams
+    print("controller A/B", flush=True)
+    prm = TrackParams(tolend=1e-13)
+    trs, sts = [], []
+    for seed in range(31, 37):
+        hom, sols = _system(6, 2, seed)
+        trs.append(Tracker(hom, deg=8))
+        sts.append(sols)
+    ctl, info = train_controller(trs, sts, prm, K=1, seed=5, epochs=30, batch=512,
+                                 threshold=0.9)
+    assert info["loss_last"] < info["loss_first"]
+    steps_plain, steps_learned = 0, 0
+    for seed in range(90, 94):            # held out
+        hom, sols = _system(6, 2, seed)
+        tr = Tracker(hom, deg=8)
+        a = tr.track(sols, K=1, params=prm)
+        b = tr.track(sols, K=1, params=prm, controller=ctl.struct(1))
+        ok_a, ok_b = a["status"] == 1, b["status"] == 1
+        assert ok_b.sum() >= ok_a.sum(), seed
+        steps_plain += int(a["work"][0].sum())
+        steps_learned += int(b["work"][0].sum())
+    print(f"predictor steps plain {steps_plain} learned {steps_learned}", flush=True)
+    assert steps_learned < 0.85 * steps_plain
