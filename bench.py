@@ -76,44 +76,51 @@ def run_plain(tr, d_start, npaths, bufs, ev, prm):
 
 def run_mixed(tr, d_start, starts, npaths, bufs, ref, ev, prm, lib, controller=None):
     """FP64 tracking (plain pole-step rule, or the learned controller when
-    given), then dd Newton at the endpoints.  Paths whose dd Newton does not
-    converge are refined in quad-double; with the learned controller, paths
-    that still fail are tracked again with the plain rule (a path the
-    controller lost is recovered, not dropped).  Returns (tracking ms,
-    endpoint ms, escalated, re-tracked)."""
+    given), then dd Newton at the endpoints of the paths the tracker
+    finished (the others are passed over on the device).  Finished paths
+    whose dd Newton does not converge are refined in quad-double; with the
+    learned controller, paths it did not finish or that still fail are
+    tracked again with the plain rule (a path the controller lost is
+    recovered, not dropped).  Returns (tracking ms, endpoint ms, escalated,
+    re-tracked)."""
     from phcpack_amd._lib import check
     ev.start()
     tr.track_device(d_start, 1, npaths, 1, bufs, prm, controller=controller)
     t_track = ev.stop_ms()
     ev.start()
     tr.refine_device(bufs.endp, 1, npaths, 2, ref["x2"], ref["dx"], ref["res"], ref["st"],
-                     iters=3, tol=1e-28)
+                     iters=3, tol=1e-28, only=bufs.status)
     t_ref = ev.stop_ms()
     t0 = time.perf_counter()
     st = ref["st"].to_host()
-    bad = np.nonzero(st != 1)[0]
+    tracked = bufs.status.to_host() == 1
+    bad = np.nonzero((st != 1) & tracked)[0]
     fix = {}
     nres = 0
+    lost = np.zeros(0, np.int64)
     if len(bad):
-        # escalate the unconverged paths to quad-double
+        # escalate the unconverged endpoints to quad-double
         r4 = tr.refine(ref["x2"].to_host()[:, bad], 4, iters=4, tol=1e-56)
         ok4 = r4["status"] == 1
         for j, p in enumerate(bad):
             fix[int(p)] = (r4["x"][:, j], bool(ok4[j]))
         lost = bad[~ok4]
-        if controller is not None and len(lost):
+    if controller is not None:
+        lost = np.union1d(lost, np.nonzero(~tracked)[0])
+        if len(lost):
             o = tr.track(starts[:, lost], K=1, params=prm)
             r2 = tr.refine(o["endp"], 2, iters=3, tol=1e-28)
             x4 = to_qd(r2["x"], 2)
             ok = (o["status"] == 1) & (r2["status"] == 1)
-            if not ok.all():
-                r4b = tr.refine(x4[:, ~ok], 4, iters=4, tol=1e-56)
-                x4[:, ~ok] = r4b["x"]
-                ok[~ok] = (o["status"][~ok] == 1) & (r4b["status"] == 1)
+            esc = (o["status"] == 1) & ~ok
+            if esc.any():
+                r4b = tr.refine(x4[:, esc], 4, iters=4, tol=1e-56)
+                x4[:, esc] = r4b["x"]
+                ok[esc] = r4b["status"] == 1
             for j, p in enumerate(lost):
                 fix[int(p)] = (x4[:, j], bool(ok[j]))
             nres = len(lost)
-        check(lib.phc_synchronize())
+    check(lib.phc_synchronize())
     t_ref += (time.perf_counter() - t0) * 1e3
     ref["fix"] = fix
     return t_track, t_ref, len(bad), nres
@@ -198,9 +205,10 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("tracker", "qr", "newton"), default="tracker",
+    ap.add_argument("--workload", choices=("tracker", "qr", "newton", "rankdef"), default="tracker",
                     help="tracker: the headline line; qr: complex quad-double blocked QR "
-                         "least-squares solve of 1024 x 1024; newton: batched dd row Newton")
+                         "least-squares solve of 1024 x 1024; newton: batched dd row Newton; "
+                         "rankdef: 14 quadratics with 2 dependent equations, 16384 paths")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip controller training; the headline is then the plain-rule mixed tracker")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -216,6 +224,8 @@ def main():
         return run_qr(args)
     if args.workload == "newton":
         return run_newton(args)
+    if args.workload == "rankdef":
+        return run_rankdef(args)
     return run(args)
 
 
@@ -293,6 +303,11 @@ def run(args):
 
     ctl, train_info = (None, None) if args.no_learned else train_learned(n, prm_mixed, ev)
     cstruct = ctl.struct(1) if ctl is not None else None
+    # launch shape per precision, tuned on this system before the timed region
+    from phcpack_amd.autotune import tune_waves
+    tuned = {}
+    for K_, b_, p_, c_ in ((2, b_plain, prm_plain, None), (1, b_mixed, prm_mixed, None)):
+        tuned[K_] = tune_waves(tr, d_start, 1, npaths, K_, b_, p_, controller=c_)
 
     for _ in range(args.warmup):
         run_plain(tr, d_start, npaths, b_plain, ev, prm_plain)
@@ -390,6 +405,9 @@ def run(args):
         "speedup_vs_plain": round(float(head["v"] / v_plain), 3) if v_plain > 0 else None,
         "speedup_mixed_vs_plain": round(float(v_m / v_plain), 3) if v_plain > 0 else None,
         "lost_paths_vs_plain": int(head["lost"]),
+        "waves_per_workgroup": {"plain_dd": tuned[2][0], "fp64": tuned[1][0],
+                                "tuning_ms": {str(k): {str(w): round(ms, 2) for w, ms in v[1].items()}
+                                              for k, v in tuned.items()}},
         "roofline": {
             "bound": "mfma",
             "achieved": round(achieved, 4),
@@ -435,7 +453,7 @@ def run_qr(args, n=1024, szt=128, K=4, seed=SEED):
     device.  Checks (outside the timed region, on the GPU in quad-double):
     max |Q^H Q - I|, max |QR - A|, max |x - x0|."""
     from phcpack_amd._lib import DeviceArray, check, load
-    from phcpack_amd.baqr import BlockedQR, gemm_device, qh_times, upper_tiled_solver_device
+    from phcpack_amd.baqr import BlockedQR, gemm_device, upper_tiled_solver_device
     from phcpack_amd.flops import BUILD, baqr_canonical, baqr_work, complex_mac
     lib = load()
     rng = np.random.default_rng(seed)
@@ -454,7 +472,7 @@ def run_qr(args, n=1024, szt=128, K=4, seed=SEED):
 
     def step():
         qr.factor_device(dA, dQ, dR)
-        qh_times(K, dQ, db, n, dy)
+        qr.qhb_device(dQ, db, dy)
         upper_tiled_solver_device(K, n, szt, n // szt, dR, dy, dx)
 
     for _ in range(args.warmup):
@@ -466,7 +484,7 @@ def run_qr(args, n=1024, szt=128, K=4, seed=SEED):
         qr.factor_device(dA, dQ, dR)
         t_qr.append(ev.stop_ms())
         ev.start()
-        qh_times(K, dQ, db, n, dy)
+        qr.qhb_device(dQ, db, dy)
         upper_tiled_solver_device(K, n, szt, n // szt, dR, dy, dx)
         t_all.append(t_qr[-1] + ev.stop_ms())
     check(lib.phc_synchronize())
@@ -500,6 +518,81 @@ def run_qr(args, n=1024, szt=128, K=4, seed=SEED):
                      "traffic": None, "kernel": "blocked QR (all kernels)",
                      "flops_executed": executed, "flops_canonical": canonical,
                      "canonical_tflops": round(canonical / (ms_qr / 1e3) / 1e12, 3)},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def run_rankdef(args, n=14, rank_deficient=2, seed=SEED):
+    """Mixed-precision tracking of a rank-deficient family: 14 dense
+    quadratics (16384 total-degree paths) whose last 2 top-degree forms are
+    combinations of the top-degree forms of two others, so the leading forms
+    are rank-deficient and part of the paths diverge or end at singular
+    points.  FP64 tracking + dd endpoint Newton of the finished paths (qd for
+    those dd does not settle) against the plain dd tracker; value = paths/s
+    counting endpoints with max_i |p_i(x)| <= 1e-24 in quad-double."""
+    from phcpack_amd._lib import DeviceArray, check, load
+    from phcpack_amd.autotune import tune_waves
+    from phcpack_amd.homotopy import dense_system, total_degree_homotopy
+    from phcpack_amd.tracker import (TrackBuffers, Tracker, TrackParams, planes_to_points,
+                                     points_to_planes)
+    lib = load()
+    rng = np.random.default_rng(seed)
+    p = dense_system(n, 2, rng, rank_deficient=rank_deficient)
+    hom, sols = total_degree_homotopy(p, rng)
+    npaths = len(sols)
+    tr = Tracker(hom, deg=8)
+    h_start = points_to_planes(sols)
+    d_start = DeviceArray.from_host(h_start)
+    ev = Events(lib)
+    bm, bp = TrackBuffers(npaths, n, 1), TrackBuffers(npaths, n, 2)
+    ref = dict(x2=DeviceArray((4, npaths, n)), dx=DeviceArray((npaths,)),
+               res=DeviceArray((npaths,)), st=DeviceArray((npaths,), np.int32))
+    prm_m, prm_p = TrackParams(tolend=1e-13), TrackParams(tolend=1e-28)
+    tuned = {K_: tune_waves(tr, d_start, 1, npaths, K_, b_, p_)
+             for K_, b_, p_ in ((2, bp, prm_p), (1, bm, prm_m))}
+    for _ in range(args.warmup):
+        run_mixed(tr, d_start, h_start, npaths, bm, ref, ev, prm_m, lib)
+        run_plain(tr, d_start, npaths, bp, ev, prm_p)
+    check(lib.phc_synchronize())
+    rows = [run_mixed(tr, d_start, h_start, npaths, bm, ref, ev, prm_m, lib) for _ in range(args.steps)]
+    t_plain = float(np.mean([run_plain(tr, d_start, npaths, bp, ev, prm_p) for _ in range(args.steps)]))
+    check(lib.phc_synchronize())
+    om, op = bm.to_host(), bp.to_host()
+    xm, res_m, ok_m = verify(tr, om, ref, 2)
+    xp, res_p, ok_p = verify(tr, op, None, 2)
+    cm, cp = planes_to_points(xm, 4), planes_to_points(xp, 4)
+    t_track = float(np.mean([r[0] for r in rows]))
+    t_m = float(np.mean([r[0] + r[1] for r in rows]))
+    v_m, v_p = ok_m.sum() / (t_m / 1e3), ok_p.sum() / (t_plain / 1e3)
+    names = ["running", "success", "minstep", "maxsteps", "singular", "diverged"]
+    cls = lambda st: {names[i]: int(c) for i, c in enumerate(np.bincount(st, minlength=6)) if c}
+    line = {
+        "metric": "paths/sec at <=1e-24 endpoint residual, 14 dense quadratics with "
+                  "rank-deficient leading forms (16384 paths), mixed vs plain dd",
+        "value": round(float(v_m), 1), "unit": "paths/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_m, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "total-degree homotopy, 14 dense complex quadratics, top-degree "
+                               f"forms of {rank_deficient} replaced by random combinations of two "
+                               "others, degree-8 series predictor",
+                   "paths": npaths, "seed": seed, "residual_tol": RES_TOL},
+        "mixed": {"ms": round(t_m, 3), "track_ms": round(t_track, 3),
+                  "good_paths": int(ok_m.sum()), "tracker_statuses": cls(om["status"]),
+                  "escalated_paths": int(rows[-1][2]),
+                  "max_good_residual": float(res_m[ok_m].max()) if ok_m.any() else None,
+                  "distinct_solutions": distinct(cm, ok_m),
+                  "steps_per_path": float(om["nsteps"].mean())},
+        "plain_dd": {"ms": round(t_plain, 3), "paths_per_s": round(float(v_p), 1),
+                     "good_paths": int(ok_p.sum()), "tracker_statuses": cls(op["status"]),
+                     "max_good_residual": float(res_p[ok_p].max()) if ok_p.any() else None,
+                     "distinct_solutions": distinct(cp, ok_p),
+                     "steps_per_path": float(op["nsteps"].mean())},
+        "lost_paths_vs_plain": int(count_lost(cp, ok_p, cm, ok_m)),
+        "speedup_mixed_vs_plain": round(float(v_m / v_p), 3) if v_p > 0 else None,
+        "waves_per_workgroup": {"plain_dd": tuned[2][0], "fp64": tuned[1][0]},
+        "roofline": None,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -224,12 +224,15 @@ int phc_mlp_train(int F, int H, int C, float *w_dev, int N, const float *X_dev,
 /* Newton refinement at fixed t in precision K (1, 2, 4; octo-double K = 8
  * is refused with -2 in this build) of points with KS limbs; xout [2*K][npaths][n]; res = max_i |H_i(xout, t)| evaluated in
  * precision K; status 1 converged (max|dx| <= tol*(1+max|x|)), 0 not, -1
- * singular.  iters = 0 only evaluates the residual.
+ * singular.  iters = 0 only evaluates the residual.  only_dev (NULL: every
+ * path) holds the tracker's status per path: paths whose status is not 1
+ * (success) are copied through unrefined with res = dx = inf and status =
+ * their tracker status, so diverged or stalled paths cost no Newton work.
  * Plays the role of the Newton drivers at the endpoints
  * (Newton/dbl2_newton_method.cpp:429, row Newton step) in dd/qd/od. */
 int phc_refine(phc_tracker *tracker, int K, int npaths, double t, const double *xin_dev, int KS,
                double *xout_dev, int iters, double tol, double *dxlast_dev, double *res_dev,
-               int *status_dev, int waves, void *stream);
+               int *status_dev, const int *only_dev, int waves, void *stream);
 
 /* ---------------- linearized power-series Newton (real series) ----------
  * Layouts are limb-planar with the batch index after the limb:
@@ -307,6 +310,13 @@ int phc_baqr_destroy(phc_baqr *plan);
  * and its dd twin GPU_cmplx2_blocked_houseqr (dbl2_baqr_kernels.cu). */
 int phc_cmplx_blocked_houseqr(phc_baqr *plan, const double *A_dev, double *Q_dev, double *R_dev,
                               void *stream);
+
+/* y = Q^H b for the plan's nrows x nrows Q and b, y of length nrows (y must
+ * not alias b): GPU_cmplx4_bals_qhb (Newton/dbl4_bals_kernels.cu:602), the
+ * right-hand side step of the least squares solve before the tiled back
+ * substitution. */
+int phc_cmplx_qhb(phc_baqr *plan, const double *Q_dev, const double *b_dev, double *y_dev,
+                  void *stream);
 
 /* U x = b for upper triangular U (dim = szt*nbt): the diagonal tiles are
  * inverted, then the tiles are applied from the last one up.  As in the

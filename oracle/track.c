@@ -56,9 +56,38 @@ static void cneg(int K, double *z) {
   int i;
   for (i = 0; i < 2 * K; i++) z[i] = -z[i];
 }
+/* FP64 tracking (K = 1) forms complex products with fused multiply-adds,
+ * as the kernel does; K > 1 keeps the multiple-double operation order. */
+static void tc_mul(int K, const double *x, const double *y, double *z) {
+  if (K == 1) {
+    const double re = fma(x[0], y[0], -(x[1] * y[1]));
+    const double im = fma(x[0], y[1], x[1] * y[0]);
+    z[0] = re;
+    z[1] = im;
+  } else {
+    omd_cmul(K, x, y, z);
+  }
+}
+static void tc_mac(int K, const double *x, const double *y, double *z) {
+  if (K == 1) {
+    z[0] = fma(x[0], y[0], z[0]);
+    z[0] = fma(-x[1], y[1], z[0]);
+    z[1] = fma(x[1], y[0], z[1]);
+    z[1] = fma(x[0], y[1], z[1]);
+  } else {
+    omd_cmac(K, x, y, z);
+  }
+}
 /* z -= x*y in the kernel's order (C<K>::msub) */
 static void cmsub(int K, const double *x, const double *y, double *z) {
   double t[8];
+  if (K == 1) {
+    z[0] = fma(-x[0], y[0], z[0]);
+    z[0] = fma(x[1], y[1], z[0]);
+    z[1] = fma(-x[1], y[0], z[1]);
+    z[1] = fma(-x[0], y[1], z[1]);
+    return;
+  }
   omd_mul(K, x, y, t);
   omd_sub(K, z, t, z);
   omd_mul(K, x + K, y + K, t);
@@ -120,12 +149,12 @@ static void product_pass(const otr_program *P, work_t *w, int k) {
   for (i = 0; i < P->nprod; i++) {
     const int *nd = P->nodes + 8 * i;
     /* terms q = 0..k alternate between two accumulators (even, odd q) */
-    omd_cmul(K, slot(w, nd[0], -1, 0), slot(w, nd[2], -1, k), acc);
+    tc_mul(K, slot(w, nd[0], -1, 0), slot(w, nd[2], -1, k), acc);
     if (k >= 1) {
       double a1[16];
-      omd_cmul(K, slot(w, nd[0], -1, 1), slot(w, nd[2], -1, k - 1), a1);
+      tc_mul(K, slot(w, nd[0], -1, 1), slot(w, nd[2], -1, k - 1), a1);
       for (q = 2; q <= k; q++)
-        omd_cmac(K, slot(w, nd[0], -1, q), slot(w, nd[2], -1, k - q), (q & 1) ? a1 : acc);
+        tc_mac(K, slot(w, nd[0], -1, q), slot(w, nd[2], -1, k - q), (q & 1) ? a1 : acc);
       omd_cadd(K, acc, a1, acc);
     }
     ccopy(K, acc, slot(w, nd[3], nd[4], k));
@@ -140,8 +169,8 @@ static void correction_pass(const otr_program *P, work_t *w, int k) {
   for (i = 0; i < P->nprod; i++) {
     const int *nd = P->nodes + 8 * i;
     da = (nd[5] < 0) ? slot(w, nd[0], -1, k) : ZP(w->dnode, nd[5]);
-    omd_cmul(K, slot(w, nd[0], -1, 0), slot(w, nd[2], -1, k), d);
-    omd_cmac(K, da, slot(w, nd[2], -1, 0), d);
+    tc_mul(K, slot(w, nd[0], -1, 0), slot(w, nd[2], -1, k), d);
+    tc_mac(K, da, slot(w, nd[2], -1, 0), d);
     if (nd[6] >= 0) ccopy(K, d, ZP(w->dnode, nd[6]));
     omd_cadd(K, slot(w, nd[3], nd[4], k), d, slot(w, nd[3], nd[4], k));
   }
@@ -154,9 +183,9 @@ static void residual(const otr_program *P, work_t *w, int k) {
   int q, l, i, g, t;
   for (q = 0; q < P->npair; q++) {
     const int *pr = P->pairs + 4 * q;
-    omd_cmul(K, ZP(w->W, 3 * pr[0]), slot(w, pr[1], pr[2], k), acc);
+    tc_mul(K, ZP(w->W, 3 * pr[0]), slot(w, pr[1], pr[2], k), acc);
     for (l = 1; l <= P->wdeg && l <= k; l++)
-      omd_cmac(K, ZP(w->W, 3 * pr[0] + l), slot(w, pr[1], pr[2], k - l), acc);
+      tc_mac(K, ZP(w->W, 3 * pr[0] + l), slot(w, pr[1], pr[2], k - l), acc);
     ccopy(K, acc, ZP(w->u, q));
   }
   for (i = 0; i < P->n; i++) {
@@ -168,7 +197,7 @@ static void residual(const otr_program *P, work_t *w, int k) {
       /* term t goes to accumulator (t - first) mod 4 */
       for (t = P->seg_ptr[s]; t < P->seg_ptr[s + 1]; t++) {
         coef(P, K, t, c);
-        omd_cmac(K, c, ZP(w->u, P->term_pair[t]), a4[(t - P->seg_ptr[s]) & 3]);
+        tc_mac(K, c, ZP(w->u, P->term_pair[t]), a4[(t - P->seg_ptr[s]) & 3]);
       }
       omd_cadd(K, a4[0], a4[1], a4[0]);
       omd_cadd(K, a4[2], a4[3], a4[2]);
@@ -186,7 +215,7 @@ static void jacobian(const otr_program *P, work_t *w) {
   double acc[16], c[16], v[16];
   int q, e, t;
   for (q = 0; q < P->njpair; q++)
-    omd_cmul(K, ZP(w->W, 3 * P->jpairs[2 * q]), ZP(w->slots, P->jpairs[2 * q + 1]), ZP(w->uj, q));
+    tc_mul(K, ZP(w->W, 3 * P->jpairs[2 * q]), ZP(w->slots, P->jpairs[2 * q + 1]), ZP(w->uj, q));
   for (e = 0; e < P->n * P->n; e++) {
     double a4[4][16];
     int r;
@@ -198,7 +227,7 @@ static void jacobian(const otr_program *P, work_t *w) {
       const int ex = jt >> (JT_TERM_BITS + JT_PAIR_BITS);
       if (ex == 1) ccopy(K, ZP(w->uj, qq), v); else cscale_d(K, ZP(w->uj, qq), (double)ex, v);
       coef(P, K, term, c);
-      omd_cmac(K, c, v, a4[(t - P->jent_ptr[e]) & 3]);
+      tc_mac(K, c, v, a4[(t - P->jent_ptr[e]) & 3]);
     }
     omd_cadd(K, a4[0], a4[1], a4[0]);
     omd_cadd(K, a4[2], a4[3], a4[2]);
@@ -235,14 +264,14 @@ static int gj_invert(const otr_program *P, work_t *w) {
     omd_cdiv(K, one, ZP(w->col, j), inv);
     for (c = 0; c < n; c++) {
       x = ZP(w->A, j * n + c);
-      if (c == j) ccopy(K, inv, x); else omd_cmul(K, x, inv, x);
+      if (c == j) ccopy(K, inv, x); else tc_mul(K, x, inv, x);
     }
     for (e = 0; e < n * n; e++) {
       i = e / n;
       c = e - i * n;
       if (i == j) continue;
       if (c == j) {
-        omd_cmul(K, ZP(w->col, i), inv, ZP(w->A, e));
+        tc_mul(K, ZP(w->col, i), inv, ZP(w->A, e));
         cneg(K, ZP(w->A, e));
       } else {
         cmsub(K, ZP(w->col, i), ZP(w->A, j * n + c), ZP(w->A, e));
@@ -261,7 +290,7 @@ static void apply_inverse(const otr_program *P, work_t *w, double *y) {
       const int jlo = (g * n) / G, jhi = ((g + 1) * n) / G;
       double *acc = ZP(w->part, g);
       czero(K, acc);
-      for (j = jlo; j < jhi; j++) omd_cmac(K, ZP(w->A, i * n + j), ZP(w->rhs, w->perm[j]), acc);
+      for (j = jlo; j < jhi; j++) tc_mac(K, ZP(w->A, i * n + j), ZP(w->rhs, w->perm[j]), acc);
     }
     tree_sum(K, w->part, G);
     ccopy(K, ZP(w->part, 0), ZP(y, i));

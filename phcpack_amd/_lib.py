@@ -96,7 +96,7 @@ SIGNATURES.update({
                               c_int, c_int, ctypes.c_float, ctypes.c_uint,
                               ctypes.POINTER(ctypes.c_float), c_void_p]),
     "phc_refine": (c_int, [c_void_p, c_int, c_int, c_double, c_void_p, c_int, c_void_p, c_int,
-                           c_double, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+                           c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "phc_houseqr_batch": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "phc_qrbs_solve_batch": (c_int, [c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
                                      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -111,6 +111,7 @@ SIGNATURES.update({
     "phc_baqr_create": (c_int, [ctypes.POINTER(c_void_p), c_int, c_int, c_int, c_int]),
     "phc_baqr_destroy": (c_int, [c_void_p]),
     "phc_cmplx_blocked_houseqr": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "phc_cmplx_qhb": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "phc_cmplx_upper_tiled_solver": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p,
                                              c_void_p, c_void_p]),
     "phc_cmplx_gemm": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int,

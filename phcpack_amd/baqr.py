@@ -35,6 +35,10 @@ class BlockedQR:
     def factor_device(self, dA, dQ, dR, stream=None):
         check(load().phc_cmplx_blocked_houseqr(self.handle, dA.ptr, dQ.ptr, dR.ptr, stream))
 
+    def qhb_device(self, dQ, db, dy, stream=None):
+        """y = Q^H b (GPU_cmplx4_bals_qhb, Newton/dbl4_bals_kernels.cu:602)"""
+        check(load().phc_cmplx_qhb(self.handle, dQ.ptr, db.ptr, dy.ptr, stream))
+
     def factor(self, A):
         """Q [2K][m][m], R [2K][m][n] of A [2K][m][n]"""
         K, m, n = self.K, self.nrows, self.ncols

@@ -6,9 +6,11 @@
 //   CPU_cmplx4_blocked_houseqr (dbl4_baqr_host.cpp:1206),
 //   GPU_cmplx4_blocked_houseqr (dbl4_baqr_kernels.cu:6197)
 //     per column: house, panel leftRupdate      house_kernel, panel_dot_kernel,
+//                                               panel_reduce_kernel,
 //                                               panel_update_kernel (many workgroups
 //                                               per column, partial sums reduced in
-//                                               a fixed order)
+//                                               a fixed order); the launches of one
+//                                               factorization replay as a HIP graph
 //     VB_to_W (dbl4_baqr_host.cpp:91)           W = Y S with S upper triangular from
 //                                               the Gram matrix G = Y^H Y: one
 //                                               wavefront per row of S
@@ -16,14 +18,18 @@
 //       C += (Y W^H) C                          instead of the rowdim x rowdim Y W^H
 //     blocked rightQupdate (:743):              Q += (Q W) Y^H, likewise
 //       Q += Q (W Y^H)
-//   CPU_cmplx4_upper_tiled_solver               tile_inverse_kernel (one workgroup per
-//     (dbl4_tabs_host.cpp:345),                 diagonal tile, one lane per column),
-//   GPU_cmplx4_upper_tiled_solver               tiled_solve_kernel (tile products in
-//     (dbl4_tabs_kernels.cu:1802)               order, one workgroup)
+//   CPU_cmplx4_upper_tiled_solver               tile_inverse_kernel (one wavefront per
+//     (dbl4_tabs_host.cpp:345),                 column of every diagonal tile), then
+//   GPU_cmplx4_upper_tiled_solver               rowdot_kernel stages (one wavefront
+//     (dbl4_tabs_kernels.cu:1802)               per row: x_k = inv_k b_k, then
+//                                               b_L -= U_Lk x_k for all L < k)
+//   GPU_cmplx4_bals_qhb                         the GEMM with a split k range
+//     (Newton/dbl4_bals_kernels.cu:602)
 //
 // Every product runs through one complex multiple-double GEMM kernel (32 x 32
-// output tiles, 8-deep k slices staged in LDS, 2 x 2 outputs per lane, each
-// output summed in ascending k).  Results agree with the reference's host
+// output tiles, 8-deep k slices staged in LDS, 2 x 2 outputs per lane); long
+// k ranges are split over workgroups whose partial sums are added in slice
+// order by a second kernel.  Results agree with the reference's host
 // code to a few units in the last limb (the order of the long sums differs);
 // the tests bound the difference.
 #include <vector>
@@ -59,26 +65,30 @@ __device__ __forceinline__ void cmac(Z<K> &acc, const Z<K> &a, const Z<K> &b) {
 constexpr int TM = 32, TN = 32, TKD = 8;
 
 // C(i, j) = [C(i, j) +] sum_k op(A(i, k)) op(B(k, j)), i < M, j < N, k < KD;
-// the sum starts from zero and is added to C at the end.
+// the sum starts from zero and is added to C at the end.  With a split of
+// the k range over gridDim.z slices, slice s writes its partial sums to
+// part[s][i][j] (plane stride pps) and csum_kernel adds the slices in order.
 template <int K, bool CA, bool CB>
 __global__ void __launch_bounds__(256)
-cgemm_kernel(int M, int N, int KD, Op A, Op B, double *Cp, size_t cps, long c0, long c1,
-             int accumulate) {
+cgemm_kernel(int M, int N, int KD, int kchunk, Op A, Op B, double *Cp, size_t cps, long c0,
+             long c1, int accumulate, double *part, size_t pps) {
   __shared__ double As[2 * K][TKD][TM];
   __shared__ double Bs[2 * K][TKD][TN];
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
   const int i0 = blockIdx.y * TM, j0 = blockIdx.x * TN;
+  const int kb = blockIdx.z * kchunk;
+  const int ke = (kb + kchunk) < KD ? (kb + kchunk) : KD;
   Z<K> acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = C<K>::zero();
-  for (int k0 = 0; k0 < KD; k0 += TKD) {
+  for (int k0 = kb; k0 < ke; k0 += TKD) {
     {
       int ii, kk;
       if (A.s0 == 1) { ii = t % TM; kk = t / TM; } else { kk = t % TKD; ii = t / TKD; }
       const int gi = i0 + ii, gk = k0 + kk;
-      const bool in = gi < M && gk < KD;
+      const bool in = gi < M && gk < ke;
       const size_t e = in ? (size_t)(A.s0 * gi + A.s1 * gk) : 0;
 #pragma unroll
       for (int q = 0; q < 2 * K; ++q) As[q][kk][ii] = in ? A.p[q * A.ps + e] : 0.0;
@@ -87,13 +97,13 @@ cgemm_kernel(int M, int N, int KD, Op A, Op B, double *Cp, size_t cps, long c0, 
       int jj, kk;
       if (B.s1 == 1) { jj = t % TN; kk = t / TN; } else { kk = t % TKD; jj = t / TKD; }
       const int gj = j0 + jj, gk = k0 + kk;
-      const bool in = gj < N && gk < KD;
+      const bool in = gj < N && gk < ke;
       const size_t e = in ? (size_t)(B.s0 * gk + B.s1 * gj) : 0;
 #pragma unroll
       for (int q = 0; q < 2 * K; ++q) Bs[q][kk][jj] = in ? B.p[q * B.ps + e] : 0.0;
     }
     __syncthreads();
-    const int kmax = (KD - k0) < TKD ? (KD - k0) : TKD;
+    const int kmax = (ke - k0) < TKD ? (ke - k0) : TKD;
     for (int kk = 0; kk < kmax; ++kk) {
       Z<K> av[2], bv[2];
 #pragma unroll
@@ -113,28 +123,63 @@ cgemm_kernel(int M, int N, int KD, Op A, Op B, double *Cp, size_t cps, long c0, 
     for (int b = 0; b < 2; ++b) {
       const int i = i0 + ty + 16 * a, j = j0 + tx + 16 * b;
       if (i < M && j < N) {
-        const size_t e = (size_t)(c0 * i + c1 * j);
-        Z<K> v = acc[a][b];
-        if (accumulate) v = C<K>::add(C<K>::load(Cp, cps, e), v);
-        C<K>::store(Cp, cps, e, v);
+        if (gridDim.z > 1) {
+          C<K>::store(part, pps, (size_t)blockIdx.z * M * N + (size_t)i * N + j, acc[a][b]);
+        } else {
+          const size_t e = (size_t)(c0 * i + c1 * j);
+          Z<K> v = acc[a][b];
+          if (accumulate) v = C<K>::add(C<K>::load(Cp, cps, e), v);
+          C<K>::store(Cp, cps, e, v);
+        }
       }
     }
 }
 
+// C(i, j) = [C(i, j) +] sum_s part[s][i][j], slices in order
+template <int K>
+__global__ void csum_kernel(int M, int N, int S, const double *part, size_t pps, double *Cp,
+                            size_t cps, long c0, long c1, int accumulate) {
+  const size_t n = (size_t)M * N;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    Z<K> v = C<K>::load(part, pps, e);
+    for (int s = 1; s < S; ++s) v = C<K>::add(v, C<K>::load(part, pps, (size_t)s * n + e));
+    const size_t i = e / N, j = e % N;
+    const size_t o = (size_t)(c0 * (long)i + c1 * (long)j);
+    if (accumulate) v = C<K>::add(C<K>::load(Cp, cps, o), v);
+    C<K>::store(Cp, cps, o, v);
+  }
+}
+
+// split of the k range: enough workgroups to fill the chip, slices of at
+// least 64 terms, and M*N*S partial sums within the workspace
+inline int ksplit(int M, int N, int KD, size_t cap) {
+  const long tiles = (long)((M + TM - 1) / TM) * ((N + TN - 1) / TN);
+  int S = 1;
+  while (tiles * S < 1024 && KD / (2 * S) >= 64 && (size_t)M * N * 2 * S <= cap) S *= 2;
+  return S;
+}
+
 template <int K>
 int cgemm(bool ca, bool cb, int M, int N, int KD, Op A, Op B, double *Cp, size_t cps, long c0,
-          long c1, bool accumulate, hipStream_t st) {
+          long c1, bool accumulate, hipStream_t st, double *part = nullptr, size_t cap = 0) {
   if (M <= 0 || N <= 0) return 0;
-  dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM);
+  const int S = part ? ksplit(M, N, KD, cap) : 1;
+  const int kchunk = ((KD + S - 1) / S + TKD - 1) / TKD * TKD;
+  dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, S);
   const int acc = accumulate ? 1 : 0;
-  if (!ca && !cb)
-    hipLaunchKernelGGL((cgemm_kernel<K, false, false>), grid, dim3(256), 0, st, M, N, KD, A, B, Cp, cps, c0, c1, acc);
-  else if (ca && !cb)
-    hipLaunchKernelGGL((cgemm_kernel<K, true, false>), grid, dim3(256), 0, st, M, N, KD, A, B, Cp, cps, c0, c1, acc);
-  else if (!ca && cb)
-    hipLaunchKernelGGL((cgemm_kernel<K, false, true>), grid, dim3(256), 0, st, M, N, KD, A, B, Cp, cps, c0, c1, acc);
-  else
-    hipLaunchKernelGGL((cgemm_kernel<K, true, true>), grid, dim3(256), 0, st, M, N, KD, A, B, Cp, cps, c0, c1, acc);
+  const size_t pps = cap;
+#define GEMM(CA_, CB_)                                                                         \
+  hipLaunchKernelGGL((cgemm_kernel<K, CA_, CB_>), grid, dim3(256), 0, st, M, N, KD, kchunk, A, \
+                     B, Cp, cps, c0, c1, acc, part, pps)
+  if (!ca && !cb) GEMM(false, false);
+  else if (ca && !cb) GEMM(true, false);
+  else if (!ca && cb) GEMM(false, true);
+  else GEMM(true, true);
+#undef GEMM
+  if (S > 1)
+    hipLaunchKernelGGL(csum_kernel<K>, dim3(phc::grid_for((size_t)M * N, 256)), dim3(256), 0, st,
+                       M, N, S, part, pps, Cp, cps, c0, c1, acc);
   PHC_HIP(hipGetLastError());
   return 0;
 }
@@ -144,14 +189,17 @@ int cgemm(bool ca, bool cb, int M, int N, int KD, Op A, Op B, double *Cp, size_t
 // Householder vector of column colidx of R (rows colidx..nrows-1), as
 // CPU_cmplx2_factors_house (Matrices/dbl2_factorizations.cpp:441-533):
 // v is written to Y row L from position L on (zeros before), beta to B[L].
+// sigma is a tree sum over 256 lanes; the independent scalar operations run
+// on separate lanes (the two square roots, then beta and the two parts of
+// 1/v0).
 template <int K>
 __global__ void __launch_bounds__(256)
 house_kernel(const double *Rm, size_t rps, int ld, int colidx, int nrows, int L, double *Y,
              size_t yps, int rowdim, double *Bv, int szt) {
   __shared__ T<K> part[256];
-  __shared__ T<K> sbeta;
-  __shared__ Z<K> sinv;
-  __shared__ int szero;
+  __shared__ T<K> sh[4];
+  __shared__ Z<K> sv0, sinv;
+  __shared__ int szero, rzero;
   const int t = threadIdx.x, n = nrows - colidx;
   auto x = [&](int i) { return C<K>::load(Rm, rps, (size_t)(colidx + i) * ld + colidx); };
   T<K> s = R<K>::zero();
@@ -166,43 +214,50 @@ house_kernel(const double *Rm, size_t rps, int ld, int colidx, int nrows, int L,
     if (t < h) part[t] = add(part[t], part[t + h]);
     __syncthreads();
   }
-  if (t == 0) {
-    T<K> sigma = part[0];
-    bool zero = true;
-    for (int q = 0; q < K; ++q) zero = zero && (sigma.x[q] == 0.0);
-    szero = zero ? 1 : 0;
-    if (zero) {
-      sbeta = R<K>::zero();
-      sinv = C<K>::from(1.0, 0.0);
-    } else {
-      const Z<K> x0 = x(0);
-      T<K> sqrx0 = sqr(x0.re);
-      sqrx0 = add(sqrx0, sqr(x0.im));
-      const T<K> x0rad = R<K>::sqrt(sqrx0);
-      sqrx0 = add(sqrx0, sigma);
-      T<K> mu = R<K>::sqrt(sqrx0);
-      Z<K> v0;
-      bool rzero = true;
-      for (int q = 0; q < K; ++q) rzero = rzero && (x0rad.x[q] == 0.0);
-      if (rzero) {
-        v0.re = neg(mu);
-        v0.im = R<K>::zero();
-      } else {
-        mu = div(mu, x0rad);
-        v0.re = sub(x0.re, mul(mu, x0.re));
-        v0.im = sub(x0.im, mul(mu, x0.im));
-      }
-      T<K> sqrv0 = sqr(v0.re);
-      sqrv0 = add(sqrv0, sqr(v0.im));
-      sigma = add(sigma, sqrv0);
-      T<K> beta = div(sqrv0, sigma);
-      sbeta = mul_d(beta, 2.0);
-      sinv.re = div(v0.re, sqrv0);
-      sinv.im = neg(div(v0.im, sqrv0));
-    }
-    R<K>::store(Bv, szt, L, sbeta);
+  const T<K> sigma = part[0];
+  const Z<K> x0 = x(0);
+  if (t < 2) {  // lane 0: |x0|, lane 1: sqrt(|x0|^2 + sigma)
+    T<K> sq = sqr(x0.re);
+    sq = add(sq, sqr(x0.im));
+    if (t == 1) sq = add(sq, sigma);
+    sh[t] = R<K>::sqrt(sq);
   }
   __syncthreads();
+  if (t == 0) {
+    bool zero = true, rz = true;
+    for (int q = 0; q < K; ++q) {
+      zero = zero && (sigma.x[q] == 0.0);
+      rz = rz && (sh[0].x[q] == 0.0);
+    }
+    szero = zero ? 1 : 0;
+    rzero = rz ? 1 : 0;
+    Z<K> v0;
+    if (rz) {
+      v0.re = neg(sh[1]);
+      v0.im = R<K>::zero();
+    } else {
+      const T<K> mu = div(sh[1], sh[0]);
+      v0.re = sub(x0.re, mul(mu, x0.re));
+      v0.im = sub(x0.im, mul(mu, x0.im));
+    }
+    sv0 = v0;
+    T<K> sqrv0 = sqr(v0.re);
+    sh[2] = add(sqrv0, sqr(v0.im));
+  }
+  __syncthreads();
+  if (t < 3 && !szero) {  // lane 0: beta, lanes 1 and 2: 1/v0 = conj(v0)/|v0|^2
+    const T<K> sqrv0 = sh[2];
+    if (t == 0) {
+      const T<K> beta = div(sqrv0, add(sigma, sqrv0));
+      sh[3] = mul_d(beta, 2.0);
+    } else if (t == 1) {
+      sinv.re = div(sv0.re, sqrv0);
+    } else {
+      sinv.im = neg(div(sv0.im, sqrv0));
+    }
+  }
+  __syncthreads();
+  if (t == 0) R<K>::store(Bv, szt, L, szero ? R<K>::zero() : sh[3]);
   const Z<K> inv = sinv;
   const bool zero = szero != 0;
   for (int i = t; i < rowdim; i += 256) {
@@ -221,7 +276,7 @@ house_kernel(const double *Rm, size_t rps, int ld, int colidx, int nrows, int L,
   }
 }
 
-constexpr int RCH = 64;  // rows per chunk of the panel dot products
+constexpr int RCH = 8;  // rows per chunk of the panel dot products
 
 // partial sums wp[chunk][j] = sum_{i in chunk} conj(R_ij) v_i over columns
 // j = colidx..endcol-1 (one lane per column)
@@ -234,9 +289,9 @@ panel_dot_kernel(const double *Rm, size_t rps, int ld, int colidx, int endcol, i
   const int chunk = blockIdx.x, t = threadIdx.x;
   const int r0 = colidx + chunk * RCH, n = nrows - colidx;
   const int rows = (n - chunk * RCH) < RCH ? (n - chunk * RCH) : RCH;
-  for (int i = t; i < rows; i += blockDim.x) {
-    const Z<K> v = C<K>::load(Y, yps, (size_t)L * rowdim + L + chunk * RCH + i);
-    for (int q = 0; q < K; ++q) { vs[q][i] = v.re.x[q]; vs[K + q][i] = v.im.x[q]; }
+  if (t < rows) {
+    const Z<K> v = C<K>::load(Y, yps, (size_t)L * rowdim + L + chunk * RCH + t);
+    for (int q = 0; q < K; ++q) { vs[q][t] = v.re.x[q]; vs[K + q][t] = v.im.x[q]; }
   }
   __syncthreads();
   const int j = colidx + t;
@@ -250,20 +305,38 @@ panel_dot_kernel(const double *Rm, size_t rps, int ld, int colidx, int endcol, i
   C<K>::store(wp, wps, (size_t)chunk * szt + t, s);
 }
 
-// w_j = beta * sum_chunks wp[.][j]; R_ij -= v_i conj(w_j) for the rows of
-// this workgroup
+// w_j = beta * sum_chunks wp[.][j]: 4 lanes per column sum interleaved
+// chunks, then a two-level tree
+template <int K>
+__global__ void __launch_bounds__(512)
+panel_reduce_kernel(const double *wp, size_t wps, int szt, int ncol, int nchunk,
+                    const double *Bv, int L, double *w) {
+  __shared__ Z<K> part[4][128];
+  const int c = threadIdx.x & 127, g = threadIdx.x >> 7;
+  Z<K> s = C<K>::zero();
+  if (c < ncol)
+    for (int k = g; k < nchunk; k += 4) s = C<K>::add(s, C<K>::load(wp, wps, (size_t)k * szt + c));
+  part[g][c] = s;
+  __syncthreads();
+  if (g == 0 && c < ncol) {
+    const Z<K> a = C<K>::add(part[0][c], part[1][c]), b = C<K>::add(part[2][c], part[3][c]);
+    Z<K> v = C<K>::add(a, b);
+    const T<K> beta = R<K>::load(Bv, szt, L);
+    v.re = mul(beta, v.re);
+    v.im = mul(beta, v.im);
+    C<K>::store(w, szt, c, v);
+  }
+}
+
+// R_ij -= v_i conj(w_j) for the rows of this workgroup (one lane per column)
 template <int K>
 __global__ void __launch_bounds__(128)
 panel_update_kernel(double *Rm, size_t rps, int ld, int colidx, int endcol, int nrows,
-                    const double *Y, size_t yps, int rowdim, int L, const double *wp,
-                    size_t wps, int szt, const double *Bv, int nchunk, int rows_per_block) {
+                    const double *Y, size_t yps, int rowdim, int L, const double *w, int szt,
+                    int rows_per_block) {
   const int t = threadIdx.x, j = colidx + t;
   if (j >= endcol) return;
-  const T<K> beta = R<K>::load(Bv, szt, L);
-  Z<K> w = C<K>::zero();
-  for (int c = 0; c < nchunk; ++c) w = C<K>::add(w, C<K>::load(wp, wps, (size_t)c * szt + t));
-  w.re = mul(beta, w.re);
-  w.im = mul(beta, w.im);
+  const Z<K> wj = C<K>::load(w, szt, t);
   const int ib = colidx + blockIdx.x * rows_per_block;
   const int ie = (ib + rows_per_block) < nrows ? ib + rows_per_block : nrows;
   for (int i = ib; i < ie; ++i) {
@@ -271,8 +344,8 @@ panel_update_kernel(double *Rm, size_t rps, int ld, int colidx, int endcol, int 
     const size_t e = (size_t)i * ld + j;
     Z<K> r = C<K>::load(Rm, rps, e);
     // r -= v * conj(w)
-    r.re = sub(r.re, add(mul(v.re, w.re), mul(v.im, w.im)));
-    r.im = sub(r.im, sub(mul(v.im, w.re), mul(v.re, w.im)));
+    r.re = sub(r.re, add(mul(v.re, wj.re), mul(v.im, wj.im)));
+    r.im = sub(r.im, sub(mul(v.im, wj.re), mul(v.re, wj.im)));
     C<K>::store(Rm, rps, e, r);
   }
 }
@@ -315,65 +388,66 @@ s_factor_kernel(const double *G, size_t gps, const double *Bv, double *S, size_t
 
 // ------------------------------------------------------- tiled solver
 
-// inverse of the diagonal tile `blockIdx.x` of U (dim x dim), one lane per
-// column: x = T^{-1} e_j by back substitution (CPU_cmplx2_factors_backward
-// order), written to inv[tile][szt][szt]
-template <int K>
-__global__ void __launch_bounds__(256)
-tile_inverse_kernel(const double *U, size_t ups, int dim, int szt, double *inv, size_t ips) {
-  const int tile = blockIdx.x, off = tile * szt;
-  const size_t ib = (size_t)tile * szt * szt;
-  for (int j = threadIdx.x; j < szt; j += blockDim.x) {
-    for (int i = szt - 1; i >= 0; --i) {
-      Z<K> x = (i == j) ? C<K>::from(1.0, 0.0) : C<K>::zero();
-      if (i <= j) {
-        for (int k = j; k > i; --k) {
-          const Z<K> u = C<K>::load(U, ups, (size_t)(off + i) * dim + off + k);
-          const Z<K> xk = C<K>::load(inv, ips, ib + (size_t)k * szt + j);
-          x = C<K>::sub(x, C<K>::mul(u, xk));
-        }
-        const Z<K> d = C<K>::load(U, ups, (size_t)(off + i) * dim + off + i);
-        x = C<K>::div(x, d);
-      }
-      C<K>::store(inv, ips, ib + (size_t)i * szt + j, x);
+// butterfly sum of a complex number over the 64 lanes of a wavefront
+template <int K> __device__ Z<K> wave_csum(Z<K> s) {
+  for (int o = 32; o > 0; o >>= 1) {
+    Z<K> u;
+    for (int q = 0; q < K; ++q) {
+      u.re.x[q] = __shfl_xor(s.re.x[q], o, 64);
+      u.im.x[q] = __shfl_xor(s.im.x[q], o, 64);
     }
+    s = C<K>::add(s, u);
   }
+  return s;
 }
 
-// the tile stages of CPU_cmplx2_upper_tiled_solver (Matrices/dbl2_tabs_host.cpp:289-513)
-// with the inverses precomputed: x_last = invT b_last; for k = nbt-1..1:
-// b_L -= U_{L,k} x_k (L < k), x_{k-1} = invT_{k-1} b_{k-1}.  One workgroup.
+// inverse of the diagonal tiles of U (dim x dim): one wavefront per column j
+// of tile t, x = T^{-1} e_j by back substitution with the sums over k split
+// across lanes; x kept in LDS, written to inv[t][szt][szt]
 template <int K>
 __global__ void __launch_bounds__(256)
-tiled_solve_kernel(const double *U, size_t ups, int dim, int szt, int nbt, const double *inv,
-                   size_t ips, double *b, size_t bps, double *x, size_t xps) {
-  const int t = threadIdx.x;
-  auto matvec = [&](int row0, const double *M, size_t mps, size_t mb, int ldm, int col0,
-                    int rowx, bool sub_from_b) {
-    for (int i = t; i < szt; i += blockDim.x) {
+tile_inverse_kernel(const double *U, size_t ups, int dim, int szt, int ntiles, double *inv,
+                    size_t ips) {
+  __shared__ double xs[4][2 * K][256];
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int id = blockIdx.x * 4 + wv;
+  if (id >= ntiles * szt) return;
+  const int tile = id / szt, j = id % szt, off = tile * szt;
+  double *x = &xs[wv][0][0];
+  const size_t ib = (size_t)tile * szt * szt;
+  for (int i = szt - 1; i >= 0; --i) {
+    Z<K> v = C<K>::zero();
+    if (i <= j) {
       Z<K> s = C<K>::zero();
-      for (int j = 0; j < szt; ++j)
-        s = C<K>::add(s, C<K>::mul(C<K>::load(M, mps, mb + (size_t)(row0 + i) * ldm + col0 + j),
-                                    C<K>::load(sub_from_b ? x : b, sub_from_b ? xps : bps,
-                                               (size_t)rowx + j)));
-      if (sub_from_b) {
-        const size_t e = (size_t)row0 + i;
-        C<K>::store(b, bps, e, C<K>::sub(C<K>::load(b, bps, e), s));
-      } else {
-        C<K>::store(x, xps, (size_t)rowx + i, s);
-      }
+      for (int k = i + 1 + l; k <= j; k += 64)
+        s = C<K>::add(s, C<K>::mul(C<K>::load(U, ups, (size_t)(off + i) * dim + off + k),
+                                    C<K>::load(x, 256, k)));
+      s = wave_csum<K>(s);
+      v = C<K>::sub(i == j ? C<K>::from(1.0, 0.0) : C<K>::zero(), s);
+      v = C<K>::div(v, C<K>::load(U, ups, (size_t)(off + i) * dim + off + i));
     }
-  };
-  // x_last
-  const int last = nbt - 1;
-  matvec(0, inv, ips, (size_t)last * szt * szt, szt, 0, last * szt, false);
-  __syncthreads();
-  for (int k = nbt - 1; k > 0; --k) {
-    for (int L = 0; L < k; ++L) matvec(L * szt, U, ups, 0, dim, k * szt, k * szt, true);
-    __syncthreads();
-    matvec(0, inv, ips, (size_t)(k - 1) * szt * szt, szt, 0, (k - 1) * szt, false);
-    __syncthreads();
+    if (l == 0) C<K>::store(x, 256, i, v);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
+  for (int i = l; i < szt; i += 64) C<K>::store(inv, ips, ib + (size_t)i * szt + j, C<K>::load(x, 256, i));
+}
+
+// rows r of y: y[r] = [y[r] -] sum_j M[r][j] v[j], one wavefront per row
+// (M row r at m + r*ldm, v at v + j); sub = 1 subtracts from y
+template <int K>
+__global__ void __launch_bounds__(256)
+rowdot_kernel(int rows, int len, const double *m, size_t mps, int ldm, const double *v,
+              size_t vps, double *y, size_t yps, int sub) {
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + wv;
+  if (r >= rows) return;
+  Z<K> s = C<K>::zero();
+  for (int j = l; j < len; j += 64)
+    s = C<K>::add(s, C<K>::mul(C<K>::load(m, mps, (size_t)r * ldm + j), C<K>::load(v, vps, j)));
+  s = wave_csum<K>(s);
+  if (l == 0) C<K>::store(y, yps, r, sub ? C<K>::sub(C<K>::load(y, yps, r), s) : s);
 }
 
 // copy the inverse tiles into the diagonal of U (the reference overwrites them)
@@ -406,8 +480,13 @@ __global__ void init_qr_kernel(int nrows, int ncols, const double *A, double *Q,
 
 struct phc_baqr {
   int K, nrows, ncols, szt, nbt, nchunk;
-  double *Y, *W, *G, *S, *T1, *T2, *wp, *B;
-  size_t yps, gps, t1ps, t2ps, wps;
+  double *Y, *W, *G, *S, *T1, *T2, *wp, *w, *B, *part;
+  size_t yps, gps, t1ps, t2ps, wps, pcap;
+  // the launch sequence of one factorization, captured for the last (A, Q, R)
+  hipStream_t cap;
+  hipGraphExec_t exec;
+  const double *gA;
+  double *gQ, *gR;
 };
 
 namespace {
@@ -415,51 +494,81 @@ namespace {
 template <int K> int run_qr(phc_baqr *p, const double *A, double *Q, double *R, hipStream_t st) {
   const int m = p->nrows, n = p->ncols, szt = p->szt;
   const size_t qps = (size_t)m * m, rps = (size_t)m * n;
-  hipLaunchKernelGGL(init_qr_kernel<K>, dim3(phc::grid_for(qps + rps, 256)), dim3(256), 0, st, m, n, A, Q, R);
+  hipLaunchKernelGGL(init_qr_kernel<K>, dim3(phc::grid_for(qps + rps, 256)), dim3(256), 0, st, m,
+                     n, A, Q, R);
   PHC_HIP(hipGetLastError());
   for (int k = 0; k < p->nbt; ++k) {
     const int rowoff = k * szt, rowdim = m - rowoff, endcol = (k + 1) * szt;
     for (int L = 0; L < szt; ++L) {
       const int colidx = rowoff + L, nr = m - colidx;
-      const int nchunk = (nr + RCH - 1) / RCH;
+      const int nchunk = (nr + RCH - 1) / RCH, ncol = endcol - colidx;
       hipLaunchKernelGGL(house_kernel<K>, dim3(1), dim3(256), 0, st, R, rps, n, colidx, m, L,
                          p->Y, p->yps, rowdim, p->B, szt);
       hipLaunchKernelGGL(panel_dot_kernel<K>, dim3(nchunk), dim3(128), 0, st, R, rps, n, colidx,
                          endcol, m, p->Y, p->yps, rowdim, L, p->wp, p->wps, szt);
-      const int rpb = 16, nblk = (nr + rpb - 1) / rpb;
+      hipLaunchKernelGGL(panel_reduce_kernel<K>, dim3(1), dim3(512), 0, st, p->wp, p->wps, szt,
+                         ncol, nchunk, p->B, L, p->w);
+      const int rpb = 8, nblk = (nr + rpb - 1) / rpb;
       hipLaunchKernelGGL(panel_update_kernel<K>, dim3(nblk), dim3(128), 0, st, R, rps, n, colidx,
-                         endcol, m, p->Y, p->yps, rowdim, L, p->wp, p->wps, szt, p->B, nchunk, rpb);
+                         endcol, m, p->Y, p->yps, rowdim, L, p->w, szt, rpb);
     }
     PHC_HIP(hipGetLastError());
     // Y, W stored as szt rows of length rowdim (row stride rowdim)
-    const Op Yt{p->Y, p->yps, 1, rowdim};          // Yhat(i, k) = Y[k][i]
-    const Op Yc{p->Y, p->yps, rowdim, 1};          // (k, i) -> Y[k][i]
+    const Op Yt{p->Y, p->yps, 1, rowdim};  // Yhat(i, k) = Y[k][i]
+    const Op Yc{p->Y, p->yps, rowdim, 1};  // (k, i) -> Y[k][i]
     // G = Yhat^H Yhat
-    if (cgemm<K>(true, false, szt, szt, rowdim, Op{p->Y, p->yps, rowdim, 1}, Yt, p->G, p->gps,
-                 szt, 1, false, st)) return -1;
+    if (cgemm<K>(true, false, szt, szt, rowdim, Yc, Yt, p->G, p->gps, szt, 1, false, st, p->part,
+                 p->pcap)) return -1;
     hipLaunchKernelGGL(s_factor_kernel<K>, dim3(szt), dim3(64), 0, st, p->G, p->gps, p->B, p->S,
                        p->gps, szt);
     PHC_HIP(hipGetLastError());
     // W = Yhat S, stored like Y
     if (cgemm<K>(false, false, rowdim, szt, szt, Yt, Op{p->S, p->gps, szt, 1}, p->W, p->yps, 1,
-                 rowdim, false, st)) return -1;
-    const Op Wt{p->W, p->yps, 1, rowdim};          // What(i, k) = W[k][i]
+                 rowdim, false, st, p->part, p->pcap)) return -1;
+    const Op Wt{p->W, p->yps, 1, rowdim};  // What(i, k) = W[k][i]
     if (k < p->nbt - 1) {
       const int coloff = endcol, coldim = n - coloff;
-      // T1 = What^H C, C = R[rowoff.., coloff..]
+      // T1 = What^H C with C = R[rowoff.., coloff..];  C += Yhat T1
       const Op Cm{R + (size_t)rowoff * n + coloff, rps, n, 1};
       if (cgemm<K>(true, false, szt, coldim, rowdim, Op{p->W, p->yps, rowdim, 1}, Cm, p->T1,
-                   p->t1ps, coldim, 1, false, st)) return -1;
-      // C += Yhat T1
+                   p->t1ps, coldim, 1, false, st, p->part, p->pcap)) return -1;
       if (cgemm<K>(false, false, rowdim, coldim, szt, Yt, Op{p->T1, p->t1ps, coldim, 1},
-                   R + (size_t)rowoff * n + coloff, rps, n, 1, true, st)) return -1;
+                   R + (size_t)rowoff * n + coloff, rps, n, 1, true, st, p->part, p->pcap))
+        return -1;
     }
     // T2 = Q[:, rowoff..] What;  Q[:, rowoff..] += T2 Yhat^H
     const Op Qm{Q + rowoff, qps, m, 1};
-    if (cgemm<K>(false, false, m, szt, rowdim, Qm, Wt, p->T2, p->t2ps, szt, 1, false, st)) return -1;
-    if (cgemm<K>(false, true, m, rowdim, szt, Op{p->T2, p->t2ps, szt, 1}, Yc, Q + rowoff, qps, m, 1,
-                 true, st)) return -1;
+    if (cgemm<K>(false, false, m, szt, rowdim, Qm, Wt, p->T2, p->t2ps, szt, 1, false, st, p->part,
+                 p->pcap)) return -1;
+    if (cgemm<K>(false, true, m, rowdim, szt, Op{p->T2, p->t2ps, szt, 1}, Yc, Q + rowoff, qps, m,
+                 1, true, st, p->part, p->pcap)) return -1;
   }
+  return 0;
+}
+
+template <int K>
+int run_tiled(int dim, int szt, int nbt, double *U, double *b, double *x, double *inv,
+              hipStream_t st) {
+  const size_t ups = (size_t)dim * dim, ips = (size_t)nbt * szt * szt;
+  hipLaunchKernelGGL(tile_inverse_kernel<K>, dim3((nbt * szt + 3) / 4), dim3(256), 0, st, U, ups,
+                     dim, szt, nbt, inv, ips);
+  // x_last = invT_last b_last; for k = nbt-1..1: b_L -= U_{L,k} x_k (L < k),
+  // x_{k-1} = invT_{k-1} b_{k-1}
+  const int last = nbt - 1;
+  hipLaunchKernelGGL(rowdot_kernel<K>, dim3((szt + 3) / 4), dim3(256), 0, st, szt, szt,
+                     inv + (size_t)last * szt * szt, ips, szt, b + (size_t)last * szt, (size_t)dim,
+                     x + (size_t)last * szt, (size_t)dim, 0);
+  for (int k = nbt - 1; k > 0; --k) {
+    hipLaunchKernelGGL(rowdot_kernel<K>, dim3((k * szt + 3) / 4), dim3(256), 0, st, k * szt, szt,
+                       U + (size_t)k * szt, ups, dim, x + (size_t)k * szt, (size_t)dim, b,
+                       (size_t)dim, 1);
+    hipLaunchKernelGGL(rowdot_kernel<K>, dim3((szt + 3) / 4), dim3(256), 0, st, szt, szt,
+                       inv + (size_t)(k - 1) * szt * szt, ips, szt, b + (size_t)(k - 1) * szt,
+                       (size_t)dim, x + (size_t)(k - 1) * szt, (size_t)dim, 0);
+  }
+  hipLaunchKernelGGL(copy_inverse_kernel<K>, dim3(phc::grid_for(ips, 256)), dim3(256), 0, st, U,
+                     ups, dim, szt, nbt, inv, ips);
+  PHC_HIP(hipGetLastError());
   return 0;
 }
 
@@ -470,8 +579,8 @@ extern "C" {
 int phc_baqr_create(phc_baqr **out, int K, int nrows, int ncols, int szt) {
   PHC_REQUIRE(out, "phc_baqr_create: null plan pointer");
   PHC_REQUIRE(K == 2 || K == 4, "phc_baqr_create: K must be 2 or 4");
-  PHC_REQUIRE(szt >= 1 && szt <= 256 && ncols >= szt && ncols % szt == 0,
-              "phc_baqr_create: ncols must be a multiple of szt (1..256)");
+  PHC_REQUIRE(szt >= 1 && szt <= 128 && ncols >= szt && ncols % szt == 0,
+              "phc_baqr_create: ncols must be a multiple of szt (1..128)");
   PHC_REQUIRE(nrows >= ncols, "phc_baqr_create: nrows must be at least ncols");
   phc_baqr *p = new phc_baqr();
   p->K = K; p->nrows = nrows; p->ncols = ncols; p->szt = szt; p->nbt = ncols / szt;
@@ -481,12 +590,21 @@ int phc_baqr_create(phc_baqr **out, int K, int nrows, int ncols, int szt) {
   p->t1ps = (size_t)szt * ncols;
   p->t2ps = (size_t)nrows * szt;
   p->wps = (size_t)p->nchunk * szt;
+  // split-k partial sums: up to 8 slices of the largest thin product, and
+  // the Q^H b products (nrows x 1 in up to 32 slices)
+  p->pcap = (size_t)nrows * szt * 8;
+  if ((size_t)nrows * 32 > p->pcap) p->pcap = (size_t)nrows * 32;
   const size_t z = sizeof(double) * 2 * K;
-  double **bufs[] = {&p->Y, &p->W, &p->G, &p->S, &p->T1, &p->T2, &p->wp, &p->B};
+  double **bufs[] = {&p->Y, &p->W, &p->G, &p->S, &p->T1, &p->T2, &p->wp, &p->w, &p->B, &p->part};
   const size_t sizes[] = {z * p->yps, z * p->yps, z * p->gps, z * p->gps, z * p->t1ps,
-                          z * p->t2ps, z * p->wps, sizeof(double) * K * szt};
-  for (int i = 0; i < 8; ++i) *bufs[i] = nullptr;
-  for (int i = 0; i < 8; ++i) {
+                          z * p->t2ps, z * p->wps, z * szt, sizeof(double) * K * szt, z * p->pcap};
+  const int nb = 10;
+  for (int i = 0; i < nb; ++i) *bufs[i] = nullptr;
+  p->exec = nullptr;
+  p->cap = nullptr;
+  p->gA = nullptr;
+  p->gQ = p->gR = nullptr;
+  for (int i = 0; i < nb; ++i) {
     if (hipMalloc(bufs[i], sizes[i]) != hipSuccess) {
       for (int j = 0; j < i; ++j) hipFree(*bufs[j]);
       delete p;
@@ -494,25 +612,66 @@ int phc_baqr_create(phc_baqr **out, int K, int nrows, int ncols, int szt) {
       return -1;
     }
   }
+  if (hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking) != hipSuccess) {
+    for (int j = 0; j < nb; ++j) hipFree(*bufs[j]);
+    delete p;
+    phc::set_error("phc_baqr_create: hipStreamCreate failed");
+    return -1;
+  }
   *out = p;
   return 0;
 }
 
 int phc_baqr_destroy(phc_baqr *p) {
   if (!p) return 0;
-  double *bufs[] = {p->Y, p->W, p->G, p->S, p->T1, p->T2, p->wp, p->B};
+  if (p->exec) hipGraphExecDestroy(p->exec);
+  if (p->cap) hipStreamDestroy(p->cap);
+  double *bufs[] = {p->Y, p->W, p->G, p->S, p->T1, p->T2, p->wp, p->w, p->B, p->part};
   for (double *b : bufs) hipFree(b);
   delete p;
   return 0;
 }
 
+// The ~4 launches per column are captured once into a HIP graph (per
+// (A, Q, R) pointers) and replayed: the panel kernels are short, so launch
+// gaps would otherwise dominate.
 int phc_cmplx_blocked_houseqr(phc_baqr *p, const double *A, double *Q, double *R, void *stream) {
   PHC_REQUIRE(p && A && Q && R, "phc_cmplx_blocked_houseqr: null argument");
   hipStream_t st = (hipStream_t)stream;
-  const int rc = (p->K == 2) ? run_qr<2>(p, A, Q, R, st) : run_qr<4>(p, A, Q, R, st);
-  if (rc) return rc;
-  PHC_HIP(hipGetLastError());
+  if (!p->exec || p->gA != A || p->gQ != Q || p->gR != R) {
+    if (p->exec) {
+      PHC_HIP(hipGraphExecDestroy(p->exec));
+      p->exec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    PHC_HIP(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
+    const int rc = (p->K == 2) ? run_qr<2>(p, A, Q, R, p->cap) : run_qr<4>(p, A, Q, R, p->cap);
+    const hipError_t e = hipStreamEndCapture(p->cap, &graph);
+    if (rc) {
+      if (graph) hipGraphDestroy(graph);
+      return rc;
+    }
+    PHC_HIP(e);
+    const hipError_t ei = hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    PHC_HIP(ei);
+    p->gA = A;
+    p->gQ = Q;
+    p->gR = R;
+  }
+  PHC_HIP(hipGraphLaunch(p->exec, st));
   return 0;
+}
+
+int phc_cmplx_qhb(phc_baqr *p, const double *Q, const double *b, double *y, void *stream) {
+  PHC_REQUIRE(p && Q && b && y, "phc_cmplx_qhb: null argument");
+  const int n = p->nrows;
+  hipStream_t st = (hipStream_t)stream;
+  const Op qa{Q, (size_t)n * n, 1, n}, bb{b, (size_t)n, 1, 0};
+  return p->K == 2 ? cgemm<2>(true, false, n, 1, n, qa, bb, y, (size_t)n, 1, 0, false, st, p->part,
+                              p->pcap)
+                   : cgemm<4>(true, false, n, 1, n, qa, bb, y, (size_t)n, 1, 0, false, st, p->part,
+                              p->pcap);
 }
 
 int phc_cmplx_gemm(int K, int conjA, int conjB, int M, int N, int KD, const double *A,
@@ -530,29 +689,18 @@ int phc_cmplx_gemm(int K, int conjA, int conjB, int M, int N, int KD, const doub
 int phc_cmplx_upper_tiled_solver(int K, int dim, int szt, int nbt, double *U, double *b,
                                  double *x, void *stream) {
   PHC_REQUIRE(K == 2 || K == 4, "phc_cmplx_upper_tiled_solver: K must be 2 or 4");
-  PHC_REQUIRE(szt >= 1 && nbt >= 1 && dim == szt * nbt,
-              "phc_cmplx_upper_tiled_solver: dim must equal szt*nbt");
+  PHC_REQUIRE(szt >= 1 && szt <= 256 && nbt >= 1 && dim == szt * nbt,
+              "phc_cmplx_upper_tiled_solver: dim must equal szt*nbt, szt <= 256");
   PHC_REQUIRE(U && b && x, "phc_cmplx_upper_tiled_solver: null array");
   hipStream_t st = (hipStream_t)stream;
   double *inv = nullptr;
   const size_t ips = (size_t)nbt * szt * szt;
   PHC_HIP(hipMalloc(&inv, sizeof(double) * 2 * K * ips));
-  const size_t ups = (size_t)dim * dim;
-#define SOLVE(KK)                                                                              \
-  do {                                                                                         \
-    hipLaunchKernelGGL(tile_inverse_kernel<KK>, dim3(nbt), dim3(256), 0, st, U, ups, dim, szt, \
-                       inv, ips);                                                              \
-    hipLaunchKernelGGL(tiled_solve_kernel<KK>, dim3(1), dim3(256), 0, st, U, ups, dim, szt,    \
-                       nbt, inv, ips, b, (size_t)dim, x, (size_t)dim);                         \
-    hipLaunchKernelGGL(copy_inverse_kernel<KK>, dim3(phc::grid_for(ips, 256)), dim3(256), 0, st, U, ups, \
-                       dim, szt, nbt, inv, ips);                                               \
-  } while (0)
-  if (K == 2) SOLVE(2); else SOLVE(4);
-#undef SOLVE
-  hipError_t e = hipGetLastError();
-  hipError_t e2 = hipStreamSynchronize(st);
+  const int rc = K == 2 ? run_tiled<2>(dim, szt, nbt, U, b, x, inv, st)
+                        : run_tiled<4>(dim, szt, nbt, U, b, x, inv, st);
+  const hipError_t e2 = hipStreamSynchronize(st);
   hipFree(inv);
-  PHC_HIP(e);
+  if (rc) return rc;
   PHC_HIP(e2);
   return 0;
 }

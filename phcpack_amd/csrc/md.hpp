@@ -167,26 +167,36 @@ MD_HD dd sqrt(const dd &a) {
 
 // ------------------------------------------------------------ K = 4, 8 -----
 
+// r[p] = v for a run-time p < K without indexing the array at run time (a
+// dynamically indexed local array would live in scratch memory)
+template <int K> MD_HD void set_limb(double *r, int p, double v) {
+#pragma unroll
+  for (int j = 0; j < K; ++j) r[j] = (p == j) ? v : r[j];
+}
+
 // renorm with a running pointer over K+1 terms into K limbs
-// (qdf_renorm4, odf_renorm8).
+// (qdf_renorm4, odf_renorm8).  The limb the pointer rereads is always the
+// one written last, so it is kept in `last` instead of being read back.
 template <int K> MD_HD void renorm(const double *f, double *r) {
-  double pr;
+  double pr, last;
   int ptr;
+#pragma unroll
   for (int i = 0; i < K; ++i) r[i] = 0.0;
   if (f[1] == 0.0) {
     pr = f[0]; ptr = 0;
-    r[0] = quick_two_sum(pr, f[2], pr);
+    r[0] = last = quick_two_sum(pr, f[2], pr);
   } else {
     r[0] = f[0]; pr = f[1]; ptr = 1;
-    r[1] = quick_two_sum(pr, f[2], pr);
+    r[1] = last = quick_two_sum(pr, f[2], pr);
   }
 #pragma unroll
   for (int i = 3; i <= K; ++i) {
-    if (pr == 0.0) pr = r[ptr]; else ptr = ptr + 1;
-    r[ptr] = quick_two_sum(pr, f[i], pr);
+    if (pr == 0.0) pr = last; else ptr = ptr + 1;
+    last = quick_two_sum(pr, f[i], pr);
+    set_limb<K>(r, ptr, last);
   }
-  if (pr == 0.0) pr = r[ptr]; else ptr = ptr + 1;
-  if (ptr < K && pr != 0.0) { r[ptr] = pr; ptr = ptr + 1; }
+  if (pr == 0.0) pr = last; else ptr = ptr + 1;
+  if (ptr < K && pr != 0.0) { set_limb<K>(r, ptr, pr); ptr = ptr + 1; }
 #pragma unroll
   for (int i = 0; i < K; ++i) if (i >= ptr) r[i] = 0.0;
 }

@@ -162,6 +162,31 @@ template <int K> struct Path {
   float *hid;
 };
 
+// Complex operations of the tracker: FP64 tracking (K = 1) forms products
+// with fused multiply-adds; K > 1 keeps the multiple-double order of C<K>.
+template <int K> struct TC : C<K> {};
+template <> struct TC<1> : C<1> {
+  typedef cplx<1> Z;
+  __device__ static Z mul(const Z &x, const Z &y) {
+    Z z;
+    z.re = __builtin_fma(x.re, y.re, -(x.im * y.im));
+    z.im = __builtin_fma(x.re, y.im, x.im * y.re);
+    return z;
+  }
+  __device__ static void mac(Z &z, const Z &x, const Z &y) {
+    z.re = __builtin_fma(x.re, y.re, z.re);
+    z.re = __builtin_fma(-x.im, y.im, z.re);
+    z.im = __builtin_fma(x.im, y.re, z.im);
+    z.im = __builtin_fma(x.re, y.im, z.im);
+  }
+  __device__ static void msub(Z &z, const Z &x, const Z &y) {
+    z.re = __builtin_fma(-x.re, y.re, z.re);
+    z.re = __builtin_fma(x.im, y.im, z.re);
+    z.im = __builtin_fma(-x.im, y.re, z.im);
+    z.im = __builtin_fma(-x.re, y.im, z.im);
+  }
+};
+
 __device__ __forceinline__ void wsync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -213,7 +238,7 @@ template <int K> __device__ __forceinline__ cplx<K> load_weight(const LProg &P, 
 
 // Taylor coefficients W[r][l] at t of w_r(t) = om0 + om1 t + om2 t^2
 template <int K> __device__ void set_weights(const LProg &P, Path<K> &w, double t) {
-  typedef C<K> Cx;
+  typedef TC<K> Cx;
   const int l = lane_id();
   if (l < P.ncomp) {
     cplx<K> o0 = load_weight<K>(P, 3 * l), o1 = load_weight<K>(P, 3 * l + 1),
@@ -228,7 +253,7 @@ template <int K> __device__ void set_weights(const LProg &P, Path<K> &w, double 
 
 // node_k = sum_q left_q * right_{k-q} for every product node, level by level
 template <int K> __device__ void product_pass(const LProg &P, Path<K> &w, int k) {
-  typedef C<K> Cx;
+  typedef TC<K> Cx;
   for (int lv = 1; lv <= P.nlev; ++lv) {
     for (int i = P.lev[lv - 1] + lane_id(); i < P.lev[lv]; i += 64) {
       const int *nd = P.nodes + 8 * i;
@@ -249,9 +274,13 @@ template <int K> __device__ void product_pass(const LProg &P, Path<K> &w, int k)
   }
 }
 
-// node_k += left_0*dright_k + dleft_k*right_0 once x_k is known
-template <int K> __device__ void correction_pass(const LProg &P, Path<K> &w, int k) {
-  typedef C<K> Cx;
+// correction of order k and products of order k+1 in one sweep over the
+// levels: a node's corrected order-k value and its order-(k+1) product read
+// only factors of lower levels (already final for both orders) and write
+// distinct slots (ring storage keeps at least two orders when deg >= 1)
+template <int K> __device__ void correct_and_product(const LProg &P, Path<K> &w, int k) {
+  typedef TC<K> Cx;
+  const int k1 = k + 1;
   for (int lv = 1; lv <= P.nlev; ++lv) {
     for (int i = P.lev[lv - 1] + lane_id(); i < P.lev[lv]; i += 64) {
       const int *nd = P.nodes + 8 * i;
@@ -262,6 +291,13 @@ template <int K> __device__ void correction_pass(const LProg &P, Path<K> &w, int
       if (sdn >= 0) w.dnode[sdn] = d;
       cplx<K> &v = w.slots[soff + (k & smask)];
       v = Cx::add(v, d);
+      cplx<K> a0 = Cx::mul(w.slots[aoff], w.slots[boff + k1]);
+      cplx<K> a1 = Cx::mul(w.slots[aoff + 1], w.slots[boff + k1 - 1]);
+      for (int q = 2; q <= k1; q += 2) {
+        Cx::mac(a0, w.slots[aoff + q], w.slots[boff + k1 - q]);
+        if (q + 1 <= k1) Cx::mac(a1, w.slots[aoff + q + 1], w.slots[boff + k1 - q - 1]);
+      }
+      w.slots[soff + (k1 & smask)] = Cx::add(a0, a1);
     }
     wsync();
   }
@@ -269,7 +305,7 @@ template <int K> __device__ void correction_pass(const LProg &P, Path<K> &w, int
 
 // rhs_i = -[H_i]_k from the weighted component values at order k
 template <int K> __device__ void residual(const LProg &P, Path<K> &w, int k) {
-  typedef C<K> Cx;
+  typedef TC<K> Cx;
   const int l = lane_id();
   for (int q = l; q < P.npair; q += 64) {
     const int *pr = P.pairs + 4 * q;
@@ -304,7 +340,7 @@ template <int K> __device__ void residual(const LProg &P, Path<K> &w, int k) {
 
 // Jacobian A_ij = sum coef * e * W_r0 * node(m/x_j)_0
 template <int K> __device__ void jacobian(const LProg &P, Path<K> &w) {
-  typedef C<K> Cx;
+  typedef TC<K> Cx;
   const int l = lane_id();
   for (int q = l; q < P.njpair; q += 64) {
     const int *jp = P.jpairs + 2 * q;
@@ -337,7 +373,7 @@ template <int K> __device__ void jacobian(const LProg &P, Path<K> &w) {
 // where the row permutation P is recorded in perm ((P r)_i = r[perm[i]]).
 // Returns false on a zero or non-finite pivot.
 template <int K> __device__ bool gj_invert(const LProg &P, Path<K> &w) {
-  typedef C<K> Cx;
+  typedef TC<K> Cx;
   const int n = P.n, l = lane_id();
   if (l < n) w.perm[l] = l;
   wsync();
@@ -384,7 +420,7 @@ template <int K> __device__ bool gj_invert(const LProg &P, Path<K> &w) {
 // the G partial sums meet in a butterfly; row i's result is in the lanes of
 // group i.
 template <int K> __device__ cplx<K> apply_inverse(const LProg &P, Path<K> &w) {
-  typedef C<K> Cx;
+  typedef TC<K> Cx;
   const int n = P.n, G = P.G, l = lane_id();
   cplx<K> acc = Cx::zero();
   if (l < n * G) {
@@ -416,7 +452,7 @@ template <int K> __device__ double max_abs_x(const LProg &P, Path<K> &w) {
 template <int K>
 __device__ int newton(const LProg &P, Path<K> &w, int iters, double tol, double *dxl, int *count,
                       double *dxh) {
-  typedef C<K> Cx;
+  typedef TC<K> Cx;
   const int l = lane_id();
   for (int it = 0; it < iters; ++it) {
     ++*count;
@@ -525,19 +561,21 @@ __device__ void store_point(const LProg &P, Path<K> &w, double *x, int npaths, i
 // Taylor coefficients x_1..x_deg at the point in the order-0 slots (weights
 // set at t0); returns false on a singular Jacobian
 template <int K> __device__ bool predictor_series(const LProg &P, Path<K> &w) {
-  typedef C<K> Cx;
   const int l = lane_id(), n = P.n, D = P.D;
   eval_point<K>(P, w, true);
   if (!gj_invert<K>(P, w)) return false;
+  if (l < n) w.slots[l * D + 1] = TC<K>::zero();
+  wsync();
+  product_pass<K>(P, w, 1);
   for (int k = 1; k < D; ++k) {
-    if (l < n) w.slots[l * D + k] = Cx::zero();
-    wsync();
-    product_pass<K>(P, w, k);
     residual<K>(P, w, k);
     const cplx<K> xk = apply_inverse<K>(P, w);
     if (l < n * P.G && (l % P.G) == 0) w.slots[(l / P.G) * D + k] = xk;
+    if (k + 1 < D && l < n) w.slots[l * D + k + 1] = TC<K>::zero();
     wsync();
-    correction_pass<K>(P, w, k);
+    // the correction of the last order would only touch values no later
+    // phase reads
+    if (k + 1 < D) correct_and_product<K>(P, w, k);
   }
   return true;
 }
@@ -590,7 +628,7 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, Params par
                                                    int *status, int *nsteps_out, int *nrej_out,
                                                    double *dx_out, int *work, Ctl ctl,
                                                    int *counter) {
-  typedef C<K> Cx;
+  typedef TC<K> Cx;
   extern __shared__ __align__(16) unsigned char lds[];
   const ProgLayout PL(G);
   const PathLayout<K> WL(G);
@@ -625,13 +663,21 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, Params par
       h = (r < INFINITY && r > 0.0) ? fmin(par.pbeta * r, par.maxstep) : par.maxstep;
       // ---- step-size controller ----
       int cand = 0;
-      double feat[PHC_CTL_NFEAT];
       if (ctl.mode != 0) {
+        double feat[PHC_CTL_NFEAT];
         step_features<K>(P, w, r, par.maxstep, hprev, t, dxh, rejprev, feat);
         if (ctl.mode == 1) cand = mlp_choose<K>(ctl, w, feat);
         else cand = (int)(mix_hash(ctl.seed, (unsigned)path, (unsigned)(npred - 1)) % (unsigned)ctl.ncand);
         h *= ctl.mult[cand];
         if (ctl.pcap > 0.0 && r < INFINITY && r > 0.0) h = fmin(h, ctl.pcap * r);
+        // the features and the choice are logged now (the outcome after the
+        // corrector), so that they are not kept live across it
+        if (ctl.mode == 2 && nlog < ctl.cap && l == 0) {
+#pragma unroll
+          for (int i = 0; i < PHC_CTL_NFEAT; ++i)
+            ctl.log_x[((size_t)path * ctl.cap + nlog) * PHC_CTL_NFEAT + i] = (float)feat[i];
+          ctl.log_c[(size_t)path * ctl.cap + nlog] = cand;
+        }
       }
       const int rej0 = rej;
       if (l < n) w.x0[l] = w.slots[l * D];
@@ -667,13 +713,7 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, Params par
       }
       rejprev = rej - rej0;
       if (ctl.mode == 2 && nlog < ctl.cap) {
-        if (l == 0) {
-#pragma unroll
-          for (int i = 0; i < PHC_CTL_NFEAT; ++i)
-            ctl.log_x[((size_t)path * ctl.cap + nlog) * PHC_CTL_NFEAT + i] = (float)feat[i];
-          ctl.log_c[(size_t)path * ctl.cap + nlog] = cand;
-          ctl.log_y[(size_t)path * ctl.cap + nlog] = (rej == rej0 && st == ST_RUNNING) ? 1 : 0;
-        }
+        if (l == 0) ctl.log_y[(size_t)path * ctl.cap + nlog] = (rej == rej0 && st == ST_RUNNING) ? 1 : 0;
         ++nlog;
       }
       if (st != ST_RUNNING) break;
@@ -712,7 +752,7 @@ __global__ void __launch_bounds__(64 * REFINE_MAX_WAVES) refine_kernel(DevProgra
                                                     const double *__restrict__ xin, int KS,
                                                     double *__restrict__ xout, int iters,
                                                     double tol, double *dx_out, double *res_out,
-                                                    int *status, int *counter) {
+                                                    int *status, const int *only, int *counter) {
   extern __shared__ __align__(16) unsigned char lds[];
   const ProgLayout PL(G);
   const PathLayout<K> WL(G);
@@ -727,6 +767,17 @@ __global__ void __launch_bounds__(64 * REFINE_MAX_WAVES) refine_kernel(DevProgra
     path = __shfl(path, 0, 64);
     if (path >= npaths) break;
     load_point<K>(P, w, xin, KS, npaths, path);
+    if (only && only[path] != ST_SUCCESS) {
+      // not a tracked endpoint: copied through, status = the tracker's
+      store_point<K>(P, w, xout, npaths, path);
+      if (l == 0) {
+        dx_out[path] = INFINITY;
+        res_out[path] = INFINITY;
+        status[path] = only[path];
+      }
+      wsync();
+      continue;
+    }
     set_weights<K>(P, w, tval);
     double dxl = 0.0;
     int nr = 1, its = 0;
@@ -818,7 +869,10 @@ int phc_program_validate(const phc_program *d) {
   auto storage_ok = [&](int off, int mask) {
     if (off < 0) return false;
     if (mask == -1) return off + D <= d->nslots;
-    return mask >= need && ((mask + 1) & mask) == 0 && off + mask + 1 <= d->nslots;
+    // a ring keeps at least two orders when deg >= 1 (the predictor corrects
+    // order k and forms order k+1 of a node in one sweep)
+    return mask >= need && (d->deg == 0 || mask >= 1) && ((mask + 1) & mask) == 0 &&
+           off + mask + 1 <= d->nslots;
   };
   PHC_REQUIRE((d->n + 1) * D <= d->nslots, "phc_program: no room for the variables and the constant");
   PHC_REQUIRE(d->lev_ptr[0] == 0 && d->lev_ptr[d->nlev] == d->nprod, "phc_program: bad levels");
@@ -994,7 +1048,7 @@ int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, d
 
 int phc_refine(phc_tracker *tr, int K, int npaths, double t, const double *xin, int KS,
                double *xout, int iters, double tol, double *dxlast, double *res, int *status,
-               int waves, void *stream) {
+               const int *only, int waves, void *stream) {
   PHC_REQUIRE(tr, "phc_refine: null tracker");
   PHC_REQUIRE(K == 1 || K == 2 || K == 4, "phc_refine: K must be 1, 2 or 4 (octo-double device "
               "refinement is switched off: its kernel stalled a device run and the cause is not "
@@ -1011,7 +1065,7 @@ int phc_refine(phc_tracker *tr, int K, int npaths, double t, const double *xin, 
   const void *fn = K == 1 ? (const void *)refine_kernel<1> : K == 2 ? (const void *)refine_kernel<2>
                                                            : (const void *)refine_kernel<4>;
   PHC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-#define RK(KK) hipLaunchKernelGGL(refine_kernel<KK>, dim3(grid), dim3(64 * wv), lds, st, tr->P, npaths, t, xin, KS, xout, iters, tol, dxlast, res, status, tr->counter)
+#define RK(KK) hipLaunchKernelGGL(refine_kernel<KK>, dim3(grid), dim3(64 * wv), lds, st, tr->P, npaths, t, xin, KS, xout, iters, tol, dxlast, res, status, only, tr->counter)
   if (K == 1) RK(1); else if (K == 2) RK(2); else RK(4);
 #undef RK
   PHC_HIP(hipGetLastError());

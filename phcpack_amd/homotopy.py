@@ -52,8 +52,11 @@ def unit_circle(rng, size=None):
 def dense_system(n, degree, rng, rank_deficient=0):
     """n dense polynomials of the given degree with complex coefficients on
     the unit circle: every monomial of degree <= degree appears.
-    rank_deficient > 0 replaces that many equations by random combinations
-    of two others (a singular target family)."""
+    rank_deficient > 0 makes the top-degree forms of that many equations
+    random combinations of the top-degree forms of two others (their lower
+    degree terms stay random): the leading forms of the system are then
+    rank-deficient, the system has common zeros at infinity and part of the
+    total-degree paths diverge or end at singular points."""
     exps = [e for tot in range(degree + 1) for e in _exponents(n, tot)]
     eqs = []
     for _ in range(n):
@@ -63,12 +66,15 @@ def dense_system(n, degree, rng, rank_deficient=0):
         i = n - 1 - k
         a, b = rng.choice(n - rank_deficient, 2, replace=False)
         la, lb = unit_circle(rng), unit_circle(rng)
-        terms = {}
+        top = {}
         for c, e in eqs[a]:
-            terms[e] = terms.get(e, 0) + la * c
+            if sum(e) == degree:
+                top[e] = top.get(e, 0) + la * c
         for c, e in eqs[b]:
-            terms[e] = terms.get(e, 0) + lb * c
-        eqs[i] = [(complex(c), e) for e, c in terms.items()]
+            if sum(e) == degree:
+                top[e] = top.get(e, 0) + lb * c
+        eqs[i] = [(c, e) for c, e in eqs[i] if sum(e) < degree] + \
+                 [(complex(c), e) for e, c in top.items()]
     return System(n, eqs)
 
 

@@ -138,17 +138,21 @@ class Tracker:
         self.prog0 = compile_homotopy(hom, 0)
         self.dev = DeviceProgram(self.prog)
         self.dev0 = DeviceProgram(self.prog0)
+        self.waves = {}  # precision -> waves per workgroup chosen by phcpack_amd.autotune
 
-    def track_device(self, d_start, KS, npaths, K, bufs, params=None, waves=12, stream=None,
+    def track_device(self, d_start, KS, npaths, K, bufs, params=None, waves=None, stream=None,
                      controller=None):
-        """controller: a PhcController (or None for the plain pole-step rule)"""
+        """controller: a PhcController (or None for the plain pole-step rule);
+        waves: wavefronts per workgroup (default: the tuned choice, else 12)"""
+        if waves is None:
+            waves = self.waves.get(K, 12)
         p = (params or TrackParams()).to_c()
         cptr = ctypes.byref(controller) if controller is not None else None
         check(load().phc_track(self.dev.handle, K, npaths, d_start.ptr, KS, bufs.endp.ptr,
                                bufs.tend.ptr, bufs.status.ptr, bufs.nsteps.ptr, bufs.nrej.ptr,
                                bufs.dx.ptr, bufs.work.ptr, ctypes.byref(p), cptr, waves, stream))
 
-    def track(self, starts, K=2, params=None, waves=12, controller=None):
+    def track(self, starts, K=2, params=None, waves=None, controller=None):
         """starts: complex128 [npaths][n] (or [2KS][npaths][n] planes)."""
         planes = starts if starts.dtype == np.float64 else points_to_planes(starts)
         KS = planes.shape[0] // 2
@@ -160,9 +164,12 @@ class Tracker:
         return bufs.to_host()
 
     def refine_device(self, d_x, KS, npaths, K, d_out, d_dx, d_res, d_st, iters, tol, t=1.0,
-                      waves=8, stream=None):
+                      waves=8, stream=None, only=None):
+        """only: device int array of tracker statuses; paths not at status 1
+        are copied through unrefined"""
         check(load().phc_refine(self.dev0.handle, K, npaths, t, d_x.ptr, KS, d_out.ptr, iters,
-                                tol, d_dx.ptr, d_res.ptr, d_st.ptr, waves, stream))
+                                tol, d_dx.ptr, d_res.ptr, d_st.ptr,
+                                only.ptr if only is not None else None, waves, stream))
 
     def refine(self, planes, K, iters, tol, t=1.0, waves=8):
         """Newton at t in precision K from [2KS][npaths][n] planes."""

@@ -33,6 +33,13 @@ def test_blocked_qr_matches_oracle(K, shape):
     assert dq < tol and dr < tol * np.sqrt(m)
 
 
+@pytest.mark.parametrize("K,shape", [(2, (256, 128, 128)), (4, (128, 128, 64))])
+def test_blocked_qr_split_sums_match_oracle(K, shape):
+    """sizes where the Gram, W and Q-update products split their k range over
+    workgroups (k >= 128) and the panel spans up to 128 columns"""
+    test_blocked_qr_matches_oracle(K, shape)
+
+
 @pytest.mark.parametrize("K", (2, 4))
 def test_upper_tiled_solver_matches_oracle(K):
     from phcpack_amd.baqr import upper_tiled_solver
@@ -54,14 +61,14 @@ def test_upper_tiled_solver_matches_oracle(K):
     assert np.abs(md_diff(K, b1, bo)).max() < tol
 
 
-@pytest.mark.parametrize("K", (2, 4))
-def test_least_squares_recovers_planted_solution(K):
-    """b = A x0 in the oracle; QR, Q^H b and the tiled solve on the GPU give x0"""
+@pytest.mark.parametrize("K,n,szt", [(2, 96, 32), (4, 96, 32), (2, 256, 64)])
+def test_least_squares_recovers_planted_solution(K, n, szt):
+    """b = A x0 in the oracle; QR, Q^H b (split k range at n = 256) and the
+    tiled solve on the GPU give x0; Q^H b agrees with the unsplit product"""
     from phcpack_amd._lib import DeviceArray, check, load
     from phcpack_amd.baqr import BlockedQR, qh_times, upper_tiled_solver_device
-    print(f"least squares K={K}", flush=True)
-    rng = np.random.default_rng(21 + K)
-    n, szt = 96, 32
+    print(f"least squares K={K} n={n} szt={szt}", flush=True)
+    rng = np.random.default_rng(21 + K + n)
     A = rand_cmd(rng, K, (n, n))
     x0 = rand_cmd(rng, K, (n,))
     b = oracle_matvec(K, A, x0)
@@ -70,7 +77,11 @@ def test_least_squares_recovers_planted_solution(K):
     dQ, dR = DeviceArray((2 * K, n, n)), DeviceArray((2 * K, n, n))
     qr.factor_device(dA, dQ, dR)
     db, dy, dx = DeviceArray.from_host(b), DeviceArray.zeros((2 * K, n)), DeviceArray.zeros((2 * K, n))
-    qh_times(K, dQ, db, n, dy)
+    qr.qhb_device(dQ, db, dy)
+    dy1 = DeviceArray.zeros((2 * K, n))
+    qh_times(K, dQ, db, n, dy1)
+    check(load().phc_synchronize())
+    assert np.abs(md_diff(K, dy.to_host(), dy1.to_host())).max() < 64 * n * ulps(K)
     upper_tiled_solver_device(K, n, szt, n // szt, dR, dy, dx)
     check(load().phc_synchronize())
     err = np.abs(md_diff(K, dx.to_host(), x0)).max()

@@ -36,6 +36,25 @@ def test_track_matches_oracle(K, n, degree, seed):
     assert np.all(g["status"] == 1)
 
 
+@pytest.mark.parametrize("K", [1, 2])
+@pytest.mark.parametrize("n,degree,seed,npaths", [(13, 2, 4, 12), (16, 1, 5, 1), (17, 1, 6, 1)])
+def test_track_matches_oracle_wider(K, n, degree, seed, npaths):
+    """Dimensions around the 16-column limit of the one-phase-per-pivot
+    Gauss-Jordan inverse (rows split unevenly over the lane groups at 13,
+    four rows per group at 16, the phased inverse at 17)."""
+    from phcpack_amd.tracker import Tracker, TrackParams, points_to_planes
+    p, hom, sols = make(n, degree, seed)
+    sols = sols[:npaths]
+    prm = TrackParams(tolend=1e-13 if K == 1 else 1e-28)
+    tr = Tracker(hom, deg=8)
+    print(f"track K={K} n={n} degree={degree} seed={seed} paths={len(sols)}", flush=True)
+    g = tr.track(sols, K=K, params=prm)
+    o = oracle_track(tr.prog, points_to_planes(sols), K, prm)
+    for key in ("status", "nsteps", "nrej", "tend", "endp", "work"):
+        assert np.array_equal(g[key], o[key]), key
+    assert np.all(g["status"] == 1)
+
+
 @pytest.mark.parametrize("waves", [1, 3, 8])
 def test_track_wave_counts_agree(waves):
     """Wavefronts of one workgroup track their paths independently: the
@@ -98,3 +117,83 @@ def test_decoupled_closed_form():
     # all sign patterns reached
     got = {tuple(np.sign(np.round((xi / np.sqrt(a)).real)).astype(int)) for xi in x}
     assert len(got) == 2 ** n
+
+
+@pytest.mark.parametrize("K", [1, 2])
+def test_rank_deficient_n14_matches_oracle(K):
+    """The dimension-14 family whose last 2 top-degree forms are combinations
+    of two others (rank-deficient leading forms): a sample of 96 of its 16384
+    paths, the ones that diverge or stall near t = 1 included, tracked
+    bit-identically to the oracle; the program at n = 14 still leaves LDS for
+    8 FP64 or 4 dd wavefronts per workgroup."""
+    from phcpack_amd.tracker import Tracker, TrackParams, points_to_planes
+    p, hom, sols = make(14, 2, 20261202, rank_deficient=2)
+    tr = Tracker(hom, deg=8)
+    prog_b, path_b = tr.dev.lds_bytes(K)
+    assert prog_b + (8 if K == 1 else 4) * path_b <= 160 * 1024
+    rng = np.random.default_rng(4)
+    sample = np.sort(rng.choice(len(sols), 96, replace=False))
+    starts = sols[sample]
+    prm = TrackParams(tolend=1e-13 if K == 1 else 1e-28)
+    print(f"rank-deficient n=14 K={K} sample={len(sample)}", flush=True)
+    g = tr.track(starts, K=K, params=prm)
+    o = oracle_track(tr.prog, points_to_planes(starts), K, prm, nthreads=8)
+    for k in ("status", "nsteps", "nrej", "tend", "endp", "work"):
+        assert np.array_equal(g[k], o[k]), k
+    print("  statuses", np.bincount(g["status"], minlength=6).tolist(), flush=True)
+    # about a quarter of the paths reach regular solutions, the rest do not finish
+    assert 0 < np.sum(g["status"] == 1) < len(sample)
+
+
+@pytest.mark.parametrize("K", [2, 4])
+def test_refine_only_finished_paths(K):
+    """refinement with the tracker's statuses as mask: finished paths get
+    the same result as without the mask, the others are copied through with
+    their tracker status and infinite residual"""
+    from phcpack_amd._lib import DeviceArray, check, load
+    from phcpack_amd.tracker import TrackBuffers, Tracker, TrackParams, points_to_planes
+    p, hom, sols = make(6, 2, 3, rank_deficient=1)
+    tr = Tracker(hom, deg=8)
+    npaths, n = len(sols), hom.n
+    d_start = DeviceArray.from_host(points_to_planes(sols))
+    b = TrackBuffers(npaths, n, 1)
+    tr.track_device(d_start, 1, npaths, 1, b, TrackParams(tolend=1e-13))
+    check(load().phc_synchronize())
+    tst = b.status.to_host()
+    assert 0 < np.sum(tst == 1) < npaths
+
+    def run(only):
+        out = DeviceArray((2 * K, npaths, n))
+        dx, res = DeviceArray((npaths,)), DeviceArray((npaths,))
+        st = DeviceArray((npaths,), np.int32)
+        tr.refine_device(b.endp, 1, npaths, K, out, dx, res, st, iters=3, tol=1e-50, only=only)
+        check(load().phc_synchronize())
+        return out.to_host(), dx.to_host(), res.to_host(), st.to_host()
+
+    x0, dx0, r0, s0 = run(None)
+    x1, dx1, r1, s1 = run(b.status)
+    fin = tst == 1
+    assert np.array_equal(x1[:, fin], x0[:, fin]) and np.array_equal(r1[fin], r0[fin])
+    assert np.array_equal(s1[fin], s0[fin]) and np.array_equal(dx1[fin], dx0[fin])
+    e = b.endp.to_host()
+    assert np.array_equal(x1[0, ~fin], e[0, ~fin]) and np.array_equal(x1[K, ~fin], e[1, ~fin])
+    assert np.all(x1[1:K, ~fin] == 0) and np.all(np.isinf(r1[~fin]))
+    assert np.array_equal(s1[~fin], tst[~fin])
+
+
+def test_autotune_picks_a_valid_launch_shape():
+    """the launch-shape tuner times every candidate and keeps the fastest;
+    results are the same whatever it picks"""
+    from phcpack_amd._lib import DeviceArray
+    from phcpack_amd.autotune import tune_waves
+    from phcpack_amd.tracker import TrackBuffers, Tracker, TrackParams, points_to_planes
+    p, hom, sols = make(5, 2, 11)
+    tr = Tracker(hom, deg=8)
+    prm = TrackParams(tolend=1e-13)
+    d = DeviceArray.from_host(points_to_planes(sols))
+    bufs = TrackBuffers(len(sols), 5, 1)
+    best, table = tune_waves(tr, d, 1, len(sols), 1, bufs, prm, candidates=(4, 8, 12))
+    assert best in (4, 8, 12) and set(table) == {4, 8, 12} and tr.waves[1] == best
+    a = tr.track(sols, K=1, params=prm)
+    b = tr.track(sols, K=1, params=prm, waves=4)
+    assert np.array_equal(a["endp"], b["endp"])
