@@ -205,10 +205,13 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("tracker", "qr", "newton", "rankdef"), default="tracker",
+    ap.add_argument("--workload", choices=("tracker", "qr", "newton", "rankdef", "learned"),
+                    default="tracker",
                     help="tracker: the headline line; qr: complex quad-double blocked QR "
                          "least-squares solve of 1024 x 1024; newton: batched dd row Newton; "
-                         "rankdef: 14 quadratics with 2 dependent equations, 16384 paths")
+                         "rankdef: 14 quadratics with rank-deficient leading forms, 16384 "
+                         "paths; learned: step controller trained on 10 dense cubic systems "
+                         "(n = 8, 65610 paths), evaluated on 5 held-out systems")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip controller training; the headline is then the plain-rule mixed tracker")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -226,6 +229,8 @@ def main():
         return run_newton(args)
     if args.workload == "rankdef":
         return run_rankdef(args)
+    if args.workload == "learned":
+        return run_learned(args)
     return run(args)
 
 
@@ -592,6 +597,123 @@ def run_rankdef(args, n=14, rank_deficient=2, seed=SEED):
         "lost_paths_vs_plain": int(count_lost(cp, ok_p, cm, ok_m)),
         "speedup_mixed_vs_plain": round(float(v_m / v_p), 3) if v_p > 0 else None,
         "waves_per_workgroup": {"plain_dd": tuned[2][0], "fp64": tuned[1][0]},
+        "roofline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def run_learned(args, n=8, degree=3, ntrain=10, neval=5, seed=SEED):
+    """Learned step control on dense cubics in 8 variables (all 165
+    monomials of degree <= 3, 3^8 = 6561 total-degree paths per system).
+    The controller is trained on explore-mode step logs of ntrain systems
+    (seeds seed+100..; 65610 paths), its threshold and pole cap tuned on the
+    last training system, then evaluated on neval held-out systems
+    (seeds seed+110..): FP64 tracking with the learned controller vs the
+    plain pole-step rule, both followed by the same dd endpoint Newton (qd
+    escalation).  value = paths/s at <= 1e-24 of the learned method over all
+    held-out systems; a path the learned controller loses is tracked again
+    with the plain rule (its time counts)."""
+    from phcpack_amd._lib import DeviceArray, check, load
+    from phcpack_amd.controller import train_controller, tune_threshold
+    from phcpack_amd.homotopy import dense_system, total_degree_homotopy
+    from phcpack_amd.tracker import TrackBuffers, Tracker, TrackParams, planes_to_points, points_to_planes
+    lib = load()
+    ev = Events(lib)
+    prm = TrackParams(tolend=1e-13)
+
+    def system(s):
+        rng = np.random.default_rng(s)
+        p = dense_system(n, degree, rng)
+        hom, sols = total_degree_homotopy(p, rng)
+        return Tracker(hom, deg=8), sols
+
+    t0 = time.perf_counter()
+    train = [system(seed + 100 + i) for i in range(ntrain)]
+    ctl, info = train_controller([t for t, _ in train], [s for _, s in train], prm, K=1, seed=7,
+                                 epochs=25)
+
+    def timer(fn):
+        ev.start()
+        fn()
+        return ev.stop_ms()
+
+    best, table = tune_threshold(ctl, train[-1][0], train[-1][1], prm, K=1, timer=timer)
+    info.update(train_paths=int(sum(len(s) for _, s in train)),
+                threshold=best[0] if best else None, pole_cap=best[1] if best else None,
+                tune_table={str(k): v for k, v in table.items()},
+                seconds=round(time.perf_counter() - t0, 2))
+    if best is None:
+        print(json.dumps({"metric": "learned step control", "value": None,
+                          "error": "no threshold keeps every path", "training": info}))
+        return 1
+    cstruct = ctl.struct(1)
+    rows = []
+    tot = dict(t_plain=0.0, t_learn=0.0, good_plain=0, good_learn=0, lost=0, paths=0)
+    for i in range(neval):
+        s = seed + 110 + i
+        tr, sols = system(s)
+        npaths = len(sols)
+        h_start = points_to_planes(sols)
+        d_start = DeviceArray.from_host(h_start)
+
+        def bufs():
+            return TrackBuffers(npaths, n, 1), dict(
+                x2=DeviceArray((4, npaths, n)), dx=DeviceArray((npaths,)),
+                res=DeviceArray((npaths,)), st=DeviceArray((npaths,), np.int32))
+
+        (bp, rp), (bl, rl) = bufs(), bufs()
+        for _ in range(args.warmup):
+            run_mixed(tr, d_start, h_start, npaths, bp, rp, ev, prm, lib)
+            run_mixed(tr, d_start, h_start, npaths, bl, rl, ev, prm, lib, cstruct)
+        check(lib.phc_synchronize())
+        tp = [run_mixed(tr, d_start, h_start, npaths, bp, rp, ev, prm, lib) for _ in range(args.steps)]
+        tl = [run_mixed(tr, d_start, h_start, npaths, bl, rl, ev, prm, lib, cstruct)
+              for _ in range(args.steps)]
+        op, ol = bp.to_host(), bl.to_host()
+        xp, res_p, ok_p = verify(tr, op, rp, 2)
+        xl, res_l, ok_l = verify(tr, ol, rl, 2)
+        cp, cl = planes_to_points(xp, 4), planes_to_points(xl, 4)
+        ms_p = float(np.mean([r[0] + r[1] for r in tp]))
+        ms_l = float(np.mean([r[0] + r[1] for r in tl]))
+        lost = int(count_lost(cp, ok_p, cl, ok_l))
+        rows.append({"seed": s, "paths": npaths,
+                     "plain": {"ms": round(ms_p, 3), "good_paths": int(ok_p.sum()),
+                               "steps_per_path": float(op["nsteps"].mean()),
+                               "rejections_per_path": float(op["nrej"].mean())},
+                     "learned": {"ms": round(ms_l, 3), "good_paths": int(ok_l.sum()),
+                                 "steps_per_path": float(ol["nsteps"].mean()),
+                                 "rejections_per_path": float(ol["nrej"].mean()),
+                                 "retracked_paths": int(tl[-1][3])},
+                     "lost_paths_vs_plain": lost})
+        tot["t_plain"] += ms_p
+        tot["t_learn"] += ms_l
+        tot["good_plain"] += int(ok_p.sum())
+        tot["good_learn"] += int(ok_l.sum())
+        tot["lost"] += lost
+        tot["paths"] += npaths
+    v_l = tot["good_learn"] / (tot["t_learn"] / 1e3)
+    v_p = tot["good_plain"] / (tot["t_plain"] / 1e3)
+    line = {
+        "metric": "paths/sec at <=1e-24 endpoint residual, learned vs plain step control "
+                  "(dense cubics n=8, held-out systems)",
+        "value": round(v_l, 1), "unit": "paths/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(tot["t_learn"] / neval, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"total-degree homotopies of {n} dense complex cubics (6561 paths "
+                               f"each); controller trained on {ntrain} systems, evaluated on "
+                               f"{neval} held-out systems; FP64 tracking + dd endpoint Newton",
+                   "train_seeds": [seed + 100 + i for i in range(ntrain)],
+                   "eval_seeds": [seed + 110 + i for i in range(neval)]},
+        "plain_rule": {"paths_per_s": round(v_p, 1), "good_paths": tot["good_plain"],
+                       "ms_total": round(tot["t_plain"], 3)},
+        "learned": {"paths_per_s": round(v_l, 1), "good_paths": tot["good_learn"],
+                    "ms_total": round(tot["t_learn"], 3)},
+        "speedup_learned_vs_plain_rule": round(v_l / v_p, 3),
+        "lost_paths_vs_plain": tot["lost"],
+        "systems": rows,
+        "training": info,
         "roofline": None,
     }
     print(json.dumps(line), flush=True)

@@ -1,0 +1,134 @@
+/* phcmd_compat.h - the reference's C++ host entry points of the
+ * multiple-double path, with the reference's signatures, implemented on top
+ * of the C-ABI of phcmd.h (libphcmd.so).  A caller of the reference's GPU
+ * functions relinks against libphcmd.so and keeps its code.
+ *
+ * As in the reference, each call takes host arrays (one array per limb and
+ * per real/imaginary part, matrices as row pointers), allocates its device
+ * buffers, copies in, runs on the GPU and copies the results back.
+ *
+ * Timers and operation counters keep their meaning where this build can
+ * give it: *walltimesec and *totlapms-style totals receive the measured
+ * time of the call; the per-phase timers of the blocked QR are left as they
+ * are (the factorization runs as one HIP graph, its phases are not timed
+ * apart); the counters receive the multiple-double operations this build
+ * executes (a complex multiply-add counts 4 mul + 4 add).  Errors of the
+ * HIP runtime are reported on stderr and leave the outputs unchanged (the
+ * reference has no error channel).
+ */
+#ifndef PHCMD_COMPAT_H
+#define PHCMD_COMPAT_H
+
+/* ------------------------------------------------ series products ------
+ * Convolutions/dbl2_convolutions_kernels.h:121,145 and the dbl, dbl4, dbl8
+ * twins: z = x*y truncated at degree deg; freq repeats the launch (the
+ * reference times freq launches), BS and padded/mode select the reference's
+ * kernel variant and are accepted for compatibility (one kernel here). */
+void GPU_dbl_product(double *x_h, double *y_h, double *z_h, int deg, int freq, int BS, int mode);
+void GPU_cmplx_product(double *xre, double *xim, double *yre, double *yim, double *zre,
+                       double *zim, int deg, int freq, int BS, int mode);
+void GPU_dbl2_product(double *xhi_h, double *xlo_h, double *yhi_h, double *ylo_h, double *zhi_h,
+                      double *zlo_h, int deg, int freq, int BS, int padded);
+void GPU_cmplx2_product(double *xrehi_h, double *xrelo_h, double *ximhi_h, double *ximlo_h,
+                        double *yrehi_h, double *yrelo_h, double *yimhi_h, double *yimlo_h,
+                        double *zrehi_h, double *zrelo_h, double *zimhi_h, double *zimlo_h,
+                        int deg, int freq, int BS, int mode);
+void GPU_dbl4_product(double *xhihi_h, double *xlohi_h, double *xhilo_h, double *xlolo_h,
+                      double *yhihi_h, double *ylohi_h, double *yhilo_h, double *ylolo_h,
+                      double *zhihi_h, double *zlohi_h, double *zhilo_h, double *zlolo_h,
+                      int deg, int freq, int BS, int padded);
+void GPU_cmplx4_product(double *xrehihi_h, double *xrelohi_h, double *xrehilo_h,
+                        double *xrelolo_h, double *ximhihi_h, double *ximlohi_h,
+                        double *ximhilo_h, double *ximlolo_h, double *yrehihi_h,
+                        double *yrelohi_h, double *yrehilo_h, double *yrelolo_h,
+                        double *yimhihi_h, double *yimlohi_h, double *yimhilo_h,
+                        double *yimlolo_h, double *zrehihi_h, double *zrelohi_h,
+                        double *zrehilo_h, double *zrelolo_h, double *zimhihi_h,
+                        double *zimlohi_h, double *zimhilo_h, double *zimlolo_h, int deg,
+                        int freq, int BS, int mode);
+
+/* ------------------------------------------------ blocked QR ------------
+ * Matrices/dbl2_baqr_kernels.h:2540 and dbl4_baqr_kernels.h:3739:
+ * A (nrows x ncols, nrows >= ncols = szt*nbt) = Q R with Q (nrows x nrows)
+ * and R (nrows x ncols). */
+void GPU_cmplx2_blocked_houseqr(
+    int nrows, int ncols, int szt, int nbt, double **Arehi, double **Arelo, double **Aimhi,
+    double **Aimlo, double **Qrehi, double **Qrelo, double **Qimhi, double **Qimlo,
+    double **Rrehi, double **Rrelo, double **Rimhi, double **Rimlo, double *houselapms,
+    double *RHvlapms, double *tileRlapms, double *vb2Wlapms, double *WYHlapms,
+    double *QWYHlapms, double *Qaddlapms, double *YWHlapms, double *YWHClapms,
+    double *Raddlapms, double *walltimesec, long long int *addcnt, long long int *mulcnt,
+    long long int *divcnt, long long int *sqrtcnt, bool verbose = true);
+void GPU_cmplx4_blocked_houseqr(
+    int nrows, int ncols, int szt, int nbt, double **Arehihi, double **Arelohi,
+    double **Arehilo, double **Arelolo, double **Aimhihi, double **Aimlohi, double **Aimhilo,
+    double **Aimlolo, double **Qrehihi, double **Qrelohi, double **Qrehilo, double **Qrelolo,
+    double **Qimhihi, double **Qimlohi, double **Qimhilo, double **Qimlolo, double **Rrehihi,
+    double **Rrelohi, double **Rrehilo, double **Rrelolo, double **Rimhihi, double **Rimlohi,
+    double **Rimhilo, double **Rimlolo, double *houselapms, double *RHvlapms,
+    double *tileRlapms, double *vb2Wlapms, double *WYHlapms, double *QWYHlapms,
+    double *Qaddlapms, double *YWHlapms, double *YWHClapms, double *Raddlapms,
+    double *walltimesec, long long int *addcnt, long long int *mulcnt, long long int *divcnt,
+    long long int *sqrtcnt, bool verbose = true);
+
+/* ------------------------------------------------ tiled back substitution
+ * Matrices/dbl2_tabs_kernels.h:392 and dbl4_tabs_kernels.h:604: U x = b
+ * for upper triangular U (dim = szt*nbt); U and b are left unchanged, x is
+ * returned. */
+void GPU_cmplx2_upper_tiled_solver(int dim, int szt, int nbt, double **Urehi, double **Urelo,
+                                   double **Uimhi, double **Uimlo, double *brehi, double *brelo,
+                                   double *bimhi, double *bimlo, double *xrehi, double *xrelo,
+                                   double *ximhi, double *ximlo, double *invlapms,
+                                   double *mullapms, double *sublapms, double *totlapms,
+                                   double *walltimesec, long long int *addcnt,
+                                   long long int *mulcnt, long long int *divcnt);
+void GPU_cmplx4_upper_tiled_solver(
+    int dim, int szt, int nbt, double **Urehihi, double **Urelohi, double **Urehilo,
+    double **Urelolo, double **Uimhihi, double **Uimlohi, double **Uimhilo, double **Uimlolo,
+    double *brehihi, double *brelohi, double *brehilo, double *brelolo, double *bimhihi,
+    double *bimlohi, double *bimhilo, double *bimlolo, double *xrehihi, double *xrelohi,
+    double *xrehilo, double *xrelolo, double *ximhihi, double *ximlohi, double *ximhilo,
+    double *ximlolo, double *invlapms, double *mullapms, double *sublapms, double *totlapms,
+    double *walltimesec, long long int *addcnt, double *addover, long long int *mulcnt,
+    double *mulover, long long int *divcnt, double *divover);
+
+/* ------------------------------------------------ Q^H b ------------------
+ * Newton/dbl2_bals_kernels.h:221 and dbl4_bals_kernels.h:305: b := Q^H b
+ * for the ncols x ncols Q of the QR. */
+void GPU_cmplx2_bals_qhb(int ncols, int szt, int nbt, double **Qrehi, double **Qrelo,
+                         double **Qimhi, double **Qimlo, double *brehi, double *brelo,
+                         double *bimhi, double *bimlo, double *totqtblapsedms, int vrblvl);
+void GPU_cmplx4_bals_qhb(int ncols, int szt, int nbt, double **Qrehihi, double **Qrelohi,
+                         double **Qrehilo, double **Qrelolo, double **Qimhihi, double **Qimlohi,
+                         double **Qimhilo, double **Qimlolo, double *brehihi, double *brelohi,
+                         double *brehilo, double *brelolo, double *bimhihi, double *bimlohi,
+                         double *bimhilo, double *bimlolo, double *totqtblapsedms, int vrblvl);
+
+/* ------------------------------------------------ block Toeplitz solve --
+ * Newton/dbl_bals_kernels.h:199, dbl2_bals_kernels.h:255,
+ * dbl4_bals_kernels.h:353: sum_{j<=k} A_j x_{k-j} = b_k, k < degp1, for the
+ * degp1 matrices mat[j] (dim x dim), with the reference's skip rules; Q, R,
+ * rhs, sol and the flags are updated as the reference's host solve
+ * (Newton/dbl2_bals_host.cpp:479) updates them, bit for bit. */
+void GPU_dbl_bals_solve(int dim, int degp1, int szt, int nbt, int tailidx, double ***mat,
+                        double **Q, double **R, double **rhs, double **sol, bool *zeroQ,
+                        bool *noqr, int *upidx, int *bsidx, int *newtail, double *totqrlapsedms,
+                        double *totqtblapsedms, double *totbslapsedms, double *totupdlapsedms,
+                        int vrblvl);
+void GPU_dbl2_bals_solve(int dim, int degp1, int szt, int nbt, int tailidx, double ***mathi,
+                         double ***matlo, double **Qhi, double **Qlo, double **Rhi, double **Rlo,
+                         double **rhshi, double **rhslo, double **solhi, double **sollo,
+                         bool *zeroQ, bool *noqr, int *upidx, int *bsidx, int *newtail,
+                         double *totqrlapsedms, double *totqtblapsedms, double *totbslapsedms,
+                         double *totupdlapsedms, int vrblvl);
+void GPU_dbl4_bals_solve(int dim, int degp1, int szt, int nbt, int tailidx, double ***mathihi,
+                         double ***matlohi, double ***mathilo, double ***matlolo,
+                         double **Qhihi, double **Qlohi, double **Qhilo, double **Qlolo,
+                         double **Rhihi, double **Rlohi, double **Rhilo, double **Rlolo,
+                         double **rhshihi, double **rhslohi, double **rhshilo, double **rhslolo,
+                         double **solhihi, double **sollohi, double **solhilo, double **sollolo,
+                         bool *zeroQ, bool *noqr, int *upidx, int *bsidx, int *newtail,
+                         double *totqrlapsedms, double *totqtblapsedms, double *totbslapsedms,
+                         double *totupdlapsedms, int vrblvl);
+
+#endif

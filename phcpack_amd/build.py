@@ -28,7 +28,9 @@ FLAGS = [
 
 
 def sources():
-    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    """HIP kernels and C-ABI (*.hip) and host-only C++ (*.cpp: the layer with
+    the reference's C++ signatures)"""
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 
 def _compile(src):
@@ -62,9 +64,22 @@ def build_oracle():
         raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")
 
 
+def build_compat_test():
+    """the parity test of the reference-signature layer; it links the
+    compiled reference (oracle/_ref), which exists only where the reference
+    sources are present"""
+    if not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libphcref.so")):
+        return
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "compat")],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compat test build failed:\n{r.stdout}\n{r.stderr}")
+
+
 def build_all():
     build_hip()
     build_oracle()
+    build_compat_test()
 
 
 if __name__ == "__main__":

@@ -371,8 +371,10 @@ template <int K> __device__ void jacobian(const LProg &P, Path<K> &w) {
 
 // In-place Gauss-Jordan inverse with partial pivoting: A := (P A)^{-1}
 // where the row permutation P is recorded in perm ((P r)_i = r[perm[i]]).
-// Returns false on a zero or non-finite pivot.
-template <int K> __device__ bool gj_invert(const LProg &P, Path<K> &w) {
+// Returns false on a zero or non-finite pivot.  Any n: the row swap, the
+// pivot column, the scaled pivot row and the elimination are separate
+// phases.
+template <int K> __device__ bool gj_invert_wide(const LProg &P, Path<K> &w) {
   typedef TC<K> Cx;
   const int n = P.n, l = lane_id();
   if (l < n) w.perm[l] = l;
@@ -411,6 +413,73 @@ template <int K> __device__ bool gj_invert(const LProg &P, Path<K> &w) {
       if (c == j) w.A[e] = Cx::neg(Cx::mul(w.col[i], inv));
       else Cx::msub(w.A[e], w.col[i], w.A[j * n + c]);
     }
+    wsync();
+  }
+  return true;
+}
+
+// The same inverse for n <= 16 with one phase per pivot.  Lane (c, g) =
+// (l % 16, l / 16) owns column c of rows g, g+4, g+8, g+12.  A row swap
+// and the elimination run in one sweep over those rows: the pivot row entry
+// and the old row j entry of the lane's column are read first, and in the
+// sweep over row i every read of row i (the lane's entry and the pivot
+// column entry, written by lane (j, g) in the same sweep step) comes before
+// the wavefront's writes of row i.  Each entry gets the operations of the
+// phased version in the same order, so the result is the same bit for bit.
+template <int K> __device__ bool gj_invert(const LProg &P, Path<K> &w) {
+  typedef TC<K> Cx;
+  const int n = P.n, l = lane_id();
+  if (n > 16) return gj_invert_wide<K>(P, w);
+  const int c = l & 15, g = l >> 4;
+  if (l < n) w.perm[l] = l;
+  wsync();
+  for (int j = 0; j < n; ++j) {
+    double v = -1.0;
+    int idx = l;
+    if (l >= j && l < n) {
+      v = absz<K>(w.A[l * n + j]);
+      if (v != v) v = INFINITY;
+    }
+    // candidates are lanes j..n-1 < 16: reduce within 16 lanes, take lane 0
+    for (int m = 1; m < 16; m <<= 1) {
+      const double ov = __shfl_xor(v, m, 64);
+      const int oi = __shfl_xor(idx, m, 64);
+      if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+    }
+    v = __shfl(v, 0, 64);
+    idx = __shfl(idx, 0, 64);
+    if (!(v > 0.0) || v == INFINITY) return false;
+    const int p = idx;
+    const cplx<K> piv = w.A[p * n + j], cjj = w.A[j * n + j];
+    const cplx<K> inv = Cx::div(Cx::from(1.0, 0.0), piv);
+    cplx<K> rc = Cx::zero(), aj = Cx::zero();
+    if (c < n) {
+      rc = w.A[p * n + c];
+      aj = w.A[j * n + c];
+    }
+    const cplx<K> r = (c == j) ? inv : Cx::mul(rc, inv);  // scaled pivot row
+    for (int i = g; i < n; i += 4) {
+      cplx<K> a = Cx::zero(), ci = Cx::zero();
+      if (c < n) {
+        a = w.A[i * n + c];
+        ci = w.A[i * n + j];
+      }
+      wsync();  // the reads of row i before the writes of row i
+      if (c < n) {
+        // entry (i, c) of the swapped matrix: source row p for i = j, j for i = p
+        const cplx<K> sv = (i == j) ? rc : ((i == p) ? aj : a);
+        const cplx<K> cs = (i == j) ? piv : ((i == p) ? cjj : ci);
+        cplx<K> nv;
+        if (i == j) nv = r;
+        else if (c == j) nv = Cx::neg(Cx::mul(cs, inv));
+        else {
+          nv = sv;
+          Cx::msub(nv, cs, r);
+        }
+        w.A[i * n + c] = nv;
+      }
+    }
+    if (l == 0 && p != j) { const int t = w.perm[j]; w.perm[j] = w.perm[p]; w.perm[p] = t; }
     wsync();
   }
   return true;

@@ -58,3 +58,30 @@ def test_programs_validate_on_cpu():
     ps = program_struct(prog)
     assert lib.phc_program_validate(ctypes.byref(ps)) == -2
     assert b"bad term" in lib.phc_last_error()
+
+
+def test_reference_signature_layer_exported():
+    """every GPU_* entry point of include/phcmd_compat.h is exported with
+    C++ linkage (demangled name and parameter list as declared)"""
+    import subprocess
+    text = open(os.path.join(ROOT, "include", "phcmd_compat.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    decls = re.findall(r"void\s+(GPU_\w+)\s*\(([^;]*)\);", text)
+    assert len(decls) >= 12
+    out = subprocess.run(["nm", "-D", "-C", "--defined-only", _lib.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {}
+    for line in out.splitlines():
+        m = re.search(r"\b(GPU_\w+)\((.*)\)$", line)
+        if m:
+            exported[m.group(1)] = m.group(2)
+    for name, params in decls:
+        assert name in exported, name
+        # parameter types in order (names and default values dropped)
+        types = []
+        for p in params.split(","):
+            p = p.split("=")[0].strip()
+            p = re.sub(r"\s*\b\w+$", "", p) if re.search(r"[\w\*]\s+\**\w+$", p) else p
+            types.append(re.sub(r"\s+", "", p.replace("long long int", "long long")))
+        got = [re.sub(r"\s+", "", t) for t in exported[name].split(",")]
+        assert got == types, (name, got, types)
