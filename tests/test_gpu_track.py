@@ -197,3 +197,25 @@ def test_autotune_picks_a_valid_launch_shape():
     a = tr.track(sols, K=1, params=prm)
     b = tr.track(sols, K=1, params=prm, waves=4)
     assert np.array_equal(a["endp"], b["endp"])
+
+
+def test_mixed_endpoints_checked_by_octo_double_oracle():
+    """FP64 tracking, dd endpoint Newton and the quad-double residual on the
+    GPU, then the octo-double residual of the same endpoints on the CPU
+    (oracle): every finished endpoint has residual <= 1e-24 in octo-double,
+    and the GPU's quad-double residual agrees with it far below that"""
+    from phcpack_amd.tracker import Tracker, TrackParams
+    p, hom, sols = make(5, 2, 13)
+    tr = Tracker(hom, deg=8)
+    g = tr.track(sols, K=1, params=TrackParams(tolend=1e-13))
+    r2 = tr.refine(g["endp"], 2, iters=3, tol=1e-28)
+    ok = (g["status"] == 1) & (r2["status"] == 1)
+    assert ok.sum() == len(sols)
+    x4 = np.zeros((8,) + r2["x"].shape[1:])
+    x4[:2], x4[4:6] = r2["x"][:2], r2["x"][2:]
+    res_qd = tr.refine(x4, 4, iters=0, tol=0.0)["res"]
+    res_od = oracle_refine(tr.prog0, r2["x"], 8, iters=0, tol=0.0)["res"]
+    print(f"  max od residual {res_od.max():.3e}, max |qd - od| {np.abs(res_qd - res_od).max():.3e}",
+          flush=True)
+    assert res_od.max() <= 1e-24
+    assert np.abs(res_qd - res_od).max() <= 1e-40
