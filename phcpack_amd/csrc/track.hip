@@ -199,29 +199,88 @@ template <int K> __device__ __forceinline__ double absz(const cplx<K> &z) {
   return fabs(R<K>::hi(z.re)) + fabs(R<K>::hi(z.im));
 }
 
-template <int K> __device__ __forceinline__ cplx<K> shfl_xor_z(const cplx<K> &z, int m) {
+// The value of the butterfly partner lane l ^ M (M = 1, 2, ..., 32) inside a
+// butterfly over stages M = 1, 2, 4, ... in increasing order, with every lane
+// of the wavefront active.  By stage M the M lanes of every aligned group
+// hold the same value, so for M = 4 and 8 the row mirrors (lane i of an 8-
+// or 16-lane row to lane 7 - i or 15 - i, in the other half as l ^ M is)
+// deliver the partner's value.  DPP moves for M < 16 and the permlane swaps
+// for M = 16, 32 (exactly lane l ^ M) run on the vector ALU instead of
+// going through LDS as __shfl_xor does.
+template <int M> __device__ __forceinline__ int bfly_b32(int x) {
+  if constexpr (M == 1) {
+    return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+  } else if constexpr (M == 2) {
+    return __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+  } else if constexpr (M == 4) {
+    return __builtin_amdgcn_mov_dpp(x, 0x141, 0xF, 0xF, false);  // row_half_mirror
+  } else if constexpr (M == 8) {
+    return __builtin_amdgcn_mov_dpp(x, 0x140, 0xF, 0xF, false);  // row_mirror
+  } else if constexpr (M == 16) {
+    // {rows 0,0,2,2 ; rows 1,1,3,3} of x
+    const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    return (threadIdx.x & 16) ? (int)r[0] : (int)r[1];
+  } else {
+    // {halves 0,0 ; halves 1,1} of x
+    const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    return (threadIdx.x & 32) ? (int)r[0] : (int)r[1];
+  }
+}
+
+template <int M> __device__ __forceinline__ double bfly(double v) {
+  const int lo = bfly_b32<M>(__double2loint(v)), hi = bfly_b32<M>(__double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+
+template <int M> __device__ __forceinline__ float bfly(float v) {
+  return __int_as_float(bfly_b32<M>(__float_as_int(v)));
+}
+
+template <int M, int K> __device__ __forceinline__ cplx<K> bfly_z(const cplx<K> &z) {
   cplx<K> r;
   const double *a = reinterpret_cast<const double *>(&z);
   double *b = reinterpret_cast<double *>(&r);
 #pragma unroll
-  for (int i = 0; i < 2 * K; ++i) b[i] = __shfl_xor(a[i], m, 64);
+  for (int i = 0; i < 2 * K; ++i) b[i] = bfly<M>(a[i]);
   return r;
 }
 
-// sum over the aligned group of G lanes (butterfly; identical in every lane)
+// sum over the aligned group of G lanes (butterfly v + partner per stage;
+// identical in every lane of the group)
 template <int K> __device__ __forceinline__ cplx<K> group_sum(cplx<K> v, int G) {
-  for (int m = 1; m < G; m <<= 1) v = C<K>::add(v, shfl_xor_z<K>(v, m));
+  if (G > 1) v = C<K>::add(v, bfly_z<1, K>(v));
+  if (G > 2) v = C<K>::add(v, bfly_z<2, K>(v));
+  if (G > 4) v = C<K>::add(v, bfly_z<4, K>(v));
+  if (G > 8) v = C<K>::add(v, bfly_z<8, K>(v));
+  if (G > 16) v = C<K>::add(v, bfly_z<16, K>(v));
+  if (G > 32) v = C<K>::add(v, bfly_z<32, K>(v));
   return v;
 }
 
 __device__ __forceinline__ double wave_max(double v) {
-  for (int m = 1; m < 64; m <<= 1) v = fmax(v, __shfl_xor(v, m, 64));
-  return v;
+  v = fmax(v, bfly<1>(v));
+  v = fmax(v, bfly<2>(v));
+  v = fmax(v, bfly<4>(v));
+  v = fmax(v, bfly<8>(v));
+  v = fmax(v, bfly<16>(v));
+  return fmax(v, bfly<32>(v));
 }
 
 __device__ __forceinline__ double wave_min(double v) {
-  for (int m = 1; m < 64; m <<= 1) v = fmin(v, __shfl_xor(v, m, 64));
-  return v;
+  v = fmin(v, bfly<1>(v));
+  v = fmin(v, bfly<2>(v));
+  v = fmin(v, bfly<4>(v));
+  v = fmin(v, bfly<8>(v));
+  v = fmin(v, bfly<16>(v));
+  return fmin(v, bfly<32>(v));
+}
+
+// (value, index) maximum with the smallest index among equal values, over
+// the butterfly stages M and larger
+template <int M> __device__ __forceinline__ void argmax_stage(double &v, int &idx) {
+  const double ov = bfly<M>(v);
+  const int oi = bfly_b32<M>(idx);
+  if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
 }
 
 template <int K> __device__ __forceinline__ cplx<K> coef(const LProg &P, int t) {
@@ -386,11 +445,12 @@ template <int K> __device__ bool gj_invert_wide(const LProg &P, Path<K> &w) {
       v = absz<K>(w.A[l * n + j]);
       if (v != v) v = INFINITY;
     }
-    for (int m = 1; m < 64; m <<= 1) {
-      const double ov = __shfl_xor(v, m, 64);
-      const int oi = __shfl_xor(idx, m, 64);
-      if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
-    }
+    argmax_stage<1>(v, idx);
+    argmax_stage<2>(v, idx);
+    argmax_stage<4>(v, idx);
+    argmax_stage<8>(v, idx);
+    argmax_stage<16>(v, idx);
+    argmax_stage<32>(v, idx);
     if (!(v > 0.0) || v == INFINITY) return false;
     const int p = idx;
     if (p != j) {
@@ -418,70 +478,88 @@ template <int K> __device__ bool gj_invert_wide(const LProg &P, Path<K> &w) {
   return true;
 }
 
-// The same inverse for n <= 16 with one phase per pivot.  Lane (c, g) =
-// (l % 16, l / 16) owns column c of rows g, g+4, g+8, g+12.  A row swap
-// and the elimination run in one sweep over those rows: the pivot row entry
-// and the old row j entry of the lane's column are read first, and in the
-// sweep over row i every read of row i (the lane's entry and the pivot
-// column entry, written by lane (j, g) in the same sweep step) comes before
-// the wavefront's writes of row i.  Each entry gets the operations of the
-// phased version in the same order, so the result is the same bit for bit.
+// The same inverse for n <= 16 (K <= 2) with the matrix in registers.  Lane (c, g) =
+// (l % 16, l / 16) holds column c of rows g, g+4, g+8, g+12.  Per pivot j:
+// the lanes of column j find the pivot row p among their rows (then across
+// the four of them); the lanes holding row p, row j or column j store those
+// entries to LDS; after one wavefront synchronisation every lane reads the
+// pivot row, the old row j and the pivot column it needs and updates its
+// entries of the swapped, scaled and eliminated matrix in registers.  Each
+// entry gets the operations of the phased version in the same order, so
+// the result is the same bit for bit.  The matrix is stored once at the end.
 template <int K> __device__ bool gj_invert(const LProg &P, Path<K> &w) {
   typedef TC<K> Cx;
   const int n = P.n, l = lane_id();
-  if (n > 16) return gj_invert_wide<K>(P, w);
+  // quad-double entries would take 64 registers per lane: phased version
+  if (n > 16 || K >= 4) return gj_invert_wide<K>(P, w);
   const int c = l & 15, g = l >> 4;
+  const bool act = c < n;
+  cplx<K> a[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int i = g + 4 * s;
+    a[s] = (act && i < n) ? w.A[i * n + c] : Cx::zero();
+  }
   if (l < n) w.perm[l] = l;
-  wsync();
   for (int j = 0; j < n; ++j) {
     double v = -1.0;
-    int idx = l;
-    if (l >= j && l < n) {
-      v = absz<K>(w.A[l * n + j]);
-      if (v != v) v = INFINITY;
+    int idx = 64;
+    if (c == j) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int i = g + 4 * s;
+        if (i >= j && i < n) {
+          double av = absz<K>(a[s]);
+          if (av != av) av = INFINITY;
+          if (av > v) { v = av; idx = i; }
+        }
+      }
     }
-    // candidates are lanes j..n-1 < 16: reduce within 16 lanes, take lane 0
-    for (int m = 1; m < 16; m <<= 1) {
-      const double ov = __shfl_xor(v, m, 64);
-      const int oi = __shfl_xor(idx, m, 64);
-      if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
-    }
-    v = __shfl(v, 0, 64);
-    idx = __shfl(idx, 0, 64);
+    // the four lanes of a column meet (l ^ 16, l ^ 32); column j's result from lane j
+    argmax_stage<16>(v, idx);
+    argmax_stage<32>(v, idx);
+    v = __shfl(v, j, 64);
+    idx = __shfl(idx, j, 64);
     if (!(v > 0.0) || v == INFINITY) return false;
     const int p = idx;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int i = g + 4 * s;
+      if (act && i < n && (i == p || i == j || c == j)) w.A[i * n + c] = a[s];
+    }
+    wsync();  // rows p and j and column j of the unswapped matrix in LDS
     const cplx<K> piv = w.A[p * n + j], cjj = w.A[j * n + j];
     const cplx<K> inv = Cx::div(Cx::from(1.0, 0.0), piv);
-    cplx<K> rc = Cx::zero(), aj = Cx::zero();
-    if (c < n) {
-      rc = w.A[p * n + c];
-      aj = w.A[j * n + c];
-    }
-    const cplx<K> r = (c == j) ? inv : Cx::mul(rc, inv);  // scaled pivot row
-    for (int i = g; i < n; i += 4) {
-      cplx<K> a = Cx::zero(), ci = Cx::zero();
-      if (c < n) {
-        a = w.A[i * n + c];
-        ci = w.A[i * n + j];
-      }
-      wsync();  // the reads of row i before the writes of row i
-      if (c < n) {
-        // entry (i, c) of the swapped matrix: source row p for i = j, j for i = p
-        const cplx<K> sv = (i == j) ? rc : ((i == p) ? aj : a);
-        const cplx<K> cs = (i == j) ? piv : ((i == p) ? cjj : ci);
-        cplx<K> nv;
-        if (i == j) nv = r;
-        else if (c == j) nv = Cx::neg(Cx::mul(cs, inv));
-        else {
-          nv = sv;
-          Cx::msub(nv, cs, r);
+    if (act) {
+      const cplx<K> rc = w.A[p * n + c], aj = w.A[j * n + c];
+      const cplx<K> r = (c == j) ? inv : Cx::mul(rc, inv);  // scaled pivot row
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int i = g + 4 * s;
+        if (i < n) {
+          // entry (i, c) of the swapped matrix: source row p for i = j, j for i = p
+          const cplx<K> cs = (i == j) ? piv : ((i == p) ? cjj : w.A[i * n + j]);
+          const cplx<K> sv = (i == j) ? rc : ((i == p) ? aj : a[s]);
+          cplx<K> nv;
+          if (i == j) nv = r;
+          else if (c == j) nv = Cx::neg(Cx::mul(cs, inv));
+          else {
+            nv = sv;
+            Cx::msub(nv, cs, r);
+          }
+          a[s] = nv;
         }
-        w.A[i * n + c] = nv;
       }
     }
     if (l == 0 && p != j) { const int t = w.perm[j]; w.perm[j] = w.perm[p]; w.perm[p] = t; }
-    wsync();
+    wsync();  // every read of this pivot before the next pivot's stores
   }
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int i = g + 4 * s;
+    if (act && i < n) w.A[i * n + c] = a[s];
+  }
+  wsync();
   return true;
 }
 
@@ -684,7 +762,12 @@ template <int K> __device__ int mlp_choose(const Ctl &ctl, Path<K> &w, const dou
   int best = 0;
   for (int c = 0; c < ctl.ncand; ++c) {
     float v = W3[j * ctl.ncand + c] * g;
-    for (int m = 1; m < 64; m <<= 1) v += __shfl_xor(v, m, 64);
+    v += bfly<1>(v);
+    v += bfly<2>(v);
+    v += bfly<4>(v);
+    v += bfly<8>(v);
+    v += bfly<16>(v);
+    v += bfly<32>(v);
     if (v + b3[c] >= ctl.thresh) best = c;
   }
   return best;
