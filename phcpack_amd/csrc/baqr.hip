@@ -36,6 +36,7 @@
 
 #include "common.hpp"
 #include "mdtypes.hpp"
+#include "wave.hpp"
 
 using namespace md;
 
@@ -186,17 +187,49 @@ int cgemm(bool ca, bool cb, int M, int N, int KD, Op A, Op B, double *Cp, size_t
 
 // --------------------------------------------------------- panel kernels
 
+// the butterfly partner's value of a complex number (wave::bfly per limb)
+template <int M, int K> __device__ __forceinline__ Z<K> bfly_cz(const Z<K> &s) {
+  Z<K> u;
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    u.re.x[q] = wave::bfly<M>(s.re.x[q]);
+    u.im.x[q] = wave::bfly<M>(s.im.x[q]);
+  }
+  return u;
+}
+
+// butterfly sum of a complex number over the 64 lanes of a wavefront (every
+// lane active; the sum is the same in every lane)
+template <int K> __device__ Z<K> wave_csum(Z<K> s) {
+  s = C<K>::add(s, bfly_cz<1, K>(s));
+  s = C<K>::add(s, bfly_cz<2, K>(s));
+  s = C<K>::add(s, bfly_cz<4, K>(s));
+  s = C<K>::add(s, bfly_cz<8, K>(s));
+  s = C<K>::add(s, bfly_cz<16, K>(s));
+  return C<K>::add(s, bfly_cz<32, K>(s));
+}
+
+// the butterfly partner's value of a real multiple-double (wave::bfly per limb)
+template <int M, int K> __device__ __forceinline__ T<K> bfly_r(const T<K> &s) {
+  T<K> u;
+#pragma unroll
+  for (int q = 0; q < K; ++q) u.x[q] = wave::bfly<M>(s.x[q]);
+  return u;
+}
+
 // Householder vector of column colidx of R (rows colidx..nrows-1), as
 // CPU_cmplx2_factors_house (Matrices/dbl2_factorizations.cpp:441-533):
-// v is written to Y row L from position L on (zeros before), beta to B[L].
-// sigma is a tree sum over 256 lanes; the independent scalar operations run
-// on separate lanes (the two square roots, then beta and the two parts of
-// 1/v0).
+// beta goes to B[L], 1/v0 and the sigma == 0 flag to hs[0..2K] (the
+// vector itself, v_i = x_i / v0, is formed by panel_dot_kernel for its
+// rows), zeros to Y row L before position L.  sigma is a butterfly sum per
+// wavefront, then over the four wavefronts; the independent scalar
+// operations run on separate lanes (the two square roots, then beta and the
+// two parts of 1/v0).
 template <int K>
 __global__ void __launch_bounds__(256)
 house_kernel(const double *Rm, size_t rps, int ld, int colidx, int nrows, int L, double *Y,
-             size_t yps, int rowdim, double *Bv, int szt) {
-  __shared__ T<K> part[256];
+             size_t yps, int rowdim, double *Bv, int szt, double *hs) {
+  __shared__ T<K> part[4];
   __shared__ T<K> sh[4];
   __shared__ Z<K> sv0, sinv;
   __shared__ int szero, rzero;
@@ -208,13 +241,16 @@ house_kernel(const double *Rm, size_t rps, int ld, int colidx, int nrows, int L,
     s = add(s, sqr(xi.re));
     s = add(s, sqr(xi.im));
   }
-  part[t] = s;
+  // butterfly over each wavefront, then the four wavefront sums
+  s = add(s, bfly_r<1, K>(s));
+  s = add(s, bfly_r<2, K>(s));
+  s = add(s, bfly_r<4, K>(s));
+  s = add(s, bfly_r<8, K>(s));
+  s = add(s, bfly_r<16, K>(s));
+  s = add(s, bfly_r<32, K>(s));
+  if ((t & 63) == 0) part[t >> 6] = s;
   __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if (t < h) part[t] = add(part[t], part[t + h]);
-    __syncthreads();
-  }
-  const T<K> sigma = part[0];
+  const T<K> sigma = add(add(part[0], part[1]), add(part[2], part[3]));
   const Z<K> x0 = x(0);
   if (t < 2) {  // lane 0: |x0|, lane 1: sqrt(|x0|^2 + sigma)
     T<K> sq = sqr(x0.re);
@@ -257,70 +293,90 @@ house_kernel(const double *Rm, size_t rps, int ld, int colidx, int nrows, int L,
     }
   }
   __syncthreads();
-  if (t == 0) R<K>::store(Bv, szt, L, szero ? R<K>::zero() : sh[3]);
-  const Z<K> inv = sinv;
-  const bool zero = szero != 0;
-  for (int i = t; i < rowdim; i += 256) {
-    Z<K> v;
-    if (i < L) v = C<K>::zero();
-    else if (i == L) v = C<K>::from(1.0, 0.0);
-    else {
-      const Z<K> xi = x(i - L);
-      if (zero) v = xi;
-      else {
-        v.re = sub(mul(xi.re, inv.re), mul(xi.im, inv.im));
-        v.im = add(mul(xi.im, inv.re), mul(xi.re, inv.im));
-      }
+  if (t == 0) {
+    R<K>::store(Bv, szt, L, szero ? R<K>::zero() : sh[3]);
+    const Z<K> inv = szero ? C<K>::zero() : sinv;
+    for (int q = 0; q < K; ++q) {
+      hs[q] = inv.re.x[q];
+      hs[K + q] = inv.im.x[q];
     }
-    C<K>::store(Y, yps, (size_t)L * rowdim + i, v);
+    hs[2 * K] = szero ? 1.0 : 0.0;
   }
+  for (int i = t; i < L; i += 256) C<K>::store(Y, yps, (size_t)L * rowdim + i, C<K>::zero());
 }
 
 constexpr int RCH = 8;  // rows per chunk of the panel dot products
 
+// the Householder vector on the rows of this chunk, v_i = x_i / v0 (x_i
+// itself when sigma = 0, 1 on the diagonal), written to Y row L, then the
 // partial sums wp[chunk][j] = sum_{i in chunk} conj(R_ij) v_i over columns
-// j = colidx..endcol-1 (one lane per column)
+// j = colidx..endcol-1: lane (h, c) = (t / 128, t % 128) takes rows h and
+// h + 4 of the chunk for column c, the two row halves meet in LDS
 template <int K>
-__global__ void __launch_bounds__(128)
+__global__ void __launch_bounds__(512)
 panel_dot_kernel(const double *Rm, size_t rps, int ld, int colidx, int endcol, int nrows,
-                 const double *Y, size_t yps, int rowdim, int L, double *wp, size_t wps,
-                 int szt) {
+                 double *Y, size_t yps, int rowdim, int L, double *wp, size_t wps,
+                 int szt, const double *hs) {
   __shared__ double vs[2 * K][RCH];
-  const int chunk = blockIdx.x, t = threadIdx.x;
+  __shared__ Z<K> part[4][128];
+  const int chunk = blockIdx.x, t = threadIdx.x, c = t & 127, h = t >> 7;
   const int r0 = colidx + chunk * RCH, n = nrows - colidx;
   const int rows = (n - chunk * RCH) < RCH ? (n - chunk * RCH) : RCH;
   if (t < rows) {
-    const Z<K> v = C<K>::load(Y, yps, (size_t)L * rowdim + L + chunk * RCH + t);
+    const int i = chunk * RCH + t;  // row colidx + i
+    Z<K> v;
+    if (i == 0) {
+      v = C<K>::from(1.0, 0.0);
+    } else {
+      const Z<K> xi = C<K>::load(Rm, rps, (size_t)(colidx + i) * ld + colidx);
+      if (hs[2 * K] != 0.0) {
+        v = xi;
+      } else {
+        Z<K> inv;
+        for (int q = 0; q < K; ++q) {
+          inv.re.x[q] = hs[q];
+          inv.im.x[q] = hs[K + q];
+        }
+        v.re = sub(mul(xi.re, inv.re), mul(xi.im, inv.im));
+        v.im = add(mul(xi.im, inv.re), mul(xi.re, inv.im));
+      }
+    }
+    C<K>::store(Y, yps, (size_t)L * rowdim + L + i, v);
     for (int q = 0; q < K; ++q) { vs[q][t] = v.re.x[q]; vs[K + q][t] = v.im.x[q]; }
   }
   __syncthreads();
-  const int j = colidx + t;
-  if (j >= endcol) return;
+  const int j = colidx + c;
   Z<K> s = C<K>::zero();
-  for (int i = 0; i < rows; ++i) {
-    const Z<K> r = C<K>::load(Rm, rps, (size_t)(r0 + i) * ld + j);
-    const Z<K> v = C<K>::load(&vs[0][0], RCH, i);
-    cmac<K, true, false>(s, r, v);
+  if (j < endcol) {
+    for (int i = h; i < rows; i += 4) {
+      const Z<K> r = C<K>::load(Rm, rps, (size_t)(r0 + i) * ld + j);
+      const Z<K> v = C<K>::load(&vs[0][0], RCH, i);
+      cmac<K, true, false>(s, r, v);
+    }
   }
-  C<K>::store(wp, wps, (size_t)chunk * szt + t, s);
+  part[h][c] = s;
+  __syncthreads();
+  if (h < 2) part[h][c] = C<K>::add(part[h][c], part[h + 2][c]);
+  __syncthreads();
+  if (h == 0 && j < endcol) C<K>::store(wp, wps, (size_t)chunk * szt + c, C<K>::add(part[0][c], part[1][c]));
 }
 
-// w_j = beta * sum_chunks wp[.][j]: 4 lanes per column sum interleaved
-// chunks, then a two-level tree
+// w_j = beta * sum_chunks wp[.][j], one workgroup per column: each lane
+// loads at most a few chunks, a butterfly per wavefront, then the four
+// wavefront sums
 template <int K>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(256)
 panel_reduce_kernel(const double *wp, size_t wps, int szt, int ncol, int nchunk,
                     const double *Bv, int L, double *w) {
-  __shared__ Z<K> part[4][128];
-  const int c = threadIdx.x & 127, g = threadIdx.x >> 7;
+  __shared__ Z<K> part[4];
+  const int c = blockIdx.x, t = threadIdx.x;
   Z<K> s = C<K>::zero();
-  if (c < ncol)
-    for (int k = g; k < nchunk; k += 4) s = C<K>::add(s, C<K>::load(wp, wps, (size_t)k * szt + c));
-  part[g][c] = s;
+  for (int k = t; k < nchunk; k += 256) s = C<K>::add(s, C<K>::load(wp, wps, (size_t)k * szt + c));
+  s = wave_csum<K>(s);
+  if ((t & 63) == 0) part[t >> 6] = s;
   __syncthreads();
-  if (g == 0 && c < ncol) {
-    const Z<K> a = C<K>::add(part[0][c], part[1][c]), b = C<K>::add(part[2][c], part[3][c]);
-    Z<K> v = C<K>::add(a, b);
+  if (t == 0) {
+    Z<K> v = C<K>::add(C<K>::add(part[0], part[1]), C<K>::add(part[2], part[3]));
     const T<K> beta = R<K>::load(Bv, szt, L);
     v.re = mul(beta, v.re);
     v.im = mul(beta, v.im);
@@ -364,14 +420,7 @@ s_factor_kernel(const double *G, size_t gps, const double *Bv, double *S, size_t
       for (int k = m + l; k < j; k += 64)
         s = C<K>::add(s, C<K>::mul(C<K>::load(&row[0][0], 256, k),
                                     C<K>::load(G, gps, (size_t)k * szt + j)));
-      for (int o = 32; o > 0; o >>= 1) {
-        Z<K> u;
-        for (int q = 0; q < K; ++q) {
-          u.re.x[q] = __shfl_xor(s.re.x[q], o, 64);
-          u.im.x[q] = __shfl_xor(s.im.x[q], o, 64);
-        }
-        s = C<K>::add(s, u);
-      }
+      s = wave_csum<K>(s);
     }
     if (l == 0) {
       const T<K> b = neg(R<K>::load(Bv, szt, j));
@@ -387,19 +436,6 @@ s_factor_kernel(const double *G, size_t gps, const double *Bv, double *S, size_t
 }
 
 // ------------------------------------------------------- tiled solver
-
-// butterfly sum of a complex number over the 64 lanes of a wavefront
-template <int K> __device__ Z<K> wave_csum(Z<K> s) {
-  for (int o = 32; o > 0; o >>= 1) {
-    Z<K> u;
-    for (int q = 0; q < K; ++q) {
-      u.re.x[q] = __shfl_xor(s.re.x[q], o, 64);
-      u.im.x[q] = __shfl_xor(s.im.x[q], o, 64);
-    }
-    s = C<K>::add(s, u);
-  }
-  return s;
-}
 
 // inverse of the diagonal tiles of U (dim x dim): one wavefront per column j
 // of tile t, x = T^{-1} e_j by back substitution with the sums over k split
@@ -480,7 +516,7 @@ __global__ void init_qr_kernel(int nrows, int ncols, const double *A, double *Q,
 
 struct phc_baqr {
   int K, nrows, ncols, szt, nbt, nchunk;
-  double *Y, *W, *G, *S, *T1, *T2, *wp, *w, *B, *part;
+  double *Y, *W, *G, *S, *T1, *T2, *wp, *w, *B, *part, *hs;
   size_t yps, gps, t1ps, t2ps, wps, pcap;
   // the launch sequence of one factorization, captured for the last (A, Q, R)
   hipStream_t cap;
@@ -503,12 +539,12 @@ template <int K> int run_qr(phc_baqr *p, const double *A, double *Q, double *R, 
       const int colidx = rowoff + L, nr = m - colidx;
       const int nchunk = (nr + RCH - 1) / RCH, ncol = endcol - colidx;
       hipLaunchKernelGGL(house_kernel<K>, dim3(1), dim3(256), 0, st, R, rps, n, colidx, m, L,
-                         p->Y, p->yps, rowdim, p->B, szt);
-      hipLaunchKernelGGL(panel_dot_kernel<K>, dim3(nchunk), dim3(128), 0, st, R, rps, n, colidx,
-                         endcol, m, p->Y, p->yps, rowdim, L, p->wp, p->wps, szt);
-      hipLaunchKernelGGL(panel_reduce_kernel<K>, dim3(1), dim3(512), 0, st, p->wp, p->wps, szt,
+                         p->Y, p->yps, rowdim, p->B, szt, p->hs);
+      hipLaunchKernelGGL(panel_dot_kernel<K>, dim3(nchunk), dim3(512), 0, st, R, rps, n, colidx,
+                         endcol, m, p->Y, p->yps, rowdim, L, p->wp, p->wps, szt, p->hs);
+      hipLaunchKernelGGL(panel_reduce_kernel<K>, dim3(ncol), dim3(256), 0, st, p->wp, p->wps, szt,
                          ncol, nchunk, p->B, L, p->w);
-      const int rpb = 8, nblk = (nr + rpb - 1) / rpb;
+      const int rpb = 1, nblk = (nr + rpb - 1) / rpb;
       hipLaunchKernelGGL(panel_update_kernel<K>, dim3(nblk), dim3(128), 0, st, R, rps, n, colidx,
                          endcol, m, p->Y, p->yps, rowdim, L, p->w, szt, rpb);
     }
@@ -595,10 +631,12 @@ int phc_baqr_create(phc_baqr **out, int K, int nrows, int ncols, int szt) {
   p->pcap = (size_t)nrows * szt * 8;
   if ((size_t)nrows * 32 > p->pcap) p->pcap = (size_t)nrows * 32;
   const size_t z = sizeof(double) * 2 * K;
-  double **bufs[] = {&p->Y, &p->W, &p->G, &p->S, &p->T1, &p->T2, &p->wp, &p->w, &p->B, &p->part};
+  double **bufs[] = {&p->Y, &p->W, &p->G, &p->S, &p->T1, &p->T2, &p->wp, &p->w, &p->B, &p->part,
+                     &p->hs};
   const size_t sizes[] = {z * p->yps, z * p->yps, z * p->gps, z * p->gps, z * p->t1ps,
-                          z * p->t2ps, z * p->wps, z * szt, sizeof(double) * K * szt, z * p->pcap};
-  const int nb = 10;
+                          z * p->t2ps, z * p->wps, z * szt, sizeof(double) * K * szt, z * p->pcap,
+                          sizeof(double) * (2 * K + 1)};
+  const int nb = 11;
   for (int i = 0; i < nb; ++i) *bufs[i] = nullptr;
   p->exec = nullptr;
   p->cap = nullptr;
@@ -626,7 +664,7 @@ int phc_baqr_destroy(phc_baqr *p) {
   if (!p) return 0;
   if (p->exec) hipGraphExecDestroy(p->exec);
   if (p->cap) hipStreamDestroy(p->cap);
-  double *bufs[] = {p->Y, p->W, p->G, p->S, p->T1, p->T2, p->wp, p->w, p->B, p->part};
+  double *bufs[] = {p->Y, p->W, p->G, p->S, p->T1, p->T2, p->wp, p->w, p->B, p->part, p->hs};
   for (double *b : bufs) hipFree(b);
   delete p;
   return 0;

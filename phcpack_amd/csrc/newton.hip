@@ -27,6 +27,7 @@
 
 #include "common.hpp"
 #include "mdtypes.hpp"
+#include "wave.hpp"
 
 using namespace md;
 using phc::grid_for;
@@ -41,10 +42,7 @@ __device__ __forceinline__ void wsync() {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
-__device__ __forceinline__ double wave_fmax(double v) {
-  for (int m = 1; m < 64; m <<= 1) v = fmax(v, __shfl_xor(v, m, 64));
-  return v;
-}
+__device__ __forceinline__ double wave_fmax(double v) { return wave::max64(v); }
 
 constexpr size_t LDS_LIMIT = 160 * 1024;
 

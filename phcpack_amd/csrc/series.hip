@@ -50,7 +50,7 @@ template <int K> struct Num<K, true> {
 // ----------------------------------------------------------- elementwise --
 
 template <int K>
-__global__ void md_batch_kernel(int op, int n, const double *a, const double *b, double *c) {
+__global__ void __launch_bounds__(256) md_batch_kernel(int op, int n, const double *a, const double *b, double *c) {
   typedef typename RealT<K>::type T;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
     T x = R<K>::load(a, n, j), z;
@@ -77,7 +77,7 @@ __device__ typename N::V coef_product(const double *x, size_t sx, size_t xo, con
 
 // z[b] = x[b]*y[b] for b < nbatch; arrays [P][nbatch][D]
 template <int K, bool CPLX>
-__global__ void product_kernel(int D, int nbatch, const double *x, const double *y, double *z) {
+__global__ void __launch_bounds__(256) product_kernel(int D, int nbatch, const double *x, const double *y, double *z) {
   typedef Num<K, CPLX> N;
   const size_t s = (size_t)nbatch * D;
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < s;
@@ -96,7 +96,7 @@ __global__ void product_kernel(int D, int nbatch, const double *x, const double 
 // separated by block barriers because coefficient k of f[j] needs
 // coefficients 0..k of f[j-1], which other lanes produce.
 template <int K, bool CPLX>
-__global__ void poly_evaldiff_kernel(int dim, int nbr, int D, const int *__restrict__ nvr,
+__global__ void __launch_bounds__(256) poly_evaldiff_kernel(int dim, int nbr, int D, const int *__restrict__ nvr,
                                      const int *__restrict__ idxptr, const int *__restrict__ idx,
                                      const double *__restrict__ cst, const double *__restrict__ cff,
                                      int nbatch, const double *__restrict__ input,
