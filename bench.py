@@ -76,13 +76,13 @@ def run_plain(tr, d_start, npaths, bufs, ev, prm):
 
 def run_mixed(tr, d_start, starts, npaths, bufs, ref, ev, prm, lib, controller=None):
     """FP64 tracking (plain pole-step rule, or the learned controller when
-    given), then dd Newton at the endpoints of the paths the tracker
-    finished (the others are passed over on the device).  Finished paths
-    whose dd Newton does not converge are refined in quad-double; with the
-    learned controller, paths it did not finish or that still fail are
-    tracked again with the plain rule (a path the controller lost is
-    recovered, not dropped).  Returns (tracking ms, endpoint ms, escalated,
-    re-tracked)."""
+    given: a path it does not finish is tracked again with the plain rule
+    inside the same kernel), then dd Newton at the endpoints of the paths the
+    tracker finished (the others are passed over on the device).  Finished
+    paths whose dd Newton does not converge are refined in quad-double; with
+    the learned controller, paths that still fail are tracked again with the
+    plain rule (a path the controller lost is recovered, not dropped).
+    Returns (tracking ms, endpoint ms, escalated, re-tracked)."""
     from phcpack_amd._lib import check
     ev.start()
     tr.track_device(d_start, 1, npaths, 1, bufs, prm, controller=controller)
@@ -106,7 +106,6 @@ def run_mixed(tr, d_start, starts, npaths, bufs, ref, ev, prm, lib, controller=N
             fix[int(p)] = (r4["x"][:, j], bool(ok4[j]))
         lost = bad[~ok4]
     if controller is not None:
-        lost = np.union1d(lost, np.nonzero(~tracked)[0])
         if len(lost):
             o = tr.track(starts[:, lost], K=1, params=prm)
             r2 = tr.refine(o["endp"], 2, iters=3, tol=1e-28)

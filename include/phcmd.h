@@ -155,6 +155,10 @@ int phc_program_validate(const phc_program *program);
  * nfeat -> 64 (ReLU) -> 64 (ReLU) -> ncand logits (phc_mlp layout).
  * mode 0: plain rule (the controller is ignored);
  * mode 1: learned: the largest c whose logit >= logit_threshold (c = 0 if none);
+ *         a path that does not finish (status other than success) is
+ *         tracked again from its start with the plain rule in the same
+ *         launch: its work counters add up both attempts, its other outputs
+ *         are the second attempt's;
  * mode 2: explore: c drawn from a hash of (seed, path, step), and every step
  *         logged: features log_x[path][i][nfeat], candidate log_c[path][i],
  *         log_y[path][i] = 1 if the first attempt was accepted, log_n[path]

@@ -471,7 +471,15 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
     double t = 0.0, h = prm->maxstep, dxl = 0.0, hprev = prm->maxstep;
     double dxh[4] = {1.0, 1.0, 1.0, 1.0};
     int st = ST_RUNNING, steps = 0, rej = 0, j, k, npred = 0, nnewt = 0, nfeval = 0;
-    int rejprev = 0, nlog = 0;
+    int rejprev = 0, nlog = 0, attempt, pmode = mode;
+    /* a path the learned controller (mode 1) does not finish is tracked
+       again from its start with the plain pole-step rule; the work counters
+       add up both attempts, the other outputs are the last attempt's */
+    for (attempt = 0; attempt < 2; attempt++) {
+    pmode = (attempt == 0) ? mode : 0;
+    t = 0.0; h = prm->maxstep; dxl = 0.0; hprev = prm->maxstep;
+    for (j = 0; j < 4; j++) dxh[j] = 1.0;
+    st = ST_RUNNING; steps = 0; rej = 0; rejprev = 0;
     load_point(P, &w, start, KS, npaths, path);
     while (st == ST_RUNNING && t < 1.0) {
       const int kd = D - 1 - prm->fabry_offset;
@@ -489,7 +497,7 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
       {
         int cand = 0, rej0, i;
         double feat[OTR_NFEAT];
-        if (mode != 0) {
+        if (pmode != 0) {
           feat[0] = otr_lg2((r < INFINITY && r > 0.0) ? r : prm->maxstep);
           feat[1] = otr_lg2(hprev);
           feat[2] = t;
@@ -500,7 +508,7 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
           }
           for (i = 0; i < 4; i++) feat[11 + i] = otr_lg2(dxh[i] + 1e-300);
           feat[15] = (double)rejprev;
-          if (mode == 1) cand = mlp_choose(ctl, feat);
+          if (pmode == 1) cand = mlp_choose(ctl, feat);
           else cand = (int)(mix_hash(ctl->seed, (unsigned)path, (unsigned)(npred - 1)) % (unsigned)ctl->ncand);
           h *= ctl->mult[cand];
           if (ctl->pole_cap > 0.0 && r < INFINITY && r > 0.0) h = fmin(h, ctl->pole_cap * r);
@@ -558,6 +566,8 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
       set_weights(P, &w, 1.0);
       nr = newton(P, &w, prm->endsteps, prm->tolend, &dxl, &nnewt, NULL);
       st = (nr < 0) ? ST_SINGULAR : ST_SUCCESS;
+    }
+    if (!(pmode == 1 && st != ST_SUCCESS)) break;
     }
     store_point(P, &w, endp, npaths, path);
     tend[path] = t;

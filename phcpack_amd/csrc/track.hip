@@ -741,11 +741,19 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, Params par
     if (l == 0) path = atomicAdd(counter, 1);
     path = __shfl(path, 0, 64);
     if (path >= npaths) break;
-    load_point<K>(P, w, start, KS, npaths, path);
     double t = 0.0, h = par.maxstep, dxl = 0.0, hprev = par.maxstep;
     double dxh[4] = {1.0, 1.0, 1.0, 1.0};
     int st = ST_RUNNING, steps = 0, rej = 0, npred = 0, nnewt = 0, nfeval = 0, rejprev = 0;
-    int nlog = 0;
+    int nlog = 0, pmode = ctl.mode;
+    // a path the learned controller (mode 1) does not finish is tracked
+    // again from its start with the plain pole-step rule; the work counters
+    // add up both attempts, the other outputs are the last attempt's
+    for (int attempt = 0; attempt < 2; ++attempt) {
+    pmode = (attempt == 0) ? ctl.mode : 0;
+    t = 0.0; h = par.maxstep; dxl = 0.0; hprev = par.maxstep;
+    dxh[0] = dxh[1] = dxh[2] = dxh[3] = 1.0;
+    st = ST_RUNNING; steps = 0; rej = 0; rejprev = 0;
+    load_point<K>(P, w, start, KS, npaths, path);
     while (st == ST_RUNNING && t < 1.0) {
       // ---- predictor: Taylor series of x(t+s) ----
       set_weights<K>(P, w, t);
@@ -762,10 +770,10 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, Params par
       h = (r < INFINITY && r > 0.0) ? fmin(par.pbeta * r, par.maxstep) : par.maxstep;
       // ---- step-size controller ----
       int cand = 0;
-      if (ctl.mode != 0) {
+      if (pmode != 0) {
         double feat[PHC_CTL_NFEAT];
         step_features<K>(P, w, r, par.maxstep, hprev, t, dxh, rejprev, feat);
-        if (ctl.mode == 1) cand = mlp_choose<K>(ctl, w, feat);
+        if (pmode == 1) cand = mlp_choose<K>(ctl, w, feat);
         else cand = (int)(mix_hash(ctl.seed, (unsigned)path, (unsigned)(npred - 1)) % (unsigned)ctl.ncand);
         h *= ctl.mult[cand];
         if (ctl.pcap > 0.0 && r < INFINITY && r > 0.0) h = fmin(h, ctl.pcap * r);
@@ -824,6 +832,8 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, Params par
       set_weights<K>(P, w, 1.0);
       const int nr = newton<K>(P, w, par.endsteps, par.tolend, &dxl, &nnewt, nullptr);
       st = (nr < 0) ? ST_SINGULAR : ST_SUCCESS;
+    }
+    if (!(pmode == 1 && st != ST_SUCCESS)) break;
     }
     store_point<K>(P, w, endp, npaths, path);
     if (l == 0) {

@@ -87,6 +87,37 @@ def test_learned_mode_matches_oracle(K):
     assert np.all(g["status"] == 1)
 
 
+@pytest.mark.parametrize("K", [1, 2])
+def test_unfinished_learned_paths_fall_back_to_the_plain_rule(K):
+    """A controller that always picks a quarter of the pole step, with a
+    step budget the plain rule meets and the quartered steps do not: every
+    path fails its first attempt (maxsteps), is tracked again with the plain
+    rule in the same launch and finishes; the work counters count both
+    attempts; GPU and oracle agree bit for bit."""
+    from phcpack_amd.controller import NFEAT, HIDDEN, StepController, nparams
+    from phcpack_amd.tracker import Tracker, TrackParams, points_to_planes
+    hom, sols = _system(4, 2, 10)
+    tr = Tracker(hom, deg=8)
+    base = TrackParams(tolend=1e-13 if K == 1 else 1e-28)
+    plain = tr.track(sols, K=K, params=base)
+    assert np.all(plain["status"] == 1)
+    prm = TrackParams(tolend=base.tolend, maxsteps=int(1.5 * plain["nsteps"].max()) + 1)
+    mults = (1.0, 0.25)
+    w = np.zeros(nparams(C=len(mults)), np.float32)
+    w[-len(mults):] = 10.0                      # every logit 10: the last candidate
+    ctl = StepController(weights=w, threshold=0.5, mults=mults, pole_cap=0.0)
+    g = tr.track(sols, K=K, params=prm, controller=ctl.struct(1))
+    oc = OracleController(1, len(sols), ctl=ctl)
+    o = oracle_track(tr.prog, points_to_planes(sols), K, prm, controller=oc)
+    for k in ("endp", "status", "nsteps", "nrej", "work", "tend"):
+        assert np.array_equal(g[k], o[k]), k
+    assert np.all(g["status"] == 1)
+    # the second attempt is the plain rule: same steps as plain tracking,
+    # and the predictor work of the failed first attempt on top
+    assert np.array_equal(g["nsteps"], plain["nsteps"])
+    assert np.array_equal(g["work"][0], plain["work"][0] + prm.maxsteps)
+
+
 def test_trained_controller_gains_over_plain_rule():
     """A/B on held-out systems: the controller trained (on the GPU, by the
     library's own kernels) on explore logs of other systems of the family
