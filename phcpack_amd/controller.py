@@ -222,7 +222,8 @@ def train_controller(trackers, starts, params, K=1, seed=1, cap=256, epochs=30, 
 
 
 def tune_threshold(ctl, tracker, starts, params, K=1,
-                   grid=((0.7, 1.0), (0.9, 1.0), (0.9, 0.75), (0.9, 1.5), (0.95, 0.75)),
+                   grid=((0.5, 1.0), (0.7, 1.0), (0.8, 1.0), (0.9, 1.0), (0.7, 1.25),
+                         (0.9, 0.75), (0.9, 1.5), (0.95, 0.75)),
                    timer=None):
     """Pick (acceptance-probability threshold, pole cap) with the fewest
     predictor steps (or the shortest time when `timer(fn)` is given) among
