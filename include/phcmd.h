@@ -198,7 +198,8 @@ int phc_tracker_lds_bytes(const phc_tracker *tracker, int K, size_t *program_byt
  * Jacobian, 5 diverged), nsteps, nrej (rejected steps), dxlast, and when
  * work is not NULL the work counters work[c*npaths + path] for c = 0
  * predictor series computed, 1 Newton iterations, 2 predicted-residual
- * evaluations.
+ * evaluations, 3 corrector calls (whose first Newton iteration reuses the
+ * predicted-residual evaluation and only forms the Jacobian).
  * Replaces GPU_Path_mult / path_mult (Path/Path_GPU/path_gpu_mult.cu:84-383)
  * behind the gpumanypaths_dd entry (Ada/PHCtoC/gpumanypaths_dd.cpp:18). */
 int phc_track(phc_tracker *tracker, int K, int npaths, const double *start_dev, int KS,

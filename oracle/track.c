@@ -370,14 +370,15 @@ static int mlp_choose(const otr_controller *ctl, const double *f) {
 }
 
 static int newton(const otr_program *P, work_t *w, int iters, double tol, double *dxl, int *count,
-                  double *dxh) {
+                  double *dxh, int evaluated) {
   const int K = w->K, D = P->deg + 1;
   double dx[16 * 64];
   int it, j;
   for (it = 0; it < iters; it++) {
     double m;
     ++*count;
-    eval_point(P, w, 1);
+    /* evaluated: products and residual at this point are already there */
+    if (it == 0 && evaluated) jacobian(P, w); else eval_point(P, w, 1);
     if (!gj_invert(P, w)) return -1;
     apply_inverse(P, w, dx);
     for (j = 0; j < P->n; j++) omd_cadd(K, ZP(w->slots, j * D), ZP(dx, j), ZP(w->slots, j * D));
@@ -470,7 +471,7 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
   for (path = p0; path < p1; path++) {
     double t = 0.0, h = prm->maxstep, dxl = 0.0, hprev = prm->maxstep;
     double dxh[4] = {1.0, 1.0, 1.0, 1.0};
-    int st = ST_RUNNING, steps = 0, rej = 0, j, k, npred = 0, nnewt = 0, nfeval = 0;
+    int st = ST_RUNNING, steps = 0, rej = 0, j, k, npred = 0, nnewt = 0, nfeval = 0, ncorr = 0;
     int rejprev = 0, nlog = 0, attempt, pmode = mode;
     /* a path the learned controller (mode 1) does not finish is tracked
        again from its start with the plain pole-step rule; the work counters
@@ -536,7 +537,8 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
         res = max_abs(K, w.rhs, n);
         ok = res <= prm->alpha * (1.0 + max_abs_x(P, &w));
         if (ok) {
-          const int nr = newton(P, &w, prm->corsteps, prm->tolcor, &dxl, &nnewt, dxh);
+          const int nr = newton(P, &w, prm->corsteps, prm->tolcor, &dxl, &nnewt, dxh, 1);
+          ++ncorr;
           if (nr < 0) { st = ST_SINGULAR; break; }
           ok = (nr == 1);
         }
@@ -564,7 +566,7 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
     if (st == ST_RUNNING) {
       int nr;
       set_weights(P, &w, 1.0);
-      nr = newton(P, &w, prm->endsteps, prm->tolend, &dxl, &nnewt, NULL);
+      nr = newton(P, &w, prm->endsteps, prm->tolend, &dxl, &nnewt, NULL, 0);
       st = (nr < 0) ? ST_SINGULAR : ST_SUCCESS;
     }
     if (!(pmode == 1 && st != ST_SUCCESS)) break;
@@ -579,6 +581,7 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
       work[path] = npred;
       work[npaths + path] = nnewt;
       work[2 * npaths + path] = nfeval;
+      work[3 * npaths + path] = ncorr;
     }
     if (mode == 2) ctl->log_n[path] = nlog;
   }
@@ -634,7 +637,7 @@ void otr_refine(const otr_program *P, int K, int npaths, double t, const double 
     int nr = 1, its = 0;
     load_point(P, &w, xin, KS, npaths, path);
     set_weights(P, &w, t);
-    if (iters > 0) nr = newton(P, &w, iters, tol, &dxl, &its, NULL);
+    if (iters > 0) nr = newton(P, &w, iters, tol, &dxl, &its, NULL, 0);
     eval_point(P, &w, 0);
     res[path] = max_abs(K, w.rhs, P->n);
     store_point(P, &w, xout, npaths, path);

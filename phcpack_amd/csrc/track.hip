@@ -542,15 +542,19 @@ template <int K> __device__ double max_abs_x(const LProg &P, Path<K> &w) {
 }
 
 // Newton at fixed t on the order-0 slots (weights already set): returns
-// 1 converged, 0 not converged, -1 singular; *dxl = last max |dx|
+// 1 converged, 0 not converged, -1 singular; *dxl = last max |dx|.
+// evaluated: the products and the residual at the current point are
+// already in place (the corrector's predicted-residual check), so the first
+// iteration only forms the Jacobian.
 template <int K>
 __device__ int newton(const LProg &P, Path<K> &w, int iters, double tol, double *dxl, int *count,
-                      double *dxh) {
+                      double *dxh, bool evaluated) {
   typedef TC<K> Cx;
   const int l = lane_id();
   for (int it = 0; it < iters; ++it) {
     ++*count;
-    eval_point<K>(P, w, true);
+    if (it == 0 && evaluated) jacobian<K>(P, w);
+    else eval_point<K>(P, w, true);
     if (!gj_invert<K>(P, w)) return -1;
     const cplx<K> dx = apply_inverse<K>(P, w);
     double mdx = 0.0;
@@ -744,6 +748,7 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, Params par
     double t = 0.0, h = par.maxstep, dxl = 0.0, hprev = par.maxstep;
     double dxh[4] = {1.0, 1.0, 1.0, 1.0};
     int st = ST_RUNNING, steps = 0, rej = 0, npred = 0, nnewt = 0, nfeval = 0, rejprev = 0;
+    int ncorr = 0;
     int nlog = 0, pmode = ctl.mode;
     // a path the learned controller (mode 1) does not finish is tracked
     // again from its start with the plain pole-step rule; the work counters
@@ -807,7 +812,8 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, Params par
         const double res = max_abs_rhs<K>(P, w);
         bool ok = res <= par.alpha * (1.0 + max_abs_x<K>(P, w));
         if (ok) {
-          const int nr = newton<K>(P, w, par.corsteps, par.tolcor, &dxl, &nnewt, dxh);
+          const int nr = newton<K>(P, w, par.corsteps, par.tolcor, &dxl, &nnewt, dxh, true);
+          ++ncorr;
           if (nr < 0) { st = ST_SINGULAR; break; }
           ok = (nr == 1);
         }
@@ -830,7 +836,7 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, Params par
     }
     if (st == ST_RUNNING) {
       set_weights<K>(P, w, 1.0);
-      const int nr = newton<K>(P, w, par.endsteps, par.tolend, &dxl, &nnewt, nullptr);
+      const int nr = newton<K>(P, w, par.endsteps, par.tolend, &dxl, &nnewt, nullptr, false);
       st = (nr < 0) ? ST_SINGULAR : ST_SUCCESS;
     }
     if (!(pmode == 1 && st != ST_SUCCESS)) break;
@@ -846,6 +852,7 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, Params par
         work[path] = npred;
         work[npaths + path] = nnewt;
         work[2 * npaths + path] = nfeval;
+        work[3 * npaths + path] = ncorr;
       }
       if (ctl.mode == 2) ctl.log_n[path] = nlog;
     }
@@ -890,7 +897,7 @@ __global__ void __launch_bounds__(64 * REFINE_MAX_WAVES) refine_kernel(DevProgra
     set_weights<K>(P, w, tval);
     double dxl = 0.0;
     int nr = 1, its = 0;
-    if (iters > 0) nr = newton<K>(P, w, iters, tol, &dxl, &its, nullptr);
+    if (iters > 0) nr = newton<K>(P, w, iters, tol, &dxl, &its, nullptr, false);
     eval_point<K>(P, w, false);
     const double res = max_abs_rhs<K>(P, w);
     store_point<K>(P, w, xout, npaths, path);

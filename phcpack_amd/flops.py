@@ -107,16 +107,19 @@ def _phase_ops(prog):
 
 
 def tracker_flops(prog, K, work, table=BUILD):
-    """FP64 flops of one tracking launch given the work counters [3][npaths]
-    (predictor series, Newton iterations, predicted-residual evaluations)."""
+    """FP64 flops of one tracking launch given the work counters [4][npaths]
+    (predictor series, Newton iterations, predicted-residual evaluations,
+    corrector calls: the first Newton iteration of a corrector call reuses
+    the predicted-residual evaluation)."""
     ph = _phase_ops(prog)
     c = complex_costs(K, table)
 
     def cost(d):
         return sum(c[k] * v for k, v in d.items())
 
-    npred, nnewt, nfeval = (int(x.sum()) for x in work)
-    return npred * cost(ph["predictor"]) + nnewt * cost(ph["newton"]) + nfeval * cost(ph["feval"])
+    npred, nnewt, nfeval, ncorr = (int(x.sum()) for x in work)
+    return (npred * cost(ph["predictor"]) + nnewt * cost(ph["newton"])
+            + nfeval * cost(ph["feval"]) - ncorr * cost(ph["eval"]))
 
 
 def refine_flops(prog0, K, npaths, iters, table=BUILD):

@@ -119,7 +119,7 @@ class TrackBuffers:
         self.nsteps = DeviceArray((npaths,), np.int32)
         self.nrej = DeviceArray((npaths,), np.int32)
         self.dx = DeviceArray((npaths,))
-        self.work = DeviceArray((3, npaths), np.int32)
+        self.work = DeviceArray((4, npaths), np.int32)
 
     def to_host(self):
         return dict(endp=self.endp.to_host(), tend=self.tend.to_host(),

@@ -146,7 +146,7 @@ def oracle_track(prog, starts_planes, K, params, nthreads=1, paths=None, control
     out = dict(endp=np.zeros((2 * K, npaths, n)), tend=np.zeros(npaths),
                status=np.zeros(npaths, np.int32), nsteps=np.zeros(npaths, np.int32),
                nrej=np.zeros(npaths, np.int32), dx=np.zeros(npaths),
-               work=np.zeros((3, npaths), np.int32))
+               work=np.zeros((4, npaths), np.int32))
     p = params.to_c()
     cp = C.byref(controller.struct) if controller is not None else None
     if nthreads > 1:
