@@ -233,11 +233,44 @@ def main():
     return run(args)
 
 
+def train_and_tune(trackers, starts, tune_tr, tune_sols, prm, ev, epochs=25):
+    """Explore-mode logs of the training systems, then two networks, one per
+    label (accepted at the first attempt; accepted within 2 Newton
+    iterations), each with its threshold and pole cap tuned by time on the
+    tuning system; the faster one is kept.  Returns (controller or None,
+    summary)."""
+    from phcpack_amd.controller import collect_training_logs, train_controller, tune_threshold
+
+    def timer(fn):
+        ev.start()
+        fn()
+        return ev.stop_ms()
+
+    logs = collect_training_logs(trackers, starts, prm, K=1, seed=7)
+    best_ctl, best_ms, runs = None, None, []
+    for mn in (None, 2):
+        ctl, info = train_controller(trackers, starts, prm, K=1, seed=7, epochs=epochs,
+                                     max_newton=mn, logs=logs)
+        best, table = tune_threshold(ctl, tune_tr, tune_sols, prm, K=1, timer=timer)
+        ms = table[f"{best[0]}/{best[1]}"]["ms"] if best is not None else None
+        info.update(threshold=best[0] if best else None, pole_cap=best[1] if best else None,
+                    tuned_ms=ms, tune_table={str(k): v for k, v in table.items()})
+        runs.append(info)
+        if ms is not None and (best_ms is None or ms < best_ms):
+            best_ctl, best_ms = ctl, ms
+    chosen = min((r for r in runs if r["tuned_ms"] is not None), key=lambda r: r["tuned_ms"],
+                 default=None)
+    summary = dict(chosen or runs[0])
+    summary["candidates"] = [{k: r[k] for k in ("max_newton", "accept_rate", "loss_last",
+                                                "threshold", "pole_cap", "tuned_ms")}
+                             for r in runs]
+    return best_ctl, summary
+
+
 def train_learned(n, prm, ev):
     """Train the step controller on systems of the same family with other
     seeds (explore-mode logs of FP64 tracking), then tune its threshold on
     a further system.  Not part of the timed region."""
-    from phcpack_amd.controller import train_controller, tune_threshold
     from phcpack_amd.tracker import Tracker
     t0 = time.perf_counter()
     trs, sts = [], []
@@ -245,21 +278,11 @@ def train_learned(n, prm, ev):
         _, hom, sols = build_workload(n, s)
         trs.append(Tracker(hom, deg=8))
         sts.append(sols)
-    ctl, info = train_controller(trs, sts, prm, K=1, seed=7, epochs=25)
     _, hom, sols = build_workload(n, TUNE_SEED)
-    tt = Tracker(hom, deg=8)
-
-    def timer(fn):
-        ev.start()
-        fn()
-        return ev.stop_ms()
-
-    best, table = tune_threshold(ctl, tt, sols, prm, K=1, timer=timer)
+    ctl, info = train_and_tune(trs, sts, Tracker(hom, deg=8), sols, prm, ev)
     info.update(train_seeds=list(TRAIN_SEEDS), tune_seed=TUNE_SEED,
-                threshold=best[0] if best else None, pole_cap=best[1] if best else None,
-                tune_table={str(k): v for k, v in table.items()},
                 seconds=round(time.perf_counter() - t0, 2))
-    return (ctl if best is not None else None), info
+    return ctl, info
 
 
 def verify(tr, out, ref, K_end):
@@ -614,7 +637,6 @@ def run_learned(args, n=8, degree=3, ntrain=10, neval=5, seed=SEED):
     held-out systems; a path the learned controller loses is tracked again
     with the plain rule (its time counts)."""
     from phcpack_amd._lib import DeviceArray, check, load
-    from phcpack_amd.controller import train_controller, tune_threshold
     from phcpack_amd.homotopy import dense_system, total_degree_homotopy
     from phcpack_amd.tracker import TrackBuffers, Tracker, TrackParams, planes_to_points, points_to_planes
     lib = load()
@@ -629,20 +651,11 @@ def run_learned(args, n=8, degree=3, ntrain=10, neval=5, seed=SEED):
 
     t0 = time.perf_counter()
     train = [system(seed + 100 + i) for i in range(ntrain)]
-    ctl, info = train_controller([t for t, _ in train], [s for _, s in train], prm, K=1, seed=7,
-                                 epochs=25)
-
-    def timer(fn):
-        ev.start()
-        fn()
-        return ev.stop_ms()
-
-    best, table = tune_threshold(ctl, train[-1][0], train[-1][1], prm, K=1, timer=timer)
+    ctl, info = train_and_tune([t for t, _ in train], [s for _, s in train], train[-1][0],
+                               train[-1][1], prm, ev)
     info.update(train_paths=int(sum(len(s) for _, s in train)),
-                threshold=best[0] if best else None, pole_cap=best[1] if best else None,
-                tune_table={str(k): v for k, v in table.items()},
                 seconds=round(time.perf_counter() - t0, 2))
-    if best is None:
+    if ctl is None:
         print(json.dumps({"metric": "learned step control", "value": None,
                           "error": "no threshold keeps every path", "training": info}))
         return 1

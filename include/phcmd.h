@@ -161,7 +161,8 @@ int phc_program_validate(const phc_program *program);
  *         are the second attempt's;
  * mode 2: explore: c drawn from a hash of (seed, path, step), and every step
  *         logged: features log_x[path][i][nfeat], candidate log_c[path][i],
- *         log_y[path][i] = 1 if the first attempt was accepted, log_n[path]
+ *         log_y[path][i] = the Newton iterations of the step if its first
+ *         attempt was accepted (>= 1), else 0, log_n[path]
  *         records (at most log_capacity per path).
  * In modes 1 and 2 the multiplied step is capped at pole_cap * r (r the
  * Fabry radius) when pole_cap > 0, so that the step stays inside the disk

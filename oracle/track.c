@@ -496,7 +496,7 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
         }
       h = (r < INFINITY && r > 0.0) ? fmin(prm->pbeta * r, prm->maxstep) : prm->maxstep;
       {
-        int cand = 0, rej0, i;
+        int cand = 0, rej0, nnewt0, i;
         double feat[OTR_NFEAT];
         if (pmode != 0) {
           feat[0] = otr_lg2((r < INFINITY && r > 0.0) ? r : prm->maxstep);
@@ -515,6 +515,7 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
           if (ctl->pole_cap > 0.0 && r < INFINITY && r > 0.0) h = fmin(h, ctl->pole_cap * r);
         }
         rej0 = rej;
+        nnewt0 = nnewt;
       for (j = 0; j < n; j++) ccopy(K, ZP(w.slots, j * D), ZP(w.x0, j));
       while (!accepted) {
         double tn = t + h, hs, res;
@@ -554,7 +555,9 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
           for (i = 0; i < OTR_NFEAT; i++)
             ctl->log_x[((size_t)path * ctl->log_capacity + nlog) * OTR_NFEAT + i] = (float)feat[i];
           ctl->log_c[(size_t)path * ctl->log_capacity + nlog] = cand;
-          ctl->log_y[(size_t)path * ctl->log_capacity + nlog] = (rej == rej0 && st == ST_RUNNING) ? 1 : 0;
+          /* accepted at the first attempt: the Newton iterations it took */
+          ctl->log_y[(size_t)path * ctl->log_capacity + nlog] =
+              (rej == rej0 && st == ST_RUNNING) ? nnewt - nnewt0 : 0;
           ++nlog;
         }
       }

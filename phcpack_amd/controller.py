@@ -11,7 +11,9 @@ the pole-step rule of the reference
 predictor feedback and the Newton corrector stay on as judge and fallback.
 
 Training data are the tracker's own step logs: in explore mode every step
-tries a random candidate and records whether its first attempt was accepted.
+tries a random candidate and records whether its first attempt was accepted
+and how many Newton iterations the corrector then took; the label is
+"accepted" or "accepted within max_newton iterations".
 The network is trained on the GPU by the library's forward/backward kernels
 (phc_mlp_train: tiled FP32 products, ReLU, binary cross-entropy, Adam).
 """
@@ -189,7 +191,9 @@ class StepLogs:
 
 
 def collect_logs(tracker, starts, params, seed=1, cap=256, K=1, controller=None):
-    """track `starts` with random candidates and return (X, cand, accepted)"""
+    """track `starts` with random candidates and return (X, cand, y, outputs):
+    y = the Newton iterations of a step accepted at its first attempt, 0 for
+    a rejected first attempt"""
     from .tracker import TrackBuffers, points_to_planes
     ctl = controller or StepController()
     planes = points_to_planes(starts)
@@ -202,22 +206,40 @@ def collect_logs(tracker, starts, params, seed=1, cap=256, K=1, controller=None)
     return logs.to_host() + (bufs.to_host(),)
 
 
-def train_controller(trackers, starts, params, K=1, seed=1, cap=256, epochs=30, batch=1024,
-                     lr=3e-3, threshold=0.8):
-    """Explore-mode logs from every (tracker, starts) pair, then GPU training.
-
-    Returns the trained StepController and a summary dict (samples,
-    acceptance rate, first and last epoch loss)."""
+def collect_training_logs(trackers, starts, params, K=1, seed=1, cap=256):
+    """explore-mode logs of every (tracker, starts) pair, concatenated:
+    (features, candidates, Newton iterations or 0)"""
     X, C, Y = [], [], []
     for i, (tr, st) in enumerate(zip(trackers, starts)):
         x, c, y, _ = collect_logs(tr, st, params, seed=seed + i, cap=cap, K=K)
         X.append(x)
         C.append(c)
         Y.append(y)
-    X, C, Y = np.concatenate(X), np.concatenate(C), np.concatenate(Y)
+    return np.concatenate(X), np.concatenate(C), np.concatenate(Y)
+
+
+def step_labels(Y, max_newton=None):
+    """1 for a step accepted at its first attempt (within max_newton Newton
+    iterations when given), else 0"""
+    ok = Y >= 1
+    if max_newton is not None:
+        ok &= Y <= max_newton
+    return ok.astype(np.float32)
+
+
+def train_controller(trackers, starts, params, K=1, seed=1, cap=256, epochs=30, batch=1024,
+                     lr=3e-3, threshold=0.8, max_newton=None, logs=None):
+    """Explore-mode logs from every (tracker, starts) pair (or the given
+    logs), then GPU training on step_labels(., max_newton).
+
+    Returns the trained StepController and a summary dict (samples,
+    positive rate, first and last epoch loss)."""
+    X, C, Y = logs if logs is not None else collect_training_logs(trackers, starts, params, K,
+                                                                  seed, cap)
+    y = step_labels(Y, max_newton)
     ctl = StepController(threshold=threshold, seed=seed)
-    losses = ctl.fit(X, C, Y, epochs=epochs, batch=batch, lr=lr, seed=seed)
-    return ctl, dict(samples=int(len(X)), accept_rate=float(Y.mean()),
+    losses = ctl.fit(X, C, y, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    return ctl, dict(samples=int(len(X)), accept_rate=float(y.mean()), max_newton=max_newton,
                      loss_first=float(losses[0]), loss_last=float(losses[-1]))
 
 

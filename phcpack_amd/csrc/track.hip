@@ -791,7 +791,7 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, Params par
           ctl.log_c[(size_t)path * ctl.cap + nlog] = cand;
         }
       }
-      const int rej0 = rej;
+      const int rej0 = rej, nnewt0 = nnewt;
       if (l < n) w.x0[l] = w.slots[l * D];
       wsync();
       // ---- feedback and corrector; halve the step on failure ----
@@ -826,7 +826,9 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, Params par
       }
       rejprev = rej - rej0;
       if (ctl.mode == 2 && nlog < ctl.cap) {
-        if (l == 0) ctl.log_y[(size_t)path * ctl.cap + nlog] = (rej == rej0 && st == ST_RUNNING) ? 1 : 0;
+        // accepted at the first attempt: the Newton iterations it took (>= 1)
+        if (l == 0)
+          ctl.log_y[(size_t)path * ctl.cap + nlog] = (rej == rej0 && st == ST_RUNNING) ? nnewt - nnewt0 : 0;
         ++nlog;
       }
       if (st != ST_RUNNING) break;

@@ -49,7 +49,9 @@ def test_oracle_explore_logs_every_step():
     assert np.array_equal(oc.n, out["work"][0])
     X, c, y = oc.logs()
     assert set(np.unique(c)) <= set(range(len(MULTS)))
-    assert set(np.unique(y)) <= {0, 1}
+    # 0 = first attempt rejected, else the corrector's Newton iterations
+    assert set(np.unique(y)) <= set(range(0, TrackParams().corsteps + 1))
+    assert np.any(y >= 1)
     assert np.all(np.isfinite(X))
 
 
