@@ -85,3 +85,38 @@ def test_reference_signature_layer_exported():
             types.append(re.sub(r"\s+", "", p.replace("long long int", "long long")))
         got = [re.sub(r"\s+", "", t) for t in exported[name].split(",")]
         assert got == types, (name, got, types)
+
+
+def test_invalid_arguments_are_refused_before_device_work():
+    """entry points check their arguments first and return -2 with a
+    message (no GPU is touched here)"""
+    import ctypes
+    lib = _lib.load()
+    null = None
+
+    def refused(rc, words):
+        msg = lib.phc_last_error().decode()
+        return rc == -2 and all(w in msg for w in words)
+
+    assert refused(lib.phc_series_product(3, 0, 4, 1, null, null, null, null), ["K must be"])
+    assert refused(lib.phc_series_product(2, 0, -1, 1, null, null, null, null), ["negative"])
+    assert refused(lib.phc_md_batch(9, 2, 4, null, null, null, null), ["op"])
+    assert refused(lib.phc_md_batch(0, 3, 4, null, null, null, null), ["K must be"])
+    h = ctypes.c_void_p()
+    assert refused(lib.phc_baqr_create(ctypes.byref(h), 3, 64, 64, 16), ["K must be"])
+    assert refused(lib.phc_baqr_create(ctypes.byref(h), 4, 64, 64, 0), ["szt"])
+    assert refused(lib.phc_baqr_create(ctypes.byref(h), 4, 512, 512, 256), ["szt"])
+    assert refused(lib.phc_baqr_create(ctypes.byref(h), 4, 64, 60, 16), ["szt"])
+    assert refused(lib.phc_baqr_create(ctypes.byref(h), 4, 32, 64, 16), ["nrows"])
+    assert refused(lib.phc_cmplx_upper_tiled_solver(4, 65, 16, 4, null, null, null, null),
+                   ["dim must equal"])
+    assert refused(lib.phc_cmplx_gemm(3, 0, 0, 4, 4, 4, null, 0, 1, 1, null, 0, 1, 1, null, 0,
+                                      1, 1, 0, null), ["K must be"])
+    assert refused(lib.phc_refine(null, 2, 1, 1.0, null, 2, null, 1, 1e-20, null, null, null,
+                                  null, 8, null), ["null tracker"])
+    nvr = (ctypes.c_int * 1)(1)
+    idx = (ctypes.c_int * 1)(0)
+    assert refused(lib.phc_poly_create(ctypes.byref(h), 2, 0, 1, 1, 300, nvr, idx, None, None),
+                   ["degree"])
+    mono = (ctypes.c_int * 2)(0, 1)
+    assert refused(lib.phc_rowsys_create(ctypes.byref(h), 65, mono, nvr, mono, idx), ["dim"])
