@@ -217,92 +217,146 @@ template <int M, int K> __device__ __forceinline__ T<K> bfly_r(const T<K> &s) {
   return u;
 }
 
+// Out-of-line multiple-double operations for the Householder kernel, whose
+// serial chain (a sum over the column, two square roots, three divisions)
+// runs on a few lanes of one workgroup: one copy of each operation's code
+// instead of inlined copies at every use (88 KB of straight-line code).
+template <int K> __device__ __noinline__ T<K> nl_add(T<K> a, T<K> b) { return add(a, b); }
+template <int K> __device__ __noinline__ T<K> nl_sub(T<K> a, T<K> b) { return sub(a, b); }
+template <int K> __device__ __noinline__ T<K> nl_mul(T<K> a, T<K> b) { return mul(a, b); }
+template <int K> __device__ __noinline__ T<K> nl_div(T<K> a, T<K> b) { return div(a, b); }
+// md::sqrt<4> (a double-double root, then one Newton correction) with its
+// product, sum and quotient as the calls above
+template <int K> __device__ __noinline__ T<K> nl_sqrt(T<K> a) {
+  if constexpr (K == 4) {
+    const T<K> b = widen_half<4>(md::sqrt(lead_half<4>(a)));
+    return mul_pwr2<4>(nl_div<K>(nl_add<K>(nl_mul<K>(b, b), a), b), 0.5);
+  } else {
+    return R<K>::sqrt(a);
+  }
+}
+// (a + re^2) + im^2, and re^2 + im^2
+template <int K> __device__ __noinline__ T<K> nl_addsq(T<K> a, Z<K> z) {
+  return add(add(a, sqr(z.re)), sqr(z.im));
+}
+template <int K> __device__ __noinline__ T<K> nl_norm2(Z<K> z) {
+  return add(sqr(z.re), sqr(z.im));
+}
+// lane src's value on every lane of the wavefront
+template <int K> __device__ __forceinline__ T<K> lane_r(const T<K> &a, int src) {
+  T<K> r;
+#pragma unroll
+  for (int q = 0; q < K; ++q) r.x[q] = __shfl(a.x[q], src);
+  return r;
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int HW = 1024;  // lanes of the Householder kernel
+
 // Householder vector of column colidx of R (rows colidx..nrows-1), as
 // CPU_cmplx2_factors_house (Matrices/dbl2_factorizations.cpp:441-533):
 // beta goes to B[L], 1/v0 and the sigma == 0 flag to hs[0..2K] (the
 // vector itself, v_i = x_i / v0, is formed by panel_dot_kernel for its
-// rows), zeros to Y row L before position L.  sigma is a butterfly sum per
-// wavefront, then over the four wavefronts; the independent scalar
-// operations run on separate lanes (the two square roots, then beta and the
-// two parts of 1/v0).
+// rows), zeros to Y row L before position L.
+//   wavefronts 1..15: sigma = sum_{i>0} |x_i|^2, one entry per lane (the
+//     first 960 rows), a butterfly per wavefront;
+//   wavefront 0, meanwhile: |x0|^2 and |x0| on lane 0; then the 15 sums
+//     (butterfly), sqrt(|x0|^2 + sigma), mu, v0 = x0 - mu x0 (re on lane 0,
+//     im on lane 1), |v0|^2, and the three quotients beta = |v0|^2 /
+//     (sigma + |v0|^2) (times 2), re and im of conj(v0) / |v0|^2 on lanes
+//     0, 1, 2 in one lane-uniform division.
+// Sums are in tree order (the reference's is sequential): results agree to
+// a few units in the last limb.
 template <int K>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(HW)
 house_kernel(const double *Rm, size_t rps, int ld, int colidx, int nrows, int L, double *Y,
              size_t yps, int rowdim, double *Bv, int szt, double *hs) {
-  __shared__ T<K> part[4];
-  __shared__ T<K> sh[4];
-  __shared__ Z<K> sv0, sinv;
-  __shared__ int szero, rzero;
-  const int t = threadIdx.x, n = nrows - colidx;
+  constexpr int NW = HW / 64;
+  __shared__ T<K> part[NW];
+  const int t = threadIdx.x, w = t >> 6, n = nrows - colidx;
   auto x = [&](int i) { return C<K>::load(Rm, rps, (size_t)(colidx + i) * ld + colidx); };
-  T<K> s = R<K>::zero();
-  for (int i = 1 + t; i < n; i += 256) {
-    const Z<K> xi = x(i);
-    s = add(s, sqr(xi.re));
-    s = add(s, sqr(xi.im));
-  }
-  // butterfly over each wavefront, then the four wavefront sums
-  s = add(s, bfly_r<1, K>(s));
-  s = add(s, bfly_r<2, K>(s));
-  s = add(s, bfly_r<4, K>(s));
-  s = add(s, bfly_r<8, K>(s));
-  s = add(s, bfly_r<16, K>(s));
-  s = add(s, bfly_r<32, K>(s));
-  if ((t & 63) == 0) part[t >> 6] = s;
-  __syncthreads();
-  const T<K> sigma = add(add(part[0], part[1]), add(part[2], part[3]));
-  const Z<K> x0 = x(0);
-  if (t < 2) {  // lane 0: |x0|, lane 1: sqrt(|x0|^2 + sigma)
-    T<K> sq = sqr(x0.re);
-    sq = add(sq, sqr(x0.im));
-    if (t == 1) sq = add(sq, sigma);
-    sh[t] = R<K>::sqrt(sq);
-  }
-  __syncthreads();
-  if (t == 0) {
-    bool zero = true, rz = true;
-    for (int q = 0; q < K; ++q) {
-      zero = zero && (sigma.x[q] == 0.0);
-      rz = rz && (sh[0].x[q] == 0.0);
-    }
-    szero = zero ? 1 : 0;
-    rzero = rz ? 1 : 0;
-    Z<K> v0;
-    if (rz) {
-      v0.re = neg(sh[1]);
-      v0.im = R<K>::zero();
-    } else {
-      const T<K> mu = div(sh[1], sh[0]);
-      v0.re = sub(x0.re, mul(mu, x0.re));
-      v0.im = sub(x0.im, mul(mu, x0.im));
-    }
-    sv0 = v0;
-    T<K> sqrv0 = sqr(v0.re);
-    sh[2] = add(sqrv0, sqr(v0.im));
-  }
-  __syncthreads();
-  if (t < 3 && !szero) {  // lane 0: beta, lanes 1 and 2: 1/v0 = conj(v0)/|v0|^2
-    const T<K> sqrv0 = sh[2];
+  for (int i = t; i < L; i += HW) C<K>::store(Y, yps, (size_t)L * rowdim + i, C<K>::zero());
+  Z<K> x0;
+  T<K> sq0, r0;
+  if (w == 0) {
+    x0 = x(0);
     if (t == 0) {
-      const T<K> beta = div(sqrv0, add(sigma, sqrv0));
-      sh[3] = mul_d(beta, 2.0);
-    } else if (t == 1) {
-      sinv.re = div(sv0.re, sqrv0);
-    } else {
-      sinv.im = neg(div(sv0.im, sqrv0));
+      sq0 = nl_norm2<K>(x0);
+      r0 = nl_sqrt<K>(sq0);
     }
+  } else {
+    T<K> s = R<K>::zero();
+    int i = t - 63;  // rows 1 ..
+    if (i < n) s = nl_norm2<K>(x(i));
+    for (i += HW - 64; i < n; i += HW - 64) s = nl_addsq<K>(s, x(i));
+    s = nl_add<K>(s, bfly_r<1, K>(s));
+    s = nl_add<K>(s, bfly_r<2, K>(s));
+    s = nl_add<K>(s, bfly_r<4, K>(s));
+    s = nl_add<K>(s, bfly_r<8, K>(s));
+    s = nl_add<K>(s, bfly_r<16, K>(s));
+    s = nl_add<K>(s, bfly_r<32, K>(s));
+    if ((t & 63) == 0) part[w - 1] = s;
   }
   __syncthreads();
+  if (w != 0) return;  // the rest is wavefront 0's
+  T<K> u = (t < NW - 1) ? part[t] : R<K>::zero();
+  u = nl_add<K>(u, bfly_r<1, K>(u));
+  u = nl_add<K>(u, bfly_r<2, K>(u));
+  u = nl_add<K>(u, bfly_r<4, K>(u));
+  u = nl_add<K>(u, bfly_r<8, K>(u));
+  const T<K> sigma = lane_r<K>(u, 0);
+  bool szero = true;
+#pragma unroll
+  for (int q = 0; q < K; ++q) szero = szero && (sigma.x[q] == 0.0);
+  T<K> mu;
+  bool rz = true;
   if (t == 0) {
-    R<K>::store(Bv, szt, L, szero ? R<K>::zero() : sh[3]);
-    const Z<K> inv = szero ? C<K>::zero() : sinv;
-    for (int q = 0; q < K; ++q) {
-      hs[q] = inv.re.x[q];
-      hs[K + q] = inv.im.x[q];
-    }
-    hs[2 * K] = szero ? 1.0 : 0.0;
+#pragma unroll
+    for (int q = 0; q < K; ++q) rz = rz && (r0.x[q] == 0.0);
+    const T<K> nrm = nl_sqrt<K>(nl_add<K>(sq0, sigma));
+    mu = rz ? nrm : nl_div<K>(nrm, r0);
   }
-  for (int i = t; i < L; i += 256) C<K>::store(Y, yps, (size_t)L * rowdim + i, C<K>::zero());
+  mu = lane_r<K>(mu, 0);
+  rz = __shfl((int)rz, 0) != 0;
+  // lane 0: v0.re, lane 1: v0.im
+  T<K> vc;
+  if (t < 2) {
+    const T<K> xc = (t == 0) ? x0.re : x0.im;
+    if (rz)
+      vc = (t == 0) ? neg(mu) : R<K>::zero();
+    else
+      vc = nl_sub<K>(xc, nl_mul<K>(mu, xc));
+  }
+  T<K> vsq;
+  if (t < 2) vsq = nl_mul<K>(vc, vc);
+  const T<K> v0re = lane_r<K>(vc, 0), v0im = lane_r<K>(vc, 1);
+  const T<K> vsqim = lane_r<K>(vsq, 1);
+  T<K> sqrv0, den;
+  if (t == 0) {
+    sqrv0 = nl_add<K>(vsq, vsqim);
+    den = nl_add<K>(sigma, sqrv0);
+  }
+  sqrv0 = lane_r<K>(sqrv0, 0);
+  den = lane_r<K>(den, 0);
+  if (t < 3) {
+    const T<K> num = (t == 0) ? sqrv0 : (t == 1) ? v0re : v0im;
+    T<K> q = nl_div<K>(num, (t == 0) ? den : sqrv0);
+    if (t == 0) q = mul_d(q, 2.0);
+    if (t == 2) q = neg(q);
+    if (szero) q = R<K>::zero();
+    if (t == 0) {
+      R<K>::store(Bv, szt, L, q);
+      hs[2 * K] = szero ? 1.0 : 0.0;
+    } else {
+#pragma unroll
+      for (int p = 0; p < K; ++p) hs[(t - 1) * K + p] = q.x[p];
+    }
+  }
 }
 
 constexpr int RCH = 8;  // rows per chunk of the panel dot products
@@ -538,7 +592,7 @@ template <int K> int run_qr(phc_baqr *p, const double *A, double *Q, double *R, 
     for (int L = 0; L < szt; ++L) {
       const int colidx = rowoff + L, nr = m - colidx;
       const int nchunk = (nr + RCH - 1) / RCH, ncol = endcol - colidx;
-      hipLaunchKernelGGL(house_kernel<K>, dim3(1), dim3(256), 0, st, R, rps, n, colidx, m, L,
+      hipLaunchKernelGGL(house_kernel<K>, dim3(1), dim3(HW), 0, st, R, rps, n, colidx, m, L,
                          p->Y, p->yps, rowdim, p->B, szt, p->hs);
       hipLaunchKernelGGL(panel_dot_kernel<K>, dim3(nchunk), dim3(512), 0, st, R, rps, n, colidx,
                          endcol, m, p->Y, p->yps, rowdim, L, p->wp, p->wps, szt, p->hs);
