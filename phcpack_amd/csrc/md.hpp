@@ -175,9 +175,28 @@ template <int K> MD_HD void set_limb(double *r, int p, double v) {
 }
 
 // renorm with a running pointer over K+1 terms into K limbs
-// (qdf_renorm4, odf_renorm8).  The limb the pointer rereads is always the
-// one written last, so it is kept in `last` instead of being read back.
+// (qdf_renorm4, odf_renorm8).
+//   Common case first: f[1] != 0 and no partial sum pr is zero before the
+// last step.  Then the pointer advances at every step and the limbs are
+// f[0] and the K-1 quick_two_sum results in order (a zero pr after the last
+// step rewrites the last limb with the value it has, or with +0 where that
+// value is a zero, which is +0 already since pr != 0 before that step), so
+// no selects are needed.  Any other lane takes the general sweep below,
+// which the wavefront skips when no lane needs it.
+//   General sweep: the limb the pointer rereads is always the one written
+// last, so it is kept in `last` instead of being read back.
 template <int K> MD_HD void renorm(const double *f, double *r) {
+  {
+    double p = f[1];
+    bool plain = f[1] != 0.0;
+    r[0] = f[0];
+#pragma unroll
+    for (int i = 2; i <= K; ++i) {
+      r[i - 1] = quick_two_sum(p, f[i], p);
+      if (i < K) plain = plain && (p != 0.0);
+    }
+    if (__builtin_expect(plain, 1)) return;
+  }
   double pr, last;
   int ptr;
 #pragma unroll
