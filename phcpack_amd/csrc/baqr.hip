@@ -63,7 +63,14 @@ __device__ __forceinline__ void cmac(Z<K> &acc, const Z<K> &a, const Z<K> &b) {
   acc.im = CB ? sub(acc.im, t4) : add(acc.im, t4);
 }
 
-constexpr int TM = 32, TN = 32, TKD = 8;
+constexpr int TKD = 8;  // k depth of the LDS slices
+
+// register block (outputs per lane along i and j) of the GEMM for K limbs:
+// the 16 x 16 lanes of a workgroup cover a (16 RA) x (16 RB) tile
+template <int K> struct GemmShape {
+  static constexpr int RA = (K >= 4) ? 1 : 2, RB = (K >= 4) ? 1 : 2;
+  static constexpr int TM = 16 * RA, TN = 16 * RB;
+};
 
 // C(i, j) = [C(i, j) +] sum_k op(A(i, k)) op(B(k, j)), i < M, j < N, k < KD;
 // the sum starts from zero and is added to C at the end.  With a split of
@@ -73,55 +80,59 @@ template <int K, bool CA, bool CB>
 __global__ void __launch_bounds__(256)
 cgemm_kernel(int M, int N, int KD, int kchunk, Op A, Op B, double *Cp, size_t cps, long c0,
              long c1, int accumulate, double *part, size_t pps) {
+  constexpr int RA = GemmShape<K>::RA, RB = GemmShape<K>::RB;
+  constexpr int TM = GemmShape<K>::TM, TN = GemmShape<K>::TN;
   __shared__ double As[2 * K][TKD][TM];
   __shared__ double Bs[2 * K][TKD][TN];
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
   const int i0 = blockIdx.y * TM, j0 = blockIdx.x * TN;
   const int kb = blockIdx.z * kchunk;
   const int ke = (kb + kchunk) < KD ? (kb + kchunk) : KD;
-  Z<K> acc[2][2];
+  Z<K> acc[RA][RB];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < RA; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = C<K>::zero();
+    for (int b = 0; b < RB; ++b) acc[a][b] = C<K>::zero();
   for (int k0 = kb; k0 < ke; k0 += TKD) {
-    {
+#pragma unroll
+    for (int e = t; e < TM * TKD; e += 256) {
       int ii, kk;
-      if (A.s0 == 1) { ii = t % TM; kk = t / TM; } else { kk = t % TKD; ii = t / TKD; }
+      if (A.s0 == 1) { ii = e % TM; kk = e / TM; } else { kk = e % TKD; ii = e / TKD; }
       const int gi = i0 + ii, gk = k0 + kk;
       const bool in = gi < M && gk < ke;
-      const size_t e = in ? (size_t)(A.s0 * gi + A.s1 * gk) : 0;
+      const size_t o = in ? (size_t)(A.s0 * gi + A.s1 * gk) : 0;
 #pragma unroll
-      for (int q = 0; q < 2 * K; ++q) As[q][kk][ii] = in ? A.p[q * A.ps + e] : 0.0;
+      for (int q = 0; q < 2 * K; ++q) As[q][kk][ii] = in ? A.p[q * A.ps + o] : 0.0;
     }
-    {
+#pragma unroll
+    for (int e = t; e < TN * TKD; e += 256) {
       int jj, kk;
-      if (B.s1 == 1) { jj = t % TN; kk = t / TN; } else { kk = t % TKD; jj = t / TKD; }
+      if (B.s1 == 1) { jj = e % TN; kk = e / TN; } else { kk = e % TKD; jj = e / TKD; }
       const int gj = j0 + jj, gk = k0 + kk;
       const bool in = gj < N && gk < ke;
-      const size_t e = in ? (size_t)(B.s0 * gk + B.s1 * gj) : 0;
+      const size_t o = in ? (size_t)(B.s0 * gk + B.s1 * gj) : 0;
 #pragma unroll
-      for (int q = 0; q < 2 * K; ++q) Bs[q][kk][jj] = in ? B.p[q * B.ps + e] : 0.0;
+      for (int q = 0; q < 2 * K; ++q) Bs[q][kk][jj] = in ? B.p[q * B.ps + o] : 0.0;
     }
     __syncthreads();
     const int kmax = (ke - k0) < TKD ? (ke - k0) : TKD;
     for (int kk = 0; kk < kmax; ++kk) {
-      Z<K> av[2], bv[2];
+      Z<K> av[RA], bv[RB];
 #pragma unroll
-      for (int a = 0; a < 2; ++a) av[a] = C<K>::load(&As[0][0][0], TKD * TM, kk * TM + ty + 16 * a);
+      for (int a = 0; a < RA; ++a) av[a] = C<K>::load(&As[0][0][0], TKD * TM, kk * TM + ty + 16 * a);
 #pragma unroll
-      for (int b = 0; b < 2; ++b) bv[b] = C<K>::load(&Bs[0][0][0], TKD * TN, kk * TN + tx + 16 * b);
+      for (int b = 0; b < RB; ++b) bv[b] = C<K>::load(&Bs[0][0][0], TKD * TN, kk * TN + tx + 16 * b);
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < RA; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) cmac<K, CA, CB>(acc[a][b], av[a], bv[b]);
+        for (int b = 0; b < RB; ++b) cmac<K, CA, CB>(acc[a][b], av[a], bv[b]);
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < RA; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < RB; ++b) {
       const int i = i0 + ty + 16 * a, j = j0 + tx + 16 * b;
       if (i < M && j < N) {
         if (gridDim.z > 1) {
@@ -154,7 +165,8 @@ __global__ void csum_kernel(int M, int N, int S, const double *part, size_t pps,
 
 // split of the k range: enough workgroups to fill the chip, slices of at
 // least 64 terms, and M*N*S partial sums within the workspace
-inline int ksplit(int M, int N, int KD, size_t cap) {
+template <int K> int ksplit(int M, int N, int KD, size_t cap) {
+  constexpr int TM = GemmShape<K>::TM, TN = GemmShape<K>::TN;
   const long tiles = (long)((M + TM - 1) / TM) * ((N + TN - 1) / TN);
   int S = 1;
   while (tiles * S < 1024 && KD / (2 * S) >= 64 && (size_t)M * N * 2 * S <= cap) S *= 2;
@@ -165,7 +177,8 @@ template <int K>
 int cgemm(bool ca, bool cb, int M, int N, int KD, Op A, Op B, double *Cp, size_t cps, long c0,
           long c1, bool accumulate, hipStream_t st, double *part = nullptr, size_t cap = 0) {
   if (M <= 0 || N <= 0) return 0;
-  const int S = part ? ksplit(M, N, KD, cap) : 1;
+  constexpr int TM = GemmShape<K>::TM, TN = GemmShape<K>::TN;
+  const int S = part ? ksplit<K>(M, N, KD, cap) : 1;
   const int kchunk = ((KD + S - 1) / S + TKD - 1) / TKD * TKD;
   dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, S);
   const int acc = accumulate ? 1 : 0;
