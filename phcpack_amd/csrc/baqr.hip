@@ -5,12 +5,13 @@
 //   reference (Matrices/)                       here
 //   CPU_cmplx4_blocked_houseqr (dbl4_baqr_host.cpp:1206),
 //   GPU_cmplx4_blocked_houseqr (dbl4_baqr_kernels.cu:6197)
-//     per column: house, panel leftRupdate      house_kernel, panel_dot_kernel,
-//                                               panel_reduce_kernel,
-//                                               panel_update_kernel (many workgroups
-//                                               per column, partial sums reduced in
-//                                               a fixed order); the launches of one
-//                                               factorization replay as a HIP graph
+//     per column: house, panel leftRupdate      house_kernel (forms v as well) and
+//                                               column_update_kernel (one workgroup
+//                                               per panel column: dot product,
+//                                               reduction, update), on a column-
+//                                               major copy of the panel; the launches
+//                                               of one factorization replay as a HIP
+//                                               graph
 //     VB_to_W (dbl4_baqr_host.cpp:91)           W = Y S with S upper triangular from
 //                                               the Gram matrix G = Y^H Y: one
 //                                               wavefront per row of S
@@ -271,31 +272,36 @@ __device__ __forceinline__ void wave_sync() {
 
 constexpr int HW = 1024;  // lanes of the Householder kernel
 
-// Householder vector of column colidx of R (rows colidx..nrows-1), as
-// CPU_cmplx2_factors_house (Matrices/dbl2_factorizations.cpp:441-533):
-// beta goes to B[L], 1/v0 and the sigma == 0 flag to hs[0..2K] (the
-// vector itself, v_i = x_i / v0, is formed by panel_dot_kernel for its
-// rows), zeros to Y row L before position L.
+// Householder vector of column colidx (rows colidx..nrows-1), as
+// CPU_cmplx2_factors_house (Matrices/dbl2_factorizations.cpp:441-533), on
+// the column-major copy Pc of the panel (column c at Pc + c*m): beta goes to
+// B[L], v (v_0 = 1, v_i = x_i / v0, or x_i when sigma = 0) to Y row L from
+// position L, zeros to Y row L before position L.
 //   wavefronts 1..15: sigma = sum_{i>0} |x_i|^2, one entry per lane (the
 //     first 960 rows), a butterfly per wavefront;
 //   wavefront 0, meanwhile: |x0|^2 and |x0| on lane 0; then the 15 sums
 //     (butterfly), sqrt(|x0|^2 + sigma), mu, v0 = x0 - mu x0 (re on lane 0,
 //     im on lane 1), |v0|^2, and the three quotients beta = |v0|^2 /
 //     (sigma + |v0|^2) (times 2), re and im of conj(v0) / |v0|^2 on lanes
-//     0, 1, 2 in one lane-uniform division.
+//     0, 1, 2 in one lane-uniform division;
+//   every lane: v_i = x_i * (1/v0) for its rows.
 // Sums are in tree order (the reference's is sequential): results agree to
 // a few units in the last limb.
 template <int K>
 __global__ void __launch_bounds__(HW)
-house_kernel(const double *Rm, size_t rps, int ld, int colidx, int nrows, int L, double *Y,
-             size_t yps, int rowdim, double *Bv, int szt, double *hs) {
+house_kernel(const double *Pc, size_t pps, int m, int rowoff, int colidx, int nrows, int L,
+             double *Y, size_t yps, int rowdim, double *Bv, int szt) {
   constexpr int NW = HW / 64;
   __shared__ T<K> part[NW];
+  __shared__ Z<K> sinv;
+  __shared__ int sflag;
   const int t = threadIdx.x, w = t >> 6, n = nrows - colidx;
-  auto x = [&](int i) { return C<K>::load(Rm, rps, (size_t)(colidx + i) * ld + colidx); };
+  const double *col = Pc + (size_t)(colidx - rowoff) * m + colidx;  // x_i at col[i]
+  auto x = [&](int i) { return C<K>::load(col, pps, i); };
   for (int i = t; i < L; i += HW) C<K>::store(Y, yps, (size_t)L * rowdim + i, C<K>::zero());
-  Z<K> x0;
+  Z<K> x0, xr;
   T<K> sq0, r0;
+  const int ir = t - 63;  // the row of lane t >= 64 in the first pass
   if (w == 0) {
     x0 = x(0);
     if (t == 0) {
@@ -304,9 +310,11 @@ house_kernel(const double *Rm, size_t rps, int ld, int colidx, int nrows, int L,
     }
   } else {
     T<K> s = R<K>::zero();
-    int i = t - 63;  // rows 1 ..
-    if (i < n) s = nl_norm2<K>(x(i));
-    for (i += HW - 64; i < n; i += HW - 64) s = nl_addsq<K>(s, x(i));
+    if (ir < n) {
+      xr = x(ir);
+      s = nl_norm2<K>(xr);
+    }
+    for (int i = ir + HW - 64; i < n; i += HW - 64) s = nl_addsq<K>(s, x(i));
     s = nl_add<K>(s, bfly_r<1, K>(s));
     s = nl_add<K>(s, bfly_r<2, K>(s));
     s = nl_add<K>(s, bfly_r<4, K>(s));
@@ -316,160 +324,148 @@ house_kernel(const double *Rm, size_t rps, int ld, int colidx, int nrows, int L,
     if ((t & 63) == 0) part[w - 1] = s;
   }
   __syncthreads();
-  if (w != 0) return;  // the rest is wavefront 0's
-  T<K> u = (t < NW - 1) ? part[t] : R<K>::zero();
-  u = nl_add<K>(u, bfly_r<1, K>(u));
-  u = nl_add<K>(u, bfly_r<2, K>(u));
-  u = nl_add<K>(u, bfly_r<4, K>(u));
-  u = nl_add<K>(u, bfly_r<8, K>(u));
-  const T<K> sigma = lane_r<K>(u, 0);
-  bool szero = true;
+  if (w == 0) {
+    T<K> u = (t < NW - 1) ? part[t] : R<K>::zero();
+    u = nl_add<K>(u, bfly_r<1, K>(u));
+    u = nl_add<K>(u, bfly_r<2, K>(u));
+    u = nl_add<K>(u, bfly_r<4, K>(u));
+    u = nl_add<K>(u, bfly_r<8, K>(u));
+    const T<K> sigma = lane_r<K>(u, 0);
+    bool szero = true;
 #pragma unroll
-  for (int q = 0; q < K; ++q) szero = szero && (sigma.x[q] == 0.0);
-  T<K> mu;
-  bool rz = true;
-  if (t == 0) {
-#pragma unroll
-    for (int q = 0; q < K; ++q) rz = rz && (r0.x[q] == 0.0);
-    const T<K> nrm = nl_sqrt<K>(nl_add<K>(sq0, sigma));
-    mu = rz ? nrm : nl_div<K>(nrm, r0);
-  }
-  mu = lane_r<K>(mu, 0);
-  rz = __shfl((int)rz, 0) != 0;
-  // lane 0: v0.re, lane 1: v0.im
-  T<K> vc;
-  if (t < 2) {
-    const T<K> xc = (t == 0) ? x0.re : x0.im;
-    if (rz)
-      vc = (t == 0) ? neg(mu) : R<K>::zero();
-    else
-      vc = nl_sub<K>(xc, nl_mul<K>(mu, xc));
-  }
-  T<K> vsq;
-  if (t < 2) vsq = nl_mul<K>(vc, vc);
-  const T<K> v0re = lane_r<K>(vc, 0), v0im = lane_r<K>(vc, 1);
-  const T<K> vsqim = lane_r<K>(vsq, 1);
-  T<K> sqrv0, den;
-  if (t == 0) {
-    sqrv0 = nl_add<K>(vsq, vsqim);
-    den = nl_add<K>(sigma, sqrv0);
-  }
-  sqrv0 = lane_r<K>(sqrv0, 0);
-  den = lane_r<K>(den, 0);
-  if (t < 3) {
-    const T<K> num = (t == 0) ? sqrv0 : (t == 1) ? v0re : v0im;
-    T<K> q = nl_div<K>(num, (t == 0) ? den : sqrv0);
-    if (t == 0) q = mul_d(q, 2.0);
-    if (t == 2) q = neg(q);
-    if (szero) q = R<K>::zero();
+    for (int q = 0; q < K; ++q) szero = szero && (sigma.x[q] == 0.0);
+    T<K> mu;
+    bool rz = true;
     if (t == 0) {
-      R<K>::store(Bv, szt, L, q);
-      hs[2 * K] = szero ? 1.0 : 0.0;
-    } else {
 #pragma unroll
-      for (int p = 0; p < K; ++p) hs[(t - 1) * K + p] = q.x[p];
+      for (int q = 0; q < K; ++q) rz = rz && (r0.x[q] == 0.0);
+      const T<K> nrm = nl_sqrt<K>(nl_add<K>(sq0, sigma));
+      mu = rz ? nrm : nl_div<K>(nrm, r0);
     }
-  }
-}
-
-constexpr int RCH = 8;  // rows per chunk of the panel dot products
-
-// the Householder vector on the rows of this chunk, v_i = x_i / v0 (x_i
-// itself when sigma = 0, 1 on the diagonal), written to Y row L, then the
-// partial sums wp[chunk][j] = sum_{i in chunk} conj(R_ij) v_i over columns
-// j = colidx..endcol-1: lane (h, c) = (t / 128, t % 128) takes rows h and
-// h + 4 of the chunk for column c, the two row halves meet in LDS
-template <int K>
-__global__ void __launch_bounds__(512)
-panel_dot_kernel(const double *Rm, size_t rps, int ld, int colidx, int endcol, int nrows,
-                 double *Y, size_t yps, int rowdim, int L, double *wp, size_t wps,
-                 int szt, const double *hs) {
-  __shared__ double vs[2 * K][RCH];
-  __shared__ Z<K> part[4][128];
-  const int chunk = blockIdx.x, t = threadIdx.x, c = t & 127, h = t >> 7;
-  const int r0 = colidx + chunk * RCH, n = nrows - colidx;
-  const int rows = (n - chunk * RCH) < RCH ? (n - chunk * RCH) : RCH;
-  if (t < rows) {
-    const int i = chunk * RCH + t;  // row colidx + i
-    Z<K> v;
-    if (i == 0) {
-      v = C<K>::from(1.0, 0.0);
-    } else {
-      const Z<K> xi = C<K>::load(Rm, rps, (size_t)(colidx + i) * ld + colidx);
-      if (hs[2 * K] != 0.0) {
-        v = xi;
+    mu = lane_r<K>(mu, 0);
+    rz = __shfl((int)rz, 0) != 0;
+    // lane 0: v0.re, lane 1: v0.im
+    T<K> vc;
+    if (t < 2) {
+      const T<K> xc = (t == 0) ? x0.re : x0.im;
+      if (rz)
+        vc = (t == 0) ? neg(mu) : R<K>::zero();
+      else
+        vc = nl_sub<K>(xc, nl_mul<K>(mu, xc));
+    }
+    T<K> vsq;
+    if (t < 2) vsq = nl_mul<K>(vc, vc);
+    const T<K> v0re = lane_r<K>(vc, 0), v0im = lane_r<K>(vc, 1);
+    const T<K> vsqim = lane_r<K>(vsq, 1);
+    T<K> sqrv0, den;
+    if (t == 0) {
+      sqrv0 = nl_add<K>(vsq, vsqim);
+      den = nl_add<K>(sigma, sqrv0);
+    }
+    sqrv0 = lane_r<K>(sqrv0, 0);
+    den = lane_r<K>(den, 0);
+    if (t < 3) {
+      const T<K> num = (t == 0) ? sqrv0 : (t == 1) ? v0re : v0im;
+      T<K> q = nl_div<K>(num, (t == 0) ? den : sqrv0);
+      if (t == 0) q = mul_d(q, 2.0);
+      if (t == 2) q = neg(q);
+      if (szero) q = R<K>::zero();
+      if (t == 0) {
+        R<K>::store(Bv, szt, L, q);
+        sflag = szero ? 1 : 0;
+      } else if (t == 1) {
+        sinv.re = q;
       } else {
-        Z<K> inv;
-        for (int q = 0; q < K; ++q) {
-          inv.re.x[q] = hs[q];
-          inv.im.x[q] = hs[K + q];
-        }
-        v.re = sub(mul(xi.re, inv.re), mul(xi.im, inv.im));
-        v.im = add(mul(xi.im, inv.re), mul(xi.re, inv.im));
+        sinv.im = q;
       }
     }
-    C<K>::store(Y, yps, (size_t)L * rowdim + L + i, v);
-    for (int q = 0; q < K; ++q) { vs[q][t] = v.re.x[q]; vs[K + q][t] = v.im.x[q]; }
   }
   __syncthreads();
-  const int j = colidx + c;
-  Z<K> s = C<K>::zero();
-  if (j < endcol) {
-    for (int i = h; i < rows; i += 4) {
-      const Z<K> r = C<K>::load(Rm, rps, (size_t)(r0 + i) * ld + j);
-      const Z<K> v = C<K>::load(&vs[0][0], RCH, i);
-      cmac<K, true, false>(s, r, v);
+  const Z<K> inv = sinv;
+  const bool flag = sflag != 0;
+  auto vof = [&](const Z<K> &xi) {
+    if (flag) return xi;
+    Z<K> v;
+    v.re = sub(mul(xi.re, inv.re), mul(xi.im, inv.im));
+    v.im = add(mul(xi.im, inv.re), mul(xi.re, inv.im));
+    return v;
+  };
+  double *yl = Y + (size_t)L * rowdim + L;  // v_i at yl[i]
+  if (t == 0) C<K>::store(yl, yps, 0, C<K>::from(1.0, 0.0));
+  if (w != 0) {
+    if (ir < n) C<K>::store(yl, yps, ir, vof(xr));
+    for (int i = ir + HW - 64; i < n; i += HW - 64) C<K>::store(yl, yps, i, vof(x(i)));
+  }
+}
+
+// One column j = colidx + blockIdx.x of the panel per workgroup, on the
+// column-major copy Pc (the panel part of the reference's leftRupdate,
+// Matrices/dbl4_baqr_host.cpp:357, one column at a time):
+//   w_j = beta_L sum_{i >= colidx} conj(R_ij) v_i, one row per lane, a
+//   butterfly per wavefront and one over the 16 wavefront sums;
+//   R_ij -= v_i conj(w_j) for the same rows.
+template <int K>
+__global__ void __launch_bounds__(HW)
+column_update_kernel(double *Pc, size_t pps, int m, int rowoff, int colidx, int nrows,
+                     const double *Y, size_t yps, int rowdim, int L, const double *Bv, int szt) {
+  constexpr int NW = HW / 64;
+  __shared__ Z<K> part[NW];
+  __shared__ Z<K> wsh;
+  const int t = threadIdx.x, w = t >> 6, n = nrows - colidx;
+  double *col = Pc + (size_t)(colidx + blockIdx.x - rowoff) * m + colidx;  // R_ij at col[i]
+  const double *yl = Y + (size_t)L * rowdim + L;                           // v_i at yl[i]
+  Z<K> s = C<K>::zero(), xt, vt;
+  if (t < n) {
+    xt = C<K>::load(col, pps, t);
+    vt = C<K>::load(yl, yps, t);
+    cmac<K, true, false>(s, xt, vt);
+  }
+  for (int i = t + HW; i < n; i += HW)
+    cmac<K, true, false>(s, C<K>::load(col, pps, i), C<K>::load(yl, yps, i));
+  s = wave_csum<K>(s);
+  if ((t & 63) == 0) part[w] = s;
+  __syncthreads();
+  if (w == 0) {
+    Z<K> u = (t < NW) ? part[t] : C<K>::zero();
+    u = C<K>::add(u, bfly_cz<1, K>(u));
+    u = C<K>::add(u, bfly_cz<2, K>(u));
+    u = C<K>::add(u, bfly_cz<4, K>(u));
+    u = C<K>::add(u, bfly_cz<8, K>(u));
+    if (t == 0) {
+      const T<K> beta = R<K>::load(Bv, szt, L);
+      Z<K> wj;
+      wj.re = mul(beta, u.re);
+      wj.im = mul(beta, u.im);
+      wsh = wj;
     }
   }
-  part[h][c] = s;
   __syncthreads();
-  if (h < 2) part[h][c] = C<K>::add(part[h][c], part[h + 2][c]);
-  __syncthreads();
-  if (h == 0 && j < endcol) C<K>::store(wp, wps, (size_t)chunk * szt + c, C<K>::add(part[0][c], part[1][c]));
-}
-
-// w_j = beta * sum_chunks wp[.][j], one workgroup per column: each lane
-// loads at most a few chunks, a butterfly per wavefront, then the four
-// wavefront sums
-template <int K>
-__global__ void __launch_bounds__(256)
-panel_reduce_kernel(const double *wp, size_t wps, int szt, int ncol, int nchunk,
-                    const double *Bv, int L, double *w) {
-  __shared__ Z<K> part[4];
-  const int c = blockIdx.x, t = threadIdx.x;
-  Z<K> s = C<K>::zero();
-  for (int k = t; k < nchunk; k += 256) s = C<K>::add(s, C<K>::load(wp, wps, (size_t)k * szt + c));
-  s = wave_csum<K>(s);
-  if ((t & 63) == 0) part[t >> 6] = s;
-  __syncthreads();
-  if (t == 0) {
-    Z<K> v = C<K>::add(C<K>::add(part[0], part[1]), C<K>::add(part[2], part[3]));
-    const T<K> beta = R<K>::load(Bv, szt, L);
-    v.re = mul(beta, v.re);
-    v.im = mul(beta, v.im);
-    C<K>::store(w, szt, c, v);
-  }
-}
-
-// R_ij -= v_i conj(w_j) for the rows of this workgroup (one lane per column)
-template <int K>
-__global__ void __launch_bounds__(128)
-panel_update_kernel(double *Rm, size_t rps, int ld, int colidx, int endcol, int nrows,
-                    const double *Y, size_t yps, int rowdim, int L, const double *w, int szt,
-                    int rows_per_block) {
-  const int t = threadIdx.x, j = colidx + t;
-  if (j >= endcol) return;
-  const Z<K> wj = C<K>::load(w, szt, t);
-  const int ib = colidx + blockIdx.x * rows_per_block;
-  const int ie = (ib + rows_per_block) < nrows ? ib + rows_per_block : nrows;
-  for (int i = ib; i < ie; ++i) {
-    const Z<K> v = C<K>::load(Y, yps, (size_t)L * rowdim + L + (i - colidx));
-    const size_t e = (size_t)i * ld + j;
-    Z<K> r = C<K>::load(Rm, rps, e);
-    // r -= v * conj(w)
+  const Z<K> wj = wsh;
+  // r -= v * conj(w)
+  auto upd = [&](Z<K> r, const Z<K> &v) {
     r.re = sub(r.re, add(mul(v.re, wj.re), mul(v.im, wj.im)));
     r.im = sub(r.im, sub(mul(v.im, wj.re), mul(v.re, wj.im)));
-    C<K>::store(Rm, rps, e, r);
+    return r;
+  };
+  if (t < n) C<K>::store(col, pps, t, upd(xt, vt));
+  for (int i = t + HW; i < n; i += HW)
+    C<K>::store(col, pps, i, upd(C<K>::load(col, pps, i), C<K>::load(yl, yps, i)));
+}
+
+// the panel columns rowoff..rowoff+szt-1 of R (rows rowoff..m-1) to and from
+// the column-major copy Pc
+template <int K>
+__global__ void panel_copy_kernel(double *Rm, size_t rps, int ld, int m, int rowoff, int szt,
+                                  double *Pc, size_t pps, int to_panel) {
+  const size_t rows = (size_t)(m - rowoff), tot = rows * szt;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e / rows), i = rowoff + (int)(e % rows);
+    const size_t re = (size_t)i * ld + rowoff + c, pe = (size_t)c * m + i;
+    if (to_panel)
+      C<K>::store(Pc, pps, pe, C<K>::load(Rm, rps, re));
+    else
+      C<K>::store(Rm, rps, re, C<K>::load(Pc, pps, pe));
   }
 }
 
@@ -582,9 +578,9 @@ __global__ void init_qr_kernel(int nrows, int ncols, const double *A, double *Q,
 }  // namespace
 
 struct phc_baqr {
-  int K, nrows, ncols, szt, nbt, nchunk;
-  double *Y, *W, *G, *S, *T1, *T2, *wp, *w, *B, *part, *hs;
-  size_t yps, gps, t1ps, t2ps, wps, pcap;
+  int K, nrows, ncols, szt, nbt;
+  double *Y, *W, *G, *S, *T1, *T2, *Pc, *B, *part;
+  size_t yps, gps, t1ps, t2ps, pps, pcap;
   // the launch sequence of one factorization, captured for the last (A, Q, R)
   hipStream_t cap;
   hipGraphExec_t exec;
@@ -602,19 +598,17 @@ template <int K> int run_qr(phc_baqr *p, const double *A, double *Q, double *R, 
   PHC_HIP(hipGetLastError());
   for (int k = 0; k < p->nbt; ++k) {
     const int rowoff = k * szt, rowdim = m - rowoff, endcol = (k + 1) * szt;
+    hipLaunchKernelGGL(panel_copy_kernel<K>, dim3(phc::grid_for((size_t)rowdim * szt, 256)),
+                       dim3(256), 0, st, R, rps, n, m, rowoff, szt, p->Pc, p->pps, 1);
     for (int L = 0; L < szt; ++L) {
-      const int colidx = rowoff + L, nr = m - colidx;
-      const int nchunk = (nr + RCH - 1) / RCH, ncol = endcol - colidx;
-      hipLaunchKernelGGL(house_kernel<K>, dim3(1), dim3(HW), 0, st, R, rps, n, colidx, m, L,
-                         p->Y, p->yps, rowdim, p->B, szt, p->hs);
-      hipLaunchKernelGGL(panel_dot_kernel<K>, dim3(nchunk), dim3(512), 0, st, R, rps, n, colidx,
-                         endcol, m, p->Y, p->yps, rowdim, L, p->wp, p->wps, szt, p->hs);
-      hipLaunchKernelGGL(panel_reduce_kernel<K>, dim3(ncol), dim3(256), 0, st, p->wp, p->wps, szt,
-                         ncol, nchunk, p->B, L, p->w);
-      const int rpb = 1, nblk = (nr + rpb - 1) / rpb;
-      hipLaunchKernelGGL(panel_update_kernel<K>, dim3(nblk), dim3(128), 0, st, R, rps, n, colidx,
-                         endcol, m, p->Y, p->yps, rowdim, L, p->w, szt, rpb);
+      const int colidx = rowoff + L, ncol = endcol - colidx;
+      hipLaunchKernelGGL(house_kernel<K>, dim3(1), dim3(HW), 0, st, p->Pc, p->pps, m, rowoff,
+                         colidx, m, L, p->Y, p->yps, rowdim, p->B, szt);
+      hipLaunchKernelGGL(column_update_kernel<K>, dim3(ncol), dim3(HW), 0, st, p->Pc, p->pps, m,
+                         rowoff, colidx, m, p->Y, p->yps, rowdim, L, p->B, szt);
     }
+    hipLaunchKernelGGL(panel_copy_kernel<K>, dim3(phc::grid_for((size_t)rowdim * szt, 256)),
+                       dim3(256), 0, st, R, rps, n, m, rowoff, szt, p->Pc, p->pps, 0);
     PHC_HIP(hipGetLastError());
     // Y, W stored as szt rows of length rowdim (row stride rowdim)
     const Op Yt{p->Y, p->yps, 1, rowdim};  // Yhat(i, k) = Y[k][i]
@@ -687,23 +681,20 @@ int phc_baqr_create(phc_baqr **out, int K, int nrows, int ncols, int szt) {
   PHC_REQUIRE(nrows >= ncols, "phc_baqr_create: nrows must be at least ncols");
   phc_baqr *p = new phc_baqr();
   p->K = K; p->nrows = nrows; p->ncols = ncols; p->szt = szt; p->nbt = ncols / szt;
-  p->nchunk = (nrows + RCH - 1) / RCH;
   p->yps = (size_t)szt * nrows;
   p->gps = (size_t)szt * szt;
   p->t1ps = (size_t)szt * ncols;
   p->t2ps = (size_t)nrows * szt;
-  p->wps = (size_t)p->nchunk * szt;
+  p->pps = (size_t)szt * nrows;
   // split-k partial sums: up to 8 slices of the largest thin product, and
   // the Q^H b products (nrows x 1 in up to 32 slices)
   p->pcap = (size_t)nrows * szt * 8;
   if ((size_t)nrows * 32 > p->pcap) p->pcap = (size_t)nrows * 32;
   const size_t z = sizeof(double) * 2 * K;
-  double **bufs[] = {&p->Y, &p->W, &p->G, &p->S, &p->T1, &p->T2, &p->wp, &p->w, &p->B, &p->part,
-                     &p->hs};
+  double **bufs[] = {&p->Y, &p->W, &p->G, &p->S, &p->T1, &p->T2, &p->Pc, &p->B, &p->part};
   const size_t sizes[] = {z * p->yps, z * p->yps, z * p->gps, z * p->gps, z * p->t1ps,
-                          z * p->t2ps, z * p->wps, z * szt, sizeof(double) * K * szt, z * p->pcap,
-                          sizeof(double) * (2 * K + 1)};
-  const int nb = 11;
+                          z * p->t2ps, z * p->pps, sizeof(double) * K * szt, z * p->pcap};
+  const int nb = 9;
   for (int i = 0; i < nb; ++i) *bufs[i] = nullptr;
   p->exec = nullptr;
   p->cap = nullptr;
@@ -731,7 +722,7 @@ int phc_baqr_destroy(phc_baqr *p) {
   if (!p) return 0;
   if (p->exec) hipGraphExecDestroy(p->exec);
   if (p->cap) hipStreamDestroy(p->cap);
-  double *bufs[] = {p->Y, p->W, p->G, p->S, p->T1, p->T2, p->wp, p->w, p->B, p->part, p->hs};
+  double *bufs[] = {p->Y, p->W, p->G, p->S, p->T1, p->T2, p->Pc, p->B, p->part};
   for (double *b : bufs) hipFree(b);
   delete p;
   return 0;
