@@ -5,13 +5,12 @@
 //   reference (Matrices/)                       here
 //   CPU_cmplx4_blocked_houseqr (dbl4_baqr_host.cpp:1206),
 //   GPU_cmplx4_blocked_houseqr (dbl4_baqr_kernels.cu:6197)
-//     per column: house, panel leftRupdate      house_kernel (forms v as well) and
-//                                               column_update_kernel (one workgroup
-//                                               per panel column: dot product,
-//                                               reduction, update), on a column-
-//                                               major copy of the panel; the launches
-//                                               of one factorization replay as a HIP
-//                                               graph
+//     per column: house, panel leftRupdate      house_kernel (forms v as well),
+//                                               column_dot_kernel and column_upd_kernel
+//                                               (workgroups per panel column and split
+//                                               of its rows), on a column-major copy
+//                                               of the panel; the launches of one
+//                                               factorization replay as a HIP graph
 //     VB_to_W (dbl4_baqr_host.cpp:91)           W = Y S with S upper triangular from
 //                                               the Gram matrix G = Y^H Y: one
 //                                               wavefront per row of S
@@ -277,9 +276,10 @@ constexpr int HW = 1024;  // lanes of the Householder kernel
 // the column-major copy Pc of the panel (column c at Pc + c*m): beta goes to
 // B[L], v (v_0 = 1, v_i = x_i / v0, or x_i when sigma = 0) to Y row L from
 // position L, zeros to Y row L before position L.
-//   wavefronts 1..15: sigma = sum_{i>0} |x_i|^2, one entry per lane (the
-//     first 960 rows), a butterfly per wavefront;
-//   wavefront 0, meanwhile: |x0|^2 and |x0| on lane 0; then the 15 sums
+//   wavefronts 1..nw-1 (nw = blockDim.x / 64 <= 16, as many as the rows
+//     need): sigma = sum_{i>0} |x_i|^2, one entry per lane and pass, a
+//     butterfly per wavefront;
+//   wavefront 0, meanwhile: |x0|^2 and |x0| on lane 0; then the nw-1 sums
 //     (butterfly), sqrt(|x0|^2 + sigma), mu, v0 = x0 - mu x0 (re on lane 0,
 //     im on lane 1), |v0|^2, and the three quotients beta = |v0|^2 /
 //     (sigma + |v0|^2) (times 2), re and im of conj(v0) / |v0|^2 on lanes
@@ -291,14 +291,14 @@ template <int K>
 __global__ void __launch_bounds__(HW)
 house_kernel(const double *Pc, size_t pps, int m, int rowoff, int colidx, int nrows, int L,
              double *Y, size_t yps, int rowdim, double *Bv, int szt) {
-  constexpr int NW = HW / 64;
-  __shared__ T<K> part[NW];
+  __shared__ T<K> part[HW / 64];
   __shared__ Z<K> sinv;
   __shared__ int sflag;
   const int t = threadIdx.x, w = t >> 6, n = nrows - colidx;
+  const int nw = blockDim.x >> 6, stride = blockDim.x - 64;
   const double *col = Pc + (size_t)(colidx - rowoff) * m + colidx;  // x_i at col[i]
   auto x = [&](int i) { return C<K>::load(col, pps, i); };
-  for (int i = t; i < L; i += HW) C<K>::store(Y, yps, (size_t)L * rowdim + i, C<K>::zero());
+  for (int i = t; i < L; i += blockDim.x) C<K>::store(Y, yps, (size_t)L * rowdim + i, C<K>::zero());
   Z<K> x0, xr;
   T<K> sq0, r0;
   const int ir = t - 63;  // the row of lane t >= 64 in the first pass
@@ -314,7 +314,7 @@ house_kernel(const double *Pc, size_t pps, int m, int rowoff, int colidx, int nr
       xr = x(ir);
       s = nl_norm2<K>(xr);
     }
-    for (int i = ir + HW - 64; i < n; i += HW - 64) s = nl_addsq<K>(s, x(i));
+    for (int i = ir + stride; i < n; i += stride) s = nl_addsq<K>(s, x(i));
     s = nl_add<K>(s, bfly_r<1, K>(s));
     s = nl_add<K>(s, bfly_r<2, K>(s));
     s = nl_add<K>(s, bfly_r<4, K>(s));
@@ -325,7 +325,7 @@ house_kernel(const double *Pc, size_t pps, int m, int rowoff, int colidx, int nr
   }
   __syncthreads();
   if (w == 0) {
-    T<K> u = (t < NW - 1) ? part[t] : R<K>::zero();
+    T<K> u = (t < nw - 1) ? part[t] : R<K>::zero();
     u = nl_add<K>(u, bfly_r<1, K>(u));
     u = nl_add<K>(u, bfly_r<2, K>(u));
     u = nl_add<K>(u, bfly_r<4, K>(u));
@@ -394,62 +394,59 @@ house_kernel(const double *Pc, size_t pps, int m, int rowoff, int colidx, int nr
   if (t == 0) C<K>::store(yl, yps, 0, C<K>::from(1.0, 0.0));
   if (w != 0) {
     if (ir < n) C<K>::store(yl, yps, ir, vof(xr));
-    for (int i = ir + HW - 64; i < n; i += HW - 64) C<K>::store(yl, yps, i, vof(x(i)));
+    for (int i = ir + stride; i < n; i += stride) C<K>::store(yl, yps, i, vof(x(i)));
   }
 }
 
-// One column j = colidx + blockIdx.x of the panel per workgroup, on the
-// column-major copy Pc (the panel part of the reference's leftRupdate,
-// Matrices/dbl4_baqr_host.cpp:357, one column at a time):
-//   w_j = beta_L sum_{i >= colidx} conj(R_ij) v_i, one row per lane, a
-//   butterfly per wavefront and one over the 16 wavefront sums;
-//   R_ij -= v_i conj(w_j) for the same rows.
+constexpr int CW = 256;  // lanes (rows) per workgroup of the column kernels
+
+// The panel part of the reference's leftRupdate (Matrices/dbl4_baqr_host.cpp:
+// 357) for the columns j = colidx + c of the column-major copy Pc, with the
+// rows split over workgroups of CW lanes (one row per lane):
+//   column_dot_kernel, workgroup (c, s): wpart[s][c] = sum over the rows of
+//   split s of conj(R_ij) v_i (a butterfly per wavefront, then the four
+//   wavefront sums);
+//   column_upd_kernel, workgroup (c, s): w_j = beta_L (wpart[0][c] + ... +
+//   wpart[ns-1][c]) and R_ij -= v_i conj(w_j) for the rows of split s.
 template <int K>
-__global__ void __launch_bounds__(HW)
-column_update_kernel(double *Pc, size_t pps, int m, int rowoff, int colidx, int nrows,
-                     const double *Y, size_t yps, int rowdim, int L, const double *Bv, int szt) {
-  constexpr int NW = HW / 64;
-  __shared__ Z<K> part[NW];
-  __shared__ Z<K> wsh;
-  const int t = threadIdx.x, w = t >> 6, n = nrows - colidx;
-  double *col = Pc + (size_t)(colidx + blockIdx.x - rowoff) * m + colidx;  // R_ij at col[i]
-  const double *yl = Y + (size_t)L * rowdim + L;                           // v_i at yl[i]
-  Z<K> s = C<K>::zero(), xt, vt;
-  if (t < n) {
-    xt = C<K>::load(col, pps, t);
-    vt = C<K>::load(yl, yps, t);
-    cmac<K, true, false>(s, xt, vt);
-  }
-  for (int i = t + HW; i < n; i += HW)
-    cmac<K, true, false>(s, C<K>::load(col, pps, i), C<K>::load(yl, yps, i));
+__global__ void __launch_bounds__(CW)
+column_dot_kernel(const double *Pc, size_t pps, int m, int rowoff, int colidx, int nrows,
+                  const double *Y, size_t yps, int rowdim, int L, double *wpart, size_t wpps,
+                  int szt) {
+  __shared__ Z<K> part[CW / 64];
+  const int t = threadIdx.x, c = blockIdx.x, i = blockIdx.y * CW + t, n = nrows - colidx;
+  const double *col = Pc + (size_t)(colidx + c - rowoff) * m + colidx;  // R_ij at col[i]
+  const double *yl = Y + (size_t)L * rowdim + L;                       // v_i at yl[i]
+  Z<K> s = C<K>::zero();
+  if (i < n) cmac<K, true, false>(s, C<K>::load(col, pps, i), C<K>::load(yl, yps, i));
   s = wave_csum<K>(s);
-  if ((t & 63) == 0) part[w] = s;
+  if ((t & 63) == 0) part[t >> 6] = s;
   __syncthreads();
-  if (w == 0) {
-    Z<K> u = (t < NW) ? part[t] : C<K>::zero();
-    u = C<K>::add(u, bfly_cz<1, K>(u));
-    u = C<K>::add(u, bfly_cz<2, K>(u));
-    u = C<K>::add(u, bfly_cz<4, K>(u));
-    u = C<K>::add(u, bfly_cz<8, K>(u));
-    if (t == 0) {
-      const T<K> beta = R<K>::load(Bv, szt, L);
-      Z<K> wj;
-      wj.re = mul(beta, u.re);
-      wj.im = mul(beta, u.im);
-      wsh = wj;
-    }
-  }
-  __syncthreads();
-  const Z<K> wj = wsh;
+  if (t == 0)
+    C<K>::store(wpart, wpps, (size_t)blockIdx.y * szt + c,
+                C<K>::add(C<K>::add(part[0], part[1]), C<K>::add(part[2], part[3])));
+}
+
+template <int K>
+__global__ void __launch_bounds__(CW)
+column_upd_kernel(double *Pc, size_t pps, int m, int rowoff, int colidx, int nrows,
+                  const double *Y, size_t yps, int rowdim, int L, const double *wpart,
+                  size_t wpps, int ns, const double *Bv, int szt) {
+  const int t = threadIdx.x, c = blockIdx.x, i = blockIdx.y * CW + t, n = nrows - colidx;
+  if (i >= n) return;
+  double *col = Pc + (size_t)(colidx + c - rowoff) * m + colidx;
+  const double *yl = Y + (size_t)L * rowdim + L;
+  Z<K> wj = C<K>::load(wpart, wpps, c);
+  for (int q = 1; q < ns; ++q) wj = C<K>::add(wj, C<K>::load(wpart, wpps, (size_t)q * szt + c));
+  const T<K> beta = R<K>::load(Bv, szt, L);
+  wj.re = mul(beta, wj.re);
+  wj.im = mul(beta, wj.im);
+  const Z<K> v = C<K>::load(yl, yps, i);
+  Z<K> r = C<K>::load(col, pps, i);
   // r -= v * conj(w)
-  auto upd = [&](Z<K> r, const Z<K> &v) {
-    r.re = sub(r.re, add(mul(v.re, wj.re), mul(v.im, wj.im)));
-    r.im = sub(r.im, sub(mul(v.im, wj.re), mul(v.re, wj.im)));
-    return r;
-  };
-  if (t < n) C<K>::store(col, pps, t, upd(xt, vt));
-  for (int i = t + HW; i < n; i += HW)
-    C<K>::store(col, pps, i, upd(C<K>::load(col, pps, i), C<K>::load(yl, yps, i)));
+  r.re = sub(r.re, add(mul(v.re, wj.re), mul(v.im, wj.im)));
+  r.im = sub(r.im, sub(mul(v.im, wj.re), mul(v.re, wj.im)));
+  C<K>::store(col, pps, i, r);
 }
 
 // the panel columns rowoff..rowoff+szt-1 of R (rows rowoff..m-1) to and from
@@ -579,8 +576,8 @@ __global__ void init_qr_kernel(int nrows, int ncols, const double *A, double *Q,
 
 struct phc_baqr {
   int K, nrows, ncols, szt, nbt;
-  double *Y, *W, *G, *S, *T1, *T2, *Pc, *B, *part;
-  size_t yps, gps, t1ps, t2ps, pps, pcap;
+  double *Y, *W, *G, *S, *T1, *T2, *Pc, *wpart, *B, *part;
+  size_t yps, gps, t1ps, t2ps, pps, wpps, pcap;
   // the launch sequence of one factorization, captured for the last (A, Q, R)
   hipStream_t cap;
   hipGraphExec_t exec;
@@ -602,10 +599,15 @@ template <int K> int run_qr(phc_baqr *p, const double *A, double *Q, double *R, 
                        dim3(256), 0, st, R, rps, n, m, rowoff, szt, p->Pc, p->pps, 1);
     for (int L = 0; L < szt; ++L) {
       const int colidx = rowoff + L, ncol = endcol - colidx;
-      hipLaunchKernelGGL(house_kernel<K>, dim3(1), dim3(HW), 0, st, p->Pc, p->pps, m, rowoff,
+      const int nr = m - colidx, ns = (nr + CW - 1) / CW;
+      const int hw = 64 * (1 + (nr - 1 + 63) / 64 < HW / 64 ? 1 + (nr - 1 + 63) / 64 : HW / 64);
+      hipLaunchKernelGGL(house_kernel<K>, dim3(1), dim3(hw), 0, st, p->Pc, p->pps, m, rowoff,
                          colidx, m, L, p->Y, p->yps, rowdim, p->B, szt);
-      hipLaunchKernelGGL(column_update_kernel<K>, dim3(ncol), dim3(HW), 0, st, p->Pc, p->pps, m,
-                         rowoff, colidx, m, p->Y, p->yps, rowdim, L, p->B, szt);
+      hipLaunchKernelGGL(column_dot_kernel<K>, dim3(ncol, ns), dim3(CW), 0, st, p->Pc, p->pps, m,
+                         rowoff, colidx, m, p->Y, p->yps, rowdim, L, p->wpart, p->wpps, szt);
+      hipLaunchKernelGGL(column_upd_kernel<K>, dim3(ncol, ns), dim3(CW), 0, st, p->Pc, p->pps, m,
+                         rowoff, colidx, m, p->Y, p->yps, rowdim, L, p->wpart, p->wpps, ns, p->B,
+                         szt);
     }
     hipLaunchKernelGGL(panel_copy_kernel<K>, dim3(phc::grid_for((size_t)rowdim * szt, 256)),
                        dim3(256), 0, st, R, rps, n, m, rowoff, szt, p->Pc, p->pps, 0);
@@ -686,15 +688,18 @@ int phc_baqr_create(phc_baqr **out, int K, int nrows, int ncols, int szt) {
   p->t1ps = (size_t)szt * ncols;
   p->t2ps = (size_t)nrows * szt;
   p->pps = (size_t)szt * nrows;
+  p->wpps = (size_t)szt * ((nrows + CW - 1) / CW);
   // split-k partial sums: up to 8 slices of the largest thin product, and
   // the Q^H b products (nrows x 1 in up to 32 slices)
   p->pcap = (size_t)nrows * szt * 8;
   if ((size_t)nrows * 32 > p->pcap) p->pcap = (size_t)nrows * 32;
   const size_t z = sizeof(double) * 2 * K;
-  double **bufs[] = {&p->Y, &p->W, &p->G, &p->S, &p->T1, &p->T2, &p->Pc, &p->B, &p->part};
+  double **bufs[] = {&p->Y,  &p->W,     &p->G, &p->S,   &p->T1,
+                     &p->T2, &p->Pc, &p->wpart, &p->B, &p->part};
   const size_t sizes[] = {z * p->yps, z * p->yps, z * p->gps, z * p->gps, z * p->t1ps,
-                          z * p->t2ps, z * p->pps, sizeof(double) * K * szt, z * p->pcap};
-  const int nb = 9;
+                          z * p->t2ps, z * p->pps, z * p->wpps, sizeof(double) * K * szt,
+                          z * p->pcap};
+  const int nb = 10;
   for (int i = 0; i < nb; ++i) *bufs[i] = nullptr;
   p->exec = nullptr;
   p->cap = nullptr;
@@ -722,7 +727,7 @@ int phc_baqr_destroy(phc_baqr *p) {
   if (!p) return 0;
   if (p->exec) hipGraphExecDestroy(p->exec);
   if (p->cap) hipStreamDestroy(p->cap);
-  double *bufs[] = {p->Y, p->W, p->G, p->S, p->T1, p->T2, p->Pc, p->B, p->part};
+  double *bufs[] = {p->Y, p->W, p->G, p->S, p->T1, p->T2, p->Pc, p->wpart, p->B, p->part};
   for (double *b : bufs) hipFree(b);
   delete p;
   return 0;
