@@ -276,14 +276,14 @@ constexpr int HW = 1024;  // lanes of the Householder kernel
 // the column-major copy Pc of the panel (column c at Pc + c*m): beta goes to
 // B[L], v (v_0 = 1, v_i = x_i / v0, or x_i when sigma = 0) to Y row L from
 // position L, zeros to Y row L before position L.
-//   wavefronts 1..nw-1 (nw = blockDim.x / 64 <= 16, as many as the rows
-//     need): sigma = sum_{i>0} |x_i|^2, one entry per lane and pass, a
-//     butterfly per wavefront;
-//   wavefront 0, meanwhile: |x0|^2 and |x0| on lane 0; then the nw-1 sums
-//     (butterfly), sqrt(|x0|^2 + sigma), mu, v0 = x0 - mu x0 (re on lane 0,
-//     im on lane 1), |v0|^2, and the three quotients beta = |v0|^2 /
-//     (sigma + |v0|^2) (times 2), re and im of conj(v0) / |v0|^2 on lanes
-//     0, 1, 2 in one lane-uniform division;
+//   every wavefront (nw = blockDim.x / 64 <= 16, as many as the rows need):
+//     sigma = sum_{i>0} |x_i|^2, one row per lane and pass, a butterfly per
+//     wavefront;
+//   wavefront 0: the nw sums (butterfly); nrm = sqrt(|x0|^2 + sigma) on
+//     lane 0 and r0 = |x0| on lane 1 in one lane-uniform root; mu = nrm /
+//     r0; v0 = x0 - mu x0 (re on lane 0, im on lane 1); |v0|^2; the three
+//     quotients beta = 2 |v0|^2 / (sigma + |v0|^2), re and im of
+//     conj(v0) / |v0|^2 on lanes 0, 1, 2 in one lane-uniform division;
 //   every lane: v_i = x_i * (1/v0) for its rows.
 // Sums are in tree order (the reference's is sequential): results agree to
 // a few units in the last limb.
@@ -295,37 +295,30 @@ house_kernel(const double *Pc, size_t pps, int m, int rowoff, int colidx, int nr
   __shared__ Z<K> sinv;
   __shared__ int sflag;
   const int t = threadIdx.x, w = t >> 6, n = nrows - colidx;
-  const int nw = blockDim.x >> 6, stride = blockDim.x - 64;
+  const int nw = blockDim.x >> 6, stride = blockDim.x;
   const double *col = Pc + (size_t)(colidx - rowoff) * m + colidx;  // x_i at col[i]
   auto x = [&](int i) { return C<K>::load(col, pps, i); };
-  for (int i = t; i < L; i += blockDim.x) C<K>::store(Y, yps, (size_t)L * rowdim + i, C<K>::zero());
-  Z<K> x0, xr;
-  T<K> sq0, r0;
-  const int ir = t - 63;  // the row of lane t >= 64 in the first pass
-  if (w == 0) {
-    x0 = x(0);
-    if (t == 0) {
-      sq0 = nl_norm2<K>(x0);
-      r0 = nl_sqrt<K>(sq0);
-    }
-  } else {
-    T<K> s = R<K>::zero();
-    if (ir < n) {
-      xr = x(ir);
-      s = nl_norm2<K>(xr);
-    }
-    for (int i = ir + stride; i < n; i += stride) s = nl_addsq<K>(s, x(i));
-    s = nl_add<K>(s, bfly_r<1, K>(s));
-    s = nl_add<K>(s, bfly_r<2, K>(s));
-    s = nl_add<K>(s, bfly_r<4, K>(s));
-    s = nl_add<K>(s, bfly_r<8, K>(s));
-    s = nl_add<K>(s, bfly_r<16, K>(s));
-    s = nl_add<K>(s, bfly_r<32, K>(s));
-    if ((t & 63) == 0) part[w - 1] = s;
+  for (int i = t; i < L; i += stride) C<K>::store(Y, yps, (size_t)L * rowdim + i, C<K>::zero());
+  const int ir = 1 + t;  // the row of lane t in the first pass
+  Z<K> xr;
+  T<K> s = R<K>::zero();
+  if (ir < n) {
+    xr = x(ir);
+    s = nl_norm2<K>(xr);
   }
+  for (int i = ir + stride; i < n; i += stride) s = nl_addsq<K>(s, x(i));
+  s = nl_add<K>(s, bfly_r<1, K>(s));
+  s = nl_add<K>(s, bfly_r<2, K>(s));
+  s = nl_add<K>(s, bfly_r<4, K>(s));
+  s = nl_add<K>(s, bfly_r<8, K>(s));
+  s = nl_add<K>(s, bfly_r<16, K>(s));
+  s = nl_add<K>(s, bfly_r<32, K>(s));
+  if ((t & 63) == 0) part[w] = s;
   __syncthreads();
   if (w == 0) {
-    T<K> u = (t < nw - 1) ? part[t] : R<K>::zero();
+    const Z<K> x0 = x(0);
+    T<K> u = (t < nw) ? part[t] : R<K>::zero();
+    const T<K> sq0 = nl_norm2<K>(x0);
     u = nl_add<K>(u, bfly_r<1, K>(u));
     u = nl_add<K>(u, bfly_r<2, K>(u));
     u = nl_add<K>(u, bfly_r<4, K>(u));
@@ -334,39 +327,37 @@ house_kernel(const double *Pc, size_t pps, int m, int rowoff, int colidx, int nr
     bool szero = true;
 #pragma unroll
     for (int q = 0; q < K; ++q) szero = szero && (sigma.x[q] == 0.0);
-    T<K> mu;
+    // lane 0: nrm = sqrt(|x0|^2 + sigma), lane 1: r0 = sqrt(|x0|^2)
+    T<K> rt = sq0;
+    if (t == 0) rt = nl_add<K>(sq0, sigma);
+    if (t < 2) rt = nl_sqrt<K>(rt);
+    const T<K> nrm = lane_r<K>(rt, 0), r0 = lane_r<K>(rt, 1);
     bool rz = true;
-    if (t == 0) {
 #pragma unroll
-      for (int q = 0; q < K; ++q) rz = rz && (r0.x[q] == 0.0);
-      const T<K> nrm = nl_sqrt<K>(nl_add<K>(sq0, sigma));
-      mu = rz ? nrm : nl_div<K>(nrm, r0);
-    }
-    mu = lane_r<K>(mu, 0);
-    rz = __shfl((int)rz, 0) != 0;
+    for (int q = 0; q < K; ++q) rz = rz && (r0.x[q] == 0.0);
+    const T<K> mu = rz ? nrm : nl_div<K>(nrm, r0);
     // lane 0: v0.re, lane 1: v0.im
-    T<K> vc;
+    T<K> vc, vsq;
     if (t < 2) {
       const T<K> xc = (t == 0) ? x0.re : x0.im;
       if (rz)
         vc = (t == 0) ? neg(mu) : R<K>::zero();
       else
         vc = nl_sub<K>(xc, nl_mul<K>(mu, xc));
+      vsq = nl_mul<K>(vc, vc);
     }
-    T<K> vsq;
-    if (t < 2) vsq = nl_mul<K>(vc, vc);
     const T<K> v0re = lane_r<K>(vc, 0), v0im = lane_r<K>(vc, 1);
-    const T<K> vsqim = lane_r<K>(vsq, 1);
-    T<K> sqrv0, den;
-    if (t == 0) {
-      sqrv0 = nl_add<K>(vsq, vsqim);
-      den = nl_add<K>(sigma, sqrv0);
-    }
-    sqrv0 = lane_r<K>(sqrv0, 0);
-    den = lane_r<K>(den, 0);
+    const T<K> sqrv0 = nl_add<K>(lane_r<K>(vsq, 0), lane_r<K>(vsq, 1));
+    T<K> num, den;
     if (t < 3) {
-      const T<K> num = (t == 0) ? sqrv0 : (t == 1) ? v0re : v0im;
-      T<K> q = nl_div<K>(num, (t == 0) ? den : sqrv0);
+      if (t == 0) {
+        num = sqrv0;
+        den = nl_add<K>(sigma, sqrv0);
+      } else {
+        num = (t == 1) ? v0re : v0im;
+        den = sqrv0;
+      }
+      T<K> q = nl_div<K>(num, den);
       if (t == 0) q = mul_d(q, 2.0);
       if (t == 2) q = neg(q);
       if (szero) q = R<K>::zero();
@@ -392,10 +383,8 @@ house_kernel(const double *Pc, size_t pps, int m, int rowoff, int colidx, int nr
   };
   double *yl = Y + (size_t)L * rowdim + L;  // v_i at yl[i]
   if (t == 0) C<K>::store(yl, yps, 0, C<K>::from(1.0, 0.0));
-  if (w != 0) {
-    if (ir < n) C<K>::store(yl, yps, ir, vof(xr));
-    for (int i = ir + stride; i < n; i += stride) C<K>::store(yl, yps, i, vof(x(i)));
-  }
+  if (ir < n) C<K>::store(yl, yps, ir, vof(xr));
+  for (int i = ir + stride; i < n; i += stride) C<K>::store(yl, yps, i, vof(x(i)));
 }
 
 constexpr int CW = 256;  // lanes (rows) per workgroup of the column kernels
@@ -600,7 +589,7 @@ template <int K> int run_qr(phc_baqr *p, const double *A, double *Q, double *R, 
     for (int L = 0; L < szt; ++L) {
       const int colidx = rowoff + L, ncol = endcol - colidx;
       const int nr = m - colidx, ns = (nr + CW - 1) / CW;
-      const int hw = 64 * (1 + (nr - 1 + 63) / 64 < HW / 64 ? 1 + (nr - 1 + 63) / 64 : HW / 64);
+      const int hwv = (nr - 1 + 63) / 64, hw = 64 * (hwv < 1 ? 1 : hwv < HW / 64 ? hwv : HW / 64);
       hipLaunchKernelGGL(house_kernel<K>, dim3(1), dim3(hw), 0, st, p->Pc, p->pps, m, rowoff,
                          colidx, m, L, p->Y, p->yps, rowdim, p->B, szt);
       hipLaunchKernelGGL(column_dot_kernel<K>, dim3(ncol, ns), dim3(CW), 0, st, p->Pc, p->pps, m,
