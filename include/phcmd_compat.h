@@ -131,4 +131,31 @@ void GPU_dbl4_bals_solve(int dim, int degp1, int szt, int nbt, int tailidx, doub
                          double *totqrlapsedms, double *totqtblapsedms, double *totbslapsedms,
                          double *totupdlapsedms, int vrblvl);
 
+/* ------------------------------------------------ linear residue -------
+ * Newton/dbl_tail_kernels.h:125, dbl2_tail_kernels.h:164,
+ * dbl4_tail_kernels.h:244: resvec_i = rhs_i - sum_{j<=i-tailidx} mat_j
+ * sol_{i-j} for i = tailidx..degp1-1 (rows below tailidx unchanged), resmax
+ * the limbs (absolute values) of the entry of largest |leading limb|; the
+ * same bits as CPU_dbl2_linear_residue (Newton/dbl2_bals_host.cpp:548).
+ * *lapms accumulates the kernel time; *add and *mul are set to the
+ * reference's count (dim*dim per matrix-vector product). */
+void GPU_dbl_linear_residue(int dim, int degp1, int szt, int nbt, int tailidx, double ***mat,
+                            double **rhs, double **sol, double **resvec, double *resmax,
+                            double *totreslapsedms, long long int *add, long long int *mul,
+                            int vrblvl);
+void GPU_dbl2_linear_residue(int dim, int degp1, int szt, int nbt, int tailidx, double ***mathi,
+                             double ***matlo, double **rhshi, double **rhslo, double **solhi,
+                             double **sollo, double **resvechi, double **resveclo,
+                             double *resmaxhi, double *resmaxlo, double *lapms,
+                             long long int *add, long long int *mul, int vrblvl);
+void GPU_dbl4_linear_residue(int dim, int degp1, int szt, int nbt, int tailidx,
+                             double ***mathihi, double ***matlohi, double ***mathilo,
+                             double ***matlolo, double **rhshihi, double **rhslohi,
+                             double **rhshilo, double **rhslolo, double **solhihi,
+                             double **sollohi, double **solhilo, double **sollolo,
+                             double **resvechihi, double **resveclohi, double **resvechilo,
+                             double **resveclolo, double *resmaxhihi, double *resmaxlohi,
+                             double *resmaxhilo, double *resmaxlolo, double *lapms,
+                             long long int *add, long long int *mul, int vrblvl);
+
 #endif

@@ -240,6 +240,49 @@ void bals_solve(int K, int dim, int degp1, int tailidx, double ***const *mat,
   phc_free(df);
 }
 
+// r_i = b_i - sum_{j <= i-tailidx} A_j x_{i-j}, i = tailidx..degp1-1, of one
+// system (K limbs, real); rows of resvec below tailidx are left as they are
+void linear_residue(int K, int dim, int degp1, int tailidx, double ***const *mat,
+                    double **const *rhs, double **const *sol, double **const *resvec,
+                    double *const *resmax, double *lapms, long long *add, long long *mul,
+                    const char *fn) {
+  const int n = dim;
+  std::vector<double> hm((size_t)K * degp1 * n * n), hb, hx, hr, hmax(K);
+  for (int q = 0; q < K; ++q)
+    for (int j = 0; j < degp1; ++j)
+      for (int r = 0; r < n; ++r)
+        for (int c = 0; c < n; ++c) hm[(((size_t)q * degp1 + j) * n + r) * n + c] = mat[q][j][r][c];
+  pack(rhs, K, degp1, n, hb);
+  pack(sol, K, degp1, n, hx);
+  pack(resvec, K, degp1, n, hr);
+  Buf dm(hm.size()), db(hb.size()), dx(hx.size()), dr(hr.size()), dmax(K);
+  if (!dm.p || !db.p || !dx.p || !dr.p || !dmax.p || !dm.up(hm) || !db.up(hb) || !dx.up(hx) ||
+      !dr.up(hr)) {
+    ok(-1, fn);
+    return;
+  }
+  void *e0 = nullptr, *e1 = nullptr;
+  phc_event_create(&e0);
+  phc_event_create(&e1);
+  phc_event_record(e0, nullptr);
+  const bool run = ok(phc_linear_residue_batch(K, n, degp1, tailidx, 1, dm.d(), db.d(), dx.d(),
+                                               dr.d(), dmax.d(), nullptr), fn);
+  phc_event_record(e1, nullptr);
+  if (run && ok(phc_synchronize(), fn) && dr.down(hr) && dmax.down(hmax)) {
+    unpack(hr, K, degp1, n, resvec);
+    for (int q = 0; q < K; ++q) *resmax[q] = hmax[q];
+    // the reference's count: dim*dim additions and multiplications per
+    // matrix-vector product, (i - tailidx + 1) products for residual i
+    long long prods = 0;
+    for (int i = tailidx; i < degp1; ++i) prods += i - tailidx + 1;
+    *add = *mul = prods * (long long)dim * dim;
+    float ms = 0.0f;
+    if (lapms && phc_event_elapsed_ms(e0, e1, &ms) == 0) *lapms += ms;
+  }
+  phc_event_destroy(e0);
+  phc_event_destroy(e1);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------ block Toeplitz solve
@@ -421,4 +464,46 @@ void GPU_cmplx4_bals_qhb(int ncols, int, int, double **Qrehihi, double **Qrelohi
   double **Q[] = {Qrehihi, Qrelohi, Qrehilo, Qrelolo, Qimhihi, Qimlohi, Qimhilo, Qimlolo};
   double *b[] = {brehihi, brelohi, brehilo, brelolo, bimhihi, bimlohi, bimhilo, bimlolo};
   qhb(4, ncols, Q, b, totqtblapsedms, "GPU_cmplx4_bals_qhb");
+}
+
+// ------------------------------------------------------------ linear residue
+
+void GPU_dbl_linear_residue(int dim, int degp1, int, int, int tailidx, double ***mat,
+                            double **rhs, double **sol, double **resvec, double *resmax,
+                            double *totreslapsedms, long long int *add, long long int *mul, int) {
+  double ***m[] = {mat};
+  double **b[] = {rhs}, **x[] = {sol}, **r[] = {resvec};
+  double *mx[] = {resmax};
+  linear_residue(1, dim, degp1, tailidx, m, b, x, r, mx, totreslapsedms, add, mul,
+                 "GPU_dbl_linear_residue");
+}
+
+void GPU_dbl2_linear_residue(int dim, int degp1, int, int, int tailidx, double ***mathi,
+                             double ***matlo, double **rhshi, double **rhslo, double **solhi,
+                             double **sollo, double **resvechi, double **resveclo,
+                             double *resmaxhi, double *resmaxlo, double *lapms,
+                             long long int *add, long long int *mul, int) {
+  double ***m[] = {mathi, matlo};
+  double **b[] = {rhshi, rhslo}, **x[] = {solhi, sollo}, **r[] = {resvechi, resveclo};
+  double *mx[] = {resmaxhi, resmaxlo};
+  linear_residue(2, dim, degp1, tailidx, m, b, x, r, mx, lapms, add, mul,
+                 "GPU_dbl2_linear_residue");
+}
+
+void GPU_dbl4_linear_residue(int dim, int degp1, int, int, int tailidx, double ***mathihi,
+                             double ***matlohi, double ***mathilo, double ***matlolo,
+                             double **rhshihi, double **rhslohi, double **rhshilo,
+                             double **rhslolo, double **solhihi, double **sollohi,
+                             double **solhilo, double **sollolo, double **resvechihi,
+                             double **resveclohi, double **resvechilo, double **resveclolo,
+                             double *resmaxhihi, double *resmaxlohi, double *resmaxhilo,
+                             double *resmaxlolo, double *lapms, long long int *add,
+                             long long int *mul, int) {
+  double ***m[] = {mathihi, matlohi, mathilo, matlolo};
+  double **b[] = {rhshihi, rhslohi, rhshilo, rhslolo};
+  double **x[] = {solhihi, sollohi, solhilo, sollolo};
+  double **r[] = {resvechihi, resveclohi, resvechilo, resveclolo};
+  double *mx[] = {resmaxhihi, resmaxlohi, resmaxhilo, resmaxlolo};
+  linear_residue(4, dim, degp1, tailidx, m, b, x, r, mx, lapms, add, mul,
+                 "GPU_dbl4_linear_residue");
 }

@@ -66,6 +66,15 @@ void CPU_dbl4_qrbs_solve(int, int, int, double ***, double ***, double ***, doub
                          double **, double **, double **, double *, double *, double *, double *,
                          bool *, bool *, int *, int *, int *, int);
 
+void CPU_dbl_linear_residue(int, int, int, double ***, double **, double **, double **, double *,
+                            int);
+void CPU_dbl2_linear_residue(int, int, int, double ***, double ***, double **, double **,
+                             double **, double **, double **, double **, double *, double *, int);
+void CPU_dbl4_linear_residue(int, int, int, double ***, double ***, double ***, double ***,
+                             double **, double **, double **, double **, double **, double **,
+                             double **, double **, double **, double **, double **, double **,
+                             double *, double *, double *, double *, int);
+
 static int failures = 0;
 
 static void check(bool ok, const char *what, double err, double tol) {
@@ -441,6 +450,56 @@ static void test_bals_solve(int K, int dim, int degp1, int tailidx) {
   for (RMat *m : mat) delete m;
 }
 
+// the residual of a block Toeplitz system through CPU_dblK_linear_residue
+// (reference) and GPU_dblK_linear_residue (this library): bit for bit, and
+// the rows below tailidx left as they were
+static void test_linear_residue(int K, int dim, int degp1, int tailidx) {
+  std::vector<RMat *> mat;
+  for (int j = 0; j < degp1; ++j) {
+    mat.push_back(new RMat(K, dim, dim));
+    mat.back()->randomize();
+  }
+  std::vector<std::vector<double **>> mp(K, std::vector<double **>(degp1));
+  for (int q = 0; q < K; ++q)
+    for (int j = 0; j < degp1; ++j) mp[q][j] = mat[j]->p(q);
+  RMat b(K, degp1, dim), x(K, degp1, dim), rc(K, degp1, dim), rg(K, degp1, dim);
+  b.randomize();
+  x.randomize();
+  rc.randomize();
+  rg.store = rc.store;
+  double mc[4] = {0, 0, 0, 0}, mg[4] = {0, 0, 0, 0}, ms = 0.0;
+  long long add = -1, mul = -1;
+  if (K == 1) {
+    CPU_dbl_linear_residue(dim, degp1, tailidx, mp[0].data(), b.p(0), x.p(0), rc.p(0), &mc[0], 0);
+    GPU_dbl_linear_residue(dim, degp1, 1, dim, tailidx, mp[0].data(), b.p(0), x.p(0), rg.p(0),
+                           &mg[0], &ms, &add, &mul, 0);
+  } else if (K == 2) {
+    CPU_dbl2_linear_residue(dim, degp1, tailidx, mp[0].data(), mp[1].data(), b.p(0), b.p(1),
+                            x.p(0), x.p(1), rc.p(0), rc.p(1), &mc[0], &mc[1], 0);
+    GPU_dbl2_linear_residue(dim, degp1, 1, dim, tailidx, mp[0].data(), mp[1].data(), b.p(0),
+                            b.p(1), x.p(0), x.p(1), rg.p(0), rg.p(1), &mg[0], &mg[1], &ms, &add,
+                            &mul, 0);
+  } else {
+    CPU_dbl4_linear_residue(dim, degp1, tailidx, mp[0].data(), mp[1].data(), mp[2].data(),
+                            mp[3].data(), b.p(0), b.p(1), b.p(2), b.p(3), x.p(0), x.p(1), x.p(2),
+                            x.p(3), rc.p(0), rc.p(1), rc.p(2), rc.p(3), &mc[0], &mc[1], &mc[2],
+                            &mc[3], 0);
+    GPU_dbl4_linear_residue(dim, degp1, 1, dim, tailidx, mp[0].data(), mp[1].data(),
+                            mp[2].data(), mp[3].data(), b.p(0), b.p(1), b.p(2), b.p(3), x.p(0),
+                            x.p(1), x.p(2), x.p(3), rg.p(0), rg.p(1), rg.p(2), rg.p(3), &mg[0],
+                            &mg[1], &mg[2], &mg[3], &ms, &add, &mul, 0);
+  }
+  long long prods = 0;
+  for (int i = tailidx; i < degp1; ++i) prods += i - tailidx + 1;
+  bool same = rc.same(rg) && add == prods * dim * dim && mul == add;
+  for (int q = 0; q < K; ++q) same = same && mc[q] == mg[q];
+  char what[96];
+  std::snprintf(what, sizeof what, "GPU_dbl%s_linear_residue dim %d degp1 %d tail %d (bitwise)",
+                K == 1 ? "" : (K == 2 ? "2" : "4"), dim, degp1, tailidx);
+  check(same, what, same ? 0.0 : 1.0, 0.0);
+  for (RMat *m : mat) delete m;
+}
+
 int main() {
   std::srand(20261202);
   test_products(15);
@@ -457,6 +516,9 @@ int main() {
   test_bals_solve(2, 8, 16, 1);
   test_bals_solve(4, 8, 8, 1);
   test_bals_solve(2, 16, 5, 1);
+  test_linear_residue(1, 8, 8, 0);
+  test_linear_residue(2, 8, 16, 3);
+  test_linear_residue(4, 12, 8, 1);
   std::printf("%d failed\n", failures);
   return failures;
 }
