@@ -47,6 +47,31 @@ void GPU_cmplx4_product(double *xrehihi_h, double *xrelohi_h, double *xrehilo_h,
                         double *zimlohi_h, double *zimhilo_h, double *zimlolo_h, int deg,
                         int freq, int BS, int mode);
 
+/* Convolutions/dbl8_convolutions_kernels.h:238,291: octo-double products. */
+void GPU_dbl8_product(double *xhihihi_h, double *xlohihi_h, double *xhilohi_h,
+                      double *xlolohi_h, double *xhihilo_h, double *xlohilo_h,
+                      double *xhilolo_h, double *xlololo_h, double *yhihihi_h,
+                      double *ylohihi_h, double *yhilohi_h, double *ylolohi_h,
+                      double *yhihilo_h, double *ylohilo_h, double *yhilolo_h,
+                      double *ylololo_h, double *zhihihi_h, double *zlohihi_h,
+                      double *zhilohi_h, double *zlolohi_h, double *zhihilo_h,
+                      double *zlohilo_h, double *zhilolo_h, double *zlololo_h, int deg, int freq,
+                      int BS, int padded);
+void GPU_cmplx8_product(
+    double *xrehihihi_h, double *xrelohihi_h, double *xrehilohi_h, double *xrelolohi_h,
+    double *xrehihilo_h, double *xrelohilo_h, double *xrehilolo_h, double *xrelololo_h,
+    double *ximhihihi_h, double *ximlohihi_h, double *ximhilohi_h, double *ximlolohi_h,
+    double *ximhihilo_h, double *ximlohilo_h, double *ximhilolo_h, double *ximlololo_h,
+    double *yrehihihi_h, double *yrelohihi_h, double *yrehilohi_h, double *yrelolohi_h,
+    double *yrehihilo_h, double *yrelohilo_h, double *yrehilolo_h, double *yrelololo_h,
+    double *yimhihihi_h, double *yimlohihi_h, double *yimhilohi_h, double *yimlolohi_h,
+    double *yimhihilo_h, double *yimlohilo_h, double *yimhilolo_h, double *yimlololo_h,
+    double *zrehihihi_h, double *zrelohihi_h, double *zrehilohi_h, double *zrelolohi_h,
+    double *zrehihilo_h, double *zrelohilo_h, double *zrehilolo_h, double *zrelololo_h,
+    double *zimhihihi_h, double *zimlohihi_h, double *zimhilohi_h, double *zimlolohi_h,
+    double *zimhihilo_h, double *zimlohilo_h, double *zimhilolo_h, double *zimlololo_h, int deg,
+    int freq, int BS, int mode);
+
 /* ------------------------------------------------ blocked QR ------------
  * Matrices/dbl2_baqr_kernels.h:2540 and dbl4_baqr_kernels.h:3739:
  * A (nrows x ncols, nrows >= ncols = szt*nbt) = Q R with Q (nrows x nrows)
@@ -91,6 +116,24 @@ void GPU_cmplx4_upper_tiled_solver(
     double *ximlolo, double *invlapms, double *mullapms, double *sublapms, double *totlapms,
     double *walltimesec, long long int *addcnt, double *addover, long long int *mulcnt,
     double *mulover, long long int *divcnt, double *divover);
+
+/* Matrices/dbl2_tabs_kernels.h:358 and dbl4_tabs_kernels.h:555: the real
+ * solvers, run as the complex ones with zero imaginary parts (U and b left
+ * unchanged, x returned); x agrees with CPU_dbl2_upper_tiled_solver to a
+ * few units in the last limb. */
+void GPU_dbl2_upper_tiled_solver(int dim, int szt, int nbt, double **Uhi, double **Ulo,
+                                 double *bhi, double *blo, double *xhi, double *xlo,
+                                 double *invlapms, double *mullapms, double *sublapms,
+                                 double *totlapms, double *walltimesec, long long int *addcnt,
+                                 long long int *mulcnt, long long int *divcnt);
+void GPU_dbl4_upper_tiled_solver(int dim, int szt, int nbt, double **Uhihi, double **Ulohi,
+                                 double **Uhilo, double **Ulolo, double *bhihi, double *blohi,
+                                 double *bhilo, double *blolo, double *xhihi, double *xlohi,
+                                 double *xhilo, double *xlolo, double *invlapms,
+                                 double *mullapms, double *sublapms, double *totlapms,
+                                 double *walltimesec, long long int *addcnt, double *addover,
+                                 long long int *mulcnt, double *mulover, long long int *divcnt,
+                                 double *divover);
 
 /* ------------------------------------------------ Q^H b ------------------
  * Newton/dbl2_bals_kernels.h:221 and dbl4_bals_kernels.h:305: b := Q^H b

@@ -446,6 +446,51 @@ void GPU_cmplx4_upper_tiled_solver(
                "GPU_cmplx4_upper_tiled_solver");
 }
 
+// real U: the complex solver with zero imaginary parts (the imaginary part
+// of x, zero, is discarded)
+void real_tiled_solver(int K, int dim, int szt, int nbt, double **const *U, double *const *b,
+                       double *const *x, double *totlapms, double *walltimesec, long long *addcnt,
+                       long long *mulcnt, long long *divcnt, const char *fn) {
+  std::vector<double> zeros((size_t)dim * dim, 0.0), xim((size_t)K * dim);
+  std::vector<double *> zrows(dim);
+  for (int i = 0; i < dim; ++i) zrows[i] = &zeros[(size_t)i * dim];
+  double **Uc[8];
+  double *bc[8], *xc[8];
+  for (int q = 0; q < K; ++q) {
+    Uc[q] = U[q];
+    Uc[K + q] = zrows.data();
+    bc[q] = b[q];
+    bc[K + q] = zeros.data();
+    xc[q] = x[q];
+    xc[K + q] = &xim[(size_t)q * dim];
+  }
+  tiled_solver(K, dim, szt, nbt, Uc, bc, xc, totlapms, walltimesec, addcnt, mulcnt, divcnt, fn);
+}
+
+void GPU_dbl2_upper_tiled_solver(int dim, int szt, int nbt, double **Uhi, double **Ulo,
+                                 double *bhi, double *blo, double *xhi, double *xlo, double *,
+                                 double *, double *, double *totlapms, double *walltimesec,
+                                 long long int *addcnt, long long int *mulcnt,
+                                 long long int *divcnt) {
+  double **U[] = {Uhi, Ulo};
+  double *b[] = {bhi, blo}, *x[] = {xhi, xlo};
+  real_tiled_solver(2, dim, szt, nbt, U, b, x, totlapms, walltimesec, addcnt, mulcnt, divcnt,
+                    "GPU_dbl2_upper_tiled_solver");
+}
+
+void GPU_dbl4_upper_tiled_solver(int dim, int szt, int nbt, double **Uhihi, double **Ulohi,
+                                 double **Uhilo, double **Ulolo, double *bhihi, double *blohi,
+                                 double *bhilo, double *blolo, double *xhihi, double *xlohi,
+                                 double *xhilo, double *xlolo, double *, double *, double *,
+                                 double *totlapms, double *walltimesec, long long int *addcnt,
+                                 double *, long long int *mulcnt, double *,
+                                 long long int *divcnt, double *) {
+  double **U[] = {Uhihi, Ulohi, Uhilo, Ulolo};
+  double *b[] = {bhihi, blohi, bhilo, blolo}, *x[] = {xhihi, xlohi, xhilo, xlolo};
+  real_tiled_solver(4, dim, szt, nbt, U, b, x, totlapms, walltimesec, addcnt, mulcnt, divcnt,
+                    "GPU_dbl4_upper_tiled_solver");
+}
+
 // ------------------------------------------------------------ Q^H b
 
 void GPU_cmplx2_bals_qhb(int ncols, int, int, double **Qrehi, double **Qrelo, double **Qimhi,
@@ -464,6 +509,53 @@ void GPU_cmplx4_bals_qhb(int ncols, int, int, double **Qrehihi, double **Qrelohi
   double **Q[] = {Qrehihi, Qrelohi, Qrehilo, Qrelolo, Qimhihi, Qimlohi, Qimhilo, Qimlolo};
   double *b[] = {brehihi, brelohi, brehilo, brelolo, bimhihi, bimlohi, bimhilo, bimlolo};
   qhb(4, ncols, Q, b, totqtblapsedms, "GPU_cmplx4_bals_qhb");
+}
+
+void GPU_dbl8_product(double *xhihihi_h, double *xlohihi_h, double *xhilohi_h,
+                      double *xlolohi_h, double *xhihilo_h, double *xlohilo_h,
+                      double *xhilolo_h, double *xlololo_h, double *yhihihi_h,
+                      double *ylohihi_h, double *yhilohi_h, double *ylolohi_h,
+                      double *yhihilo_h, double *ylohilo_h, double *yhilolo_h,
+                      double *ylololo_h, double *zhihihi_h, double *zlohihi_h,
+                      double *zhilohi_h, double *zlolohi_h, double *zhihilo_h,
+                      double *zlohilo_h, double *zhilolo_h, double *zlololo_h, int deg, int freq,
+                      int, int) {
+  double *x[] = {xhihihi_h, xlohihi_h, xhilohi_h, xlolohi_h,
+                 xhihilo_h, xlohilo_h, xhilolo_h, xlololo_h};
+  double *y[] = {yhihihi_h, ylohihi_h, yhilohi_h, ylolohi_h,
+                 yhihilo_h, ylohilo_h, yhilolo_h, ylololo_h};
+  double *z[] = {zhihihi_h, zlohihi_h, zhilohi_h, zlolohi_h,
+                 zhihilo_h, zlohilo_h, zhilolo_h, zlololo_h};
+  product(8, 0, x, y, z, deg, freq, "GPU_dbl8_product");
+}
+
+void GPU_cmplx8_product(
+    double *xrehihihi_h, double *xrelohihi_h, double *xrehilohi_h, double *xrelolohi_h,
+    double *xrehihilo_h, double *xrelohilo_h, double *xrehilolo_h, double *xrelololo_h,
+    double *ximhihihi_h, double *ximlohihi_h, double *ximhilohi_h, double *ximlolohi_h,
+    double *ximhihilo_h, double *ximlohilo_h, double *ximhilolo_h, double *ximlololo_h,
+    double *yrehihihi_h, double *yrelohihi_h, double *yrehilohi_h, double *yrelolohi_h,
+    double *yrehihilo_h, double *yrelohilo_h, double *yrehilolo_h, double *yrelololo_h,
+    double *yimhihihi_h, double *yimlohihi_h, double *yimhilohi_h, double *yimlolohi_h,
+    double *yimhihilo_h, double *yimlohilo_h, double *yimhilolo_h, double *yimlololo_h,
+    double *zrehihihi_h, double *zrelohihi_h, double *zrehilohi_h, double *zrelolohi_h,
+    double *zrehihilo_h, double *zrelohilo_h, double *zrehilolo_h, double *zrelololo_h,
+    double *zimhihihi_h, double *zimlohihi_h, double *zimhilohi_h, double *zimlolohi_h,
+    double *zimhihilo_h, double *zimlohilo_h, double *zimhilolo_h, double *zimlololo_h, int deg,
+    int freq, int, int) {
+  double *x[] = {xrehihihi_h, xrelohihi_h, xrehilohi_h, xrelolohi_h,
+                 xrehihilo_h, xrelohilo_h, xrehilolo_h, xrelololo_h,
+                 ximhihihi_h, ximlohihi_h, ximhilohi_h, ximlolohi_h,
+                 ximhihilo_h, ximlohilo_h, ximhilolo_h, ximlololo_h};
+  double *y[] = {yrehihihi_h, yrelohihi_h, yrehilohi_h, yrelolohi_h,
+                 yrehihilo_h, yrelohilo_h, yrehilolo_h, yrelololo_h,
+                 yimhihihi_h, yimlohihi_h, yimhilohi_h, yimlolohi_h,
+                 yimhihilo_h, yimlohilo_h, yimhilolo_h, yimlololo_h};
+  double *z[] = {zrehihihi_h, zrelohihi_h, zrehilohi_h, zrelolohi_h,
+                 zrehihilo_h, zrelohilo_h, zrehilolo_h, zrelololo_h,
+                 zimhihihi_h, zimlohihi_h, zimhilohi_h, zimlolohi_h,
+                 zimhihilo_h, zimlohilo_h, zimhilolo_h, zimlololo_h};
+  product(8, 1, x, y, z, deg, freq, "GPU_cmplx8_product");
 }
 
 // ------------------------------------------------------------ linear residue

@@ -75,6 +75,24 @@ void CPU_dbl4_linear_residue(int, int, int, double ***, double ***, double ***, 
                              double **, double **, double **, double **, double **, double **,
                              double *, double *, double *, double *, int);
 
+void CPU_dbl8_product(int, double *, double *, double *, double *, double *, double *, double *,
+                      double *, double *, double *, double *, double *, double *, double *,
+                      double *, double *, double *, double *, double *, double *, double *,
+                      double *, double *, double *);
+void CPU_cmplx8_product(int, double *, double *, double *, double *, double *, double *,
+                        double *, double *, double *, double *, double *, double *, double *,
+                        double *, double *, double *, double *, double *, double *, double *,
+                        double *, double *, double *, double *, double *, double *, double *,
+                        double *, double *, double *, double *, double *, double *, double *,
+                        double *, double *, double *, double *, double *, double *, double *,
+                        double *, double *, double *, double *, double *, double *, double *);
+
+void CPU_dbl2_upper_tiled_solver(int, int, int, double **, double **, double *, double *,
+                                 double *, double *, double *);
+void CPU_dbl4_upper_tiled_solver(int, int, int, double **, double **, double **, double **,
+                                 double *, double *, double *, double *, double *, double *,
+                                 double *, double *, double *);
+
 static int failures = 0;
 
 static void check(bool ok, const char *what, double err, double tol) {
@@ -238,6 +256,40 @@ static void test_products(int deg) {
                      p[12], p[13], p[14], p[15], p[24], p[25], p[26], p[27], p[28], p[29], p[30],
                      p[31], deg, 1, D, 0);
   check(same(8) == 0.0, "GPU_cmplx4_product vs CPU_cmplx4_product", same(8), 0.0);
+}
+
+// octo-double products: x in p[0..15], y in p[16..31], CPU z in p[32..47],
+// GPU z in p[48..63] (real: the first 8 planes of each)
+static void test_products8(int deg) {
+  const int D = deg + 1;
+  std::vector<double> v(64 * D);
+  double *p[64];
+  for (int i = 0; i < 64; ++i) p[i] = &v[(size_t)i * D];
+  for (auto &e : v) e = random_double();
+  auto diff = [&](int planes) {
+    double m = 0.0;
+    for (int q = 0; q < planes; ++q)
+      for (int k = 0; k < D; ++k) m = std::fmax(m, std::fabs(p[32 + q][k] - p[48 + q][k]));
+    return m;
+  };
+  CPU_dbl8_product(deg, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[16], p[17], p[18],
+                   p[19], p[20], p[21], p[22], p[23], p[32], p[33], p[34], p[35], p[36], p[37],
+                   p[38], p[39]);
+  GPU_dbl8_product(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[16], p[17], p[18], p[19],
+                   p[20], p[21], p[22], p[23], p[48], p[49], p[50], p[51], p[52], p[53], p[54],
+                   p[55], deg, 1, D, 1);
+  check(diff(8) == 0.0, "GPU_dbl8_product vs CPU_dbl8_product", diff(8), 0.0);
+  CPU_cmplx8_product(deg, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9], p[10],
+                     p[11], p[12], p[13], p[14], p[15], p[16], p[17], p[18], p[19], p[20], p[21],
+                     p[22], p[23], p[24], p[25], p[26], p[27], p[28], p[29], p[30], p[31], p[32],
+                     p[33], p[34], p[35], p[36], p[37], p[38], p[39], p[40], p[41], p[42], p[43],
+                     p[44], p[45], p[46], p[47]);
+  GPU_cmplx8_product(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9], p[10], p[11],
+                     p[12], p[13], p[14], p[15], p[16], p[17], p[18], p[19], p[20], p[21], p[22],
+                     p[23], p[24], p[25], p[26], p[27], p[28], p[29], p[30], p[31], p[48], p[49],
+                     p[50], p[51], p[52], p[53], p[54], p[55], p[56], p[57], p[58], p[59], p[60],
+                     p[61], p[62], p[63], deg, 1, D, 0);
+  check(diff(16) == 0.0, "GPU_cmplx8_product vs CPU_cmplx8_product", diff(16), 0.0);
 }
 
 // ------------------------------------------------------------ QR, solver
@@ -500,16 +552,66 @@ static void test_linear_residue(int K, int dim, int degp1, int tailidx) {
   for (RMat *m : mat) delete m;
 }
 
+// a real upper triangular system through CPU_dblK_upper_tiled_solver
+// (reference, on copies: it overwrites U and b) and GPU_dblK_upper_tiled_solver
+static void test_tiled_real(int K, int n, int szt) {
+  RMat U(K, n, n), Uc(K, n, n), b(K, 1, n), bc(K, 1, n), xc(K, 1, n), xg(K, 1, n);
+  U.randomize();
+  b.randomize();
+  for (int q = 0; q < K; ++q)
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < i; ++j) U.p(q)[i][j] = 0.0;
+  for (int i = 0; i < n; ++i) U.p(0)[i][i] += 4.0;  // diagonally dominant enough
+  Uc.store = U.store;
+  bc.store = b.store;
+  const std::vector<std::vector<double>> U0 = U.store, b0 = b.store;
+  double t[4] = {0, 0, 0, 0};
+  long long cnt[3] = {0, 0, 0};
+  double over[3] = {0, 0, 0};
+  if (K == 2) {
+    CPU_dbl2_upper_tiled_solver(n, szt, n / szt, Uc.p(0), Uc.p(1), bc.p(0)[0], bc.p(1)[0],
+                                xc.p(0)[0], xc.p(1)[0], &t[0]);
+    GPU_dbl2_upper_tiled_solver(n, szt, n / szt, U.p(0), U.p(1), b.p(0)[0], b.p(1)[0],
+                                xg.p(0)[0], xg.p(1)[0], &t[0], &t[1], &t[2], &t[3], &t[0],
+                                &cnt[0], &cnt[1], &cnt[2]);
+  } else {
+    CPU_dbl4_upper_tiled_solver(n, szt, n / szt, Uc.p(0), Uc.p(1), Uc.p(2), Uc.p(3), bc.p(0)[0],
+                                bc.p(1)[0], bc.p(2)[0], bc.p(3)[0], xc.p(0)[0], xc.p(1)[0],
+                                xc.p(2)[0], xc.p(3)[0], &t[0]);
+    GPU_dbl4_upper_tiled_solver(n, szt, n / szt, U.p(0), U.p(1), U.p(2), U.p(3), b.p(0)[0],
+                                b.p(1)[0], b.p(2)[0], b.p(3)[0], xg.p(0)[0], xg.p(1)[0],
+                                xg.p(2)[0], xg.p(3)[0], &t[0], &t[1], &t[2], &t[3], &t[0],
+                                &cnt[0], &over[0], &cnt[1], &over[1], &cnt[2], &over[2]);
+  }
+  double err = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double a[4], c[4];
+    for (int q = 0; q < K; ++q) { a[q] = xg.p(q)[0][i]; c[q] = xc.p(q)[0][i]; }
+    err = std::fmax(err, md_diff(K, a, c));
+  }
+  const double tol = 64.0 * n * std::pow(0x1p-52, K);
+  char what[96];
+  std::snprintf(what, sizeof what, "GPU_dbl%d_upper_tiled_solver %d szt %d: x", K, n, szt);
+  check(err < tol, what, err, tol);
+  std::snprintf(what, sizeof what, "GPU_dbl%d_upper_tiled_solver: U, b kept", K);
+  const bool kept = U.store == U0 && b.store == b0;
+  check(kept, what, kept ? 0.0 : 1.0, 0.0);
+}
+
 int main() {
   std::srand(20261202);
   test_products(15);
   test_products(8);
+  test_products8(15);
+  test_products8(8);
   test_qr(2, 64, 64, 16);
   test_qr(4, 64, 64, 16);
   test_qr(2, 96, 64, 32);
   test_qr(4, 48, 32, 8);
   test_tiled(2, 64, 16);
   test_tiled(4, 96, 32);
+  test_tiled_real(2, 64, 16);
+  test_tiled_real(4, 96, 32);
   test_least_squares(2, 64, 16);
   test_least_squares(4, 64, 32);
   test_bals_solve(1, 8, 8, 1);
