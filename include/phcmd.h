@@ -135,6 +135,13 @@ typedef struct {
   double tolcor;    /* corrector: max|dx| <= tolcor*(1+max|x|) */
   double tolend;    /* endpoint Newton tolerance */
   double divergence;/* path declared diverged when max|x| exceeds this */
+  /* predictor (Predictor_Feedback evaluates the Pade approximants of
+   * Pade_Vector, standard_predictor_convolutions.adb:312,958-994;
+   * Denominator_System/Pade, Deformations/Curves/standard_rational_approximations.adb):
+   * pade_dendeg 0: Taylor polynomial of degree pade_numdeg (0: all deg orders);
+   * pade_dendeg 1: [pade_numdeg/1] Pade approximant per component,
+   * 1 <= pade_numdeg <= deg - 1 (the reference's default is [5/1]). */
+  int pade_numdeg, pade_dendeg;
 } phc_track_params;
 
 /* Checks every index of a program against its sizes (no device work);

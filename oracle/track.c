@@ -443,6 +443,62 @@ static void store_point(const otr_program *P, work_t *w, double *x, int npaths, 
   }
 }
 
+/* [L/1] Pade approximant of one component's series c[0..L+1]
+ * (Deformations/Curves/standard_rational_approximations.adb: Denominator_System
+ * :7-32 puts c_L into the 1x1 matrix and -c_{L+1} into the right-hand side;
+ * Pade :80-117 solves it by lufac/lusolve, dencff = (1, d), and
+ * Numerator_Coefficients :34-56 gives c_i + d c_{i-1}; a zero pivot leaves
+ * dencff = (1, 0) and a zero numerator unless the right-hand side is zero,
+ * when the numerator is the Taylor coefficients).  In place: c_1..c_L become
+ * the numerator's, c_{L+1} becomes d; x0 (= c_0, kept apart) is zeroed with
+ * the numerator. */
+void otr_pade_coefficients(int K, double *c, int L, double *x0) {
+  double *cl = ZP(c, L), *cn = ZP(c, L + 1);
+  int i;
+  if (cl[0] == 0.0 && cl[K] == 0.0) {
+    if (cn[0] != 0.0 || cn[K] != 0.0) {
+      for (i = 1; i <= L; i++) czero(K, ZP(c, i));
+      czero(K, x0);
+    }
+    czero(K, cn);
+  } else {
+    double d[16], t[16];
+    ccopy(K, cn, t);
+    cneg(K, t);
+    omd_cdiv(K, t, cl, d);
+    for (i = L; i >= 1; i--) {
+      tc_mul(K, d, ZP(c, i - 1), t);
+      omd_cadd(K, ZP(c, i), t, ZP(c, i));
+    }
+    ccopy(K, d, cn);
+  }
+}
+
+/* predicted component at step s: the numerator (or Taylor polynomial of
+ * degree L when pden = 0) by Horner from c_L down, x0 added last, divided
+ * by 1 + d s when pden = 1 (Evaluate, standard_rational_approximations.adb:151-205) */
+void otr_pade_evaluate(int K, const double *c, int L, int pden, const double *x0, double s,
+                       double *out) {
+  double acc[16], tmp[16], p[16];
+  int k;
+  ccopy(K, ZP(c, L), acc);
+  for (k = L - 1; k >= 1; k--) {
+    cscale_d(K, acc, s, tmp);
+    omd_cadd(K, tmp, ZP(c, k), acc);
+  }
+  cscale_d(K, acc, s, tmp);
+  omd_cadd(K, x0, tmp, p);
+  if (pden == 1) {
+    double one[16];
+    cset(K, one, 1.0, 0.0);
+    cscale_d(K, ZP(c, L + 1), s, tmp);
+    omd_cadd(K, tmp, one, acc);
+    omd_cdiv(K, p, acc, out);
+  } else {
+    ccopy(K, p, out);
+  }
+}
+
 static int predictor_series(const otr_program *P, work_t *w) {
   const int K = w->K, n = P->n, D = P->deg + 1;
   double xk[16 * 64];
@@ -465,6 +521,8 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
                double *dxlast, int *work, const otr_params *prm, const otr_controller *ctl) {
   const int n = P->n, D = P->deg + 1;
   const int mode = ctl ? ctl->mode : 0;
+  const int pden = prm->pade_dendeg;
+  const int pnum = (pden == 0 && prm->pade_numdeg == 0) ? D - 1 : prm->pade_numdeg;
   work_t w;
   int path;
   if (n > 64 || !work_alloc(P, K, &w)) { work_free(&w); return; }
@@ -517,21 +575,15 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
         rej0 = rej;
         nnewt0 = nnewt;
       for (j = 0; j < n; j++) ccopy(K, ZP(w.slots, j * D), ZP(w.x0, j));
+      if (pden == 1)
+        for (j = 0; j < n; j++) otr_pade_coefficients(K, ZP(w.slots, j * D), pnum, ZP(w.x0, j));
       while (!accepted) {
         double tn = t + h, hs, res;
         int ok;
         if (tn >= 1.0 - prm->minstep) tn = 1.0;
         hs = tn - t;
-        for (j = 0; j < n; j++) {
-          double acc[16], tmp[16];
-          ccopy(K, ZP(w.slots, j * D + D - 1), acc);
-          for (k = D - 2; k >= 1; k--) {
-            cscale_d(K, acc, hs, tmp);
-            omd_cadd(K, tmp, ZP(w.slots, j * D + k), acc);
-          }
-          cscale_d(K, acc, hs, tmp);
-          omd_cadd(K, ZP(w.x0, j), tmp, ZP(w.slots, j * D));
-        }
+        for (j = 0; j < n; j++)
+          otr_pade_evaluate(K, ZP(w.slots, j * D), pnum, pden, ZP(w.x0, j), hs, ZP(w.slots, j * D));
         set_weights(P, &w, tn);
         ++nfeval;
         eval_point(P, &w, 0);

@@ -65,6 +65,7 @@ class PhcTrackParams(ctypes.Structure):
         ("maxsteps", c_int), ("corsteps", c_int), ("endsteps", c_int), ("fabry_offset", c_int),
         ("maxstep", c_double), ("minstep", c_double), ("pbeta", c_double), ("alpha", c_double),
         ("tolcor", c_double), ("tolend", c_double), ("divergence", c_double),
+        ("pade_numdeg", c_int), ("pade_dendeg", c_int),
     ]
 
 

@@ -12,6 +12,10 @@
 //                    Bounded_Update (three_way_minima.adb:96-109), and the
 //                    predictor feedback that halves the step while the
 //                    predicted residual exceeds alpha (Predictor_Feedback)
+//   predictor        the Taylor polynomial of the series, or its [L/1] Pade
+//                    approximant per component (Pade_Vector at
+//                    standard_predictor_convolutions.adb:312; Denominator_System
+//                    and Pade in standard_rational_approximations.adb)
 //   corrector        Newton steps, step halved on failure down to minstep
 //   batching         GPU_Path_mult (Path/Path_GPU/path_gpu_mult.cu:84-308)
 //                    compacts live paths on the host after every step; here
@@ -71,6 +75,7 @@ struct DevProgram {
 struct Params {
   int maxsteps, corsteps, endsteps, fabry_offset;
   double maxstep, minstep, pbeta, alpha, tolcor, tolend, divergence;
+  int pnum, pden;  // predictor: numerator degree (>= 1), denominator degree (0 or 1)
 };
 
 // step-size controller (device view of phc_controller)
@@ -724,7 +729,10 @@ template <int K> __device__ int mlp_choose(const Ctl &ctl, Path<K> &w, const dou
   return best;
 }
 
-template <int K, int MW>
+// PD: denominator degree of the predictor (0: Taylor polynomial, 1: [L/1]
+// Pade approximant), a template parameter so that the Taylor variant keeps
+// its registers
+template <int K, int MW, int PD>
 __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, Params par, int npaths,
                                                    const double *__restrict__ start, int KS,
                                                    double *__restrict__ endp, double *tend,
@@ -793,6 +801,34 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, Params par
       }
       const int rej0 = rej, nnewt0 = nnewt;
       if (l < n) w.x0[l] = w.slots[l * D];
+      // ---- [L/1] Pade coefficients per component ----
+      // The 1x1 denominator system c_L d = -c_{L+1} gives the denominator
+      // 1 + d s; the numerator coefficients c_i + d c_{i-1} (i = 1..L)
+      // replace the series in place and d takes the place of c_{L+1}.  A
+      // zero c_L leaves the denominator 1 and the Taylor numerator when
+      // c_{L+1} is zero too, else a zero numerator (Pade, lufac info /= 0).
+      // ---- [L/1] Pade coefficients per component ----
+      // The 1x1 denominator system c_L d = -c_{L+1} gives the denominator
+      // 1 + d s; the numerator coefficients c_i + d c_{i-1} (i = 1..L)
+      // replace the series in place and d takes the place of c_{L+1}.  A
+      // zero c_L leaves the denominator 1 and the Taylor numerator when
+      // c_{L+1} is zero too, else a zero numerator (Pade, lufac info /= 0).
+      if (PD == 1 && l < n) {
+        const int L = par.pnum;
+        cplx<K> *c = w.slots + l * D;
+        const cplx<K> cl = c[L], cn = c[L + 1];
+        if (R<K>::hi(cl.re) == 0.0 && R<K>::hi(cl.im) == 0.0) {
+          if (R<K>::hi(cn.re) != 0.0 || R<K>::hi(cn.im) != 0.0) {
+            for (int i = 1; i <= L; ++i) c[i] = Cx::zero();
+            w.x0[l] = Cx::zero();
+          }
+          c[L + 1] = Cx::zero();
+        } else {
+          const cplx<K> d = Cx::div(Cx::neg(cn), cl);
+          for (int i = L; i >= 1; --i) c[i] = Cx::add(c[i], Cx::mul(d, c[i - 1]));
+          c[L + 1] = d;
+        }
+      }
       wsync();
       // ---- feedback and corrector; halve the step on failure ----
       bool accepted = false;
@@ -801,9 +837,15 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, Params par
         if (tn >= 1.0 - par.minstep) tn = 1.0;
         const double hs = tn - t;
         if (l < n) {
-          cplx<K> acc = w.slots[l * D + D - 1];
-          for (int k = D - 2; k >= 1; --k) acc = Cx::add(Cx::scale_d(acc, hs), w.slots[l * D + k]);
-          w.slots[l * D] = Cx::add(w.x0[l], Cx::scale_d(acc, hs));
+          // numerator (the Taylor polynomial of degree L when pden = 0) by
+          // Horner, then the quotient by the denominator 1 + d s
+          const int L = par.pnum;
+          cplx<K> acc = w.slots[l * D + L];
+          for (int k = L - 1; k >= 1; --k) acc = Cx::add(Cx::scale_d(acc, hs), w.slots[l * D + k]);
+          cplx<K> p = Cx::add(w.x0[l], Cx::scale_d(acc, hs));
+          if (PD == 1)
+            p = Cx::div(p, Cx::add(Cx::scale_d(w.slots[l * D + L + 1], hs), Cx::from(1.0, 0.0)));
+          w.slots[l * D] = p;
         }
         wsync();
         set_weights<K>(P, w, tn);
@@ -1111,6 +1153,12 @@ int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, d
               "phc_track: fabry_offset must be below the series degree");
   PHC_REQUIRE(prm->maxstep > 0 && prm->minstep > 0 && prm->maxsteps > 0 && prm->corsteps > 0 &&
               prm->endsteps >= 0, "phc_track: step parameters must be positive");
+  PHC_REQUIRE(prm->pade_dendeg == 0 || prm->pade_dendeg == 1,
+              "phc_track: pade_dendeg must be 0 or 1");
+  PHC_REQUIRE(prm->pade_dendeg == 0
+                  ? (prm->pade_numdeg >= 0 && prm->pade_numdeg <= tr->P.D - 1)
+                  : (prm->pade_numdeg >= 1 && prm->pade_numdeg <= tr->P.D - 2),
+              "phc_track: pade_numdeg must be in 0..deg (Taylor) or 1..deg-1 (Pade [L/1])");
   if (npaths == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   int grid = 0, wv = 0;
@@ -1122,6 +1170,8 @@ int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, d
   par.fabry_offset = prm->fabry_offset; par.maxstep = prm->maxstep; par.minstep = prm->minstep;
   par.pbeta = prm->pbeta; par.alpha = prm->alpha; par.tolcor = prm->tolcor;
   par.tolend = prm->tolend; par.divergence = prm->divergence;
+  par.pden = prm->pade_dendeg;
+  par.pnum = (prm->pade_dendeg == 0 && prm->pade_numdeg == 0) ? tr->P.D - 1 : prm->pade_numdeg;
   Ctl ctl;
   ctl.mode = 0; ctl.ncand = 1; ctl.cap = 0; ctl.w = ctl.mu = ctl.sd = nullptr; ctl.thresh = 0.0f;
   ctl.seed = 0; ctl.log_x = nullptr; ctl.log_c = ctl.log_y = ctl.log_n = nullptr; ctl.pcap = 0.0;
@@ -1148,17 +1198,23 @@ int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, d
   PHC_HIP(hipMemsetAsync(tr->counter, 0, sizeof(int), st));
   // up to 8 waves: the variant with 256 VGPRs per lane; 9 to 12 waves (FP64
   // tracking only): the variant compiled for three waves per SIMD
-#define TK(KK, MW)                                                                            \
+#define TK(KK, MW, PD)                                                                        \
   do {                                                                                        \
-    PHC_HIP(hipFuncSetAttribute((const void *)track_kernel<KK, MW>,                          \
+    PHC_HIP(hipFuncSetAttribute((const void *)track_kernel<KK, MW, PD>,                      \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));       \
-    hipLaunchKernelGGL((track_kernel<KK, MW>), dim3(grid), dim3(64 * wv), lds, st, tr->P, par, \
-                       npaths, start, KS, endp, tend, status, nsteps, nrej, dxlast, work, ctl, \
-                       tr->counter);                                                          \
+    hipLaunchKernelGGL((track_kernel<KK, MW, PD>), dim3(grid), dim3(64 * wv), lds, st, tr->P, \
+                       par, npaths, start, KS, endp, tend, status, nsteps, nrej, dxlast, work, \
+                       ctl, tr->counter);                                                     \
   } while (0)
-  if (K == 1 && wv > 8) TK(1, TRACK_MAX_WAVES);
-  else if (K == 1) TK(1, 8);
-  else TK(2, 8);
+  if (par.pden == 0) {
+    if (K == 1 && wv > 8) TK(1, TRACK_MAX_WAVES, 0);
+    else if (K == 1) TK(1, 8, 0);
+    else TK(2, 8, 0);
+  } else {
+    if (K == 1 && wv > 8) TK(1, TRACK_MAX_WAVES, 1);
+    else if (K == 1) TK(1, 8, 1);
+    else TK(2, 8, 1);
+  }
 #undef TK
   PHC_HIP(hipGetLastError());
   return 0;

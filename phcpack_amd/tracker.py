@@ -33,11 +33,15 @@ class TrackParams:
     tolcor: float = 1e-8
     tolend: float = 1e-28
     divergence: float = 1e8
+    # predictor: Taylor polynomial (pade_dendeg 0, degree pade_numdeg or all
+    # orders if 0) or the [pade_numdeg/1] Pade approximant (pade_dendeg 1)
+    pade_numdeg: int = 0
+    pade_dendeg: int = 0
 
     def to_c(self):
         return PhcTrackParams(self.maxsteps, self.corsteps, self.endsteps, self.fabry_offset,
                               self.maxstep, self.minstep, self.pbeta, self.alpha, self.tolcor,
-                              self.tolend, self.divergence)
+                              self.tolend, self.divergence, self.pade_numdeg, self.pade_dendeg)
 
 
 def program_struct(prog):

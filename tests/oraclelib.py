@@ -96,6 +96,8 @@ def _track_sigs():
     o.otr_eval.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_double, P_d, P_d, P_d]
     o.otr_series.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_double, P_d, P_d]
     o.otr_series.restype = C.c_int
+    o.otr_pade_coefficients.argtypes = [C.c_int, P_d, C.c_int, P_d]
+    o.otr_pade_evaluate.argtypes = [C.c_int, P_d, C.c_int, C.c_int, P_d, C.c_double, P_d]
     return o
 
 
@@ -159,6 +161,21 @@ def oracle_track(prog, starts_planes, K, params, nthreads=1, paths=None, control
                     pi(out["status"]), pi(out["nsteps"]), pi(out["nrej"]), pd(out["dx"]),
                     pi(out["work"]), C.byref(p), cp)
     return out
+
+
+def oracle_pade(K, c, L, pden, s):
+    """The tracker's predictor on one component: c float64 [m][2K] (per
+    coefficient its K real limbs, then its K imaginary limbs).  Returns the
+    coefficients after the [L/1] construction (pden 1; unchanged for pden 0),
+    the constant term x0 and the predicted value [2K] at step s."""
+    o = _track_sigs()
+    c = np.array(c, dtype=np.float64, order="C")
+    x0 = c[0].copy()
+    if pden == 1:
+        o.otr_pade_coefficients(K, pd(c), L, pd(x0))
+    out = np.zeros(2 * K)
+    o.otr_pade_evaluate(K, pd(c), L, pden, pd(x0), s, pd(out))
+    return c, x0, out
 
 
 def oracle_refine(prog, planes, K, iters, tol, t=1.0):

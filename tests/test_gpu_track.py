@@ -55,6 +55,61 @@ def test_track_matches_oracle_wider(K, n, degree, seed, npaths):
     assert np.all(g["status"] == 1)
 
 
+@pytest.mark.parametrize("K,waves", [(1, 12), (1, 8), (2, 8)])
+@pytest.mark.parametrize("n,degree,seed,num,den", [(3, 2, 1, 5, 1), (4, 2, 2, 6, 1),
+                                                   (3, 3, 3, 2, 1), (4, 2, 2, 4, 0)])
+def test_track_pade_matches_oracle(K, waves, n, degree, seed, num, den):
+    """The [num/1] Pade predictor (den 1) and the Taylor polynomial of
+    lower degree (den 0): kernel and oracle agree bit for bit, in the 12-
+    and 8-wavefront variants of the FP64 kernel and in double-double."""
+    from phcpack_amd.tracker import Tracker, TrackParams, points_to_planes
+    p, hom, sols = make(n, degree, seed)
+    prm = TrackParams(tolend=1e-13 if K == 1 else 1e-28, pade_numdeg=num, pade_dendeg=den)
+    tr = Tracker(hom, deg=8)
+    print(f"track pade [{num}/{den}] K={K} waves={waves} n={n} degree={degree} seed={seed} "
+          f"paths={len(sols)}", flush=True)
+    g = tr.track(sols, K=K, params=prm, waves=waves)
+    o = oracle_track(tr.prog, points_to_planes(sols), K, prm)
+    for key in ("status", "nsteps", "nrej", "tend", "endp", "work"):
+        assert np.array_equal(g[key], o[key]), key
+    assert np.mean(g["status"] == 1) >= 0.75
+
+
+@pytest.mark.parametrize("K", [1, 2])
+def test_track_pade_rational_path(K):
+    """x(t) = (a + b t)/(r - t): the [5/1] predictor is exact, so every
+    corrector takes one Newton iteration; kernel and oracle agree."""
+    from phcpack_amd.homotopy import Homotopy, System
+    from phcpack_amd.tracker import Tracker, TrackParams, planes_to_points, points_to_planes
+    a, b, r = 0.6 + 0.2j, -0.3 + 0.9j, 1.25
+    hom = Homotopy(System(1, [[(1.0 + 0j, (1,))]]), System(1, [[(-1.0 + 0j, (0,))]]), 1.0)
+    hom.components = [(np.array([r, -1.0, 0.0], np.complex128), hom.target),
+                      (np.array([a, b, 0.0], np.complex128), hom.start)]
+    tr = Tracker(hom, deg=8)
+    prm = TrackParams(pade_numdeg=5, pade_dendeg=1, endsteps=0)
+    sols = np.array([[a / r]])
+    print(f"track pade rational path K={K}", flush=True)
+    g = tr.track(sols, K=K, params=prm)
+    o = oracle_track(tr.prog, points_to_planes(sols), K, prm)
+    for key in ("status", "nsteps", "nrej", "tend", "endp", "work"):
+        assert np.array_equal(g[key], o[key]), key
+    assert g["status"][0] == 1 and g["work"][1, 0] == g["work"][3, 0]
+    want = (a + b) / (r - 1)
+    assert abs(planes_to_points(g["endp"], K)[0, 0] - want) < 1e-13 * abs(want)
+
+
+def test_track_pade_degrees_refused():
+    """denominator degrees other than 0 and 1, and numerator degrees that
+    leave no coefficient for the denominator, are refused"""
+    from phcpack_amd._lib import PhcError
+    from phcpack_amd.tracker import Tracker, TrackParams
+    p, hom, sols = make(3, 2, 7)
+    tr = Tracker(hom, deg=8)
+    for num, den in ((5, 2), (8, 1), (0, 1), (9, 0)):
+        with pytest.raises(PhcError, match="pade"):
+            tr.track(sols, K=1, params=TrackParams(pade_numdeg=num, pade_dendeg=den))
+
+
 @pytest.mark.parametrize("waves", [1, 3, 8])
 def test_track_wave_counts_agree(waves):
     """Wavefronts of one workgroup track their paths independently: the

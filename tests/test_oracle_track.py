@@ -190,3 +190,127 @@ def test_fabry_pole_step_closed_form():
     z = s[:2, 0].sum(0) + 1j * s[2:, 0].sum(0)
     assert np.allclose(z, c * (1.0 / r) ** np.arange(9), rtol=1e-14)
     assert abs(abs(z[5] / z[6]) - r) < 1e-12
+
+
+# ------------------------------------------------------------ Pade predictor
+
+def md_coeffs(z, K):
+    """complex128 [m] -> [m][2K] (limbs re, then im; exact doubles)"""
+    out = np.zeros((len(z), 2 * K))
+    out[:, 0], out[:, K] = np.real(z), np.imag(z)
+    return out
+
+
+def md_value(v, K):
+    return v[:K].sum() + 1j * v[K:].sum()
+
+
+@pytest.mark.parametrize("L", [1, 2, 5, 7])
+def test_pade_coefficients_match_scipy(L):
+    """[L/1] numerator and denominator against scipy.interpolate.pade (a
+    library routine that solves the same Toeplitz system): a swapped index
+    or sign in the denominator system or in the numerator sum fails."""
+    from scipy.interpolate import pade
+    from oraclelib import oracle_pade
+    rng = np.random.default_rng(30 + L)
+    c = rng.normal(size=L + 2) + 1j * rng.normal(size=L + 2)
+    cc, x0, val = oracle_pade(1, md_coeffs(c, 1), L, 1, 0.3)
+    p, q = pade(c, 1, L)
+    num = np.concatenate([[x0[0] + 1j * x0[1]], cc[1:L + 1, 0] + 1j * cc[1:L + 1, 1]])
+    assert np.allclose(num, p.coeffs[::-1], rtol=1e-12, atol=1e-13)
+    d = cc[L + 1, 0] + 1j * cc[L + 1, 1]
+    assert np.allclose([d, 1.0], q.coeffs, rtol=1e-12, atol=1e-13)
+    assert abs(md_value(val, 1) - p(0.3) / q(0.3)) < 1e-12 * abs(p(0.3) / q(0.3))
+
+
+@pytest.mark.parametrize("K", [1, 2])
+@pytest.mark.parametrize("L", [1, 4, 6])
+def test_pade_reproduces_a_rational_curve(K, L):
+    """x(t) = (a + b t)/(1 - t/2) with dyadic a, b: its Taylor coefficients
+    are exact doubles, every [L/1] approximant is the function itself, so
+    the prediction at s agrees with the exact quotient (Fractions) to the
+    working precision while the Taylor polynomial of the same coefficients
+    misses by the truncation term."""
+    from fractions import Fraction as F
+    from oraclelib import oracle_pade
+    a, b = 0.75 + 0.5j, 0.25 - 1.0j
+    c = np.array([a] + [(a / 2 + b) * 0.5 ** (k - 1) for k in range(1, L + 2)])
+    s = 0.3
+    cc, x0, val = oracle_pade(K, md_coeffs(c, K), L, 1, s)
+    # d = -1/2 exactly, the numerator is a + b s
+    assert np.array_equal(cc[L + 1], md_coeffs([-0.5], K)[0])
+    assert np.array_equal(cc[1], md_coeffs([b], K)[0])
+    assert not np.any(cc[2:L + 1])
+    fs = F(s)
+    den = 1 - fs / 2
+    want_re = (F(a.real) + F(b.real) * fs) / den
+    want_im = (F(a.imag) + F(b.imag) * fs) / den
+    got_re = sum(F(v) for v in val[:K])
+    got_im = sum(F(v) for v in val[K:])
+    tol = 1e-15 if K == 1 else 1e-30
+    assert abs(float(got_re - want_re)) < tol and abs(float(got_im - want_im)) < tol
+    _, _, tay = oracle_pade(K, md_coeffs(c, K), L + 1, 0, s)
+    assert abs(md_value(tay, K) - complex(float(want_re), float(want_im))) > 1e-4 * (s / 2) ** (L + 1)
+
+
+@pytest.mark.parametrize("K", [1, 2])
+def test_pade_singular_denominator(K):
+    """c_L = 0: the reference's Pade keeps the denominator 1 and takes the
+    Taylor numerator when c_{L+1} = 0 too (a polynomial curve), else a zero
+    numerator (lufac reports the singular 1x1 system)."""
+    from oraclelib import oracle_pade
+    L, s = 3, 0.25
+    poly = np.array([1.0 + 2j, -0.5 + 1j, 0.25, 0.0, 0.0])
+    cc, x0, val = oracle_pade(K, md_coeffs(poly, K), L, 1, s)
+    assert abs(md_value(val, K) - np.polyval(poly[::-1], s)) < 1e-15
+    assert not np.any(cc[L + 1])
+    other = np.array([1.0 + 2j, -0.5 + 1j, 0.25, 0.0, 3.0])
+    cc, x0, val = oracle_pade(K, md_coeffs(other, K), L, 1, s)
+    assert not np.any(val) and not np.any(x0) and not np.any(cc[1:L + 2])
+
+
+@pytest.mark.parametrize("K", [1, 2])
+def test_taylor_predictor_of_lower_degree(K):
+    """pade_dendeg 0 with numerator degree L: the Taylor polynomial of
+    degree L of the series (Horner against numpy's polyval)"""
+    from oraclelib import oracle_pade
+    rng = np.random.default_rng(5)
+    c = rng.normal(size=9) + 1j * rng.normal(size=9)
+    for L in (1, 3, 8):
+        _, _, val = oracle_pade(K, md_coeffs(c, K), L, 0, 0.2)
+        assert abs(md_value(val, K) - np.polyval(c[:L + 1][::-1], 0.2)) < 1e-14
+
+
+def rational_path_homotopy(a, b, r):
+    """(r - t) x - (a + b t) = 0: x(t) = (a + b t)/(r - t), a [1/1]
+    rational curve with a pole at t = r"""
+    from phcpack_amd.homotopy import Homotopy
+    hom = Homotopy(System(1, [[(1.0 + 0j, (1,))]]), System(1, [[(-1.0 + 0j, (0,))]]), 1.0)
+    hom.components = [(np.array([r, -1.0, 0.0], np.complex128), hom.target),
+                      (np.array([a, b, 0.0], np.complex128), hom.start)]
+    return hom
+
+
+@pytest.mark.parametrize("K", [1, 2])
+def test_pade_tracker_on_a_rational_path(K):
+    """On x(t) = (a + b t)/(r - t) the [5/1] predictor is exact: no step is
+    rejected and every corrector converges at its first Newton iteration,
+    where the Taylor predictor of the same series (degree 8, same pole
+    steps) needs more; both end at (a + b)/(r - 1)."""
+    a, b, r = 0.6 + 0.2j, -0.3 + 0.9j, 1.25
+    prog = compile_homotopy(rational_path_homotopy(a, b, r), 8)
+    start = points_to_planes(np.array([[a / r]]), 1)
+    want = (a + b) / (r - 1)
+    tol = 1e-13 if K == 1 else 1e-26
+    outs = {}
+    for name, prm in (("pade", TrackParams(pade_numdeg=5, pade_dendeg=1, endsteps=0)),
+                      ("taylor", TrackParams(endsteps=0))):
+        o = oracle_track(prog, start, K, prm)
+        assert o["status"][0] == 1 and o["nrej"][0] == 0
+        x = planes_to_points(o["endp"], K)[0, 0]
+        assert abs(x - want) < tol * abs(want), name
+        outs[name] = o
+    wp, wt = outs["pade"]["work"][:, 0], outs["taylor"]["work"][:, 0]
+    assert outs["pade"]["nsteps"][0] == outs["taylor"]["nsteps"][0]
+    assert wp[1] == wp[3]  # one Newton iteration per corrector
+    assert wt[1] > wt[3]
