@@ -15,11 +15,16 @@ far below the 1e-24 threshold).
 
 Workload: total-degree homotopy gamma*(1-t)*q + t*p with p = 12 dense
 quadratics in 12 variables (all 91 monomials of degree <= 2, complex
-coefficients on the unit circle), q_i = x_i^2 - c_i, 4096 paths, predictor
-series of degree 8, seed 20261202.  One step = tracking all 4096 paths from
-t = 0 to t = 1 plus the endpoint refinement, inputs resident on the device.
+coefficients on the unit circle), q_i = x_i^2 - c_i, 4096 paths, power
+series of degree 8 at every step, seed 20261202.  Every method (and the CPU
+baseline) predicts the next point with the [7/1] Pade approximant of the
+series per component, the reference's predictor form (its default is [5/1]
+of a degree-8 series); --predictor taylor8 uses the Taylor polynomial.  One
+step = tracking all 4096 paths from t = 0 to t = 1 plus the endpoint
+refinement, inputs resident on the device.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-learned]
+                      [--predictor pade7/1|taylor8|...] [--workload ...]
 """
 import argparse
 import ctypes
@@ -172,7 +177,7 @@ def measured_traffic(kernel):
         return None
 
 
-def cpu_baseline(hom, sols, seconds=15.0):
+def cpu_baseline(hom, sols, seconds=15.0, pade=(0, 0)):
     """The oracle's plain dd tracker on a bounded sample of the paths, all
     host threads (capped at 16)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -181,7 +186,7 @@ def cpu_baseline(hom, sols, seconds=15.0):
     from phcpack_amd.tracker import TrackParams, points_to_planes
     prog = compile_homotopy(hom, 8)
     threads = max(1, min(16, os.cpu_count() or 1))
-    prm = TrackParams(tolend=1e-28)
+    prm = TrackParams(tolend=1e-28, pade_numdeg=pade[0], pade_dendeg=pade[1])
     # calibrate on one path per thread, then size the sample to ~seconds
     t0 = time.perf_counter()
     oracle_track(prog, points_to_planes(sols[:threads]), 2, prm, nthreads=threads)
@@ -196,6 +201,24 @@ def cpu_baseline(hom, sols, seconds=15.0):
     return dict(value=round(m / dt, 3), unit="paths/s", cores=threads, kind="port",
                 sample=f"first {m} of {len(sols)} paths, plain dd tracker (oracle/track.c), "
                        f"{threads} threads, {dt:.1f} s, {ok} succeeded")
+
+
+def parse_predictor(name, deg=8):
+    """'taylorL' -> (L, 0), 'padeL/1' -> (L, 1): TrackParams' pade_numdeg
+    and pade_dendeg (Taylor of the full degree is (0, 0))"""
+    import re
+    m = re.fullmatch(r"taylor(\d+)|pade(\d+)/1", name)
+    if not m:
+        raise SystemExit(f"--predictor {name!r}: expected taylorL or padeL/1")
+    if m.group(1) is not None:
+        L = int(m.group(1))
+        if not 1 <= L <= deg:
+            raise SystemExit(f"--predictor {name!r}: Taylor degree must be in 1..{deg}")
+        return (0 if L == deg else L, 0)
+    L = int(m.group(2))
+    if not 1 <= L <= deg - 1:
+        raise SystemExit(f"--predictor {name!r}: Pade numerator degree must be in 1..{deg - 1}")
+    return (L, 1)
 
 
 def main():
@@ -213,10 +236,15 @@ def main():
                          "(n = 8, 65610 paths), evaluated on 5 held-out systems")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip controller training; the headline is then the plain-rule mixed tracker")
+    ap.add_argument("--predictor", default="pade7/1",
+                    help="tracker predictor of every method (and of the CPU baseline): "
+                         "taylorL (Taylor polynomial of degree L <= 8 of the degree-8 series) "
+                         "or padeL/1 (the [L/1] Pade approximant, L <= 7)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also launch one 256 MiB device copy kernel (HBM counter scale)")
     args = ap.parse_args()
+    args.pade = parse_predictor(args.predictor)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the paths are "
               "independent and would be split evenly across GPUs, but this build measures "
@@ -311,8 +339,9 @@ def run(args):
     h_start = points_to_planes(sols)
     d_start = DeviceArray.from_host(h_start)
     ev = Events(lib)
-    prm_plain = TrackParams(tolend=1e-28)
-    prm_mixed = TrackParams(tolend=1e-13)
+    pnum, pden = args.pade
+    prm_plain = TrackParams(tolend=1e-28, pade_numdeg=pnum, pade_dendeg=pden)
+    prm_mixed = TrackParams(tolend=1e-13, pade_numdeg=pnum, pade_dendeg=pden)
     b_plain = TrackBuffers(npaths, n, 2)
     b_mixed = TrackBuffers(npaths, n, 1)
     b_learn = TrackBuffers(npaths, n, 1)
@@ -416,9 +445,10 @@ def run(args):
         "data": "synthetic",
         "config": {
             "workload": "total-degree homotopy, 12 dense complex quadratics in 12 variables, "
-                        "4096 paths, degree-8 series predictor, seed 20261202",
+                        f"4096 paths, degree-8 series, {args.predictor} predictor, seed 20261202",
             "method": head["name"] + ": FP64 tracking + dd endpoint Newton (qd escalation)",
-            "paths": npaths, "n": n, "series_degree": 8, "seed": SEED,
+            "paths": npaths, "n": n, "series_degree": 8, "predictor": args.predictor,
+            "seed": SEED,
             "residual_tol": RES_TOL,
             "residual_check": "max_i |p_i(x)| at the exact endpoint, quad-double on the GPU",
         },
@@ -452,7 +482,7 @@ def run(args):
         },
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(hom, sols, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(hom, sols, args.cpu_seconds, args.pade)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
@@ -575,7 +605,9 @@ def run_rankdef(args, n=14, rank_deficient=2, seed=SEED):
     bm, bp = TrackBuffers(npaths, n, 1), TrackBuffers(npaths, n, 2)
     ref = dict(x2=DeviceArray((4, npaths, n)), dx=DeviceArray((npaths,)),
                res=DeviceArray((npaths,)), st=DeviceArray((npaths,), np.int32))
-    prm_m, prm_p = TrackParams(tolend=1e-13), TrackParams(tolend=1e-28)
+    pnum, pden = args.pade
+    prm_m = TrackParams(tolend=1e-13, pade_numdeg=pnum, pade_dendeg=pden)
+    prm_p = TrackParams(tolend=1e-28, pade_numdeg=pnum, pade_dendeg=pden)
     tuned = {K_: tune_waves(tr, d_start, 1, npaths, K_, b_, p_)
              for K_, b_, p_ in ((2, bp, prm_p), (1, bm, prm_m))}
     for _ in range(args.warmup):
@@ -603,7 +635,7 @@ def run_rankdef(args, n=14, rank_deficient=2, seed=SEED):
         "data": "synthetic",
         "config": {"workload": "total-degree homotopy, 14 dense complex quadratics, top-degree "
                                f"forms of {rank_deficient} replaced by random combinations of two "
-                               "others, degree-8 series predictor",
+                               f"others, degree-8 series, {args.predictor} predictor",
                    "paths": npaths, "seed": seed, "residual_tol": RES_TOL},
         "mixed": {"ms": round(t_m, 3), "track_ms": round(t_track, 3),
                   "good_paths": int(ok_m.sum()), "tracker_statuses": cls(om["status"]),
@@ -641,7 +673,7 @@ def run_learned(args, n=8, degree=3, ntrain=10, neval=5, seed=SEED):
     from phcpack_amd.tracker import TrackBuffers, Tracker, TrackParams, planes_to_points, points_to_planes
     lib = load()
     ev = Events(lib)
-    prm = TrackParams(tolend=1e-13)
+    prm = TrackParams(tolend=1e-13, pade_numdeg=args.pade[0], pade_dendeg=args.pade[1])
 
     def system(s):
         rng = np.random.default_rng(s)
@@ -715,7 +747,8 @@ def run_learned(args, n=8, degree=3, ntrain=10, neval=5, seed=SEED):
         "data": "synthetic",
         "config": {"workload": f"total-degree homotopies of {n} dense complex cubics (6561 paths "
                                f"each); controller trained on {ntrain} systems, evaluated on "
-                               f"{neval} held-out systems; FP64 tracking + dd endpoint Newton",
+                               f"{neval} held-out systems; FP64 tracking ({args.predictor} "
+                               "predictor) + dd endpoint Newton",
                    "train_seeds": [seed + 100 + i for i in range(ntrain)],
                    "eval_seeds": [seed + 110 + i for i in range(neval)]},
         "plain_rule": {"paths_per_s": round(v_p, 1), "good_paths": tot["good_plain"],

@@ -47,14 +47,18 @@ def _system(n, degree, seed):
     return hom, sols
 
 
+PREDICTORS = [(0, 0), (7, 1)]  # (pade_numdeg, pade_dendeg): Taylor, [7/1] Pade
+
+
+@pytest.mark.parametrize("pade", PREDICTORS)
 @pytest.mark.parametrize("K", [1, 2])
-def test_explore_mode_matches_oracle(K):
+def test_explore_mode_matches_oracle(K, pade):
     from phcpack_amd.controller import StepController, StepLogs
     from phcpack_amd.tracker import Tracker, TrackParams, TrackBuffers, points_to_planes
     from phcpack_amd._lib import DeviceArray, check, load
     hom, sols = _system(3, 2, 9)
     tr = Tracker(hom, deg=8)
-    prm = TrackParams(tolend=1e-13 if K == 1 else 1e-28)
+    prm = TrackParams(tolend=1e-13 if K == 1 else 1e-28, pade_numdeg=pade[0], pade_dendeg=pade[1])
     ctl = StepController()
     logs = StepLogs(len(sols), 128)
     bufs = TrackBuffers(len(sols), 3, K)
@@ -71,13 +75,14 @@ def test_explore_mode_matches_oracle(K):
     assert np.array_equal(gx, ox) and np.array_equal(gc, oc_) and np.array_equal(gy, oy)
 
 
+@pytest.mark.parametrize("pade", PREDICTORS)
 @pytest.mark.parametrize("K", [1, 2])
-def test_learned_mode_matches_oracle(K):
+def test_learned_mode_matches_oracle(K, pade):
     from phcpack_amd.controller import StepController
     from phcpack_amd.tracker import Tracker, TrackParams, points_to_planes
     hom, sols = _system(4, 2, 10)
     tr = Tracker(hom, deg=8)
-    prm = TrackParams(tolend=1e-13 if K == 1 else 1e-28)
+    prm = TrackParams(tolend=1e-13 if K == 1 else 1e-28, pade_numdeg=pade[0], pade_dendeg=pade[1])
     ctl = StepController(seed=3, threshold=0.5)
     g = tr.track(sols, K=K, params=prm, controller=ctl.struct(1))
     oc = OracleController(1, len(sols), ctl=ctl)
