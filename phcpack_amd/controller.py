@@ -245,12 +245,14 @@ def train_controller(trackers, starts, params, K=1, seed=1, cap=256, epochs=30, 
 
 def tune_threshold(ctl, tracker, starts, params, K=1,
                    grid=((0.5, 1.0), (0.7, 1.0), (0.8, 1.0), (0.9, 1.0), (0.7, 1.25),
-                         (0.9, 0.75), (0.9, 1.5), (0.95, 0.75)),
-                   timer=None):
+                         (0.9, 0.75), (0.9, 1.5), (0.95, 0.75), (0.7, 1.5), (0.8, 1.5),
+                         (0.7, 2.0), (0.9, 2.0)),
+                   timer=None, repeats=2):
     """Pick (acceptance-probability threshold, pole cap) with the fewest
-    predictor steps (or the shortest time when `timer(fn)` is given) among
-    those that lose no successful path relative to the plain pole-step rule
-    on a tuning system.  Sets them on ctl; returns (choice or None, table)."""
+    predictor steps (or the shortest time when `timer(fn)` is given, the
+    least of `repeats` runs) among those that lose no successful path
+    relative to the plain pole-step rule on a tuning system.  Sets them on
+    ctl; returns (choice or None, table)."""
     from .tracker import TrackBuffers, points_to_planes
     planes = points_to_planes(starts)
     npaths = planes.shape[1]
@@ -262,7 +264,7 @@ def tune_threshold(ctl, tracker, starts, params, K=1,
         check(load().phc_synchronize())
 
     def measure(c):
-        ms = timer(lambda: run(c)) if timer else None
+        ms = min(timer(lambda: run(c)) for _ in range(repeats)) if timer else None
         if ms is None:
             run(c)
         out = bufs.to_host()
