@@ -17,14 +17,14 @@ Workload: total-degree homotopy gamma*(1-t)*q + t*p with p = 12 dense
 quadratics in 12 variables (all 91 monomials of degree <= 2, complex
 coefficients on the unit circle), q_i = x_i^2 - c_i, 4096 paths, power
 series of degree 8 at every step, seed 20261202.  Every method (and the CPU
-baseline) predicts the next point with the [7/1] Pade approximant of the
+baseline) predicts the next point with the [6/2] Pade approximant of the
 series per component, the reference's predictor form (its default is [5/1]
-of a degree-8 series); --predictor taylor8 uses the Taylor polynomial.  One
+of a degree-8 series); --predictor pade7/1 or taylor8 choose another.  One
 step = tracking all 4096 paths from t = 0 to t = 1 plus the endpoint
 refinement, inputs resident on the device.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-learned]
-                      [--predictor pade7/1|taylor8|...] [--workload ...]
+                      [--predictor pade6/2|pade7/1|taylor8|...] [--workload ...]
 """
 import argparse
 import ctypes
@@ -204,21 +204,21 @@ def cpu_baseline(hom, sols, seconds=15.0, pade=(0, 0)):
 
 
 def parse_predictor(name, deg=8):
-    """'taylorL' -> (L, 0), 'padeL/1' -> (L, 1): TrackParams' pade_numdeg
-    and pade_dendeg (Taylor of the full degree is (0, 0))"""
+    """'taylorL' -> (L, 0), 'padeL/M' -> (L, M), M = 1, 2: TrackParams'
+    pade_numdeg and pade_dendeg (Taylor of the full degree is (0, 0))"""
     import re
-    m = re.fullmatch(r"taylor(\d+)|pade(\d+)/1", name)
+    m = re.fullmatch(r"taylor(\d+)|pade(\d+)/([12])", name)
     if not m:
-        raise SystemExit(f"--predictor {name!r}: expected taylorL or padeL/1")
+        raise SystemExit(f"--predictor {name!r}: expected taylorL, padeL/1 or padeL/2")
     if m.group(1) is not None:
         L = int(m.group(1))
         if not 1 <= L <= deg:
             raise SystemExit(f"--predictor {name!r}: Taylor degree must be in 1..{deg}")
         return (0 if L == deg else L, 0)
-    L = int(m.group(2))
-    if not 1 <= L <= deg - 1:
-        raise SystemExit(f"--predictor {name!r}: Pade numerator degree must be in 1..{deg - 1}")
-    return (L, 1)
+    L, M = int(m.group(2)), int(m.group(3))
+    if not 1 <= L <= deg - M:
+        raise SystemExit(f"--predictor {name!r}: Pade numerator degree must be in 1..{deg - M}")
+    return (L, M)
 
 
 def main():
@@ -236,10 +236,10 @@ def main():
                          "(n = 8, 65610 paths), evaluated on 5 held-out systems")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip controller training; the headline is then the plain-rule mixed tracker")
-    ap.add_argument("--predictor", default="pade7/1",
+    ap.add_argument("--predictor", default="pade6/2",
                     help="tracker predictor of every method (and of the CPU baseline): "
                          "taylorL (Taylor polynomial of degree L <= 8 of the degree-8 series) "
-                         "or padeL/1 (the [L/1] Pade approximant, L <= 7)")
+                         "or padeL/M (the [L/M] Pade approximant, M = 1, 2, L <= 8 - M)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also launch one 256 MiB device copy kernel (HBM counter scale)")

@@ -139,8 +139,8 @@ typedef struct {
    * Pade_Vector, standard_predictor_convolutions.adb:312,958-994;
    * Denominator_System/Pade, Deformations/Curves/standard_rational_approximations.adb):
    * pade_dendeg 0: Taylor polynomial of degree pade_numdeg (0: all deg orders);
-   * pade_dendeg 1: [pade_numdeg/1] Pade approximant per component,
-   * 1 <= pade_numdeg <= deg - 1 (the reference's default is [5/1]). */
+   * pade_dendeg M = 1 or 2: [pade_numdeg/M] Pade approximant per component,
+   * 1 <= pade_numdeg <= deg - M (the reference's default is [5/1]). */
   int pade_numdeg, pade_dendeg;
 } phc_track_params;
 

@@ -443,40 +443,95 @@ static void store_point(const otr_program *P, work_t *w, double *x, int npaths, 
   }
 }
 
-/* [L/1] Pade approximant of one component's series c[0..L+1]
- * (Deformations/Curves/standard_rational_approximations.adb: Denominator_System
- * :7-32 puts c_L into the 1x1 matrix and -c_{L+1} into the right-hand side;
- * Pade :80-117 solves it by lufac/lusolve, dencff = (1, d), and
- * Numerator_Coefficients :34-56 gives c_i + d c_{i-1}; a zero pivot leaves
- * dencff = (1, 0) and a zero numerator unless the right-hand side is zero,
- * when the numerator is the Taylor coefficients).  In place: c_1..c_L become
- * the numerator's, c_{L+1} becomes d; x0 (= c_0, kept apart) is zeroed with
- * the numerator. */
-void otr_pade_coefficients(int K, double *c, int L, double *x0) {
-  double *cl = ZP(c, L), *cn = ZP(c, L + 1);
-  int i;
-  if (cl[0] == 0.0 && cl[K] == 0.0) {
-    if (cn[0] != 0.0 || cn[K] != 0.0) {
+/* [L/M] Pade approximant of one component's series c[0..L+M] (complex,
+ * 2K doubles each), Deformations/Curves/standard_rational_approximations.adb:
+ * Denominator_System (:7-32) sets a(i,j) = c_{L-M+i+j-1} (zero where that
+ * index is below 1) and b_i = -c_{L+i}; Pade (:80-117) factors a by lufac
+ * (Math_Lib/Reduction/standard_complex_linear_solvers.adb:123-178: partial
+ * pivoting on |re| + |im|, multipliers -a_ik/a_kk, info /= 0 on a zero
+ * pivot) and solves by lusolve (:552-583); the denominator coefficients are
+ * the solution reversed, d_i = b_{M+1-i}, and Numerator_Coefficients
+ * (:34-56) gives c_i + sum_{j=1..min(i,M)} d_j c_{i-j}.  A singular system
+ * leaves the denominator 1 and a zero numerator, or the Taylor numerator
+ * when b is zero.  Pivots are compared on the leading limbs.  In place:
+ * c_1..c_L become the numerator's, c_{L+1}..c_{L+M} become d_1..d_M; x0
+ * (= c_0, kept apart) is zeroed with the numerator. */
+/* z += x*y as the reference writes it: the product, then the sum */
+static void cadd_mul(int K, const double *x, const double *y, double *z) {
+  double u[16];
+  tc_mul(K, x, y, u);
+  omd_cadd(K, z, u, z);
+}
+
+void otr_pade_coefficients(int K, double *c, int L, int M, double *x0) {
+  double a[8][8][16], b[8][16], t[16], one[16];
+  int piv[8], info = 0, i, j, k;
+  if (M < 1 || M > 8) return;
+  for (i = 0; i < M; i++) {
+    for (j = 0; j < M; j++) {
+      const int idx = L - M + i + j + 1;
+      if (idx < 1) czero(K, a[i][j]); else ccopy(K, ZP(c, idx), a[i][j]);
+    }
+    ccopy(K, ZP(c, L + i + 1), b[i]);
+    cneg(K, b[i]);
+  }
+  /* lufac */
+  for (k = 0; k < M - 1; k++) {
+    int l = k;
+    double smax = absz(K, a[k][k]);
+    for (i = k + 1; i < M; i++) {
+      const double v = absz(K, a[i][k]);
+      if (v > smax) { l = i; smax = v; }
+    }
+    piv[k] = l;
+    if (smax == 0.0) { info = k + 1; continue; }
+    if (l != k) { ccopy(K, a[l][k], t); ccopy(K, a[k][k], a[l][k]); ccopy(K, t, a[k][k]); }
+    cset(K, one, 1.0, 0.0);
+    omd_cdiv(K, one, a[k][k], t);
+    cneg(K, t);
+    for (i = k + 1; i < M; i++) { double u[16]; tc_mul(K, t, a[i][k], u); ccopy(K, u, a[i][k]); }
+    for (j = k + 1; j < M; j++) {
+      double tj[16];
+      ccopy(K, a[l][j], tj);
+      if (l != k) { ccopy(K, a[k][j], a[l][j]); ccopy(K, tj, a[k][j]); }
+      for (i = k + 1; i < M; i++) cadd_mul(K, tj, a[i][k], a[i][j]);
+    }
+  }
+  piv[M - 1] = M - 1;
+  if (a[M - 1][M - 1][0] == 0.0 && a[M - 1][M - 1][K] == 0.0) info = M;
+  if (info != 0) {
+    int rhs0 = 1;
+    for (i = 0; i < M; i++) rhs0 = rhs0 && b[i][0] == 0.0 && b[i][K] == 0.0;
+    if (!rhs0) {
       for (i = 1; i <= L; i++) czero(K, ZP(c, i));
       czero(K, x0);
     }
-    czero(K, cn);
-  } else {
-    double d[16], t[16];
-    ccopy(K, cn, t);
-    cneg(K, t);
-    omd_cdiv(K, t, cl, d);
-    for (i = L; i >= 1; i--) {
-      tc_mul(K, d, ZP(c, i - 1), t);
-      omd_cadd(K, ZP(c, i), t, ZP(c, i));
-    }
-    ccopy(K, d, cn);
+    for (i = 1; i <= M; i++) czero(K, ZP(c, L + i));
+    return;
   }
+  /* lusolve */
+  for (k = 0; k < M - 1; k++) {
+    const int l = piv[k];
+    ccopy(K, b[l], t);
+    if (l != k) { ccopy(K, b[k], b[l]); ccopy(K, t, b[k]); }
+    for (i = k + 1; i < M; i++) cadd_mul(K, t, a[i][k], b[i]);
+  }
+  for (k = M - 1; k >= 0; k--) {
+    omd_cdiv(K, b[k], a[k][k], t);
+    ccopy(K, t, b[k]);
+    cneg(K, t);
+    for (j = 0; j < k; j++) cadd_mul(K, t, a[j][k], b[j]);
+  }
+  /* numerator, from the top down so that c_{i-j} is still the series' */
+  for (i = L; i >= 1; i--)
+    for (j = 1; j <= (i < M ? i : M); j++) cadd_mul(K, b[M - j], ZP(c, i - j), ZP(c, i));
+  for (i = 1; i <= M; i++) ccopy(K, b[M - i], ZP(c, L + i));
 }
 
 /* predicted component at step s: the numerator (or Taylor polynomial of
  * degree L when pden = 0) by Horner from c_L down, x0 added last, divided
- * by 1 + d s when pden = 1 (Evaluate, standard_rational_approximations.adb:151-205) */
+ * by the denominator 1 + d_1 s + ... + d_M s^M by Horner from d_M down
+ * (Evaluate, standard_rational_approximations.adb:151-205) */
 void otr_pade_evaluate(int K, const double *c, int L, int pden, const double *x0, double s,
                        double *out) {
   double acc[16], tmp[16], p[16];
@@ -488,10 +543,15 @@ void otr_pade_evaluate(int K, const double *c, int L, int pden, const double *x0
   }
   cscale_d(K, acc, s, tmp);
   omd_cadd(K, x0, tmp, p);
-  if (pden == 1) {
+  if (pden >= 1) {
     double one[16];
     cset(K, one, 1.0, 0.0);
-    cscale_d(K, ZP(c, L + 1), s, tmp);
+    ccopy(K, ZP(c, L + pden), acc);
+    for (k = pden - 1; k >= 1; k--) {
+      cscale_d(K, acc, s, tmp);
+      omd_cadd(K, tmp, ZP(c, L + k), acc);
+    }
+    cscale_d(K, acc, s, tmp);
     omd_cadd(K, tmp, one, acc);
     omd_cdiv(K, p, acc, out);
   } else {
@@ -575,8 +635,8 @@ void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const do
         rej0 = rej;
         nnewt0 = nnewt;
       for (j = 0; j < n; j++) ccopy(K, ZP(w.slots, j * D), ZP(w.x0, j));
-      if (pden == 1)
-        for (j = 0; j < n; j++) otr_pade_coefficients(K, ZP(w.slots, j * D), pnum, ZP(w.x0, j));
+      if (pden >= 1)
+        for (j = 0; j < n; j++) otr_pade_coefficients(K, ZP(w.slots, j * D), pnum, pden, ZP(w.x0, j));
       while (!accepted) {
         double tn = t + h, hs, res;
         int ok;

@@ -69,13 +69,13 @@ void otr_eval(const otr_program *P, int K, double t, const double *x, double *va
 /* Taylor coefficients of the solution curve at (x, t): series [2K][n][deg+1] */
 int otr_series(const otr_program *P, int K, double t, const double *x, double *series);
 
-/* [L/1] Pade coefficients of one component's series c[0..L+1] (complex,
- * 2K doubles each) in place: c_1..c_L the numerator's, c_{L+1} the
- * denominator's d; x0 (the constant term, kept apart) is zeroed when the
- * numerator is */
-void otr_pade_coefficients(int K, double *c, int L, double *x0);
+/* [L/M] Pade coefficients (1 <= M <= 8) of one component's series
+ * c[0..L+M] (complex, 2K doubles each) in place: c_1..c_L the numerator's,
+ * c_{L+1}..c_{L+M} the denominator's d_1..d_M; x0 (the constant term, kept
+ * apart) is zeroed when the numerator is */
+void otr_pade_coefficients(int K, double *c, int L, int M, double *x0);
 /* the predicted component at step s: numerator (Taylor polynomial of degree
- * L when pden = 0) with constant term x0, over 1 + d s when pden = 1 */
+ * L when pden = 0) with constant term x0, over 1 + d_1 s + ... + d_pden s^pden */
 void otr_pade_evaluate(int K, const double *c, int L, int pden, const double *x0, double s,
                        double *out);
 

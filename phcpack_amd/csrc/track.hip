@@ -12,8 +12,8 @@
 //                    Bounded_Update (three_way_minima.adb:96-109), and the
 //                    predictor feedback that halves the step while the
 //                    predicted residual exceeds alpha (Predictor_Feedback)
-//   predictor        the Taylor polynomial of the series, or its [L/1] Pade
-//                    approximant per component (Pade_Vector at
+//   predictor        the Taylor polynomial of the series, or its [L/1] or
+//                    [L/2] Pade approximant per component (Pade_Vector at
 //                    standard_predictor_convolutions.adb:312; Denominator_System
 //                    and Pade in standard_rational_approximations.adb)
 //   corrector        Newton steps, step halved on failure down to minstep
@@ -75,7 +75,7 @@ struct DevProgram {
 struct Params {
   int maxsteps, corsteps, endsteps, fabry_offset;
   double maxstep, minstep, pbeta, alpha, tolcor, tolend, divergence;
-  int pnum, pden;  // predictor: numerator degree (>= 1), denominator degree (0 or 1)
+  int pnum, pden;  // predictor: numerator degree (>= 1), denominator degree (0, 1 or 2)
 };
 
 // step-size controller (device view of phc_controller)
@@ -729,9 +729,59 @@ template <int K> __device__ int mlp_choose(const Ctl &ctl, Path<K> &w, const dou
   return best;
 }
 
-// PD: denominator degree of the predictor (0: Taylor polynomial, 1: [L/1]
-// Pade approximant), a template parameter so that the Taylor variant keeps
-// its registers
+// [L/2] Pade coefficients of one component's series c[0..L+2] in place
+// (the reference's Denominator_System, lufac and lusolve for a 2x2 system):
+// a = [c_{L-1} c_L; c_L c_{L+1}] (c_{L-1} taken as zero when L = 1),
+// b = -(c_{L+1}, c_{L+2}); partial pivoting on |re| + |im| of the leading
+// limbs, multiplier -a_10/a_00; the solution reversed is (d_1, d_2), the
+// numerator c_i + d_1 c_{i-1} + d_2 c_{i-2}; d_1, d_2 take the places of
+// c_{L+1}, c_{L+2}.  A zero pivot leaves the denominator 1 and a zero
+// numerator, or the Taylor numerator when b is zero.
+template <int K> __device__ __forceinline__ void pade2_coefficients(cplx<K> *c, int L, cplx<K> *x0) {
+  typedef TC<K> Cx;
+  cplx<K> a00 = (L >= 2) ? c[L - 1] : Cx::zero();
+  cplx<K> a01 = c[L], a10 = c[L], a11 = c[L + 1];
+  cplx<K> b0 = Cx::neg(c[L + 1]), b1 = Cx::neg(c[L + 2]);
+  const double m0 = absz<K>(a00), m1 = absz<K>(a10);
+  const bool sw = m1 > m0;
+  bool sing = (sw ? m1 : m0) == 0.0;
+  if (!sing) {
+    if (sw) { const cplx<K> t = a10; a10 = a00; a00 = t; }
+    const cplx<K> t = Cx::neg(Cx::div(Cx::from(1.0, 0.0), a00));
+    a10 = Cx::mul(t, a10);
+    const cplx<K> tj = sw ? a11 : a01;
+    if (sw) { a11 = a01; a01 = tj; }
+    a11 = Cx::add(a11, Cx::mul(tj, a10));
+  }
+  if (R<K>::hi(a11.re) == 0.0 && R<K>::hi(a11.im) == 0.0) sing = true;
+  if (sing) {
+    const bool rhs0 = R<K>::hi(b0.re) == 0.0 && R<K>::hi(b0.im) == 0.0 &&
+                      R<K>::hi(b1.re) == 0.0 && R<K>::hi(b1.im) == 0.0;
+    if (!rhs0) {
+      for (int i = 1; i <= L; ++i) c[i] = Cx::zero();
+      *x0 = Cx::zero();
+    }
+    c[L + 1] = Cx::zero();
+    c[L + 2] = Cx::zero();
+    return;
+  }
+  if (sw) { const cplx<K> t = b0; b0 = b1; b1 = t; }
+  b1 = Cx::add(b1, Cx::mul(b0, a10));
+  b1 = Cx::div(b1, a11);
+  b0 = Cx::add(b0, Cx::mul(Cx::neg(b1), a01));
+  b0 = Cx::div(b0, a00);
+  for (int i = L; i >= 1; --i) {
+    cplx<K> v = Cx::add(c[i], Cx::mul(b1, c[i - 1]));
+    if (i >= 2) v = Cx::add(v, Cx::mul(b0, c[i - 2]));
+    c[i] = v;
+  }
+  c[L + 1] = b1;
+  c[L + 2] = b0;
+}
+
+// PD: denominator degree of the predictor (0: Taylor polynomial, 1 or 2:
+// [L/PD] Pade approximant), a template parameter so that the Taylor variant
+// keeps its registers
 template <int K, int MW, int PD>
 __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, Params par, int npaths,
                                                    const double *__restrict__ start, int KS,
@@ -829,6 +879,7 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, Params par
           c[L + 1] = d;
         }
       }
+      if (PD == 2 && l < n) pade2_coefficients<K>(w.slots + l * D, par.pnum, w.x0 + l);
       wsync();
       // ---- feedback and corrector; halve the step on failure ----
       bool accepted = false;
@@ -843,8 +894,11 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, Params par
           cplx<K> acc = w.slots[l * D + L];
           for (int k = L - 1; k >= 1; --k) acc = Cx::add(Cx::scale_d(acc, hs), w.slots[l * D + k]);
           cplx<K> p = Cx::add(w.x0[l], Cx::scale_d(acc, hs));
-          if (PD == 1)
-            p = Cx::div(p, Cx::add(Cx::scale_d(w.slots[l * D + L + 1], hs), Cx::from(1.0, 0.0)));
+          if (PD >= 1) {
+            cplx<K> den = w.slots[l * D + L + PD];
+            for (int k = PD - 1; k >= 1; --k) den = Cx::add(Cx::scale_d(den, hs), w.slots[l * D + L + k]);
+            p = Cx::div(p, Cx::add(Cx::scale_d(den, hs), Cx::from(1.0, 0.0)));
+          }
           w.slots[l * D] = p;
         }
         wsync();
@@ -1153,12 +1207,12 @@ int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, d
               "phc_track: fabry_offset must be below the series degree");
   PHC_REQUIRE(prm->maxstep > 0 && prm->minstep > 0 && prm->maxsteps > 0 && prm->corsteps > 0 &&
               prm->endsteps >= 0, "phc_track: step parameters must be positive");
-  PHC_REQUIRE(prm->pade_dendeg == 0 || prm->pade_dendeg == 1,
-              "phc_track: pade_dendeg must be 0 or 1");
+  PHC_REQUIRE(prm->pade_dendeg >= 0 && prm->pade_dendeg <= 2,
+              "phc_track: pade_dendeg must be 0, 1 or 2");
   PHC_REQUIRE(prm->pade_dendeg == 0
                   ? (prm->pade_numdeg >= 0 && prm->pade_numdeg <= tr->P.D - 1)
-                  : (prm->pade_numdeg >= 1 && prm->pade_numdeg <= tr->P.D - 2),
-              "phc_track: pade_numdeg must be in 0..deg (Taylor) or 1..deg-1 (Pade [L/1])");
+                  : (prm->pade_numdeg >= 1 && prm->pade_numdeg <= tr->P.D - 1 - prm->pade_dendeg),
+              "phc_track: pade_numdeg must be in 0..deg (Taylor) or 1..deg-M (Pade [L/M])");
   if (npaths == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   int grid = 0, wv = 0;
@@ -1210,10 +1264,14 @@ int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, d
     if (K == 1 && wv > 8) TK(1, TRACK_MAX_WAVES, 0);
     else if (K == 1) TK(1, 8, 0);
     else TK(2, 8, 0);
-  } else {
+  } else if (par.pden == 1) {
     if (K == 1 && wv > 8) TK(1, TRACK_MAX_WAVES, 1);
     else if (K == 1) TK(1, 8, 1);
     else TK(2, 8, 1);
+  } else {
+    if (K == 1 && wv > 8) TK(1, TRACK_MAX_WAVES, 2);
+    else if (K == 1) TK(1, 8, 2);
+    else TK(2, 8, 2);
   }
 #undef TK
   PHC_HIP(hipGetLastError());

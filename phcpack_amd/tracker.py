@@ -34,7 +34,7 @@ class TrackParams:
     tolend: float = 1e-28
     divergence: float = 1e8
     # predictor: Taylor polynomial (pade_dendeg 0, degree pade_numdeg or all
-    # orders if 0) or the [pade_numdeg/1] Pade approximant (pade_dendeg 1)
+    # orders if 0) or the [pade_numdeg/M] Pade approximant (pade_dendeg M = 1, 2)
     pade_numdeg: int = 0
     pade_dendeg: int = 0
 

@@ -96,7 +96,7 @@ def _track_sigs():
     o.otr_eval.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_double, P_d, P_d, P_d]
     o.otr_series.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_double, P_d, P_d]
     o.otr_series.restype = C.c_int
-    o.otr_pade_coefficients.argtypes = [C.c_int, P_d, C.c_int, P_d]
+    o.otr_pade_coefficients.argtypes = [C.c_int, P_d, C.c_int, C.c_int, P_d]
     o.otr_pade_evaluate.argtypes = [C.c_int, P_d, C.c_int, C.c_int, P_d, C.c_double, P_d]
     return o
 
@@ -166,13 +166,13 @@ def oracle_track(prog, starts_planes, K, params, nthreads=1, paths=None, control
 def oracle_pade(K, c, L, pden, s):
     """The tracker's predictor on one component: c float64 [m][2K] (per
     coefficient its K real limbs, then its K imaginary limbs).  Returns the
-    coefficients after the [L/1] construction (pden 1; unchanged for pden 0),
+    coefficients after the [L/pden] construction (unchanged for pden 0),
     the constant term x0 and the predicted value [2K] at step s."""
     o = _track_sigs()
     c = np.array(c, dtype=np.float64, order="C")
     x0 = c[0].copy()
-    if pden == 1:
-        o.otr_pade_coefficients(K, pd(c), L, pd(x0))
+    if pden >= 1:
+        o.otr_pade_coefficients(K, pd(c), L, pden, pd(x0))
     out = np.zeros(2 * K)
     o.otr_pade_evaluate(K, pd(c), L, pden, pd(x0), s, pd(out))
     return c, x0, out

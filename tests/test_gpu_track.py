@@ -57,9 +57,11 @@ def test_track_matches_oracle_wider(K, n, degree, seed, npaths):
 
 @pytest.mark.parametrize("K,waves", [(1, 12), (1, 8), (2, 8)])
 @pytest.mark.parametrize("n,degree,seed,num,den", [(3, 2, 1, 5, 1), (4, 2, 2, 6, 1),
-                                                   (3, 3, 3, 2, 1), (4, 2, 2, 4, 0)])
+                                                   (3, 3, 3, 2, 1), (4, 2, 2, 4, 0),
+                                                   (3, 2, 1, 6, 2), (4, 2, 2, 4, 2),
+                                                   (3, 3, 3, 1, 2)])
 def test_track_pade_matches_oracle(K, waves, n, degree, seed, num, den):
-    """The [num/1] Pade predictor (den 1) and the Taylor polynomial of
+    """The [num/den] Pade predictor (den 1, 2) and the Taylor polynomial of
     lower degree (den 0): kernel and oracle agree bit for bit, in the 12-
     and 8-wavefront variants of the FP64 kernel and in double-double."""
     from phcpack_amd.tracker import Tracker, TrackParams, points_to_planes
@@ -99,13 +101,13 @@ def test_track_pade_rational_path(K):
 
 
 def test_track_pade_degrees_refused():
-    """denominator degrees other than 0 and 1, and numerator degrees that
-    leave no coefficient for the denominator, are refused"""
+    """denominator degrees other than 0, 1 and 2, and numerator degrees that
+    leave too few coefficients for the denominator, are refused"""
     from phcpack_amd._lib import PhcError
     from phcpack_amd.tracker import Tracker, TrackParams
     p, hom, sols = make(3, 2, 7)
     tr = Tracker(hom, deg=8)
-    for num, den in ((5, 2), (8, 1), (0, 1), (9, 0)):
+    for num, den in ((5, 3), (7, 2), (8, 1), (0, 1), (0, 2), (9, 0)):
         with pytest.raises(PhcError, match="pade"):
             tr.track(sols, K=1, params=TrackParams(pade_numdeg=num, pade_dendeg=den))
 

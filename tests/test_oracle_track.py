@@ -253,6 +253,60 @@ def test_pade_reproduces_a_rational_curve(K, L):
     assert abs(md_value(tay, K) - complex(float(want_re), float(want_im))) > 1e-4 * (s / 2) ** (L + 1)
 
 
+@pytest.mark.parametrize("L,M", [(5, 2), (4, 2), (2, 2), (1, 2), (4, 3)])
+def test_pade_higher_denominators_match_scipy(L, M):
+    """[L/M] through the oracle's lufac/lusolve restatement: the denominator
+    against scipy.interpolate.pade, the numerator against the truncated
+    product of denominator and series (numpy convolution).  For L < M the
+    reference's Denominator_System puts zeros where the series index falls
+    below 1, so c_0 drops out of the denominator system: scipy then gets
+    the series with c_0 = 0 (the numerator keeps c_0)."""
+    from scipy.interpolate import pade
+    from oraclelib import oracle_pade
+    rng = np.random.default_rng(40 + 10 * L + M)
+    c = rng.normal(size=L + M + 1) + 1j * rng.normal(size=L + M + 1)
+    cc, x0, val = oracle_pade(1, md_coeffs(c, 1), L, M, 0.3)
+    cmod = c.copy()
+    if L < M:
+        cmod[0] = 0.0
+    _, q = pade(cmod, M, L)
+    num = np.concatenate([[x0[0] + 1j * x0[1]], cc[1:L + 1, 0] + 1j * cc[1:L + 1, 1]])
+    den = np.concatenate([[1.0], cc[L + 1:L + M + 1, 0] + 1j * cc[L + 1:L + M + 1, 1]])
+    assert np.allclose(den, q.coeffs[::-1], rtol=1e-10, atol=1e-11)
+    assert np.allclose(num, np.convolve(den, c)[:L + 1], rtol=1e-10, atol=1e-11)
+    want = np.polyval(num[::-1], 0.3) / np.polyval(den[::-1], 0.3)
+    assert abs(md_value(val, 1) - want) < 1e-12 * abs(want)
+
+
+@pytest.mark.parametrize("K", [1, 2])
+@pytest.mark.parametrize("L", [2, 3, 5])
+def test_pade_2_reproduces_a_rational_curve(K, L):
+    """x(t) = (a + b t)/((1 - t/2)(1 - t/4)): dyadic Taylor coefficients,
+    denominator 1 - 3t/4 + t^2/8; every [L/2] approximant with L >= 2 is
+    the function itself (at L = 1 the reference's system leaves out c_0)"""
+    from fractions import Fraction as F
+    from oraclelib import oracle_pade
+    ar, ai, br, bi = F(3, 4), F(1, 2), F(1, 4), F(-1)
+    # series of 1/((1-t/2)(1-t/4)) = sum_k (2^{k+1} - 1)/4^k t^k
+    g = [F(2 ** (k + 1) - 1, 4 ** k) for k in range(L + 3)]
+    cre = [ar * g[0]] + [ar * g[k] + br * g[k - 1] for k in range(1, L + 3)]
+    cim = [ai * g[0]] + [ai * g[k] + bi * g[k - 1] for k in range(1, L + 3)]
+    c = np.array([float(x) + 1j * float(y) for x, y in zip(cre, cim)])
+    assert all(F(float(x)) == x for x in cre + cim)  # exact doubles
+    s = 0.3
+    cc, x0, val = oracle_pade(K, md_coeffs(c, K), L, 2, s)
+    tol = 1e-14 if K == 1 else 1e-29
+    # the complex elimination rounds (the 2x2 system of coefficients ~2^-L
+    # is mildly conditioned): d_1 = -3/4, d_2 = 1/8 to working precision
+    for idx, want in ((L + 1, F(-3, 4)), (L + 2, F(1, 8))):
+        assert abs(float(sum(F(v) for v in cc[idx][:K]) - want)) < tol
+        assert abs(float(sum(F(v) for v in cc[idx][K:]))) < tol
+    fs = F(s)
+    den = (1 - fs / 2) * (1 - fs / 4)
+    assert abs(float(sum(F(v) for v in val[:K]) - (ar + br * fs) / den)) < tol
+    assert abs(float(sum(F(v) for v in val[K:]) - (ai + bi * fs) / den)) < tol
+
+
 @pytest.mark.parametrize("K", [1, 2])
 def test_pade_singular_denominator(K):
     """c_L = 0: the reference's Pade keeps the denominator 1 and takes the
@@ -267,6 +321,12 @@ def test_pade_singular_denominator(K):
     other = np.array([1.0 + 2j, -0.5 + 1j, 0.25, 0.0, 3.0])
     cc, x0, val = oracle_pade(K, md_coeffs(other, K), L, 1, s)
     assert not np.any(val) and not np.any(x0) and not np.any(cc[1:L + 2])
+    # [2/2] of a quadratic: the 2x2 system [[c1 c2] [c2 c3]] = [[c1 c2] [c2 0]]
+    # is regular; of a linear series it is singular with a zero right-hand side
+    lin = np.array([1.0 + 2j, -0.5 + 1j, 0.0, 0.0, 0.0])
+    cc, x0, val = oracle_pade(K, md_coeffs(lin, K), 2, 2, s)
+    assert abs(md_value(val, K) - np.polyval(lin[::-1], s)) < 1e-15
+    assert not np.any(cc[3:5])
 
 
 @pytest.mark.parametrize("K", [1, 2])
