@@ -783,12 +783,17 @@ template <int K> __device__ __forceinline__ void pade2_coefficients(cplx<K> *c, 
 // [L/PD] Pade approximant), a template parameter so that the Taylor variant
 // keeps its registers
 template <int K, int MW, int PD>
-__global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, Params par, int npaths,
+__global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, const Params *__restrict__ parp, int npaths,
                                                    const double *__restrict__ start, int KS,
                                                    double *__restrict__ endp, double *tend,
                                                    int *status, int *nsteps_out, int *nrej_out,
-                                                   double *dx_out, int *work, Ctl ctl,
-                                                   int *counter) {
+                                                   double *dx_out, int *work,
+                                                   const Ctl *__restrict__ ctlp, int *counter) {
+  // the step parameters and the controller come through device memory and
+  // are read where used (scalar loads), not held in registers for the whole
+  // kernel as by-value arguments would be
+  const Params &par = *parp;
+  const Ctl &ctl = *ctlp;
   typedef TC<K> Cx;
   extern __shared__ __align__(16) unsigned char lds[];
   const ProgLayout PL(G);
@@ -1016,7 +1021,21 @@ struct phc_tracker {
   DevProgram P;
   std::vector<void *> bufs;
   int *counter = nullptr;
+  void *args = nullptr;  // the kernel's Params and Ctl (one block per tracker)
 };
+
+namespace {
+struct TrackArgs {
+  Params par;
+  Ctl ctl;
+};
+
+// writes the argument block in stream order; the block travels as a kernel
+// argument, captured at launch, so the host copy need not outlive the call
+__global__ void store_args_kernel(TrackArgs a, TrackArgs *dst) {
+  if (threadIdx.x == 0) *dst = a;
+}
+}  // namespace
 
 namespace {
 
@@ -1167,13 +1186,16 @@ int phc_tracker_create(phc_tracker **out, const phc_program *d) {
   rc |= upload<int>(tr, d->jent_ptr, (size_t)d->n * d->n + 1, &P.jent_ptr);
   rc |= upload<int>(tr, d->jterms, d->njterm, &P.jterms);
   rc |= upload<double>(tr, d->weights, (size_t)16 * 3 * d->ncomp, &P.weights);
-  void *c = nullptr;
-  if (rc != 0 || hipMalloc(&c, sizeof(int)) != hipSuccess) {
+  void *c = nullptr, *a = nullptr;
+  if (rc == 0 && hipMalloc(&a, sizeof(TrackArgs)) == hipSuccess) tr->bufs.push_back(a);
+  else a = nullptr;
+  if (rc != 0 || !a || hipMalloc(&c, sizeof(int)) != hipSuccess) {
     phc_tracker_destroy(tr);
     if (rc == 0) phc::set_error("phc_tracker_create: device allocation failed");
     return -1;
   }
   tr->counter = (int *)c;
+  tr->args = a;
   *out = tr;
   return 0;
 }
@@ -1250,6 +1272,15 @@ int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, d
     ctl.pcap = ctlp->pole_cap;
   }
   PHC_HIP(hipMemsetAsync(tr->counter, 0, sizeof(int), st));
+  // the argument block is written in stream order, like the queue counter:
+  // calls on one tracker must not overlap (the same stream, or synchronized)
+  TrackArgs blk;
+  blk.par = par;
+  blk.ctl = ctl;
+  hipLaunchKernelGGL(store_args_kernel, dim3(1), dim3(64), 0, st, blk, (TrackArgs *)tr->args);
+  PHC_HIP(hipGetLastError());
+  const Params *dpar = (const Params *)((char *)tr->args + offsetof(TrackArgs, par));
+  const Ctl *dctl = (const Ctl *)((char *)tr->args + offsetof(TrackArgs, ctl));
   // up to 8 waves: the variant with 256 VGPRs per lane; 9 to 12 waves (FP64
   // tracking only): the variant compiled for three waves per SIMD
 #define TK(KK, MW, PD)                                                                        \
@@ -1257,8 +1288,8 @@ int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, d
     PHC_HIP(hipFuncSetAttribute((const void *)track_kernel<KK, MW, PD>,                      \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));       \
     hipLaunchKernelGGL((track_kernel<KK, MW, PD>), dim3(grid), dim3(64 * wv), lds, st, tr->P, \
-                       par, npaths, start, KS, endp, tend, status, nsteps, nrej, dxlast, work, \
-                       ctl, tr->counter);                                                     \
+                       dpar, npaths, start, KS, endp, tend, status, nsteps, nrej, dxlast, work, \
+                       dctl, tr->counter);                                                    \
   } while (0)
   if (par.pden == 0) {
     if (K == 1 && wv > 8) TK(1, TRACK_MAX_WAVES, 0);
