@@ -208,6 +208,9 @@ int phc_tracker_lds_bytes(const phc_tracker *tracker, int K, size_t *program_byt
  * predictor series computed, 1 Newton iterations, 2 predicted-residual
  * evaluations, 3 corrector calls (whose first Newton iteration reuses the
  * predicted-residual evaluation and only forms the Jacobian).
+ * Calls on one tracker must not overlap (issue them on one stream, or
+ * synchronize between streams): the path queue counter and the block of
+ * step parameters the kernel reads belong to the tracker.
  * Replaces GPU_Path_mult / path_mult (Path/Path_GPU/path_gpu_mult.cu:84-383)
  * behind the gpumanypaths_dd entry (Ada/PHCtoC/gpumanypaths_dd.cpp:18). */
 int phc_track(phc_tracker *tracker, int K, int npaths, const double *start_dev, int KS,
