@@ -93,17 +93,29 @@ int phc_poly_evaldiff(phc_poly *plan, int nbatch, const double *input_dev, doubl
  *     (own storage: coefficient a at soff + (a & smask); smask = -1 for full
  *     series, 2^m - 1 for a ring of the last 2^m orders), adn, sdn (index of
  *     the left factor's / own order-k correction in the correction buffer,
- *     -1 for none), 0.
+ *     -1 for none), left (node number of the left factor: j < n the
+ *     variable x_j, n the constant, n+1+i product i).
  *   lev_ptr [nlev+1]: nodes of level l are lev_ptr[l-1]..lev_ptr[l]-1.
- *   pairs [4*npair]: component r, storage offset, mask, 0 of a node whose
- *     weighted value u = sum_l W_{r,l} node_{k-l} enters the residual.
+ *   pairs [4*npair]: component r, storage offset, mask, node number of a
+ *     node whose weighted value u = sum_l W_{r,l} node_{k-l} enters the
+ *     residual.
  *   seg_ptr [n*G+1]: terms of segment i*G+g of equation i (G a power of 2,
  *     n*G <= 64); term_pair [nterm], term_coef [2][nterm] (real plane,
- *     imaginary plane; coefficients are complex doubles).
+ *     imaginary plane; coefficients are complex doubles).  Terms of one
+ *     monomial (its components) are adjacent in one segment: the mixed
+ *     residual takes the modulus of their sum.
  *   jpairs [2*njpair]: component r, storage offset of node m/x_j.
  *   jent_ptr [n*n+1], jterms [njterm]: Jacobian entry i*n+j sums
  *     coef[term] * e * W_{r,0} * node_0 with jterm = term | jpair << 16 | e << 28.
  *   weights [2][8][3*ncomp]: octo-double coefficients of 1, t, t^2 of w_r.
+ *   hsig [n]: largest singular value of the Hessian of equation i, kept
+ *     from t = 0 for equations of degree <= 2 as Cached_Singular_Values
+ *     (Deformations/Curves/standard_predictor_convolutions.adb:575-646)
+ *     keeps it; -1 for equations whose Hessian is evaluated every step.
+ *   hent_ptr [n*n*(n+1)/2+1], hterms [3*nhterm], hpairs [2*nhpair]: Hessian
+ *     entry (i; a <= b) at i*n*(n+1)/2 + a*n - a*(a-1)/2 + b - a sums
+ *     mult * coef[term] * W_{r,0} * node_0 over hterms (term, hpair, mult),
+ *     hpair = (component r, storage offset of the node x^(m - e_a - e_b)).
  * Replaces the homotopy set-up of gpumanypaths_dd
  * (Ada/PHCtoC/gpumanypaths_dd.cpp:18-57, which reads PHCpack's containers)
  * and the per-call job planning of GPU_dbl2_poly_evaldiff. */
@@ -121,27 +133,65 @@ typedef struct {
   int njterm;
   const int *jent_ptr, *jterms;
   const double *weights;
+  const double *hsig;
+  int nhterm;
+  const int *hent_ptr, *hterms;
+  int nhpair;
+  const int *hpairs;
 } phc_program;
 
 /* Step control of the tracker; defaults of the reference
  * (Deformations/Curves/homotopy_continuation_parameters.adb:11-22):
- * maxstep 0.1, minstep 1e-6, pbeta 0.5, alpha 1e-3, corsteps 4, maxsteps 1000. */
+ * maxstep 0.1, minstep 1e-6, pbeta 0.5, cbeta 0.005, alpha 1e-3,
+ * tolres 1e-8, corsteps 4, maxsteps 1000, [5/1] Pade predictor.
+ * Every step follows LU_Prediction and Predictor_Corrector_Loop
+ * (Deformations/Curves/standard_predictor_convolutions.adb:1142-1185,
+ * predictor_corrector_loops.adb:16-83):
+ *   pole step  pbeta*r, r = |c_{d-1-o}/c_{d-o}| minimized over the
+ *              components whose c_{d-o} is not zero (Is_Zero: both parts
+ *              vanish next to 1), maxstep when all are zero
+ *              (Math_Lib/Circuits/convergence_radius_estimates.adb:28-38,138-163,354-390);
+ *   curvature  (cbeta*eta/nrm)^(1/deg), eta = 2 sigma_min(J) /
+ *              sqrt(sum_i sigma_max(Hessian_i)^2) (Hesse_Pade, Distance,
+ *              :558-572,648-900), nrm the 2-norm of the Pade error
+ *              estimates (Homotopy_Pade_Approximants.Solution_Error);
+ *   step       Minimum(pole, curvature, maxstep) (three_way_minima.adb),
+ *              set to 1 - t when t + step >= 1 - minstep (Bounded_Update);
+ *   feedback   halve the step while the mixed residual of the predicted
+ *              point, mean_i |f_i|/(|f|_i + 1) with |f|_i equation i with
+ *              coefficient moduli at |x| (standard_mixed_residuals.adb:472-484),
+ *              is >= alpha and the step >= minstep (Predictor_Feedback, :958-993);
+ *   corrector  Newton (LU_Newton_Steps, standard_inlined_newton_circuits.adb:
+ *              388-422): converged when the mixed residual and max|dx| are
+ *              both <= tolcor, then one extra step; stops early when both
+ *              grow; on failure the step is halved and the predictor
+ *              re-evaluated, down to minstep.
+ * The build's departures: the point is tracked on with t advanced by the
+ * step at which it was corrected (the reference advances t before the
+ * feedback and corrects at the halved step); a path whose corrector gives up
+ * below minstep ends with status 2 (the reference continues from the
+ * unconverged point); divergence is a status of its own; the series is
+ * computed order by order from the corrected point (the reference runs
+ * series Newton); the singular values are computed by power iteration on
+ * the Gram matrix (see phc_track). */
 typedef struct {
   int maxsteps;     /* steps per path before giving up */
-  int corsteps;     /* Newton iterations per corrector */
-  int endsteps;     /* Newton iterations at t = 1 */
-  int fabry_offset; /* pole radius |c_{d-1-o}/c_{d-o}| */
+  int corsteps;     /* Newton iterations per corrector (plus one extra) */
+  int endsteps;     /* Newton iterations at t = 1 (plus one extra) */
+  int fabry_offset; /* pole radius |c_{d-1-o}/c_{d-o}| (the reference: 2) */
   double maxstep, minstep, pbeta, alpha;
-  double tolcor;    /* corrector: max|dx| <= tolcor*(1+max|x|) */
-  double tolend;    /* endpoint Newton tolerance */
+  double tolcor;    /* corrector: mixed residual and max|dx| <= tolcor */
+  double tolend;    /* the same at t = 1 */
   double divergence;/* path declared diverged when max|x| exceeds this */
   /* predictor (Predictor_Feedback evaluates the Pade approximants of
    * Pade_Vector, standard_predictor_convolutions.adb:312,958-994;
    * Denominator_System/Pade, Deformations/Curves/standard_rational_approximations.adb):
-   * pade_dendeg 0: Taylor polynomial of degree pade_numdeg (0: all deg orders);
-   * pade_dendeg M = 1 or 2: [pade_numdeg/M] Pade approximant per component,
-   * 1 <= pade_numdeg <= deg - M (the reference's default is [5/1]). */
+   * pade_dendeg 0: Taylor polynomial of degree pade_numdeg (0: deg - 2);
+   * pade_dendeg M = 1 or 2: [pade_numdeg/M] Pade approximant per component;
+   * pade_numdeg + pade_dendeg + 2 <= deg, as the reference requires for
+   * its error estimate (standard_predictor_convolutions.ads:34). */
   int pade_numdeg, pade_dendeg;
+  double cbeta;     /* curvature step factor (the reference's beta2) */
 } phc_track_params;
 
 /* Checks every index of a program against its sizes (no device work);
@@ -153,8 +203,9 @@ int phc_program_validate(const phc_program *program);
  * Deformations/Curves/standard_predictor_convolutions.adb:1142-1185 - stays
  * the judge and the fallback: the predicted-residual feedback and the
  * Newton corrector still accept or reject every step, and a rejected step
- * is halved as before).  The controller multiplies the pole step h0 =
- * min(pbeta*r, maxstep) by one of ncand candidates mult[c].  Each step the
+ * is halved as before).  The controller multiplies the step h0 =
+ * Minimum(pole step, curvature step, maxstep) of the reference's rule by
+ * one of ncand candidates mult[c], before Bounded_Update.  Each step the
  * tracker forms PHC_CTL_NFEAT features (log2 of the Fabry radius, of the
  * previous step, t, log2 of the largest series coefficient of orders 1..8,
  * log2 of the last four Newton corrections, rejections of the previous
@@ -207,7 +258,16 @@ int phc_tracker_lds_bytes(const phc_tracker *tracker, int K, size_t *program_byt
  * work is not NULL the work counters work[c*npaths + path] for c = 0
  * predictor series computed, 1 Newton iterations, 2 predicted-residual
  * evaluations, 3 corrector calls (whose first Newton iteration reuses the
- * predicted-residual evaluation and only forms the Jacobian).
+ * predicted-residual evaluation and only forms the Jacobian), 4, 5, 6 the
+ * steps set by the pole step, the curvature step and maxstep (the
+ * reference's nbpole, nbhess, nbmaxm), so work holds 7*npaths ints.
+ * Singular values: sigma_max(M)^2 is the Rayleigh quotient |M v|^2/|v|^2
+ * of v = the column of largest diagonal entry of (M^H M)^64 (the Gram
+ * matrix scaled by its trace and squared six times), in FP64 from the
+ * leading limbs; sigma_min(J) = 1/sigma_max(J^{-1}) with the inverse the
+ * tracker already forms.  The reference's SVD (zsvdc) gives the same
+ * values to rounding when the two largest singular values are apart, and
+ * otherwise a value within a factor (1 + 1e-3 n) of them (tested).
  * Calls on one tracker must not overlap (issue them on one stream, or
  * synchronize between streams): the path queue counter and the block of
  * step parameters the kernel reads belong to the tracker.

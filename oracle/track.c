@@ -2,27 +2,42 @@
  * (test infrastructure only).
  *
  * Restates the series step control of PHCpack's Ada trackers:
- *   Track_One_Path / Predictor_Corrector_Loop
- *     (Deformations/Curves/predictor_corrector_trackers.adb,
- *      predictor_corrector_loops.adb:16-83)
- *   LU_Prediction: pole step pbeta*r with Fabry's ratio (offset 2) and the
- *     predictor feedback halving the step while the predicted residual is
- *     above alpha (standard_predictor_convolutions.adb:1142-1185, Predictor_Feedback)
- *   Convergence_Radius_Estimates.Fabry (Math_Lib/Circuits/convergence_radius_estimates.adb:138-163,354-390)
- *   Bounded_Update (Deformations/Curves/three_way_minima.adb:96-109)
+ *   Track_One_Path / Track_All_Paths (Deformations/Curves/
+ *     predictor_corrector_trackers.adb:29-76,160-245) and
+ *     Predictor_Corrector_Loop (predictor_corrector_loops.adb:16-83);
+ *   LU_Prediction (standard_predictor_convolutions.adb:1142-1185): the pole
+ *     step pbeta*r with Fabry's ratio (Math_Lib/Circuits/
+ *     convergence_radius_estimates.adb:28-38,138-163,354-390), the curvature
+ *     step of Hesse_Pade (:558-572,648-900; Step_Distance in
+ *     series_and_predictors.adb:813-825; Solution_Error in
+ *     homotopy_pade_approximants.adb:558-622), Minimum and Bounded_Update
+ *     (three_way_minima.adb), and Predictor_Feedback (:958-993) on the
+ *     mixed residual (Math_Lib/.../standard_mixed_residuals.adb:472-484);
+ *   the corrector LU_Newton_Steps with mixed residuals
+ *     (standard_inlined_newton_circuits.adb:157-175,388-422);
  * and evaluates the homotopy through the program of phcpack_amd/homotopy.py.
- * Every value is computed with the same operations in the same order as the
- * device kernel: sums over the G segments of an equation and over the G
- * parts of a matrix-vector product combine pairwise as a butterfly does
+ * Departures from the reference: t advances by the step at which the point
+ * was corrected (the reference advances t before the feedback, keeps it
+ * when a step is halved, and lets the next step's series Newton move the
+ * point there); the series is solved order by order from the corrected
+ * point; a corrector that gives up below minstep ends the path (status
+ * minstep); a singular Jacobian ends the path; the step-control quantities
+ * (radius, singular values, residual moduli) are computed in FP64 from the
+ * leading limbs; the singular values come from powers of the Gram matrix
+ * and a Rayleigh quotient (otr_sigma_max_sq), not from an SVD.
+ * Every value is computed with the same operations in the same order as
+ * the device kernel: sums over the G segments of an equation and over the
+ * G parts of a matrix-vector product combine pairwise as a butterfly does
  * ((p0+p1)+(p2+p3) for G = 4), the Jacobian is inverted in place by
  * Gauss-Jordan with partial pivoting (first maximal |re|+|im| of the leading
  * limbs, non-finite values counted as infinite).
  *
  * Parity status: the evaluation (value, Jacobian, Taylor series) is pinned
  * against the reference-order polynomial evaluator (series.c, itself
- * bit-exact with CPU_*_poly_evaldiff) and against closed forms; the step
- * sequence of the Ada tracker cannot be run here (no Ada compiler), so step
- * counts are "parity unpinned" and checked by invariants only.
+ * bit-exact with CPU_*_poly_evaldiff) and against closed forms; the pieces
+ * of the step rule against closed forms and numpy (tests/test_oracle_step.py);
+ * the Ada tracker itself cannot be run here (no Ada compiler), so whole
+ * step sequences are "parity unpinned" and checked by invariants.
  */
 #include <math.h>
 #include <pthread.h>
@@ -40,6 +55,7 @@ enum { ST_RUNNING = 0, ST_SUCCESS = 1, ST_MINSTEP = 2, ST_MAXSTEPS = 3, ST_SINGU
 typedef struct {
   int K;
   double *slots, *u, *uj, *A, *rhs, *x0, *col, *W, *dnode, *part;
+  double *rn, *mr, *H, *SP, *SQ;
   int *perm;
 } work_t;
 
@@ -404,13 +420,19 @@ static int work_alloc(const otr_program *P, int K, work_t *w) {
   w->dnode = calloc((size_t)(P->nfactor > 0 ? P->nfactor : 1), zs);
   w->part = calloc((size_t)(P->G > P->n ? P->G : P->n) + 64, zs);
   w->perm = calloc((size_t)P->n, sizeof(int));
+  w->rn = calloc((size_t)(P->n + 1 + P->nprod), sizeof(double));
+  w->mr = calloc((size_t)P->n, sizeof(double));
+  w->H = calloc((size_t)2 * P->n * P->n, sizeof(double));
+  w->SP = calloc((size_t)2 * P->n * P->n, sizeof(double));
+  w->SQ = calloc((size_t)2 * P->n * P->n, sizeof(double));
   return w->slots && w->u && w->uj && w->A && w->rhs && w->x0 && w->col && w->W && w->dnode &&
-         w->part && w->perm;
+         w->part && w->perm && w->rn && w->mr && w->H && w->SP && w->SQ;
 }
 
 static void work_free(work_t *w) {
   free(w->slots); free(w->u); free(w->uj); free(w->A); free(w->rhs); free(w->x0);
   free(w->col); free(w->W); free(w->dnode); free(w->part); free(w->perm);
+  free(w->rn); free(w->mr); free(w->H); free(w->SP); free(w->SQ);
 }
 
 static void load_point(const otr_program *P, work_t *w, const double *x, int KS, int npaths,
@@ -559,11 +581,246 @@ void otr_pade_evaluate(int K, const double *c, int L, int pden, const double *x0
   }
 }
 
-static int predictor_series(const otr_program *P, work_t *w) {
+/* ---------------------------------------------------- the step rule ---- */
+
+/* x^(1/k) for x >= 0 from IEEE double operations only (host and device
+ * agree bit for bit): log2 from the binary exponent and the series of
+ * ln((1+z)/(1-z)) to z^21, 2^y from the binary exponent and the Taylor
+ * series of e^g to g^16, then one Newton step on r^k = x. */
+double otr_root(double x, int k) {
+  int e = 0, i;
+  double m, z, z2, s, y, fl, g, p, r, rk1;
+  if (x != x) return x;
+  if (!(x > 0.0)) return 0.0;
+  if (x == INFINITY || k == 1) return x;
+  m = 2.0 * frexp(x, &e);
+  z = (m - 1.0) / (m + 1.0);
+  z2 = z * z;
+  s = 1.0 / 21.0;
+  s = 1.0 / 19.0 + z2 * s;
+  s = 1.0 / 17.0 + z2 * s;
+  s = 1.0 / 15.0 + z2 * s;
+  s = 1.0 / 13.0 + z2 * s;
+  s = 1.0 / 11.0 + z2 * s;
+  s = 1.0 / 9.0 + z2 * s;
+  s = 1.0 / 7.0 + z2 * s;
+  s = 1.0 / 5.0 + z2 * s;
+  s = 1.0 / 3.0 + z2 * s;
+  s = 1.0 + z2 * s;
+  y = ((double)(e - 1) + (z * s) * 2.8853900817779268) / (double)k;
+  fl = floor(y);
+  g = (y - fl) * 0.69314718055994531;
+  p = 1.0;
+  for (i = 16; i >= 1; i--) p = 1.0 + (g / (double)i) * p;
+  r = ldexp(p, (int)fl);
+  rk1 = 1.0;
+  for (i = 1; i < k; i++) rk1 = rk1 * r;
+  return r - (rk1 * r - x) / ((double)k * rk1);
+}
+
+/* complex double helpers of the step rule (the device's operation order) */
+static void zmac(double *z, const double *x, const double *y) { /* z += x*y */
+  z[0] = fma(x[0], y[0], z[0]);
+  z[0] = fma(-x[1], y[1], z[0]);
+  z[1] = fma(x[1], y[0], z[1]);
+  z[1] = fma(x[0], y[1], z[1]);
+}
+static void zcjmac(double *z, const double *x, const double *y) { /* z += conj(x)*y */
+  z[0] = fma(x[0], y[0], z[0]);
+  z[0] = fma(x[1], y[1], z[0]);
+  z[1] = fma(-x[1], y[0], z[1]);
+  z[1] = fma(x[0], y[1], z[1]);
+}
+static void zmul(const double *x, const double *y, double *z) {
+  const double re = fma(x[0], y[0], -(x[1] * y[1]));
+  const double im = fma(x[0], y[1], x[1] * y[0]);
+  z[0] = re;
+  z[1] = im;
+}
+static double zrad(double re, double im) { return sqrt(re * re + im * im); }
+
+/* largest eigenvalue of M^H M (sigma_max(M)^2) for M n x n complex,
+ * interleaved (re, im) row-major: the Gram matrix scaled by its trace and
+ * squared six times, then the Rayleigh quotient |M v|^2 / |v|^2 of the
+ * column v of largest diagonal entry.  P and Q are 2*n*n scratch doubles. */
+double otr_sigma_max_sq(int n, const double *M, double *P, double *Q) {
+  int i, j, k, s, jm;
+  double tr, sc, num, den, best;
+  for (i = 0; i < n; i++)
+    for (j = 0; j < n; j++) {
+      double acc[2] = {0.0, 0.0};
+      for (k = 0; k < n; k++) zcjmac(acc, M + 2 * (k * n + i), M + 2 * (k * n + j));
+      P[2 * (i * n + j)] = acc[0];
+      P[2 * (i * n + j) + 1] = acc[1];
+    }
+  tr = 0.0;
+  for (i = 0; i < n; i++) tr = tr + P[2 * (i * n + i)];
+  if (tr != tr) return tr;
+  if (!(tr > 0.0)) return 0.0;
+  for (s = 0; s <= 6; s++) {
+    sc = 1.0 / tr;
+    for (i = 0; i < 2 * n * n; i++) P[i] = P[i] * sc;
+    if (s == 6) break;
+    for (i = 0; i < n; i++)
+      for (j = 0; j < n; j++) {
+        double acc[2] = {0.0, 0.0};
+        for (k = 0; k < n; k++) zmac(acc, P + 2 * (i * n + k), P + 2 * (k * n + j));
+        Q[2 * (i * n + j)] = acc[0];
+        Q[2 * (i * n + j) + 1] = acc[1];
+      }
+    memcpy(P, Q, sizeof(double) * 2 * n * n);
+    tr = 0.0;
+    for (i = 0; i < n; i++) tr = tr + P[2 * (i * n + i)];
+  }
+  jm = 0;
+  best = P[0];
+  for (i = 1; i < n; i++)
+    if (P[2 * (i * n + i)] > best) { best = P[2 * (i * n + i)]; jm = i; }
+  num = 0.0;
+  den = 0.0;
+  for (i = 0; i < n; i++) {
+    double y[2] = {0.0, 0.0};
+    for (k = 0; k < n; k++) zmac(y, M + 2 * (i * n + k), P + 2 * (k * n + jm));
+    num = num + y[0] * y[0] + y[1] * y[1];
+  }
+  for (k = 0; k < n; k++) {
+    const double *v = P + 2 * (k * n + jm);
+    den = den + v[0] * v[0] + v[1] * v[1];
+  }
+  return num / den;
+}
+
+/* leading limbs of an n x n matrix of K-limb complex numbers, interleaved */
+static void lead_matrix(int K, int n, const double *A, double *M) {
+  int e;
+  for (e = 0; e < n * n; e++) {
+    M[2 * e] = ZP(A, e)[0];
+    M[2 * e + 1] = ZP(A, e)[K];
+  }
+}
+
+/* the mixed residual mean_i |f_i| / (|f|_i + 1) at the point in the order-0
+ * slots, after eval_point (rhs = -f, weights set): |f|_i is equation i with
+ * the moduli of its coefficients (the components of a monomial summed
+ * first) evaluated at the moduli of x (Standard_Mixed_Residuals.AbsVal and
+ * Mixed_Residual, standard_mixed_residuals.adb:7-18,472-484; EvalCffRad,
+ * standard_predictor_convolutions.adb:935-956), all in FP64 on the leading
+ * limbs; segment sums meet as a butterfly */
+static double mixed_residual(const otr_program *P, work_t *w) {
+  const int K = w->K, n = P->n, D = P->deg + 1, G = P->G;
+  double *rn = w->rn, mix = 0.0;
+  int i, j, g, t;
+  for (j = 0; j < n; j++) rn[j] = zrad(ZP(w->slots, j * D)[0], ZP(w->slots, j * D)[K]);
+  rn[n] = 1.0;
+  for (i = 0; i < P->nprod; i++) {
+    const int *nd = P->nodes + 8 * i;
+    rn[n + 1 + i] = rn[nd[7]] * rn[nd[2] / D];
+  }
+  for (i = 0; i < n; i++) {
+    double part[64];
+    for (g = 0; g < G; g++) {
+      const int s = i * G + g;
+      double acc = 0.0, gz[2] = {0.0, 0.0};
+      for (t = P->seg_ptr[s]; t < P->seg_ptr[s + 1]; t++) {
+        const int q = P->term_pair[t], m = P->pairs[4 * q + 3], r = P->pairs[4 * q];
+        const double c[2] = {P->term_coef[t], P->term_coef[P->nterm + t]};
+        const double wz[2] = {ZP(w->W, 3 * r)[0], ZP(w->W, 3 * r)[K]};
+        double cw[2];
+        zmul(c, wz, cw);
+        gz[0] = gz[0] + cw[0];
+        gz[1] = gz[1] + cw[1];
+        if (t + 1 == P->seg_ptr[s + 1] || P->pairs[4 * P->term_pair[t + 1] + 3] != m) {
+          acc = acc + zrad(gz[0], gz[1]) * rn[m];
+          gz[0] = gz[1] = 0.0;
+        }
+      }
+      part[g] = acc;
+    }
+    {
+      int mm;
+      for (mm = 1; mm < G; mm <<= 1)
+        for (g = 0; g + mm < G; g += 2 * mm) part[g] = part[g] + part[g + mm];
+    }
+    w->mr[i] = part[0];
+  }
+  for (i = 0; i < n; i++)
+    mix = mix + zrad(ZP(w->rhs, i)[0], ZP(w->rhs, i)[K]) / (w->mr[i] + 1.0);
+  return mix / (double)n;
+}
+
+double otr_mixed_residual(const otr_program *P, int K, double t, const double *x) {
+  work_t w = {0};
+  double mix;
+  if (!work_alloc(P, K, &w)) { work_free(&w); return -1.0; }
+  load_point(P, &w, x, K, 1, 0);
+  set_weights(P, &w, t);
+  eval_point(P, &w, 0);
+  mix = mixed_residual(P, &w);
+  work_free(&w);
+  return mix;
+}
+
+/* sum over the equations of sigma_max(Hessian_i)^2 (Distance's accsum,
+ * standard_predictor_convolutions.adb:558-572): the program's constant for
+ * equations of degree <= 2, else the Hessian at the order-0 point (right
+ * after eval_point, every node at order 0) from the Hessian terms */
+static double hessian_sum(const otr_program *P, work_t *w) {
+  const int K = w->K, n = P->n, nn = n * (n + 1) / 2;
+  double acc = 0.0;
+  int i, a, b, h;
+  for (i = 0; i < n; i++) {
+    double s = P->hsig[i];
+    if (s < 0.0) {
+      for (a = 0; a < n; a++)
+        for (b = a; b < n; b++) {
+          const int e = i * nn + a * n - a * (a - 1) / 2 + (b - a);
+          double z[2] = {0.0, 0.0};
+          for (h = P->hent_ptr[e]; h < P->hent_ptr[e + 1]; h++) {
+            const int term = P->hterms[3 * h], hp = P->hterms[3 * h + 1];
+            const double mult = (double)P->hterms[3 * h + 2];
+            const int r = P->hpairs[2 * hp], off = P->hpairs[2 * hp + 1];
+            const double c[2] = {P->term_coef[term], P->term_coef[P->nterm + term]};
+            const double wz[2] = {ZP(w->W, 3 * r)[0], ZP(w->W, 3 * r)[K]};
+            const double nd[2] = {ZP(w->slots, off)[0], ZP(w->slots, off)[K]};
+            double cw[2], v[2];
+            zmul(c, wz, cw);
+            zmul(cw, nd, v);
+            z[0] = z[0] + mult * v[0];
+            z[1] = z[1] + mult * v[1];
+          }
+          w->H[2 * (a * n + b)] = z[0];
+          w->H[2 * (a * n + b) + 1] = z[1];
+          w->H[2 * (b * n + a)] = z[0];
+          w->H[2 * (b * n + a) + 1] = z[1];
+        }
+      s = sqrt(otr_sigma_max_sq(n, w->H, w->SP, w->SQ));
+    }
+    acc = acc + s * s;
+  }
+  return acc;
+}
+
+double otr_hessian_sum(const otr_program *P, int K, double t, const double *x) {
+  work_t w = {0};
+  double hs2;
+  if (!work_alloc(P, K, &w)) { work_free(&w); return -1.0; }
+  load_point(P, &w, x, K, 1, 0);
+  set_weights(P, &w, t);
+  eval_point(P, &w, 1);
+  hs2 = hessian_sum(P, &w);
+  work_free(&w);
+  return hs2;
+}
+
+/* Taylor coefficients x_1..x_deg at the point in the order-0 slots
+ * (weights set at t); *hs2 the Hessian sum at the point; 0 on a singular
+ * Jacobian, which then stays in A, else A holds its inverse */
+static int predictor_series(const otr_program *P, work_t *w, double *hs2) {
   const int K = w->K, n = P->n, D = P->deg + 1;
   double xk[16 * 64];
   int k, j;
   eval_point(P, w, 1);
+  *hs2 = hessian_sum(P, w);
   if (!gj_invert(P, w)) return 0;
   for (k = 1; k < D; k++) {
     for (j = 0; j < n; j++) czero(K, ZP(w->slots, j * D + k));
@@ -576,131 +833,307 @@ static int predictor_series(const otr_program *P, work_t *w) {
   return 1;
 }
 
-void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const double *start,
-               int KS, double *endp, double *tend, int *status, int *nsteps, int *nrej,
-               double *dxlast, int *work, const otr_params *prm, const otr_controller *ctl) {
-  const int n = P->n, D = P->deg + 1;
+/* the pole radius of Fabry's ratio with the given offset, minimized over
+ * the components (Convergence_Radius_Estimates.Fabry, convergence_radius_
+ * estimates.adb:28-38 Is_Zero, :138-163 per series, :354-390 over the
+ * vector; the first component whose c_kd is not zero starts the minimum),
+ * in FP64 on the leading limbs; returns 1 when every c_kd is zero */
+static int fabry_radius(const otr_program *P, work_t *w, int offset, double *r) {
+  const int K = w->K, D = P->deg + 1, kd = D - 1 - offset;
+  int j, fail = 1;
+  *r = 0.0;
+  for (j = 0; j < P->n; j++) {
+    const double *a = ZP(w->slots, j * D + kd - 1), *b = ZP(w->slots, j * D + kd);
+    const double ar = a[0], ai = a[K], br = b[0], bi = b[K];
+    double nrm, zr, zi, rad;
+    if (br + 1.0 == 1.0 && bi + 1.0 == 1.0) continue;
+    nrm = br * br + bi * bi;
+    zr = (ar * br + ai * bi) / nrm;
+    zi = (ai * br - ar * bi) / nrm;
+    rad = zrad(zr, zi);
+    if (fail) { *r = rad; fail = 0; }
+    else if (rad < *r) *r = rad;
+  }
+  return fail;
+}
+
+/* The [L/M] Pade coefficients of one component in place (as
+ * otr_pade_coefficients; M = 0 keeps the Taylor polynomial) and the error
+ * estimate of Homotopy_Pade_Approximants.Solution_Error_Estimate
+ * (homotopy_pade_approximants.adb:558-568 Threshold_Index, :597-622): top =
+ * the largest i <= L+M+2 with |c_i| > 1e-5 (|.| = |re| + |im|, c_0 = x0),
+ * est = c_top + sum_{i=1..min(M,top)} d_i c_{top-i}, est = num_top - est when
+ * top <= L, est = 1 when top < 0 or |est| < 1e-5; leading limbs in err */
+void otr_pade_error(int K, double *c, int L, int M, double *x0, double *err) {
+  double orig[9][16], res[16], one[16];
+  int top = -1, i;
+  for (i = L + M + 2; i >= 0; i--) {
+    const double v = absz(K, i == 0 ? x0 : ZP(c, i));
+    if (v > 1e-5) { top = i; break; }
+  }
+  for (i = 0; i <= M && i <= top; i++) ccopy(K, (top - i == 0) ? x0 : ZP(c, top - i), orig[i]);
+  if (M >= 1) otr_pade_coefficients(K, c, L, M, x0);
+  cset(K, one, 1.0, 0.0);
+  if (top < 0) {
+    ccopy(K, one, res);
+  } else {
+    ccopy(K, orig[0], res);
+    for (i = 1; i <= M; i++) {
+      if (i > top) break;
+      cadd_mul(K, ZP(c, L + i), orig[i], res);
+    }
+    if (top <= L) omd_csub(K, top == 0 ? x0 : ZP(c, top), res, res);
+  }
+  if (absz(K, res) < 1e-5) ccopy(K, one, res);
+  err[0] = res[0];
+  err[1] = res[K];
+}
+
+/* Newton corrector with the mixed residual (LU_Newton_Steps,
+ * standard_inlined_newton_circuits.adb:388-422, each step as LU_Newton_Step
+ * :157-175): converged when the mixed residual and max|dx| are both <= tol,
+ * then one extra step unless the residual or the update is zero; stops when
+ * both grow.  Returns 1 converged, 0 not, -1 singular Jacobian.
+ * evaluated: the products and the residual at the point are in place. */
+static int lu_newton_steps(const otr_program *P, work_t *w, int maxit, double tol, int extra,
+                           int evaluated, double *dxl, int *count, double *dxh) {
+  const int K = w->K, D = P->deg + 1;
+  double dx[16 * 64], prev_err = 1.0, prev_mix = 1.0;
+  int k, j, cnt = 0;
+  for (k = 1; k <= maxit + extra; k++) {
+    double err, res, mix;
+    ++*count;
+    if (k == 1 && !evaluated) eval_point(P, w, 1); else jacobian(P, w);
+    if (!gj_invert(P, w)) return -1;
+    apply_inverse(P, w, dx);
+    for (j = 0; j < P->n; j++) omd_cadd(K, ZP(w->slots, j * D), ZP(dx, j), ZP(w->slots, j * D));
+    err = max_abs(K, dx, P->n);
+    *dxl = err;
+    if (dxh) { dxh[3] = dxh[2]; dxh[2] = dxh[1]; dxh[1] = dxh[0]; dxh[0] = err; }
+    eval_point(P, w, 0);
+    res = max_abs(K, w->rhs, P->n);
+    mix = mixed_residual(P, w);
+    if (mix <= tol && err <= tol) {
+      if (cnt == extra || res == 0.0 || err == 0.0) return 1;
+      cnt++;
+    } else if (k > 1 && mix > prev_mix && err > prev_err) {
+      return 0;
+    }
+    prev_err = err;
+    prev_mix = mix;
+  }
+  return 0;
+}
+
+/* a + b + c minimum of Three_Way_Minima.Minimum (three_way_minima.adb:49-62) */
+static double minimum3(double a, double b, double c, int *which) {
+  if (a < b) {
+    if (a < c) { *which = 0; return a; }
+    *which = 2;
+    return c;
+  }
+  if (b < c) { *which = 1; return b; }
+  *which = 2;
+  return c;
+}
+
+#define OTR_TRACE 20
+
+/* one path (both attempts of the learned controller); trace (may be NULL)
+ * receives OTR_TRACE doubles per step of the last attempt, at most cap:
+ * t, Fabry radius (-1 when every c_kd is zero), pole step, curvature
+ * step, eta, nrm, Minimum (0 pole, 1 curvature, 2 maxstep), the step
+ * of the rule, the step after the controller and Bounded_Update, the
+ * step at the end, feedback halvings, corrector halvings, last feedback
+ * mixed residual, Newton iterations of the last corrector, accepted (1)
+ * or not, sigma_min(J), Hessian sum, t after the step, candidate */
+static void track_path(const otr_program *P, work_t *w, int npaths, int path, const double *start,
+                       int KS, double *endp, double *tend, int *status, int *nsteps, int *nrej,
+                       double *dxlast, int *work, const otr_params *prm,
+                       const otr_controller *ctl, double *trace, int cap, int *ntrace) {
+  const int K = w->K, n = P->n, D = P->deg + 1;
   const int mode = ctl ? ctl->mode : 0;
   const int pden = prm->pade_dendeg;
-  const int pnum = (pden == 0 && prm->pade_numdeg == 0) ? D - 1 : prm->pade_numdeg;
-  work_t w;
-  int path;
-  if (n > 64 || !work_alloc(P, K, &w)) { work_free(&w); return; }
-  for (path = p0; path < p1; path++) {
-    double t = 0.0, h = prm->maxstep, dxl = 0.0, hprev = prm->maxstep;
-    double dxh[4] = {1.0, 1.0, 1.0, 1.0};
-    int st = ST_RUNNING, steps = 0, rej = 0, j, k, npred = 0, nnewt = 0, nfeval = 0, ncorr = 0;
-    int rejprev = 0, nlog = 0, attempt, pmode = mode;
-    /* a path the learned controller (mode 1) does not finish is tracked
-       again from its start with the plain pole-step rule; the work counters
-       add up both attempts, the other outputs are the last attempt's */
-    for (attempt = 0; attempt < 2; attempt++) {
+  const int pnum = (pden == 0 && prm->pade_numdeg == 0) ? D - 3 : prm->pade_numdeg;
+  double t = 0.0, dxl = 0.0, hprev = prm->maxstep;
+  double dxh[4] = {1.0, 1.0, 1.0, 1.0};
+  int st = ST_RUNNING, steps = 0, rej = 0, j, k, npred = 0, nnewt = 0, nfeval = 0, ncorr = 0;
+  int nbpole = 0, nbhess = 0, nbmaxm = 0;
+  int rejprev = 0, nlog = 0, attempt, pmode = mode;
+  /* a path the learned controller (mode 1) does not finish is tracked
+     again from its start with the plain rule; the work counters add up
+     both attempts, the other outputs are the last attempt's */
+  for (attempt = 0; attempt < 2; attempt++) {
     pmode = (attempt == 0) ? mode : 0;
-    t = 0.0; h = prm->maxstep; dxl = 0.0; hprev = prm->maxstep;
+    t = 0.0; dxl = 0.0; hprev = prm->maxstep;
     for (j = 0; j < 4; j++) dxh[j] = 1.0;
     st = ST_RUNNING; steps = 0; rej = 0; rejprev = 0;
-    load_point(P, &w, start, KS, npaths, path);
+    if (ntrace) *ntrace = 0;
+    load_point(P, w, start, KS, npaths, path);
     while (st == ST_RUNNING && t < 1.0) {
-      const int kd = D - 1 - prm->fabry_offset;
-      double r = INFINITY;
-      int accepted = 0;
-      set_weights(P, &w, t);
+      double r, pole, curv, eta, nrm, hs2, sig1, step, tn, mix = 0.0, rule, step1;
+      double err[128];
+      const double t0 = t;
+      int pfail, which, evaluated, cand = 0, rej0, nnewt0, nfb, accepted = 0, nit = 0;
+      set_weights(P, w, t);
       ++npred;
-      if (!predictor_series(P, &w)) { st = ST_SINGULAR; break; }
-      if (kd >= 1)
-        for (j = 0; j < n; j++) {
-          const double den = absz(K, ZP(w.slots, j * D + kd));
-          if (den > 0.0) r = fmin(r, absz(K, ZP(w.slots, j * D + kd - 1)) / den);
-        }
-      h = (r < INFINITY && r > 0.0) ? fmin(prm->pbeta * r, prm->maxstep) : prm->maxstep;
-      {
-        int cand = 0, rej0, nnewt0, i;
+      if (!predictor_series(P, w, &hs2)) { st = ST_SINGULAR; break; }
+      /* pole step (LU_Prediction, standard_predictor_convolutions.adb:1168-1172) */
+      pfail = fabry_radius(P, w, prm->fabry_offset, &r);
+      pole = pfail ? prm->maxstep : prm->pbeta * r;
+      /* the controller reads the Taylor series (before the Pade
+         coefficients replace it) */
+      if (pmode != 0) {
         double feat[OTR_NFEAT];
-        if (pmode != 0) {
-          feat[0] = otr_lg2((r < INFINITY && r > 0.0) ? r : prm->maxstep);
-          feat[1] = otr_lg2(hprev);
-          feat[2] = t;
-          for (k = 1; k <= 8; k++) {
-            double c = 0.0;
-            for (j = 0; j < n; j++) c = fmax(c, k < D ? absz(K, ZP(w.slots, j * D + k)) : 0.0);
-            feat[2 + k] = otr_lg2(c + 1e-300);
-          }
-          for (i = 0; i < 4; i++) feat[11 + i] = otr_lg2(dxh[i] + 1e-300);
-          feat[15] = (double)rejprev;
-          if (pmode == 1) cand = mlp_choose(ctl, feat);
-          else cand = (int)(mix_hash(ctl->seed, (unsigned)path, (unsigned)(npred - 1)) % (unsigned)ctl->ncand);
-          h *= ctl->mult[cand];
-          if (ctl->pole_cap > 0.0 && r < INFINITY && r > 0.0) h = fmin(h, ctl->pole_cap * r);
+        feat[0] = otr_lg2(pfail ? prm->maxstep : r);
+        feat[1] = otr_lg2(hprev);
+        feat[2] = t;
+        for (k = 1; k <= 8; k++) {
+          double c = 0.0;
+          for (j = 0; j < n; j++) c = fmax(c, k < D ? absz(K, ZP(w->slots, j * D + k)) : 0.0);
+          feat[2 + k] = otr_lg2(c + 1e-300);
         }
-        rej0 = rej;
-        nnewt0 = nnewt;
-      for (j = 0; j < n; j++) ccopy(K, ZP(w.slots, j * D), ZP(w.x0, j));
-      if (pden >= 1)
-        for (j = 0; j < n; j++) otr_pade_coefficients(K, ZP(w.slots, j * D), pnum, pden, ZP(w.x0, j));
-      while (!accepted) {
-        double tn = t + h, hs, res;
-        int ok;
-        if (tn >= 1.0 - prm->minstep) tn = 1.0;
-        hs = tn - t;
-        for (j = 0; j < n; j++)
-          otr_pade_evaluate(K, ZP(w.slots, j * D), pnum, pden, ZP(w.x0, j), hs, ZP(w.slots, j * D));
-        set_weights(P, &w, tn);
-        ++nfeval;
-        eval_point(P, &w, 0);
-        res = max_abs(K, w.rhs, n);
-        ok = res <= prm->alpha * (1.0 + max_abs_x(P, &w));
-        if (ok) {
-          const int nr = newton(P, &w, prm->corsteps, prm->tolcor, &dxl, &nnewt, dxh, 1);
-          ++ncorr;
-          if (nr < 0) { st = ST_SINGULAR; break; }
-          ok = (nr == 1);
-        }
-        if (ok) { t = tn; accepted = 1; hprev = hs; }
-        else {
-          ++rej;
-          h *= 0.5;
-          if (h < prm->minstep) { st = ST_MINSTEP; break; }
+        for (j = 0; j < 4; j++) feat[11 + j] = otr_lg2(dxh[j] + 1e-300);
+        feat[15] = (double)rejprev;
+        if (pmode == 1) cand = mlp_choose(ctl, feat);
+        else cand = (int)(mix_hash(ctl->seed, (unsigned)path, (unsigned)(npred - 1)) % (unsigned)ctl->ncand);
+        if (mode == 2 && nlog < ctl->log_capacity) {
+          for (j = 0; j < OTR_NFEAT; j++)
+            ctl->log_x[((size_t)path * ctl->log_capacity + nlog) * OTR_NFEAT + j] = (float)feat[j];
+          ctl->log_c[(size_t)path * ctl->log_capacity + nlog] = cand;
         }
       }
-        rejprev = rej - rej0;
-        if (mode == 2 && nlog < ctl->log_capacity) {
-          for (i = 0; i < OTR_NFEAT; i++)
-            ctl->log_x[((size_t)path * ctl->log_capacity + nlog) * OTR_NFEAT + i] = (float)feat[i];
-          ctl->log_c[(size_t)path * ctl->log_capacity + nlog] = cand;
-          /* accepted at the first attempt: the Newton iterations it took */
-          ctl->log_y[(size_t)path * ctl->log_capacity + nlog] =
-              (rej == rej0 && st == ST_RUNNING) ? nnewt - nnewt0 : 0;
-          ++nlog;
-        }
+      /* predictor coefficients and their error estimates (Solution_Error) */
+      for (j = 0; j < n; j++) ccopy(K, ZP(w->slots, j * D), ZP(w->x0, j));
+      for (j = 0; j < n; j++)
+        otr_pade_error(K, ZP(w->slots, j * D), pnum, pden, ZP(w->x0, j), err + 2 * j);
+      nrm = 0.0;
+      for (j = 0; j < n; j++) nrm = nrm + err[2 * j] * err[2 * j] + err[2 * j + 1] * err[2 * j + 1];
+      nrm = sqrt(nrm);
+      /* curvature step (Hesse_Pade, Distance, Step_Distance of
+         series_and_predictors.adb:813-825): sigma_min(J) = 1/sigma_max(J^{-1}) */
+      lead_matrix(K, n, w->A, w->H);
+      sig1 = 1.0 / sqrt(otr_sigma_max_sq(n, w->H, w->SP, w->SQ));
+      eta = (2.0 * sig1) / sqrt(hs2);
+      curv = otr_root((prm->cbeta * eta) / nrm, D - 1);
+      rule = minimum3(pole, curv, prm->maxstep, &which);
+      if (which == 0) ++nbpole; else if (which == 1) ++nbhess; else ++nbmaxm;
+      step = rule;
+      if (pmode != 0) {
+        step *= ctl->mult[cand];
+        if (ctl->pole_cap > 0.0 && !pfail && r > 0.0) step = fmin(step, ctl->pole_cap * r);
+      }
+      /* Bounded_Update (three_way_minima.adb:96-109) */
+      if (t + step >= 1.0 - prm->minstep) { step = 1.0 - t; tn = 1.0; }
+      else tn = t + step;
+      step1 = step;
+      rej0 = rej;
+      nnewt0 = nnewt;
+      /* Predictor_Feedback (standard_predictor_convolutions.adb:958-993) */
+      evaluated = 0;
+      nfb = 0;
+      for (;;) {
+        for (j = 0; j < n; j++)
+          otr_pade_evaluate(K, ZP(w->slots, j * D), pnum, pden, ZP(w->x0, j), step, ZP(w->slots, j * D));
+        set_weights(P, w, tn);
+        ++nfeval;
+        eval_point(P, w, 0);
+        mix = mixed_residual(P, w);
+        if (mix < prm->alpha) { evaluated = 1; break; }
+        step = step / 2.0;
+        ++rej;
+        ++nfb;
+        tn = t + step;
+        if (step < prm->minstep) break;
+      }
+      /* corrector (Predictor_Corrector_Loop, predictor_corrector_loops.adb:59-81) */
+      for (;;) {
+        int nr, it0 = nnewt;
+        if (!evaluated) set_weights(P, w, tn);
+        nr = lu_newton_steps(P, w, prm->corsteps, prm->tolcor, 1, evaluated, &dxl, &nnewt, dxh);
+        ++ncorr;
+        nit = nnewt - it0;
+        if (nr < 0) { st = ST_SINGULAR; break; }
+        if (nr == 1) { t = tn; hprev = step; accepted = 1; break; }
+        step = step / 2.0;
+        ++rej;
+        if (step < prm->minstep) { st = ST_MINSTEP; break; }
+        tn = t + step;
+        for (j = 0; j < n; j++)
+          otr_pade_evaluate(K, ZP(w->slots, j * D), pnum, pden, ZP(w->x0, j), step, ZP(w->slots, j * D));
+        evaluated = 0;
+      }
+      if (trace && ntrace && *ntrace < cap) {
+        double *q = trace + (size_t)OTR_TRACE * *ntrace;
+        q[0] = t0;
+        q[1] = pfail ? -1.0 : r;
+        q[2] = pole; q[3] = curv; q[4] = eta; q[5] = nrm; q[6] = which; q[7] = rule;
+        q[8] = step1; q[9] = step; q[10] = nfb; q[11] = rej - rej0 - nfb; q[12] = mix;
+        q[13] = nit; q[14] = accepted; q[15] = sig1; q[16] = hs2; q[17] = t; q[18] = cand;
+        q[19] = 0.0;
+        ++*ntrace;
+      }
+      rejprev = rej - rej0;
+      if (mode == 2 && nlog < ctl->log_capacity) {
+        /* accepted at the first attempt: the Newton iterations it took */
+        ctl->log_y[(size_t)path * ctl->log_capacity + nlog] =
+            (rej == rej0 && st == ST_RUNNING) ? nnewt - nnewt0 : 0;
+        ++nlog;
       }
       if (st != ST_RUNNING) break;
       ++steps;
-      if (max_abs_x(P, &w) > prm->divergence) { st = ST_DIVERGED; break; }
+      if (max_abs_x(P, w) > prm->divergence) { st = ST_DIVERGED; break; }
       if (steps >= prm->maxsteps && t < 1.0) { st = ST_MAXSTEPS; break; }
     }
     if (st == ST_RUNNING) {
       int nr;
-      set_weights(P, &w, 1.0);
-      nr = newton(P, &w, prm->endsteps, prm->tolend, &dxl, &nnewt, NULL, 0);
+      set_weights(P, w, 1.0);
+      nr = lu_newton_steps(P, w, prm->endsteps, prm->tolend, 1, 0, &dxl, &nnewt, NULL);
       st = (nr < 0) ? ST_SINGULAR : ST_SUCCESS;
     }
     if (!(pmode == 1 && st != ST_SUCCESS)) break;
-    }
-    store_point(P, &w, endp, npaths, path);
-    tend[path] = t;
-    status[path] = st;
-    nsteps[path] = steps;
-    nrej[path] = rej;
-    dxlast[path] = dxl;
-    if (work) {
-      work[path] = npred;
-      work[npaths + path] = nnewt;
-      work[2 * npaths + path] = nfeval;
-      work[3 * npaths + path] = ncorr;
-    }
-    if (mode == 2) ctl->log_n[path] = nlog;
   }
+  store_point(P, w, endp, npaths, path);
+  tend[path] = t;
+  status[path] = st;
+  nsteps[path] = steps;
+  nrej[path] = rej;
+  dxlast[path] = dxl;
+  if (work) {
+    work[path] = npred;
+    work[npaths + path] = nnewt;
+    work[2 * npaths + path] = nfeval;
+    work[3 * npaths + path] = ncorr;
+    work[4 * npaths + path] = nbpole;
+    work[5 * npaths + path] = nbhess;
+    work[6 * npaths + path] = nbmaxm;
+  }
+  if (mode == 2) ctl->log_n[path] = nlog;
+}
+
+void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const double *start,
+               int KS, double *endp, double *tend, int *status, int *nsteps, int *nrej,
+               double *dxlast, int *work, const otr_params *prm, const otr_controller *ctl) {
+  work_t w = {0};
+  int path;
+  if (P->n > 64 || !work_alloc(P, K, &w)) { work_free(&w); return; }
+  for (path = p0; path < p1; path++)
+    track_path(P, &w, npaths, path, start, KS, endp, tend, status, nsteps, nrej, dxlast, work,
+               prm, ctl, NULL, 0, NULL);
   work_free(&w);
+}
+
+int otr_track_trace(const otr_program *P, int K, int npaths, int path, const double *start,
+                    int KS, double *endp, double *tend, int *status, int *nsteps, int *nrej,
+                    double *dxlast, int *work, const otr_params *prm, const otr_controller *ctl,
+                    double *trace, int cap) {
+  work_t w = {0};
+  int nt = 0;
+  if (P->n > 64 || !work_alloc(P, K, &w)) { work_free(&w); return -1; }
+  track_path(P, &w, npaths, path, start, KS, endp, tend, status, nsteps, nrej, dxlast, work,
+             prm, ctl, trace, cap, &nt);
+  work_free(&w);
+  return nt;
 }
 
 typedef struct {
@@ -744,7 +1177,7 @@ void otr_track_mt(const otr_program *P, int K, int npaths, const double *start, 
 
 void otr_refine(const otr_program *P, int K, int npaths, double t, const double *xin, int KS,
                 double *xout, int iters, double tol, double *dxlast, double *res, int *status) {
-  work_t w;
+  work_t w = {0};
   int path;
   if (P->n > 64 || !work_alloc(P, K, &w)) { work_free(&w); return; }
   for (path = 0; path < npaths; path++) {
@@ -764,7 +1197,7 @@ void otr_refine(const otr_program *P, int K, int npaths, double t, const double 
 
 void otr_eval(const otr_program *P, int K, double t, const double *x, double *val, double *jac) {
   const int n = P->n;
-  work_t w;
+  work_t w = {0};
   int i, j, e;
   if (!work_alloc(P, K, &w)) { work_free(&w); return; }
   load_point(P, &w, x, K, 1, 0);
@@ -787,12 +1220,15 @@ void otr_eval(const otr_program *P, int K, double t, const double *x, double *va
 
 int otr_series(const otr_program *P, int K, double t, const double *x, double *series) {
   const int n = P->n, D = P->deg + 1;
-  work_t w;
+  work_t w = {0};
   int j, k, i, ok;
   if (P->n > 64 || !work_alloc(P, K, &w)) { work_free(&w); return 0; }
   load_point(P, &w, x, K, 1, 0);
   set_weights(P, &w, t);
-  ok = predictor_series(P, &w);
+  {
+    double hs2;
+    ok = predictor_series(P, &w, &hs2);
+  }
   for (j = 0; j < n; j++)
     for (k = 0; k < D; k++)
       for (i = 0; i < K; i++) {

@@ -22,12 +22,18 @@ typedef struct {
   int njterm;
   const int *jent_ptr, *jterms;
   const double *weights;
+  const double *hsig;
+  int nhterm;
+  const int *hent_ptr, *hterms;
+  int nhpair;
+  const int *hpairs;
 } otr_program;
 
 typedef struct {
   int maxsteps, corsteps, endsteps, fabry_offset;
   double maxstep, minstep, pbeta, alpha, tolcor, tolend, divergence;
   int pade_numdeg, pade_dendeg;
+  double cbeta;
 } otr_params;
 
 /* step-size controller, layout of phc_controller (pointers are host memory) */
@@ -49,6 +55,21 @@ typedef struct {
 void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const double *start,
                int KS, double *endp, double *tend, int *status, int *nsteps, int *nrej,
                double *dxlast, int *work, const otr_params *prm, const otr_controller *ctl);
+
+/* one path with a log of its steps: trace receives 20 doubles per step of
+ * the last attempt (at most cap steps; fields listed at track_path in
+ * track.c); returns the number of steps logged */
+int otr_track_trace(const otr_program *P, int K, int npaths, int path, const double *start,
+                    int KS, double *endp, double *tend, int *status, int *nsteps, int *nrej,
+                    double *dxlast, int *work, const otr_params *prm, const otr_controller *ctl,
+                    double *trace, int cap);
+
+/* pieces of the step rule, exposed for tests */
+double otr_root(double x, int k);
+double otr_sigma_max_sq(int n, const double *M, double *P, double *Q);
+double otr_mixed_residual(const otr_program *P, int K, double t, const double *x);
+double otr_hessian_sum(const otr_program *P, int K, double t, const double *x);
+void otr_pade_error(int K, double *c, int L, int M, double *x0, double *err);
 
 /* same as otr_track over all paths with nthreads POSIX threads */
 void otr_track_mt(const otr_program *P, int K, int npaths, const double *start, int KS,

@@ -57,6 +57,9 @@ class PhcProgram(ctypes.Structure):
         ("njpair", c_int), ("jpairs", P_int),
         ("njterm", c_int), ("jent_ptr", P_int), ("jterms", P_int),
         ("weights", P_double),
+        ("hsig", P_double),
+        ("nhterm", c_int), ("hent_ptr", P_int), ("hterms", P_int),
+        ("nhpair", c_int), ("hpairs", P_int),
     ]
 
 
@@ -66,6 +69,7 @@ class PhcTrackParams(ctypes.Structure):
         ("maxstep", c_double), ("minstep", c_double), ("pbeta", c_double), ("alpha", c_double),
         ("tolcor", c_double), ("tolend", c_double), ("divergence", c_double),
         ("pade_numdeg", c_int), ("pade_dendeg", c_int),
+        ("cbeta", c_double),
     ]
 
 

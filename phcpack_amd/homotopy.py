@@ -21,6 +21,19 @@ coefficient k is sum_a left_a * right_{k-a}; the weighted component values
 u[q] = sum_l W[r,l] * node[m]_{k-l} (W the Taylor coefficients of w_r at the
 current t) feed the residual H_i = sum over terms c * u[q], and the Jacobian
 entries A_ij = sum over terms c * e * W[r,0] * node[m/x_j]_0.
+
+The step rule needs two more things per equation.  The mixed residual
+(standard_mixed_residuals.adb:472-484) evaluates every equation with the
+moduli of its coefficients at |x|: the terms of an equation are grouped by
+monomial (the components of one monomial are adjacent terms of one
+segment), so a group's coefficient sum_r w_r(t) c[r,i,m] is formed before
+its modulus is taken.  The curvature step (Hesse_Pade,
+standard_predictor_convolutions.adb:648-900) needs the largest singular
+value of the Hessian of every equation: for equations of degree <= 2 the
+reference computes it once, at the first point it meets, and keeps it
+(Cached_Singular_Values, :575-646), so here it is computed once on the host
+at t = 0 (where a Hessian of degree <= 2 does not depend on x); equations
+of higher degree get Hessian terms that the tracker evaluates at every step.
 """
 import itertools
 import math
@@ -152,18 +165,27 @@ class Program:
     """Flat tracker program (layout of phc_program in include/phcmd.h).
 
     nodes [8*nprod]: per product node (ordered by DAG level)
-        aoff, amask, boff, soff, smask, adn, sdn, 0
+        aoff, amask, boff, soff, smask, adn, sdn, left
       aoff/amask: storage of the left factor (a variable or a product node);
       boff: series of the right factor (a variable); soff/smask: own storage
       (coefficient a at soff + (a & smask)); adn/sdn: index of the left
-      factor's / own order-k correction in the dnode buffer (-1: none).
-    pairs [4*npair]: component r, node storage off, mask, 0
-    seg_ptr [n*G+1]: terms of segment s = i*G + g (G segments per equation)
+      factor's / own order-k correction in the dnode buffer (-1: none);
+      left: node number of the left factor.
+    pairs [4*npair]: component r, node storage off, mask, node number
+    seg_ptr [n*G+1]: terms of segment s = i*G + g (G segments per equation);
+      the terms of one monomial of an equation are adjacent in one segment
     term_pair [nterm], term_coef [2][nterm] (real, imaginary parts)
     jpairs [2*njpair]: component r, node storage off (order 0)
     jent_ptr [n*n+1]: Jacobian terms of entry i*n+j
     jterms [njterm]: term index | jpair << 16 | exponent << 28
     weights [2][8][3*ncomp]: coefficients of 1, t, t^2 of every w_r
+    hsig [n]: largest singular value of the Hessian of equation i at t = 0
+      when its degree is <= 2, -1 when the tracker evaluates it every step
+    hent_ptr [n*n*(n+1)/2+1]: Hessian terms of entry (i; a, b), a <= b, at
+      index i*n*(n+1)/2 + a*n - a*(a-1)/2 + (b-a)
+    hterms [3*nhterm]: term index, hpair, multiplicity e_a*e_b (a != b) or
+      e_a*(e_a-1) (a = b) of d^2 x^m / dx_a dx_b = mult * x^(m - e_a - e_b)
+    hpairs [2*nhpair]: component r, storage offset of the node x^(m-e_a-e_b)
     """
     n: int
     deg: int
@@ -182,6 +204,10 @@ class Program:
     jent_ptr: np.ndarray
     jterms: np.ndarray
     weights: np.ndarray
+    hsig: np.ndarray = None
+    hent_ptr: np.ndarray = None
+    hterms: np.ndarray = None
+    hpairs: np.ndarray = None
     node_exps: list = None
     var_off: np.ndarray = None
 
@@ -249,6 +275,11 @@ def compile_homotopy(hom, deg, lanes=64):
                 merged[e] = merged.get(e, 0) + c
             for e, c in merged.items():
                 comp_terms.append((r, i, e, complex(c)))
+    # polynomial degree of every equation of the homotopy
+    pdg = [0] * n
+    for _, i, e, c in comp_terms:
+        if c != 0:
+            pdg[i] = max(pdg[i], sum(e))
     monos = sorted({e for _, _, e, _ in comp_terms}, key=lambda e: (sum(e), e))
     for e in monos:
         ensure(e)
@@ -257,6 +288,10 @@ def compile_homotopy(hom, deg, lanes=64):
                 d = list(e)
                 d[j] -= 1
                 ensure(tuple(d))
+    for _, i, e, _ in comp_terms:
+        if pdg[i] > 2:
+            for _, div, _ in _hessian_divisors(e):
+                ensure(div)
     nprod = len(left)
     order = sorted(range(n + 1, n + 1 + nprod), key=lambda i: (level[i], i))
     remap = {i: i for i in range(n + 1)}
@@ -308,8 +343,9 @@ def compile_homotopy(hom, deg, lanes=64):
     for i in range(nprod):
         a, b, sidx = int(L[i]), int(Rt[i]), n + 1 + i
         nodes[i] = (off[a], mask[a], off[b], off[sidx], mask[sidx], dn[a] if a > n else -1,
-                    dn[sidx], 0)
-    # pairs (component, node) and terms, segmented per equation
+                    dn[sidx], a)
+    # pairs (component, node) and terms, segmented per equation; the terms of
+    # one monomial (one per component) are adjacent and never split
     pair_of, pairs = {}, []
     eq_terms = [[] for _ in range(n)]
     term_key = {}
@@ -318,19 +354,24 @@ def compile_homotopy(hom, deg, lanes=64):
         if key not in pair_of:
             pair_of[key] = len(pairs)
             m = node_of[e]
-            pairs.append((r, off[m], mask[m], 0))
+            pairs.append((r, off[m], mask[m], m))
         eq_terms[i].append((pair_of[key], c, (r, i, e)))
     G = segments_per_equation(n, lanes)
     seg_ptr, term_pair, term_cf = [0], [], []
     for i in range(n):
-        terms = eq_terms[i]
-        chunks = np.array_split(np.arange(len(terms)), G)
-        for ch in chunks:
-            for t in ch:
-                q, c, key = terms[t]
-                term_key[key] = len(term_pair)
-                term_pair.append(q)
-                term_cf.append(c)
+        terms = sorted(eq_terms[i], key=lambda tm: node_of[tm[2][2]])  # stable
+        groups = []
+        for tm in terms:
+            if groups and node_of[groups[-1][-1][2][2]] == node_of[tm[2][2]]:
+                groups[-1].append(tm)
+            else:
+                groups.append([tm])
+        for ch in np.array_split(np.arange(len(groups)), G):
+            for gi in ch:
+                for q, c, key in groups[gi]:
+                    term_key[key] = len(term_pair)
+                    term_pair.append(q)
+                    term_cf.append(c)
             seg_ptr.append(len(term_pair))
     # Jacobian: entry (i,j) sums coef[term] * e_j * W_r0 * node(m/x_j)_0
     jpair_of, jpairs = {}, []
@@ -354,6 +395,44 @@ def compile_homotopy(hom, deg, lanes=64):
             jterms.append(t | (q << JT_TERM_BITS) | (ex << (JT_TERM_BITS + JT_PAIR_BITS)))
         jent_ptr.append(len(jterms))
     cf = np.array(term_cf, np.complex128)
+    # Hessians: equations of degree <= 2 get the largest singular value of
+    # their (constant) Hessian at t = 0, the others Hessian terms
+    nn = n * (n + 1) // 2
+    hsig = np.zeros(n)
+    hent = [[] for _ in range(n * nn)]
+    hpair_of, hpairs = {}, []
+    for i in range(n):
+        if pdg[i] <= 1:
+            continue
+        if pdg[i] == 2:
+            Hm = np.zeros((n, n), np.complex128)
+            for r, (w, _) in enumerate(hom.components):
+                for rr, ii, e, c in comp_terms:
+                    if rr != r or ii != i:
+                        continue
+                    for a, b, mult in _hessian_pairs(e):
+                        Hm[a, b] += w[0] * c * mult
+                        if a != b:
+                            Hm[b, a] += w[0] * c * mult
+            hsig[i] = float(np.linalg.svd(Hm, compute_uv=False)[0])
+            continue
+        hsig[i] = -1.0
+    for r, i, e, c in comp_terms:
+        if pdg[i] <= 2:
+            continue
+        for (a, b, mult), (_, div, _) in zip(_hessian_pairs(e), _hessian_divisors(e)):
+            m = node_of[div]
+            key = (r, m)
+            if key not in hpair_of:
+                hpair_of[key] = len(hpairs)
+                hpairs.append((r, off[m]))
+            hent[i * nn + a * n - a * (a - 1) // 2 + (b - a)].append(
+                (term_key[(r, i, e)], hpair_of[key], mult))
+    hent_ptr, hterms = [0], []
+    for lst in hent:
+        for trio in lst:
+            hterms.extend(trio)
+        hent_ptr.append(len(hterms) // 3)
     return Program(
         n=n, deg=deg, ncomp=len(hom.components), wdeg=wdeg, G=G,
         nodes=nodes.reshape(-1), lev_ptr=lev_ptr, nslots=int(acc), nfactor=nfactor,
@@ -362,4 +441,31 @@ def compile_homotopy(hom, deg, lanes=64):
         jpairs=np.array(jpairs, np.int32).reshape(-1), jent_ptr=np.array(jent_ptr, np.int32),
         jterms=np.array(jterms, np.uint32).view(np.int32),
         weights=_cplx_planes(np.concatenate([w for w, _ in hom.components])),
+        hsig=hsig, hent_ptr=np.array(hent_ptr, np.int32), hterms=np.array(hterms, np.int32),
+        hpairs=np.array(hpairs, np.int32).reshape(-1),
         node_exps=node_exps, var_off=off[:n + 1].astype(np.int32))
+
+
+def _hessian_pairs(e):
+    """(a, b, multiplicity) of the nonzero second derivatives
+    d^2 x^e / dx_a dx_b, a <= b"""
+    n = len(e)
+    out = []
+    for a in range(n):
+        for b in range(a, n):
+            if a == b and e[a] >= 2:
+                out.append((a, b, e[a] * (e[a] - 1)))
+            elif a != b and e[a] >= 1 and e[b] >= 1:
+                out.append((a, b, e[a] * e[b]))
+    return out
+
+
+def _hessian_divisors(e):
+    """(a, x^(e - e_a - e_b) exponents, b) for the pairs of _hessian_pairs"""
+    out = []
+    for a, b, _ in _hessian_pairs(e):
+        d = list(e)
+        d[a] -= 1
+        d[b] -= 1
+        out.append((a, tuple(d), b))
+    return out

@@ -33,15 +33,18 @@ class TrackParams:
     tolcor: float = 1e-8
     tolend: float = 1e-28
     divergence: float = 1e8
-    # predictor: Taylor polynomial (pade_dendeg 0, degree pade_numdeg or all
-    # orders if 0) or the [pade_numdeg/M] Pade approximant (pade_dendeg M = 1, 2)
-    pade_numdeg: int = 0
-    pade_dendeg: int = 0
+    # predictor: the [pade_numdeg/M] Pade approximant (pade_dendeg M = 1, 2;
+    # the reference's default [5/1]) or the Taylor polynomial (pade_dendeg 0,
+    # degree pade_numdeg, deg - 2 if 0); pade_numdeg + M + 2 <= deg
+    pade_numdeg: int = 5
+    pade_dendeg: int = 1
+    cbeta: float = 0.005
 
     def to_c(self):
         return PhcTrackParams(self.maxsteps, self.corsteps, self.endsteps, self.fabry_offset,
                               self.maxstep, self.minstep, self.pbeta, self.alpha, self.tolcor,
-                              self.tolend, self.divergence, self.pade_numdeg, self.pade_dendeg)
+                              self.tolend, self.divergence, self.pade_numdeg, self.pade_dendeg,
+                              self.cbeta)
 
 
 def program_struct(prog):
@@ -68,7 +71,9 @@ def program_struct(prog):
         len(prog.term_pair), ia(prog.seg_ptr), ia(prog.term_pair), da(prog.term_coef),
         len(prog.jpairs) // 2, ia(prog.jpairs),
         len(prog.jterms), ia(prog.jent_ptr), ia(prog.jterms),
-        da(prog.weights))
+        da(prog.weights), da(prog.hsig),
+        len(prog.hterms) // 3, ia(prog.hent_ptr), ia(prog.hterms),
+        len(prog.hpairs) // 2, ia(prog.hpairs))
     s._keep = keep
     return s
 
@@ -123,7 +128,7 @@ class TrackBuffers:
         self.nsteps = DeviceArray((npaths,), np.int32)
         self.nrej = DeviceArray((npaths,), np.int32)
         self.dx = DeviceArray((npaths,))
-        self.work = DeviceArray((4, npaths), np.int32)
+        self.work = DeviceArray((7, npaths), np.int32)
 
     def to_host(self):
         return dict(endp=self.endp.to_host(), tend=self.tend.to_host(),

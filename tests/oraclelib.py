@@ -97,6 +97,19 @@ def _track_sigs():
     o.otr_series.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_double, P_d, P_d]
     o.otr_series.restype = C.c_int
     o.otr_pade_coefficients.argtypes = [C.c_int, P_d, C.c_int, C.c_int, P_d]
+    o.otr_pade_error.argtypes = [C.c_int, P_d, C.c_int, C.c_int, P_d, P_d]
+    o.otr_root.argtypes = [C.c_double, C.c_int]
+    o.otr_root.restype = C.c_double
+    o.otr_sigma_max_sq.argtypes = [C.c_int, P_d, P_d, P_d]
+    o.otr_sigma_max_sq.restype = C.c_double
+    o.otr_mixed_residual.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_double, P_d]
+    o.otr_mixed_residual.restype = C.c_double
+    o.otr_hessian_sum.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_double, P_d]
+    o.otr_hessian_sum.restype = C.c_double
+    o.otr_track_trace.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_int, C.c_int, P_d, C.c_int,
+                                  P_d, P_d, P_i, P_i, P_i, P_d, P_i, C.POINTER(PhcTrackParams),
+                                  C.POINTER(PhcController), P_d, C.c_int]
+    o.otr_track_trace.restype = C.c_int
     o.otr_pade_evaluate.argtypes = [C.c_int, P_d, C.c_int, C.c_int, P_d, C.c_double, P_d]
     return o
 
@@ -148,7 +161,7 @@ def oracle_track(prog, starts_planes, K, params, nthreads=1, paths=None, control
     out = dict(endp=np.zeros((2 * K, npaths, n)), tend=np.zeros(npaths),
                status=np.zeros(npaths, np.int32), nsteps=np.zeros(npaths, np.int32),
                nrej=np.zeros(npaths, np.int32), dx=np.zeros(npaths),
-               work=np.zeros((4, npaths), np.int32))
+               work=np.zeros((7, npaths), np.int32))
     p = params.to_c()
     cp = C.byref(controller.struct) if controller is not None else None
     if nthreads > 1:
@@ -161,6 +174,78 @@ def oracle_track(prog, starts_planes, K, params, nthreads=1, paths=None, control
                     pi(out["status"]), pi(out["nsteps"]), pi(out["nrej"]), pd(out["dx"]),
                     pi(out["work"]), C.byref(p), cp)
     return out
+
+
+TRACE_FIELDS = ("t", "r", "pole", "curv", "eta", "nrm", "which", "rule", "step1", "step",
+                "nfb", "ncr", "mix", "nit", "accepted", "sigma1", "hs2", "t_after", "cand",
+                "unused")
+
+
+def oracle_track_trace(prog, starts_planes, K, params, path=0, controller=None, cap=2000):
+    """One path with the oracle's step log: (the outputs as oracle_track's,
+    filled for that path only; per-step dicts with the fields of
+    TRACE_FIELDS)"""
+    import ctypes as C
+    from phcpack_amd.tracker import program_struct
+    o = _track_sigs()
+    ps = program_struct(prog)
+    st = np.ascontiguousarray(starts_planes, dtype=np.float64)
+    KS, npaths, n = st.shape[0] // 2, st.shape[1], st.shape[2]
+    out = dict(endp=np.zeros((2 * K, npaths, n)), tend=np.zeros(npaths),
+               status=np.zeros(npaths, np.int32), nsteps=np.zeros(npaths, np.int32),
+               nrej=np.zeros(npaths, np.int32), dx=np.zeros(npaths),
+               work=np.zeros((7, npaths), np.int32))
+    trace = np.zeros((cap, 20))
+    p = params.to_c()
+    cp = C.byref(controller.struct) if controller is not None else None
+    nt = o.otr_track_trace(C.byref(ps), K, npaths, path, pd(st), KS, pd(out["endp"]),
+                           pd(out["tend"]), pi(out["status"]), pi(out["nsteps"]),
+                           pi(out["nrej"]), pd(out["dx"]), pi(out["work"]), C.byref(p), cp,
+                           pd(trace), cap)
+    steps = [dict(zip(TRACE_FIELDS, row)) for row in trace[:nt]]
+    return out, steps
+
+
+def oracle_root(x, k):
+    return _track_sigs().otr_root(x, k)
+
+
+def oracle_sigma_max(M):
+    """sqrt of the oracle's estimate of the largest eigenvalue of M^H M"""
+    M = np.ascontiguousarray(np.asarray(M, np.complex128))
+    n = M.shape[0]
+    a = np.ascontiguousarray(M.view(np.float64).reshape(-1))
+    P, Q = np.zeros(2 * n * n), np.zeros(2 * n * n)
+    return float(np.sqrt(_track_sigs().otr_sigma_max_sq(n, pd(a), pd(P), pd(Q))))
+
+
+def oracle_mixed_residual(prog, x_planes, K, t):
+    import ctypes as C
+    from phcpack_amd.tracker import program_struct
+    o = _track_sigs()
+    ps = program_struct(prog)
+    x = np.ascontiguousarray(x_planes, dtype=np.float64)
+    return o.otr_mixed_residual(C.byref(ps), K, t, pd(x))
+
+
+def oracle_hessian_sum(prog, x_planes, K, t):
+    import ctypes as C
+    from phcpack_amd.tracker import program_struct
+    o = _track_sigs()
+    ps = program_struct(prog)
+    x = np.ascontiguousarray(x_planes, dtype=np.float64)
+    return o.otr_hessian_sum(C.byref(ps), K, t, pd(x))
+
+
+def oracle_pade_error(K, c, L, M):
+    """c float64 [m][2K] with c[0] the constant term: (coefficients after
+    the [L/M] construction, x0, leading limbs of the error estimate)"""
+    o = _track_sigs()
+    c = np.array(c, dtype=np.float64, order="C")
+    x0 = c[0].copy()
+    err = np.zeros(2)
+    o.otr_pade_error(K, pd(c), L, M, pd(x0), pd(err))
+    return c, x0, complex(err[0], err[1])
 
 
 def oracle_pade(K, c, L, pden, s):

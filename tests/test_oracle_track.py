@@ -354,9 +354,11 @@ def rational_path_homotopy(a, b, r):
 @pytest.mark.parametrize("K", [1, 2])
 def test_pade_tracker_on_a_rational_path(K):
     """On x(t) = (a + b t)/(r - t) the [5/1] predictor is exact: no step is
-    rejected and every corrector converges at its first Newton iteration,
-    where the Taylor predictor of the same series (degree 8, same pole
-    steps) needs more; both end at (a + b)/(r - 1)."""
+    rejected and every corrector converges at its first Newton iteration
+    (then one extra iteration unless the update or residual is exactly
+    zero, as LU_Newton_Steps does), where the Taylor predictor of the same
+    series (degree 6, same pole steps: the equation is linear, so there is
+    no curvature step) needs more; both end at (a + b)/(r - 1)."""
     a, b, r = 0.6 + 0.2j, -0.3 + 0.9j, 1.25
     prog = compile_homotopy(rational_path_homotopy(a, b, r), 8)
     start = points_to_planes(np.array([[a / r]]), 1)
@@ -364,7 +366,7 @@ def test_pade_tracker_on_a_rational_path(K):
     tol = 1e-13 if K == 1 else 1e-26
     outs = {}
     for name, prm in (("pade", TrackParams(pade_numdeg=5, pade_dendeg=1, endsteps=0)),
-                      ("taylor", TrackParams(endsteps=0))):
+                      ("taylor", TrackParams(pade_numdeg=0, pade_dendeg=0, endsteps=0))):
         o = oracle_track(prog, start, K, prm)
         assert o["status"][0] == 1 and o["nrej"][0] == 0
         x = planes_to_points(o["endp"], K)[0, 0]
@@ -372,5 +374,5 @@ def test_pade_tracker_on_a_rational_path(K):
         outs[name] = o
     wp, wt = outs["pade"]["work"][:, 0], outs["taylor"]["work"][:, 0]
     assert outs["pade"]["nsteps"][0] == outs["taylor"]["nsteps"][0]
-    assert wp[1] == wp[3]  # one Newton iteration per corrector
-    assert wt[1] > wt[3]
+    assert wp[3] <= wp[1] <= 2 * wp[3]  # converged at once, plus the extra step
+    assert wt[1] > wp[1]
