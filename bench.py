@@ -17,14 +17,19 @@ Workload: total-degree homotopy gamma*(1-t)*q + t*p with p = 12 dense
 quadratics in 12 variables (all 91 monomials of degree <= 2, complex
 coefficients on the unit circle), q_i = x_i^2 - c_i, 4096 paths, power
 series of degree 8 at every step, seed 20261202.  Every method (and the CPU
-baseline) predicts the next point with the [6/2] Pade approximant of the
-series per component, the reference's predictor form (its default is [5/1]
-of a degree-8 series); --predictor pade7/1 or taylor8 choose another.  One
-step = tracking all 4096 paths from t = 0 to t = 1 plus the endpoint
-refinement, inputs resident on the device.
+baseline) follows the reference's step rule (pole step, Hesse-Pade
+curvature step, mixed-residual feedback and corrector) and predicts the
+next point with the [5/1] Pade approximant of the series per component,
+the reference's default for a degree-8 series; --predictor pade4/2 or
+taylor6 choose another.  One step = tracking all 4096 paths from t = 0 to
+t = 1 plus the endpoint refinement, inputs resident on the device.  The
+line also carries the rank-deficient family (14 quadratics, 16384 paths)
+with the same controller, and the CPU baseline on one core and on all
+host cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-learned]
-                      [--predictor pade6/2|pade7/1|taylor8|...] [--workload ...]
+                      [--no-rankdef] [--predictor pade5/1|pade4/2|taylor6|...]
+                      [--workload ...]
 """
 import argparse
 import ctypes
@@ -177,47 +182,79 @@ def measured_traffic(kernel):
         return None
 
 
-def cpu_baseline(hom, sols, seconds=15.0, pade=(0, 0)):
-    """The oracle's plain dd tracker on a bounded sample of the paths, all
-    host threads (capped at 16)."""
+def host_cpu():
+    """the host's CPU model and the cores this process may use"""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return dict(model=model, nproc=avail, os_cpu_count=os.cpu_count())
+
+
+def cpu_baseline(hom, sols, seconds=15.0, pade=(5, 1)):
+    """The oracle's plain dd tracker (the reference's step rule restated)
+    on a bounded random sample of the paths: first on one core, as the
+    reference's host path runs, then on all usable host cores (capped at
+    16) over independent paths as the Ada Tasking trackers do."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oraclelib import oracle_track
     from phcpack_amd.homotopy import compile_homotopy
     from phcpack_amd.tracker import TrackParams, points_to_planes
     prog = compile_homotopy(hom, 8)
-    threads = max(1, min(16, os.cpu_count() or 1))
+    cpu = host_cpu()
+    threads = max(1, min(16, cpu["nproc"]))
     prm = TrackParams(tolend=1e-28, pade_numdeg=pade[0], pade_dendeg=pade[1])
-    # calibrate on one path per thread, then size the sample to ~seconds
-    t0 = time.perf_counter()
-    oracle_track(prog, points_to_planes(sols[:threads]), 2, prm, nthreads=threads)
-    dt = time.perf_counter() - t0
-    rate = threads / max(dt, 1e-9)
-    m = int(min(len(sols), max(threads, rate * seconds)))
-    m = max(threads, (m // threads) * threads)
-    t0 = time.perf_counter()
-    out = oracle_track(prog, points_to_planes(sols[:m]), 2, prm, nthreads=threads)
-    dt = time.perf_counter() - t0
-    ok = int((out["status"] == 1).sum())
-    return dict(value=round(m / dt, 3), unit="paths/s", cores=threads, kind="port",
-                sample=f"first {m} of {len(sols)} paths, plain dd tracker (oracle/track.c), "
-                       f"{threads} threads, {dt:.1f} s, {ok} succeeded")
+    order = np.random.default_rng(SEED + 7).permutation(len(sols))
+
+    def timed(m, nth):
+        pick = np.sort(order[:m])
+        t0 = time.perf_counter()
+        out = oracle_track(prog, points_to_planes(sols[pick]), 2, prm, nthreads=nth)
+        return time.perf_counter() - t0, int((out["status"] == 1).sum())
+
+    # one core: calibrate on 2 paths, then about a third of the time budget
+    dt, _ = timed(2, 1)
+    m1 = int(min(len(sols), max(2, 2 / max(dt, 1e-9) * seconds / 3)))
+    dt1, ok1 = timed(m1, 1)
+    # all cores: the rest of the budget
+    mt = int(min(len(sols), max(threads, m1 / dt1 * threads * seconds * 2 / 3)))
+    mt = max(threads, (mt // threads) * threads)
+    dtt, okt = timed(mt, threads)
+    return dict(value=round(mt / dtt, 3), unit="paths/s", cores=threads, kind="port",
+                sample=f"random {mt} of {len(sols)} paths (seed {SEED + 7}), plain dd tracker "
+                       f"(oracle/track.c, the reference's step rule), {threads} threads, "
+                       f"{dtt:.1f} s, {okt} succeeded",
+                one_core={"value": round(m1 / dt1, 3), "unit": "paths/s", "cores": 1,
+                          "sample": f"random {m1} paths, {dt1:.1f} s, {ok1} succeeded"},
+                cpu_model=cpu["model"], nproc=cpu["nproc"])
 
 
 def parse_predictor(name, deg=8):
     """'taylorL' -> (L, 0), 'padeL/M' -> (L, M), M = 1, 2: TrackParams'
-    pade_numdeg and pade_dendeg (Taylor of the full degree is (0, 0))"""
+    pade_numdeg and pade_dendeg; L + M + 2 <= deg (the error estimate of
+    the curvature step reads coefficient L + M + 2)"""
     import re
     m = re.fullmatch(r"taylor(\d+)|pade(\d+)/([12])", name)
     if not m:
         raise SystemExit(f"--predictor {name!r}: expected taylorL, padeL/1 or padeL/2")
     if m.group(1) is not None:
         L = int(m.group(1))
-        if not 1 <= L <= deg:
-            raise SystemExit(f"--predictor {name!r}: Taylor degree must be in 1..{deg}")
-        return (0 if L == deg else L, 0)
+        if not 1 <= L <= deg - 2:
+            raise SystemExit(f"--predictor {name!r}: Taylor degree must be in 1..{deg - 2}")
+        return (L, 0)
     L, M = int(m.group(2)), int(m.group(3))
-    if not 1 <= L <= deg - M:
-        raise SystemExit(f"--predictor {name!r}: Pade numerator degree must be in 1..{deg - M}")
+    if not 1 <= L <= deg - 2 - M:
+        raise SystemExit(f"--predictor {name!r}: Pade numerator degree must be in "
+                         f"1..{deg - 2 - M}")
     return (L, M)
 
 
@@ -236,11 +273,13 @@ def main():
                          "(n = 8, 65610 paths), evaluated on 5 held-out systems")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip controller training; the headline is then the plain-rule mixed tracker")
-    ap.add_argument("--predictor", default="pade6/2",
+    ap.add_argument("--predictor", default="pade5/1",
                     help="tracker predictor of every method (and of the CPU baseline): "
-                         "taylorL (Taylor polynomial of degree L <= 8 of the degree-8 series) "
-                         "or padeL/M (the [L/M] Pade approximant, M = 1, 2, L <= 8 - M)")
+                         "padeL/M (the [L/M] Pade approximant, M = 1, 2, L <= 6 - M; the "
+                         "reference's default [5/1]) or taylorL (L <= 6) of the degree-8 series")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-rankdef", action="store_true",
+                    help="leave the rank-deficient n = 14 block out of the headline line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also launch one 256 MiB device copy kernel (HBM counter scale)")
     args = ap.parse_args()
@@ -466,7 +505,7 @@ def run(args):
                                 "tuning_ms": {str(k): {str(w): round(ms, 2) for w, ms in v[1].items()}
                                               for k, v in tuned.items()}},
         "roofline": {
-            "bound": "mfma",
+            "bound": "fp64-valu",
             "achieved": round(achieved, 4),
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
@@ -474,13 +513,17 @@ def run(args):
             "traffic": measured_traffic("track_kernel<1>"),
             "traffic_source": "profiles/tracker_c3_hbm_traffic.md",
             "kernel": "track_kernel<1> (FP64 tracking, " + head["name"] + ")",
-            "note": "FP64 vector ALU; MI355X FP64 peak 78.6 TFLOP/s for both VALU and MFMA; "
-                    "flops counted from the program and per-path work counters "
+            "note": "compute bound on the FP64 vector ALU (no MFMA: the multiple-double "
+                    "arithmetic is scalar); MI355X FP64 peak 78.6 TFLOP/s for both VALU and "
+                    "MFMA; flops counted from the program and per-path work counters "
                     "(phcpack_amd/flops.py, FMA = 2)",
             "flops_per_launch": float(fl_track),
             "kernel_ms": round(head["tt"], 3),
         },
     }
+    # the rank-deficient family (n = 14, 16384 paths) with the same controller
+    if not args.no_rankdef:
+        line["rankdef_n14"] = rankdef_block(args, lib, ev, cstruct)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(hom, sols, args.cpu_seconds, args.pade)
     else:
@@ -580,20 +623,20 @@ def run_qr(args, n=1024, szt=128, K=4, seed=SEED):
     return 0
 
 
-def run_rankdef(args, n=14, rank_deficient=2, seed=SEED):
+def rankdef_block(args, lib, ev, cstruct=None, n=14, rank_deficient=2, seed=SEED):
     """Mixed-precision tracking of a rank-deficient family: 14 dense
     quadratics (16384 total-degree paths) whose last 2 top-degree forms are
     combinations of the top-degree forms of two others, so the leading forms
     are rank-deficient and part of the paths diverge or end at singular
     points.  FP64 tracking + dd endpoint Newton of the finished paths (qd for
-    those dd does not settle) against the plain dd tracker; value = paths/s
-    counting endpoints with max_i |p_i(x)| <= 1e-24 in quad-double."""
-    from phcpack_amd._lib import DeviceArray, check, load
+    those dd does not settle) against the plain dd tracker, and with a
+    learned controller (cstruct) when one is given; paths/s count endpoints
+    with max_i |p_i(x)| <= 1e-24 in quad-double."""
+    from phcpack_amd._lib import DeviceArray, check
     from phcpack_amd.autotune import tune_waves
+    from phcpack_amd.flops import tracker_flops
     from phcpack_amd.homotopy import dense_system, total_degree_homotopy
-    from phcpack_amd.tracker import (TrackBuffers, Tracker, TrackParams, planes_to_points,
-                                     points_to_planes)
-    lib = load()
+    from phcpack_amd.tracker import TrackBuffers, Tracker, TrackParams, planes_to_points, points_to_planes
     rng = np.random.default_rng(seed)
     p = dense_system(n, 2, rng, rank_deficient=rank_deficient)
     hom, sols = total_degree_homotopy(p, rng)
@@ -601,10 +644,13 @@ def run_rankdef(args, n=14, rank_deficient=2, seed=SEED):
     tr = Tracker(hom, deg=8)
     h_start = points_to_planes(sols)
     d_start = DeviceArray.from_host(h_start)
-    ev = Events(lib)
-    bm, bp = TrackBuffers(npaths, n, 1), TrackBuffers(npaths, n, 2)
-    ref = dict(x2=DeviceArray((4, npaths, n)), dx=DeviceArray((npaths,)),
-               res=DeviceArray((npaths,)), st=DeviceArray((npaths,), np.int32))
+    bm, bp, bl = TrackBuffers(npaths, n, 1), TrackBuffers(npaths, n, 2), TrackBuffers(npaths, n, 1)
+
+    def refbufs():
+        return dict(x2=DeviceArray((4, npaths, n)), dx=DeviceArray((npaths,)),
+                    res=DeviceArray((npaths,)), st=DeviceArray((npaths,), np.int32))
+
+    ref, ref_l = refbufs(), refbufs()
     pnum, pden = args.pade
     prm_m = TrackParams(tolend=1e-13, pade_numdeg=pnum, pade_dendeg=pden)
     prm_p = TrackParams(tolend=1e-28, pade_numdeg=pnum, pade_dendeg=pden)
@@ -613,45 +659,77 @@ def run_rankdef(args, n=14, rank_deficient=2, seed=SEED):
     for _ in range(args.warmup):
         run_mixed(tr, d_start, h_start, npaths, bm, ref, ev, prm_m, lib)
         run_plain(tr, d_start, npaths, bp, ev, prm_p)
+        if cstruct is not None:
+            run_mixed(tr, d_start, h_start, npaths, bl, ref_l, ev, prm_m, lib, cstruct)
     check(lib.phc_synchronize())
     rows = [run_mixed(tr, d_start, h_start, npaths, bm, ref, ev, prm_m, lib) for _ in range(args.steps)]
+    rows_l = ([run_mixed(tr, d_start, h_start, npaths, bl, ref_l, ev, prm_m, lib, cstruct)
+               for _ in range(args.steps)] if cstruct is not None else None)
     t_plain = float(np.mean([run_plain(tr, d_start, npaths, bp, ev, prm_p) for _ in range(args.steps)]))
     check(lib.phc_synchronize())
     om, op = bm.to_host(), bp.to_host()
     xm, res_m, ok_m = verify(tr, om, ref, 2)
     xp, res_p, ok_p = verify(tr, op, None, 2)
     cm, cp = planes_to_points(xm, 4), planes_to_points(xp, 4)
-    t_track = float(np.mean([r[0] for r in rows]))
-    t_m = float(np.mean([r[0] + r[1] for r in rows]))
-    v_m, v_p = ok_m.sum() / (t_m / 1e3), ok_p.sum() / (t_plain / 1e3)
+    v_p = ok_p.sum() / (t_plain / 1e3)
     names = ["running", "success", "minstep", "maxsteps", "singular", "diverged"]
-    cls = lambda st: {names[i]: int(c) for i, c in enumerate(np.bincount(st, minlength=6)) if c}
-    line = {
-        "metric": "paths/sec at <=1e-24 endpoint residual, 14 dense quadratics with "
-                  "rank-deficient leading forms (16384 paths), mixed vs plain dd",
-        "value": round(float(v_m), 1), "unit": "paths/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_m, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "total-degree homotopy, 14 dense complex quadratics, top-degree "
-                               f"forms of {rank_deficient} replaced by random combinations of two "
-                               f"others, degree-8 series, {args.predictor} predictor",
-                   "paths": npaths, "seed": seed, "residual_tol": RES_TOL},
-        "mixed": {"ms": round(t_m, 3), "track_ms": round(t_track, 3),
-                  "good_paths": int(ok_m.sum()), "tracker_statuses": cls(om["status"]),
-                  "escalated_paths": int(rows[-1][2]),
-                  "max_good_residual": float(res_m[ok_m].max()) if ok_m.any() else None,
-                  "distinct_solutions": distinct(cm, ok_m),
-                  "steps_per_path": float(om["nsteps"].mean())},
+    cls = lambda st: {names[i]: int(c) for i, c in enumerate(np.bincount(st, minlength=6)) if c}  # noqa: E731
+
+    def method(o, rows_, x, res, ok, c):
+        t_track = float(np.mean([r[0] for r in rows_]))
+        t = float(np.mean([r[0] + r[1] for r in rows_]))
+        fl = tracker_flops(tr.prog, 1, o["work"])
+        return ok.sum() / (t / 1e3), {
+            "ms": round(t, 3), "track_ms": round(t_track, 3),
+            "paths_per_s": round(float(ok.sum() / (t / 1e3)), 1),
+            "good_paths": int(ok.sum()), "tracker_statuses": cls(o["status"]),
+            "escalated_paths": int(rows_[-1][2]), "retracked_paths": int(rows_[-1][3]),
+            "max_good_residual": float(res[ok].max()) if ok.any() else None,
+            "distinct_solutions": distinct(c, ok),
+            "steps_per_path": float(o["nsteps"].mean()),
+            "lost_paths_vs_plain": int(count_lost(cp, ok_p, c, ok)),
+            "fp64_tflops": round(fl / (t_track / 1e3) / 1e12, 3)}
+
+    v_m, blk_m = method(om, rows, xm, res_m, ok_m, cm)
+    blk = {
+        "workload": "total-degree homotopy, 14 dense complex quadratics, top-degree forms of "
+                    f"{rank_deficient} replaced by random combinations of two others, "
+                    f"{npaths} paths, degree-8 series, {args.predictor} predictor, seed {seed}",
+        "mixed": blk_m,
         "plain_dd": {"ms": round(t_plain, 3), "paths_per_s": round(float(v_p), 1),
                      "good_paths": int(ok_p.sum()), "tracker_statuses": cls(op["status"]),
                      "max_good_residual": float(res_p[ok_p].max()) if ok_p.any() else None,
                      "distinct_solutions": distinct(cp, ok_p),
                      "steps_per_path": float(op["nsteps"].mean())},
-        "lost_paths_vs_plain": int(count_lost(cp, ok_p, cm, ok_m)),
         "speedup_mixed_vs_plain": round(float(v_m / v_p), 3) if v_p > 0 else None,
         "waves_per_workgroup": {"plain_dd": tuned[2][0], "fp64": tuned[1][0]},
-        "roofline": None,
+    }
+    v_head, lost = v_m, blk_m["lost_paths_vs_plain"]
+    if rows_l is not None:
+        ol = bl.to_host()
+        xl, res_l, ok_l = verify(tr, ol, ref_l, 2)
+        v_l, blk_l = method(ol, rows_l, xl, res_l, ok_l, planes_to_points(xl, 4))
+        blk["learned_mixed"] = blk_l
+        blk["speedup_learned_vs_plain"] = round(float(v_l / v_p), 3) if v_p > 0 else None
+        v_head, lost = v_l, blk_l["lost_paths_vs_plain"]
+    blk["value"] = round(float(v_head), 1)
+    blk["lost_paths_vs_plain"] = int(lost)
+    return blk
+
+
+def run_rankdef(args):
+    """the rank-deficient family on its own line (plain dd vs mixed)"""
+    from phcpack_amd._lib import load
+    lib = load()
+    blk = rankdef_block(args, lib, Events(lib))
+    line = {
+        "metric": "paths/sec at <=1e-24 endpoint residual, 14 dense quadratics with "
+                  "rank-deficient leading forms (16384 paths), mixed vs plain dd",
+        "value": blk["value"], "unit": "paths/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": blk["mixed"]["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic", "config": {"workload": blk["workload"], "residual_tol": RES_TOL},
+        "rankdef": blk, "roofline": None,
     }
     print(json.dumps(line), flush=True)
     return 0
@@ -767,9 +845,13 @@ def run_learned(args, n=8, degree=3, ntrain=10, neval=5, seed=SEED):
 
 def run_newton(args, nsys=8192, seed=SEED):
     """Batched staggered row Newton in double-double: nsys systems of 8
-    polynomials in 8 variables, 8 monomials of 3 distinct variables each,
-    coefficient series of degree 15 uniform in [-1,1], 8 Newton steps from a
-    random constant start series, one wavefront per system."""
+    polynomials in 8 variables, 8 monomials of 3 distinct variables each
+    with exponents uniform in {1, 2} (the common-factor trick), coefficient
+    series of degree 15 uniform in [-1,1], 8 Newton steps from a random
+    constant start series, one wavefront per system.  Outside the timed
+    region: 16 systems checked bit for bit against the oracle, and the
+    oracle's row Newton timed on one host core (the reference's host path
+    is single-threaded)."""
     from phcpack_amd._lib import DeviceArray, check, load
     from phcpack_amd.newton import RowSystem
     lib = load()
@@ -781,7 +863,8 @@ def run_newton(args, nsys=8192, seed=SEED):
     nvr = np.full(nmono, nvar, np.int32)
     idx = np.concatenate([np.sort(rng.choice(dim, nvar, replace=False)) for _ in range(nmono)]).astype(np.int32)
     idxptr = np.concatenate([[0], np.cumsum(nvr)]).astype(np.int32)
-    sysd = RowSystem(dim, monoptr, nvr, idxptr, idx)
+    exps = rng.choice(np.array([1, 2], np.int32), size=len(idx)).astype(np.int32)
+    sysd = RowSystem(dim, monoptr, nvr, idxptr, idx, exps)
     cst = np.zeros((K, nsys, dim, D))
     cff = np.zeros((K, nsys, nmono, D))
     cst[0] = rng.uniform(-1, 1, (nsys, dim, D))
@@ -806,14 +889,31 @@ def run_newton(args, nsys=8192, seed=SEED):
     check(lib.phc_synchronize())
     st = steps.to_host()
     t = float(np.mean(ms))
+    xg = dx.to_host()
+    # parity and the CPU baseline (outside the timed region)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oraclelib import RowSystem as ORowSystem, oracle_row_newton
+    nchk, bad = 16, 0
+    t0 = time.perf_counter()
+    for b in range(nchk):
+        osys = ORowSystem(dim, deg, monoptr, nvr, idxptr, idx, np.ascontiguousarray(cst[:, b]),
+                          np.ascontiguousarray(cff[:, b]), exps)
+        o = oracle_row_newton(K, osys, np.ascontiguousarray(x0[:, b]), 8)
+        bad += int(not np.array_equal(o["x"], xg[:, b]))
+    cpu_s = (time.perf_counter() - t0) / nchk
     line = {
-        "metric": "dd row Newton systems/sec (8 polynomials, 8 monomials of 3 variables, degree 15, 8 steps)",
+        "metric": "dd row Newton systems/sec (8 polynomials, 8 monomials of 3 variables with "
+                  "exponents in {1,2}, degree 15, 8 steps)",
         "value": round(nsys / (t / 1e3), 1), "unit": "systems/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(t, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64 (double-double)", "data": "synthetic",
         "config": {"workload": f"{nsys} independent systems, one wavefront each", "seed": seed},
         "newton_steps_per_s": round(float(st.sum()) / (t / 1e3), 1),
+        "parity": {"systems_checked": nchk, "bit_identical_to_oracle": nchk - bad},
         "roofline": None,
+        "cpu_baseline": {"value": round(1.0 / cpu_s, 2), "unit": "systems/s", "cores": 1,
+                         "kind": "port", "sample": f"{nchk} of the {nsys} systems, the oracle's "
+                         "row Newton (bit-identical to the compiled reference)"},
     }
     print(json.dumps(line), flush=True)
     return 0

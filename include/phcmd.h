@@ -260,7 +260,9 @@ int phc_tracker_lds_bytes(const phc_tracker *tracker, int K, size_t *program_byt
  * evaluations, 3 corrector calls (whose first Newton iteration reuses the
  * predicted-residual evaluation and only forms the Jacobian), 4, 5, 6 the
  * steps set by the pole step, the curvature step and maxstep (the
- * reference's nbpole, nbhess, nbmaxm), so work holds 7*npaths ints.
+ * reference's nbpole, nbhess, nbmaxm), 7 Newton runs whose first iteration
+ * evaluates the point (corrector retries and the Newton at t = 1), so work
+ * holds 8*npaths ints.
  * Singular values: sigma_max(M)^2 is the Rayleigh quotient |M v|^2/|v|^2
  * of v = the column of largest diagonal entry of (M^H M)^64 (the Gram
  * matrix scaled by its trace and squared six times), in FP64 from the
@@ -268,9 +270,10 @@ int phc_tracker_lds_bytes(const phc_tracker *tracker, int K, size_t *program_byt
  * tracker already forms.  The reference's SVD (zsvdc) gives the same
  * values to rounding when the two largest singular values are apart, and
  * otherwise a value within a factor (1 + 1e-3 n) of them (tested).
- * Calls on one tracker must not overlap (issue them on one stream, or
- * synchronize between streams): the path queue counter and the block of
- * step parameters the kernel reads belong to the tracker.
+ * Calls on one tracker are ordered: the path queue counter and the block
+ * of step parameters the kernel reads belong to the tracker, so a call on
+ * another stream than the tracker's previous call first waits (on the
+ * device) for that call's kernels to end.
  * Replaces GPU_Path_mult / path_mult (Path/Path_GPU/path_gpu_mult.cu:84-383)
  * behind the gpumanypaths_dd entry (Ada/PHCtoC/gpumanypaths_dd.cpp:18). */
 int phc_track(phc_tracker *tracker, int K, int npaths, const double *start_dev, int KS,
@@ -350,6 +353,17 @@ int phc_linear_residue_batch(int K, int n, int degp1, int tailidx, int nbatch,
 typedef struct phc_rowsys phc_rowsys;
 int phc_rowsys_create(phc_rowsys **sys, int dim, const int *monoptr, const int *nvr,
                       const int *idxptr, const int *idx);
+/* the same with the exponent of every variable occurrence, exp[] parallel
+ * to idx[] (1..64; NULL: all ones): the exp/nbrfac/expfac arguments of
+ * CPU_dbl2_evaluate_monomials (Newton/dbl2_systems_host.h:7-46, its
+ * common-factor trick, Newton/dbl2_systems_host.cpp:12-111) for the
+ * monomials of a row system: each coefficient is multiplied by
+ * prod_j x_j^(e_j - 1), one factor at a time in the order of idx, the
+ * distinct variables are differentiated as CPU_dbl2_poly_evaldiff does,
+ * and each derivative contribution is multiplied by its exponent before it
+ * is added to the polynomial's. */
+int phc_rowsys_create_exp(phc_rowsys **sys, int dim, const int *monoptr, const int *nvr,
+                          const int *idxptr, const int *idx, const int *exp);
 int phc_rowsys_destroy(phc_rowsys *sys);
 /* LDS bytes one system occupies at precision K and degree deg */
 int phc_rowsys_lds_bytes(const phc_rowsys *sys, int K, int deg, size_t *bytes);

@@ -16,6 +16,9 @@
  *                          CPU_dbl2_qrbs_tail (:228-335), CPU_dbl2_qrbs_solve (:479-506)
  *   residual               CPU_dbl2_linear_residue (:548-600)
  *   Newton step            dbl2_row_newton_qrstep host branch (Newton/dbl2_newton_method.cpp:429-594),
+ *                          with exponents > 1 by the common-factor trick of
+ *                          CPU_dbl2_evaluate_monomials (Newton/dbl2_systems_host.cpp:12-111)
+ *                          applied per monomial (series.c, osr_poly_evaldiff_exp),
  *                          dbl2_update_newton_qrstep host branch (:134-202),
  *                          dbl2_map_evaldiff_output (Newton/dbl2_systems_host.cpp:709-757),
  *                          dbl2_update_series (Newton/dbl2_newton_testers.cpp:116-149)
@@ -363,9 +366,9 @@ void onw_linear_residue(int K, int dim, int degp1, int tailidx, const double *ma
 }
 
 int onw_row_newton(int K, int dim, int deg, int nbsteps, const int *monoptr, const int *nvr,
-                   const int *idxptr, const int *idx, const double *cst, const double *cff,
-                   double *input, int *wdeg_out, int *tail_out, int *up_out, int *bs_out,
-                   double *history, double *Q, double *R) {
+                   const int *idxptr, const int *idx, const int *exps, const double *cst,
+                   const double *cff, double *input, int *wdeg_out, int *tail_out, int *up_out,
+                   int *bs_out, double *history, double *Q, double *R) {
   const int D = deg + 1;
   const int nmono = monoptr[dim];
   const size_t nn = (size_t)dim * dim;
@@ -405,7 +408,7 @@ int onw_row_newton(int K, int dim, int deg, int nbsteps, const int *monoptr, con
         nv[j] = nvr[m0 + j];
         ip[j] = idxptr[m0 + j];
       }
-      osr_poly_evaldiff(K, 0, dim, nb, wrkdeg, nv, ip, idx, pcst, pcff, pin, pout);
+      osr_poly_evaldiff_exp(K, 0, dim, nb, wrkdeg, nv, ip, idx, exps, pcst, pcff, pin, pout);
       /* rhs[k][i] = -value_i[k]; jac[k][i][j] = d value_i / d x_j [k] */
       for (k = 0; k < Dw; k++) {
         double *dst = E(rhs, (size_t)k * dim + i);

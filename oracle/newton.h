@@ -38,15 +38,16 @@ void onw_linear_residue(int K, int dim, int degp1, int tailidx, const double *ma
                         const double *rhs, const double *sol, double *resvec, double *resmax);
 
 /* staggered row Newton: nbsteps steps at working degrees 0, 1, 2, 4, 7, ...
+ * (exps parallel to idx, NULL for all ones: the exponents of the variables)
  * capped at deg, stopping when the tail index reaches deg.  input is
  * updated in place.  Per step s (when the pointers are non-null):
  * wdeg[s], tail[s] (after the step), up[s], bs[s], and history
  * [nbsteps][K][dim][deg+1] = input after step s.  Q and R ([K][dim][dim])
  * receive the last factorization.  Returns the number of steps run. */
 int onw_row_newton(int K, int dim, int deg, int nbsteps, const int *monoptr, const int *nvr,
-                   const int *idxptr, const int *idx, const double *cst, const double *cff,
-                   double *input, int *wdeg, int *tail, int *up, int *bs, double *history,
-                   double *Q, double *R);
+                   const int *idxptr, const int *idx, const int *exps, const double *cst,
+                   const double *cff, double *input, int *wdeg, int *tail, int *up, int *bs,
+                   double *history, double *Q, double *R);
 
 #ifdef __cplusplus
 }

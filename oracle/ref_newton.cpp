@@ -23,6 +23,14 @@
 #include "dbl2_bals_host.h"
 #include "dbl4_bals_host.h"
 #include "dbl8_bals_host.h"
+#include "dbl_convolutions_host.h"
+#include "dbl2_convolutions_host.h"
+#include "dbl4_convolutions_host.h"
+#include "dbl8_convolutions_host.h"
+#include "dbl_monomials_host.h"
+#include "dbl2_monomials_host.h"
+#include "dbl4_monomials_host.h"
+#include "dbl8_monomials_host.h"
 #include "dbl_polynomials_host.h"
 #include "dbl2_polynomials_host.h"
 #include "dbl4_polynomials_host.h"
@@ -80,6 +88,13 @@ template <int K> struct Ref;
       CPU_##PFX##_poly_evaldiff(dim, nbr, deg, nvr, idx, LX(c), LX(f), LX(in), LX(out), &el,  \
                                 0);                                                           \
     }                                                                                         \
+    static void product(int deg, double *x[], double *y[], double *z[]) {                     \
+      CPU_##PFX##_product(deg, LX(x), LX(y), LX(z));                                          \
+    }                                                                                         \
+    static void speel(int nvr, int deg, int *idx, double *c[], double **in[], double **f[],   \
+                      double **b[], double **x[]) {                                           \
+      CPU_##PFX##_speel(nvr, deg, idx, LX(c), LX(in), LX(f), LX(b), LX(x));                   \
+    }                                                                                         \
   };
 DEFREF(1, dbl, L1)
 DEFREF(2, dbl2, L2)
@@ -98,6 +113,109 @@ template <> void inc<4>(double *x[], const double *dx[]) {
 template <> void inc<8>(double *x[], const double *dx[]) {
   odf_inc(x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], *dx[0], *dx[1], *dx[2], *dx[3],
           *dx[4], *dx[5], *dx[6], *dx[7]);
+}
+
+// z = factor * z with the multiplication by a double of each precision, as
+// CPU_*_evaluate_monomials scales a derivative by its exponent
+// (Newton/dbl2_systems_host.cpp:103-110 and its twins)
+template <int K> void mul_d(double factor, double *z[]);
+template <> void mul_d<1>(double factor, double *z[]) { *z[0] = factor * *z[0]; }
+template <> void mul_d<2>(double factor, double *z[]) {
+  double hi, lo;
+  ddf_mul_d_dd(factor, *z[0], *z[1], &hi, &lo);
+  *z[0] = hi;
+  *z[1] = lo;
+}
+template <> void mul_d<4>(double factor, double *z[]) {
+  double c[4];
+  qdf_mul_qd_d(*z[0], *z[1], *z[2], *z[3], factor, &c[0], &c[1], &c[2], &c[3]);
+  for (int p = 0; p < 4; ++p) *z[p] = c[p];
+}
+template <> void mul_d<8>(double factor, double *z[]) {
+  double c[8];
+  odf_mul_od_d(*z[0], *z[1], *z[2], *z[3], *z[4], *z[5], *z[6], *z[7], factor, &c[0], &c[1],
+               &c[2], &c[3], &c[4], &c[5], &c[6], &c[7]);
+  for (int p = 0; p < 8; ++p) *z[p] = c[p];
+}
+
+// Value and gradient of polynomial cst + sum_m cff_m x^e_m at the series in,
+// composed from the reference's own functions: the coefficient of every
+// monomial times its common factor prod_j x_j^(e_j-1) by CPU_*_product (the
+// first loop of CPU_dbl2_evaluate_monomials, Newton/dbl2_systems_host.cpp:
+// 18-45), then the products of CPU_*_poly_speel for its distinct variables
+// (Polynomials/dbl2_polynomials_host.cpp:13-108; CPU_*_speel for three
+// or more), each derivative contribution multiplied by its exponent (the
+// last loop of CPU_dbl2_evaluate_monomials, :103-110) before it is added.
+// out[p][0..dim] series (rows of dim+1), c/cff/in as row pointers per plane.
+template <int K>
+void poly_evaldiff_exp(int dim, int nbr, int deg, const int *nvr, int **idx, int **exps,
+                       double *c[], double **cff[], double **in[], double **out[]) {
+  const int D = deg + 1;
+  std::vector<double> buf((size_t)K * (3 * dim + 2) * D);
+  std::vector<double *> rows((size_t)K * (3 * dim + 2));
+  for (size_t r = 0; r < rows.size(); ++r) rows[r] = buf.data() + r * D;
+  double **fw[K], **bw[K], **cr[K], *cf[K], *acc[K];
+  for (int p = 0; p < K; ++p) {
+    double **base = rows.data() + (size_t)p * (3 * dim + 2);
+    fw[p] = base; bw[p] = base + dim; cr[p] = base + 2 * dim;
+    cf[p] = base[3 * dim]; acc[p] = base[3 * dim + 1];
+  }
+  for (int p = 0; p < K; ++p)
+    for (int j = 0; j < D; ++j) {
+      out[p][dim][j] = c[p][j];
+      for (int i = 0; i < dim; ++i) out[p][i][j] = 0.0;
+    }
+  // out[row] += factor * src[srow] (the multiplication only for factors > 1)
+  auto add = [&](int row, double **src[], int srow, int e) {
+    const double factor = (double)e;
+    for (int j = 0; j < D; ++j) {
+      double v[K], *pv[K], *po[K];
+      for (int p = 0; p < K; ++p) { v[p] = src[p][srow][j]; pv[p] = &v[p]; po[p] = &out[p][row][j]; }
+      if (e > 1) mul_d<K>(factor, pv);
+      inc<K>(po, (const double **)pv);
+    }
+  };
+  for (int m = 0; m < nbr; ++m) {
+    const int nv = nvr[m];
+    int *ix = idx[m];
+    for (int p = 0; p < K; ++p) std::memcpy(cf[p], cff[p][m], sizeof(double) * D);
+    for (int j = 0; j < nv; ++j)
+      for (int e = 1; e < exps[m][j]; ++e) {
+        double *xin[K];
+        for (int p = 0; p < K; ++p) xin[p] = in[p][ix[j]];
+        Ref<K>::product(deg, xin, cf, acc);
+        for (int p = 0; p < K; ++p) std::memcpy(cf[p], acc[p], sizeof(double) * D);
+      }
+    double **cfr[K];
+    double *cfrow[K][1];
+    for (int p = 0; p < K; ++p) { cfrow[p][0] = cf[p]; cfr[p] = cfrow[p]; }
+    auto expo = [&](int j) { return exps[m][j]; };
+    if (nv == 1) {
+      double *xin[K], *f0[K];
+      for (int p = 0; p < K; ++p) { xin[p] = in[p][ix[0]]; f0[p] = fw[p][0]; }
+      Ref<K>::product(deg, xin, cf, f0);
+      add(dim, fw, 0, 1);
+      add(ix[0], cfr, 0, expo(0));
+    } else if (nv == 2) {
+      double *x0[K], *x1[K], *f0[K], *f1[K], *b0[K];
+      for (int p = 0; p < K; ++p) {
+        x0[p] = in[p][ix[0]]; x1[p] = in[p][ix[1]];
+        f0[p] = fw[p][0]; f1[p] = fw[p][1]; b0[p] = bw[p][0];
+      }
+      Ref<K>::product(deg, cf, x0, f0);
+      add(ix[1], fw, 0, expo(1));
+      Ref<K>::product(deg, cf, x1, b0);
+      add(ix[0], bw, 0, expo(0));
+      Ref<K>::product(deg, f0, x1, f1);
+      add(dim, fw, 1, 1);
+    } else {
+      Ref<K>::speel(nv, deg, ix, cf, in, fw, bw, cr);
+      add(dim, fw, nv - 1, 1);
+      add(ix[nv - 1], fw, nv - 2, expo(nv - 1));
+      add(ix[0], bw, nv - 3, expo(0));
+      for (int k = 1; k < nv - 1; ++k) add(ix[k], cr, k - 1, expo(k));
+    }
+  }
 }
 
 template <int K> void houseqr_k(int nr, int nc, double *A, double *Q, double *R) {
@@ -143,14 +261,19 @@ void residue_k(int dim, int degp1, int tail, double *mat, double *rhs, double *s
 // CPU_*_poly_evaldiff and CPU_*_qrbs_solve.
 template <int K>
 int row_newton_k(int dim, int deg, int nbsteps, const int *monoptr, const int *nvr,
-                 const int *idxptr, const int *idxflat, double *cst, double *cff, double *input,
-                 int *wdeg, int *tails, int *ups, int *bss, double *history, double *Qo,
-                 double *Ro) {
+                 const int *idxptr, const int *idxflat, const int *expflat, double *cst,
+                 double *cff, double *input, int *wdeg, int *tails, int *ups, int *bss,
+                 double *history, double *Qo, double *Ro) {
   const int D = deg + 1, nmono = monoptr[dim];
   std::vector<int> nv(nvr, nvr + (nmono > 0 ? nmono : 1));
   std::vector<int> idxc(idxflat, idxflat + idxptr[nmono]);
-  std::vector<int *> idx(nmono > 0 ? nmono : 1);
-  for (int m = 0; m < nmono; ++m) idx[m] = idxc.data() + idxptr[m];
+  std::vector<int> expc(idxptr[nmono] > 0 ? idxptr[nmono] : 1, 1);
+  if (expflat) std::copy(expflat, expflat + idxptr[nmono], expc.begin());
+  std::vector<int *> idx(nmono > 0 ? nmono : 1), exps(nmono > 0 ? nmono : 1);
+  for (int m = 0; m < nmono; ++m) {
+    idx[m] = idxc.data() + idxptr[m];
+    exps[m] = expc.data() + idxptr[m];
+  }
   Mats in(input, K, 1, dim, D);                     // input[p][i] -> series
   Mats cf(cff, K, 1, nmono > 0 ? nmono : 1, D);     // cff[p][m]
   std::vector<double> out((size_t)K * dim * (dim + 1) * D);
@@ -176,8 +299,12 @@ int row_newton_k(int dim, int deg, int nbsteps, const int *monoptr, const int *n
         f[p] = cf.mat(p) + monoptr[i];
         o[p] = ou.ser(p)[i];
       }
-      Ref<K>::evaldiff(dim, monoptr[i + 1] - monoptr[i], wrkdeg, nv.data() + monoptr[i],
-                       idx.data() + monoptr[i], c, f, pin, o);
+      if (expflat)
+        poly_evaldiff_exp<K>(dim, monoptr[i + 1] - monoptr[i], wrkdeg, nv.data() + monoptr[i],
+                             idx.data() + monoptr[i], exps.data() + monoptr[i], c, f, pin, o);
+      else
+        Ref<K>::evaldiff(dim, monoptr[i + 1] - monoptr[i], wrkdeg, nv.data() + monoptr[i],
+                         idx.data() + monoptr[i], c, f, pin, o);
     }
     // dbl2_map_evaldiff_output and rhs = -funval, then the saved copy urhs
     for (int p = 0; p < K; ++p)
@@ -258,14 +385,14 @@ int ref_linear_residue(int K, int dim, int degp1, int tailidx, double *mat, doub
 }
 
 int ref_row_newton(int K, int dim, int deg, int nbsteps, const int *monoptr, const int *nvr,
-                   const int *idxptr, const int *idx, double *cst, double *cff, double *input,
-                   int *wdeg, int *tail, int *up, int *bs, double *history, double *Q,
-                   double *R) {
+                   const int *idxptr, const int *idx, const int *exps, double *cst, double *cff,
+                   double *input, int *wdeg, int *tail, int *up, int *bs, double *history,
+                   double *Q, double *R) {
   switch (K) {
 #define CASE(KK)                                                                              \
     case KK:                                                                                  \
-      return row_newton_k<KK>(dim, deg, nbsteps, monoptr, nvr, idxptr, idx, cst, cff, input,  \
-                              wdeg, tail, up, bs, history, Q, R);
+      return row_newton_k<KK>(dim, deg, nbsteps, monoptr, nvr, idxptr, idx, exps, cst, cff,   \
+                              input, wdeg, tail, up, bs, history, Q, R);
     CASE(1) CASE(2) CASE(4) CASE(8)
 #undef CASE
   }

@@ -86,17 +86,55 @@ static void ser_inc(int K, int cplx, int deg, double *dst, size_t sd, const doub
   }
 }
 
+/* output[dst] += e * src (coefficientwise; the multiplication by e as
+ * CPU_dbl2_evaluate_monomials scales a derivative, only for e > 1) */
+static void ser_inc_e(int K, int cplx, int deg, double *dst, size_t sd, const double *src,
+                      size_t ss, int e) {
+  const int P = cplx ? 2 * K : K;
+  double a[16], b[16];
+  int j;
+  if (e <= 1) {
+    ser_inc(K, cplx, deg, dst, sd, src, ss);
+    return;
+  }
+  for (j = 0; j <= deg; j++) {
+    ld(dst, P, sd, j, a);
+    ld(src, P, ss, j, b);
+    omd_mul_d(K, b, (double)e, b);
+    if (cplx) omd_mul_d(K, b + K, (double)e, b + K);
+    num_inc(K, cplx, a, b);
+    st(dst, P, sd, j, a);
+  }
+}
+
 void osr_poly_evaldiff(int K, int cplx, int dim, int nbr, int deg, const int *nvr,
                        const int *idxptr, const int *idx, const double *cst,
                        const double *cff, const double *input, double *output) {
+  osr_poly_evaldiff_exp(K, cplx, dim, nbr, deg, nvr, idxptr, idx, NULL, cst, cff, input, output);
+}
+
+/* With exponents (exps parallel to idx, NULL for all ones): the coefficient
+ * of every monomial times its common factor prod_j x_j^(e_j-1), the factors
+ * multiplied in one at a time in the order of idx (the first loop of
+ * CPU_dbl2_evaluate_monomials, Newton/dbl2_systems_host.cpp:18-45), then the
+ * products of CPU_dbl2_poly_speel on its distinct variables, each
+ * derivative contribution multiplied by its exponent (the last loop,
+ * :103-110) before it is added. */
+void osr_poly_evaldiff_exp(int K, int cplx, int dim, int nbr, int deg, const int *nvr,
+                           const int *idxptr, const int *idx, const int *exps, const double *cst,
+                           const double *cff, const double *input, double *output) {
   const int P = cplx ? 2 * K : K;
   const size_t D = deg + 1;
-  const size_t sc = (size_t)nbr * D, sx = (size_t)dim * D, so = (size_t)(dim + 1) * D;
+  const size_t sx = (size_t)dim * D, so = (size_t)(dim + 1) * D;
+  size_t sc = (size_t)nbr * D;
   const size_t sw = (size_t)dim * D; /* work arrays of dim series */
   double *fw = (double *)calloc((size_t)P * sw, sizeof(double));
   double *bw = (double *)calloc((size_t)P * sw, sizeof(double));
   double *cr = (double *)calloc((size_t)P * sw, sizeof(double));
-  int i, j, p, k;
+  double *cf = (double *)calloc((size_t)P * D, sizeof(double));
+  double *acc = (double *)calloc((size_t)P * D, sizeof(double));
+  const size_t sc0 = sc;
+  int i, j, p, k, e;
 
   for (p = 0; p < P; p++)
     for (j = 0; j <= deg; j++) {
@@ -106,16 +144,31 @@ void osr_poly_evaldiff(int K, int cplx, int dim, int nbr, int deg, const int *nv
   for (i = 0; i < nbr; i++) {
     const int nv = nvr[i];
     const int *ix = idx + idxptr[i];
+    const int *ex = exps ? exps + idxptr[i] : NULL;
     const double *c = SER(cff, i);
+    int common = 0;
+    sc = sc0;
+    for (j = 0; ex && j < nv; j++) common = common || ex[j] > 1;
+    if (common) {
+      for (p = 0; p < P; p++) memcpy(cf + p * D, c + p * sc0, sizeof(double) * D);
+      for (j = 0; j < nv; j++)
+        for (e = 1; e < ex[j]; e++) {
+          product_s(K, cplx, deg, SER(input, ix[j]), sx, cf, D, acc, D);
+          memcpy(cf, acc, sizeof(double) * P * D);
+        }
+      c = cf;
+      sc = D;
+    }
+#define EXP(j) (ex ? ex[j] : 1)
     if (nv == 1) {
       product_s(K, cplx, deg, SER(input, ix[0]), sx, c, sc, fw, sw);
       ser_inc(K, cplx, deg, SER(output, dim), so, fw, sw);
-      ser_inc(K, cplx, deg, SER(output, ix[0]), so, c, sc);
+      ser_inc_e(K, cplx, deg, SER(output, ix[0]), so, c, sc, EXP(0));
     } else if (nv == 2) {
       product_s(K, cplx, deg, c, sc, SER(input, ix[0]), sx, fw, sw);
-      ser_inc(K, cplx, deg, SER(output, ix[1]), so, fw, sw);
+      ser_inc_e(K, cplx, deg, SER(output, ix[1]), so, fw, sw, EXP(1));
       product_s(K, cplx, deg, c, sc, SER(input, ix[1]), sx, bw, sw);
-      ser_inc(K, cplx, deg, SER(output, ix[0]), so, bw, sw);
+      ser_inc_e(K, cplx, deg, SER(output, ix[0]), so, bw, sw, EXP(0));
       product_s(K, cplx, deg, fw, sw, SER(input, ix[1]), sx, SER(fw, 1), sw);
       ser_inc(K, cplx, deg, SER(output, dim), so, SER(fw, 1), sw);
     } else if (nv > 2) {
@@ -139,13 +192,16 @@ void osr_poly_evaldiff(int K, int cplx, int dim, int nbr, int deg, const int *nv
                   SER(cr, nv - 3), sw);
       }
       ser_inc(K, cplx, deg, SER(output, dim), so, SER(fw, nv - 1), sw);
-      ser_inc(K, cplx, deg, SER(output, ix[nv - 1]), so, SER(fw, nv - 2), sw);
-      ser_inc(K, cplx, deg, SER(output, ix[0]), so, SER(bw, nv - 3), sw);
+      ser_inc_e(K, cplx, deg, SER(output, ix[nv - 1]), so, SER(fw, nv - 2), sw, EXP(nv - 1));
+      ser_inc_e(K, cplx, deg, SER(output, ix[0]), so, SER(bw, nv - 3), sw, EXP(0));
       for (k = 1; k < nv - 1; k++)
-        ser_inc(K, cplx, deg, SER(output, ix[k]), so, SER(cr, k - 1), sw);
+        ser_inc_e(K, cplx, deg, SER(output, ix[k]), so, SER(cr, k - 1), sw, EXP(k));
     }
+#undef EXP
   }
   free(fw);
   free(bw);
   free(cr);
+  free(cf);
+  free(acc);
 }

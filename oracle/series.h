@@ -15,6 +15,12 @@ void osr_poly_evaldiff(int K, int cplx, int dim, int nbr, int deg, const int *nv
                        const int *idxptr, const int *idx, const double *cst,
                        const double *cff, const double *input, double *output);
 
+/* the same with exponents exps[] parallel to idx[] (NULL: all ones): the
+ * common-factor trick of CPU_dbl2_evaluate_monomials per monomial */
+void osr_poly_evaldiff_exp(int K, int cplx, int dim, int nbr, int deg, const int *nvr,
+                           const int *idxptr, const int *idx, const int *exps, const double *cst,
+                           const double *cff, const double *input, double *output);
+
 #ifdef __cplusplus
 }
 #endif

@@ -642,16 +642,20 @@ static double zrad(double re, double im) { return sqrt(re * re + im * im); }
 /* largest eigenvalue of M^H M (sigma_max(M)^2) for M n x n complex,
  * interleaved (re, im) row-major: the Gram matrix scaled by its trace and
  * squared six times, then the Rayleigh quotient |M v|^2 / |v|^2 of the
- * column v of largest diagonal entry.  P and Q are 2*n*n scratch doubles. */
+ * column v of largest diagonal entry.  The Gram matrix and its powers are
+ * Hermitian: entries i <= j are computed, entries i > j are their
+ * conjugates.  P and Q are 2*n*n scratch doubles. */
 double otr_sigma_max_sq(int n, const double *M, double *P, double *Q) {
   int i, j, k, s, jm;
   double tr, sc, num, den, best;
   for (i = 0; i < n; i++)
-    for (j = 0; j < n; j++) {
+    for (j = i; j < n; j++) {
       double acc[2] = {0.0, 0.0};
       for (k = 0; k < n; k++) zcjmac(acc, M + 2 * (k * n + i), M + 2 * (k * n + j));
       P[2 * (i * n + j)] = acc[0];
       P[2 * (i * n + j) + 1] = acc[1];
+      P[2 * (j * n + i)] = acc[0];
+      P[2 * (j * n + i) + 1] = (i == j) ? acc[1] : -acc[1];
     }
   tr = 0.0;
   for (i = 0; i < n; i++) tr = tr + P[2 * (i * n + i)];
@@ -662,11 +666,13 @@ double otr_sigma_max_sq(int n, const double *M, double *P, double *Q) {
     for (i = 0; i < 2 * n * n; i++) P[i] = P[i] * sc;
     if (s == 6) break;
     for (i = 0; i < n; i++)
-      for (j = 0; j < n; j++) {
+      for (j = i; j < n; j++) {
         double acc[2] = {0.0, 0.0};
         for (k = 0; k < n; k++) zmac(acc, P + 2 * (i * n + k), P + 2 * (k * n + j));
         Q[2 * (i * n + j)] = acc[0];
         Q[2 * (i * n + j) + 1] = acc[1];
+        Q[2 * (j * n + i)] = acc[0];
+        Q[2 * (j * n + i) + 1] = (i == j) ? acc[1] : -acc[1];
       }
     memcpy(P, Q, sizeof(double) * 2 * n * n);
     tr = 0.0;
@@ -896,14 +902,14 @@ void otr_pade_error(int K, double *c, int L, int M, double *x0, double *err) {
  * both grow.  Returns 1 converged, 0 not, -1 singular Jacobian.
  * evaluated: the products and the residual at the point are in place. */
 static int lu_newton_steps(const otr_program *P, work_t *w, int maxit, double tol, int extra,
-                           int evaluated, double *dxl, int *count, double *dxh) {
+                           int evaluated, double *dxl, int *count, int *nfirst, double *dxh) {
   const int K = w->K, D = P->deg + 1;
   double dx[16 * 64], prev_err = 1.0, prev_mix = 1.0;
   int k, j, cnt = 0;
   for (k = 1; k <= maxit + extra; k++) {
     double err, res, mix;
     ++*count;
-    if (k == 1 && !evaluated) eval_point(P, w, 1); else jacobian(P, w);
+    if (k == 1 && !evaluated) { eval_point(P, w, 1); ++*nfirst; } else jacobian(P, w);
     if (!gj_invert(P, w)) return -1;
     apply_inverse(P, w, dx);
     for (j = 0; j < P->n; j++) omd_cadd(K, ZP(w->slots, j * D), ZP(dx, j), ZP(w->slots, j * D));
@@ -958,7 +964,7 @@ static void track_path(const otr_program *P, work_t *w, int npaths, int path, co
   double t = 0.0, dxl = 0.0, hprev = prm->maxstep;
   double dxh[4] = {1.0, 1.0, 1.0, 1.0};
   int st = ST_RUNNING, steps = 0, rej = 0, j, k, npred = 0, nnewt = 0, nfeval = 0, ncorr = 0;
-  int nbpole = 0, nbhess = 0, nbmaxm = 0;
+  int nbpole = 0, nbhess = 0, nbmaxm = 0, nfirst = 0;
   int rejprev = 0, nlog = 0, attempt, pmode = mode;
   /* a path the learned controller (mode 1) does not finish is tracked
      again from its start with the plain rule; the work counters add up
@@ -1050,7 +1056,8 @@ static void track_path(const otr_program *P, work_t *w, int npaths, int path, co
       for (;;) {
         int nr, it0 = nnewt;
         if (!evaluated) set_weights(P, w, tn);
-        nr = lu_newton_steps(P, w, prm->corsteps, prm->tolcor, 1, evaluated, &dxl, &nnewt, dxh);
+        nr = lu_newton_steps(P, w, prm->corsteps, prm->tolcor, 1, evaluated, &dxl, &nnewt,
+                             &nfirst, dxh);
         ++ncorr;
         nit = nnewt - it0;
         if (nr < 0) { st = ST_SINGULAR; break; }
@@ -1088,7 +1095,7 @@ static void track_path(const otr_program *P, work_t *w, int npaths, int path, co
     if (st == ST_RUNNING) {
       int nr;
       set_weights(P, w, 1.0);
-      nr = lu_newton_steps(P, w, prm->endsteps, prm->tolend, 1, 0, &dxl, &nnewt, NULL);
+      nr = lu_newton_steps(P, w, prm->endsteps, prm->tolend, 1, 0, &dxl, &nnewt, &nfirst, NULL);
       st = (nr < 0) ? ST_SINGULAR : ST_SUCCESS;
     }
     if (!(pmode == 1 && st != ST_SUCCESS)) break;
@@ -1107,6 +1114,7 @@ static void track_path(const otr_program *P, work_t *w, int npaths, int path, co
     work[4 * npaths + path] = nbpole;
     work[5 * npaths + path] = nbhess;
     work[6 * npaths + path] = nbmaxm;
+    work[7 * npaths + path] = nfirst;
   }
   if (mode == 2) ctl->log_n[path] = nlog;
 }

@@ -274,7 +274,16 @@ __device__ void qrbs_solve(int n, int degp1, int tailidx, const M &mat, const Ar
 struct RowSys {
   int dim, nmono, nvmax, nbrmax;
   const int *monoptr, *nvr, *idxptr, *idx;
+  const int *exp;  // exponents parallel to idx, nullptr when all are 1
 };
+
+// v * e, the derivative of a monomial times its exponent, with the
+// multiplication by a double of each precision (ddf_mul_d_dd and its twins,
+// as CPU_dbl2_evaluate_monomials does, Newton/dbl2_systems_host.cpp:103-110)
+__device__ __forceinline__ double times_exp(double v, double e) { return e * v; }
+template <int K> __device__ __forceinline__ mdr<K> times_exp(const mdr<K> &v, double e) {
+  return mul_d(v, e);
+}
 
 // coefficient k of x*y: x_0 y_k, then += x_i y_{k-i} (CPU_dbl2_product,
 // Convolutions/dbl2_convolutions_host.cpp:7-26)
@@ -291,18 +300,25 @@ __device__ __forceinline__ typename RealT<K>::type cprod(const FX &x, const FY &
 // (Polynomials/dbl2_polynomials_host.cpp, Convolutions/dbl2_monomials_host.cpp:28-87).
 // Slot s (Dp lanes) evaluates polynomials s, s + nslot, ...; lane k of a slot
 // owns coefficient k.  cst/cff planes: cst + p*D, cff + m*D with plane strides.
+// A monomial's coefficient is staged in the slot's row CF; with exponents
+// it is first multiplied by the common factor prod_j x_j^(e_j - 1), one
+// factor at a time in the order of idx (the first loop of
+// CPU_dbl2_evaluate_monomials, Newton/dbl2_systems_host.cpp:18-45), and each
+// derivative contribution is multiplied by its exponent before it is added.
 template <int K>
 __device__ void eval_system(const RowSys &S, int D, int Dw, int Dp, const double *cst,
                             size_t cstride, const double *cff, size_t fstride, const Arr<K> &X,
                             const Arr<K> &OUT, const Arr<K> &FW, const Arr<K> &BW,
-                            const Arr<K> &CR) {
+                            const Arr<K> &CR, const Arr<K> &CF) {
   typedef typename RealT<K>::type T;
   const int l = lane_id(), nslot = 64 / Dp, s = l / Dp, k = l % Dp;
   const int dim = S.dim, D1 = dim + 1;
   const int eo = s * S.nvmax * D;  // this slot's work rows
+  const int co = s * D;            // this slot's coefficient row
   auto xin = [&](int var) { return [&X, var, D](int j) { return X.ld(var * D + j); }; };
   auto fw = [&](int r) { return [&FW, eo, r, D](int j) { return FW.ld(eo + r * D + j); }; };
   auto bw = [&](int r) { return [&BW, eo, r, D](int j) { return BW.ld(eo + r * D + j); }; };
+  auto cf = [&CF, co](int j) { return CF.ld(co + j); };
   for (int r0 = 0; r0 < dim; r0 += nslot) {
     const int p = r0 + s;
     const bool on = (p < dim) && (k < Dw);
@@ -316,24 +332,38 @@ __device__ void eval_system(const RowSys &S, int D, int Dw, int Dp, const double
       const int mono = act ? S.monoptr[p] + mi : 0;
       const int nv = act ? S.nvr[mono] : 0;
       const int *ix = S.idx + (act ? S.idxptr[mono] : 0);
-      auto cf = [cff, fstride, mono, D](int j) { return R<K>::load(cff, fstride, (size_t)mono * D + j); };
-      auto addto = [&](int row, const T &v) {
+      const int *ex = (act && S.exp) ? S.exp + S.idxptr[mono] : nullptr;
+      // v times the exponent of variable position j (positions with
+      // exponent 1 are added as they are)
+      auto addto = [&](int row, const T &v, int j) {
         const int o = (p * D1 + row) * D + k;
-        OUT.st(o, add(OUT.ld(o), v));
+        const int e = (ex && j >= 0) ? ex[j] : 1;
+        OUT.st(o, add(OUT.ld(o), e > 1 ? times_exp(v, (double)e) : v));
       };
+      if (act) CF.st(co + k, R<K>::load(cff, fstride, (size_t)mono * D + k));
+      wsync();
+      if (ex) {
+        for (int j = 0; j < nv; ++j)
+          for (int e = 1; e < ex[j]; ++e) {
+            const T a = cprod<K>(xin(ix[j]), cf, k);
+            wsync();
+            CF.st(co + k, a);
+            wsync();
+          }
+      }
       if (nv == 1) {
         const T f = cprod<K>(xin(ix[0]), cf, k);
-        addto(dim, f);
-        addto(ix[0], cf(k));
+        addto(dim, f, -1);
+        addto(ix[0], cf(k), 0);
       } else if (nv == 2) {
         const T f0 = cprod<K>(cf, xin(ix[0]), k);
         FW.st(eo + k, f0);
-        addto(ix[1], f0);
+        addto(ix[1], f0, 1);
         const T b0 = cprod<K>(cf, xin(ix[1]), k);
-        addto(ix[0], b0);
+        addto(ix[0], b0, 0);
         wsync();
         const T f1 = cprod<K>(fw(0), xin(ix[1]), k);
-        addto(dim, f1);
+        addto(dim, f1, -1);
       } else if (nv > 2) {
         // forward and backward chains advance together
         FW.st(eo + k, cprod<K>(cf, xin(ix[0]), k));
@@ -354,10 +384,10 @@ __device__ void eval_system(const RowSys &S, int D, int Dw, int Dp, const double
           for (int j = 0; j < nv - 3; ++j) CR.st(eo + j * D + k, cprod<K>(fw(j), bw(nv - 4 - j), k));
           CR.st(eo + (nv - 3) * D + k, cprod<K>(fw(nv - 3), xin(ix[nv - 1]), k));
         }
-        addto(dim, FW.ld(eo + (nv - 1) * D + k));
-        addto(ix[nv - 1], FW.ld(eo + (nv - 2) * D + k));
-        addto(ix[0], BW.ld(eo + (nv - 3) * D + k));
-        for (int j = 1; j < nv - 1; ++j) addto(ix[j], CR.ld(eo + (j - 1) * D + k));
+        addto(dim, FW.ld(eo + (nv - 1) * D + k), -1);
+        addto(ix[nv - 1], FW.ld(eo + (nv - 2) * D + k), nv - 1);
+        addto(ix[0], BW.ld(eo + (nv - 3) * D + k), 0);
+        for (int j = 1; j < nv - 1; ++j) addto(ix[j], CR.ld(eo + (j - 1) * D + k), j);
       }
       wsync();
     }
@@ -368,10 +398,10 @@ __device__ void eval_system(const RowSys &S, int D, int Dw, int Dp, const double
 // ------------------------------------------------------------ kernels
 
 struct NewtonLayout {
-  int X, OUT, RHS, SOL, Q, R, V, W1, W2, WRK, SC, FW, BW, CR;  // sizes in numbers
+  int X, OUT, RHS, SOL, Q, R, V, W1, W2, WRK, SC, FW, BW, CR, CF;  // sizes in numbers
   __host__ __device__ size_t bytes(int K) const {
     return sizeof(double) * (size_t)K *
-           ((size_t)X + OUT + RHS + SOL + Q + R + V + W1 + W2 + WRK + SC + FW + BW + CR);
+           ((size_t)X + OUT + RHS + SOL + Q + R + V + W1 + W2 + WRK + SC + FW + BW + CR + CF);
   }
 };
 
@@ -386,7 +416,7 @@ __host__ __device__ inline NewtonLayout newton_layout(const RowSys &S, int D) {
   NewtonLayout L;
   L.X = n * D; L.OUT = n * (n + 1) * D; L.RHS = D * n; L.SOL = D * n;
   L.Q = n * n; L.R = n * n; L.V = n; L.W1 = n; L.W2 = n; L.WRK = n; L.SC = 3;
-  L.FW = ev; L.BW = ev; L.CR = ev;
+  L.FW = ev; L.BW = ev; L.CR = ev; L.CF = nslot * D;
   return L;
 }
 
@@ -417,7 +447,8 @@ row_newton_kernel(RowSys S, int deg, int nbsteps, int nsys, const double *cst, i
   const Arr<K> X = carve<K>(p, L.X), OUT = carve<K>(p, L.OUT), RHS = carve<K>(p, L.RHS),
                SOL = carve<K>(p, L.SOL);
   const LinWork<K> w = carve_lin<K>(p, n);
-  const Arr<K> FW = carve<K>(p, L.FW), BW = carve<K>(p, L.BW), CR = carve<K>(p, L.CR);
+  const Arr<K> FW = carve<K>(p, L.FW), BW = carve<K>(p, L.BW), CR = carve<K>(p, L.CR),
+               CF = carve<K>(p, L.CF);
   const size_t xs = (size_t)nsys * n * D, xb = (size_t)sysid * n * D;
   const size_t cs = (size_t)(cshared ? 1 : nsys) * n * D;
   const size_t fs = (size_t)(fshared ? 1 : nsys) * S.nmono * D;
@@ -431,7 +462,7 @@ row_newton_kernel(RowSys S, int deg, int nbsteps, int nsys, const double *cst, i
   int tailidx = 1, wrkdeg = 0, stepcnt = 0;
   for (int step = 0; step < nbsteps; ++step) {
     const int Dw = wrkdeg + 1;
-    eval_system<K>(S, D, Dw, Dp, cbase, cs, fbase, fs, X, OUT, FW, BW, CR);
+    eval_system<K>(S, D, Dw, Dp, cbase, cs, fbase, fs, X, OUT, FW, BW, CR, CF);
     for (int t = l; t < Dw * n; t += 64) {
       const int kk = t / n, i = t % n;
       RHS.st(t, neg(OUT.ld((i * D1 + n) * D + kk)));
@@ -597,8 +628,16 @@ struct phc_rowsys {
 
 extern "C" {
 
+int phc_rowsys_create_exp(phc_rowsys **out, int dim, const int *monoptr, const int *nvr,
+                          const int *idxptr, const int *idx, const int *exp);
+
 int phc_rowsys_create(phc_rowsys **out, int dim, const int *monoptr, const int *nvr,
                       const int *idxptr, const int *idx) {
+  return phc_rowsys_create_exp(out, dim, monoptr, nvr, idxptr, idx, nullptr);
+}
+
+int phc_rowsys_create_exp(phc_rowsys **out, int dim, const int *monoptr, const int *nvr,
+                          const int *idxptr, const int *idx, const int *exp) {
   PHC_REQUIRE(out && monoptr && nvr && idxptr, "phc_rowsys_create: null argument");
   PHC_REQUIRE(dim >= 1 && dim <= 64, "phc_rowsys_create: dim must be in 1..64");
   const int nmono = monoptr[dim];
@@ -617,7 +656,12 @@ int phc_rowsys_create(phc_rowsys **out, int dim, const int *monoptr, const int *
     if (nvr[m] > nvmax) nvmax = nvr[m];
   }
   const int nidx = idxptr[nmono];
-  const size_t ints = (size_t)(dim + 1) + 2 * (size_t)nmono + 1 + nidx;
+  bool any_exp = false;
+  for (int j = 0; exp && j < nidx; ++j) {
+    PHC_REQUIRE(exp[j] >= 1 && exp[j] <= 64, "phc_rowsys_create: exponents must be in 1..64");
+    any_exp = any_exp || exp[j] > 1;
+  }
+  const size_t ints = (size_t)(dim + 1) + 2 * (size_t)nmono + 1 + nidx + (any_exp ? nidx : 0);
   phc_rowsys *s = new phc_rowsys();
   if (hipMalloc(&s->dmem, sizeof(int) * (ints > 0 ? ints : 1)) != hipSuccess) {
     delete s;
@@ -630,10 +674,12 @@ int phc_rowsys_create(phc_rowsys **out, int dim, const int *monoptr, const int *
   h.insert(h.end(), nvr, nvr + nmono);
   h.insert(h.end(), idxptr, idxptr + nmono + 1);
   if (nidx > 0) h.insert(h.end(), idx, idx + nidx);
+  if (any_exp) h.insert(h.end(), exp, exp + nidx);
   PHC_HIP(hipMemcpy(p, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
   s->S.dim = dim; s->S.nmono = nmono; s->S.nvmax = nvmax; s->S.nbrmax = nbrmax;
   s->S.monoptr = p; s->S.nvr = p + dim + 1; s->S.idxptr = p + dim + 1 + nmono;
   s->S.idx = p + dim + 1 + 2 * nmono + 1;
+  s->S.exp = any_exp ? s->S.idx + nidx : nullptr;
   *out = s;
   return 0;
 }

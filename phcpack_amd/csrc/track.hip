@@ -70,12 +70,16 @@ struct DevProgram {
   const int *nodes, *lev_ptr, *pairs, *seg_ptr, *term_pair, *jpairs, *jent_ptr, *jterms;
   const double *term_coef;  // [2][nterm]
   const double *weights;    // [2][8][3*ncomp]
+  const double *hsig;       // [n]: Hessian singular values, -1: evaluated every step
+  const int *hent_ptr, *hterms, *hpairs;
+  int hdyn;                 // 1 when some equation's Hessian is evaluated every step
 };
 
 struct Params {
   int maxsteps, corsteps, endsteps, fabry_offset;
   double maxstep, minstep, pbeta, alpha, tolcor, tolend, divergence;
   int pnum, pden;  // predictor: numerator degree (>= 1), denominator degree (0, 1 or 2)
+  double cbeta;    // curvature step factor
 };
 
 // step-size controller (device view of phc_controller)
@@ -133,6 +137,18 @@ struct ProgLayout {
   }
 };
 
+// scratch of the step rule, in doubles, aliasing the weighted-pair arrays
+// u and uj (free outside the residual and the Jacobian): the moduli of the
+// nodes for the mixed residual, or the matrices of the singular values
+// (Hessian H and Gram power P, and Q where n > 16 keeps no entries in
+// registers)
+__host__ __device__ inline size_t step_scratch_doubles(const DevProgram &P) {
+  const size_t nn = (size_t)P.n * P.n, q = P.n > 16 ? 1 : 0;
+  const size_t sv = 2 * nn * ((P.hdyn ? 2 : 1) + q);
+  const size_t rn = (size_t)(P.n + 1 + P.nprod);
+  return sv > rn ? sv : rn;
+}
+
 // per-wavefront path region (bytes)
 template <int K> struct PathLayout {
   size_t slots, u, uj, A, rhs, x0, col, W, dnode, perm, hid, total;
@@ -140,7 +156,13 @@ template <int K> struct PathLayout {
     const size_t z = sizeof(cplx<K>);
     size_t o = 0;
     slots = o; o += z * P.nslots;
-    u = o; o += z * P.npair;
+    u = o;
+    {
+      // u, uj contiguous and together at least the step rule's scratch
+      const size_t need = al16(sizeof(double) * step_scratch_doubles(P));
+      const size_t have = z * (P.npair + P.njpair);
+      o += z * P.npair + (need > have ? need - have : 0);
+    }
     uj = o; o += z * P.njpair;
     A = o; o += z * P.n * P.n;
     rhs = o; o += z * P.n;
@@ -160,6 +182,8 @@ struct LProg {
   const int *nodes, *lev, *pairs, *seg, *tpair, *jpairs, *jent, *jterms;
   const double2 *tcoef;
   const double *weights;  // global
+  const double *hsig;     // global
+  const int *hent, *hterms, *hpairs;  // global
 };
 
 template <int K> struct Path {
@@ -577,6 +601,299 @@ __device__ int newton(const LProg &P, Path<K> &w, int iters, double tol, double 
   return 0;
 }
 
+// ------------------------------------------------------- the step rule ----
+
+// x^(1/k) for x >= 0 from IEEE double operations only (the oracle's
+// otr_root, bit for bit): log2 from the binary exponent and the series of
+// ln((1+z)/(1-z)) to z^21, 2^y from the binary exponent and the Taylor
+// series of e^g to g^16, then one Newton step on r^k = x
+__host__ __device__ inline double kth_root(double x, int k) {
+  if (x != x) return x;
+  if (!(x > 0.0)) return 0.0;
+  if (x == INFINITY || k == 1) return x;
+  int e = 0;
+  const double m = 2.0 * frexp(x, &e);
+  const double z = (m - 1.0) / (m + 1.0), z2 = z * z;
+  double s = 1.0 / 21.0;
+  s = 1.0 / 19.0 + z2 * s;
+  s = 1.0 / 17.0 + z2 * s;
+  s = 1.0 / 15.0 + z2 * s;
+  s = 1.0 / 13.0 + z2 * s;
+  s = 1.0 / 11.0 + z2 * s;
+  s = 1.0 / 9.0 + z2 * s;
+  s = 1.0 / 7.0 + z2 * s;
+  s = 1.0 / 5.0 + z2 * s;
+  s = 1.0 / 3.0 + z2 * s;
+  s = 1.0 + z2 * s;
+  const double y = ((double)(e - 1) + (z * s) * 2.8853900817779268) / (double)k;
+  const double fl = floor(y);
+  const double g = (y - fl) * 0.69314718055994531;
+  double p = 1.0;
+  for (int i = 16; i >= 1; --i) p = 1.0 + (g / (double)i) * p;
+  const double r = ldexp(p, (int)fl);
+  double rk1 = 1.0;
+  for (int i = 1; i < k; ++i) rk1 = rk1 * r;
+  return r - (rk1 * r - x) / ((double)k * rk1);
+}
+
+// complex double helpers of the step rule (fused multiply-adds in the
+// oracle's order)
+__device__ __forceinline__ void zmac(double2 &z, const double2 &x, const double2 &y) {
+  z.x = __builtin_fma(x.x, y.x, z.x);
+  z.x = __builtin_fma(-x.y, y.y, z.x);
+  z.y = __builtin_fma(x.y, y.x, z.y);
+  z.y = __builtin_fma(x.x, y.y, z.y);
+}
+__device__ __forceinline__ void zcjmac(double2 &z, const double2 &x, const double2 &y) {
+  z.x = __builtin_fma(x.x, y.x, z.x);
+  z.x = __builtin_fma(x.y, y.y, z.x);
+  z.y = __builtin_fma(-x.y, y.x, z.y);
+  z.y = __builtin_fma(x.x, y.y, z.y);
+}
+__device__ __forceinline__ double2 zmul(const double2 &x, const double2 &y) {
+  return make_double2(__builtin_fma(x.x, y.x, -(x.y * y.y)), __builtin_fma(x.x, y.y, x.y * y.x));
+}
+__device__ __forceinline__ double zrad(double re, double im) { return sqrt(re * re + im * im); }
+template <int K> __device__ __forceinline__ double2 lead(const cplx<K> &z) {
+  return make_double2(R<K>::hi(z.re), R<K>::hi(z.im));
+}
+
+// largest eigenvalue of M^H M (sigma_max(M)^2), M(i, j) the leading limbs
+// of an n x n complex matrix: the Gram matrix scaled by its trace and
+// squared six times, then the Rayleigh quotient |M v|^2/|v|^2 of the column
+// v of largest diagonal entry (the oracle's otr_sigma_max_sq, same
+// operations).  The Gram matrix and its powers are Hermitian: lanes own
+// upper entries l, l+64, ... and store them with their conjugates; for
+// n <= 16 a squaring keeps its new entries in registers until every lane
+// has read the old ones, beyond that it goes through Q; Y holds M v (n
+// entries).  Every lane returns the value.
+// row and column of entry q of the upper triangle (row-major, i <= j)
+__device__ __forceinline__ void upper_ij(int n, int q, int &i, int &j) {
+  i = 0;
+  while (q >= n - i) { q -= n - i; ++i; }
+  j = i + q;
+}
+
+template <typename MF>
+__device__ double sigma_max_sq(int n, MF M, double2 *P, double2 *Q, double2 *Y) {
+  const int l = lane_id(), nu = n * (n + 1) / 2;
+  // Hermitian: entries i <= j computed, i > j their conjugates
+  for (int q = l; q < nu; q += 64) {
+    int i, j;
+    upper_ij(n, q, i, j);
+    double2 acc = make_double2(0.0, 0.0);
+    for (int k = 0; k < n; ++k) zcjmac(acc, M(k, i), M(k, j));
+    P[i * n + j] = acc;
+    P[j * n + i] = make_double2(acc.x, i == j ? acc.y : -acc.y);
+  }
+  wsync();
+  double tr = 0.0;
+  for (int i = 0; i < n; ++i) tr = tr + P[i * n + i].x;
+  if (tr != tr) return tr;
+  if (!(tr > 0.0)) return 0.0;
+  // P holds the unscaled power; its scaled entries (x*sc) are formed where
+  // read, with the roundings of the oracle's scaled copy
+  double sc = 1.0 / tr;
+  for (int s = 0; s < 6; ++s) {
+    auto sp = [&](int e) { const double2 v = P[e]; return make_double2(v.x * sc, v.y * sc); };
+    if (n <= 16) {
+      // at most 136 upper entries: three per lane
+      double2 q[3];
+      int qi[3], qj[3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const int e = l + 64 * r;
+        q[r] = make_double2(0.0, 0.0);
+        qi[r] = qj[r] = 0;
+        if (e < nu) {
+          upper_ij(n, e, qi[r], qj[r]);
+          for (int k = 0; k < n; ++k) zmac(q[r], sp(qi[r] * n + k), sp(k * n + qj[r]));
+        }
+      }
+      wsync();
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+        if (l + 64 * r < nu) {
+          P[qi[r] * n + qj[r]] = q[r];
+          P[qj[r] * n + qi[r]] = make_double2(q[r].x, qi[r] == qj[r] ? q[r].y : -q[r].y);
+        }
+    } else {
+      for (int e = l; e < nu; e += 64) {
+        int i, j;
+        upper_ij(n, e, i, j);
+        double2 acc = make_double2(0.0, 0.0);
+        for (int k = 0; k < n; ++k) zmac(acc, sp(i * n + k), sp(k * n + j));
+        Q[i * n + j] = acc;
+        Q[j * n + i] = make_double2(acc.x, i == j ? acc.y : -acc.y);
+      }
+      wsync();
+      for (int e = l; e < n * n; e += 64) P[e] = Q[e];
+    }
+    wsync();
+    tr = 0.0;
+    for (int i = 0; i < n; ++i) tr = tr + P[i * n + i].x;
+    sc = 1.0 / tr;
+  }
+  int jm = 0;
+  double best = P[0].x * sc;
+  for (int i = 1; i < n; ++i) {
+    const double v = P[i * n + i].x * sc;
+    if (v > best) { best = v; jm = i; }
+  }
+  // y = M v in the lanes of the rows, then the sums in row order
+  double2 y = make_double2(0.0, 0.0);
+  if (l < n)
+    for (int k = 0; k < n; ++k) {
+      const double2 pv = P[k * n + jm];
+      zmac(y, M(l, k), make_double2(pv.x * sc, pv.y * sc));
+    }
+  wsync();
+  if (l < n) Y[l] = y;
+  wsync();
+  double num = 0.0, den = 0.0;
+  for (int i = 0; i < n; ++i) num = num + Y[i].x * Y[i].x + Y[i].y * Y[i].y;
+  for (int k = 0; k < n; ++k) {
+    const double2 pv = P[k * n + jm];
+    const double vx = pv.x * sc, vy = pv.y * sc;
+    den = den + vx * vx + vy * vy;
+  }
+  wsync();
+  return num / den;
+}
+
+// the mixed residual mean_i |f_i| / (|f|_i + 1) at the point in the order-0
+// slots after eval_point (rhs = -f, weights set): |f|_i is equation i with
+// the moduli of its coefficients (the components of a monomial added
+// first) at the moduli of x (standard_mixed_residuals.adb:472-484), in FP64
+// on the leading limbs; node moduli go to the scratch, equation values to col
+template <int K> __device__ double mixed_residual(const LProg &P, Path<K> &w) {
+  const int l = lane_id(), n = P.n, D = P.D, G = P.G;
+  double *rn = reinterpret_cast<double *>(w.u);
+  double *mr = reinterpret_cast<double *>(w.col);
+  for (int j = l; j <= n; j += 64)
+    rn[j] = (j < n) ? zrad(R<K>::hi(w.slots[j * D].re), R<K>::hi(w.slots[j * D].im)) : 1.0;
+  wsync();
+  for (int lv = 1; lv <= P.nlev; ++lv) {
+    for (int i = P.lev[lv - 1] + l; i < P.lev[lv]; i += 64) {
+      const int *nd = P.nodes + 8 * i;
+      rn[n + 1 + i] = rn[nd[7]] * rn[nd[2] / D];
+    }
+    wsync();
+  }
+  double acc = 0.0;
+  if (l < n * G) {
+    double2 gz = make_double2(0.0, 0.0);
+    const int t1 = P.seg[l + 1];
+    for (int t = P.seg[l]; t < t1; ++t) {
+      const int q = P.tpair[t], m = P.pairs[4 * q + 3], r = P.pairs[4 * q];
+      const double2 cw = zmul(P.tcoef[t], lead<K>(w.W[3 * r]));
+      gz.x = gz.x + cw.x;
+      gz.y = gz.y + cw.y;
+      if (t + 1 == t1 || P.pairs[4 * P.tpair[t + 1] + 3] != m) {
+        acc = acc + zrad(gz.x, gz.y) * rn[m];
+        gz = make_double2(0.0, 0.0);
+      }
+    }
+  }
+  if (G > 1) acc = acc + bfly<1>(acc);
+  if (G > 2) acc = acc + bfly<2>(acc);
+  if (G > 4) acc = acc + bfly<4>(acc);
+  if (G > 8) acc = acc + bfly<8>(acc);
+  if (G > 16) acc = acc + bfly<16>(acc);
+  if (G > 32) acc = acc + bfly<32>(acc);
+  if (l < n * G && (l % G) == 0) mr[l / G] = acc;
+  wsync();
+  double mix = 0.0;
+  for (int i = 0; i < n; ++i)
+    mix = mix + zrad(R<K>::hi(w.rhs[i].re), R<K>::hi(w.rhs[i].im)) / (mr[i] + 1.0);
+  wsync();
+  return mix / (double)n;
+}
+
+// sum over the equations of sigma_max(Hessian_i)^2 right after the order-0
+// evaluation (every node at order 0): the program's constant for equations
+// of degree <= 2, the Hessian from its terms otherwise (Distance,
+// standard_predictor_convolutions.adb:558-572, with Cached_Singular_Values)
+template <int K> __device__ double hessian_sum(const LProg &P, Path<K> &w) {
+  const int l = lane_id(), n = P.n, nn = n * (n + 1) / 2;
+  double2 *H = reinterpret_cast<double2 *>(w.u), *SP = H + n * n, *SQ = SP + n * n;
+  double acc = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double s = P.hsig[i];
+    if (s < 0.0) {
+      for (int q = l; q < nn; q += 64) {
+        int a = 0, rem = q;
+        while (rem >= n - a) { rem -= n - a; ++a; }
+        const int b = a + rem, e = i * nn + q;
+        double2 z = make_double2(0.0, 0.0);
+        for (int h = P.hent[e]; h < P.hent[e + 1]; ++h) {
+          const int term = P.hterms[3 * h], hp = P.hterms[3 * h + 1];
+          const double mult = (double)P.hterms[3 * h + 2];
+          const int r = P.hpairs[2 * hp], off = P.hpairs[2 * hp + 1];
+          const double2 v = zmul(zmul(P.tcoef[term], lead<K>(w.W[3 * r])), lead<K>(w.slots[off]));
+          z.x = z.x + mult * v.x;
+          z.y = z.y + mult * v.y;
+        }
+        H[a * n + b] = z;
+        H[b * n + a] = z;
+      }
+      wsync();
+      s = sqrt(sigma_max_sq(n, [&](int a, int b) { return H[a * n + b]; }, SP, SQ,
+                            reinterpret_cast<double2 *>(w.col)));
+    }
+    acc = acc + s * s;
+  }
+  return acc;
+}
+
+// Newton corrector with the mixed residual (LU_Newton_Steps,
+// standard_inlined_newton_circuits.adb:388-422): converged when the mixed
+// residual and max|dx| are both <= tol, then one extra step unless the
+// residual or the update is zero; stops when both grow.  1 converged, 0
+// not, -1 singular.  evaluated: the products and the residual at the point
+// are in place, so the first iteration only forms the Jacobian.
+template <int K>
+__device__ int lu_newton_steps(const LProg &P, Path<K> &w, int maxit, double tol, bool evaluated,
+                               double *dxl, int *count, int *nfirst, double *dxh) {
+  typedef TC<K> Cx;
+  const int l = lane_id();
+  double prev_err = 1.0, prev_mix = 1.0;
+  int cnt = 0;
+  for (int k = 1; k <= maxit + 1; ++k) {
+    ++*count;
+    if (k == 1 && !evaluated) {
+      eval_point<K>(P, w, true);
+      ++*nfirst;
+    } else {
+      jacobian<K>(P, w);
+    }
+    if (!gj_invert<K>(P, w)) return -1;
+    const cplx<K> dx = apply_inverse<K>(P, w);
+    double mdx = 0.0;
+    if (l < P.n * P.G && (l % P.G) == 0) {
+      cplx<K> &x = w.slots[(l / P.G) * P.D];
+      x = Cx::add(x, dx);
+      mdx = absz<K>(dx);
+    }
+    wsync();
+    const double err = wave_max(mdx);
+    *dxl = err;
+    if (dxh) { dxh[3] = dxh[2]; dxh[2] = dxh[1]; dxh[1] = dxh[0]; dxh[0] = err; }
+    eval_point<K>(P, w, false);
+    const double res = max_abs_rhs<K>(P, w);
+    const double mix = mixed_residual<K>(P, w);
+    if (mix <= tol && err <= tol) {
+      if (cnt == 1 || res == 0.0 || err == 0.0) return 1;
+      ++cnt;
+    } else if (k > 1 && mix > prev_mix && err > prev_err) {
+      return 0;
+    }
+    prev_err = err;
+    prev_mix = mix;
+  }
+  return 0;
+}
+
 __device__ void stage_program(const DevProgram &G, const ProgLayout &L, unsigned char *lds) {
   auto cp32 = [&](size_t off, const void *src, size_t bytes) {
     const unsigned *s = (const unsigned *)src;
@@ -611,6 +928,10 @@ __device__ LProg lprog(const DevProgram &G, const ProgLayout &L, unsigned char *
   P.jent = (const int *)(lds + L.jent);
   P.jterms = (const int *)(lds + L.jterms);
   P.weights = G.weights;
+  P.hsig = G.hsig;
+  P.hent = G.hent_ptr;
+  P.hterms = G.hterms;
+  P.hpairs = G.hpairs;
   return P;
 }
 
@@ -662,10 +983,12 @@ __device__ void store_point(const LProg &P, Path<K> &w, double *x, int npaths, i
 }
 
 // Taylor coefficients x_1..x_deg at the point in the order-0 slots (weights
-// set at t0); returns false on a singular Jacobian
-template <int K> __device__ bool predictor_series(const LProg &P, Path<K> &w) {
+// set at t0) and *hs2, the Hessian sum at the point; returns false on a
+// singular Jacobian, else A holds its inverse
+template <int K> __device__ bool predictor_series(const LProg &P, Path<K> &w, double *hs2) {
   const int l = lane_id(), n = P.n, D = P.D;
   eval_point<K>(P, w, true);
+  *hs2 = hessian_sum<K>(P, w);
   if (!gj_invert<K>(P, w)) return false;
   if (l < n) w.slots[l * D + 1] = TC<K>::zero();
   wsync();
@@ -688,7 +1011,7 @@ template <int K>
 __device__ void step_features(const LProg &P, Path<K> &w, double rad, double maxstep, double hprev,
                               double t, const double *dxh, int rejprev, double *f) {
   const int l = lane_id(), n = P.n, D = P.D;
-  f[0] = lg2((rad < INFINITY && rad > 0.0) ? rad : maxstep);
+  f[0] = lg2(rad);
   f[1] = lg2(hprev);
   f[2] = t;
   for (int k = 1; k <= 8; ++k) {
@@ -779,9 +1102,80 @@ template <int K> __device__ __forceinline__ void pade2_coefficients(cplx<K> *c, 
   c[L + 2] = b0;
 }
 
+// [L/PD] Pade coefficients of one component in place (PD = 0 keeps the
+// Taylor polynomial) and the error estimate of Solution_Error_Estimate
+// (homotopy_pade_approximants.adb:558-568,597-622) from the series before
+// it is replaced: top = the largest i <= L+PD+2 with |c_i| > 1e-5 (c_0 =
+// x0), est = c_top + sum_{i<=min(PD,top)} d_i c_{top-i}, est = num_top - est
+// when top <= L, est = 1 when top < 0 or |est| < 1e-5 (leading limbs)
+template <int K, int PD>
+__device__ double2 pade_with_error(cplx<K> *c, int L, cplx<K> *x0) {
+  typedef TC<K> Cx;
+  int top = -1;
+  for (int i = L + PD + 2; i >= 0; --i) {
+    if (absz<K>(i == 0 ? *x0 : c[i]) > 1e-5) { top = i; break; }
+  }
+  cplx<K> o0 = Cx::zero(), o1 = Cx::zero(), o2 = Cx::zero();
+  if (top >= 0) o0 = (top == 0) ? *x0 : c[top];
+  if (PD >= 1 && top >= 1) o1 = (top == 1) ? *x0 : c[top - 1];
+  if (PD >= 2 && top >= 2) o2 = (top == 2) ? *x0 : c[top - 2];
+  if (PD == 1) {
+    // 1x1 denominator system c_L d = -c_{L+1}: numerator c_i + d c_{i-1}
+    // in place, d in place of c_{L+1}; a zero c_L leaves the denominator 1
+    // and the Taylor numerator when c_{L+1} is zero too, else a zero
+    // numerator (Pade, lufac info /= 0)
+    const cplx<K> cl = c[L], cn = c[L + 1];
+    if (R<K>::hi(cl.re) == 0.0 && R<K>::hi(cl.im) == 0.0) {
+      if (R<K>::hi(cn.re) != 0.0 || R<K>::hi(cn.im) != 0.0) {
+        for (int i = 1; i <= L; ++i) c[i] = Cx::zero();
+        *x0 = Cx::zero();
+      }
+      c[L + 1] = Cx::zero();
+    } else {
+      const cplx<K> d = Cx::div(Cx::neg(cn), cl);
+      for (int i = L; i >= 1; --i) c[i] = Cx::add(c[i], Cx::mul(d, c[i - 1]));
+      c[L + 1] = d;
+    }
+  }
+  if (PD == 2) pade2_coefficients<K>(c, L, x0);
+  cplx<K> res;
+  if (top < 0) {
+    res = Cx::from(1.0, 0.0);
+  } else {
+    res = o0;
+    if (PD >= 1 && top >= 1) res = Cx::add(res, Cx::mul(c[L + 1], o1));
+    if (PD >= 2 && top >= 2) res = Cx::add(res, Cx::mul(c[L + 2], o2));
+    if (top <= L) res = Cx::sub(top == 0 ? *x0 : c[top], res);
+  }
+  if (absz<K>(res) < 1e-5) res = Cx::from(1.0, 0.0);
+  return lead<K>(res);
+}
+
+// the predicted component at step s: the numerator (Taylor polynomial of
+// degree L when PD = 0) by Horner with x0 added last, over 1 + d_1 s + ...
+template <int K, int PD>
+__device__ cplx<K> pade_value(const cplx<K> *c, int L, const cplx<K> &x0, double s) {
+  typedef TC<K> Cx;
+  cplx<K> acc = c[L];
+  for (int k = L - 1; k >= 1; --k) acc = Cx::add(Cx::scale_d(acc, s), c[k]);
+  cplx<K> p = Cx::add(x0, Cx::scale_d(acc, s));
+  if (PD >= 1) {
+    cplx<K> den = c[L + PD];
+    for (int k = PD - 1; k >= 1; --k) den = Cx::add(Cx::scale_d(den, s), c[L + k]);
+    p = Cx::div(p, Cx::add(Cx::scale_d(den, s), Cx::from(1.0, 0.0)));
+  }
+  return p;
+}
+
 // PD: denominator degree of the predictor (0: Taylor polynomial, 1 or 2:
 // [L/PD] Pade approximant), a template parameter so that the Taylor variant
-// keeps its registers
+// keeps its registers.  One step (LU_Prediction,
+// standard_predictor_convolutions.adb:1142-1185, and
+// Predictor_Corrector_Loop, predictor_corrector_loops.adb:16-83): series,
+// pole step from Fabry's ratio, curvature step from the singular values and
+// the Pade error estimates, Minimum, Bounded_Update, the feedback halving
+// the step while the mixed residual of the prediction is >= alpha, then the
+// corrector, halving the step on failure.
 template <int K, int MW, int PD>
 __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, const Params *__restrict__ parp, int npaths,
                                                    const double *__restrict__ start, int KS,
@@ -803,22 +1197,23 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, const Para
   const LProg P = lprog(G, PL, lds);
   Path<K> w = carve<K>(WL, lds + PL.total + (threadIdx.x >> 6) * WL.total);
   const int l = lane_id(), n = P.n, D = P.D;
+  double2 *mr = reinterpret_cast<double2 *>(w.col);
   for (;;) {
     int path = 0;
     if (l == 0) path = atomicAdd(counter, 1);
     path = __shfl(path, 0, 64);
     if (path >= npaths) break;
-    double t = 0.0, h = par.maxstep, dxl = 0.0, hprev = par.maxstep;
+    double t = 0.0, dxl = 0.0, hprev = par.maxstep;
     double dxh[4] = {1.0, 1.0, 1.0, 1.0};
     int st = ST_RUNNING, steps = 0, rej = 0, npred = 0, nnewt = 0, nfeval = 0, rejprev = 0;
-    int ncorr = 0;
+    int ncorr = 0, nbpole = 0, nbhess = 0, nbmaxm = 0, nfirst = 0;
     int nlog = 0, pmode = ctl.mode;
     // a path the learned controller (mode 1) does not finish is tracked
-    // again from its start with the plain pole-step rule; the work counters
-    // add up both attempts, the other outputs are the last attempt's
+    // again from its start with the plain rule; the work counters add up
+    // both attempts, the other outputs are the last attempt's
     for (int attempt = 0; attempt < 2; ++attempt) {
     pmode = (attempt == 0) ? ctl.mode : 0;
-    t = 0.0; h = par.maxstep; dxl = 0.0; hprev = par.maxstep;
+    t = 0.0; dxl = 0.0; hprev = par.maxstep;
     dxh[0] = dxh[1] = dxh[2] = dxh[3] = 1.0;
     st = ST_RUNNING; steps = 0; rej = 0; rejprev = 0;
     load_point<K>(P, w, start, KS, npaths, path);
@@ -826,25 +1221,39 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, const Para
       // ---- predictor: Taylor series of x(t+s) ----
       set_weights<K>(P, w, t);
       ++npred;
-      if (!predictor_series<K>(P, w)) { st = ST_SINGULAR; break; }
-      // ---- pole step from the Fabry ratio ----
+      double hs2;
+      if (!predictor_series<K>(P, w, &hs2)) { st = ST_SINGULAR; break; }
+      // ---- pole step: Fabry's ratio, minimized over the components whose
+      // c_kd is not zero (Is_Zero), the first of them starting the minimum
       const int kd = D - 1 - par.fabry_offset;
-      double r = INFINITY;
-      if (kd >= 1 && l < n) {
-        const double den = absz<K>(w.slots[l * D + kd]);
-        if (den > 0.0) r = absz<K>(w.slots[l * D + kd - 1]) / den;
+      if (l < n) {
+        const double2 a = lead<K>(w.slots[l * D + kd - 1]), b = lead<K>(w.slots[l * D + kd]);
+        double rad = -1.0;
+        if (!(b.x + 1.0 == 1.0 && b.y + 1.0 == 1.0)) {
+          const double nrm = b.x * b.x + b.y * b.y;
+          const double zr = (a.x * b.x + a.y * b.y) / nrm, zi = (a.y * b.x - a.x * b.y) / nrm;
+          rad = zrad(zr, zi);
+        }
+        mr[l].x = rad;
       }
-      r = wave_min(r);
-      h = (r < INFINITY && r > 0.0) ? fmin(par.pbeta * r, par.maxstep) : par.maxstep;
-      // ---- step-size controller ----
+      wsync();
+      bool pfail = true;
+      double r = 0.0;
+      for (int j = 0; j < n; ++j) {
+        const double v = mr[j].x;
+        if (v == -1.0) continue;
+        if (pfail) { r = v; pfail = false; }
+        else if (v < r) r = v;
+      }
+      wsync();
+      const double pole = pfail ? par.maxstep : par.pbeta * r;
+      // ---- step-size controller (reads the Taylor series) ----
       int cand = 0;
       if (pmode != 0) {
         double feat[PHC_CTL_NFEAT];
-        step_features<K>(P, w, r, par.maxstep, hprev, t, dxh, rejprev, feat);
+        step_features<K>(P, w, pfail ? par.maxstep : r, par.maxstep, hprev, t, dxh, rejprev, feat);
         if (pmode == 1) cand = mlp_choose<K>(ctl, w, feat);
         else cand = (int)(mix_hash(ctl.seed, (unsigned)path, (unsigned)(npred - 1)) % (unsigned)ctl.ncand);
-        h *= ctl.mult[cand];
-        if (ctl.pcap > 0.0 && r < INFINITY && r > 0.0) h = fmin(h, ctl.pcap * r);
         // the features and the choice are logged now (the outcome after the
         // corrector), so that they are not kept live across it
         if (ctl.mode == 2 && nlog < ctl.cap && l == 0) {
@@ -855,75 +1264,70 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, const Para
         }
       }
       const int rej0 = rej, nnewt0 = nnewt;
-      if (l < n) w.x0[l] = w.slots[l * D];
-      // ---- [L/1] Pade coefficients per component ----
-      // The 1x1 denominator system c_L d = -c_{L+1} gives the denominator
-      // 1 + d s; the numerator coefficients c_i + d c_{i-1} (i = 1..L)
-      // replace the series in place and d takes the place of c_{L+1}.  A
-      // zero c_L leaves the denominator 1 and the Taylor numerator when
-      // c_{L+1} is zero too, else a zero numerator (Pade, lufac info /= 0).
-      // ---- [L/1] Pade coefficients per component ----
-      // The 1x1 denominator system c_L d = -c_{L+1} gives the denominator
-      // 1 + d s; the numerator coefficients c_i + d c_{i-1} (i = 1..L)
-      // replace the series in place and d takes the place of c_{L+1}.  A
-      // zero c_L leaves the denominator 1 and the Taylor numerator when
-      // c_{L+1} is zero too, else a zero numerator (Pade, lufac info /= 0).
-      if (PD == 1 && l < n) {
-        const int L = par.pnum;
-        cplx<K> *c = w.slots + l * D;
-        const cplx<K> cl = c[L], cn = c[L + 1];
-        if (R<K>::hi(cl.re) == 0.0 && R<K>::hi(cl.im) == 0.0) {
-          if (R<K>::hi(cn.re) != 0.0 || R<K>::hi(cn.im) != 0.0) {
-            for (int i = 1; i <= L; ++i) c[i] = Cx::zero();
-            w.x0[l] = Cx::zero();
-          }
-          c[L + 1] = Cx::zero();
-        } else {
-          const cplx<K> d = Cx::div(Cx::neg(cn), cl);
-          for (int i = L; i >= 1; --i) c[i] = Cx::add(c[i], Cx::mul(d, c[i - 1]));
-          c[L + 1] = d;
-        }
+      // ---- predictor coefficients and their error estimates ----
+      if (l < n) {
+        w.x0[l] = w.slots[l * D];
+        mr[l] = pade_with_error<K, PD>(w.slots + l * D, par.pnum, w.x0 + l);
       }
-      if (PD == 2 && l < n) pade2_coefficients<K>(w.slots + l * D, par.pnum, w.x0 + l);
       wsync();
-      // ---- feedback and corrector; halve the step on failure ----
-      bool accepted = false;
-      while (!accepted) {
-        double tn = t + h;
-        if (tn >= 1.0 - par.minstep) tn = 1.0;
-        const double hs = tn - t;
-        if (l < n) {
-          // numerator (the Taylor polynomial of degree L when pden = 0) by
-          // Horner, then the quotient by the denominator 1 + d s
-          const int L = par.pnum;
-          cplx<K> acc = w.slots[l * D + L];
-          for (int k = L - 1; k >= 1; --k) acc = Cx::add(Cx::scale_d(acc, hs), w.slots[l * D + k]);
-          cplx<K> p = Cx::add(w.x0[l], Cx::scale_d(acc, hs));
-          if (PD >= 1) {
-            cplx<K> den = w.slots[l * D + L + PD];
-            for (int k = PD - 1; k >= 1; --k) den = Cx::add(Cx::scale_d(den, hs), w.slots[l * D + L + k]);
-            p = Cx::div(p, Cx::add(Cx::scale_d(den, hs), Cx::from(1.0, 0.0)));
-          }
-          w.slots[l * D] = p;
-        }
+      double nrm = 0.0;
+      for (int j = 0; j < n; ++j) nrm = nrm + mr[j].x * mr[j].x + mr[j].y * mr[j].y;
+      nrm = sqrt(nrm);
+      wsync();
+      // ---- curvature step: sigma_min(J) = 1 / sigma_max(J^{-1}) ----
+      double step;
+      {
+        const double lam = sigma_max_sq(
+            n, [&](int a, int b) { return lead<K>(w.A[a * n + b]); },
+            reinterpret_cast<double2 *>(w.u), reinterpret_cast<double2 *>(w.u) + n * n, mr);
+        const double sig1 = 1.0 / sqrt(lam);
+        const double eta = (2.0 * sig1) / sqrt(hs2);
+        const double curv = kth_root((par.cbeta * eta) / nrm, D - 1);
+        // Minimum (three_way_minima.adb:49-62)
+        if (pole < curv) {
+          if (pole < par.maxstep) { step = pole; ++nbpole; }
+          else { step = par.maxstep; ++nbmaxm; }
+        } else if (curv < par.maxstep) { step = curv; ++nbhess; }
+        else { step = par.maxstep; ++nbmaxm; }
+      }
+      if (pmode != 0) {
+        step *= ctl.mult[cand];
+        if (ctl.pcap > 0.0 && !pfail && r > 0.0) step = fmin(step, ctl.pcap * r);
+      }
+      // ---- Bounded_Update (three_way_minima.adb:96-109) ----
+      double tn;
+      if (t + step >= 1.0 - par.minstep) { step = 1.0 - t; tn = 1.0; }
+      else tn = t + step;
+      // ---- Predictor_Feedback (standard_predictor_convolutions.adb:958-993) ----
+      bool evaluated = false;
+      for (;;) {
+        if (l < n) w.slots[l * D] = pade_value<K, PD>(w.slots + l * D, par.pnum, w.x0[l], step);
         wsync();
         set_weights<K>(P, w, tn);
         ++nfeval;
         eval_point<K>(P, w, false);
-        const double res = max_abs_rhs<K>(P, w);
-        bool ok = res <= par.alpha * (1.0 + max_abs_x<K>(P, w));
-        if (ok) {
-          const int nr = newton<K>(P, w, par.corsteps, par.tolcor, &dxl, &nnewt, dxh, true);
-          ++ncorr;
-          if (nr < 0) { st = ST_SINGULAR; break; }
-          ok = (nr == 1);
-        }
-        if (ok) { t = tn; accepted = true; hprev = hs; }
-        else {
-          ++rej;
-          h *= 0.5;
-          if (h < par.minstep) { st = ST_MINSTEP; break; }
-        }
+        const double mix = mixed_residual<K>(P, w);
+        if (mix < par.alpha) { evaluated = true; break; }
+        step = step / 2.0;
+        ++rej;
+        tn = t + step;
+        if (step < par.minstep) break;
+      }
+      // ---- corrector, halving the step on failure ----
+      for (;;) {
+        if (!evaluated) set_weights<K>(P, w, tn);
+        const int nr =
+            lu_newton_steps<K>(P, w, par.corsteps, par.tolcor, evaluated, &dxl, &nnewt, &nfirst, dxh);
+        ++ncorr;
+        if (nr < 0) { st = ST_SINGULAR; break; }
+        if (nr == 1) { t = tn; hprev = step; break; }
+        step = step / 2.0;
+        ++rej;
+        if (step < par.minstep) { st = ST_MINSTEP; break; }
+        tn = t + step;
+        if (l < n) w.slots[l * D] = pade_value<K, PD>(w.slots + l * D, par.pnum, w.x0[l], step);
+        wsync();
+        evaluated = false;
       }
       rejprev = rej - rej0;
       if (ctl.mode == 2 && nlog < ctl.cap) {
@@ -939,7 +1343,8 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, const Para
     }
     if (st == ST_RUNNING) {
       set_weights<K>(P, w, 1.0);
-      const int nr = newton<K>(P, w, par.endsteps, par.tolend, &dxl, &nnewt, nullptr, false);
+      const int nr =
+          lu_newton_steps<K>(P, w, par.endsteps, par.tolend, false, &dxl, &nnewt, &nfirst, nullptr);
       st = (nr < 0) ? ST_SINGULAR : ST_SUCCESS;
     }
     if (!(pmode == 1 && st != ST_SUCCESS)) break;
@@ -956,6 +1361,10 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, const Para
         work[npaths + path] = nnewt;
         work[2 * npaths + path] = nfeval;
         work[3 * npaths + path] = ncorr;
+        work[4 * npaths + path] = nbpole;
+        work[5 * npaths + path] = nbhess;
+        work[6 * npaths + path] = nbmaxm;
+        work[7 * npaths + path] = nfirst;
       }
       if (ctl.mode == 2) ctl.log_n[path] = nlog;
     }
@@ -1022,7 +1431,27 @@ struct phc_tracker {
   std::vector<void *> bufs;
   int *counter = nullptr;
   void *args = nullptr;  // the kernel's Params and Ctl (one block per tracker)
+  // the last call's stream and an event after its kernels: a call on another
+  // stream first waits for it, so calls on one tracker never overlap
+  hipEvent_t done = nullptr;
+  void *last_stream = nullptr;
+  bool used = false;
 };
+
+namespace {
+// orders a call on `stream` after the tracker's previous call
+int tracker_enter(phc_tracker *tr, void *stream) {
+  if (tr->used && stream != tr->last_stream)
+    PHC_HIP(hipStreamWaitEvent((hipStream_t)stream, tr->done, 0));
+  return 0;
+}
+int tracker_leave(phc_tracker *tr, void *stream) {
+  PHC_HIP(hipEventRecord(tr->done, (hipStream_t)stream));
+  tr->last_stream = stream;
+  tr->used = true;
+  return 0;
+}
+}  // namespace
 
 namespace {
 struct TrackArgs {
@@ -1163,6 +1592,38 @@ int phc_program_validate(const phc_program *d) {
     const int ex = jt >> (JT_TERM_BITS + JT_PAIR_BITS);
     PHC_REQUIRE(term < d->nterm && q < d->njpair && ex >= 1, "phc_program: bad Jacobian term");
   }
+  // node numbers: the left factor's and every pair's, against their storage
+  auto node_off = [&](int id) {
+    if (id < 0 || id > d->n + d->nprod) return -1;
+    return id <= d->n ? id * D : d->nodes[8 * (id - d->n - 1) + 3];
+  };
+  for (int i = 0; i < d->nprod; ++i) {
+    const int *nd = d->nodes + 8 * i;
+    PHC_REQUIRE(nd[7] < d->n + 1 + i && node_off(nd[7]) == nd[0],
+                "phc_program: the left factor's node number does not match its storage");
+  }
+  for (int q = 0; q < d->npair; ++q)
+    PHC_REQUIRE(node_off(d->pairs[4 * q + 3]) == d->pairs[4 * q + 1],
+                "phc_program: a pair's node number does not match its storage");
+  // Hessians
+  PHC_REQUIRE(d->hsig && d->hent_ptr && d->nhterm >= 0 && d->nhpair >= 0,
+              "phc_program: missing Hessian data");
+  const int nn = d->n * (d->n + 1) / 2;
+  for (int i = 0; i < d->n; ++i)
+    PHC_REQUIRE(d->hsig[i] >= 0.0 || d->hsig[i] == -1.0,
+                "phc_program: hsig must be >= 0 or -1");
+  PHC_REQUIRE(d->hent_ptr[0] == 0 && d->hent_ptr[d->n * nn] == d->nhterm,
+              "phc_program: bad Hessian entries");
+  for (int e = 0; e < d->n * nn; ++e)
+    PHC_REQUIRE(d->hent_ptr[e + 1] >= d->hent_ptr[e], "phc_program: Hessian entries not increasing");
+  for (int h = 0; h < d->nhterm; ++h)
+    PHC_REQUIRE(d->hterms[3 * h] >= 0 && d->hterms[3 * h] < d->nterm && d->hterms[3 * h + 1] >= 0 &&
+                    d->hterms[3 * h + 1] < d->nhpair && d->hterms[3 * h + 2] >= 1,
+                "phc_program: bad Hessian term");
+  for (int q = 0; q < d->nhpair; ++q)
+    PHC_REQUIRE(d->hpairs[2 * q] >= 0 && d->hpairs[2 * q] < d->ncomp && d->hpairs[2 * q + 1] >= 0 &&
+                    d->hpairs[2 * q + 1] < d->nslots,
+                "phc_program: bad Hessian pair");
   return 0;
 }
 
@@ -1186,6 +1647,13 @@ int phc_tracker_create(phc_tracker **out, const phc_program *d) {
   rc |= upload<int>(tr, d->jent_ptr, (size_t)d->n * d->n + 1, &P.jent_ptr);
   rc |= upload<int>(tr, d->jterms, d->njterm, &P.jterms);
   rc |= upload<double>(tr, d->weights, (size_t)16 * 3 * d->ncomp, &P.weights);
+  rc |= upload<double>(tr, d->hsig, d->n, &P.hsig);
+  rc |= upload<int>(tr, d->hent_ptr, (size_t)d->n * (d->n * (d->n + 1) / 2) + 1, &P.hent_ptr);
+  rc |= upload<int>(tr, d->hterms, (size_t)3 * d->nhterm, &P.hterms);
+  rc |= upload<int>(tr, d->hpairs, (size_t)2 * d->nhpair, &P.hpairs);
+  P.hdyn = 0;
+  for (int i = 0; i < d->n; ++i)
+    if (d->hsig[i] < 0.0) P.hdyn = 1;
   void *c = nullptr, *a = nullptr;
   if (rc == 0 && hipMalloc(&a, sizeof(TrackArgs)) == hipSuccess) tr->bufs.push_back(a);
   else a = nullptr;
@@ -1196,6 +1664,12 @@ int phc_tracker_create(phc_tracker **out, const phc_program *d) {
   }
   tr->counter = (int *)c;
   tr->args = a;
+  if (hipEventCreateWithFlags(&tr->done, hipEventDisableTiming) != hipSuccess) {
+    tr->done = nullptr;
+    phc_tracker_destroy(tr);
+    phc::set_error("phc_tracker_create: event creation failed");
+    return -1;
+  }
   *out = tr;
   return 0;
 }
@@ -1204,6 +1678,7 @@ int phc_tracker_destroy(phc_tracker *tr) {
   if (!tr) return 0;
   for (void *p : tr->bufs) (void)hipFree(p);
   (void)hipFree(tr->counter);
+  if (tr->done) (void)hipEventDestroy(tr->done);
   delete tr;
   return 0;
 }
@@ -1231,10 +1706,13 @@ int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, d
               prm->endsteps >= 0, "phc_track: step parameters must be positive");
   PHC_REQUIRE(prm->pade_dendeg >= 0 && prm->pade_dendeg <= 2,
               "phc_track: pade_dendeg must be 0, 1 or 2");
+  PHC_REQUIRE(tr->P.D >= 4, "phc_track: the step rule needs series degree >= 3");
   PHC_REQUIRE(prm->pade_dendeg == 0
-                  ? (prm->pade_numdeg >= 0 && prm->pade_numdeg <= tr->P.D - 1)
-                  : (prm->pade_numdeg >= 1 && prm->pade_numdeg <= tr->P.D - 1 - prm->pade_dendeg),
-              "phc_track: pade_numdeg must be in 0..deg (Taylor) or 1..deg-M (Pade [L/M])");
+                  ? (prm->pade_numdeg >= 0 && prm->pade_numdeg <= tr->P.D - 3)
+                  : (prm->pade_numdeg >= 1 && prm->pade_numdeg + prm->pade_dendeg <= tr->P.D - 3),
+              "phc_track: pade_numdeg + pade_dendeg + 2 must not exceed the series degree "
+              "(the error estimate reads coefficient L+M+2)");
+  PHC_REQUIRE(prm->cbeta >= 0.0, "phc_track: cbeta must be >= 0");
   if (npaths == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   int grid = 0, wv = 0;
@@ -1247,7 +1725,8 @@ int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, d
   par.pbeta = prm->pbeta; par.alpha = prm->alpha; par.tolcor = prm->tolcor;
   par.tolend = prm->tolend; par.divergence = prm->divergence;
   par.pden = prm->pade_dendeg;
-  par.pnum = (prm->pade_dendeg == 0 && prm->pade_numdeg == 0) ? tr->P.D - 1 : prm->pade_numdeg;
+  par.pnum = (prm->pade_dendeg == 0 && prm->pade_numdeg == 0) ? tr->P.D - 3 : prm->pade_numdeg;
+  par.cbeta = prm->cbeta;
   Ctl ctl;
   ctl.mode = 0; ctl.ncand = 1; ctl.cap = 0; ctl.w = ctl.mu = ctl.sd = nullptr; ctl.thresh = 0.0f;
   ctl.seed = 0; ctl.log_x = nullptr; ctl.log_c = ctl.log_y = ctl.log_n = nullptr; ctl.pcap = 0.0;
@@ -1271,9 +1750,10 @@ int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, d
     PHC_REQUIRE(ctlp->pole_cap >= 0.0, "phc_track: pole_cap must be >= 0");
     ctl.pcap = ctlp->pole_cap;
   }
+  if (tracker_enter(tr, stream) != 0) return -1;
   PHC_HIP(hipMemsetAsync(tr->counter, 0, sizeof(int), st));
-  // the argument block is written in stream order, like the queue counter:
-  // calls on one tracker must not overlap (the same stream, or synchronized)
+  // the argument block is written in stream order, like the queue counter
+  // (tracker_enter orders this call after the previous one on the tracker)
   TrackArgs blk;
   blk.par = par;
   blk.ctl = ctl;
@@ -1306,7 +1786,7 @@ int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, d
   }
 #undef TK
   PHC_HIP(hipGetLastError());
-  return 0;
+  return tracker_leave(tr, stream);
 }
 
 int phc_refine(phc_tracker *tr, int K, int npaths, double t, const double *xin, int KS,
@@ -1324,6 +1804,7 @@ int phc_refine(phc_tracker *tr, int K, int npaths, double t, const double *xin, 
   int grid = 0, wv = 0;
   size_t lds = 0;
   if (geometry(tr->P, K, waves, REFINE_MAX_WAVES, npaths, &grid, &wv, &lds) != 0) return -2;
+  if (tracker_enter(tr, stream) != 0) return -1;
   PHC_HIP(hipMemsetAsync(tr->counter, 0, sizeof(int), st));
   const void *fn = K == 1 ? (const void *)refine_kernel<1> : K == 2 ? (const void *)refine_kernel<2>
                                                            : (const void *)refine_kernel<4>;
@@ -1332,7 +1813,7 @@ int phc_refine(phc_tracker *tr, int K, int npaths, double t, const double *xin, 
   if (K == 1) RK(1); else if (K == 2) RK(2); else RK(4);
 #undef RK
   PHC_HIP(hipGetLastError());
-  return 0;
+  return tracker_leave(tr, stream);
 }
 
 }  // extern "C"

@@ -72,14 +72,44 @@ def complex_costs(K, table=BUILD):
                 cdiv=6 * t["mul"] + 3 * t["add"] + 2 * t["div"])
 
 
+def _groups(prog):
+    """monomial groups of the mixed residual: runs of terms of one node in a
+    segment"""
+    nseg = len(prog.seg_ptr) - 1
+    node = prog.pairs.reshape(-1, 4)[:, 3]
+    g = 0
+    for sgi in range(nseg):
+        prev = None
+        for t in range(prog.seg_ptr[sgi], prog.seg_ptr[sgi + 1]):
+            m = node[prog.term_pair[t]]
+            if m != prev:
+                g += 1
+            prev = m
+    return g
+
+
+def _sigma_flops(n):
+    """FP64 flops of one largest-singular-value estimate of an n x n complex
+    matrix: the upper triangle of the Gram matrix (n(n+1)/2 entries of n
+    complex multiply-adds, 8 flops each), of six squarings whose operands
+    are scaled as they are read (12 flops per multiply-add), traces, the
+    column and the Rayleigh quotient"""
+    nu = n * (n + 1) // 2
+    return 8 * nu * n + 6 * 12 * nu * n + 7 * n + 10 * n * n + 6 * n + 2
+
+
 def _phase_ops(prog):
     """complex operation counts of each phase of the tracker kernel:
-    dict phase -> dict op -> count (cmac also stands for cmsub)"""
+    dict phase -> dict op -> count (cmac also stands for cmsub); "fp64"
+    counts plain FP64 flops of the step-control quantities"""
     n, D, G = prog.n, prog.deg + 1, prog.G
     nprod = prog.nprod
     npair, nterm = len(prog.pairs) // 4, len(prog.term_pair)
     njpair, njterm = len(prog.jpairs) // 2, len(prog.jterms)
     nscale = int(((prog.jterms.view(np.uint32) >> 28) != 1).sum()) if njterm else 0
+    ngroup = _groups(prog)
+    ndyn = int((np.asarray(prog.hsig) < 0).sum()) if prog.hsig is not None else 0
+    nhterm = len(prog.hterms) // 3 if prog.hterms is not None else 0
 
     def add(*ds):
         out = {}
@@ -93,43 +123,55 @@ def _phase_ops(prog):
 
     products0 = dict(cmul=nprod)
     ev = add(products0, residual(0))
+    # mixed residual: node moduli, coefficient of every term times its
+    # weight, group moduli, segment sums, the mean (FP64)
+    radial = dict(fp64=4 * n + nprod + 8 * nterm + 6 * ngroup + n * (G - 1) + 7 * n + 1)
     jac = dict(cmul=njpair, cmac=njterm, scale_d=nscale)
     gj = dict(cdiv=n, cmul=2 * n * (n - 1), cmac=n * (n - 1) ** 2)
     apply = dict(cmac=n * n, cadd=n * (G - 1))
-    newton = add(ev, jac, gj, apply, dict(cadd=n))
+    newton = add(jac, gj, apply, dict(cadd=n), ev, radial)
     orders = {}
     for k in range(1, D):
         orders = add(orders, dict(cmul=nprod, cmac=nprod * k), residual(k), apply,
                      dict(cmul=nprod, cmac=nprod, cadd=nprod))
-    pred = add(ev, jac, gj, orders)
-    feval = add(ev, dict(scale_d=n * (D - 1), cadd=n * (D - 1)))
+    # curvature step: sigma_min of the inverse Jacobian, the Hessians of the
+    # equations of degree > 2 and their singular values, Fabry's ratio and
+    # the Pade error estimates
+    step = dict(fp64=_sigma_flops(n) + ndyn * _sigma_flops(n) + 14 * nhterm + 12 * n + 60)
+    pred = add(ev, jac, gj, orders, step)
+    feval = add(ev, dict(scale_d=n * (D - 1), cadd=n * (D - 1)), radial)
     return dict(predictor=pred, newton=newton, feval=feval, eval=ev)
 
 
 def tracker_flops(prog, K, work, table=BUILD):
-    """FP64 flops of one tracking launch given the work counters [4][npaths]
+    """FP64 flops of one tracking launch given the work counters [8][npaths]
     (predictor series, Newton iterations, predicted-residual evaluations,
-    corrector calls: the first Newton iteration of a corrector call reuses
-    the predicted-residual evaluation)."""
+    corrector calls, the three step counts, Newton runs whose first
+    iteration evaluates the point: every other Newton iteration forms the
+    Jacobian at the point evaluated at the end of the previous iteration or
+    by the feedback)."""
     ph = _phase_ops(prog)
     c = complex_costs(K, table)
+    c = dict(c, fp64=1)
 
     def cost(d):
         return sum(c[k] * v for k, v in d.items())
 
-    npred, nnewt, nfeval, ncorr = (int(x.sum()) for x in work)
+    tot = [int(x.sum()) for x in work]
+    npred, nnewt, nfeval, nfirst = tot[0], tot[1], tot[2], tot[7]
     return (npred * cost(ph["predictor"]) + nnewt * cost(ph["newton"])
-            + nfeval * cost(ph["feval"]) - ncorr * cost(ph["eval"]))
+            + nfeval * cost(ph["feval"]) + nfirst * cost(ph["eval"]))
 
 
 def refine_flops(prog0, K, npaths, iters, table=BUILD):
     """FP64 flops of a refinement launch of `iters` Newton iterations on
-    every path plus the final residual evaluation (upper bound: converged
-    paths stop early)."""
+    every path (evaluation, Jacobian, inverse, update) plus the final
+    residual evaluation (upper bound: converged paths stop early)."""
     ph = _phase_ops(prog0)
-    c = complex_costs(K, table)
+    c = dict(complex_costs(K, table), fp64=1)
     cost = lambda d: sum(c[k] * v for k, v in d.items())  # noqa: E731
-    return npaths * (iters * cost(ph["newton"]) + cost(ph["eval"]))
+    it = cost(ph["newton"]) - ph["newton"].get("fp64", 0)   # no mixed residual here
+    return npaths * (iters * it + cost(ph["eval"]))
 
 
 def complex_mac(K, table=BUILD):

@@ -7,7 +7,8 @@ Mirrors the reference's host interface for this path:
   linear_residue CPU_dbl2_linear_residue (Newton/dbl2_bals_host.cpp:548)
   row_newton     dbl2_row_newton_qrstep (Newton/dbl2_newton_method.cpp:429)
                  iterated with staggered degrees as test_dbl2_row_newton
-                 (:1447-1494)
+                 (:1447-1494); exponents > 1 by the common-factor trick of
+                 CPU_dbl2_evaluate_monomials (Newton/dbl2_systems_host.cpp:12-111)
 for K = 1, 2, 4 limbs (dbl, dbl2, dbl4), batched: every array carries a batch
 index after the limb index.  Arrays are numpy float64 in the limb-planar
 layout of include/phcmd.h; the work runs on the GPU through libphcmd.so.
@@ -68,18 +69,23 @@ def linear_residue(K, mat, rhs, sol, tailidx):
 
 
 class RowSystem:
-    """Supports of a polynomial system on the device (phc_rowsys)."""
+    """Supports of a polynomial system on the device (phc_rowsys); exp: the
+    exponent of every variable occurrence, parallel to idx (None: all 1),
+    evaluated with the common-factor trick of CPU_dbl2_evaluate_monomials
+    (Newton/dbl2_systems_host.cpp:12-111)."""
 
-    def __init__(self, dim, monoptr, nvr, idxptr, idx):
+    def __init__(self, dim, monoptr, nvr, idxptr, idx, exp=None):
         self.dim = dim
         self.monoptr, self.nvr = _ip(monoptr), _ip(nvr)
         self.idxptr, self.idx = _ip(idxptr), _ip(idx if len(idx) else [0])
+        self.exp = None if exp is None else _ip(exp if len(exp) else [1])
         self.nmono = int(self.monoptr[-1])
         h = ctypes.c_void_p()
-        check(load().phc_rowsys_create(ctypes.byref(h), dim, self.monoptr.ctypes.data_as(P_int),
-                                       self.nvr.ctypes.data_as(P_int),
-                                       self.idxptr.ctypes.data_as(P_int),
-                                       self.idx.ctypes.data_as(P_int)))
+        check(load().phc_rowsys_create_exp(
+            ctypes.byref(h), dim, self.monoptr.ctypes.data_as(P_int),
+            self.nvr.ctypes.data_as(P_int), self.idxptr.ctypes.data_as(P_int),
+            self.idx.ctypes.data_as(P_int),
+            None if self.exp is None else self.exp.ctypes.data_as(P_int)))
         self.handle = h
 
     def lds_bytes(self, K, deg):

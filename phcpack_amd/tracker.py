@@ -128,7 +128,7 @@ class TrackBuffers:
         self.nsteps = DeviceArray((npaths,), np.int32)
         self.nrej = DeviceArray((npaths,), np.int32)
         self.dx = DeviceArray((npaths,))
-        self.work = DeviceArray((7, npaths), np.int32)
+        self.work = DeviceArray((8, npaths), np.int32)
 
     def to_host(self):
         return dict(endp=self.endp.to_host(), tend=self.tend.to_host(),
@@ -151,8 +151,10 @@ class Tracker:
 
     def track_device(self, d_start, KS, npaths, K, bufs, params=None, waves=None, stream=None,
                      controller=None):
-        """controller: a PhcController (or None for the plain pole-step rule);
-        waves: wavefronts per workgroup (default: the tuned choice, else 12)"""
+        """controller: a PhcController (or None for the plain step rule);
+        waves: wavefronts per workgroup (default: the tuned choice, else 12).
+        Calls on one tracker are ordered by the library: a call on another
+        stream waits on the device for the tracker's previous call."""
         if waves is None:
             waves = self.waves.get(K, 12)
         p = (params or TrackParams()).to_c()

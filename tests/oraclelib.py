@@ -161,7 +161,7 @@ def oracle_track(prog, starts_planes, K, params, nthreads=1, paths=None, control
     out = dict(endp=np.zeros((2 * K, npaths, n)), tend=np.zeros(npaths),
                status=np.zeros(npaths, np.int32), nsteps=np.zeros(npaths, np.int32),
                nrej=np.zeros(npaths, np.int32), dx=np.zeros(npaths),
-               work=np.zeros((7, npaths), np.int32))
+               work=np.zeros((8, npaths), np.int32))
     p = params.to_c()
     cp = C.byref(controller.struct) if controller is not None else None
     if nthreads > 1:
@@ -194,7 +194,7 @@ def oracle_track_trace(prog, starts_planes, K, params, path=0, controller=None, 
     out = dict(endp=np.zeros((2 * K, npaths, n)), tend=np.zeros(npaths),
                status=np.zeros(npaths, np.int32), nsteps=np.zeros(npaths, np.int32),
                nrej=np.zeros(npaths, np.int32), dx=np.zeros(npaths),
-               work=np.zeros((7, npaths), np.int32))
+               work=np.zeros((8, npaths), np.int32))
     trace = np.zeros((cap, 20))
     p = params.to_c()
     cp = C.byref(controller.struct) if controller is not None else None
@@ -308,7 +308,7 @@ class RowSystem:
     monomials monoptr[i]..monoptr[i+1]-1, monomial m the nvr[m] distinct
     variables idx[idxptr[m]:idxptr[m+1]]; cst [K][dim][D], cff [K][nmono][D]."""
 
-    def __init__(self, dim, deg, monoptr, nvr, idxptr, idx, cst, cff):
+    def __init__(self, dim, deg, monoptr, nvr, idxptr, idx, cst, cff, exp=None):
         self.dim, self.deg = dim, deg
         self.monoptr = np.ascontiguousarray(monoptr, np.int32)
         self.nvr = np.ascontiguousarray(nvr, np.int32)
@@ -316,6 +316,8 @@ class RowSystem:
         self.idx = np.ascontiguousarray(idx, np.int32)
         self.cst = np.ascontiguousarray(cst, np.float64)
         self.cff = np.ascontiguousarray(cff, np.float64)
+        # exponents parallel to idx (None: all ones)
+        self.exp = None if exp is None else np.ascontiguousarray(exp, np.int32)
 
     @property
     def nmono(self):
@@ -350,8 +352,9 @@ def _row_newton(lib, fn, K, sysm, start, nbsteps):
     hist = np.zeros((nbsteps, K, dim, D))
     Q, R = np.zeros((K, dim, dim)), np.zeros((K, dim, dim))
     idx = sysm.idx if sysm.idx.size else np.zeros(1, np.int32)
+    exp = None if sysm.exp is None else pi(sysm.exp)
     steps = getattr(lib, fn)(K, dim, sysm.deg, nbsteps, pi(sysm.monoptr), pi(sysm.nvr),
-                             pi(sysm.idxptr), pi(idx), pd(sysm.cst), pd(sysm.cff), pd(x),
+                             pi(sysm.idxptr), pi(idx), exp, pd(sysm.cst), pd(sysm.cff), pd(x),
                              pi(wdeg), pi(tail), pi(up), pi(bs), pd(hist), pd(Q), pd(R))
     return dict(x=x, steps=steps, wdeg=wdeg[:steps], tail=tail[:steps], up=up[:steps],
                 bs=bs[:steps], history=hist[:steps], Q=Q, R=R)
@@ -444,13 +447,19 @@ def poly_value(K, sysm, x):
         cff = np.ascontiguousarray(sysm.cff[:, m0:m1])
         out = np.zeros((K, dim + 1, D))
         idx = sysm.idx if sysm.idx.size else np.zeros(1, np.int32)
-        o.osr_poly_evaldiff(K, 0, dim, int(m1 - m0), sysm.deg, pi(nv), pi(ip), pi(idx), pd(cst),
-                            pd(cff), pd(np.ascontiguousarray(x)), pd(out))
+        exp = None if sysm.exp is None else pi(sysm.exp)
+        o.osr_poly_evaldiff_exp(K, 0, dim, int(m1 - m0), sysm.deg, pi(nv), pi(ip), pi(idx), exp,
+                                pd(cst), pd(cff), pd(np.ascontiguousarray(x)), pd(out))
         val[:, i] = out[:, dim]
     return val
 
 
-def planted_row_system(rng, K, dim, deg, nper=8, nvar=3, supports=None):
+def random_exponents(rng, sysm, values=(1, 2)):
+    """exponents drawn uniformly from values, one per variable occurrence"""
+    return rng.choice(np.array(values, np.int32), size=len(sysm.idx)).astype(np.int32)
+
+
+def planted_row_system(rng, K, dim, deg, nper=8, nvar=3, supports=None, exp=None):
     """random K-limb coefficient series and an exponential-series root
     x_i(t) = c_i exp(r_i t) (c_i in [0.5, 1], r_i in [-1, 1]); the constants
     are set to minus the value of the other terms at that root.  Returns the
@@ -465,7 +474,7 @@ def planted_row_system(rng, K, dim, deg, nper=8, nvar=3, supports=None):
     fact = np.cumprod(np.concatenate([[1.0], np.arange(1, D)]))
     root = np.zeros((K, dim, D))
     root[0] = c[:, None] * r[:, None] ** np.arange(D)[None, :] / fact[None, :]
-    sysm = RowSystem(dim, deg, monoptr, nvr, idxptr, idx, np.zeros((K, dim, D)), cff)
+    sysm = RowSystem(dim, deg, monoptr, nvr, idxptr, idx, np.zeros((K, dim, D)), cff, exp)
     sysm.cst = np.ascontiguousarray(-poly_value(K, sysm, root))
     start = np.zeros((K, dim, D))
     start[0, :, 0] = root[0, :, 0] + 1e-10

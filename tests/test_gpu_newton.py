@@ -80,7 +80,7 @@ def test_linear_residue_batch_matches_oracle(K):
 
 def _device_system(sysm):
     from phcpack_amd.newton import RowSystem
-    return RowSystem(sysm.dim, sysm.monoptr, sysm.nvr, sysm.idxptr, sysm.idx)
+    return RowSystem(sysm.dim, sysm.monoptr, sysm.nvr, sysm.idxptr, sysm.idx, sysm.exp)
 
 
 def test_cyclic_row_newton_matches_fixture():
@@ -151,3 +151,67 @@ def test_row_newton_rejects_bad_arguments():
     dev = RowSystem(2, [0, 1, 2], [1, 1], [0, 1, 2], [0, 1])
     with pytest.raises(PhcError):
         dev.row_newton(8, 4, 3, np.zeros((8, 1, 2, 5)), np.zeros((8, 1, 2, 5)), np.zeros((8, 1, 2, 5)))
+
+
+def c1_systems(rng, K, nsys, dim=8, deg=15):
+    """the row-Newton configuration with exponents: 8 polynomials in 8
+    variables of 8 monomials of 3 distinct variables, exponents uniform in
+    {1, 2}, coefficient series of degree 15 uniform in [-1, 1], random
+    constant start series; nsys systems with these supports"""
+    from oraclelib import random_exponents, rand_md
+    D = deg + 1
+    sup = random_supports(rng, dim, 8, 3)
+    out = []
+    exp = None
+    for _ in range(nsys):
+        sysm = ORowSystem(dim, deg, *sup, rand_md(rng, K, (dim, D)), rand_md(rng, K, (64, D)))
+        if exp is None:
+            exp = random_exponents(rng, sysm)
+        sysm.exp = exp
+        start = np.zeros((K, dim, D))
+        start[0, :, 0] = rng.uniform(-1.0, 1.0, dim)
+        out.append((sysm, start))
+    return out
+
+
+@pytest.mark.parametrize("K", KS)
+def test_row_newton_with_exponents_matches_oracle(K):
+    """exponents {1, 2} (the common-factor trick), the configuration's shape:
+    a batch of systems, 8 staggered steps, every step bit-identical"""
+    print(f"row newton exponents K={K}", flush=True)
+    rng = np.random.default_rng(900 + K)
+    systems = c1_systems(rng, K, 5)
+    assert set(np.unique(systems[0][0].exp)) == {1, 2}
+    cst = np.stack([s.cst for s, _ in systems], axis=1)
+    cff = np.stack([s.cff for s, _ in systems], axis=1)
+    x0 = np.stack([st for _, st in systems], axis=1)
+    dev = _device_system(systems[0][0])
+    out = dev.row_newton(K, 15, 8, cst, cff, x0, history=True)
+    for b, (sysm, start) in enumerate(systems):
+        o = oracle_row_newton(K, sysm, start, 8)
+        st = o["steps"]
+        assert out["steps"][b] == st == 8, b
+        for k in ("wdeg", "tail", "up", "bs"):
+            assert np.array_equal(out[k][b, :st], o[k]), (b, k)
+        for s in range(st):
+            assert np.array_equal(out["history"][s, :, b], o["history"][s]), (b, s)
+        assert np.array_equal(out["Q"][:, b], o["Q"]) and np.array_equal(out["R"][:, b], o["R"])
+
+
+def test_row_newton_with_larger_exponents_matches_oracle():
+    """one, two and four variables per monomial with exponents up to 3"""
+    from oraclelib import random_exponents, rand_md
+    print("row newton exponents up to 3", flush=True)
+    for nvar in (1, 2, 4):
+        rng = np.random.default_rng(910 + nvar)
+        dim, deg = 5, 6
+        sup = random_supports(rng, dim, 4, nvar)
+        sysm = ORowSystem(dim, deg, *sup, rand_md(rng, 2, (dim, deg + 1)),
+                          rand_md(rng, 2, (int(sup[0][-1]), deg + 1)))
+        sysm.exp = random_exponents(rng, sysm, (1, 2, 3))
+        start = np.zeros((2, dim, deg + 1))
+        start[0, :, 0] = rng.uniform(-1.0, 1.0, dim)
+        out = _device_system(sysm).row_newton(2, deg, 6, sysm.cst[:, None], sysm.cff[:, None],
+                                              start[:, None])
+        o = oracle_row_newton(2, sysm, start, 6)
+        assert np.array_equal(out["x"][:, 0], o["x"]), nvar

@@ -56,10 +56,10 @@ def test_track_matches_oracle_wider(K, n, degree, seed, npaths):
 
 
 @pytest.mark.parametrize("K,waves", [(1, 12), (1, 8), (2, 8)])
-@pytest.mark.parametrize("n,degree,seed,num,den", [(3, 2, 1, 5, 1), (4, 2, 2, 6, 1),
+@pytest.mark.parametrize("n,degree,seed,num,den", [(3, 2, 1, 5, 1), (4, 2, 2, 4, 1),
                                                    (3, 3, 3, 2, 1), (4, 2, 2, 4, 0),
-                                                   (3, 2, 1, 6, 2), (4, 2, 2, 4, 2),
-                                                   (3, 3, 3, 1, 2)])
+                                                   (3, 2, 1, 6, 0), (4, 2, 2, 4, 2),
+                                                   (3, 3, 3, 1, 2), (3, 3, 3, 3, 2)])
 def test_track_pade_matches_oracle(K, waves, n, degree, seed, num, den):
     """The [num/den] Pade predictor (den 1, 2) and the Taylor polynomial of
     lower degree (den 0): kernel and oracle agree bit for bit, in the 12-
@@ -74,13 +74,14 @@ def test_track_pade_matches_oracle(K, waves, n, degree, seed, num, den):
     o = oracle_track(tr.prog, points_to_planes(sols), K, prm)
     for key in ("status", "nsteps", "nrej", "tend", "endp", "work"):
         assert np.array_equal(g[key], o[key]), key
-    assert np.mean(g["status"] == 1) >= 0.75
+    assert np.all(g["status"] == 1), np.bincount(g["status"], minlength=6)
 
 
 @pytest.mark.parametrize("K", [1, 2])
 def test_track_pade_rational_path(K):
     """x(t) = (a + b t)/(r - t): the [5/1] predictor is exact, so every
-    corrector takes one Newton iteration; kernel and oracle agree."""
+    corrector converges at its first Newton iteration (plus the extra one
+    unless the update vanishes); kernel and oracle agree."""
     from phcpack_amd.homotopy import Homotopy, System
     from phcpack_amd.tracker import Tracker, TrackParams, planes_to_points, points_to_planes
     a, b, r = 0.6 + 0.2j, -0.3 + 0.9j, 1.25
@@ -95,19 +96,19 @@ def test_track_pade_rational_path(K):
     o = oracle_track(tr.prog, points_to_planes(sols), K, prm)
     for key in ("status", "nsteps", "nrej", "tend", "endp", "work"):
         assert np.array_equal(g[key], o[key]), key
-    assert g["status"][0] == 1 and g["work"][1, 0] == g["work"][3, 0]
+    assert g["status"][0] == 1 and g["work"][3, 0] <= g["work"][1, 0] <= 2 * g["work"][3, 0]
     want = (a + b) / (r - 1)
     assert abs(planes_to_points(g["endp"], K)[0, 0] - want) < 1e-13 * abs(want)
 
 
 def test_track_pade_degrees_refused():
-    """denominator degrees other than 0, 1 and 2, and numerator degrees that
-    leave too few coefficients for the denominator, are refused"""
+    """denominator degrees other than 0, 1 and 2, and degrees that leave
+    the error estimate's coefficient L+M+2 beyond the series, are refused"""
     from phcpack_amd._lib import PhcError
     from phcpack_amd.tracker import Tracker, TrackParams
     p, hom, sols = make(3, 2, 7)
     tr = Tracker(hom, deg=8)
-    for num, den in ((5, 3), (7, 2), (8, 1), (0, 1), (0, 2), (9, 0)):
+    for num, den in ((5, 3), (7, 2), (8, 1), (0, 1), (0, 2), (9, 0), (6, 1), (5, 2), (7, 0)):
         with pytest.raises(PhcError, match="pade"):
             tr.track(sols, K=1, params=TrackParams(pade_numdeg=num, pade_dendeg=den))
 

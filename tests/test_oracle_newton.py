@@ -210,3 +210,108 @@ def test_row_newton_converges_to_planted_root(K):
         errs[k] = np.abs(lead(out["x"]) - lead(root)).max()
     assert errs[2] < 1e-16
     assert errs[K] < errs[2] * 1e-10 or errs[K] == 0.0
+
+
+# ------------------------------------------ exponents > 1 (common factors) --
+
+def c1_system(rng, K, dim=8, deg=15, nper=8, nvar=3):
+    """the row-Newton configuration with exponents: dim polynomials of nper
+    monomials of nvar distinct variables, exponents uniform in {1, 2},
+    coefficient series of degree deg with coefficients uniform in [-1, 1],
+    the constant series random too; start: a random constant series"""
+    from oraclelib import random_exponents, random_supports
+    D = deg + 1
+    monoptr, nvr, idxptr, idx = random_supports(rng, dim, nper, nvar)
+    cff = rand_md(rng, K, (int(monoptr[-1]), D))
+    cst = rand_md(rng, K, (dim, D))
+    sysm = RowSystem(dim, deg, monoptr, nvr, idxptr, idx, cst, cff)
+    sysm.exp = random_exponents(rng, sysm)
+    start = np.zeros((K, dim, D))
+    start[0, :, 0] = rng.uniform(-1.0, 1.0, dim)
+    return sysm, start
+
+
+@needs_ref
+@pytest.mark.parametrize("K", KS)
+def test_row_newton_with_exponents_matches_reference(K):
+    """8 staggered steps at the configuration's shape (8 x 8, three
+    variables per monomial, exponents in {1, 2}, degree 15): the oracle's
+    common-factor evaluation and Newton steps against the reference's
+    CPU_*_product, CPU_*_speel and *_mul_*_d composed in the driver"""
+    rng = np.random.default_rng(800 + K)
+    sysm, start = c1_system(rng, K)
+    assert set(np.unique(sysm.exp)) == {1, 2}
+    a = ref_row_newton(K, sysm, start, 8)
+    b = oracle_row_newton(K, sysm, start, 8)
+    assert a["steps"] == b["steps"] == 8
+    for k in ("wdeg", "tail", "up", "bs", "history", "x", "Q", "R"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+@needs_ref
+@pytest.mark.parametrize("nvar", [1, 2, 4])
+def test_row_newton_with_exponents_other_sizes_match_reference(nvar):
+    """monomials of one, two and four variables (the special cases of the
+    speel products) with exponents up to 3"""
+    from oraclelib import random_exponents
+    rng = np.random.default_rng(810 + nvar)
+    sysm, start = c1_system(rng, 2, dim=5, deg=6, nper=4, nvar=nvar)
+    sysm.exp = random_exponents(rng, sysm, (1, 2, 3))
+    a = ref_row_newton(2, sysm, start, 6)
+    b = oracle_row_newton(2, sysm, start, 6)
+    for k in ("wdeg", "tail", "up", "bs", "history", "x"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_exponents_equal_repeated_variables():
+    """x_j^2 written with the exponent and written as the variable repeated
+    give the same value and gradient (the repeated variable's derivative
+    terms add up to 2 x_j ...), to double-double rounding"""
+    from oraclelib import pi, random_exponents, random_supports
+    rng = np.random.default_rng(820)
+    K, dim, deg = 2, 10, 7   # dim >= the largest total degree (work rows)
+    D = deg + 1
+    monoptr, nvr, idxptr, idx = random_supports(rng, dim, 5, 3)
+    nmono = int(monoptr[-1])
+    exp = rng.choice(np.array([1, 2, 3], np.int32), size=len(idx)).astype(np.int32)
+    cff = rand_md(rng, K, (nmono, D))
+    cst = rand_md(rng, K, (1, D))[:, 0]
+    x = rand_md(rng, K, (dim, D))
+    o = oracle()
+    # polynomial made of the first 5 monomials
+    nb = 5
+    nv, ip = np.ascontiguousarray(nvr[:nb]), np.ascontiguousarray(idxptr[:nb])
+    out_e = np.zeros((K, dim + 1, D))
+    o.osr_poly_evaldiff_exp(K, 0, dim, nb, deg, pi(nv), pi(ip), pi(idx), pi(exp),
+                            pd(np.ascontiguousarray(cst)), pd(np.ascontiguousarray(cff[:, :nb])),
+                            pd(x), pd(out_e))
+    # the same monomials with every variable repeated e times
+    idx_r, nvr_r = [], []
+    for m in range(nb):
+        vs = [int(idx[j]) for j in range(idxptr[m], idxptr[m + 1]) for _ in range(exp[j])]
+        idx_r += vs
+        nvr_r.append(len(vs))
+    idx_r = np.array(idx_r, np.int32)
+    nvr_r = np.array(nvr_r, np.int32)
+    ip_r = np.concatenate([[0], np.cumsum(nvr_r)[:-1]]).astype(np.int32)
+    out_r = np.zeros((K, dim + 1, D))
+    o.osr_poly_evaldiff(K, 0, dim, nb, deg, pi(nvr_r), pi(ip_r), pi(idx_r),
+                        pd(np.ascontiguousarray(cst)), pd(np.ascontiguousarray(cff[:, :nb])),
+                        pd(x), pd(out_r))
+    a, b = out_e[0] + out_e[1], out_r[0] + out_r[1]
+    assert np.abs(a - b).max() <= 1e-28 * max(1.0, np.abs(b).max())
+    assert np.abs(a).max() > 1e-3
+
+
+@pytest.mark.parametrize("K", (2, 4))
+def test_row_newton_with_exponents_converges_to_planted_root(K):
+    """with exponents {1, 2} the staggered Newton method still reaches a
+    planted exponential-series root"""
+    from oraclelib import random_exponents, random_supports
+    rng = np.random.default_rng(830)
+    sup = random_supports(rng, 6, 6, 3)
+    exp = rng.choice(np.array([1, 2], np.int32), size=len(sup[3])).astype(np.int32)
+    sysm, root, start = planted_row_system(np.random.default_rng(831), K, 6, 8, supports=sup,
+                                           exp=exp)
+    out = oracle_row_newton(K, sysm, start, 12)
+    assert np.abs(lead(out["x"]) - lead(root)).max() < {2: 1e-20, 4: 1e-40}[K]
