@@ -88,12 +88,16 @@ def run_mixed(tr, d_start, starts, npaths, bufs, ref, ev, prm, lib, controller=N
     """FP64 tracking (plain pole-step rule, or the learned controller when
     given: a path it does not finish is tracked again with the plain rule
     inside the same kernel), then dd Newton at the endpoints of the paths the
-    tracker finished (the others are passed over on the device).  Finished
-    paths whose dd Newton does not converge are refined in quad-double; with
-    the learned controller, paths that still fail are tracked again with the
-    plain rule (a path the controller lost is recovered, not dropped).
-    Returns (tracking ms, endpoint ms, escalated, re-tracked)."""
+    tracker finished (the others are passed over on the device).  Paths that
+    FP64 tracking could not finish inside the interval (a step below minstep
+    at t < 0.99, say) are tracked again in double-double from their start
+    (precision escalation).  Finished paths whose dd Newton does not converge
+    are refined in quad-double; with the learned controller, paths that
+    still fail are tracked again with the plain rule (a path the controller
+    lost is recovered, not dropped).
+    Returns (tracking ms, endpoint ms, qd-escalated, re-tracked, dd-escalated)."""
     from phcpack_amd._lib import check
+    from phcpack_amd.tracker import escalate_to_dd
     ev.start()
     tr.track_device(d_start, 1, npaths, 1, bufs, prm, controller=controller)
     t_track = ev.stop_ms()
@@ -103,9 +107,16 @@ def run_mixed(tr, d_start, starts, npaths, bufs, ref, ev, prm, lib, controller=N
     t_ref = ev.stop_ms()
     t0 = time.perf_counter()
     st = ref["st"].to_host()
-    tracked = bufs.status.to_host() == 1
+    status1 = bufs.status.to_host()
+    tracked = status1 == 1
     bad = np.nonzero((st != 1) & tracked)[0]
     fix = {}
+    # precision escalation: interior failures of FP64 tracking, tracked in dd
+    esc_idx, esc = escalate_to_dd(tr, starts, status1, bufs.tend.to_host(), prm)
+    if esc is not None:
+        xe = to_qd(esc["endp"], 2)
+        for j, p in enumerate(esc_idx):
+            fix[int(p)] = (xe[:, j], bool(esc["status"][j] == 1))
     nres = 0
     lost = np.zeros(0, np.int64)
     if len(bad):
@@ -132,7 +143,7 @@ def run_mixed(tr, d_start, starts, npaths, bufs, ref, ev, prm, lib, controller=N
     check(lib.phc_synchronize())
     t_ref += (time.perf_counter() - t0) * 1e3
     ref["fix"] = fix
-    return t_track, t_ref, len(bad), nres
+    return t_track, t_ref, len(bad), nres, len(esc_idx)
 
 
 def qd_residuals(tr, x_planes):
@@ -445,6 +456,7 @@ def run(args):
             "good_paths": int(ok.sum()), "max_residual": float(res.max()),
             "track_ms": round(t_track, 3), "refine_ms": round(t_ref, 3),
             "escalated_paths": int(rows[-1][2]), "retracked_paths": int(rows[-1][3]),
+            "dd_escalated_paths": int(rows[-1][4]),
             "steps_per_path": float(out["nsteps"].mean()),
             "rejections_per_path": float(out["nrej"].mean()),
             "newton_steps_per_s": round(int(out["work"][1].sum()) / (t / 1e3), 1),
@@ -684,6 +696,7 @@ def rankdef_block(args, lib, ev, cstruct=None, n=14, rank_deficient=2, seed=SEED
             "paths_per_s": round(float(ok.sum() / (t / 1e3)), 1),
             "good_paths": int(ok.sum()), "tracker_statuses": cls(o["status"]),
             "escalated_paths": int(rows_[-1][2]), "retracked_paths": int(rows_[-1][3]),
+            "dd_escalated_paths": int(rows_[-1][4]),
             "max_good_residual": float(res[ok].max()) if ok.any() else None,
             "distinct_solutions": distinct(c, ok),
             "steps_per_path": float(o["nsteps"].mean()),
@@ -806,7 +819,8 @@ def run_learned(args, n=8, degree=3, ntrain=10, neval=5, seed=SEED):
                      "learned": {"ms": round(ms_l, 3), "good_paths": int(ok_l.sum()),
                                  "steps_per_path": float(ol["nsteps"].mean()),
                                  "rejections_per_path": float(ol["nrej"].mean()),
-                                 "retracked_paths": int(tl[-1][3])},
+                                 "retracked_paths": int(tl[-1][3]),
+                                 "dd_escalated_paths": int(tl[-1][4])},
                      "lost_paths_vs_plain": lost})
         tot["t_plain"] += ms_p
         tot["t_learn"] += ms_l

@@ -19,6 +19,167 @@
 #ifndef PHCMD_COMPAT_H
 #define PHCMD_COMPAT_H
 
+#include <iostream>
+#include <vector>
+
+/* ------------------------------------------------------ job planners ----
+ * ConvolutionJob(s) and AdditionJob(s) with the reference's public API
+ * (Polynomials/convolution_job.h:9-77, convolution_jobs.h, addition_job.h,
+ * addition_jobs.h:12-218): the same layers of the same jobs in the same
+ * order, so callers that build the schedules (and walk or print them) keep
+ * their code.  GPU_*_poly_evaldiff below takes them by value, as the
+ * reference does, and evaluates from a plan of its own.
+ * Job types: -1 coefficient, 0 input, 1 forward, 2 backward, 3 cross. */
+class ConvolutionJob {
+ public:
+  ConvolutionJob(int monomialix, int input1tp, int input1ix, int input2tp, int input2ix,
+                 int outputtp, int outputix);
+  int get_monomial_index(void) const;
+  int get_first_type(void) const;
+  int get_first_input(void) const;
+  int get_second_type(void) const;
+  int get_second_input(void) const;
+  int get_output_type(void) const;
+  int get_output_index(void) const;
+  friend std::ostream &operator<<(std::ostream &os, const ConvolutionJob &job);
+
+ private:
+  int monidx, inp1tp, inp1ix, inp2tp, inp2ix, outptp, outidx;
+};
+
+class ConvolutionJobs {
+ public:
+  explicit ConvolutionJobs(int dim);
+  /* the layers of convolution jobs of a polynomial with nbr monomials,
+   * monomial k a product of the nvr[k] distinct variables idx[k][...] */
+  void make(int nbr, int *nvr, int **idx, bool verbose);
+  int get_dimension(void) const;
+  int get_count(void) const;
+  int get_layer_count(int k) const;
+  int get_depth(void) const;
+  ConvolutionJob get_job(int k, int i) const;
+  ~ConvolutionJobs(void);
+
+ private:
+  int dimension, jobcount, laydepth;
+  std::vector<int> freqlaycnt;
+  std::vector<std::vector<ConvolutionJob> > jobs;
+  void add_job(int layer, const ConvolutionJob &job, bool verbose);
+  void make_monomial(int nvr, int *idx, int monidx, bool verbose);
+};
+
+class AdditionJob {
+ public:
+  /* atp, itp: update and increment type (1 forward, 2 backward, 3 cross);
+   * monidx2 = -1 when the increment is the constant or a coefficient */
+  AdditionJob(int atp, int itp, int monidx1, int monidx2, int ix1, int ix2);
+  int get_addition_type(void) const;
+  int get_increment_type(void) const;
+  int get_update_monomial(void) const;
+  int get_update_index(void) const;
+  int get_increment_monomial(void) const;
+  int get_increment_index(void) const;
+  friend std::ostream &operator<<(std::ostream &os, const AdditionJob &job);
+
+ private:
+  int adtype, intype, updmon, incmon, updidx, incidx;
+};
+
+class AdditionJobs {
+ public:
+  AdditionJobs(int dim, int nbr);
+  void make(int nbr, int *nvr, int **idx, bool verbose);
+  int get_number_of_variables(void) const;
+  int get_number_of_monomials(void) const;
+  int get_count(void) const;
+  int get_layer_count(int k) const;
+  int get_depth(void) const;
+  int get_differential_count(int k) const;
+  int get_differential_index(int k, int i) const;
+  int position(int n, int *idx, int k);
+  AdditionJob get_job(int k, int i) const;
+  ~AdditionJobs(void);
+
+ private:
+  int nbrvar, nbrmon, jobcount, laydepth;
+  std::vector<int> freqlaycnt, difcnt;
+  std::vector<std::vector<int> > difidx;
+  std::vector<std::vector<AdditionJob> > jobs;
+  void add_job(int level, const AdditionJob &job, bool verbose);
+  void tree(int kind, int level, int stride, int nbr, int *nvr, int **idx, int var,
+            bool verbose);
+};
+
+/* ---------------------------------------- polynomial evaluation --------
+ * Polynomials/dbl2_polynomials_kernels.h:812,870 and the dbl, dbl4, dbl8
+ * twins: the value (output[dim]) and the dim derivatives (output[k]) of
+ * cst + sum_k cff[k] * prod_i input[idx[k][i]] at power series truncated at
+ * degree deg, with the result of the reference's CPU_*_poly_evaldiff bit for
+ * bit.  BS (deg + 1 in the reference) and the job schedules are accepted and
+ * not needed; *cnvlapms receives the kernel time, *addlapms 0 (the sums run
+ * in the same kernel), *elapsedms their sum, *walltimesec the wall time of
+ * the call. */
+void GPU_dbl_poly_evaldiff(int BS, int dim, int nbr, int deg, int *nvr, int **idx, double *cst,
+                           double **cff, double **input, double **output,
+                           ConvolutionJobs cnvjobs, AdditionJobs addjobs, double *cnvlapms,
+                           double *addlapms, double *elapsedms, double *walltimesec, int vrblvl);
+void GPU_dbl2_poly_evaldiff(int BS, int dim, int nbr, int deg, int *nvr, int **idx,
+                            double *csthi, double *cstlo, double **cffhi, double **cfflo,
+                            double **inputhi, double **inputlo, double **outputhi,
+                            double **outputlo, ConvolutionJobs cnvjobs, AdditionJobs addjobs,
+                            double *cnvlapms, double *addlapms, double *elapsedms,
+                            double *walltimesec, int vrblvl);
+void GPU_dbl4_poly_evaldiff(int BS, int dim, int nbr, int deg, int *nvr, int **idx,
+                            double *csthihi, double *cstlohi, double *csthilo, double *cstlolo,
+                            double **cffhihi, double **cfflohi, double **cffhilo,
+                            double **cfflolo, double **inputhihi, double **inputlohi,
+                            double **inputhilo, double **inputlolo, double **outputhihi,
+                            double **outputlohi, double **outputhilo, double **outputlolo,
+                            ConvolutionJobs cnvjobs, AdditionJobs addjobs, double *cnvlapms,
+                            double *addlapms, double *elapsedms, double *walltimesec,
+                            int vrblvl);
+void GPU_dbl8_poly_evaldiff(
+    int BS, int dim, int nbr, int deg, int *nvr, int **idx, double *csthihihi,
+    double *cstlohihi, double *csthilohi, double *cstlolohi, double *csthihilo,
+    double *cstlohilo, double *csthilolo, double *cstlololo, double **cffhihihi,
+    double **cfflohihi, double **cffhilohi, double **cfflolohi, double **cffhihilo,
+    double **cfflohilo, double **cffhilolo, double **cfflololo, double **inputhihihi,
+    double **inputlohihi, double **inputhilohi, double **inputlolohi, double **inputhihilo,
+    double **inputlohilo, double **inputhilolo, double **inputlololo, double **outputhihihi,
+    double **outputlohihi, double **outputhilohi, double **outputlolohi,
+    double **outputhihilo, double **outputlohilo, double **outputhilolo,
+    double **outputlololo, ConvolutionJobs cnvjobs, AdditionJobs addjobs, double *cnvlapms,
+    double *addlapms, double *elapsedms, double *walltimesec, int vrblvl);
+/* Polynomials/dbl_polynomials_kernels.h:675, dbl2_polynomials_kernels.h:870,
+ * dbl4_polynomials_kernels.h:1254: complex coefficients and series. */
+void GPU_cmplx_poly_evaldiff(int BS, int dim, int nbr, int deg, int *nvr, int **idx,
+                             double *cstre, double *cstim, double **cffre, double **cffim,
+                             double **inputre, double **inputim, double **outputre,
+                             double **outputim, ConvolutionJobs cnvjobs, AdditionJobs addjobs,
+                             double *cnvlapms, double *addlapms, double *elapsedms,
+                             double *walltimesec, int vrblvl);
+void GPU_cmplx2_poly_evaldiff(int BS, int dim, int nbr, int deg, int *nvr, int **idx,
+                              double *cstrehi, double *cstrelo, double *cstimhi,
+                              double *cstimlo, double **cffrehi, double **cffrelo,
+                              double **cffimhi, double **cffimlo, double **inputrehi,
+                              double **inputrelo, double **inputimhi, double **inputimlo,
+                              double **outputrehi, double **outputrelo, double **outputimhi,
+                              double **outputimlo, ConvolutionJobs cnvjobs,
+                              AdditionJobs addjobs, double *cnvlapms, double *addlapms,
+                              double *elapsedms, double *walltimesec, int vrblvl);
+void GPU_cmplx4_poly_evaldiff(
+    int BS, int dim, int nbr, int deg, int *nvr, int **idx, double *cstrehihi,
+    double *cstrelohi, double *cstrehilo, double *cstrelolo, double *cstimhihi,
+    double *cstimlohi, double *cstimhilo, double *cstimlolo, double **cffrehihi,
+    double **cffrelohi, double **cffrehilo, double **cffrelolo, double **cffimhihi,
+    double **cffimlohi, double **cffimhilo, double **cffimlolo, double **inputrehihi,
+    double **inputrelohi, double **inputrehilo, double **inputrelolo, double **inputimhihi,
+    double **inputimlohi, double **inputimhilo, double **inputimlolo, double **outputrehihi,
+    double **outputrelohi, double **outputrehilo, double **outputrelolo,
+    double **outputimhihi, double **outputimlohi, double **outputimhilo,
+    double **outputimlolo, ConvolutionJobs cnvjobs, AdditionJobs addjobs, double *cnvlapms,
+    double *addlapms, double *elapsedms, double *walltimesec, int vrblvl);
+
 /* ------------------------------------------------ series products ------
  * Convolutions/dbl2_convolutions_kernels.h:121,145 and the dbl, dbl4, dbl8
  * twins: z = x*y truncated at degree deg; freq repeats the launch (the

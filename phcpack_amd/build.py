@@ -65,11 +65,11 @@ def build_oracle():
 
 
 def build_compat_test():
-    """the parity test of the reference-signature layer; it links the
-    compiled reference (oracle/_ref), which exists only where the reference
-    sources are present"""
-    if not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libphcref.so")):
-        return
+    """the test programs of the reference-signature layer (tests/compat):
+    test_compat checks the GPU_* entry points against the CPU oracle,
+    jobs_dump prints the job schedules; jobs_dump_ref (the same program
+    against the reference's job classes) is built where the reference
+    sources and oracle/_ref are present"""
     r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "compat")],
                        capture_output=True, text=True)
     if r.returncode != 0:

@@ -283,6 +283,53 @@ void linear_residue(int K, int dim, int degp1, int tailidx, double ***const *mat
   phc_event_destroy(e1);
 }
 
+// value and gradient of one polynomial at series (P = K or 2K planes):
+// cst[q] has deg+1 doubles, cff[q][k] the deg+1 coefficients of monomial k,
+// input[q][i] the series of variable i, output[q][i] (i <= dim) receives
+// the derivative to x_i, output[q][dim] the value
+void poly_evaldiff(int K, int cplx, int dim, int nbr, int deg, const int *nvr, int **idx,
+                   double *const *cst, double **const *cff, double **const *input,
+                   double **const *output, double *cnvlapms, double *addlapms,
+                   double *elapsedms, double *walltimesec, const char *fn) {
+  const Clock::time_point t0 = Clock::now();
+  const int P = cplx ? 2 * K : K, D = deg + 1;
+  std::vector<int> flat;
+  for (int k = 0; k < nbr; ++k)
+    for (int i = 0; i < nvr[k]; ++i) flat.push_back(idx[k][i]);
+  std::vector<double> hcst((size_t)P * D), hcff((size_t)P * nbr * D), hin, hout;
+  for (int q = 0; q < P; ++q)
+    for (int j = 0; j < D; ++j) hcst[(size_t)q * D + j] = cst[q][j];
+  for (int q = 0; q < P; ++q)
+    for (int k = 0; k < nbr; ++k)
+      for (int j = 0; j < D; ++j) hcff[((size_t)q * nbr + k) * D + j] = cff[q][k][j];
+  pack(input, P, dim, D, hin);
+  phc_poly *plan = nullptr;
+  if (!ok(phc_poly_create(&plan, K, cplx, dim, nbr, deg, nvr, flat.data(), hcst.data(),
+                          hcff.data()), fn))
+    return;
+  float ms = 0.0f;
+  {
+    Buf din(hin.size()), dout((size_t)P * (dim + 1) * D);
+    void *e0 = nullptr, *e1 = nullptr;
+    if (!din.p || !dout.p || !din.up(hin)) {
+      ok(-1, fn);
+    } else if (ok(phc_event_create(&e0), fn) && ok(phc_event_create(&e1), fn) &&
+               ok(phc_event_record(e0, nullptr), fn) &&
+               ok(phc_poly_evaldiff(plan, 1, din.d(), dout.d(), nullptr), fn) &&
+               ok(phc_event_record(e1, nullptr), fn) && ok(phc_synchronize(), fn) &&
+               ok(phc_event_elapsed_ms(e0, e1, &ms), fn) && dout.down(hout)) {
+      unpack(hout, P, dim + 1, D, output);
+    }
+    if (e0) phc_event_destroy(e0);
+    if (e1) phc_event_destroy(e1);
+  }
+  phc_poly_destroy(plan);
+  if (cnvlapms) *cnvlapms = ms;
+  if (addlapms) *addlapms = 0.0;
+  if (elapsedms) *elapsedms = ms;
+  if (walltimesec) *walltimesec = std::chrono::duration<double>(Clock::now() - t0).count();
+}
+
 }  // namespace
 
 // ------------------------------------------------------------ block Toeplitz solve
@@ -598,4 +645,117 @@ void GPU_dbl4_linear_residue(int dim, int degp1, int, int, int tailidx, double *
   double *mx[] = {resmaxhihi, resmaxlohi, resmaxhilo, resmaxlolo};
   linear_residue(4, dim, degp1, tailidx, m, b, x, r, mx, lapms, add, mul,
                  "GPU_dbl4_linear_residue");
+}
+
+// ------------------------------------------------ polynomial evaluation
+
+void GPU_dbl_poly_evaldiff(int, int dim, int nbr, int deg, int *nvr, int **idx, double *cst,
+                           double **cff, double **input, double **output, ConvolutionJobs,
+                           AdditionJobs, double *cnvlapms, double *addlapms, double *elapsedms,
+                           double *walltimesec, int) {
+  double *c[] = {cst};
+  double **f[] = {cff}, **x[] = {input}, **y[] = {output};
+  poly_evaldiff(1, 0, dim, nbr, deg, nvr, idx, c, f, x, y, cnvlapms, addlapms, elapsedms,
+                walltimesec, "GPU_dbl_poly_evaldiff");
+}
+
+void GPU_dbl2_poly_evaldiff(int, int dim, int nbr, int deg, int *nvr, int **idx, double *csthi,
+                            double *cstlo, double **cffhi, double **cfflo, double **inputhi,
+                            double **inputlo, double **outputhi, double **outputlo,
+                            ConvolutionJobs, AdditionJobs, double *cnvlapms, double *addlapms,
+                            double *elapsedms, double *walltimesec, int) {
+  double *c[] = {csthi, cstlo};
+  double **f[] = {cffhi, cfflo}, **x[] = {inputhi, inputlo}, **y[] = {outputhi, outputlo};
+  poly_evaldiff(2, 0, dim, nbr, deg, nvr, idx, c, f, x, y, cnvlapms, addlapms, elapsedms,
+                walltimesec, "GPU_dbl2_poly_evaldiff");
+}
+
+void GPU_dbl4_poly_evaldiff(int, int dim, int nbr, int deg, int *nvr, int **idx,
+                            double *csthihi, double *cstlohi, double *csthilo, double *cstlolo,
+                            double **cffhihi, double **cfflohi, double **cffhilo,
+                            double **cfflolo, double **inputhihi, double **inputlohi,
+                            double **inputhilo, double **inputlolo, double **outputhihi,
+                            double **outputlohi, double **outputhilo, double **outputlolo,
+                            ConvolutionJobs, AdditionJobs, double *cnvlapms, double *addlapms,
+                            double *elapsedms, double *walltimesec, int) {
+  double *c[] = {csthihi, cstlohi, csthilo, cstlolo};
+  double **f[] = {cffhihi, cfflohi, cffhilo, cfflolo};
+  double **x[] = {inputhihi, inputlohi, inputhilo, inputlolo};
+  double **y[] = {outputhihi, outputlohi, outputhilo, outputlolo};
+  poly_evaldiff(4, 0, dim, nbr, deg, nvr, idx, c, f, x, y, cnvlapms, addlapms, elapsedms,
+                walltimesec, "GPU_dbl4_poly_evaldiff");
+}
+
+void GPU_dbl8_poly_evaldiff(
+    int, int dim, int nbr, int deg, int *nvr, int **idx, double *csthihihi, double *cstlohihi,
+    double *csthilohi, double *cstlolohi, double *csthihilo, double *cstlohilo,
+    double *csthilolo, double *cstlololo, double **cffhihihi, double **cfflohihi,
+    double **cffhilohi, double **cfflolohi, double **cffhihilo, double **cfflohilo,
+    double **cffhilolo, double **cfflololo, double **inputhihihi, double **inputlohihi,
+    double **inputhilohi, double **inputlolohi, double **inputhihilo, double **inputlohilo,
+    double **inputhilolo, double **inputlololo, double **outputhihihi, double **outputlohihi,
+    double **outputhilohi, double **outputlolohi, double **outputhihilo, double **outputlohilo,
+    double **outputhilolo, double **outputlololo, ConvolutionJobs, AdditionJobs,
+    double *cnvlapms, double *addlapms, double *elapsedms, double *walltimesec, int) {
+  double *c[] = {csthihihi, cstlohihi, csthilohi, cstlolohi,
+                 csthihilo, cstlohilo, csthilolo, cstlololo};
+  double **f[] = {cffhihihi, cfflohihi, cffhilohi, cfflolohi,
+                  cffhihilo, cfflohilo, cffhilolo, cfflololo};
+  double **x[] = {inputhihihi, inputlohihi, inputhilohi, inputlolohi,
+                  inputhihilo, inputlohilo, inputhilolo, inputlololo};
+  double **y[] = {outputhihihi, outputlohihi, outputhilohi, outputlolohi,
+                  outputhihilo, outputlohilo, outputhilolo, outputlololo};
+  poly_evaldiff(8, 0, dim, nbr, deg, nvr, idx, c, f, x, y, cnvlapms, addlapms, elapsedms,
+                walltimesec, "GPU_dbl8_poly_evaldiff");
+}
+
+void GPU_cmplx_poly_evaldiff(int, int dim, int nbr, int deg, int *nvr, int **idx, double *cstre,
+                             double *cstim, double **cffre, double **cffim, double **inputre,
+                             double **inputim, double **outputre, double **outputim,
+                             ConvolutionJobs, AdditionJobs, double *cnvlapms, double *addlapms,
+                             double *elapsedms, double *walltimesec, int) {
+  double *c[] = {cstre, cstim};
+  double **f[] = {cffre, cffim}, **x[] = {inputre, inputim}, **y[] = {outputre, outputim};
+  poly_evaldiff(1, 1, dim, nbr, deg, nvr, idx, c, f, x, y, cnvlapms, addlapms, elapsedms,
+                walltimesec, "GPU_cmplx_poly_evaldiff");
+}
+
+void GPU_cmplx2_poly_evaldiff(int, int dim, int nbr, int deg, int *nvr, int **idx,
+                              double *cstrehi, double *cstrelo, double *cstimhi,
+                              double *cstimlo, double **cffrehi, double **cffrelo,
+                              double **cffimhi, double **cffimlo, double **inputrehi,
+                              double **inputrelo, double **inputimhi, double **inputimlo,
+                              double **outputrehi, double **outputrelo, double **outputimhi,
+                              double **outputimlo, ConvolutionJobs, AdditionJobs,
+                              double *cnvlapms, double *addlapms, double *elapsedms,
+                              double *walltimesec, int) {
+  double *c[] = {cstrehi, cstrelo, cstimhi, cstimlo};
+  double **f[] = {cffrehi, cffrelo, cffimhi, cffimlo};
+  double **x[] = {inputrehi, inputrelo, inputimhi, inputimlo};
+  double **y[] = {outputrehi, outputrelo, outputimhi, outputimlo};
+  poly_evaldiff(2, 1, dim, nbr, deg, nvr, idx, c, f, x, y, cnvlapms, addlapms, elapsedms,
+                walltimesec, "GPU_cmplx2_poly_evaldiff");
+}
+
+void GPU_cmplx4_poly_evaldiff(
+    int, int dim, int nbr, int deg, int *nvr, int **idx, double *cstrehihi, double *cstrelohi,
+    double *cstrehilo, double *cstrelolo, double *cstimhihi, double *cstimlohi,
+    double *cstimhilo, double *cstimlolo, double **cffrehihi, double **cffrelohi,
+    double **cffrehilo, double **cffrelolo, double **cffimhihi, double **cffimlohi,
+    double **cffimhilo, double **cffimlolo, double **inputrehihi, double **inputrelohi,
+    double **inputrehilo, double **inputrelolo, double **inputimhihi, double **inputimlohi,
+    double **inputimhilo, double **inputimlolo, double **outputrehihi, double **outputrelohi,
+    double **outputrehilo, double **outputrelolo, double **outputimhihi, double **outputimlohi,
+    double **outputimhilo, double **outputimlolo, ConvolutionJobs, AdditionJobs,
+    double *cnvlapms, double *addlapms, double *elapsedms, double *walltimesec, int) {
+  double *c[] = {cstrehihi, cstrelohi, cstrehilo, cstrelolo,
+                 cstimhihi, cstimlohi, cstimhilo, cstimlolo};
+  double **f[] = {cffrehihi, cffrelohi, cffrehilo, cffrelolo,
+                  cffimhihi, cffimlohi, cffimhilo, cffimlolo};
+  double **x[] = {inputrehihi, inputrelohi, inputrehilo, inputrelolo,
+                  inputimhihi, inputimlohi, inputimhilo, inputimlolo};
+  double **y[] = {outputrehihi, outputrelohi, outputrehilo, outputrelolo,
+                  outputimhihi, outputimlohi, outputimhilo, outputimlolo};
+  poly_evaldiff(4, 1, dim, nbr, deg, nvr, idx, c, f, x, y, cnvlapms, addlapms, elapsedms,
+                walltimesec, "GPU_cmplx4_poly_evaldiff");
 }

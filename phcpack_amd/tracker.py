@@ -9,6 +9,7 @@ uses, with the step-control defaults of
 Deformations/Curves/homotopy_continuation_parameters.adb:11-22.
 """
 import ctypes
+import dataclasses
 from dataclasses import dataclass
 
 import numpy as np
@@ -194,3 +195,31 @@ class Tracker:
         check(load().phc_synchronize())
         return dict(x=d_out.to_host(), dx=d_dx.to_host(), res=d_res.to_host(),
                     status=d_st.to_host())
+
+
+# statuses of paths that stopped inside [0, 1) for numerical reasons: step
+# below minstep, maxsteps reached, singular Jacobian (not divergence)
+INTERIOR_FAILURES = (2, 3, 4)
+
+
+def interior_failures(status, tend, t_interior=0.99):
+    """indices of the paths of a tracking run that failed at t < t_interior
+    with a status of INTERIOR_FAILURES.  With the gamma trick the paths are
+    regular for t < 1, so such a failure is one of precision or step
+    control; failures close to t = 1 belong to singular or diverging
+    endpoints, which more precision in the tracker does not repair."""
+    status = np.asarray(status)
+    return np.nonzero(np.isin(status, INTERIOR_FAILURES) & (np.asarray(tend) < t_interior))[0]
+
+
+def escalate_to_dd(tr, start_planes, status, tend, params=None, t_interior=0.99):
+    """Precision escalation of the mixed method: the paths that FP64 tracking
+    could not finish inside the interval (interior_failures) are tracked
+    again from their start in double-double with the plain step rule.
+    start_planes: host [2KS][npaths][n]; returns (indices, tracker output of
+    those paths in double-double, or None when there are none)."""
+    idx = interior_failures(status, tend, t_interior)
+    if len(idx) == 0:
+        return idx, None
+    prm = dataclasses.replace(params or TrackParams(), tolend=1e-28)
+    return idx, tr.track(np.ascontiguousarray(start_planes[:, idx]), K=2, params=prm)

@@ -1,314 +1,267 @@
 // Parity of the reference-signature entry points of libphcmd.so
-// (include/phcmd_compat.h) with the reference's own host functions, compiled
-// from the reference sources into oracle/_ref/libphcref.so.  Written like the
+// (include/phcmd_compat.h) with the CPU oracle (oracle/liboracle.so: this
+// repository's C restatement of the reference's host code, bit-exact with
+// the compiled reference, which the CPU tests check).  Written like the
 // reference's testers (Convolutions/test_dbl*_convolutions.cpp,
-// Matrices/test_dbl4_baqr.cpp, test_dbl4_tabs.cpp): random input, the CPU
+// Polynomials/test_dbl*_polynomials.cpp, Matrices/test_dbl4_baqr.cpp,
+// test_dbl4_tabs.cpp, Newton/test_dbl*_bals.cpp): random input, the CPU
 // function and the GPU function on the same data, the largest difference.
+// The caller side uses the reference's argument conventions throughout (one
+// array per limb and per real/imaginary part, matrices as row pointers,
+// job schedules built by ConvolutionJobs / AdditionJobs).
 //
-// Series products must agree bit for bit; the blocked QR, the tiled solver
+// Series products, polynomial evaluation, the block Toeplitz solve and the
+// linear residue must agree bit for bit; the blocked QR, the tiled solvers
 // and Q^H b split long sums across lanes, so they must agree within
 // 64 * dim * eps^K, and the least-squares solve must recover a planted x0.
 // Exit status = number of failed checks.
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
-#include <cstdlib>
 #include <vector>
 
 #include "../../include/phcmd_compat.h"
-
-// the reference's host functions (oracle/_ref/libphcref.so)
-double random_double();
-void ddf_sub(double, double, double, double, double *, double *);
-void ddf_add(double, double, double, double, double *, double *);
-void ddf_mul(double, double, double, double, double *, double *);
-void qdf_sub(double, double, double, double, double, double, double, double, double *, double *,
-             double *, double *);
-void qdf_add(double, double, double, double, double, double, double, double, double *, double *,
-             double *, double *);
-void qdf_mul(double, double, double, double, double, double, double, double, double *, double *,
-             double *, double *);
-void CPU_dbl_product(int, double *, double *, double *);
-void CPU_cmplx_product(int, double *, double *, double *, double *, double *, double *);
-void CPU_dbl2_product(int, double *, double *, double *, double *, double *, double *);
-void CPU_cmplx2_product(int, double *, double *, double *, double *, double *, double *, double *,
-                        double *, double *, double *, double *, double *);
-void CPU_dbl4_product(int, double *, double *, double *, double *, double *, double *, double *,
-                      double *, double *, double *, double *, double *);
-void CPU_cmplx4_product(int, double *, double *, double *, double *, double *, double *, double *,
-                        double *, double *, double *, double *, double *, double *, double *,
-                        double *, double *, double *, double *, double *, double *, double *,
-                        double *, double *, double *);
-void CPU_cmplx2_blocked_houseqr(int, int, int, int, double **, double **, double **, double **,
-                                double **, double **, double **, double **, double **, double **,
-                                double **, double **, double *, bool);
-void CPU_cmplx4_blocked_houseqr(int, int, int, int, double **, double **, double **, double **,
-                                double **, double **, double **, double **, double **, double **,
-                                double **, double **, double **, double **, double **, double **,
-                                double **, double **, double **, double **, double **, double **,
-                                double **, double **, double *, bool);
-void CPU_cmplx2_upper_tiled_solver(int, int, int, double **, double **, double **, double **,
-                                   double *, double *, double *, double *, double *, double *,
-                                   double *, double *, double *);
-void CPU_cmplx4_upper_tiled_solver(int, int, int, double **, double **, double **, double **,
-                                   double **, double **, double **, double **, double *, double *,
-                                   double *, double *, double *, double *, double *, double *,
-                                   double *, double *, double *, double *, double *, double *,
-                                   double *, double *, double *);
-
-void CPU_dbl_qrbs_solve(int, int, int, double ***, double **, double **, double **, double **,
-                        double *, bool *, bool *, int *, int *, int *, int);
-void CPU_dbl2_qrbs_solve(int, int, int, double ***, double ***, double **, double **, double **,
-                         double **, double **, double **, double **, double **, double *,
-                         double *, bool *, bool *, int *, int *, int *, int);
-void CPU_dbl4_qrbs_solve(int, int, int, double ***, double ***, double ***, double ***, double **,
-                         double **, double **, double **, double **, double **, double **,
-                         double **, double **, double **, double **, double **, double **,
-                         double **, double **, double **, double *, double *, double *, double *,
-                         bool *, bool *, int *, int *, int *, int);
-
-void CPU_dbl_linear_residue(int, int, int, double ***, double **, double **, double **, double *,
-                            int);
-void CPU_dbl2_linear_residue(int, int, int, double ***, double ***, double **, double **,
-                             double **, double **, double **, double **, double *, double *, int);
-void CPU_dbl4_linear_residue(int, int, int, double ***, double ***, double ***, double ***,
-                             double **, double **, double **, double **, double **, double **,
-                             double **, double **, double **, double **, double **, double **,
-                             double *, double *, double *, double *, int);
-
-void CPU_dbl8_product(int, double *, double *, double *, double *, double *, double *, double *,
-                      double *, double *, double *, double *, double *, double *, double *,
-                      double *, double *, double *, double *, double *, double *, double *,
-                      double *, double *, double *);
-void CPU_cmplx8_product(int, double *, double *, double *, double *, double *, double *,
-                        double *, double *, double *, double *, double *, double *, double *,
-                        double *, double *, double *, double *, double *, double *, double *,
-                        double *, double *, double *, double *, double *, double *, double *,
-                        double *, double *, double *, double *, double *, double *, double *,
-                        double *, double *, double *, double *, double *, double *, double *,
-                        double *, double *, double *, double *, double *, double *, double *);
-
-void CPU_dbl2_upper_tiled_solver(int, int, int, double **, double **, double *, double *,
-                                 double *, double *, double *);
-void CPU_dbl4_upper_tiled_solver(int, int, int, double **, double **, double **, double **,
-                                 double *, double *, double *, double *, double *, double *,
-                                 double *, double *, double *);
+#include "baqr.h"
+#include "md.h"
+#include "newton.h"
+#include "series.h"
 
 static int failures = 0;
 
 static void check(bool ok, const char *what, double err, double tol) {
-  std::printf("%-44s max diff %.3e  tol %.3e  %s\n", what, err, tol, ok ? "ok" : "FAILED");
+  std::printf("%-52s max diff %.3e  tol %.3e  %s\n", what, err, tol, ok ? "ok" : "FAILED");
   if (!ok) ++failures;
 }
 
-// a K-limb number (K = 1, 2, 4) with random limbs below the leading one
+// uniform doubles in [-1, 1] from a 64-bit LCG (the top 53 bits)
+static uint64_t rng_state = 20261202ULL;
+static double uniform() {
+  rng_state = rng_state * 6364136223846793005ULL + 1442695040888963407ULL;
+  return (double)(rng_state >> 11) * 0x1p-52 - 1.0;
+}
+
+// a K-limb number: a random leading limb plus random lower-order parts,
+// renormalized by an md addition
 static void random_md(int K, double *x) {
-  double a[4] = {random_double(), 0.0, 0.0, 0.0};
-  if (K == 2) {
-    ddf_add(a[0], 0.0, random_double() * 0x1p-53, 0.0, &x[0], &x[1]);
-  } else if (K == 4) {
-    double t[4];
-    qdf_add(a[0], 0.0, 0.0, 0.0, random_double() * 0x1p-53, random_double() * 0x1p-106,
-            random_double() * 0x1p-159, 0.0, &t[0], &t[1], &t[2], &t[3]);
-    for (int i = 0; i < 4; ++i) x[i] = t[i];
-  } else {
-    x[0] = a[0];
-  }
+  double a[8] = {uniform(), 0, 0, 0, 0, 0, 0, 0}, b[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double s = 0x1p-53;
+  for (int l = 1; l < K; ++l, s *= 0x1p-53) b[l - 1] = uniform() * s;
+  if (K == 1) x[0] = a[0];
+  else omd_add(K, a, b, x);
 }
 
 // |leading limb of a - b| in K-limb arithmetic
 static double md_diff(int K, const double *a, const double *b) {
-  double c[4];
+  double c[8];
   if (K == 1) return std::fabs(a[0] - b[0]);
-  if (K == 2) {
-    ddf_sub(a[0], a[1], b[0], b[1], &c[0], &c[1]);
-    return std::fabs(c[0]);
-  }
-  qdf_sub(a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3], &c[0], &c[1], &c[2], &c[3]);
+  omd_sub(K, a, b, c);
   return std::fabs(c[0]);
 }
 
-// complex K-limb matrix, one row-pointer array per plane (K re, then K im)
-struct CMat {
-  int K, rows, cols;
-  std::vector<std::vector<double>> store;
+// P planes of rows x cols doubles, contiguous ([P][rows][cols]), with row
+// pointers per plane for the reference-style calls
+struct Planes {
+  int P, rows, cols;
+  std::vector<double> v;
   std::vector<std::vector<double *>> ptr;
-  CMat(int K_, int r, int c) : K(K_), rows(r), cols(c), store(2 * K_), ptr(2 * K_) {
-    for (int q = 0; q < 2 * K; ++q) store[q].assign((size_t)r * c, 0.0);
-    rows_of();
-  }
-  // a copy owns its numbers: its row pointers point into its own storage
-  CMat(const CMat &o) : K(o.K), rows(o.rows), cols(o.cols), store(o.store), ptr(2 * o.K) {
-    rows_of();
-  }
-  CMat &operator=(const CMat &) = delete;
-  void rows_of() {
-    for (int q = 0; q < 2 * K; ++q) {
-      ptr[q].resize(rows);
-      for (int i = 0; i < rows; ++i) ptr[q][i] = &store[q][(size_t)i * cols];
+  Planes(int P_, int r, int c) : P(P_), rows(r), cols(c), v((size_t)P_ * r * c, 0.0), ptr(P_) {
+    for (int q = 0; q < P; ++q) {
+      ptr[q].resize(r);
+      for (int i = 0; i < r; ++i) ptr[q][i] = &v[((size_t)q * r + i) * c];
     }
   }
-  double **p(int q) { return ptr[q].data(); }
-  void get(int i, int j, double *re, double *im) const {
-    for (int l = 0; l < K; ++l) {
-      re[l] = store[l][(size_t)i * cols + j];
-      im[l] = store[K + l][(size_t)i * cols + j];
-    }
+  Planes(const Planes &o) : Planes(o.P, o.rows, o.cols) { v = o.v; }
+  Planes &operator=(const Planes &) = delete;
+  double **p(int q) { return ptr[q].data(); }   // row pointers of plane q
+  double *row(int q, int i) { return ptr[q][i]; }
+  double *data() { return v.data(); }
+  // K-limb number (plane offset off) at (i, j)
+  void get(int K, int off, int i, int j, double *x) const {
+    for (int l = 0; l < K; ++l) x[l] = v[((size_t)(off + l) * rows + i) * cols + j];
   }
-  void set(int i, int j, const double *re, const double *im) {
-    for (int l = 0; l < K; ++l) {
-      store[l][(size_t)i * cols + j] = re[l];
-      store[K + l][(size_t)i * cols + j] = im[l];
-    }
+  void set(int K, int off, int i, int j, const double *x) {
+    for (int l = 0; l < K; ++l) v[((size_t)(off + l) * rows + i) * cols + j] = x[l];
   }
-  void randomize() {
-    double re[4], im[4];
-    for (int i = 0; i < rows; ++i)
-      for (int j = 0; j < cols; ++j) {
-        random_md(K, re);
-        random_md(K, im);
-        set(i, j, re, im);
-      }
+  // every entry random; K limbs per number, P / K numbers per entry
+  void randomize(int K) {
+    double x[8];
+    for (int g = 0; g < P / K; ++g)
+      for (int i = 0; i < rows; ++i)
+        for (int j = 0; j < cols; ++j) {
+          random_md(K, x);
+          set(K, g * K, i, j, x);
+        }
   }
-  double diff(const CMat &o) const {
-    double m = 0.0, a[4], b[4], c[4], d[4];
-    for (int i = 0; i < rows; ++i)
-      for (int j = 0; j < cols; ++j) {
-        get(i, j, a, b);
-        o.get(i, j, c, d);
-        m = std::fmax(m, std::fmax(md_diff(K, a, c), md_diff(K, b, d)));
-      }
+  // the largest |leading limb of a - b| over all entries (K-limb numbers)
+  double diff(int K, const Planes &o) const {
+    double m = 0.0, a[8], b[8];
+    for (int g = 0; g < P / K; ++g)
+      for (int i = 0; i < rows; ++i)
+        for (int j = 0; j < cols; ++j) {
+          get(K, g * K, i, j, a);
+          o.get(K, g * K, i, j, b);
+          m = std::fmax(m, md_diff(K, a, b));
+        }
     return m;
   }
+  bool same(const Planes &o) const { return v == o.v; }
 };
-
-// ------------------------------------------------------------ md helpers
-
-static void md_add(int K, const double *a, const double *b, double *c) {
-  if (K == 2) ddf_add(a[0], a[1], b[0], b[1], &c[0], &c[1]);
-  else qdf_add(a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3], &c[0], &c[1], &c[2], &c[3]);
-}
-
-static void md_mul(int K, const double *a, const double *b, double *c) {
-  if (K == 2) ddf_mul(a[0], a[1], b[0], b[1], &c[0], &c[1]);
-  else qdf_mul(a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3], &c[0], &c[1], &c[2], &c[3]);
-}
-
-static void md_neg(int K, double *a) {
-  for (int l = 0; l < K; ++l) a[l] = -a[l];
-}
-
-// b = A x (complex, K limbs), x and b as n x 1 matrices
-static void matvec(const CMat &A, const CMat &x, CMat &b) {
-  const int K = A.K;
-  for (int i = 0; i < A.rows; ++i) {
-    double sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
-    for (int j = 0; j < A.cols; ++j) {
-      double ar[4], ai[4], xr[4], xi[4], t[4];
-      A.get(i, j, ar, ai);
-      x.get(j, 0, xr, xi);
-      md_mul(K, ar, xr, t); md_add(K, sr, t, sr);
-      md_mul(K, ai, xi, t); md_neg(K, t); md_add(K, sr, t, sr);
-      md_mul(K, ar, xi, t); md_add(K, si, t, si);
-      md_mul(K, ai, xr, t); md_add(K, si, t, si);
-    }
-    b.set(i, 0, sr, si);
-  }
-}
 
 // ------------------------------------------------------------ products
 
-static void test_products(int deg) {
+// z = x*y through the GPU_* entry point and osr_product, K limbs, real and
+// complex: bit for bit
+static void test_products(int K, int deg) {
   const int D = deg + 1;
-  std::vector<double> v(64 * D);
-  double *p[64];
-  for (int i = 0; i < 64; ++i) p[i] = &v[(size_t)i * D];
-  for (auto &e : v) e = random_double();
-  // inputs x: p[0..7], y: p[8..15]; outputs CPU p[16..23], GPU p[24..31]
-  auto same = [&](int planes) {
-    double m = 0.0;
-    for (int q = 0; q < planes; ++q)
-      for (int k = 0; k < D; ++k) m = std::fmax(m, std::fabs(p[16 + q][k] - p[24 + q][k]));
-    return m;
-  };
-  CPU_dbl_product(deg, p[0], p[8], p[16]);
-  GPU_dbl_product(p[0], p[8], p[24], deg, 1, D, 0);
-  check(same(1) == 0.0, "GPU_dbl_product vs CPU_dbl_product", same(1), 0.0);
-  CPU_cmplx_product(deg, p[0], p[1], p[8], p[9], p[16], p[17]);
-  GPU_cmplx_product(p[0], p[1], p[8], p[9], p[24], p[25], deg, 1, D, 0);
-  check(same(2) == 0.0, "GPU_cmplx_product vs CPU_cmplx_product", same(2), 0.0);
-  CPU_dbl2_product(deg, p[0], p[1], p[8], p[9], p[16], p[17]);
-  GPU_dbl2_product(p[0], p[1], p[8], p[9], p[24], p[25], deg, 1, D, 1);
-  check(same(2) == 0.0, "GPU_dbl2_product vs CPU_dbl2_product", same(2), 0.0);
-  CPU_cmplx2_product(deg, p[0], p[1], p[2], p[3], p[8], p[9], p[10], p[11], p[16], p[17], p[18],
-                     p[19]);
-  GPU_cmplx2_product(p[0], p[1], p[2], p[3], p[8], p[9], p[10], p[11], p[24], p[25], p[26], p[27],
-                     deg, 1, D, 0);
-  check(same(4) == 0.0, "GPU_cmplx2_product vs CPU_cmplx2_product", same(4), 0.0);
-  CPU_dbl4_product(deg, p[0], p[1], p[2], p[3], p[8], p[9], p[10], p[11], p[16], p[17], p[18],
-                   p[19]);
-  GPU_dbl4_product(p[0], p[1], p[2], p[3], p[8], p[9], p[10], p[11], p[24], p[25], p[26], p[27],
-                   deg, 1, D, 1);
-  check(same(4) == 0.0, "GPU_dbl4_product vs CPU_dbl4_product", same(4), 0.0);
-  CPU_cmplx4_product(deg, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9], p[10],
-                     p[11], p[12], p[13], p[14], p[15], p[16], p[17], p[18], p[19], p[20], p[21],
-                     p[22], p[23]);
-  GPU_cmplx4_product(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9], p[10], p[11],
-                     p[12], p[13], p[14], p[15], p[24], p[25], p[26], p[27], p[28], p[29], p[30],
-                     p[31], deg, 1, D, 0);
-  check(same(8) == 0.0, "GPU_cmplx4_product vs CPU_cmplx4_product", same(8), 0.0);
+  for (int cplx = 0; cplx <= 1; ++cplx) {
+    const int P = cplx ? 2 * K : K;
+    Planes x(P, 1, D), y(P, 1, D), zc(P, 1, D), zg(P, 1, D);
+    x.randomize(K);
+    y.randomize(K);
+    osr_product(K, cplx, deg, x.data(), y.data(), zc.data());
+    double *a[16], *b[16], *c[16];
+    for (int q = 0; q < P; ++q) { a[q] = x.row(q, 0); b[q] = y.row(q, 0); c[q] = zg.row(q, 0); }
+    const char *name = "";
+    if (K == 1 && !cplx) { GPU_dbl_product(a[0], b[0], c[0], deg, 1, D, 0); name = "GPU_dbl_product"; }
+    if (K == 1 && cplx) {
+      GPU_cmplx_product(a[0], a[1], b[0], b[1], c[0], c[1], deg, 1, D, 0);
+      name = "GPU_cmplx_product";
+    }
+    if (K == 2 && !cplx) {
+      GPU_dbl2_product(a[0], a[1], b[0], b[1], c[0], c[1], deg, 1, D, 1);
+      name = "GPU_dbl2_product";
+    }
+    if (K == 2 && cplx) {
+      GPU_cmplx2_product(a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3], c[0], c[1], c[2], c[3],
+                         deg, 1, D, 0);
+      name = "GPU_cmplx2_product";
+    }
+    if (K == 4 && !cplx) {
+      GPU_dbl4_product(a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3], c[0], c[1], c[2], c[3],
+                       deg, 1, D, 1);
+      name = "GPU_dbl4_product";
+    }
+    if (K == 4 && cplx) {
+      GPU_cmplx4_product(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], b[0], b[1], b[2], b[3],
+                         b[4], b[5], b[6], b[7], c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7],
+                         deg, 1, D, 0);
+      name = "GPU_cmplx4_product";
+    }
+    if (K == 8 && !cplx) {
+      GPU_dbl8_product(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], b[0], b[1], b[2], b[3],
+                       b[4], b[5], b[6], b[7], c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7],
+                       deg, 1, D, 1);
+      name = "GPU_dbl8_product";
+    }
+    if (K == 8 && cplx) {
+      GPU_cmplx8_product(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10],
+                         a[11], a[12], a[13], a[14], a[15], b[0], b[1], b[2], b[3], b[4], b[5],
+                         b[6], b[7], b[8], b[9], b[10], b[11], b[12], b[13], b[14], b[15], c[0],
+                         c[1], c[2], c[3], c[4], c[5], c[6], c[7], c[8], c[9], c[10], c[11],
+                         c[12], c[13], c[14], c[15], deg, 1, D, 0);
+      name = "GPU_cmplx8_product";
+    }
+    char what[96];
+    std::snprintf(what, sizeof what, "%s deg %d (bitwise)", name, deg);
+    check(zg.same(zc), what, zg.diff(K, zc), 0.0);
+  }
 }
 
-// octo-double products: x in p[0..15], y in p[16..31], CPU z in p[32..47],
-// GPU z in p[48..63] (real: the first 8 planes of each)
-static void test_products8(int deg) {
-  const int D = deg + 1;
-  std::vector<double> v(64 * D);
-  double *p[64];
-  for (int i = 0; i < 64; ++i) p[i] = &v[(size_t)i * D];
-  for (auto &e : v) e = random_double();
-  auto diff = [&](int planes) {
-    double m = 0.0;
-    for (int q = 0; q < planes; ++q)
-      for (int k = 0; k < D; ++k) m = std::fmax(m, std::fabs(p[32 + q][k] - p[48 + q][k]));
-    return m;
-  };
-  CPU_dbl8_product(deg, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[16], p[17], p[18],
-                   p[19], p[20], p[21], p[22], p[23], p[32], p[33], p[34], p[35], p[36], p[37],
-                   p[38], p[39]);
-  GPU_dbl8_product(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[16], p[17], p[18], p[19],
-                   p[20], p[21], p[22], p[23], p[48], p[49], p[50], p[51], p[52], p[53], p[54],
-                   p[55], deg, 1, D, 1);
-  check(diff(8) == 0.0, "GPU_dbl8_product vs CPU_dbl8_product", diff(8), 0.0);
-  CPU_cmplx8_product(deg, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9], p[10],
-                     p[11], p[12], p[13], p[14], p[15], p[16], p[17], p[18], p[19], p[20], p[21],
-                     p[22], p[23], p[24], p[25], p[26], p[27], p[28], p[29], p[30], p[31], p[32],
-                     p[33], p[34], p[35], p[36], p[37], p[38], p[39], p[40], p[41], p[42], p[43],
-                     p[44], p[45], p[46], p[47]);
-  GPU_cmplx8_product(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9], p[10], p[11],
-                     p[12], p[13], p[14], p[15], p[16], p[17], p[18], p[19], p[20], p[21], p[22],
-                     p[23], p[24], p[25], p[26], p[27], p[28], p[29], p[30], p[31], p[48], p[49],
-                     p[50], p[51], p[52], p[53], p[54], p[55], p[56], p[57], p[58], p[59], p[60],
-                     p[61], p[62], p[63], deg, 1, D, 0);
-  check(diff(16) == 0.0, "GPU_cmplx8_product vs CPU_cmplx8_product", diff(16), 0.0);
+// --------------------------------------------------- polynomial evaluation
+
+// a polynomial in dim variables with nbr monomials of sorted distinct
+// variables (sizes 1..maxnvr) through GPU_*_poly_evaldiff, with the job
+// schedules built by the caller as the reference's testers do, against
+// osr_poly_evaldiff (CPU_*_poly_evaldiff restated): bit for bit
+static void test_poly_evaldiff(int K, int cplx, int dim, int nbr, int maxnvr, int deg) {
+  const int D = deg + 1, P = cplx ? 2 * K : K;
+  std::vector<int> nvr(nbr), idxptr(nbr + 1, 0), flat;
+  std::vector<std::vector<int>> sup(nbr);
+  for (int k = 0; k < nbr; ++k) {
+    nvr[k] = 1 + (int)((uniform() + 1.0) * 0.5 * maxnvr) % maxnvr;
+    std::vector<int> pick(dim);
+    for (int i = 0; i < dim; ++i) pick[i] = i;
+    for (int i = 0; i < nvr[k]; ++i) {
+      const int j = i + (int)((uniform() + 1.0) * 0.5 * (dim - i)) % (dim - i);
+      std::swap(pick[i], pick[j]);
+    }
+    sup[k].assign(pick.begin(), pick.begin() + nvr[k]);
+    std::sort(sup[k].begin(), sup[k].end());
+    for (int v : sup[k]) flat.push_back(v);
+    idxptr[k + 1] = (int)flat.size();
+  }
+  std::vector<int *> idx(nbr);
+  for (int k = 0; k < nbr; ++k) idx[k] = sup[k].data();
+  Planes cst(P, 1, D), cff(P, nbr, D), input(P, dim, D), outc(P, dim + 1, D), outg(P, dim + 1, D);
+  cst.randomize(K);
+  cff.randomize(K);
+  input.randomize(K);
+  osr_poly_evaldiff(K, cplx, dim, nbr, deg, nvr.data(), idxptr.data(), flat.data(), cst.data(),
+                    cff.data(), input.data(), outc.data());
+  ConvolutionJobs cnvjobs(dim);
+  cnvjobs.make(nbr, nvr.data(), idx.data(), false);
+  AdditionJobs addjobs(dim, nbr);
+  addjobs.make(nbr, nvr.data(), idx.data(), false);
+  double *c[16], **f[16], **x[16], **y[16];
+  for (int q = 0; q < P; ++q) {
+    c[q] = cst.row(q, 0);
+    f[q] = cff.p(q);
+    x[q] = input.p(q);
+    y[q] = outg.p(q);
+  }
+  double cnv = -1.0, add = -1.0, ela = -1.0, wall = -1.0;
+  const char *name = "";
+  if (K == 1 && !cplx) {
+    GPU_dbl_poly_evaldiff(D, dim, nbr, deg, nvr.data(), idx.data(), c[0], f[0], x[0], y[0],
+                          cnvjobs, addjobs, &cnv, &add, &ela, &wall, 0);
+    name = "GPU_dbl_poly_evaldiff";
+  } else if (K == 2 && !cplx) {
+    GPU_dbl2_poly_evaldiff(D, dim, nbr, deg, nvr.data(), idx.data(), c[0], c[1], f[0], f[1],
+                           x[0], x[1], y[0], y[1], cnvjobs, addjobs, &cnv, &add, &ela, &wall, 0);
+    name = "GPU_dbl2_poly_evaldiff";
+  } else if (K == 4 && !cplx) {
+    GPU_dbl4_poly_evaldiff(D, dim, nbr, deg, nvr.data(), idx.data(), c[0], c[1], c[2], c[3],
+                           f[0], f[1], f[2], f[3], x[0], x[1], x[2], x[3], y[0], y[1], y[2], y[3],
+                           cnvjobs, addjobs, &cnv, &add, &ela, &wall, 0);
+    name = "GPU_dbl4_poly_evaldiff";
+  } else if (K == 8 && !cplx) {
+    GPU_dbl8_poly_evaldiff(D, dim, nbr, deg, nvr.data(), idx.data(), c[0], c[1], c[2], c[3],
+                           c[4], c[5], c[6], c[7], f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7],
+                           x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], y[0], y[1], y[2], y[3],
+                           y[4], y[5], y[6], y[7], cnvjobs, addjobs, &cnv, &add, &ela, &wall, 0);
+    name = "GPU_dbl8_poly_evaldiff";
+  } else if (K == 1 && cplx) {
+    GPU_cmplx_poly_evaldiff(D, dim, nbr, deg, nvr.data(), idx.data(), c[0], c[1], f[0], f[1],
+                            x[0], x[1], y[0], y[1], cnvjobs, addjobs, &cnv, &add, &ela, &wall,
+                            0);
+    name = "GPU_cmplx_poly_evaldiff";
+  } else if (K == 2 && cplx) {
+    GPU_cmplx2_poly_evaldiff(D, dim, nbr, deg, nvr.data(), idx.data(), c[0], c[1], c[2], c[3],
+                             f[0], f[1], f[2], f[3], x[0], x[1], x[2], x[3], y[0], y[1], y[2],
+                             y[3], cnvjobs, addjobs, &cnv, &add, &ela, &wall, 0);
+    name = "GPU_cmplx2_poly_evaldiff";
+  } else {
+    GPU_cmplx4_poly_evaldiff(D, dim, nbr, deg, nvr.data(), idx.data(), c[0], c[1], c[2], c[3],
+                             c[4], c[5], c[6], c[7], f[0], f[1], f[2], f[3], f[4], f[5], f[6],
+                             f[7], x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], y[0], y[1],
+                             y[2], y[3], y[4], y[5], y[6], y[7], cnvjobs, addjobs, &cnv, &add,
+                             &ela, &wall, 0);
+    name = "GPU_cmplx4_poly_evaldiff";
+  }
+  char what[96];
+  std::snprintf(what, sizeof what, "%s dim %d nbr %d deg %d (bitwise)", name, dim, nbr, deg);
+  check(outg.same(outc), what, outg.diff(K, outc), 0.0);
+  if (!(cnv >= 0.0 && add == 0.0 && ela == cnv && wall > 0.0)) {
+    std::printf("%s: timers not set\n", name);
+    ++failures;
+  }
 }
 
 // ------------------------------------------------------------ QR, solver
 
-static void qr_cpu(CMat &A, CMat &Q, CMat &R, int szt) {
-  const int K = A.K, m = A.rows, n = A.cols;
-  double sec = 0.0;
-  if (K == 2)
-    CPU_cmplx2_blocked_houseqr(m, n, szt, n / szt, A.p(0), A.p(1), A.p(2), A.p(3), Q.p(0), Q.p(1),
-                               Q.p(2), Q.p(3), R.p(0), R.p(1), R.p(2), R.p(3), &sec, false);
-  else
-    CPU_cmplx4_blocked_houseqr(m, n, szt, n / szt, A.p(0), A.p(1), A.p(2), A.p(3), A.p(4), A.p(5),
-                               A.p(6), A.p(7), Q.p(0), Q.p(1), Q.p(2), Q.p(3), Q.p(4), Q.p(5),
-                               Q.p(6), Q.p(7), R.p(0), R.p(1), R.p(2), R.p(3), R.p(4), R.p(5),
-                               R.p(6), R.p(7), &sec, false);
-}
-
-static void qr_gpu(CMat &A, CMat &Q, CMat &R, int szt) {
-  const int K = A.K, m = A.rows, n = A.cols;
+static void qr_gpu(int K, Planes &A, Planes &Q, Planes &R, int szt) {
+  const int m = A.rows, n = A.cols;
   double t[10] = {0}, wall = 0.0;
   long long add = 0, mul = 0, div = 0, sq = 0;
   if (K == 2)
@@ -329,39 +282,25 @@ static void qr_gpu(CMat &A, CMat &Q, CMat &R, int szt) {
 }
 
 static void test_qr(int K, int m, int n, int szt) {
-  CMat A(K, m, n), Qc(K, m, m), Rc(K, m, n), Qg(K, m, m), Rg(K, m, n);
-  A.randomize();
-  qr_cpu(A, Qc, Rc, szt);
-  qr_gpu(A, Qg, Rg, szt);
+  Planes A(2 * K, m, n), Qc(2 * K, m, m), Rc(2 * K, m, n), Qg(2 * K, m, m), Rg(2 * K, m, n);
+  A.randomize(K);
+  obq_blocked_houseqr(K, m, n, szt, n / szt, A.data(), Qc.data(), Rc.data());
+  qr_gpu(K, A, Qg, Rg, szt);
   const double tol = 64.0 * m * std::pow(0x1p-52, K);
   char what[96];
   std::snprintf(what, sizeof what, "GPU_cmplx%d_blocked_houseqr %dx%d szt %d: Q", K, m, n, szt);
-  check(Qg.diff(Qc) < tol, what, Qg.diff(Qc), tol);
+  check(Qg.diff(K, Qc) < tol, what, Qg.diff(K, Qc), tol);
   std::snprintf(what, sizeof what, "GPU_cmplx%d_blocked_houseqr %dx%d szt %d: R", K, m, n, szt);
-  check(Rg.diff(Rc) < tol * std::sqrt((double)m), what, Rg.diff(Rc), tol * std::sqrt((double)m));
+  check(Rg.diff(K, Rc) < tol * std::sqrt((double)m), what, Rg.diff(K, Rc),
+        tol * std::sqrt((double)m));
 }
 
-static void solve_cpu(CMat U, CMat b, CMat &x, int szt) {  // U, b by value: the CPU version updates them
-  const int K = U.K, n = U.rows;
-  double sec = 0.0;
-  std::vector<double *> bp(2 * K), xp(2 * K);
-  for (int q = 0; q < 2 * K; ++q) { bp[q] = b.store[q].data(); xp[q] = x.store[q].data(); }
-  if (K == 2)
-    CPU_cmplx2_upper_tiled_solver(n, szt, n / szt, U.p(0), U.p(1), U.p(2), U.p(3), bp[0], bp[1],
-                                  bp[2], bp[3], xp[0], xp[1], xp[2], xp[3], &sec);
-  else
-    CPU_cmplx4_upper_tiled_solver(n, szt, n / szt, U.p(0), U.p(1), U.p(2), U.p(3), U.p(4), U.p(5),
-                                  U.p(6), U.p(7), bp[0], bp[1], bp[2], bp[3], bp[4], bp[5], bp[6],
-                                  bp[7], xp[0], xp[1], xp[2], xp[3], xp[4], xp[5], xp[6], xp[7],
-                                  &sec);
-}
-
-static void solve_gpu(CMat &U, CMat &b, CMat &x, int szt) {
-  const int K = U.K, n = U.rows;
+static void solve_gpu(int K, Planes &U, Planes &b, Planes &x, int szt) {
+  const int n = U.rows;
   double t[4] = {0}, wall = 0.0, over[3] = {0};
   long long add = 0, mul = 0, div = 0;
-  std::vector<double *> bp(2 * K), xp(2 * K);
-  for (int q = 0; q < 2 * K; ++q) { bp[q] = b.store[q].data(); xp[q] = x.store[q].data(); }
+  double *bp[8], *xp[8];
+  for (int q = 0; q < 2 * K; ++q) { bp[q] = b.row(q, 0); xp[q] = x.row(q, 0); }
   if (K == 2)
     GPU_cmplx2_upper_tiled_solver(n, szt, n / szt, U.p(0), U.p(1), U.p(2), U.p(3), bp[0], bp[1],
                                   bp[2], bp[3], xp[0], xp[1], xp[2], xp[3], &t[0], &t[1], &t[2],
@@ -374,11 +313,11 @@ static void solve_gpu(CMat &U, CMat &b, CMat &x, int szt) {
                                   &over[1], &div, &over[2]);
 }
 
-static void qhb_gpu(CMat &Q, CMat &b) {
-  const int K = Q.K, n = Q.rows;
+static void qhb_gpu(int K, Planes &Q, Planes &b) {
+  const int n = Q.rows;
   double ms = 0.0;
-  std::vector<double *> bp(2 * K);
-  for (int q = 0; q < 2 * K; ++q) bp[q] = b.store[q].data();
+  double *bp[8];
+  for (int q = 0; q < 2 * K; ++q) bp[q] = b.row(q, 0);
   if (K == 2)
     GPU_cmplx2_bals_qhb(n, 1, n, Q.p(0), Q.p(1), Q.p(2), Q.p(3), bp[0], bp[1], bp[2], bp[3], &ms,
                         0);
@@ -387,159 +326,163 @@ static void qhb_gpu(CMat &Q, CMat &b) {
                         bp[0], bp[1], bp[2], bp[3], bp[4], bp[5], bp[6], bp[7], &ms, 0);
 }
 
+// an upper triangular system, diagonally dominant enough; P = 2K planes
+// (complex) or K (real)
+static void upper_system(int K, Planes &U, Planes &b) {
+  U.randomize(K);
+  b.randomize(K);
+  const double zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int g = 0; g < U.P / K; ++g)
+    for (int i = 0; i < U.rows; ++i)
+      for (int j = 0; j < i; ++j) U.set(K, g * K, i, j, zero);
+  for (int i = 0; i < U.rows; ++i) U.v[(size_t)i * U.cols + i] += 4.0;
+}
+
 static void test_tiled(int K, int n, int szt) {
-  CMat U(K, n, n), b(K, n, 1), xc(K, n, 1), xg(K, n, 1);
-  U.randomize();
-  b.randomize();
-  double re[4], im[4], zero[4] = {0, 0, 0, 0};
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < i; ++j) U.set(i, j, zero, zero);
-  for (int i = 0; i < n; ++i) {  // diagonally dominant enough
-    U.get(i, i, re, im);
-    re[0] += 4.0;
-    U.set(i, i, re, im);
-  }
-  const CMat U0 = U, b0 = b;
-  solve_cpu(U, b, xc, szt);
-  solve_gpu(U, b, xg, szt);
+  Planes U(2 * K, n, n), b(2 * K, 1, n), xg(2 * K, 1, n);
+  upper_system(K, U, b);
+  Planes Uc(U), bc(b), xc(2 * K, 1, n);
+  obq_upper_tiled_solver(K, n, szt, n / szt, Uc.data(), bc.data(), xc.data());
+  const Planes U0(U), b0(b);
+  solve_gpu(K, U, b, xg, szt);
   const double tol = 64.0 * n * std::pow(0x1p-52, K);
   char what[96];
   std::snprintf(what, sizeof what, "GPU_cmplx%d_upper_tiled_solver %d szt %d: x", K, n, szt);
-  check(xg.diff(xc) < tol, what, xg.diff(xc), tol);
+  check(xg.diff(K, xc) < tol, what, xg.diff(K, xc), tol);
   std::snprintf(what, sizeof what, "GPU_cmplx%d_upper_tiled_solver: U, b kept", K);
-  check(U.diff(U0) == 0.0 && b.diff(b0) == 0.0, what, std::fmax(U.diff(U0), b.diff(b0)), 0.0);
+  check(U.same(U0) && b.same(b0), what, std::fmax(U.diff(K, U0), b.diff(K, b0)), 0.0);
 }
 
 // b = A x0; Q R = A (GPU); y = Q^H b (GPU); R x = y (GPU): x = x0
 static void test_least_squares(int K, int n, int szt) {
-  CMat A(K, n, n), x0(K, n, 1), b(K, n, 1), Q(K, n, n), R(K, n, n), x(K, n, 1);
-  A.randomize();
-  x0.randomize();
-  matvec(A, x0, b);
-  qr_gpu(A, Q, R, szt);
-  qhb_gpu(Q, b);
-  solve_gpu(R, b, x, szt);
+  Planes A(2 * K, n, n), x0(2 * K, 1, n), b(2 * K, 1, n), Q(2 * K, n, n), R(2 * K, n, n),
+      x(2 * K, 1, n);
+  A.randomize(K);
+  x0.randomize(K);
+  obq_matvec(K, n, n, A.data(), x0.data(), b.data());
+  qr_gpu(K, A, Q, R, szt);
+  qhb_gpu(K, Q, b);
+  solve_gpu(K, R, b, x, szt);
   const double tol = 1e6 * std::pow(0x1p-52, K);
   char what[96];
   std::snprintf(what, sizeof what, "least squares K=%d n=%d: x - x0", K, n);
-  check(x.diff(x0) < tol, what, x.diff(x0), tol);
+  check(x.diff(K, x0) < tol, what, x.diff(K, x0), tol);
+}
+
+// a real upper triangular system through GPU_dblK_upper_tiled_solver
+// against the complex oracle solver with zero imaginary parts
+static void test_tiled_real(int K, int n, int szt) {
+  Planes U(K, n, n), b(K, 1, n), xg(K, 1, n);
+  upper_system(K, U, b);
+  Planes Uc(2 * K, n, n), bc(2 * K, 1, n), xc(2 * K, 1, n);
+  std::copy(U.v.begin(), U.v.end(), Uc.v.begin());
+  std::copy(b.v.begin(), b.v.end(), bc.v.begin());
+  obq_upper_tiled_solver(K, n, szt, n / szt, Uc.data(), bc.data(), xc.data());
+  const Planes U0(U), b0(b);
+  double t[4] = {0, 0, 0, 0}, over[3] = {0, 0, 0};
+  long long cnt[3] = {0, 0, 0};
+  if (K == 2) {
+    GPU_dbl2_upper_tiled_solver(n, szt, n / szt, U.p(0), U.p(1), b.row(0, 0), b.row(1, 0),
+                                xg.row(0, 0), xg.row(1, 0), &t[0], &t[1], &t[2], &t[3], &t[0],
+                                &cnt[0], &cnt[1], &cnt[2]);
+  } else {
+    GPU_dbl4_upper_tiled_solver(n, szt, n / szt, U.p(0), U.p(1), U.p(2), U.p(3), b.row(0, 0),
+                                b.row(1, 0), b.row(2, 0), b.row(3, 0), xg.row(0, 0),
+                                xg.row(1, 0), xg.row(2, 0), xg.row(3, 0), &t[0], &t[1], &t[2],
+                                &t[3], &t[0], &cnt[0], &over[0], &cnt[1], &over[1], &cnt[2],
+                                &over[2]);
+  }
+  Planes xr(K, 1, n);
+  std::copy(xc.v.begin(), xc.v.begin() + (size_t)K * n, xr.v.begin());
+  const double tol = 64.0 * n * std::pow(0x1p-52, K);
+  char what[96];
+  std::snprintf(what, sizeof what, "GPU_dbl%d_upper_tiled_solver %d szt %d: x", K, n, szt);
+  check(xg.diff(K, xr) < tol, what, xg.diff(K, xr), tol);
+  std::snprintf(what, sizeof what, "GPU_dbl%d_upper_tiled_solver: U, b kept", K);
+  check(U.same(U0) && b.same(b0), what, U.same(U0) && b.same(b0) ? 0.0 : 1.0, 0.0);
 }
 
 // ------------------------------------------------------------ bals solve
 
-// K limb planes of a real rows x cols matrix with row pointers
-struct RMat {
-  int K, rows, cols;
-  std::vector<std::vector<double>> store;
-  std::vector<std::vector<double *>> ptr;
-  RMat(int K_, int r, int c) : K(K_), rows(r), cols(c), store(K_), ptr(K_) {
-    for (int q = 0; q < K; ++q) {
-      store[q].assign((size_t)r * c, 0.0);
-      ptr[q].resize(r);
-      for (int i = 0; i < r; ++i) ptr[q][i] = &store[q][(size_t)i * c];
-    }
+// the matrix series [K][degp1][dim][dim] with row pointers mp[q][j][i]
+struct MatSeries {
+  int K, degp1, dim;
+  Planes all;
+  std::vector<std::vector<double **>> mp;
+  std::vector<std::vector<std::vector<double *>>> rows;
+  MatSeries(int K_, int d, int n)
+      : K(K_), degp1(d), dim(n), all(K_, d * n, n), mp(K_, std::vector<double **>(d)),
+        rows(K_, std::vector<std::vector<double *>>(d, std::vector<double *>(n))) {
+    for (int q = 0; q < K; ++q)
+      for (int j = 0; j < d; ++j) {
+        for (int i = 0; i < n; ++i) rows[q][j][i] = all.row(q, j * n + i);
+        mp[q][j] = rows[q][j].data();
+      }
   }
-  RMat(const RMat &) = delete;
-  double **p(int q) { return ptr[q].data(); }
-  void randomize() {
-    double v[4];
-    for (int i = 0; i < rows * cols; ++i) {
-      random_md(K, v);
-      for (int q = 0; q < K; ++q) store[q][i] = v[q];
-    }
-  }
-  bool same(const RMat &o) const { return store == o.store; }
 };
 
-// the same block Toeplitz system through CPU_dblK_qrbs_solve (reference) and
-// GPU_dblK_bals_solve (this library): every output bit for bit
+// the same block Toeplitz system through onw_qrbs_solve (CPU_dblK_qrbs_solve
+// restated) and GPU_dblK_bals_solve: every output bit for bit
 static void test_bals_solve(int K, int dim, int degp1, int tailidx) {
-  std::vector<RMat *> mat;
-  for (int j = 0; j < degp1; ++j) {
-    mat.push_back(new RMat(K, dim, dim));
-    mat.back()->randomize();
-  }
-  std::vector<std::vector<double **>> mp(K, std::vector<double **>(degp1));
-  for (int q = 0; q < K; ++q)
-    for (int j = 0; j < degp1; ++j) mp[q][j] = mat[j]->p(q);
-  RMat rc(K, degp1, dim), rg(K, degp1, dim), xc(K, degp1, dim), xg(K, degp1, dim);
-  RMat Qc(K, dim, dim), Qg(K, dim, dim), Rc(K, dim, dim), Rg(K, dim, dim);
-  rc.randomize();
-  rg.store = rc.store;
-  std::vector<std::vector<double>> wrk(K, std::vector<double>(dim));
-  bool zc = true, nc = false, zg = true, ng = false;
-  int uc = 0, bc = 0, tc = 0, ug = 0, bg = 0, tg = 0;
+  MatSeries mat(K, degp1, dim);
+  mat.all.randomize(K);
+  Planes rc(K, degp1, dim), xc(K, degp1, dim), Qc(K, dim, dim), Rc(K, dim, dim);
+  rc.randomize(K);
+  Planes rg(rc), xg(K, degp1, dim), Qg(K, dim, dim), Rg(K, dim, dim);
+  int zc = 1, nc = 0, uc = 0, bc = 0, tc = 0;
+  onw_qrbs_solve(K, dim, degp1, tailidx, mat.all.data(), rc.data(), xc.data(), Qc.data(),
+                 Rc.data(), &zc, &nc, &uc, &bc, &tc);
+  bool zg = true, ng = false;
+  int ug = 0, bg = 0, tg = 0;
   double t[4] = {0, 0, 0, 0};
   if (K == 1) {
-    CPU_dbl_qrbs_solve(dim, degp1, tailidx, mp[0].data(), rc.p(0), xc.p(0), Qc.p(0), Rc.p(0),
-                       wrk[0].data(), &zc, &nc, &uc, &bc, &tc, 0);
-    GPU_dbl_bals_solve(dim, degp1, 1, dim, tailidx, mp[0].data(), Qg.p(0), Rg.p(0), rg.p(0),
+    GPU_dbl_bals_solve(dim, degp1, 1, dim, tailidx, mat.mp[0].data(), Qg.p(0), Rg.p(0), rg.p(0),
                        xg.p(0), &zg, &ng, &ug, &bg, &tg, &t[0], &t[1], &t[2], &t[3], 0);
   } else if (K == 2) {
-    CPU_dbl2_qrbs_solve(dim, degp1, tailidx, mp[0].data(), mp[1].data(), rc.p(0), rc.p(1),
-                        xc.p(0), xc.p(1), Qc.p(0), Qc.p(1), Rc.p(0), Rc.p(1), wrk[0].data(),
-                        wrk[1].data(), &zc, &nc, &uc, &bc, &tc, 0);
-    GPU_dbl2_bals_solve(dim, degp1, 1, dim, tailidx, mp[0].data(), mp[1].data(), Qg.p(0),
+    GPU_dbl2_bals_solve(dim, degp1, 1, dim, tailidx, mat.mp[0].data(), mat.mp[1].data(), Qg.p(0),
                         Qg.p(1), Rg.p(0), Rg.p(1), rg.p(0), rg.p(1), xg.p(0), xg.p(1), &zg, &ng,
                         &ug, &bg, &tg, &t[0], &t[1], &t[2], &t[3], 0);
   } else {
-    CPU_dbl4_qrbs_solve(dim, degp1, tailidx, mp[0].data(), mp[1].data(), mp[2].data(),
-                        mp[3].data(), rc.p(0), rc.p(1), rc.p(2), rc.p(3), xc.p(0), xc.p(1),
-                        xc.p(2), xc.p(3), Qc.p(0), Qc.p(1), Qc.p(2), Qc.p(3), Rc.p(0), Rc.p(1),
-                        Rc.p(2), Rc.p(3), wrk[0].data(), wrk[1].data(), wrk[2].data(),
-                        wrk[3].data(), &zc, &nc, &uc, &bc, &tc, 0);
-    GPU_dbl4_bals_solve(dim, degp1, 1, dim, tailidx, mp[0].data(), mp[1].data(), mp[2].data(),
-                        mp[3].data(), Qg.p(0), Qg.p(1), Qg.p(2), Qg.p(3), Rg.p(0), Rg.p(1),
-                        Rg.p(2), Rg.p(3), rg.p(0), rg.p(1), rg.p(2), rg.p(3), xg.p(0), xg.p(1),
-                        xg.p(2), xg.p(3), &zg, &ng, &ug, &bg, &tg, &t[0], &t[1], &t[2], &t[3], 0);
+    GPU_dbl4_bals_solve(dim, degp1, 1, dim, tailidx, mat.mp[0].data(), mat.mp[1].data(),
+                        mat.mp[2].data(), mat.mp[3].data(), Qg.p(0), Qg.p(1), Qg.p(2), Qg.p(3),
+                        Rg.p(0), Rg.p(1), Rg.p(2), Rg.p(3), rg.p(0), rg.p(1), rg.p(2), rg.p(3),
+                        xg.p(0), xg.p(1), xg.p(2), xg.p(3), &zg, &ng, &ug, &bg, &tg, &t[0], &t[1],
+                        &t[2], &t[3], 0);
   }
-  const bool same = xc.same(xg) && rc.same(rg) && Qc.same(Qg) && Rc.same(Rg) && zc == zg &&
-                    nc == ng && uc == ug && bc == bg && tc == tg;
+  const bool same = xc.same(xg) && rc.same(rg) && Qc.same(Qg) && Rc.same(Rg) &&
+                    (zc != 0) == zg && (nc != 0) == ng && uc == ug && bc == bg && tc == tg;
   char what[96];
   std::snprintf(what, sizeof what, "GPU_dbl%s_bals_solve dim %d degp1 %d (bitwise)",
                 K == 1 ? "" : (K == 2 ? "2" : "4"), dim, degp1);
   check(same, what, same ? 0.0 : 1.0, 0.0);
-  for (RMat *m : mat) delete m;
 }
 
-// the residual of a block Toeplitz system through CPU_dblK_linear_residue
-// (reference) and GPU_dblK_linear_residue (this library): bit for bit, and
-// the rows below tailidx left as they were
+// the residual of a block Toeplitz system through onw_linear_residue
+// (CPU_dblK_linear_residue restated) and GPU_dblK_linear_residue: bit for
+// bit, and the rows below tailidx left as they were
 static void test_linear_residue(int K, int dim, int degp1, int tailidx) {
-  std::vector<RMat *> mat;
-  for (int j = 0; j < degp1; ++j) {
-    mat.push_back(new RMat(K, dim, dim));
-    mat.back()->randomize();
-  }
-  std::vector<std::vector<double **>> mp(K, std::vector<double **>(degp1));
-  for (int q = 0; q < K; ++q)
-    for (int j = 0; j < degp1; ++j) mp[q][j] = mat[j]->p(q);
-  RMat b(K, degp1, dim), x(K, degp1, dim), rc(K, degp1, dim), rg(K, degp1, dim);
-  b.randomize();
-  x.randomize();
-  rc.randomize();
-  rg.store = rc.store;
+  MatSeries mat(K, degp1, dim);
+  mat.all.randomize(K);
+  Planes b(K, degp1, dim), x(K, degp1, dim), rc(K, degp1, dim);
+  b.randomize(K);
+  x.randomize(K);
+  rc.randomize(K);
+  Planes rg(rc);
   double mc[4] = {0, 0, 0, 0}, mg[4] = {0, 0, 0, 0}, ms = 0.0;
+  onw_linear_residue(K, dim, degp1, tailidx, mat.all.data(), b.data(), x.data(), rc.data(), mc);
   long long add = -1, mul = -1;
   if (K == 1) {
-    CPU_dbl_linear_residue(dim, degp1, tailidx, mp[0].data(), b.p(0), x.p(0), rc.p(0), &mc[0], 0);
-    GPU_dbl_linear_residue(dim, degp1, 1, dim, tailidx, mp[0].data(), b.p(0), x.p(0), rg.p(0),
-                           &mg[0], &ms, &add, &mul, 0);
+    GPU_dbl_linear_residue(dim, degp1, 1, dim, tailidx, mat.mp[0].data(), b.p(0), x.p(0),
+                           rg.p(0), &mg[0], &ms, &add, &mul, 0);
   } else if (K == 2) {
-    CPU_dbl2_linear_residue(dim, degp1, tailidx, mp[0].data(), mp[1].data(), b.p(0), b.p(1),
-                            x.p(0), x.p(1), rc.p(0), rc.p(1), &mc[0], &mc[1], 0);
-    GPU_dbl2_linear_residue(dim, degp1, 1, dim, tailidx, mp[0].data(), mp[1].data(), b.p(0),
-                            b.p(1), x.p(0), x.p(1), rg.p(0), rg.p(1), &mg[0], &mg[1], &ms, &add,
-                            &mul, 0);
+    GPU_dbl2_linear_residue(dim, degp1, 1, dim, tailidx, mat.mp[0].data(), mat.mp[1].data(),
+                            b.p(0), b.p(1), x.p(0), x.p(1), rg.p(0), rg.p(1), &mg[0], &mg[1], &ms,
+                            &add, &mul, 0);
   } else {
-    CPU_dbl4_linear_residue(dim, degp1, tailidx, mp[0].data(), mp[1].data(), mp[2].data(),
-                            mp[3].data(), b.p(0), b.p(1), b.p(2), b.p(3), x.p(0), x.p(1), x.p(2),
-                            x.p(3), rc.p(0), rc.p(1), rc.p(2), rc.p(3), &mc[0], &mc[1], &mc[2],
-                            &mc[3], 0);
-    GPU_dbl4_linear_residue(dim, degp1, 1, dim, tailidx, mp[0].data(), mp[1].data(),
-                            mp[2].data(), mp[3].data(), b.p(0), b.p(1), b.p(2), b.p(3), x.p(0),
-                            x.p(1), x.p(2), x.p(3), rg.p(0), rg.p(1), rg.p(2), rg.p(3), &mg[0],
-                            &mg[1], &mg[2], &mg[3], &ms, &add, &mul, 0);
+    GPU_dbl4_linear_residue(dim, degp1, 1, dim, tailidx, mat.mp[0].data(), mat.mp[1].data(),
+                            mat.mp[2].data(), mat.mp[3].data(), b.p(0), b.p(1), b.p(2), b.p(3),
+                            x.p(0), x.p(1), x.p(2), x.p(3), rg.p(0), rg.p(1), rg.p(2), rg.p(3),
+                            &mg[0], &mg[1], &mg[2], &mg[3], &ms, &add, &mul, 0);
   }
   long long prods = 0;
   for (int i = tailidx; i < degp1; ++i) prods += i - tailidx + 1;
@@ -549,61 +492,19 @@ static void test_linear_residue(int K, int dim, int degp1, int tailidx) {
   std::snprintf(what, sizeof what, "GPU_dbl%s_linear_residue dim %d degp1 %d tail %d (bitwise)",
                 K == 1 ? "" : (K == 2 ? "2" : "4"), dim, degp1, tailidx);
   check(same, what, same ? 0.0 : 1.0, 0.0);
-  for (RMat *m : mat) delete m;
-}
-
-// a real upper triangular system through CPU_dblK_upper_tiled_solver
-// (reference, on copies: it overwrites U and b) and GPU_dblK_upper_tiled_solver
-static void test_tiled_real(int K, int n, int szt) {
-  RMat U(K, n, n), Uc(K, n, n), b(K, 1, n), bc(K, 1, n), xc(K, 1, n), xg(K, 1, n);
-  U.randomize();
-  b.randomize();
-  for (int q = 0; q < K; ++q)
-    for (int i = 0; i < n; ++i)
-      for (int j = 0; j < i; ++j) U.p(q)[i][j] = 0.0;
-  for (int i = 0; i < n; ++i) U.p(0)[i][i] += 4.0;  // diagonally dominant enough
-  Uc.store = U.store;
-  bc.store = b.store;
-  const std::vector<std::vector<double>> U0 = U.store, b0 = b.store;
-  double t[4] = {0, 0, 0, 0};
-  long long cnt[3] = {0, 0, 0};
-  double over[3] = {0, 0, 0};
-  if (K == 2) {
-    CPU_dbl2_upper_tiled_solver(n, szt, n / szt, Uc.p(0), Uc.p(1), bc.p(0)[0], bc.p(1)[0],
-                                xc.p(0)[0], xc.p(1)[0], &t[0]);
-    GPU_dbl2_upper_tiled_solver(n, szt, n / szt, U.p(0), U.p(1), b.p(0)[0], b.p(1)[0],
-                                xg.p(0)[0], xg.p(1)[0], &t[0], &t[1], &t[2], &t[3], &t[0],
-                                &cnt[0], &cnt[1], &cnt[2]);
-  } else {
-    CPU_dbl4_upper_tiled_solver(n, szt, n / szt, Uc.p(0), Uc.p(1), Uc.p(2), Uc.p(3), bc.p(0)[0],
-                                bc.p(1)[0], bc.p(2)[0], bc.p(3)[0], xc.p(0)[0], xc.p(1)[0],
-                                xc.p(2)[0], xc.p(3)[0], &t[0]);
-    GPU_dbl4_upper_tiled_solver(n, szt, n / szt, U.p(0), U.p(1), U.p(2), U.p(3), b.p(0)[0],
-                                b.p(1)[0], b.p(2)[0], b.p(3)[0], xg.p(0)[0], xg.p(1)[0],
-                                xg.p(2)[0], xg.p(3)[0], &t[0], &t[1], &t[2], &t[3], &t[0],
-                                &cnt[0], &over[0], &cnt[1], &over[1], &cnt[2], &over[2]);
-  }
-  double err = 0.0;
-  for (int i = 0; i < n; ++i) {
-    double a[4], c[4];
-    for (int q = 0; q < K; ++q) { a[q] = xg.p(q)[0][i]; c[q] = xc.p(q)[0][i]; }
-    err = std::fmax(err, md_diff(K, a, c));
-  }
-  const double tol = 64.0 * n * std::pow(0x1p-52, K);
-  char what[96];
-  std::snprintf(what, sizeof what, "GPU_dbl%d_upper_tiled_solver %d szt %d: x", K, n, szt);
-  check(err < tol, what, err, tol);
-  std::snprintf(what, sizeof what, "GPU_dbl%d_upper_tiled_solver: U, b kept", K);
-  const bool kept = U.store == U0 && b.store == b0;
-  check(kept, what, kept ? 0.0 : 1.0, 0.0);
 }
 
 int main() {
-  std::srand(20261202);
-  test_products(15);
-  test_products(8);
-  test_products8(15);
-  test_products8(8);
+  for (int K : {1, 2, 4, 8}) {
+    test_products(K, 15);
+    test_products(K, 8);
+  }
+  // the evaluation shapes of the reference's polynomial testers: random
+  // supports, degree 15 series; all real and complex precisions
+  for (int K : {1, 2, 4, 8}) test_poly_evaldiff(K, 0, 8, 8, 3, 15);
+  for (int K : {1, 2, 4}) test_poly_evaldiff(K, 1, 8, 8, 3, 15);
+  test_poly_evaldiff(2, 0, 12, 40, 6, 8);
+  test_poly_evaldiff(4, 1, 6, 17, 6, 4);
   test_qr(2, 64, 64, 16);
   test_qr(4, 64, 64, 16);
   test_qr(2, 96, 64, 32);

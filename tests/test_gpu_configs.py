@@ -45,21 +45,32 @@ def test_c3_sample_matches_oracle(K):
 
 
 def test_c3_all_endpoints_by_properties():
-    """the mixed method on all 4096 paths: every endpoint's residual
-    max_i |p_i(x)| <= 1e-24 evaluated in quad-double at the endpoint, and the
-    4096 endpoints are distinct (the Bezout number of a generic system)"""
-    from phcpack_amd.homotopy import compile_homotopy  # noqa: F401
-    from phcpack_amd.tracker import Tracker, TrackParams, planes_to_points
+    """the mixed method on all 4096 paths: FP64 tracking ([5/1] Pade
+    predictor), double-double tracking from the start of the paths that FP64
+    could not finish inside the interval, dd/qd endpoint Newton; every
+    endpoint's residual max_i |p_i(x)| <= 1e-24 evaluated in quad-double at
+    the endpoint, and the 4096 endpoints are distinct (the Bezout number of
+    a generic system)"""
+    from phcpack_amd.tracker import (Tracker, TrackParams, escalate_to_dd, planes_to_points,
+                                     points_to_planes)
     p, hom, sols = headline_system()
     tr = Tracker(hom, deg=8)
     print("C3 all 4096 paths, mixed precision", flush=True)
-    g = tr.track(sols, K=1, params=TrackParams(tolend=1e-13))
-    assert np.all(g["status"] == 1), np.bincount(g["status"], minlength=6)
+    prm = TrackParams(tolend=1e-13)
+    g = tr.track(sols, K=1, params=prm)
+    starts = points_to_planes(sols)
+    idx, esc = escalate_to_dd(tr, starts, g["status"], g["tend"], prm)
+    print(f"  FP64 statuses {np.bincount(g['status'], minlength=6).tolist()}, "
+          f"escalated to dd tracking: {idx.tolist()}", flush=True)
+    ok = g["status"] == 1
+    assert np.all(ok | np.isin(np.arange(len(sols)), idx)), "a failure close to t = 1"
     r2 = tr.refine(g["endp"], 2, iters=3, tol=1e-28)
-    x = r2["x"]
-    bad = np.nonzero(r2["status"] != 1)[0]
-    x4 = np.zeros((8,) + x.shape[1:])
-    x4[:2], x4[4:6] = x[:2], x[2:]
+    x4 = np.zeros((8,) + g["endp"].shape[1:])
+    x4[:2], x4[4:6] = r2["x"][:2], r2["x"][2:]
+    if esc is not None:
+        assert np.all(esc["status"] == 1)
+        x4[:2, idx], x4[4:6, idx] = esc["endp"][:2], esc["endp"][2:]
+    bad = np.nonzero((r2["status"] != 1) & ok)[0]
     if len(bad):
         r4 = tr.refine(x4[:, bad], 4, iters=4, tol=1e-56)
         assert np.all(r4["status"] == 1)

@@ -376,3 +376,32 @@ def test_pade_tracker_on_a_rational_path(K):
     assert outs["pade"]["nsteps"][0] == outs["taylor"]["nsteps"][0]
     assert wp[3] <= wp[1] <= 2 * wp[3]  # converged at once, plus the extra step
     assert wt[1] > wp[1]
+
+
+def test_interior_failures_select_precision_escalation():
+    """the mixed method tracks again in double-double the paths that failed
+    inside the interval for numerical reasons (minstep, maxsteps, singular
+    Jacobian at t < 0.99), not successes, divergences or failures at t -> 1"""
+    from phcpack_amd.tracker import interior_failures
+    status = np.array([1, 2, 3, 4, 5, 2, 2, 4, 0])
+    tend = np.array([1.0, 0.5, 0.3, 0.98, 0.2, 0.995, 0.989, 1.0, 0.1])
+    assert interior_failures(status, tend).tolist() == [1, 2, 3, 6]
+    assert interior_failures(status, tend, t_interior=0.999).tolist() == [1, 2, 3, 5, 6]
+
+
+def test_dd_rescues_an_fp64_interior_failure():
+    """paths that FP64 tracking stops inside the interval (here by a step
+    budget of 12, which ends most paths at maxsteps) are the ones the rule
+    selects, and dd tracking from their start with the default parameters
+    finishes every one of them"""
+    from phcpack_amd.tracker import interior_failures
+    rng = np.random.default_rng(5)
+    hom, sols = total_degree_homotopy(dense_system(4, 2, rng), rng)
+    prog = compile_homotopy(hom, 8)
+    planes = points_to_planes(sols)
+    o1 = oracle_track(prog, planes, 1, TrackParams(tolend=1e-13, maxsteps=12))
+    idx = interior_failures(o1["status"], o1["tend"])
+    assert set(idx.tolist()) == set(np.nonzero((o1["status"] == 3) & (o1["tend"] < 0.99))[0].tolist())
+    assert len(idx) > 0
+    o2 = oracle_track(prog, planes[:, idx], 2, TrackParams(tolend=1e-28))
+    assert np.all(o2["status"] == 1)
