@@ -293,6 +293,10 @@ def main():
                     help="leave the rank-deficient n = 14 block out of the headline line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also launch one 256 MiB device copy kernel (HBM counter scale)")
+    ap.add_argument("--save-controller", metavar="NPZ",
+                    help="write the trained step controller to this file")
+    ap.add_argument("--load-controller", metavar="NPZ",
+                    help="use a saved step controller instead of training one (profiled runs)")
     args = ap.parse_args()
     args.pade = parse_predictor(args.predictor)
     if args.gpus != 1:
@@ -407,7 +411,16 @@ def run(args):
         check(lib.phc_device_copy(dst.ptr, big.ptr, big.nbytes, None))
         check(lib.phc_synchronize())
 
-    ctl, train_info = (None, None) if args.no_learned else train_learned(n, prm_mixed, ev)
+    if args.no_learned:
+        ctl, train_info = None, None
+    elif args.load_controller:
+        from phcpack_amd.controller import StepController
+        ctl = StepController.load(args.load_controller)
+        train_info = {"loaded_from": args.load_controller}
+    else:
+        ctl, train_info = train_learned(n, prm_mixed, ev)
+    if ctl is not None and args.save_controller:
+        ctl.save(args.save_controller)
     cstruct = ctl.struct(1) if ctl is not None else None
     # launch shape per precision, tuned on this system before the timed region
     from phcpack_amd.autotune import tune_waves

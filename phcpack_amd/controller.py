@@ -124,6 +124,18 @@ class StepController:
         self.pole_cap = pole_cap  # multiplied steps stay <= pole_cap * Fabry radius
         self._dev = None
 
+    def save(self, path):
+        """weights, standardization, candidates, threshold and pole cap as
+        an .npz file (plain arrays, no pickled objects)"""
+        np.savez(path, w=self.w, mu=self.mu, sd=self.sd, mults=np.asarray(self.mults),
+                 threshold=np.float64(self.threshold), pole_cap=np.float64(self.pole_cap))
+
+    @classmethod
+    def load(cls, path):
+        with np.load(path, allow_pickle=False) as z:
+            return cls(weights=z["w"], mu=z["mu"], sd=z["sd"], threshold=float(z["threshold"]),
+                       mults=tuple(float(m) for m in z["mults"]), pole_cap=float(z["pole_cap"]))
+
     @property
     def logit_threshold(self):
         return float(np.log(self.threshold / (1.0 - self.threshold)))
