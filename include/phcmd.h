@@ -417,6 +417,17 @@ int phc_cmplx_qhb(phc_baqr *plan, const double *Q_dev, const double *b_dev, doub
 int phc_cmplx_upper_tiled_solver(int K, int dim, int szt, int nbt, double *U_dev, double *b_dev,
                                  double *x_dev, void *stream);
 
+/* The same solve without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc); no counterpart in the reference.  The tile inverses
+ * live in a static device scratch of the library of 1048576 doubles (8 MiB)
+ * per GPU: 2*K*dim*szt must not exceed it (dim 1024 in tiles of 128 in
+ * quad-double fits), else -2.  Queued on the stream, no synchronization.
+ * Every call on one GPU shares that scratch: calls that overlap on one GPU
+ * (two streams, two threads) race and must be ordered by the caller. */
+int phc_cmplx_upper_tiled_solver_noalloc(int K, int dim, int szt, int nbt, double *U_dev,
+                                         double *b_dev, double *x_dev, void *stream);
+
 /* C(i,j) = [C(i,j) +] sum_k op(A(i,k)) op(B(k,j)) for complex
  * multiple-double operands given by base pointer, plane stride and the two
  * element strides (element (r,c) of A at A + a0*r + a1*c in every plane);

@@ -121,6 +121,8 @@ SIGNATURES.update({
     "phc_cmplx_qhb": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "phc_cmplx_upper_tiled_solver": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p,
                                              c_void_p, c_void_p]),
+    "phc_cmplx_upper_tiled_solver_noalloc": (c_int, [c_int, c_int, c_int, c_int, c_void_p,
+                                                     c_void_p, c_void_p, c_void_p]),
     "phc_cmplx_gemm": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int,
                                c_void_p, c_size_t, ctypes.c_long, ctypes.c_long,
                                c_void_p, c_size_t, ctypes.c_long, ctypes.c_long,

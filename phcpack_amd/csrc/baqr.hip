@@ -32,6 +32,7 @@
 // order by a second kernel.  Results agree with the reference's host
 // code to a few units in the last limb (the order of the long sums differs);
 // the tests bound the difference.
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -660,6 +661,32 @@ int run_tiled(int dim, int szt, int nbt, double *U, double *b, double *x, double
   return 0;
 }
 
+// Device scratch of the allocation-free tiled solver: the inverses of the
+// diagonal tiles, 2*K*nbt*szt*szt doubles, at most kTiledScratch (8 MiB:
+// dim 1024 in tiles of 128 in complex quad-double).  Every GPU holds its own
+// copy for the life of the process; calls on one GPU share it.
+constexpr size_t kTiledScratch = size_t(1) << 20;
+static __device__ __align__(256) double scratch[kTiledScratch];
+
+// The device address of `scratch` on the current device, looked up once
+// per device (the host-side name is not a device address).
+int tiled_scratch(double **out) {
+  static std::mutex mu;
+  static std::vector<double *> ptrs;
+  int dev = 0;
+  PHC_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)ptrs.size() <= dev) ptrs.resize(dev + 1, nullptr);
+  if (!ptrs[dev]) {
+    void *p = nullptr;
+    PHC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    PHC_REQUIRE(p != nullptr, "phc_cmplx_upper_tiled_solver_noalloc: no scratch address");
+    ptrs[dev] = (double *)p;
+  }
+  *out = ptrs[dev];
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -793,6 +820,26 @@ int phc_cmplx_upper_tiled_solver(int K, int dim, int szt, int nbt, double *U, do
   if (rc) return rc;
   PHC_HIP(e2);
   return 0;
+}
+
+// The same solve with the tile inverses in the library's static device
+// scratch: no allocation and no synchronization (the work is queued on the
+// stream).  Calls that overlap on one GPU share the scratch and race.
+int phc_cmplx_upper_tiled_solver_noalloc(int K, int dim, int szt, int nbt, double *U, double *b,
+                                         double *x, void *stream) {
+  PHC_REQUIRE(K == 2 || K == 4, "phc_cmplx_upper_tiled_solver_noalloc: K must be 2 or 4");
+  PHC_REQUIRE(szt >= 1 && szt <= 256 && nbt >= 1 && dim == szt * nbt,
+              "phc_cmplx_upper_tiled_solver_noalloc: dim must equal szt*nbt, szt <= 256");
+  PHC_REQUIRE(U && b && x, "phc_cmplx_upper_tiled_solver_noalloc: null array");
+  PHC_REQUIRE((size_t)2 * K * dim * szt <= kTiledScratch,
+              "phc_cmplx_upper_tiled_solver_noalloc: the tile inverses (2*K*dim*szt doubles) "
+              "exceed the 1048576-double (8 MiB) scratch");
+  double *inv = nullptr;
+  const int rs = tiled_scratch(&inv);
+  if (rs) return rs;
+  hipStream_t st = (hipStream_t)stream;
+  return K == 2 ? run_tiled<2>(dim, szt, nbt, U, b, x, inv, st)
+                : run_tiled<4>(dim, szt, nbt, U, b, x, inv, st);
 }
 
 }  // extern "C"
