@@ -112,7 +112,7 @@ def run_mixed(tr, d_start, starts, npaths, bufs, ref, ev, prm, lib, controller=N
     bad = np.nonzero((st != 1) & tracked)[0]
     fix = {}
     # precision escalation: interior failures of FP64 tracking, tracked in dd
-    esc_idx, esc = escalate_to_dd(tr, starts, status1, bufs.tend.to_host(), prm)
+    esc_idx, esc = escalate_to_dd(tr, starts, status1, bufs.tend.to_host(), prm, endp=bufs.endp)
     if esc is not None:
         xe = to_qd(esc["endp"], 2)
         for j, p in enumerate(esc_idx):
@@ -700,6 +700,26 @@ def rankdef_block(args, lib, ev, cstruct=None, n=14, rank_deficient=2, seed=SEED
     names = ["running", "success", "minstep", "maxsteps", "singular", "diverged"]
     cls = lambda st: {names[i]: int(c) for i, c in enumerate(np.bincount(st, minlength=6)) if c}  # noqa: E731
 
+    def end_classes(o, K):
+        """endgame estimates at every path's last point (phc_endgame, timed)
+        and the classes of classify_endpoints"""
+        from phcpack_amd.tracker import END_NAMES, classify_endpoints
+        planes = np.ascontiguousarray(o["endp"])
+        d_x = DeviceArray.from_host(planes)
+        d_t = DeviceArray.from_host(np.ascontiguousarray(o["tend"]))
+        est = {k: DeviceArray((npaths,)) for k in ("alpha", "rho", "rco", "xmax")}
+        ev.start()
+        tr.endgame_device(d_x, K, npaths, d_t, K, est["alpha"], est["rho"], est["rco"],
+                          est["xmax"])
+        ms = ev.stop_ms()
+        e = {k: v.to_host() for k, v in est.items()}
+        cls, wind = classify_endpoints(o["status"], o["tend"], e)
+        out = {END_NAMES[i]: int(c) for i, c in enumerate(np.bincount(cls, minlength=4)) if c}
+        ws = wind[cls == 2]
+        return {"classes": out, "winding_numbers": {str(int(w)): int(c) for w, c in
+                                                    zip(*np.unique(ws, return_counts=True))},
+                "endgame_ms": round(ms, 3)}
+
     def method(o, rows_, x, res, ok, c):
         t_track = float(np.mean([r[0] for r in rows_]))
         t = float(np.mean([r[0] + r[1] for r in rows_]))
@@ -714,7 +734,8 @@ def rankdef_block(args, lib, ev, cstruct=None, n=14, rank_deficient=2, seed=SEED
             "distinct_solutions": distinct(c, ok),
             "steps_per_path": float(o["nsteps"].mean()),
             "lost_paths_vs_plain": int(count_lost(cp, ok_p, c, ok)),
-            "fp64_tflops": round(fl / (t_track / 1e3) / 1e12, 3)}
+            "fp64_tflops": round(fl / (t_track / 1e3) / 1e12, 3),
+            "endgame": end_classes(o, 1)}
 
     v_m, blk_m = method(om, rows, xm, res_m, ok_m, cm)
     blk = {
@@ -726,7 +747,8 @@ def rankdef_block(args, lib, ev, cstruct=None, n=14, rank_deficient=2, seed=SEED
                      "good_paths": int(ok_p.sum()), "tracker_statuses": cls(op["status"]),
                      "max_good_residual": float(res_p[ok_p].max()) if ok_p.any() else None,
                      "distinct_solutions": distinct(cp, ok_p),
-                     "steps_per_path": float(op["nsteps"].mean())},
+                     "steps_per_path": float(op["nsteps"].mean()),
+                     "endgame": end_classes(op, 2)},
         "speedup_mixed_vs_plain": round(float(v_m / v_p), 3) if v_p > 0 else None,
         "waves_per_workgroup": {"plain_dd": tuned[2][0], "fp64": tuned[1][0]},
     }

@@ -281,6 +281,17 @@ int phc_track(phc_tracker *tracker, int K, int npaths, const double *start_dev, 
               int *nrej_dev, double *dxlast_dev, int *work_dev, const phc_track_params *params,
               const phc_controller *controller, int waves, void *stream);
 
+/* phc_track with a start parameter per path, tstart [npaths] (NULL: 0): path
+ * p is tracked from the point start[p] at t = tstart[p] to t = 1.  A path
+ * that stops (status 2-5) ends at its last accepted point, with tend its
+ * parameter, so a run in another precision can continue it from there (the
+ * mixed method's escalation of FP64 stops to double-double). */
+int phc_track_from(phc_tracker *tracker, int K, int npaths, const double *start_dev, int KS,
+                   const double *tstart_dev, double *endp_dev, double *tend_dev, int *status_dev,
+                   int *nsteps_dev, int *nrej_dev, double *dxlast_dev, int *work_dev,
+                   const phc_track_params *params, const phc_controller *controller, int waves,
+                   void *stream);
+
 /* ------------------------------------------------------- controller MLP */
 
 /* number of floats of an MLP F -> H -> H -> C (W1[F][H], b1, W2[H][H], b2,
@@ -312,6 +323,27 @@ int phc_mlp_train(int F, int H, int C, float *w_dev, int N, const float *X_dev,
 int phc_refine(phc_tracker *tracker, int K, int npaths, double t, const double *xin_dev, int KS,
                double *xout_dev, int iters, double tol, double *dxlast_dev, double *res_dev,
                int *status_dev, const int *only_dev, int waves, void *stream);
+
+/* Where the paths go as t -> 1: the question the endgames of
+ * Deformations/End_Games answer (the winding number of a singular endpoint,
+ * Ada's *_vlprs_algorithm extrapolation, or divergence to infinity),
+ * estimated here from the Taylor series of x(t + s) at each path's last
+ * point x [2*KS][npaths][n] and parameter tend [npaths], in precision K (1 or
+ * 2): ratio method on the last three Taylor coefficients of each component
+ * (q_k = c_{k+1}/c_k at k = deg - 1): the nearest singularity lies at
+ * s* = 1/((k+1) q_k - k q_{k-1}) and the component behaves like
+ * (s* - s)^alpha with alpha = k - (k+1) s* q_k, exact for c (s* - s)^alpha.
+ * Per path: alpha and rho = Re s* of the component whose last coefficient
+ * weighs most at the distance 1 - tend (+inf when every component has a
+ * zero coefficient); rco = 1/(sigma_max(J) sigma_max(J^{-1})), the inverse
+ * 2-norm condition number estimate of the Jacobian at the point (0 when J
+ * is singular); xmax = max_i |x_i|.  alpha < 0 with rho ~ 1 - tend: a pole
+ * at t = 1 (the path diverges); 0 < alpha < 1: a branch point, winding
+ * number 1/alpha.  No counterpart in the reference's GPU code; the
+ * classification built on it is phcpack_amd.tracker.classify_endpoints. */
+int phc_endgame(phc_tracker *tracker, int K, int npaths, const double *x_dev, int KS,
+                const double *tend_dev, double *alpha_dev, double *rho_dev, double *rco_dev,
+                double *xmax_dev, int waves, void *stream);
 
 /* ---------------- linearized power-series Newton (real series) ----------
  * Layouts are limb-planar with the batch index after the limb:

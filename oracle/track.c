@@ -954,8 +954,8 @@ static double minimum3(double a, double b, double c, int *which) {
  * mixed residual, Newton iterations of the last corrector, accepted (1)
  * or not, sigma_min(J), Hessian sum, t after the step, candidate */
 static void track_path(const otr_program *P, work_t *w, int npaths, int path, const double *start,
-                       int KS, double *endp, double *tend, int *status, int *nsteps, int *nrej,
-                       double *dxlast, int *work, const otr_params *prm,
+                       int KS, const double *tstart, double *endp, double *tend, int *status,
+                       int *nsteps, int *nrej, double *dxlast, int *work, const otr_params *prm,
                        const otr_controller *ctl, double *trace, int cap, int *ntrace) {
   const int K = w->K, n = P->n, D = P->deg + 1;
   const int mode = ctl ? ctl->mode : 0;
@@ -971,7 +971,8 @@ static void track_path(const otr_program *P, work_t *w, int npaths, int path, co
      both attempts, the other outputs are the last attempt's */
   for (attempt = 0; attempt < 2; attempt++) {
     pmode = (attempt == 0) ? mode : 0;
-    t = 0.0; dxl = 0.0; hprev = prm->maxstep;
+    t = tstart ? tstart[path] : 0.0;
+    dxl = 0.0; hprev = prm->maxstep;
     for (j = 0; j < 4; j++) dxh[j] = 1.0;
     st = ST_RUNNING; steps = 0; rej = 0; rejprev = 0;
     if (ntrace) *ntrace = 0;
@@ -1060,11 +1061,18 @@ static void track_path(const otr_program *P, work_t *w, int npaths, int path, co
                              &nfirst, dxh);
         ++ncorr;
         nit = nnewt - it0;
-        if (nr < 0) { st = ST_SINGULAR; break; }
-        if (nr == 1) { t = tn; hprev = step; accepted = 1; break; }
-        step = step / 2.0;
-        ++rej;
-        if (step < prm->minstep) { st = ST_MINSTEP; break; }
+        if (nr < 0) st = ST_SINGULAR;
+        else if (nr == 1) { t = tn; hprev = step; accepted = 1; break; }
+        else {
+          step = step / 2.0;
+          ++rej;
+          if (step < prm->minstep) st = ST_MINSTEP;
+        }
+        if (st != ST_RUNNING) {
+          /* the path ends at its last accepted point (t, x0) */
+          for (j = 0; j < n; j++) ccopy(K, ZP(w->x0, j), ZP(w->slots, j * D));
+          break;
+        }
         tn = t + step;
         for (j = 0; j < n; j++)
           otr_pade_evaluate(K, ZP(w->slots, j * D), pnum, pden, ZP(w->x0, j), step, ZP(w->slots, j * D));
@@ -1120,14 +1128,15 @@ static void track_path(const otr_program *P, work_t *w, int npaths, int path, co
 }
 
 void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const double *start,
-               int KS, double *endp, double *tend, int *status, int *nsteps, int *nrej,
-               double *dxlast, int *work, const otr_params *prm, const otr_controller *ctl) {
+               int KS, const double *tstart, double *endp, double *tend, int *status,
+               int *nsteps, int *nrej, double *dxlast, int *work, const otr_params *prm,
+               const otr_controller *ctl) {
   work_t w = {0};
   int path;
   if (P->n > 64 || !work_alloc(P, K, &w)) { work_free(&w); return; }
   for (path = p0; path < p1; path++)
-    track_path(P, &w, npaths, path, start, KS, endp, tend, status, nsteps, nrej, dxlast, work,
-               prm, ctl, NULL, 0, NULL);
+    track_path(P, &w, npaths, path, start, KS, tstart, endp, tend, status, nsteps, nrej, dxlast,
+               work, prm, ctl, NULL, 0, NULL);
   work_free(&w);
 }
 
@@ -1138,8 +1147,8 @@ int otr_track_trace(const otr_program *P, int K, int npaths, int path, const dou
   work_t w = {0};
   int nt = 0;
   if (P->n > 64 || !work_alloc(P, K, &w)) { work_free(&w); return -1; }
-  track_path(P, &w, npaths, path, start, KS, endp, tend, status, nsteps, nrej, dxlast, work,
-             prm, ctl, trace, cap, &nt);
+  track_path(P, &w, npaths, path, start, KS, NULL, endp, tend, status, nsteps, nrej, dxlast,
+             work, prm, ctl, trace, cap, &nt);
   work_free(&w);
   return nt;
 }
@@ -1147,7 +1156,7 @@ int otr_track_trace(const otr_program *P, int K, int npaths, int path, const dou
 typedef struct {
   const otr_program *P;
   int K, npaths, p0, p1, KS;
-  const double *start;
+  const double *start, *tstart;
   double *endp, *tend, *dxlast;
   int *status, *nsteps, *nrej, *work;
   const otr_params *prm;
@@ -1156,13 +1165,14 @@ typedef struct {
 
 static void *run_job(void *arg) {
   job_t *j = (job_t *)arg;
-  otr_track(j->P, j->K, j->npaths, j->p0, j->p1, j->start, j->KS, j->endp, j->tend, j->status,
-            j->nsteps, j->nrej, j->dxlast, j->work, j->prm, j->ctl);
+  otr_track(j->P, j->K, j->npaths, j->p0, j->p1, j->start, j->KS, j->tstart, j->endp, j->tend,
+            j->status, j->nsteps, j->nrej, j->dxlast, j->work, j->prm, j->ctl);
   return NULL;
 }
 
 void otr_track_mt(const otr_program *P, int K, int npaths, const double *start, int KS,
-                  double *endp, double *tend, int *status, int *nsteps, int *nrej,
+                  const double *tstart, double *endp, double *tend, int *status, int *nsteps,
+                  int *nrej,
                   double *dxlast, int *work, const otr_params *prm, const otr_controller *ctl,
                   int nthreads) {
   pthread_t th[256];
@@ -1174,7 +1184,7 @@ void otr_track_mt(const otr_program *P, int K, int npaths, const double *start, 
     jobs[i].P = P; jobs[i].K = K; jobs[i].npaths = npaths; jobs[i].KS = KS;
     jobs[i].p0 = (int)((long)npaths * i / nthreads);
     jobs[i].p1 = (int)((long)npaths * (i + 1) / nthreads);
-    jobs[i].start = start; jobs[i].endp = endp; jobs[i].tend = tend; jobs[i].dxlast = dxlast;
+    jobs[i].start = start; jobs[i].tstart = tstart; jobs[i].endp = endp; jobs[i].tend = tend; jobs[i].dxlast = dxlast;
     jobs[i].status = status; jobs[i].nsteps = nsteps; jobs[i].nrej = nrej; jobs[i].prm = prm;
     jobs[i].work = work;
     jobs[i].ctl = ctl;
@@ -1199,6 +1209,88 @@ void otr_refine(const otr_program *P, int K, int npaths, double t, const double 
     store_point(P, &w, xout, npaths, path);
     dxlast[path] = dxl;
     status[path] = nr;
+  }
+  work_free(&w);
+}
+
+/* Where a path goes as t -> 1, from the Taylor series of x(t+s) at its last
+ * point (the question the endgames of Deformations/End_Games answer: a
+ * regular endpoint, a singular one with its winding number, or divergence
+ * to infinity).  Ratio method on the last three coefficients of every
+ * component, FP64 on the leading limbs: with q_k = c_{k+1}/c_k at k = D-2,
+ * the nearest singularity of the component lies at
+ * s* = 1 / ((k+1) q_k - k q_{k-1}) and the component behaves like
+ * (s* - s)^alpha with alpha = k - (k+1) s* q_k (both exact for
+ * c (s* - s)^alpha).  The component whose last coefficient weighs most at
+ * the distance 1 - t (|c_{D-1}| (1-t)^(D-1)) gives alpha = Re and
+ * rho = Re s*; components with a zero coefficient (Fabry's Is_Zero) are
+ * passed over, and when all are, alpha = rho = +inf (no singularity
+ * within reach).  rco = 1 / (sigma_max(J) sigma_max(J^{-1})) of the
+ * Jacobian at the point (0 when singular), xmax = max_i |x_i|. */
+static int is_zero2(const double *z) { return z[0] + 1.0 == 1.0 && z[1] + 1.0 == 1.0; }
+
+static void zdiv(const double *a, const double *b, double *z) {
+  const double nrm = b[0] * b[0] + b[1] * b[1];
+  z[0] = (a[0] * b[0] + a[1] * b[1]) / nrm;
+  z[1] = (a[1] * b[0] - a[0] * b[1]) / nrm;
+}
+
+static void endgame_point(const otr_program *P, work_t *w, double t, double *alpha, double *rho,
+                          double *rco, double *xmax) {
+  const int K = w->K, n = P->n, D = P->deg + 1, k = D - 2;
+  double M[2 * 64 * 64], lamJ, lamI, hs2, best = -1.0, dist = 1.0 - t, xm = 0.0;
+  int j, i;
+  set_weights(P, w, t);
+  eval_point(P, w, 1);
+  lead_matrix(K, n, w->A, M);
+  lamJ = otr_sigma_max_sq(n, M, w->SP, w->SQ);
+  *alpha = INFINITY;
+  *rho = INFINITY;
+  if (!predictor_series(P, w, &hs2)) {
+    *rco = 0.0;
+  } else {
+    lead_matrix(K, n, w->A, M);
+    lamI = otr_sigma_max_sq(n, M, w->SP, w->SQ);
+    *rco = 1.0 / sqrt(lamJ * lamI);
+  }
+  for (j = 0; j < n; j++) {
+    const double *z = ZP(w->slots, j * D);
+    const double r = zrad(z[0], z[K]);
+    if (r > xm) xm = r;
+  }
+  *xmax = xm;
+  if (*rco == 0.0 || k < 2) return;
+  for (j = 0; j < n; j++) {
+    const double *c0 = ZP(w->slots, j * D + k - 1), *c1 = ZP(w->slots, j * D + k),
+                 *c2 = ZP(w->slots, j * D + k + 1);
+    double a0[2] = {c0[0], c0[K]}, a1[2] = {c1[0], c1[K]}, a2[2] = {c2[0], c2[K]};
+    double q0[2], q1[2], den[2], one[2] = {1.0, 0.0}, ss[2], sq[2], pw = 1.0, wt;
+    if (is_zero2(a0) || is_zero2(a1) || is_zero2(a2)) continue;
+    zdiv(a1, a0, q0);
+    zdiv(a2, a1, q1);
+    den[0] = (double)(k + 1) * q1[0] - (double)k * q0[0];
+    den[1] = (double)(k + 1) * q1[1] - (double)k * q0[1];
+    if (den[0] == 0.0 && den[1] == 0.0) continue;
+    zdiv(one, den, ss);
+    zmul(ss, q1, sq);
+    for (i = 0; i <= k; i++) pw = pw * dist;
+    wt = zrad(a2[0], a2[1]) * pw;
+    if (wt > best) {
+      best = wt;
+      *alpha = (double)k - (double)(k + 1) * sq[0];
+      *rho = ss[0];
+    }
+  }
+}
+
+void otr_endgame(const otr_program *P, int K, int npaths, const double *x, int KS,
+                 const double *tend, double *alpha, double *rho, double *rco, double *xmax) {
+  work_t w = {0};
+  int path;
+  if (P->n > 64 || !work_alloc(P, K, &w)) { work_free(&w); return; }
+  for (path = 0; path < npaths; path++) {
+    load_point(P, &w, x, KS, npaths, path);
+    endgame_point(P, &w, tend[path], alpha + path, rho + path, rco + path, xmax + path);
   }
   work_free(&w);
 }

@@ -51,10 +51,12 @@ typedef struct {
   double pole_cap;
 } otr_controller;
 
-/* paths p0 <= p < p1 of npaths; arrays as in phc_track; ctl may be NULL */
+/* paths p0 <= p < p1 of npaths; arrays as in phc_track_from (tstart NULL:
+ * every path starts at t = 0); ctl may be NULL */
 void otr_track(const otr_program *P, int K, int npaths, int p0, int p1, const double *start,
-               int KS, double *endp, double *tend, int *status, int *nsteps, int *nrej,
-               double *dxlast, int *work, const otr_params *prm, const otr_controller *ctl);
+               int KS, const double *tstart, double *endp, double *tend, int *status,
+               int *nsteps, int *nrej, double *dxlast, int *work, const otr_params *prm,
+               const otr_controller *ctl);
 
 /* one path with a log of its steps: trace receives 20 doubles per step of
  * the last attempt (at most cap steps; fields listed at track_path in
@@ -73,7 +75,8 @@ void otr_pade_error(int K, double *c, int L, int M, double *x0, double *err);
 
 /* same as otr_track over all paths with nthreads POSIX threads */
 void otr_track_mt(const otr_program *P, int K, int npaths, const double *start, int KS,
-                  double *endp, double *tend, int *status, int *nsteps, int *nrej,
+                  const double *tstart, double *endp, double *tend, int *status, int *nsteps,
+                  int *nrej,
                   double *dxlast, int *work, const otr_params *prm, const otr_controller *ctl,
                   int nthreads);
 
@@ -89,6 +92,14 @@ void otr_eval(const otr_program *P, int K, double t, const double *x, double *va
 
 /* Taylor coefficients of the solution curve at (x, t): series [2K][n][deg+1] */
 int otr_series(const otr_program *P, int K, double t, const double *x, double *series);
+
+/* per path: alpha, rho (the exponent and the distance of the nearest
+ * singularity of the dominant component of x(tend + s), ratio method on
+ * its last three Taylor coefficients; +inf when none is within reach),
+ * rco = 1/cond2 of the Jacobian estimate and xmax = max |x_i|, at the
+ * points x [2KS][npaths][n] and parameters tend [npaths] */
+void otr_endgame(const otr_program *P, int K, int npaths, const double *x, int KS,
+                 const double *tend, double *alpha, double *rho, double *rco, double *xmax);
 
 /* [L/M] Pade coefficients (1 <= M <= 8) of one component's series
  * c[0..L+M] (complex, 2K doubles each) in place: c_1..c_L the numerator's,

@@ -93,6 +93,10 @@ SIGNATURES.update({
     "phc_track": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                           c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(PhcTrackParams),
                           ctypes.POINTER(PhcController), c_int, c_void_p]),
+    "phc_track_from": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p,
+                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                               ctypes.POINTER(PhcTrackParams), ctypes.POINTER(PhcController),
+                               c_int, c_void_p]),
     "phc_mlp_size": (c_int, [c_int, c_int, c_int, P_int]),
     "phc_mlp_forward": (c_int, [c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "phc_mlp_grad": (c_int, [c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
@@ -100,6 +104,8 @@ SIGNATURES.update({
     "phc_mlp_train": (c_int, [c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                               c_int, c_int, ctypes.c_float, ctypes.c_uint,
                               ctypes.POINTER(ctypes.c_float), c_void_p]),
+    "phc_endgame": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p,
+                            c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "phc_refine": (c_int, [c_void_p, c_int, c_int, c_double, c_void_p, c_int, c_void_p, c_int,
                            c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "phc_houseqr_batch": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -1179,6 +1179,7 @@ __device__ cplx<K> pade_value(const cplx<K> *c, int L, const cplx<K> &x0, double
 template <int K, int MW, int PD>
 __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, const Params *__restrict__ parp, int npaths,
                                                    const double *__restrict__ start, int KS,
+                                                   const double *__restrict__ tstart,
                                                    double *__restrict__ endp, double *tend,
                                                    int *status, int *nsteps_out, int *nrej_out,
                                                    double *dx_out, int *work,
@@ -1213,7 +1214,8 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, const Para
     // both attempts, the other outputs are the last attempt's
     for (int attempt = 0; attempt < 2; ++attempt) {
     pmode = (attempt == 0) ? ctl.mode : 0;
-    t = 0.0; dxl = 0.0; hprev = par.maxstep;
+    t = tstart ? tstart[path] : 0.0;
+    dxl = 0.0; hprev = par.maxstep;
     dxh[0] = dxh[1] = dxh[2] = dxh[3] = 1.0;
     st = ST_RUNNING; steps = 0; rej = 0; rejprev = 0;
     load_point<K>(P, w, start, KS, npaths, path);
@@ -1319,11 +1321,21 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, const Para
         const int nr =
             lu_newton_steps<K>(P, w, par.corsteps, par.tolcor, evaluated, &dxl, &nnewt, &nfirst, dxh);
         ++ncorr;
-        if (nr < 0) { st = ST_SINGULAR; break; }
-        if (nr == 1) { t = tn; hprev = step; break; }
-        step = step / 2.0;
-        ++rej;
-        if (step < par.minstep) { st = ST_MINSTEP; break; }
+        if (nr < 0) {
+          st = ST_SINGULAR;
+        } else if (nr == 1) {
+          t = tn; hprev = step; break;
+        } else {
+          step = step / 2.0;
+          ++rej;
+          if (step < par.minstep) st = ST_MINSTEP;
+        }
+        if (st != ST_RUNNING) {
+          // the path ends at its last accepted point (t, x0)
+          if (l < n) w.slots[l * D] = w.x0[l];
+          wsync();
+          break;
+        }
         tn = t + step;
         if (l < n) w.slots[l * D] = pade_value<K, PD>(w.slots + l * D, par.pnum, w.x0[l], step);
         wsync();
@@ -1417,6 +1429,92 @@ __global__ void __launch_bounds__(64 * REFINE_MAX_WAVES) refine_kernel(DevProgra
       dx_out[path] = dxl;
       res_out[path] = res;
       status[path] = nr;
+    }
+    wsync();
+  }
+}
+
+// Where a path goes as t -> 1 (the question of the endgames of
+// Deformations/End_Games), from the Taylor series of x(t+s) at the path's
+// last point: ratio method on the last three coefficients of each component
+// (FP64, leading limbs; q_k = c_{k+1}/c_k at k = D-2): the nearest
+// singularity at s* = 1/((k+1) q_k - k q_{k-1}), the component like
+// (s* - s)^alpha with alpha = k - (k+1) s* q_k.  The component whose last
+// coefficient weighs most at the distance 1 - t gives alpha = Re and
+// rho = Re s* (+inf when every component has a zero coefficient); rco =
+// 1/(sigma_max(J) sigma_max(J^{-1})) (0 when J is singular), xmax =
+// max |x_i|.  The oracle's otr_endgame, same operations.
+__device__ __forceinline__ bool is_zero2(double2 z) { return z.x + 1.0 == 1.0 && z.y + 1.0 == 1.0; }
+
+__device__ __forceinline__ double2 zdiv(double2 a, double2 b) {
+  const double nrm = b.x * b.x + b.y * b.y;
+  return make_double2((a.x * b.x + a.y * b.y) / nrm, (a.y * b.x - a.x * b.y) / nrm);
+}
+
+template <int K>
+__global__ void __launch_bounds__(64 * REFINE_MAX_WAVES) endgame_kernel(DevProgram G, int npaths,
+                                                     const double *__restrict__ x, int KS,
+                                                     const double *__restrict__ tend,
+                                                     double *alpha_out, double *rho_out,
+                                                     double *rco_out, double *xmax_out,
+                                                     int *counter) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const ProgLayout PL(G);
+  const PathLayout<K> WL(G);
+  stage_program(G, PL, lds);
+  __syncthreads();
+  const LProg P = lprog(G, PL, lds);
+  Path<K> w = carve<K>(WL, lds + PL.total + (threadIdx.x >> 6) * WL.total);
+  const int l = lane_id(), n = P.n, D = P.D, k = D - 2;
+  double2 *U = reinterpret_cast<double2 *>(w.u), *Y = reinterpret_cast<double2 *>(w.col);
+  for (;;) {
+    int path = 0;
+    if (l == 0) path = atomicAdd(counter, 1);
+    path = __shfl(path, 0, 64);
+    if (path >= npaths) break;
+    load_point<K>(P, w, x, KS, npaths, path);
+    const double t = tend[path], dist = 1.0 - t;
+    set_weights<K>(P, w, t);
+    eval_point<K>(P, w, true);
+    const double lamJ =
+        sigma_max_sq(n, [&](int a, int b) { return lead<K>(w.A[a * n + b]); }, U, U + n * n, Y);
+    double hs2, rco = 0.0;
+    if (predictor_series<K>(P, w, &hs2)) {
+      const double lamI =
+          sigma_max_sq(n, [&](int a, int b) { return lead<K>(w.A[a * n + b]); }, U, U + n * n, Y);
+      rco = 1.0 / sqrt(lamJ * lamI);
+    }
+    double xm = 0.0, alpha = INFINITY, rho = INFINITY, best = -1.0;
+    for (int j = 0; j < n; ++j) {
+      const double2 z = lead<K>(w.slots[j * D]);
+      const double r = zrad(z.x, z.y);
+      if (r > xm) xm = r;
+    }
+    if (rco != 0.0 && k >= 2) {
+      for (int j = 0; j < n; ++j) {
+        const double2 a0 = lead<K>(w.slots[j * D + k - 1]), a1 = lead<K>(w.slots[j * D + k]),
+                      a2 = lead<K>(w.slots[j * D + k + 1]);
+        if (is_zero2(a0) || is_zero2(a1) || is_zero2(a2)) continue;
+        const double2 q0 = zdiv(a1, a0), q1 = zdiv(a2, a1);
+        const double2 den = make_double2((double)(k + 1) * q1.x - (double)k * q0.x,
+                                         (double)(k + 1) * q1.y - (double)k * q0.y);
+        if (den.x == 0.0 && den.y == 0.0) continue;
+        const double2 ss = zdiv(make_double2(1.0, 0.0), den), sq = zmul(ss, q1);
+        double pw = 1.0;
+        for (int i = 0; i <= k; ++i) pw = pw * dist;
+        const double wt = zrad(a2.x, a2.y) * pw;
+        if (wt > best) {
+          best = wt;
+          alpha = (double)k - (double)(k + 1) * sq.x;
+          rho = ss.x;
+        }
+      }
+    }
+    if (l == 0) {
+      alpha_out[path] = alpha;
+      rho_out[path] = rho;
+      rco_out[path] = rco;
+      xmax_out[path] = xm;
     }
     wsync();
   }
@@ -1694,6 +1792,14 @@ int phc_tracker_lds_bytes(const phc_tracker *tr, int K, size_t *program_bytes, s
 int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, double *endp,
               double *tend, int *status, int *nsteps, int *nrej, double *dxlast, int *work,
               const phc_track_params *prm, const phc_controller *ctlp, int waves, void *stream) {
+  return phc_track_from(tr, K, npaths, start, KS, nullptr, endp, tend, status, nsteps, nrej,
+                        dxlast, work, prm, ctlp, waves, stream);
+}
+
+int phc_track_from(phc_tracker *tr, int K, int npaths, const double *start, int KS,
+                   const double *tstart, double *endp, double *tend, int *status, int *nsteps,
+                   int *nrej, double *dxlast, int *work, const phc_track_params *prm,
+                   const phc_controller *ctlp, int waves, void *stream) {
   PHC_REQUIRE(tr && prm, "phc_track: null argument");
   PHC_REQUIRE(K == 1 || K == 2, "phc_track: tracking precision K must be 1 or 2");
   PHC_REQUIRE(KS >= 1 && KS <= 8, "phc_track: start limbs KS must be in 1..8");
@@ -1768,7 +1874,7 @@ int phc_track(phc_tracker *tr, int K, int npaths, const double *start, int KS, d
     PHC_HIP(hipFuncSetAttribute((const void *)track_kernel<KK, MW, PD>,                      \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));       \
     hipLaunchKernelGGL((track_kernel<KK, MW, PD>), dim3(grid), dim3(64 * wv), lds, st, tr->P, \
-                       dpar, npaths, start, KS, endp, tend, status, nsteps, nrej, dxlast, work, \
+                       dpar, npaths, start, KS, tstart, endp, tend, status, nsteps, nrej, dxlast, work, \
                        dctl, tr->counter);                                                    \
   } while (0)
   if (par.pden == 0) {
@@ -1812,6 +1918,34 @@ int phc_refine(phc_tracker *tr, int K, int npaths, double t, const double *xin, 
 #define RK(KK) hipLaunchKernelGGL(refine_kernel<KK>, dim3(grid), dim3(64 * wv), lds, st, tr->P, npaths, t, xin, KS, xout, iters, tol, dxlast, res, status, only, tr->counter)
   if (K == 1) RK(1); else if (K == 2) RK(2); else RK(4);
 #undef RK
+  PHC_HIP(hipGetLastError());
+  return tracker_leave(tr, stream);
+}
+
+int phc_endgame(phc_tracker *tr, int K, int npaths, const double *x, int KS, const double *tend,
+                double *alpha, double *rho, double *rco, double *xmax, int waves, void *stream) {
+  PHC_REQUIRE(tr, "phc_endgame: null tracker");
+  PHC_REQUIRE(K == 1 || K == 2, "phc_endgame: K must be 1 or 2");
+  PHC_REQUIRE(KS >= 1 && KS <= 8, "phc_endgame: KS must be in 1..8");
+  PHC_REQUIRE(npaths >= 0, "phc_endgame: npaths < 0");
+  PHC_REQUIRE(waves >= 1 && waves <= REFINE_MAX_WAVES, "phc_endgame: waves per workgroup must be in 1..8");
+  PHC_REQUIRE(tr->P.D >= 4, "phc_endgame: the ratio method needs series degree >= 3");
+  if (npaths == 0) return 0;
+  PHC_REQUIRE(x && tend && alpha && rho && rco && xmax, "phc_endgame: null array");
+  hipStream_t st = (hipStream_t)stream;
+  int grid = 0, wv = 0;
+  size_t lds = 0;
+  if (geometry(tr->P, K, waves, REFINE_MAX_WAVES, npaths, &grid, &wv, &lds) != 0) return -2;
+  if (tracker_enter(tr, stream) != 0) return -1;
+  PHC_HIP(hipMemsetAsync(tr->counter, 0, sizeof(int), st));
+  const void *fn = K == 1 ? (const void *)endgame_kernel<1> : (const void *)endgame_kernel<2>;
+  PHC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (K == 1)
+    hipLaunchKernelGGL(endgame_kernel<1>, dim3(grid), dim3(64 * wv), lds, st, tr->P, npaths, x, KS,
+                       tend, alpha, rho, rco, xmax, tr->counter);
+  else
+    hipLaunchKernelGGL(endgame_kernel<2>, dim3(grid), dim3(64 * wv), lds, st, tr->P, npaths, x, KS,
+                       tend, alpha, rho, rco, xmax, tr->counter);
   PHC_HIP(hipGetLastError());
   return tracker_leave(tr, stream);
 }

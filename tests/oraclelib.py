@@ -84,10 +84,10 @@ def _track_sigs():
     from phcpack_amd._lib import PhcController, PhcProgram, PhcTrackParams
     o = oracle()
     o.otr_track.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_int, C.c_int, C.c_int, P_d, C.c_int,
-                            P_d, P_d, P_i, P_i, P_i, P_d, P_i, C.POINTER(PhcTrackParams),
+                            P_d, P_d, P_d, P_i, P_i, P_i, P_d, P_i, C.POINTER(PhcTrackParams),
                             C.POINTER(PhcController)]
     o.otr_track_mt.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_int, P_d, C.c_int, P_d, P_d,
-                               P_i, P_i, P_i, P_d, P_i, C.POINTER(PhcTrackParams),
+                               P_d, P_i, P_i, P_i, P_d, P_i, C.POINTER(PhcTrackParams),
                                C.POINTER(PhcController), C.c_int]
     o.otr_lg2.argtypes = [C.c_double]
     o.otr_lg2.restype = C.c_double
@@ -96,6 +96,8 @@ def _track_sigs():
     o.otr_eval.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_double, P_d, P_d, P_d]
     o.otr_series.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_double, P_d, P_d]
     o.otr_series.restype = C.c_int
+    o.otr_endgame.argtypes = [C.POINTER(PhcProgram), C.c_int, C.c_int, P_d, C.c_int, P_d, P_d,
+                              P_d, P_d, P_d]
     o.otr_pade_coefficients.argtypes = [C.c_int, P_d, C.c_int, C.c_int, P_d]
     o.otr_pade_error.argtypes = [C.c_int, P_d, C.c_int, C.c_int, P_d, P_d]
     o.otr_root.argtypes = [C.c_double, C.c_int]
@@ -149,9 +151,11 @@ class OracleController:
         return self.x[keep], self.c[keep], self.y[keep]
 
 
-def oracle_track(prog, starts_planes, K, params, nthreads=1, paths=None, controller=None):
+def oracle_track(prog, starts_planes, K, params, nthreads=1, paths=None, controller=None,
+                 tstart=None):
     """Track with the C oracle; starts_planes [2KS][npaths][n];
-    controller: an OracleController or None."""
+    controller: an OracleController or None; tstart [npaths] the start
+    parameter of every path (None: 0)."""
     import ctypes as C
     from phcpack_amd.tracker import program_struct
     o = _track_sigs()
@@ -164,15 +168,17 @@ def oracle_track(prog, starts_planes, K, params, nthreads=1, paths=None, control
                work=np.zeros((8, npaths), np.int32))
     p = params.to_c()
     cp = C.byref(controller.struct) if controller is not None else None
+    ts = None if tstart is None else np.ascontiguousarray(tstart, dtype=np.float64)
+    tp = pd(ts) if ts is not None else None
     if nthreads > 1:
-        o.otr_track_mt(C.byref(ps), K, npaths, pd(st), KS, pd(out["endp"]), pd(out["tend"]),
+        o.otr_track_mt(C.byref(ps), K, npaths, pd(st), KS, tp, pd(out["endp"]), pd(out["tend"]),
                        pi(out["status"]), pi(out["nsteps"]), pi(out["nrej"]), pd(out["dx"]),
                        pi(out["work"]), C.byref(p), cp, nthreads)
     else:
         p0, p1 = (0, npaths) if paths is None else paths
-        o.otr_track(C.byref(ps), K, npaths, p0, p1, pd(st), KS, pd(out["endp"]), pd(out["tend"]),
-                    pi(out["status"]), pi(out["nsteps"]), pi(out["nrej"]), pd(out["dx"]),
-                    pi(out["work"]), C.byref(p), cp)
+        o.otr_track(C.byref(ps), K, npaths, p0, p1, pd(st), KS, tp, pd(out["endp"]),
+                    pd(out["tend"]), pi(out["status"]), pi(out["nsteps"]), pi(out["nrej"]),
+                    pd(out["dx"]), pi(out["work"]), C.byref(p), cp)
     return out
 
 
@@ -288,6 +294,21 @@ def oracle_eval(prog, x_planes, K, t):
     val, jac = np.zeros((2 * K, n)), np.zeros((2 * K, n * n))
     o.otr_eval(C.byref(ps), K, t, pd(x), pd(val), pd(jac))
     return val, jac
+
+
+def oracle_endgame(prog, planes, K, tend):
+    """alpha, rho, rco, xmax [npaths] of otr_endgame at the points [2KS][npaths][n]"""
+    import ctypes as C
+    from phcpack_amd.tracker import program_struct
+    o = _track_sigs()
+    ps = program_struct(prog)
+    x = np.ascontiguousarray(planes, dtype=np.float64)
+    KS, npaths = x.shape[0] // 2, x.shape[1]
+    t = np.ascontiguousarray(tend, dtype=np.float64)
+    out = {k: np.zeros(npaths) for k in ("alpha", "rho", "rco", "xmax")}
+    o.otr_endgame(C.byref(ps), K, npaths, pd(x), KS, pd(t), pd(out["alpha"]), pd(out["rho"]),
+                  pd(out["rco"]), pd(out["xmax"]))
+    return out
 
 
 def oracle_series(prog, x_planes, K, t):

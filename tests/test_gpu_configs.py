@@ -59,7 +59,7 @@ def test_c3_all_endpoints_by_properties():
     prm = TrackParams(tolend=1e-13)
     g = tr.track(sols, K=1, params=prm)
     starts = points_to_planes(sols)
-    idx, esc = escalate_to_dd(tr, starts, g["status"], g["tend"], prm)
+    idx, esc = escalate_to_dd(tr, starts, g["status"], g["tend"], prm, endp=g["endp"])
     print(f"  FP64 statuses {np.bincount(g['status'], minlength=6).tolist()}, "
           f"escalated to dd tracking: {idx.tolist()}", flush=True)
     ok = g["status"] == 1

@@ -277,3 +277,21 @@ def test_mixed_endpoints_checked_by_octo_double_oracle():
           flush=True)
     assert res_od.max() <= 1e-24
     assert np.abs(res_qd - res_od).max() <= 1e-40
+
+
+@pytest.mark.parametrize("K", [1, 2])
+def test_track_from_start_parameters_matches_oracle(K):
+    """phc_track_from: paths continued from the last points of a run stopped
+    after 6 steps, each at its own tend, bit-identical to the oracle"""
+    from phcpack_amd.tracker import Tracker, TrackParams
+    p, hom, sols = make(3, 2, 11)
+    tr = Tracker(hom, deg=8)
+    print(f"track from tstart K={K}", flush=True)
+    part = tr.track(sols, K=1, params=TrackParams(tolend=1e-13, maxsteps=6))
+    assert np.all(part["status"] == 3)
+    prm = TrackParams(tolend=1e-13 if K == 1 else 1e-28)
+    g = tr.track(part["endp"], K=K, params=prm, tstart=part["tend"])
+    o = oracle_track(tr.prog, part["endp"], K, prm, tstart=part["tend"])
+    for key in ("status", "nsteps", "nrej", "tend", "endp", "work", "dx"):
+        assert np.array_equal(g[key], o[key]), key
+    assert np.all(g["status"] == 1)
