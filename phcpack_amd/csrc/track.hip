@@ -49,6 +49,17 @@
 
 using namespace md;
 
+// pointers into LDS: address space 3, so that helpers the compiler does not
+// inline still reach LDS with DS instructions (a generic pointer would make
+// them FLAT accesses).  The host pass of hipcc only parses the device code
+// and has no such address space, so the qualifier is given to the device
+// pass alone.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define LDS __attribute__((address_space(3)))
+#else
+#define LDS
+#endif
+
 namespace {
 
 enum { ST_RUNNING = 0, ST_SUCCESS = 1, ST_MINSTEP = 2, ST_MAXSTEPS = 3, ST_SINGULAR = 4,
@@ -179,17 +190,17 @@ template <int K> struct PathLayout {
 // the program as seen from LDS
 struct LProg {
   int n, D, ncomp, wdeg, G, nprod, nlev, npair, nterm, njpair;
-  const int *nodes, *lev, *pairs, *seg, *tpair, *jpairs, *jent, *jterms;
-  const double2 *tcoef;
+  const LDS int *nodes, *lev, *pairs, *seg, *tpair, *jpairs, *jent, *jterms;
+  const LDS double2 *tcoef;
   const double *weights;  // global
   const double *hsig;     // global
   const int *hent, *hterms, *hpairs;  // global
 };
 
 template <int K> struct Path {
-  cplx<K> *slots, *u, *uj, *A, *rhs, *x0, *col, *W, *dnode;
-  int *perm;
-  float *hid;
+  LDS cplx<K> *slots, *u, *uj, *A, *rhs, *x0, *col, *W, *dnode;
+  LDS int *perm;
+  LDS float *hid;
 };
 
 // Complex operations of the tracker: FP64 tracking (K = 1) forms products
@@ -272,7 +283,7 @@ template <int K> __device__ __forceinline__ cplx<K> load_weight(const LProg &P, 
 }
 
 // Taylor coefficients W[r][l] at t of w_r(t) = om0 + om1 t + om2 t^2
-template <int K> __device__ void set_weights(const LProg &P, Path<K> &w, double t) {
+template <int K> __device__ void set_weights(const LProg &__restrict__ P, Path<K> &__restrict__ w, double t) {
   typedef TC<K> Cx;
   const int l = lane_id();
   if (l < P.ncomp) {
@@ -287,11 +298,11 @@ template <int K> __device__ void set_weights(const LProg &P, Path<K> &w, double 
 }
 
 // node_k = sum_q left_q * right_{k-q} for every product node, level by level
-template <int K> __device__ void product_pass(const LProg &P, Path<K> &w, int k) {
+template <int K> __device__ void product_pass(const LProg &__restrict__ P, Path<K> &__restrict__ w, int k) {
   typedef TC<K> Cx;
   for (int lv = 1; lv <= P.nlev; ++lv) {
     for (int i = P.lev[lv - 1] + lane_id(); i < P.lev[lv]; i += 64) {
-      const int *nd = P.nodes + 8 * i;
+      const LDS int *nd = P.nodes + 8 * i;
       const int aoff = nd[0], boff = nd[2], soff = nd[3], smask = nd[4];
       // terms q = 0..k alternate between two accumulators (even, odd q)
       cplx<K> a0 = Cx::mul(w.slots[aoff], w.slots[boff + k]);
@@ -313,18 +324,18 @@ template <int K> __device__ void product_pass(const LProg &P, Path<K> &w, int k)
 // levels: a node's corrected order-k value and its order-(k+1) product read
 // only factors of lower levels (already final for both orders) and write
 // distinct slots (ring storage keeps at least two orders when deg >= 1)
-template <int K> __device__ void correct_and_product(const LProg &P, Path<K> &w, int k) {
+template <int K> __device__ void correct_and_product(const LProg &__restrict__ P, Path<K> &__restrict__ w, int k) {
   typedef TC<K> Cx;
   const int k1 = k + 1;
   for (int lv = 1; lv <= P.nlev; ++lv) {
     for (int i = P.lev[lv - 1] + lane_id(); i < P.lev[lv]; i += 64) {
-      const int *nd = P.nodes + 8 * i;
+      const LDS int *nd = P.nodes + 8 * i;
       const int aoff = nd[0], boff = nd[2], soff = nd[3], smask = nd[4], adn = nd[5], sdn = nd[6];
       const cplx<K> da = (adn < 0) ? w.slots[aoff + k] : w.dnode[adn];
       cplx<K> d = Cx::mul(w.slots[aoff], w.slots[boff + k]);
       Cx::mac(d, da, w.slots[boff]);
       if (sdn >= 0) w.dnode[sdn] = d;
-      cplx<K> &v = w.slots[soff + (k & smask)];
+      LDS cplx<K> &v = w.slots[soff + (k & smask)];
       v = Cx::add(v, d);
       cplx<K> a0 = Cx::mul(w.slots[aoff], w.slots[boff + k1]);
       cplx<K> a1 = Cx::mul(w.slots[aoff + 1], w.slots[boff + k1 - 1]);
@@ -339,11 +350,11 @@ template <int K> __device__ void correct_and_product(const LProg &P, Path<K> &w,
 }
 
 // rhs_i = -[H_i]_k from the weighted component values at order k
-template <int K> __device__ void residual(const LProg &P, Path<K> &w, int k) {
+template <int K> __device__ void residual(const LProg &__restrict__ P, Path<K> &__restrict__ w, int k) {
   typedef TC<K> Cx;
   const int l = lane_id();
   for (int q = l; q < P.npair; q += 64) {
-    const int *pr = P.pairs + 4 * q;
+    const LDS int *pr = P.pairs + 4 * q;
     const int r = pr[0], off = pr[1], mask = pr[2];
     cplx<K> acc = Cx::mul(w.W[3 * r], w.slots[off + (k & mask)]);
     for (int j = 1; j <= P.wdeg && j <= k; ++j)
@@ -374,11 +385,11 @@ template <int K> __device__ void residual(const LProg &P, Path<K> &w, int k) {
 }
 
 // Jacobian A_ij = sum coef * e * W_r0 * node(m/x_j)_0
-template <int K> __device__ void jacobian(const LProg &P, Path<K> &w) {
+template <int K> __device__ void jacobian(const LProg &__restrict__ P, Path<K> &__restrict__ w) {
   typedef TC<K> Cx;
   const int l = lane_id();
   for (int q = l; q < P.njpair; q += 64) {
-    const int *jp = P.jpairs + 2 * q;
+    const LDS int *jp = P.jpairs + 2 * q;
     w.uj[q] = Cx::mul(w.W[3 * jp[0]], w.slots[jp[1]]);
   }
   wsync();
@@ -409,7 +420,7 @@ template <int K> __device__ void jacobian(const LProg &P, Path<K> &w) {
 // Returns false on a zero or non-finite pivot.  Any n: the row swap, the
 // pivot column, the scaled pivot row and the elimination are separate
 // phases.
-template <int K> __device__ bool gj_invert_wide(const LProg &P, Path<K> &w) {
+template <int K> __device__ bool gj_invert_wide(const LProg &__restrict__ P, Path<K> &__restrict__ w) {
   typedef TC<K> Cx;
   const int n = P.n, l = lane_id();
   if (l < n) w.perm[l] = l;
@@ -463,7 +474,7 @@ template <int K> __device__ bool gj_invert_wide(const LProg &P, Path<K> &w) {
 // entries of the swapped, scaled and eliminated matrix in registers.  Each
 // entry gets the operations of the phased version in the same order, so
 // the result is the same bit for bit.  The matrix is stored once at the end.
-template <int K> __device__ bool gj_invert(const LProg &P, Path<K> &w) {
+template <int K> __device__ bool gj_invert(const LProg &__restrict__ P, Path<K> &__restrict__ w) {
   typedef TC<K> Cx;
   const int n = P.n, l = lane_id();
   // quad-double entries would take 64 registers per lane: phased version
@@ -542,7 +553,7 @@ template <int K> __device__ bool gj_invert(const LProg &P, Path<K> &w) {
 // y = (P A)^{-1} (P rhs): lanes i*G+g sum columns jlo..jhi of row i, then
 // the G partial sums meet in a butterfly; row i's result is in the lanes of
 // group i.
-template <int K> __device__ cplx<K> apply_inverse(const LProg &P, Path<K> &w) {
+template <int K> __device__ cplx<K> apply_inverse(const LProg &__restrict__ P, Path<K> &__restrict__ w) {
   typedef TC<K> Cx;
   const int n = P.n, G = P.G, l = lane_id();
   cplx<K> acc = Cx::zero();
@@ -554,18 +565,18 @@ template <int K> __device__ cplx<K> apply_inverse(const LProg &P, Path<K> &w) {
   return group_sum<K>(acc, G);
 }
 
-template <int K> __device__ void eval_point(const LProg &P, Path<K> &w, bool jac) {
+template <int K> __device__ void eval_point(const LProg &__restrict__ P, Path<K> &__restrict__ w, bool jac) {
   product_pass<K>(P, w, 0);
   residual<K>(P, w, 0);
   if (jac) jacobian<K>(P, w);
 }
 
-template <int K> __device__ double max_abs_rhs(const LProg &P, Path<K> &w) {
+template <int K> __device__ double max_abs_rhs(const LProg &__restrict__ P, Path<K> &__restrict__ w) {
   const int l = lane_id();
   return wave_max(l < P.n ? absz<K>(w.rhs[l]) : 0.0);
 }
 
-template <int K> __device__ double max_abs_x(const LProg &P, Path<K> &w) {
+template <int K> __device__ double max_abs_x(const LProg &__restrict__ P, Path<K> &__restrict__ w) {
   const int l = lane_id();
   return wave_max(l < P.n ? absz<K>(w.slots[l * P.D]) : 0.0);
 }
@@ -576,7 +587,7 @@ template <int K> __device__ double max_abs_x(const LProg &P, Path<K> &w) {
 // already in place (the corrector's predicted-residual check), so the first
 // iteration only forms the Jacobian.
 template <int K>
-__device__ int newton(const LProg &P, Path<K> &w, int iters, double tol, double *dxl, int *count,
+__device__ int newton(const LProg &__restrict__ P, Path<K> &__restrict__ w, int iters, double tol, double *dxl, int *count,
                       double *dxh, bool evaluated) {
   typedef TC<K> Cx;
   const int l = lane_id();
@@ -588,7 +599,7 @@ __device__ int newton(const LProg &P, Path<K> &w, int iters, double tol, double 
     const cplx<K> dx = apply_inverse<K>(P, w);
     double mdx = 0.0;
     if (l < P.n * P.G && (l % P.G) == 0) {
-      cplx<K> &x = w.slots[(l / P.G) * P.D];
+      LDS cplx<K> &x = w.slots[(l / P.G) * P.D];
       x = Cx::add(x, dx);
       mdx = absz<K>(dx);
     }
@@ -675,7 +686,7 @@ __device__ __forceinline__ void upper_ij(int n, int q, int &i, int &j) {
 }
 
 template <typename MF>
-__device__ double sigma_max_sq(int n, MF M, double2 *P, double2 *Q, double2 *Y) {
+__device__ double sigma_max_sq(int n, MF M, LDS double2 *P, LDS double2 *Q, LDS double2 *Y) {
   const int l = lane_id(), nu = n * (n + 1) / 2;
   // Hermitian: entries i <= j computed, i > j their conjugates
   for (int q = l; q < nu; q += 64) {
@@ -766,16 +777,16 @@ __device__ double sigma_max_sq(int n, MF M, double2 *P, double2 *Q, double2 *Y) 
 // the moduli of its coefficients (the components of a monomial added
 // first) at the moduli of x (standard_mixed_residuals.adb:472-484), in FP64
 // on the leading limbs; node moduli go to the scratch, equation values to col
-template <int K> __device__ double mixed_residual(const LProg &P, Path<K> &w) {
+template <int K> __device__ double mixed_residual(const LProg &__restrict__ P, Path<K> &__restrict__ w) {
   const int l = lane_id(), n = P.n, D = P.D, G = P.G;
-  double *rn = reinterpret_cast<double *>(w.u);
-  double *mr = reinterpret_cast<double *>(w.col);
+  LDS double *rn = reinterpret_cast<LDS double *>(w.u);
+  LDS double *mr = reinterpret_cast<LDS double *>(w.col);
   for (int j = l; j <= n; j += 64)
     rn[j] = (j < n) ? zrad(R<K>::hi(w.slots[j * D].re), R<K>::hi(w.slots[j * D].im)) : 1.0;
   wsync();
   for (int lv = 1; lv <= P.nlev; ++lv) {
     for (int i = P.lev[lv - 1] + l; i < P.lev[lv]; i += 64) {
-      const int *nd = P.nodes + 8 * i;
+      const LDS int *nd = P.nodes + 8 * i;
       rn[n + 1 + i] = rn[nd[7]] * rn[nd[2] / D];
     }
     wsync();
@@ -814,9 +825,9 @@ template <int K> __device__ double mixed_residual(const LProg &P, Path<K> &w) {
 // evaluation (every node at order 0): the program's constant for equations
 // of degree <= 2, the Hessian from its terms otherwise (Distance,
 // standard_predictor_convolutions.adb:558-572, with Cached_Singular_Values)
-template <int K> __device__ double hessian_sum(const LProg &P, Path<K> &w) {
+template <int K> __device__ double hessian_sum(const LProg &__restrict__ P, Path<K> &__restrict__ w) {
   const int l = lane_id(), n = P.n, nn = n * (n + 1) / 2;
-  double2 *H = reinterpret_cast<double2 *>(w.u), *SP = H + n * n, *SQ = SP + n * n;
+  LDS double2 *H = reinterpret_cast<LDS double2 *>(w.u), *SP = H + n * n, *SQ = SP + n * n;
   double acc = 0.0;
   for (int i = 0; i < n; ++i) {
     double s = P.hsig[i];
@@ -839,7 +850,7 @@ template <int K> __device__ double hessian_sum(const LProg &P, Path<K> &w) {
       }
       wsync();
       s = sqrt(sigma_max_sq(n, [&](int a, int b) { return H[a * n + b]; }, SP, SQ,
-                            reinterpret_cast<double2 *>(w.col)));
+                            reinterpret_cast<LDS double2 *>(w.col)));
     }
     acc = acc + s * s;
   }
@@ -853,7 +864,7 @@ template <int K> __device__ double hessian_sum(const LProg &P, Path<K> &w) {
 // not, -1 singular.  evaluated: the products and the residual at the point
 // are in place, so the first iteration only forms the Jacobian.
 template <int K>
-__device__ int lu_newton_steps(const LProg &P, Path<K> &w, int maxit, double tol, bool evaluated,
+__device__ int lu_newton_steps(const LProg &__restrict__ P, Path<K> &__restrict__ w, int maxit, double tol, bool evaluated,
                                double *dxl, int *count, int *nfirst, double *dxh) {
   typedef TC<K> Cx;
   const int l = lane_id();
@@ -871,7 +882,7 @@ __device__ int lu_newton_steps(const LProg &P, Path<K> &w, int maxit, double tol
     const cplx<K> dx = apply_inverse<K>(P, w);
     double mdx = 0.0;
     if (l < P.n * P.G && (l % P.G) == 0) {
-      cplx<K> &x = w.slots[(l / P.G) * P.D];
+      LDS cplx<K> &x = w.slots[(l / P.G) * P.D];
       x = Cx::add(x, dx);
       mdx = absz<K>(dx);
     }
@@ -918,15 +929,15 @@ __device__ LProg lprog(const DevProgram &G, const ProgLayout &L, unsigned char *
   LProg P;
   P.n = G.n; P.D = G.D; P.ncomp = G.ncomp; P.wdeg = G.wdeg; P.G = G.G; P.nprod = G.nprod;
   P.nlev = G.nlev; P.npair = G.npair; P.nterm = G.nterm; P.njpair = G.njpair;
-  P.nodes = (const int *)(lds + L.nodes);
-  P.lev = (const int *)(lds + L.lev);
-  P.pairs = (const int *)(lds + L.pairs);
-  P.seg = (const int *)(lds + L.seg);
-  P.tpair = (const int *)(lds + L.tpair);
-  P.tcoef = (const double2 *)(lds + L.tcoef);
-  P.jpairs = (const int *)(lds + L.jpairs);
-  P.jent = (const int *)(lds + L.jent);
-  P.jterms = (const int *)(lds + L.jterms);
+  P.nodes = (const LDS int *)(lds + L.nodes);
+  P.lev = (const LDS int *)(lds + L.lev);
+  P.pairs = (const LDS int *)(lds + L.pairs);
+  P.seg = (const LDS int *)(lds + L.seg);
+  P.tpair = (const LDS int *)(lds + L.tpair);
+  P.tcoef = (const LDS double2 *)(lds + L.tcoef);
+  P.jpairs = (const LDS int *)(lds + L.jpairs);
+  P.jent = (const LDS int *)(lds + L.jent);
+  P.jterms = (const LDS int *)(lds + L.jterms);
   P.weights = G.weights;
   P.hsig = G.hsig;
   P.hent = G.hent_ptr;
@@ -937,22 +948,22 @@ __device__ LProg lprog(const DevProgram &G, const ProgLayout &L, unsigned char *
 
 template <int K> __device__ Path<K> carve(const PathLayout<K> &L, unsigned char *base) {
   Path<K> w;
-  w.slots = (cplx<K> *)(base + L.slots);
-  w.u = (cplx<K> *)(base + L.u);
-  w.uj = (cplx<K> *)(base + L.uj);
-  w.A = (cplx<K> *)(base + L.A);
-  w.rhs = (cplx<K> *)(base + L.rhs);
-  w.x0 = (cplx<K> *)(base + L.x0);
-  w.col = (cplx<K> *)(base + L.col);
-  w.W = (cplx<K> *)(base + L.W);
-  w.dnode = (cplx<K> *)(base + L.dnode);
-  w.perm = (int *)(base + L.perm);
-  w.hid = (float *)(base + L.hid);
+  w.slots = (LDS cplx<K> *)(base + L.slots);
+  w.u = (LDS cplx<K> *)(base + L.u);
+  w.uj = (LDS cplx<K> *)(base + L.uj);
+  w.A = (LDS cplx<K> *)(base + L.A);
+  w.rhs = (LDS cplx<K> *)(base + L.rhs);
+  w.x0 = (LDS cplx<K> *)(base + L.x0);
+  w.col = (LDS cplx<K> *)(base + L.col);
+  w.W = (LDS cplx<K> *)(base + L.W);
+  w.dnode = (LDS cplx<K> *)(base + L.dnode);
+  w.perm = (LDS int *)(base + L.perm);
+  w.hid = (LDS float *)(base + L.hid);
   return w;
 }
 
 template <int K>
-__device__ void load_point(const LProg &P, Path<K> &w, const double *x, int KS, int npaths, int path) {
+__device__ void load_point(const LProg &__restrict__ P, Path<K> &__restrict__ w, const double *x, int KS, int npaths, int path) {
   const int l = lane_id(), n = P.n;
   if (l < n) {
     cplx<K> z = C<K>::zero();
@@ -969,7 +980,7 @@ __device__ void load_point(const LProg &P, Path<K> &w, const double *x, int KS, 
 }
 
 template <int K>
-__device__ void store_point(const LProg &P, Path<K> &w, double *x, int npaths, int path) {
+__device__ void store_point(const LProg &__restrict__ P, Path<K> &__restrict__ w, double *x, int npaths, int path) {
   const int l = lane_id(), n = P.n;
   if (l < n) {
     const cplx<K> z = w.slots[l * P.D];
@@ -985,7 +996,7 @@ __device__ void store_point(const LProg &P, Path<K> &w, double *x, int npaths, i
 // Taylor coefficients x_1..x_deg at the point in the order-0 slots (weights
 // set at t0) and *hs2, the Hessian sum at the point; returns false on a
 // singular Jacobian, else A holds its inverse
-template <int K> __device__ bool predictor_series(const LProg &P, Path<K> &w, double *hs2) {
+template <int K> __device__ bool predictor_series(const LProg &__restrict__ P, Path<K> &__restrict__ w, double *hs2) {
   const int l = lane_id(), n = P.n, D = P.D;
   eval_point<K>(P, w, true);
   *hs2 = hessian_sum<K>(P, w);
@@ -1008,7 +1019,7 @@ template <int K> __device__ bool predictor_series(const LProg &P, Path<K> &w, do
 
 // features of the current step (identical in every lane)
 template <int K>
-__device__ void step_features(const LProg &P, Path<K> &w, double rad, double maxstep, double hprev,
+__device__ void step_features(const LProg &__restrict__ P, Path<K> &__restrict__ w, double rad, double maxstep, double hprev,
                               double t, const double *dxh, int rejprev, double *f) {
   const int l = lane_id(), n = P.n, D = P.D;
   f[0] = lg2(rad);
@@ -1060,7 +1071,7 @@ template <int K> __device__ int mlp_choose(const Ctl &ctl, Path<K> &w, const dou
 // numerator c_i + d_1 c_{i-1} + d_2 c_{i-2}; d_1, d_2 take the places of
 // c_{L+1}, c_{L+2}.  A zero pivot leaves the denominator 1 and a zero
 // numerator, or the Taylor numerator when b is zero.
-template <int K> __device__ __forceinline__ void pade2_coefficients(cplx<K> *c, int L, cplx<K> *x0) {
+template <int K> __device__ __forceinline__ void pade2_coefficients(LDS cplx<K> *c, int L, LDS cplx<K> *x0) {
   typedef TC<K> Cx;
   cplx<K> a00 = (L >= 2) ? c[L - 1] : Cx::zero();
   cplx<K> a01 = c[L], a10 = c[L], a11 = c[L + 1];
@@ -1109,7 +1120,7 @@ template <int K> __device__ __forceinline__ void pade2_coefficients(cplx<K> *c, 
 // x0), est = c_top + sum_{i<=min(PD,top)} d_i c_{top-i}, est = num_top - est
 // when top <= L, est = 1 when top < 0 or |est| < 1e-5 (leading limbs)
 template <int K, int PD>
-__device__ double2 pade_with_error(cplx<K> *c, int L, cplx<K> *x0) {
+__device__ double2 pade_with_error(LDS cplx<K> *c, int L, LDS cplx<K> *x0) {
   typedef TC<K> Cx;
   int top = -1;
   for (int i = L + PD + 2; i >= 0; --i) {
@@ -1154,7 +1165,7 @@ __device__ double2 pade_with_error(cplx<K> *c, int L, cplx<K> *x0) {
 // the predicted component at step s: the numerator (Taylor polynomial of
 // degree L when PD = 0) by Horner with x0 added last, over 1 + d_1 s + ...
 template <int K, int PD>
-__device__ cplx<K> pade_value(const cplx<K> *c, int L, const cplx<K> &x0, double s) {
+__device__ cplx<K> pade_value(const LDS cplx<K> *c, int L, const cplx<K> x0, double s) {
   typedef TC<K> Cx;
   cplx<K> acc = c[L];
   for (int k = L - 1; k >= 1; --k) acc = Cx::add(Cx::scale_d(acc, s), c[k]);
@@ -1198,7 +1209,7 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, const Para
   const LProg P = lprog(G, PL, lds);
   Path<K> w = carve<K>(WL, lds + PL.total + (threadIdx.x >> 6) * WL.total);
   const int l = lane_id(), n = P.n, D = P.D;
-  double2 *mr = reinterpret_cast<double2 *>(w.col);
+  LDS double2 *mr = reinterpret_cast<LDS double2 *>(w.col);
   for (;;) {
     int path = 0;
     if (l == 0) path = atomicAdd(counter, 1);
@@ -1281,7 +1292,7 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, const Para
       {
         const double lam = sigma_max_sq(
             n, [&](int a, int b) { return lead<K>(w.A[a * n + b]); },
-            reinterpret_cast<double2 *>(w.u), reinterpret_cast<double2 *>(w.u) + n * n, mr);
+            reinterpret_cast<LDS double2 *>(w.u), reinterpret_cast<LDS double2 *>(w.u) + n * n, mr);
         const double sig1 = 1.0 / sqrt(lam);
         const double eta = (2.0 * sig1) / sqrt(hs2);
         const double curv = kth_root((par.cbeta * eta) / nrm, D - 1);
@@ -1466,7 +1477,7 @@ __global__ void __launch_bounds__(64 * REFINE_MAX_WAVES) endgame_kernel(DevProgr
   const LProg P = lprog(G, PL, lds);
   Path<K> w = carve<K>(WL, lds + PL.total + (threadIdx.x >> 6) * WL.total);
   const int l = lane_id(), n = P.n, D = P.D, k = D - 2;
-  double2 *U = reinterpret_cast<double2 *>(w.u), *Y = reinterpret_cast<double2 *>(w.col);
+  LDS double2 *U = reinterpret_cast<LDS double2 *>(w.u), *Y = reinterpret_cast<LDS double2 *>(w.col);
   for (;;) {
     int path = 0;
     if (l == 0) path = atomicAdd(counter, 1);
