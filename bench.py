@@ -571,12 +571,36 @@ def md_lead_diff(lib, K, da, db, nelem):
     return float(max(np.abs(c[0]).max(), np.abs(c[K]).max()))
 
 
+def pad_limbs(d, K, K2):
+    """complex planar [2K][...] device array -> [2K2][...] with zero limbs"""
+    from phcpack_amd._lib import DeviceArray
+    h = d.to_host()
+    out = np.zeros((2 * K2,) + h.shape[1:])
+    out[:K], out[K2:K2 + K] = h[:K], h[K:]
+    return DeviceArray.from_host(out)
+
+
+def octo_double_checks(lib, K, n, dA, dx, db, dx0):
+    """max |A x - b| and max |x - x0| (leading limbs) evaluated in
+    octo-double on the GPU for the K-limb solve A x = b"""
+    from phcpack_amd._lib import DeviceArray, check
+    from phcpack_amd.baqr import gemm_device
+    A8, x8, b8, x08 = (pad_limbs(d, K, 8) for d in (dA, dx, db, dx0))
+    r8 = DeviceArray.zeros((16, n))
+    gemm_device(8, False, False, n, 1, n, (A8, 0, n, 1), (x8, 0, 1, 0), (r8, 0, 1, 0))
+    check(lib.phc_synchronize())
+    return {"max_abs_Ax_minus_b": md_lead_diff(lib, 8, r8, b8, n),
+            "max_abs_x_minus_x0": md_lead_diff(lib, 8, x8, x08, n),
+            "note": "A, x, b, x0 padded to eight limbs; r = A x by the octo-double complex GEMM"}
+
+
 def run_qr(args, n=1024, szt=128, K=4, seed=SEED):
     """Blocked Householder QR least-squares solve in complex quad-double:
     A (n x n, re and im uniform in [-1,1]) = QR, y = Q^H b, R x = y by the
     tiled back substitution, b = A x0 with x0 random; inputs resident on the
-    device.  Checks (outside the timed region, on the GPU in quad-double):
-    max |Q^H Q - I|, max |QR - A|, max |x - x0|."""
+    device.  Checks (outside the timed region, on the GPU): max |Q^H Q - I|,
+    max |QR - A|, max |x - x0| in quad-double, and the residual max |A x - b|
+    and max |x - x0| of the quad-double solution in octo-double."""
     from phcpack_amd._lib import DeviceArray, check, load
     from phcpack_amd.baqr import BlockedQR, gemm_device, upper_tiled_solver_device
     from phcpack_amd.flops import BUILD, baqr_canonical, baqr_work, complex_mac
@@ -624,6 +648,10 @@ def run_qr(args, n=1024, szt=128, K=4, seed=SEED):
     gemm_device(K, False, False, n, n, n, (dQ, 0, n, 1), (dR, 0, n, 1), (dQR, 0, n, 1))
     qra = md_lead_diff(lib, K, dQR, dA, n * n)
     xerr = md_lead_diff(lib, K, dx, dx0, n)
+    # the error of the quad-double solution in octo-double on the GPU: A, x
+    # and b padded with zero limbs, r = A x - b by the octo-double complex
+    # GEMM, and x - x0 (exact in octo-double)
+    od = octo_double_checks(lib, K, n, dA, dx, db, dx0)
     ms_qr, ms = float(np.mean(t_qr)), float(np.mean(t_all))
     executed = baqr_work(n, n, szt) * complex_mac(K)
     cm, ca = baqr_canonical(n)
@@ -636,8 +664,9 @@ def run_qr(args, n=1024, szt=128, K=4, seed=SEED):
         "config": {"workload": f"complex quad-double {n}x{n}, {n // szt} tiles of {szt} columns, "
                                "entries uniform in [-1,1], b = A x0", "seed": seed},
         "qr_ms": round(ms_qr, 3),
-        "checks": {"max_abs_QHQ_minus_I": qq, "max_abs_QR_minus_A": qra, "max_abs_x_minus_x0": xerr},
-        "roofline": {"bound": "mfma", "achieved": round(executed / (ms_qr / 1e3) / 1e12, 3),
+        "checks": {"max_abs_QHQ_minus_I": qq, "max_abs_QR_minus_A": qra, "max_abs_x_minus_x0": xerr,
+                   "octo_double": od},
+        "roofline": {"bound": "fp64-valu", "achieved": round(executed / (ms_qr / 1e3) / 1e12, 3),
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(executed / (ms_qr / 1e3) / 1e12 / FP64_PEAK_TFLOPS, 4),
                      "traffic": None, "kernel": "blocked QR (all kernels)",

@@ -461,7 +461,8 @@ int phc_cmplx_upper_tiled_solver_noalloc(int K, int dim, int szt, int nbt, doubl
                                          double *b_dev, double *x_dev, void *stream);
 
 /* C(i,j) = [C(i,j) +] sum_k op(A(i,k)) op(B(k,j)) for complex
- * multiple-double operands given by base pointer, plane stride and the two
+ * multiple-double operands (K = 2, 4, 8; K = 8 serves octo-double checks of
+ * quad-double results) given by base pointer, plane stride and the two
  * element strides (element (r,c) of A at A + a0*r + a1*c in every plane);
  * op conjugates when the flag is set.  The product kernel of the QR (the
  * trailing updates of dbl4_baqr_host.cpp:357,743), exposed for checks such as

@@ -261,6 +261,43 @@ void GPU_cmplx4_blocked_houseqr(
  * Matrices/dbl2_tabs_kernels.h:392 and dbl4_tabs_kernels.h:604: U x = b
  * for upper triangular U (dim = szt*nbt); U and b are left unchanged, x is
  * returned. */
+/* Matrices/dbl8_baqr_kernels.h:4882: the octo-double complex blocked QR. */
+void GPU_cmplx8_blocked_houseqr(
+    int nrows, int ncols, int szt, int nbt, double **Arehihihi, double **Arelohihi,
+    double **Arehilohi, double **Arelolohi, double **Arehihilo, double **Arelohilo,
+    double **Arehilolo, double **Arelololo, double **Aimhihihi, double **Aimlohihi,
+    double **Aimhilohi, double **Aimlolohi, double **Aimhihilo, double **Aimlohilo,
+    double **Aimhilolo, double **Aimlololo, double **Qrehihihi, double **Qrelohihi,
+    double **Qrehilohi, double **Qrelolohi, double **Qrehihilo, double **Qrelohilo,
+    double **Qrehilolo, double **Qrelololo, double **Qimhihihi, double **Qimlohihi,
+    double **Qimhilohi, double **Qimlolohi, double **Qimhihilo, double **Qimlohilo,
+    double **Qimhilolo, double **Qimlololo, double **Rrehihihi, double **Rrelohihi,
+    double **Rrehilohi, double **Rrelolohi, double **Rrehihilo, double **Rrelohilo,
+    double **Rrehilolo, double **Rrelololo, double **Rimhihihi, double **Rimlohihi,
+    double **Rimhilohi, double **Rimlolohi, double **Rimhihilo, double **Rimlohilo,
+    double **Rimhilolo, double **Rimlololo, double *houselapms, double *RHvlapms,
+    double *tileRlapms, double *vb2Wlapms, double *WYHlapms, double *QWYHlapms,
+    double *Qaddlapms, double *YWHlapms, double *YWHClapms, double *Raddlapms,
+    double *walltimesec, long long int *addcnt, long long int *mulcnt, long long int *divcnt,
+    long long int *sqrtcnt, bool verbose = true);
+
+/* Matrices/dbl8_tabs_kernels.h:1011: the octo-double complex tiled solver. */
+void GPU_cmplx8_upper_tiled_solver(
+    int dim, int szt, int nbt, double **Urehihihi, double **Urelohihi, double **Urehilohi,
+    double **Urelolohi, double **Urehihilo, double **Urelohilo, double **Urehilolo,
+    double **Urelololo, double **Uimhihihi, double **Uimlohihi, double **Uimhilohi,
+    double **Uimlolohi, double **Uimhihilo, double **Uimlohilo, double **Uimhilolo,
+    double **Uimlololo, double *brehihihi, double *brelohihi, double *brehilohi,
+    double *brelolohi, double *brehihilo, double *brelohilo, double *brehilolo,
+    double *brelololo, double *bimhihihi, double *bimlohihi, double *bimhilohi,
+    double *bimlolohi, double *bimhihilo, double *bimlohilo, double *bimhilolo,
+    double *bimlololo, double *xrehihihi, double *xrelohihi, double *xrehilohi,
+    double *xrelolohi, double *xrehihilo, double *xrelohilo, double *xrehilolo,
+    double *xrelololo, double *ximhihihi, double *ximlohihi, double *ximhilohi,
+    double *ximlolohi, double *ximhihilo, double *ximlohilo, double *ximhilolo,
+    double *ximlololo, double *invlapms, double *mullapms, double *sublapms, double *totlapms,
+    double *walltimesec, long long int *addcnt, long long int *mulcnt, long long int *divcnt);
+
 void GPU_cmplx2_upper_tiled_solver(int dim, int szt, int nbt, double **Urehi, double **Urelo,
                                    double **Uimhi, double **Uimlo, double *brehi, double *brelo,
                                    double *bimhi, double *bimlo, double *xrehi, double *xrelo,

@@ -693,7 +693,7 @@ extern "C" {
 
 int phc_baqr_create(phc_baqr **out, int K, int nrows, int ncols, int szt) {
   PHC_REQUIRE(out, "phc_baqr_create: null plan pointer");
-  PHC_REQUIRE(K == 2 || K == 4, "phc_baqr_create: K must be 2 or 4");
+  PHC_REQUIRE(K == 2 || K == 4 || K == 8, "phc_baqr_create: K must be 2, 4 or 8");
   PHC_REQUIRE(szt >= 1 && szt <= 128 && ncols >= szt && ncols % szt == 0,
               "phc_baqr_create: ncols must be a multiple of szt (1..128)");
   PHC_REQUIRE(nrows >= ncols, "phc_baqr_create: nrows must be at least ncols");
@@ -762,7 +762,9 @@ int phc_cmplx_blocked_houseqr(phc_baqr *p, const double *A, double *Q, double *R
     }
     hipGraph_t graph = nullptr;
     PHC_HIP(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
-    const int rc = (p->K == 2) ? run_qr<2>(p, A, Q, R, p->cap) : run_qr<4>(p, A, Q, R, p->cap);
+    const int rc = (p->K == 2)   ? run_qr<2>(p, A, Q, R, p->cap)
+                   : (p->K == 4) ? run_qr<4>(p, A, Q, R, p->cap)
+                                 : run_qr<8>(p, A, Q, R, p->cap);
     const hipError_t e = hipStreamEndCapture(p->cap, &graph);
     if (rc) {
       if (graph) hipGraphDestroy(graph);
@@ -785,6 +787,8 @@ int phc_cmplx_qhb(phc_baqr *p, const double *Q, const double *b, double *y, void
   const int n = p->nrows;
   hipStream_t st = (hipStream_t)stream;
   const Op qa{Q, (size_t)n * n, 1, n}, bb{b, (size_t)n, 1, 0};
+  if (p->K == 8)
+    return cgemm<8>(true, false, n, 1, n, qa, bb, y, (size_t)n, 1, 0, false, st, p->part, p->pcap);
   return p->K == 2 ? cgemm<2>(true, false, n, 1, n, qa, bb, y, (size_t)n, 1, 0, false, st, p->part,
                               p->pcap)
                    : cgemm<4>(true, false, n, 1, n, qa, bb, y, (size_t)n, 1, 0, false, st, p->part,
@@ -794,18 +798,19 @@ int phc_cmplx_qhb(phc_baqr *p, const double *Q, const double *b, double *y, void
 int phc_cmplx_gemm(int K, int conjA, int conjB, int M, int N, int KD, const double *A,
                    size_t aps, long a0, long a1, const double *B, size_t bps, long b0, long b1,
                    double *Cm, size_t cps, long c0, long c1, int accumulate, void *stream) {
-  PHC_REQUIRE(K == 2 || K == 4, "phc_cmplx_gemm: K must be 2 or 4");
+  PHC_REQUIRE(K == 2 || K == 4 || K == 8, "phc_cmplx_gemm: K must be 2, 4 or 8");
   PHC_REQUIRE(M >= 0 && N >= 0 && KD >= 0, "phc_cmplx_gemm: negative size");
   PHC_REQUIRE(A && B && Cm, "phc_cmplx_gemm: null array");
   hipStream_t st = (hipStream_t)stream;
   const Op opa{A, aps, a0, a1}, opb{B, bps, b0, b1};
-  return K == 2 ? cgemm<2>(conjA, conjB, M, N, KD, opa, opb, Cm, cps, c0, c1, accumulate, st)
-                : cgemm<4>(conjA, conjB, M, N, KD, opa, opb, Cm, cps, c0, c1, accumulate, st);
+  if (K == 2) return cgemm<2>(conjA, conjB, M, N, KD, opa, opb, Cm, cps, c0, c1, accumulate, st);
+  if (K == 4) return cgemm<4>(conjA, conjB, M, N, KD, opa, opb, Cm, cps, c0, c1, accumulate, st);
+  return cgemm<8>(conjA, conjB, M, N, KD, opa, opb, Cm, cps, c0, c1, accumulate, st);
 }
 
 int phc_cmplx_upper_tiled_solver(int K, int dim, int szt, int nbt, double *U, double *b,
                                  double *x, void *stream) {
-  PHC_REQUIRE(K == 2 || K == 4, "phc_cmplx_upper_tiled_solver: K must be 2 or 4");
+  PHC_REQUIRE(K == 2 || K == 4 || K == 8, "phc_cmplx_upper_tiled_solver: K must be 2, 4 or 8");
   PHC_REQUIRE(szt >= 1 && szt <= 256 && nbt >= 1 && dim == szt * nbt,
               "phc_cmplx_upper_tiled_solver: dim must equal szt*nbt, szt <= 256");
   PHC_REQUIRE(U && b && x, "phc_cmplx_upper_tiled_solver: null array");
@@ -813,8 +818,9 @@ int phc_cmplx_upper_tiled_solver(int K, int dim, int szt, int nbt, double *U, do
   double *inv = nullptr;
   const size_t ips = (size_t)nbt * szt * szt;
   PHC_HIP(hipMalloc(&inv, sizeof(double) * 2 * K * ips));
-  const int rc = K == 2 ? run_tiled<2>(dim, szt, nbt, U, b, x, inv, st)
-                        : run_tiled<4>(dim, szt, nbt, U, b, x, inv, st);
+  const int rc = K == 2   ? run_tiled<2>(dim, szt, nbt, U, b, x, inv, st)
+                 : K == 4 ? run_tiled<4>(dim, szt, nbt, U, b, x, inv, st)
+                          : run_tiled<8>(dim, szt, nbt, U, b, x, inv, st);
   const hipError_t e2 = hipStreamSynchronize(st);
   hipFree(inv);
   if (rc) return rc;

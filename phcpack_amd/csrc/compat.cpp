@@ -462,6 +462,36 @@ void GPU_cmplx4_blocked_houseqr(
              "GPU_cmplx4_blocked_houseqr");
 }
 
+void GPU_cmplx8_blocked_houseqr(
+    int nrows, int ncols, int szt, int nbt, double **Arehihihi, double **Arelohihi,
+    double **Arehilohi, double **Arelolohi, double **Arehihilo, double **Arelohilo,
+    double **Arehilolo, double **Arelololo, double **Aimhihihi, double **Aimlohihi,
+    double **Aimhilohi, double **Aimlolohi, double **Aimhihilo, double **Aimlohilo,
+    double **Aimhilolo, double **Aimlololo, double **Qrehihihi, double **Qrelohihi,
+    double **Qrehilohi, double **Qrelolohi, double **Qrehihilo, double **Qrelohilo,
+    double **Qrehilolo, double **Qrelololo, double **Qimhihihi, double **Qimlohihi,
+    double **Qimhilohi, double **Qimlolohi, double **Qimhihilo, double **Qimlohilo,
+    double **Qimhilolo, double **Qimlololo, double **Rrehihihi, double **Rrelohihi,
+    double **Rrehilohi, double **Rrelolohi, double **Rrehihilo, double **Rrelohilo,
+    double **Rrehilolo, double **Rrelololo, double **Rimhihihi, double **Rimlohihi,
+    double **Rimhilohi, double **Rimlolohi, double **Rimhihilo, double **Rimlohilo,
+    double **Rimhilolo, double **Rimlololo, double *, double *, double *, double *, double *,
+    double *, double *, double *, double *, double *, double *walltimesec,
+    long long int *addcnt, long long int *mulcnt, long long int *divcnt,
+    long long int *sqrtcnt, bool) {
+  double **A[] = {Arehihihi, Arelohihi, Arehilohi, Arelolohi, Arehihilo, Arelohilo,
+                  Arehilolo, Arelololo, Aimhihihi, Aimlohihi, Aimhilohi, Aimlolohi,
+                  Aimhihilo, Aimlohilo, Aimhilolo, Aimlololo};
+  double **Q[] = {Qrehihihi, Qrelohihi, Qrehilohi, Qrelolohi, Qrehihilo, Qrelohilo,
+                  Qrehilolo, Qrelololo, Qimhihihi, Qimlohihi, Qimhilohi, Qimlolohi,
+                  Qimhihilo, Qimlohilo, Qimhilolo, Qimlololo};
+  double **R[] = {Rrehihihi, Rrelohihi, Rrehilohi, Rrelolohi, Rrehihilo, Rrelohilo,
+                  Rrehilolo, Rrelololo, Rimhihihi, Rimlohihi, Rimhilohi, Rimlolohi,
+                  Rimhihilo, Rimlohilo, Rimhilolo, Rimlololo};
+  blocked_qr(8, nrows, ncols, szt, nbt, A, Q, R, walltimesec, addcnt, mulcnt, divcnt, sqrtcnt,
+             "GPU_cmplx8_blocked_houseqr");
+}
+
 // ------------------------------------------------------------ tiled solver
 
 void GPU_cmplx2_upper_tiled_solver(int dim, int szt, int nbt, double **Urehi, double **Urelo,
@@ -491,6 +521,34 @@ void GPU_cmplx4_upper_tiled_solver(
   double *x[] = {xrehihi, xrelohi, xrehilo, xrelolo, ximhihi, ximlohi, ximhilo, ximlolo};
   tiled_solver(4, dim, szt, nbt, U, b, x, totlapms, walltimesec, addcnt, mulcnt, divcnt,
                "GPU_cmplx4_upper_tiled_solver");
+}
+
+void GPU_cmplx8_upper_tiled_solver(
+    int dim, int szt, int nbt, double **Urehihihi, double **Urelohihi, double **Urehilohi,
+    double **Urelolohi, double **Urehihilo, double **Urelohilo, double **Urehilolo,
+    double **Urelololo, double **Uimhihihi, double **Uimlohihi, double **Uimhilohi,
+    double **Uimlolohi, double **Uimhihilo, double **Uimlohilo, double **Uimhilolo,
+    double **Uimlololo, double *brehihihi, double *brelohihi, double *brehilohi,
+    double *brelolohi, double *brehihilo, double *brelohilo, double *brehilolo,
+    double *brelololo, double *bimhihihi, double *bimlohihi, double *bimhilohi,
+    double *bimlolohi, double *bimhihilo, double *bimlohilo, double *bimhilolo,
+    double *bimlololo, double *xrehihihi, double *xrelohihi, double *xrehilohi,
+    double *xrelolohi, double *xrehihilo, double *xrelohilo, double *xrehilolo,
+    double *xrelololo, double *ximhihihi, double *ximlohihi, double *ximhilohi,
+    double *ximlolohi, double *ximhihilo, double *ximlohilo, double *ximhilolo,
+    double *ximlololo, double *, double *, double *, double *totlapms, double *walltimesec,
+    long long int *addcnt, long long int *mulcnt, long long int *divcnt) {
+  double **U[] = {Urehihihi, Urelohihi, Urehilohi, Urelolohi, Urehihilo, Urelohilo,
+                  Urehilolo, Urelololo, Uimhihihi, Uimlohihi, Uimhilohi, Uimlolohi,
+                  Uimhihilo, Uimlohilo, Uimhilolo, Uimlololo};
+  double *b[] = {brehihihi, brelohihi, brehilohi, brelolohi, brehihilo, brelohilo,
+                 brehilolo, brelololo, bimhihihi, bimlohihi, bimhilohi, bimlolohi,
+                 bimhihilo, bimlohilo, bimhilolo, bimlololo};
+  double *x[] = {xrehihihi, xrelohihi, xrehilohi, xrelolohi, xrehihilo, xrelohilo,
+                 xrehilolo, xrelololo, ximhihihi, ximlohihi, ximhilohi, ximlolohi,
+                 ximhihilo, ximlohilo, ximhilolo, ximlololo};
+  tiled_solver(8, dim, szt, nbt, U, b, x, totlapms, walltimesec, addcnt, mulcnt, divcnt,
+               "GPU_cmplx8_upper_tiled_solver");
 }
 
 // real U: the complex solver with zero imaginary parts (the imaginary part

@@ -269,6 +269,15 @@ static void qr_gpu(int K, Planes &A, Planes &Q, Planes &R, int szt) {
                                Q.p(2), Q.p(3), R.p(0), R.p(1), R.p(2), R.p(3), &t[0], &t[1],
                                &t[2], &t[3], &t[4], &t[5], &t[6], &t[7], &t[8], &t[9], &wall,
                                &add, &mul, &div, &sq, false);
+  else if (K == 8)
+    GPU_cmplx8_blocked_houseqr(
+        m, n, szt, n / szt, A.p(0), A.p(1), A.p(2), A.p(3), A.p(4), A.p(5), A.p(6), A.p(7),
+        A.p(8), A.p(9), A.p(10), A.p(11), A.p(12), A.p(13), A.p(14), A.p(15), Q.p(0), Q.p(1),
+        Q.p(2), Q.p(3), Q.p(4), Q.p(5), Q.p(6), Q.p(7), Q.p(8), Q.p(9), Q.p(10), Q.p(11),
+        Q.p(12), Q.p(13), Q.p(14), Q.p(15), R.p(0), R.p(1), R.p(2), R.p(3), R.p(4), R.p(5),
+        R.p(6), R.p(7), R.p(8), R.p(9), R.p(10), R.p(11), R.p(12), R.p(13), R.p(14), R.p(15),
+        &t[0], &t[1], &t[2], &t[3], &t[4], &t[5], &t[6], &t[7], &t[8], &t[9], &wall, &add, &mul,
+        &div, &sq, false);
   else
     GPU_cmplx4_blocked_houseqr(m, n, szt, n / szt, A.p(0), A.p(1), A.p(2), A.p(3), A.p(4), A.p(5),
                                A.p(6), A.p(7), Q.p(0), Q.p(1), Q.p(2), Q.p(3), Q.p(4), Q.p(5),
@@ -299,9 +308,16 @@ static void solve_gpu(int K, Planes &U, Planes &b, Planes &x, int szt) {
   const int n = U.rows;
   double t[4] = {0}, wall = 0.0, over[3] = {0};
   long long add = 0, mul = 0, div = 0;
-  double *bp[8], *xp[8];
+  double *bp[16], *xp[16];
   for (int q = 0; q < 2 * K; ++q) { bp[q] = b.row(q, 0); xp[q] = x.row(q, 0); }
-  if (K == 2)
+  if (K == 8)
+    GPU_cmplx8_upper_tiled_solver(
+        n, szt, n / szt, U.p(0), U.p(1), U.p(2), U.p(3), U.p(4), U.p(5), U.p(6), U.p(7), U.p(8),
+        U.p(9), U.p(10), U.p(11), U.p(12), U.p(13), U.p(14), U.p(15), bp[0], bp[1], bp[2], bp[3],
+        bp[4], bp[5], bp[6], bp[7], bp[8], bp[9], bp[10], bp[11], bp[12], bp[13], bp[14], bp[15],
+        xp[0], xp[1], xp[2], xp[3], xp[4], xp[5], xp[6], xp[7], xp[8], xp[9], xp[10], xp[11],
+        xp[12], xp[13], xp[14], xp[15], &t[0], &t[1], &t[2], &t[3], &wall, &add, &mul, &div);
+  else if (K == 2)
     GPU_cmplx2_upper_tiled_solver(n, szt, n / szt, U.p(0), U.p(1), U.p(2), U.p(3), bp[0], bp[1],
                                   bp[2], bp[3], xp[0], xp[1], xp[2], xp[3], &t[0], &t[1], &t[2],
                                   &t[3], &wall, &add, &mul, &div);
@@ -509,8 +525,10 @@ int main() {
   test_qr(4, 64, 64, 16);
   test_qr(2, 96, 64, 32);
   test_qr(4, 48, 32, 8);
+  test_qr(8, 32, 32, 8);
   test_tiled(2, 64, 16);
   test_tiled(4, 96, 32);
+  test_tiled(8, 32, 8);
   test_tiled_real(2, 64, 16);
   test_tiled_real(4, 96, 32);
   test_least_squares(2, 64, 16);

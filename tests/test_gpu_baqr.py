@@ -17,8 +17,9 @@ def ulps(K):
     return (2.0 ** -52) ** K
 
 
-@pytest.mark.parametrize("K", (2, 4))
-@pytest.mark.parametrize("shape", [(32, 32, 8), (48, 32, 8), (64, 64, 16), (40, 40, 40)])
+@pytest.mark.parametrize("K,shape", [(K, s) for K in (2, 4)
+                                     for s in [(32, 32, 8), (48, 32, 8), (64, 64, 16), (40, 40, 40)]]
+                         + [(8, (32, 32, 8)), (8, (40, 24, 8))])
 def test_blocked_qr_matches_oracle(K, shape):
     from phcpack_amd.baqr import BlockedQR
     m, n, szt = shape
@@ -40,7 +41,7 @@ def test_blocked_qr_split_sums_match_oracle(K, shape):
     test_blocked_qr_matches_oracle(K, shape)
 
 
-@pytest.mark.parametrize("K", (2, 4))
+@pytest.mark.parametrize("K", (2, 4, 8))
 def test_upper_tiled_solver_matches_oracle(K):
     from phcpack_amd.baqr import upper_tiled_solver
     print(f"tiled solver K={K}", flush=True)
@@ -87,3 +88,26 @@ def test_least_squares_recovers_planted_solution(K, n, szt):
     err = np.abs(md_diff(K, dx.to_host(), x0)).max()
     print(f"  error {err:.2e}", flush=True)
     assert err < 1e6 * ulps(K)
+
+
+def test_octo_double_gemm_matches_oracle():
+    """the complex octo-double GEMM (phc_cmplx_gemm, K = 8: the octo-double
+    checks of the quad-double least-squares solve) against the oracle's
+    octo-double matrix-vector product: every k term summed in order, so the
+    two differ only by the order of the four real products of a complex
+    multiply-add: within 4 n eps^8"""
+    from phcpack_amd._lib import DeviceArray, check, load
+    from phcpack_amd.baqr import gemm_device
+    K, m, n = 8, 40, 72
+    rng = np.random.default_rng(88)
+    A = rand_cmd(rng, K, (m, n))
+    x = rand_cmd(rng, K, (n,))
+    dA, dx = DeviceArray.from_host(A), DeviceArray.from_host(x)
+    dy = DeviceArray.zeros((2 * K, m))
+    print("octo-double gemm", flush=True)
+    gemm_device(K, False, False, m, 1, n, (dA, 0, n, 1), (dx, 0, 1, 0), (dy, 0, 1, 0))
+    check(load().phc_synchronize())
+    y = oracle_matvec(K, A, x)
+    err = np.abs(md_diff(K, dy.to_host(), y)).max()
+    print(f"  max |y - yo| {err:.2e}", flush=True)
+    assert err < 4 * n * ulps(K)
