@@ -84,7 +84,8 @@ int phc_poly_evaldiff(phc_poly *plan, int nbatch, const double *input_dev, doubl
 /* --------------------------------------------------------------- tracking */
 
 /* Compiled homotopy H_i(x,t) = sum_r w_r(t) sum_m c[r,i,m] x^m (built by
- * compile_homotopy in phcpack_amd/homotopy.py).  Series coefficients live in
+ * phc_program_compile below, or by compile_homotopy in
+ * phcpack_amd/homotopy.py: the same arrays).  Series coefficients live in
  * "slots" (complex units): the variables x_j at j*(deg+1), the constant 1 at
  * n*(deg+1), product nodes where nodes[] says.
  *   nodes [8*nprod], per product node in DAG-level order:
@@ -197,6 +198,28 @@ typedef struct {
 /* Checks every index of a program against its sizes (no device work);
  * phc_tracker_create calls it first. */
 int phc_program_validate(const phc_program *program);
+
+/* Host compiler of the program (host only, no device work): the homotopy
+ * H(x,t) = sum_r w_r(t) P_r(x) with ncomp components, w_r(t) = sum_l
+ * weights[r][l][0] t^l + i weights[r][l][1] t^l (l = 0, 1, 2), and P_r
+ * given term by term: nterms[r] terms after those of component r-1, term k
+ * in equation eqs[k] (0..n-1) with exponents exps[k][n] (0..15) and the
+ * complex coefficient in the limb planes coef[2*CL][sum nterms] (CL = 1:
+ * double, 2: double-double; real limbs, then imaginary limbs).  The
+ * program is built for series of degree deg over `lanes` lanes per path
+ * (64), the same arrays as compile_homotopy of phcpack_amd/homotopy.py.
+ * A double-double coefficient gives two terms of its monomial (high part,
+ * low part).  The homotopy of the reference's batched tracker,
+ * gamma (1 - t) q + t p (Ada/PHCtoC/gpumanypaths_dd.cpp:88, alpha = gamma),
+ * is ncomp = 2 with weights {0, 1, 0} for p and {gamma, -gamma, 0} for q.
+ * phc_compiled_program fills a phc_program whose arrays belong to the
+ * compiled object (valid until phc_compiled_destroy). */
+typedef struct phc_compiled phc_compiled;
+int phc_program_compile(phc_compiled **out, int n, int deg, int lanes, int ncomp,
+                        const double *weights, const int *nterms, const int *eqs, const int *exps,
+                        int CL, const double *coef);
+int phc_compiled_program(const phc_compiled *compiled, phc_program *program);
+int phc_compiled_destroy(phc_compiled *compiled);
 
 /* Learned step-size control (no counterpart in the reference, whose step
  * rule - the pole step of LU_Prediction,

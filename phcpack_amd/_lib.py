@@ -88,6 +88,10 @@ SIGNATURES.update({
     "phc_tracker_create": (c_int, [ctypes.POINTER(c_void_p), ctypes.POINTER(PhcProgram)]),
     "phc_tracker_destroy": (c_int, [c_void_p]),
     "phc_program_validate": (c_int, [ctypes.POINTER(PhcProgram)]),
+    "phc_program_compile": (c_int, [P_void_p, c_int, c_int, c_int, c_int, P_double, P_int, P_int,
+                                    P_int, c_int, P_double]),
+    "phc_compiled_program": (c_int, [c_void_p, ctypes.POINTER(PhcProgram)]),
+    "phc_compiled_destroy": (c_int, [c_void_p]),
     "phc_tracker_lds_bytes": (c_int, [c_void_p, c_int, ctypes.POINTER(ctypes.c_size_t),
                                       ctypes.POINTER(ctypes.c_size_t)]),
     "phc_track": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p,

@@ -2,7 +2,8 @@
 
 ``python -m phcpack_amd.build`` compiles every ``csrc/*.hip`` source with
 hipcc for gfx950 into ``phcpack_amd/libphcmd.so`` (in-tree, so it travels with
-the repository) and runs ``make -C oracle``.
+the repository), links the PHCtoC entry (``phctoc/``) into
+``phcpack_amd/libphcmd_phctoc.so`` on top of it, and runs ``make -C oracle``.
 """
 import os
 import subprocess
@@ -14,6 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libphcmd.so")
+PHCTOC_LIB = os.path.join(HERE, "libphcmd_phctoc.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PHC_OFFLOAD_ARCH", "gfx950")
@@ -57,6 +59,22 @@ def build_hip(jobs=None):
     return LIB
 
 
+def build_phctoc():
+    """libphcmd_phctoc.so: gpumanypaths_dd (include/phcmd_phctoc.h) over
+    libphcmd.so; PHCpack's container functions stay undefined, to be
+    resolved from the program that loads it"""
+    src = os.path.join(HERE, "phctoc", "gpumanypaths_dd.cpp")
+    tmp = PHCTOC_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-fPIC", "-shared", src, "-o", tmp,
+           "-L" + HERE, "-lphcmd", "-Wl,-rpath,$ORIGIN",
+           "-Wl,-rpath-link," + os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"PHCtoC build failed:\n{r.stderr}")
+    os.replace(tmp, PHCTOC_LIB)
+    return PHCTOC_LIB
+
+
 def build_oracle():
     r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")],
                        capture_output=True, text=True)
@@ -78,6 +96,7 @@ def build_compat_test():
 
 def build_all():
     build_hip()
+    build_phctoc()
     build_oracle()
     build_compat_test()
 

@@ -469,3 +469,97 @@ def _hessian_divisors(e):
         d[b] -= 1
         out.append((a, tuple(d), b))
     return out
+
+
+def compile_terms(n, deg, weights, comps, lanes=64):
+    """The tracker program built by the C++ compiler of libphcmd.so
+    (phc_program_compile), for callers that hold the terms as arrays.
+    weights: complex [ncomp][3] (coefficients of 1, t, t^2 of every w_r);
+    comps: per component (eqs [m] int, exps [m][n] int, coef) with coef
+    complex [m] (double coefficients) or float [4][m] double-double limb
+    planes (real high, real low, imaginary high, imaginary low)."""
+    import ctypes
+    from ._lib import P_double, P_int, PhcProgram, check, load
+    lib = load()
+    w = np.asarray(weights, np.complex128).reshape(-1, 3)
+    ncomp = w.shape[0]
+    wflat = np.ascontiguousarray(np.stack([w.real, w.imag], axis=-1).reshape(-1))
+    cl = 2 if any(np.asarray(c).dtype != np.complex128 for _, _, c in comps) else 1
+    eqs, exps, planes, nterms = [], [], [], []
+    for e_i, x_i, c in comps:
+        e_i = np.asarray(e_i, np.int32).reshape(-1)
+        nterms.append(len(e_i))
+        eqs.append(e_i)
+        exps.append(np.asarray(x_i, np.int32).reshape(len(e_i), n))
+        c = np.asarray(c)
+        if c.dtype == np.complex128:
+            p = np.zeros((2 * cl, len(e_i)))
+            p[0], p[cl] = c.real, c.imag
+        else:
+            p = np.asarray(c, np.float64).reshape(4, len(e_i))
+        planes.append(p)
+    eqs = np.ascontiguousarray(np.concatenate(eqs) if eqs else np.zeros(0, np.int32), np.int32)
+    exps = np.ascontiguousarray(np.concatenate(exps) if exps else np.zeros((0, n), np.int32),
+                                np.int32)
+    coef = np.ascontiguousarray(np.concatenate(planes, axis=1) if planes else np.zeros((2 * cl, 0)))
+    nt = np.ascontiguousarray(nterms, np.int32)
+    handle = ctypes.c_void_p()
+
+    def ptr(a, typ):
+        return a.ctypes.data_as(typ) if a.size else None
+
+    check(lib.phc_program_compile(ctypes.byref(handle), n, deg, lanes, ncomp,
+                                  wflat.ctypes.data_as(P_double), nt.ctypes.data_as(P_int),
+                                  ptr(eqs, P_int), ptr(exps, P_int), cl, ptr(coef, P_double)))
+    try:
+        s = PhcProgram()
+        check(lib.phc_compiled_program(handle, ctypes.byref(s)))
+
+        def arr(p, size, dtype):
+            if size == 0:
+                return np.zeros(0, dtype)
+            return np.ctypeslib.as_array(p, shape=(size,)).astype(dtype, copy=True)
+
+        nw = max(3 * ncomp, 1)
+        nh = n * (n + 1) // 2
+        return Program(
+            n=s.n, deg=s.deg, ncomp=s.ncomp, wdeg=s.wdeg, G=s.G,
+            nodes=arr(s.nodes, 8 * s.nprod, np.int32), lev_ptr=arr(s.lev_ptr, s.nlev + 1, np.int32),
+            nslots=s.nslots, nfactor=s.nfactor,
+            pairs=arr(s.pairs, 4 * s.npair, np.int32),
+            seg_ptr=arr(s.seg_ptr, n * s.G + 1, np.int32),
+            term_pair=arr(s.term_pair, s.nterm, np.int32),
+            term_coef=arr(s.term_coef, 2 * s.nterm, np.float64).reshape(2, s.nterm),
+            jpairs=arr(s.jpairs, 2 * s.njpair, np.int32),
+            jent_ptr=arr(s.jent_ptr, n * n + 1, np.int32),
+            jterms=arr(s.jterms, s.njterm, np.int32),
+            weights=arr(s.weights, 2 * 8 * nw, np.float64).reshape(2, 8, nw),
+            hsig=arr(s.hsig, n, np.float64),
+            hent_ptr=arr(s.hent_ptr, n * nh + 1, np.int32),
+            hterms=arr(s.hterms, 3 * s.nhterm, np.int32),
+            hpairs=arr(s.hpairs, 2 * s.nhpair, np.int32),
+            var_off=np.arange(n + 1, dtype=np.int32) * (deg + 1))
+    finally:
+        lib.phc_compiled_destroy(handle)
+
+
+def homotopy_terms(hom):
+    """(weights [ncomp][3], comps) of a Homotopy in the arrays of compile_terms"""
+    comps = []
+    for _, sysm in hom.components:
+        eqs, exps, cf = [], [], []
+        for i, eq in enumerate(sysm.eqs):
+            for c, e in eq:
+                eqs.append(i)
+                exps.append(e)
+                cf.append(c)
+        comps.append((np.array(eqs, np.int32), np.array(exps, np.int32).reshape(len(eqs), hom.n),
+                      np.array(cf, np.complex128)))
+    return np.array([w for w, _ in hom.components], np.complex128), comps
+
+
+def compile_homotopy_native(hom, deg, lanes=64):
+    """compile_homotopy through the C++ compiler (the same arrays; the
+    Hessian singular values by a Jacobi SVD, to rounding)"""
+    w, comps = homotopy_terms(hom)
+    return compile_terms(hom.n, deg, w, comps, lanes)
