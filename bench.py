@@ -614,7 +614,13 @@ def run_qr(args, n=1024, szt=128, K=4, seed=SEED):
     dA, dx0 = DeviceArray.from_host(A), DeviceArray.from_host(x0)
     db = DeviceArray.zeros((P, n))
     gemm_device(K, False, False, n, 1, n, (dA, 0, n, 1), (dx0, 0, 1, 0), (db, 0, 1, 0))
+    # the GEMM register block and k split of the plan, tuned on this matrix
+    # (the tile size is the workload's: 8 tiles of 128)
+    from phcpack_amd.autotune import QR_BLOCKS, tune_qr
+    best, table = tune_qr(K, n, n, dA, szts=[szt],
+                          log=lambda s: print(s, file=sys.stderr, flush=True))
     qr = BlockedQR(K, n, n, szt)
+    qr.set_tuning(best[1], best[2])
     dQ, dR = DeviceArray((P, n, n)), DeviceArray((P, n, n))
     dy, dx, dbw = DeviceArray((P, n)), DeviceArray((P, n)), DeviceArray((P, n))
     ev = Events(lib)
@@ -664,6 +670,9 @@ def run_qr(args, n=1024, szt=128, K=4, seed=SEED):
         "config": {"workload": f"complex quad-double {n}x{n}, {n // szt} tiles of {szt} columns, "
                                "entries uniform in [-1,1], b = A x0", "seed": seed},
         "qr_ms": round(ms_qr, 3),
+        "tuning": {"gemm_block": QR_BLOCKS[best[1]], "split_target": best[2],
+                   "qr_ms_by_block_and_split": {f"{QR_BLOCKS[b]}/{sp}": round(v, 2)
+                                                for (_, b, sp), v in sorted(table.items())}},
         "checks": {"max_abs_QHQ_minus_I": qq, "max_abs_QR_minus_A": qra, "max_abs_x_minus_x0": xerr,
                    "octo_double": od},
         "roofline": {"bound": "fp64-valu", "achieved": round(executed / (ms_qr / 1e3) / 1e12, 3),

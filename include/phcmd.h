@@ -440,14 +440,27 @@ int phc_row_newton(phc_rowsys *sys, int K, int deg, int nbsteps, int nsys, const
                    double *history_dev, double *Q_dev, double *R_dev, void *stream);
 
 /* ------------- complex blocked Householder QR, tiled back substitution ----
- * K = 2 or 4.  Matrices are complex limb-planar [2K][rows][cols], row-major.
- * Results agree with the reference's host code to a few units in the last
- * limb (long sums are split across lanes). */
+ * K = 2, 4 or 8.  Matrices are complex limb-planar [2K][rows][cols],
+ * row-major.  Results agree with the reference's host code to a few units in
+ * the last limb (long sums are split across lanes). */
 
 /* workspace for A (nrows x ncols, nrows >= ncols, ncols = szt*nbt) */
 typedef struct phc_baqr phc_baqr;
 int phc_baqr_create(phc_baqr **plan, int K, int nrows, int ncols, int szt);
 int phc_baqr_destroy(phc_baqr *plan);
+
+/* Launch configuration of the plan's matrix products (the trailing updates,
+ * W = Y S, the Gram matrix, Q^H b), chosen by the autotuner
+ * (phcpack_amd/autotune.py, tune_qr); no counterpart in the reference, whose
+ * kernels fix their launch shapes.  gemm_block: outputs per lane, 0 the
+ * default (2 x 2 for K = 2, 1 x 1 for K = 4, 8), 1: 1 x 1, 2: 2 x 1,
+ * 3: 1 x 2, 4: 2 x 2; split_target: the k range of a product is split into
+ * slices, doubling while tiles x slices < split_target (0: 1024), slices
+ * of >= 64 terms, within the plan's workspace.  The split changes the order
+ * of the long sums (results to rounding); the block does not.  get returns
+ * the values in use (the defaults resolved). */
+int phc_baqr_set_tuning(phc_baqr *plan, int gemm_block, int split_target);
+int phc_baqr_get_tuning(const phc_baqr *plan, int *gemm_block, int *split_target);
 
 /* A = Q R with Q (nrows x nrows) explicit and R (nrows x ncols), in tiles of
  * szt columns: CPU_cmplx4_blocked_houseqr (Matrices/dbl4_baqr_host.cpp:1206),

@@ -32,6 +32,17 @@ class BlockedQR:
         except Exception:
             pass
 
+    def set_tuning(self, gemm_block=0, split_target=0):
+        """launch configuration of the plan's matrix products
+        (phc_baqr_set_tuning; 0, 0: the defaults)"""
+        check(load().phc_baqr_set_tuning(self.handle, int(gemm_block), int(split_target)))
+
+    def tuning(self):
+        """(gemm_block, split_target) in use"""
+        b, s = ctypes.c_int(), ctypes.c_int()
+        check(load().phc_baqr_get_tuning(self.handle, ctypes.byref(b), ctypes.byref(s)))
+        return b.value, s.value
+
     def factor_device(self, dA, dQ, dR, stream=None):
         check(load().phc_cmplx_blocked_houseqr(self.handle, dA.ptr, dQ.ptr, dR.ptr, stream))
 

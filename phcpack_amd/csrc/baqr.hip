@@ -66,23 +66,30 @@ __device__ __forceinline__ void cmac(Z<K> &acc, const Z<K> &a, const Z<K> &b) {
 
 constexpr int TKD = 8;  // k depth of the LDS slices
 
-// register block (outputs per lane along i and j) of the GEMM for K limbs:
-// the 16 x 16 lanes of a workgroup cover a (16 RA) x (16 RB) tile
-template <int K> struct GemmShape {
-  static constexpr int RA = (K >= 4) ? 1 : 2, RB = (K >= 4) ? 1 : 2;
-  static constexpr int TM = 16 * RA, TN = 16 * RB;
+// Launch configuration of the GEMM, chosen per QR plan (phc_baqr_set_tuning,
+// by the autotuner of phcpack_amd/autotune.py):
+//   block: register block RA x RB (outputs per lane along i and j; the
+//     16 x 16 lanes of a workgroup cover a (16 RA) x (16 RB) tile):
+//     0 the default of the limb count (2 x 2 for dd, 1 x 1 for qd and od),
+//     1: 1 x 1, 2: 2 x 1, 3: 1 x 2, 4: 2 x 2;
+//   target: the k range is split into slices (doubling) while the tiles
+//     times the slices stay below target (0: 1024).
+struct GemmCfg {
+  int block = 0, target = 0;
 };
+template <int K> constexpr int default_block() { return (K >= 4) ? 1 : 4; }
+inline int block_ra(int blk) { return (blk == 2 || blk == 4) ? 2 : 1; }
+inline int block_rb(int blk) { return (blk >= 3) ? 2 : 1; }
 
 // C(i, j) = [C(i, j) +] sum_k op(A(i, k)) op(B(k, j)), i < M, j < N, k < KD;
 // the sum starts from zero and is added to C at the end.  With a split of
 // the k range over gridDim.z slices, slice s writes its partial sums to
 // part[s][i][j] (plane stride pps) and csum_kernel adds the slices in order.
-template <int K, bool CA, bool CB>
+template <int K, bool CA, bool CB, int RA, int RB>
 __global__ void __launch_bounds__(256)
 cgemm_kernel(int M, int N, int KD, int kchunk, Op A, Op B, double *Cp, size_t cps, long c0,
              long c1, int accumulate, double *part, size_t pps) {
-  constexpr int RA = GemmShape<K>::RA, RB = GemmShape<K>::RB;
-  constexpr int TM = GemmShape<K>::TM, TN = GemmShape<K>::TN;
+  constexpr int TM = 16 * RA, TN = 16 * RB;
   __shared__ double As[2 * K][TKD][TM];
   __shared__ double Bs[2 * K][TKD][TN];
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
@@ -164,34 +171,48 @@ __global__ void csum_kernel(int M, int N, int S, const double *part, size_t pps,
   }
 }
 
-// split of the k range: enough workgroups to fill the chip, slices of at
-// least 64 terms, and M*N*S partial sums within the workspace
-template <int K> int ksplit(int M, int N, int KD, size_t cap) {
-  constexpr int TM = GemmShape<K>::TM, TN = GemmShape<K>::TN;
+// split of the k range: enough workgroups to fill the chip (tiles times
+// slices up to the target), slices of at least 64 terms, and M*N*S partial
+// sums within the workspace
+inline int ksplit(int M, int N, int KD, size_t cap, int TM, int TN, int target) {
   const long tiles = (long)((M + TM - 1) / TM) * ((N + TN - 1) / TN);
   int S = 1;
-  while (tiles * S < 1024 && KD / (2 * S) >= 64 && (size_t)M * N * 2 * S <= cap) S *= 2;
+  while (tiles * S < target && KD / (2 * S) >= 64 && (size_t)M * N * 2 * S <= cap) S *= 2;
   return S;
+}
+
+template <int K, bool CA, bool CB>
+void launch_gemm(int blk, dim3 grid, hipStream_t st, int M, int N, int KD, int kchunk, Op A,
+                 Op B, double *Cp, size_t cps, long c0, long c1, int acc, double *part,
+                 size_t pps) {
+#define GEMM(RA_, RB_)                                                                           \
+  hipLaunchKernelGGL((cgemm_kernel<K, CA, CB, RA_, RB_>), grid, dim3(256), 0, st, M, N, KD, kchunk, \
+                     A, B, Cp, cps, c0, c1, acc, part, pps)
+  switch (blk) {
+    case 1: GEMM(1, 1); break;
+    case 2: GEMM(2, 1); break;
+    case 3: GEMM(1, 2); break;
+    default: GEMM(2, 2); break;
+  }
+#undef GEMM
 }
 
 template <int K>
 int cgemm(bool ca, bool cb, int M, int N, int KD, Op A, Op B, double *Cp, size_t cps, long c0,
-          long c1, bool accumulate, hipStream_t st, double *part = nullptr, size_t cap = 0) {
+          long c1, bool accumulate, hipStream_t st, double *part = nullptr, size_t cap = 0,
+          GemmCfg cfg = GemmCfg()) {
   if (M <= 0 || N <= 0) return 0;
-  constexpr int TM = GemmShape<K>::TM, TN = GemmShape<K>::TN;
-  const int S = part ? ksplit<K>(M, N, KD, cap) : 1;
+  const int blk = cfg.block ? cfg.block : default_block<K>();
+  const int TM = 16 * block_ra(blk), TN = 16 * block_rb(blk);
+  const int S = part ? ksplit(M, N, KD, cap, TM, TN, cfg.target ? cfg.target : 1024) : 1;
   const int kchunk = ((KD + S - 1) / S + TKD - 1) / TKD * TKD;
   dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, S);
   const int acc = accumulate ? 1 : 0;
   const size_t pps = cap;
-#define GEMM(CA_, CB_)                                                                         \
-  hipLaunchKernelGGL((cgemm_kernel<K, CA_, CB_>), grid, dim3(256), 0, st, M, N, KD, kchunk, A, \
-                     B, Cp, cps, c0, c1, acc, part, pps)
-  if (!ca && !cb) GEMM(false, false);
-  else if (ca && !cb) GEMM(true, false);
-  else if (!ca && cb) GEMM(false, true);
-  else GEMM(true, true);
-#undef GEMM
+  if (!ca && !cb) launch_gemm<K, false, false>(blk, grid, st, M, N, KD, kchunk, A, B, Cp, cps, c0, c1, acc, part, pps);
+  else if (ca && !cb) launch_gemm<K, true, false>(blk, grid, st, M, N, KD, kchunk, A, B, Cp, cps, c0, c1, acc, part, pps);
+  else if (!ca && cb) launch_gemm<K, false, true>(blk, grid, st, M, N, KD, kchunk, A, B, Cp, cps, c0, c1, acc, part, pps);
+  else launch_gemm<K, true, true>(blk, grid, st, M, N, KD, kchunk, A, B, Cp, cps, c0, c1, acc, part, pps);
   if (S > 1)
     hipLaunchKernelGGL(csum_kernel<K>, dim3(phc::grid_for((size_t)M * N, 256)), dim3(256), 0, st,
                        M, N, S, part, pps, Cp, cps, c0, c1, acc);
@@ -566,6 +587,7 @@ __global__ void init_qr_kernel(int nrows, int ncols, const double *A, double *Q,
 
 struct phc_baqr {
   int K, nrows, ncols, szt, nbt;
+  GemmCfg cfg;  // GEMM launch configuration of the plan's products
   double *Y, *W, *G, *S, *T1, *T2, *Pc, *wpart, *B, *part;
   size_t yps, gps, t1ps, t2ps, pps, wpps, pcap;
   // the launch sequence of one factorization, captured for the last (A, Q, R)
@@ -607,30 +629,30 @@ template <int K> int run_qr(phc_baqr *p, const double *A, double *Q, double *R, 
     const Op Yc{p->Y, p->yps, rowdim, 1};  // (k, i) -> Y[k][i]
     // G = Yhat^H Yhat
     if (cgemm<K>(true, false, szt, szt, rowdim, Yc, Yt, p->G, p->gps, szt, 1, false, st, p->part,
-                 p->pcap)) return -1;
+                 p->pcap, p->cfg)) return -1;
     hipLaunchKernelGGL(s_factor_kernel<K>, dim3(szt), dim3(64), 0, st, p->G, p->gps, p->B, p->S,
                        p->gps, szt);
     PHC_HIP(hipGetLastError());
     // W = Yhat S, stored like Y
     if (cgemm<K>(false, false, rowdim, szt, szt, Yt, Op{p->S, p->gps, szt, 1}, p->W, p->yps, 1,
-                 rowdim, false, st, p->part, p->pcap)) return -1;
+                 rowdim, false, st, p->part, p->pcap, p->cfg)) return -1;
     const Op Wt{p->W, p->yps, 1, rowdim};  // What(i, k) = W[k][i]
     if (k < p->nbt - 1) {
       const int coloff = endcol, coldim = n - coloff;
       // T1 = What^H C with C = R[rowoff.., coloff..];  C += Yhat T1
       const Op Cm{R + (size_t)rowoff * n + coloff, rps, n, 1};
       if (cgemm<K>(true, false, szt, coldim, rowdim, Op{p->W, p->yps, rowdim, 1}, Cm, p->T1,
-                   p->t1ps, coldim, 1, false, st, p->part, p->pcap)) return -1;
+                   p->t1ps, coldim, 1, false, st, p->part, p->pcap, p->cfg)) return -1;
       if (cgemm<K>(false, false, rowdim, coldim, szt, Yt, Op{p->T1, p->t1ps, coldim, 1},
-                   R + (size_t)rowoff * n + coloff, rps, n, 1, true, st, p->part, p->pcap))
+                   R + (size_t)rowoff * n + coloff, rps, n, 1, true, st, p->part, p->pcap, p->cfg))
         return -1;
     }
     // T2 = Q[:, rowoff..] What;  Q[:, rowoff..] += T2 Yhat^H
     const Op Qm{Q + rowoff, qps, m, 1};
     if (cgemm<K>(false, false, m, szt, rowdim, Qm, Wt, p->T2, p->t2ps, szt, 1, false, st, p->part,
-                 p->pcap)) return -1;
+                 p->pcap, p->cfg)) return -1;
     if (cgemm<K>(false, true, m, rowdim, szt, Op{p->T2, p->t2ps, szt, 1}, Yc, Q + rowoff, qps, m,
-                 1, true, st, p->part, p->pcap)) return -1;
+                 1, true, st, p->part, p->pcap, p->cfg)) return -1;
   }
   return 0;
 }
@@ -749,6 +771,29 @@ int phc_baqr_destroy(phc_baqr *p) {
   return 0;
 }
 
+int phc_baqr_set_tuning(phc_baqr *p, int gemm_block, int split_target) {
+  PHC_REQUIRE(p, "phc_baqr_set_tuning: null plan");
+  PHC_REQUIRE(gemm_block >= 0 && gemm_block <= 4, "phc_baqr_set_tuning: gemm_block must be 0..4");
+  PHC_REQUIRE(split_target >= 0 && split_target <= (1 << 20),
+              "phc_baqr_set_tuning: split_target must be 0..2^20");
+  p->cfg.block = gemm_block;
+  p->cfg.target = split_target;
+  // the captured launch sequence holds the old configuration
+  if (p->exec) {
+    PHC_HIP(hipGraphExecDestroy(p->exec));
+    p->exec = nullptr;
+  }
+  return 0;
+}
+
+int phc_baqr_get_tuning(const phc_baqr *p, int *gemm_block, int *split_target) {
+  PHC_REQUIRE(p && gemm_block && split_target, "phc_baqr_get_tuning: null argument");
+  *gemm_block = p->cfg.block ? p->cfg.block
+                : (p->K >= 4 ? default_block<4>() : default_block<2>());
+  *split_target = p->cfg.target ? p->cfg.target : 1024;
+  return 0;
+}
+
 // The ~4 launches per column are captured once into a HIP graph (per
 // (A, Q, R) pointers) and replayed: the panel kernels are short, so launch
 // gaps would otherwise dominate.
@@ -788,11 +833,12 @@ int phc_cmplx_qhb(phc_baqr *p, const double *Q, const double *b, double *y, void
   hipStream_t st = (hipStream_t)stream;
   const Op qa{Q, (size_t)n * n, 1, n}, bb{b, (size_t)n, 1, 0};
   if (p->K == 8)
-    return cgemm<8>(true, false, n, 1, n, qa, bb, y, (size_t)n, 1, 0, false, st, p->part, p->pcap);
+    return cgemm<8>(true, false, n, 1, n, qa, bb, y, (size_t)n, 1, 0, false, st, p->part, p->pcap,
+                    p->cfg);
   return p->K == 2 ? cgemm<2>(true, false, n, 1, n, qa, bb, y, (size_t)n, 1, 0, false, st, p->part,
-                              p->pcap)
+                              p->pcap, p->cfg)
                    : cgemm<4>(true, false, n, 1, n, qa, bb, y, (size_t)n, 1, 0, false, st, p->part,
-                              p->pcap);
+                              p->pcap, p->cfg);
 }
 
 int phc_cmplx_gemm(int K, int conjA, int conjB, int M, int N, int KD, const double *A,

@@ -125,3 +125,41 @@ def test_octo_double_gemm_matches_oracle():
     err = np.abs(md_diff(K, dy.to_host(), y)).max()
     print(f"  max |y - yo| {err:.2e}", flush=True)
     assert err < 4 * n * ulps(K)
+
+
+@pytest.mark.parametrize("K,shape", [(2, (96, 64, 32)), (4, (128, 128, 32)), (8, (64, 32, 16))])
+def test_tuned_configurations_match_oracle(K, shape):
+    """every register block and k split the autotuner may choose: the same
+    tolerance against the oracle; the register block alone does not change
+    a bit (each output's k sum keeps its order), only the split does"""
+    from phcpack_amd._lib import DeviceArray, check, load
+    from phcpack_amd.autotune import QR_BLOCKS, tune_qr
+    from phcpack_amd.baqr import BlockedQR
+    m, n, szt = shape
+    rng = np.random.default_rng(77 + K)
+    A = rand_cmd(rng, K, (m, n))
+    Qo, Ro = oracle_blocked_houseqr(K, A, szt, n // szt)
+    qr = BlockedQR(K, m, n, szt)
+    dA = DeviceArray.from_host(np.ascontiguousarray(A))
+    dQ, dR = DeviceArray((2 * K, m, m)), DeviceArray((2 * K, m, n))
+    tol = QR_UNITS * m * ulps(K)
+    first = {}
+    for split in (64, 1024, 8192):
+        for blk in (1, 2, 3, 4):
+            qr.set_tuning(blk, split)
+            assert qr.tuning() == (blk, split)
+            qr.factor_device(dA, dQ, dR)
+            check(load().phc_synchronize())
+            Q, R = dQ.to_host(), dR.to_host()
+            dq, dr = np.abs(md_diff(K, Q, Qo)).max(), np.abs(md_diff(K, R, Ro)).max()
+            print(f"  K={K} {QR_BLOCKS[blk]} split {split}: {dq / ulps(K):.1f} u / {dr / ulps(K):.1f} u",
+                  flush=True)
+            assert dq < tol and dr < tol * np.sqrt(m)
+            if split in first:
+                assert np.array_equal(Q, first[split][0]) and np.array_equal(R, first[split][1])
+            else:
+                first[split] = (Q, R)
+    qr.set_tuning(0, 0)
+    assert qr.tuning() == ((4, 1024) if K == 2 else (1, 1024))
+    best, table = tune_qr(K, m, n, dA, szts=[szt, n], blocks=(1, 4), splits=(1024,), reps=1)
+    assert len(table) == 4 and best in table and table[best] == min(table.values())
