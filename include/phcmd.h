@@ -247,10 +247,19 @@ int phc_compiled_destroy(phc_compiled *compiled);
  *         records (at most log_capacity per path).
  * In modes 1 and 2 the multiplied step is capped at pole_cap * r (r the
  * Fabry radius) when pole_cap > 0, so that the step stays inside the disk
- * where the series converges; candidate 0 (multiplier 1) is the plain rule. */
+ * where the series converges; candidate 0 (multiplier 1) is the plain rule.
+ * Predictor order: in modes 1 and 2 the tracker first computes the Taylor
+ * series to degree deg0 (every degree 0: the program's), forms the
+ * features from it (orders above deg0 read as zero, the Fabry radius from
+ * c_{deg0-1-o} / c_{deg0-o}), and then continues the series to the chosen
+ * candidate's degree[c] (deg0 <= degree[c] <= the program's degree), from
+ * which the step takes its pole radius, its [L-(deg-degree[c])/M] Pade
+ * predictor and its curvature step (root degree[c]) as the reference's rule
+ * does for a series of that degree.  A candidate is thus a (multiplier,
+ * series degree) pair, and a lower degree saves the orders it leaves out. */
 #define PHC_CTL_NFEAT 16
 #define PHC_CTL_HIDDEN 64
-#define PHC_CTL_MAXCAND 8
+#define PHC_CTL_MAXCAND 16
 typedef struct {
   int mode, ncand;
   double mult[PHC_CTL_MAXCAND];
@@ -261,6 +270,8 @@ typedef struct {
   float *log_x;                   /* device */
   int *log_c, *log_y, *log_n;     /* device */
   double pole_cap;                /* 0: no cap */
+  int deg0;                       /* series degree before the choice (0: the program's) */
+  int degree[PHC_CTL_MAXCAND];    /* series degree of candidate c's step (0: the program's) */
 } phc_controller;
 
 typedef struct phc_tracker phc_tracker;

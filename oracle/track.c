@@ -818,17 +818,13 @@ double otr_hessian_sum(const otr_program *P, int K, double t, const double *x) {
   return hs2;
 }
 
-/* Taylor coefficients x_1..x_deg at the point in the order-0 slots
- * (weights set at t); *hs2 the Hessian sum at the point; 0 on a singular
- * Jacobian, which then stays in A, else A holds its inverse */
-static int predictor_series(const otr_program *P, work_t *w, double *hs2) {
+/* Taylor coefficients x_kb..x_{ke-1} (A holds the inverse Jacobian, the
+ * coefficients below kb are computed) */
+static void series_orders(const otr_program *P, work_t *w, int kb, int ke) {
   const int K = w->K, n = P->n, D = P->deg + 1;
   double xk[16 * 64];
   int k, j;
-  eval_point(P, w, 1);
-  *hs2 = hessian_sum(P, w);
-  if (!gj_invert(P, w)) return 0;
-  for (k = 1; k < D; k++) {
+  for (k = kb; k < ke; k++) {
     for (j = 0; j < n; j++) czero(K, ZP(w->slots, j * D + k));
     product_pass(P, w, k);
     residual(P, w, k);
@@ -836,6 +832,16 @@ static int predictor_series(const otr_program *P, work_t *w, double *hs2) {
     for (j = 0; j < n; j++) ccopy(K, ZP(xk, j), ZP(w->slots, j * D + k));
     correction_pass(P, w, k);
   }
+}
+
+/* Taylor coefficients x_1..x_{ke-1} (ke <= deg + 1) at the point in the
+ * order-0 slots (weights set at t); *hs2 the Hessian sum at the point; 0 on
+ * a singular Jacobian, which then stays in A, else A holds its inverse */
+static int predictor_series(const otr_program *P, work_t *w, double *hs2, int ke) {
+  eval_point(P, w, 1);
+  *hs2 = hessian_sum(P, w);
+  if (!gj_invert(P, w)) return 0;
+  series_orders(P, w, 1, ke);
   return 1;
 }
 
@@ -844,8 +850,8 @@ static int predictor_series(const otr_program *P, work_t *w, double *hs2) {
  * estimates.adb:28-38 Is_Zero, :138-163 per series, :354-390 over the
  * vector; the first component whose c_kd is not zero starts the minimum),
  * in FP64 on the leading limbs; returns 1 when every c_kd is zero */
-static int fabry_radius(const otr_program *P, work_t *w, int offset, double *r) {
-  const int K = w->K, D = P->deg + 1, kd = D - 1 - offset;
+static int fabry_radius(const otr_program *P, work_t *w, int kd, double *r) {
+  const int K = w->K, D = P->deg + 1;
   int j, fail = 1;
   *r = 0.0;
   for (j = 0; j < P->n; j++) {
@@ -952,7 +958,8 @@ static double minimum3(double a, double b, double c, int *which) {
  * of the rule, the step after the controller and Bounded_Update, the
  * step at the end, feedback halvings, corrector halvings, last feedback
  * mixed residual, Newton iterations of the last corrector, accepted (1)
- * or not, sigma_min(J), Hessian sum, t after the step, candidate */
+ * or not, sigma_min(J), Hessian sum, t after the step, candidate, series
+ * degree of the step */
 static void track_path(const otr_program *P, work_t *w, int npaths, int path, const double *start,
                        int KS, const double *tstart, double *endp, double *tend, int *status,
                        int *nsteps, int *nrej, double *dxlast, int *work, const otr_params *prm,
@@ -961,6 +968,7 @@ static void track_path(const otr_program *P, work_t *w, int npaths, int path, co
   const int mode = ctl ? ctl->mode : 0;
   const int pden = prm->pade_dendeg;
   const int pnum = (pden == 0 && prm->pade_numdeg == 0) ? D - 3 : prm->pade_numdeg;
+  const int d0 = (ctl && ctl->deg0 ? ctl->deg0 : P->deg) + 1;
   double t = 0.0, dxl = 0.0, hprev = prm->maxstep;
   double dxh[4] = {1.0, 1.0, 1.0, 1.0};
   int st = ST_RUNNING, steps = 0, rej = 0, j, k, npred = 0, nnewt = 0, nfeval = 0, ncorr = 0;
@@ -979,15 +987,18 @@ static void track_path(const otr_program *P, work_t *w, int npaths, int path, co
     load_point(P, w, start, KS, npaths, path);
     while (st == ST_RUNNING && t < 1.0) {
       double r, pole, curv, eta, nrm, hs2, sig1, step, tn, mix = 0.0, rule, step1;
+      int Dk, pnumk;
       double err[128];
       const double t0 = t;
       int pfail, which, evaluated, cand = 0, rej0, nnewt0, nfb, accepted = 0, nit = 0;
       set_weights(P, w, t);
       ++npred;
-      if (!predictor_series(P, w, &hs2)) { st = ST_SINGULAR; break; }
-      /* pole step (LU_Prediction, standard_predictor_convolutions.adb:1168-1172) */
-      pfail = fabry_radius(P, w, prm->fabry_offset, &r);
-      pole = pfail ? prm->maxstep : prm->pbeta * r;
+      /* the controller chooses after d0 coefficients; its candidate's
+         series degree may add orders (the plain rule: all D at once) */
+      Dk = pmode != 0 ? d0 : D;
+      if (!predictor_series(P, w, &hs2, Dk)) { st = ST_SINGULAR; break; }
+      /* pole radius (LU_Prediction, standard_predictor_convolutions.adb:1168-1172) */
+      pfail = fabry_radius(P, w, Dk - 1 - prm->fabry_offset, &r);
       /* the controller reads the Taylor series (before the Pade
          coefficients replace it) */
       if (pmode != 0) {
@@ -997,7 +1008,7 @@ static void track_path(const otr_program *P, work_t *w, int npaths, int path, co
         feat[2] = t;
         for (k = 1; k <= 8; k++) {
           double c = 0.0;
-          for (j = 0; j < n; j++) c = fmax(c, k < D ? absz(K, ZP(w->slots, j * D + k)) : 0.0);
+          for (j = 0; j < n; j++) c = fmax(c, k < Dk ? absz(K, ZP(w->slots, j * D + k)) : 0.0);
           feat[2 + k] = otr_lg2(c + 1e-300);
         }
         for (j = 0; j < 4; j++) feat[11 + j] = otr_lg2(dxh[j] + 1e-300);
@@ -1009,11 +1020,22 @@ static void track_path(const otr_program *P, work_t *w, int npaths, int path, co
             ctl->log_x[((size_t)path * ctl->log_capacity + nlog) * OTR_NFEAT + j] = (float)feat[j];
           ctl->log_c[(size_t)path * ctl->log_capacity + nlog] = cand;
         }
+        {
+          const int dc = (ctl->degree[cand] ? ctl->degree[cand] : P->deg) + 1;
+          if (dc > Dk) {
+            series_orders(P, w, Dk, dc);
+            Dk = dc;
+            pfail = fabry_radius(P, w, Dk - 1 - prm->fabry_offset, &r);
+          }
+        }
       }
+      pole = pfail ? prm->maxstep : prm->pbeta * r;
+      /* the [L/M] predictor of a series of Dk coefficients */
+      pnumk = pnum - (D - Dk);
       /* predictor coefficients and their error estimates (Solution_Error) */
       for (j = 0; j < n; j++) ccopy(K, ZP(w->slots, j * D), ZP(w->x0, j));
       for (j = 0; j < n; j++)
-        otr_pade_error(K, ZP(w->slots, j * D), pnum, pden, ZP(w->x0, j), err + 2 * j);
+        otr_pade_error(K, ZP(w->slots, j * D), pnumk, pden, ZP(w->x0, j), err + 2 * j);
       nrm = 0.0;
       for (j = 0; j < n; j++) nrm = nrm + err[2 * j] * err[2 * j] + err[2 * j + 1] * err[2 * j + 1];
       nrm = sqrt(nrm);
@@ -1022,7 +1044,7 @@ static void track_path(const otr_program *P, work_t *w, int npaths, int path, co
       lead_matrix(K, n, w->A, w->H);
       sig1 = 1.0 / sqrt(otr_sigma_max_sq(n, w->H, w->SP, w->SQ));
       eta = (2.0 * sig1) / sqrt(hs2);
-      curv = otr_root((prm->cbeta * eta) / nrm, D - 1);
+      curv = otr_root((prm->cbeta * eta) / nrm, Dk - 1);
       rule = minimum3(pole, curv, prm->maxstep, &which);
       if (which == 0) ++nbpole; else if (which == 1) ++nbhess; else ++nbmaxm;
       step = rule;
@@ -1041,7 +1063,7 @@ static void track_path(const otr_program *P, work_t *w, int npaths, int path, co
       nfb = 0;
       for (;;) {
         for (j = 0; j < n; j++)
-          otr_pade_evaluate(K, ZP(w->slots, j * D), pnum, pden, ZP(w->x0, j), step, ZP(w->slots, j * D));
+          otr_pade_evaluate(K, ZP(w->slots, j * D), pnumk, pden, ZP(w->x0, j), step, ZP(w->slots, j * D));
         set_weights(P, w, tn);
         ++nfeval;
         eval_point(P, w, 0);
@@ -1075,7 +1097,7 @@ static void track_path(const otr_program *P, work_t *w, int npaths, int path, co
         }
         tn = t + step;
         for (j = 0; j < n; j++)
-          otr_pade_evaluate(K, ZP(w->slots, j * D), pnum, pden, ZP(w->x0, j), step, ZP(w->slots, j * D));
+          otr_pade_evaluate(K, ZP(w->slots, j * D), pnumk, pden, ZP(w->x0, j), step, ZP(w->slots, j * D));
         evaluated = 0;
       }
       if (trace && ntrace && *ntrace < cap) {
@@ -1085,7 +1107,7 @@ static void track_path(const otr_program *P, work_t *w, int npaths, int path, co
         q[2] = pole; q[3] = curv; q[4] = eta; q[5] = nrm; q[6] = which; q[7] = rule;
         q[8] = step1; q[9] = step; q[10] = nfb; q[11] = rej - rej0 - nfb; q[12] = mix;
         q[13] = nit; q[14] = accepted; q[15] = sig1; q[16] = hs2; q[17] = t; q[18] = cand;
-        q[19] = 0.0;
+        q[19] = Dk - 1;
         ++*ntrace;
       }
       rejprev = rej - rej0;
@@ -1246,7 +1268,7 @@ static void endgame_point(const otr_program *P, work_t *w, double t, double *alp
   lamJ = otr_sigma_max_sq(n, M, w->SP, w->SQ);
   *alpha = INFINITY;
   *rho = INFINITY;
-  if (!predictor_series(P, w, &hs2)) {
+  if (!predictor_series(P, w, &hs2, P->deg + 1)) {
     *rco = 0.0;
   } else {
     lead_matrix(K, n, w->A, M);
@@ -1327,7 +1349,7 @@ int otr_series(const otr_program *P, int K, double t, const double *x, double *s
   set_weights(P, &w, t);
   {
     double hs2;
-    ok = predictor_series(P, &w, &hs2);
+    ok = predictor_series(P, &w, &hs2, D);
   }
   for (j = 0; j < n; j++)
     for (k = 0; k < D; k++)

@@ -39,9 +39,10 @@ typedef struct {
 /* step-size controller, layout of phc_controller (pointers are host memory) */
 #define OTR_NFEAT 16
 #define OTR_HIDDEN 64
+#define OTR_MAXCAND 16
 typedef struct {
   int mode, ncand;
-  double mult[8];
+  double mult[OTR_MAXCAND];
   const float *weights, *mu, *sd;
   float logit_threshold;
   unsigned seed;
@@ -49,6 +50,8 @@ typedef struct {
   float *log_x;
   int *log_c, *log_y, *log_n;
   double pole_cap;
+  int deg0;                 /* series degree before the choice (0: the program's) */
+  int degree[OTR_MAXCAND];  /* series degree of candidate c's step (0: the program's) */
 } otr_controller;
 
 /* paths p0 <= p < p1 of npaths; arrays as in phc_track_from (tstart NULL:

@@ -74,13 +74,17 @@ class PhcTrackParams(ctypes.Structure):
 
 
 
+CTL_MAXCAND = 16
+
+
 class PhcController(ctypes.Structure):
     _fields_ = [
-        ("mode", c_int), ("ncand", c_int), ("mult", c_double * 8),
+        ("mode", c_int), ("ncand", c_int), ("mult", c_double * CTL_MAXCAND),
         ("weights", c_void_p), ("mu", c_void_p), ("sd", c_void_p),
         ("logit_threshold", ctypes.c_float), ("seed", ctypes.c_uint), ("log_capacity", c_int),
         ("log_x", c_void_p), ("log_c", c_void_p), ("log_y", c_void_p), ("log_n", c_void_p),
         ("pole_cap", c_double),
+        ("deg0", c_int), ("degree", c_int * CTL_MAXCAND),
     ]
 
 

@@ -27,6 +27,19 @@ NFEAT, HIDDEN = 16, 64
 MULTS = tuple(float(2.0 ** (0.5 * i)) for i in range(6))
 
 
+def order_candidates(mults=MULTS, low=6):
+    """(multipliers, series degrees) of the joint step-size and
+    predictor-order candidates: every multiplier with the program's series
+    degree (0) and with degree `low`, the lower degree after the full one
+    of the same multiplier, since learned mode takes the last candidate that
+    clears the threshold (a step with fewer series orders costs less)."""
+    m, d = [], []
+    for x in mults:
+        m += [x, x]
+        d += [0, low]
+    return tuple(m), tuple(d)
+
+
 def nparams(F=NFEAT, H=HIDDEN, C=len(MULTS)):
     return F * H + H + H * H + H + H * C + C
 
@@ -113,9 +126,15 @@ class StepController:
     """Weights, input standardization and candidates of the controller."""
 
     def __init__(self, weights=None, mu=None, sd=None, threshold=0.8, mults=MULTS, seed=0,
-                 pole_cap=1.0):
+                 pole_cap=1.0, degrees=None, deg0=0):
         self.mults = tuple(mults)
         self.C = len(self.mults)
+        # series degree of every candidate's step (0: the program's) and the
+        # degree computed before the choice (0: the program's)
+        self.degrees = tuple(int(d) for d in (degrees or (0,) * self.C))
+        if len(self.degrees) != self.C:
+            raise ValueError("one series degree per candidate multiplier")
+        self.deg0 = int(deg0)
         rng = np.random.default_rng(seed)
         self.w = init_weights(rng, C=self.C) if weights is None else np.asarray(weights, np.float32)
         self.mu = np.zeros(NFEAT, np.float32) if mu is None else np.asarray(mu, np.float32)
@@ -128,13 +147,17 @@ class StepController:
         """weights, standardization, candidates, threshold and pole cap as
         an .npz file (plain arrays, no pickled objects)"""
         np.savez(path, w=self.w, mu=self.mu, sd=self.sd, mults=np.asarray(self.mults),
-                 threshold=np.float64(self.threshold), pole_cap=np.float64(self.pole_cap))
+                 threshold=np.float64(self.threshold), pole_cap=np.float64(self.pole_cap),
+                 degrees=np.asarray(self.degrees, np.int32), deg0=np.int32(self.deg0))
 
     @classmethod
     def load(cls, path):
         with np.load(path, allow_pickle=False) as z:
+            degrees = tuple(int(d) for d in z["degrees"]) if "degrees" in z else None
+            deg0 = int(z["deg0"]) if "deg0" in z else 0
             return cls(weights=z["w"], mu=z["mu"], sd=z["sd"], threshold=float(z["threshold"]),
-                       mults=tuple(float(m) for m in z["mults"]), pole_cap=float(z["pole_cap"]))
+                       mults=tuple(float(m) for m in z["mults"]), pole_cap=float(z["pole_cap"]),
+                       degrees=degrees, deg0=deg0)
 
     @property
     def logit_threshold(self):
@@ -152,6 +175,9 @@ class StepController:
         c.ncand = self.C
         for i, m in enumerate(self.mults):
             c.mult[i] = m
+        c.deg0 = self.deg0
+        for i, d in enumerate(self.degrees):
+            c.degree[i] = d
         if mode == 1:
             dw, dmu, dsd = self.device()
             c.weights, c.mu, c.sd = dw.ptr.value, dmu.ptr.value, dsd.ptr.value
@@ -218,12 +244,14 @@ def collect_logs(tracker, starts, params, seed=1, cap=256, K=1, controller=None)
     return logs.to_host() + (bufs.to_host(),)
 
 
-def collect_training_logs(trackers, starts, params, K=1, seed=1, cap=256):
+def collect_training_logs(trackers, starts, params, K=1, seed=1, cap=256, controller=None):
     """explore-mode logs of every (tracker, starts) pair, concatenated:
-    (features, candidates, Newton iterations or 0)"""
+    (features, candidates, Newton iterations or 0); controller: the
+    candidates to explore (default: the multipliers MULTS)"""
     X, C, Y = [], [], []
     for i, (tr, st) in enumerate(zip(trackers, starts)):
-        x, c, y, _ = collect_logs(tr, st, params, seed=seed + i, cap=cap, K=K)
+        x, c, y, _ = collect_logs(tr, st, params, seed=seed + i, cap=cap, K=K,
+                                  controller=controller)
         X.append(x)
         C.append(c)
         Y.append(y)
@@ -240,16 +268,20 @@ def step_labels(Y, max_newton=None):
 
 
 def train_controller(trackers, starts, params, K=1, seed=1, cap=256, epochs=30, batch=1024,
-                     lr=3e-3, threshold=0.8, max_newton=None, logs=None):
+                     lr=3e-3, threshold=0.8, max_newton=None, logs=None, mults=MULTS,
+                     degrees=None, deg0=0):
     """Explore-mode logs from every (tracker, starts) pair (or the given
-    logs), then GPU training on step_labels(., max_newton).
+    logs), then GPU training on step_labels(., max_newton).  The candidates
+    are the multipliers mults with the series degrees `degrees` (0: the
+    program's; deg0 the degree computed before the choice).
 
     Returns the trained StepController and a summary dict (samples,
     positive rate, first and last epoch loss)."""
+    explore = StepController(mults=mults, degrees=degrees, deg0=deg0)
     X, C, Y = logs if logs is not None else collect_training_logs(trackers, starts, params, K,
-                                                                  seed, cap)
+                                                                  seed, cap, explore)
     y = step_labels(Y, max_newton)
-    ctl = StepController(threshold=threshold, seed=seed)
+    ctl = StepController(threshold=threshold, seed=seed, mults=mults, degrees=degrees, deg0=deg0)
     losses = ctl.fit(X, C, y, epochs=epochs, batch=batch, lr=lr, seed=seed)
     return ctl, dict(samples=int(len(X)), accept_rate=float(y.mean()), max_newton=max_newton,
                      loss_first=float(losses[0]), loss_last=float(losses[-1]))

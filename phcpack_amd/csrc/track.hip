@@ -103,6 +103,10 @@ struct Ctl {
   float *log_x;
   int *log_c, *log_y, *log_n;
   double pcap;  // step cap as a fraction of the Fabry radius (0: none)
+  // series coefficients computed before the choice (d0) and for the step of
+  // candidate c (dc[c] >= d0), at most the program's deg + 1
+  int d0;
+  int dc[PHC_CTL_MAXCAND];
 };
 
 // log2 of x > 0 from the binary exponent and the series
@@ -996,15 +1000,11 @@ __device__ void store_point(const LProg &__restrict__ P, Path<K> &__restrict__ w
 // Taylor coefficients x_1..x_deg at the point in the order-0 slots (weights
 // set at t0) and *hs2, the Hessian sum at the point; returns false on a
 // singular Jacobian, else A holds its inverse
-template <int K> __device__ bool predictor_series(const LProg &__restrict__ P, Path<K> &__restrict__ w, double *hs2) {
+// Taylor coefficients of orders kb..ke-1 (the state after order kb-1 as
+// series_orders leaves it: coefficient kb zeroed, its products formed)
+template <int K> __device__ void series_orders(const LProg &__restrict__ P, Path<K> &__restrict__ w, int kb, int ke) {
   const int l = lane_id(), n = P.n, D = P.D;
-  eval_point<K>(P, w, true);
-  *hs2 = hessian_sum<K>(P, w);
-  if (!gj_invert<K>(P, w)) return false;
-  if (l < n) w.slots[l * D + 1] = TC<K>::zero();
-  wsync();
-  product_pass<K>(P, w, 1);
-  for (int k = 1; k < D; ++k) {
+  for (int k = kb; k < ke; ++k) {
     residual<K>(P, w, k);
     const cplx<K> xk = apply_inverse<K>(P, w);
     if (l < n * P.G && (l % P.G) == 0) w.slots[(l / P.G) * D + k] = xk;
@@ -1014,20 +1014,65 @@ template <int K> __device__ bool predictor_series(const LProg &__restrict__ P, P
     // phase reads
     if (k + 1 < D) correct_and_product<K>(P, w, k);
   }
+}
+
+// Fabry's pole radius |c_{kd-1} / c_kd| (leading limbs, FP64) minimized
+// over the components whose c_kd is not zero (Is_Zero: both parts vanish
+// next to 1), the first of them starting the minimum; *pfail when every
+// c_kd is zero.  mr: n entries of LDS scratch.
+template <int K>
+__device__ __forceinline__ void fabry_radius(const LProg &__restrict__ P, Path<K> &__restrict__ w, int kd,
+                                             LDS double2 *mr, bool *pfail, double *r) {
+  const int l = lane_id(), n = P.n, D = P.D;
+  if (l < n) {
+    const double2 a = lead<K>(w.slots[l * D + kd - 1]), b = lead<K>(w.slots[l * D + kd]);
+    double rad = -1.0;
+    if (!(b.x + 1.0 == 1.0 && b.y + 1.0 == 1.0)) {
+      const double nrm = b.x * b.x + b.y * b.y;
+      const double zr = (a.x * b.x + a.y * b.y) / nrm, zi = (a.y * b.x - a.x * b.y) / nrm;
+      rad = zrad(zr, zi);
+    }
+    mr[l].x = rad;
+  }
+  wsync();
+  bool fail = true;
+  double m = 0.0;
+  for (int j = 0; j < n; ++j) {
+    const double v = mr[j].x;
+    if (v == -1.0) continue;
+    if (fail) { m = v; fail = false; }
+    else if (v < m) m = v;
+  }
+  wsync();
+  *pfail = fail;
+  *r = m;
+}
+
+// the point's value, Jacobian (inverted), Hessian sum, and the Taylor
+// coefficients of orders 1..ke-1 (ke <= D; series_orders continues them)
+template <int K> __device__ bool predictor_series(const LProg &__restrict__ P, Path<K> &__restrict__ w, double *hs2, int ke) {
+  const int l = lane_id(), n = P.n, D = P.D;
+  eval_point<K>(P, w, true);
+  *hs2 = hessian_sum<K>(P, w);
+  if (!gj_invert<K>(P, w)) return false;
+  if (l < n) w.slots[l * D + 1] = TC<K>::zero();
+  wsync();
+  product_pass<K>(P, w, 1);
+  series_orders<K>(P, w, 1, ke);
   return true;
 }
 
 // features of the current step (identical in every lane)
 template <int K>
 __device__ void step_features(const LProg &__restrict__ P, Path<K> &__restrict__ w, double rad, double maxstep, double hprev,
-                              double t, const double *dxh, int rejprev, double *f) {
+                              double t, const double *dxh, int rejprev, int Dk, double *f) {
   const int l = lane_id(), n = P.n, D = P.D;
   f[0] = lg2(rad);
   f[1] = lg2(hprev);
   f[2] = t;
   for (int k = 1; k <= 8; ++k) {
     double c = 0.0;
-    if (k < D && l < n) c = absz<K>(w.slots[l * D + k]);
+    if (k < Dk && l < n) c = absz<K>(w.slots[l * D + k]);
     f[2 + k] = lg2(wave_max(c) + 1e-300);
   }
   for (int i = 0; i < 4; ++i) f[11 + i] = lg2(dxh[i] + 1e-300);
@@ -1235,36 +1280,20 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, const Para
       set_weights<K>(P, w, t);
       ++npred;
       double hs2;
-      if (!predictor_series<K>(P, w, &hs2)) { st = ST_SINGULAR; break; }
+      // the controller chooses after d0 coefficients; its candidate's
+      // series degree may add orders (the plain rule: all D at once)
+      int Dk = (pmode != 0) ? ctl.d0 : D;
+      if (!predictor_series<K>(P, w, &hs2, Dk)) { st = ST_SINGULAR; break; }
       // ---- pole step: Fabry's ratio, minimized over the components whose
       // c_kd is not zero (Is_Zero), the first of them starting the minimum
-      const int kd = D - 1 - par.fabry_offset;
-      if (l < n) {
-        const double2 a = lead<K>(w.slots[l * D + kd - 1]), b = lead<K>(w.slots[l * D + kd]);
-        double rad = -1.0;
-        if (!(b.x + 1.0 == 1.0 && b.y + 1.0 == 1.0)) {
-          const double nrm = b.x * b.x + b.y * b.y;
-          const double zr = (a.x * b.x + a.y * b.y) / nrm, zi = (a.y * b.x - a.x * b.y) / nrm;
-          rad = zrad(zr, zi);
-        }
-        mr[l].x = rad;
-      }
-      wsync();
       bool pfail = true;
       double r = 0.0;
-      for (int j = 0; j < n; ++j) {
-        const double v = mr[j].x;
-        if (v == -1.0) continue;
-        if (pfail) { r = v; pfail = false; }
-        else if (v < r) r = v;
-      }
-      wsync();
-      const double pole = pfail ? par.maxstep : par.pbeta * r;
-      // ---- step-size controller (reads the Taylor series) ----
+      fabry_radius<K>(P, w, Dk - 1 - par.fabry_offset, mr, &pfail, &r);
+      // ---- step-size and predictor-order controller (reads the Taylor series) ----
       int cand = 0;
       if (pmode != 0) {
         double feat[PHC_CTL_NFEAT];
-        step_features<K>(P, w, pfail ? par.maxstep : r, par.maxstep, hprev, t, dxh, rejprev, feat);
+        step_features<K>(P, w, pfail ? par.maxstep : r, par.maxstep, hprev, t, dxh, rejprev, Dk, feat);
         if (pmode == 1) cand = mlp_choose<K>(ctl, w, feat);
         else cand = (int)(mix_hash(ctl.seed, (unsigned)path, (unsigned)(npred - 1)) % (unsigned)ctl.ncand);
         // the features and the choice are logged now (the outcome after the
@@ -1275,12 +1304,22 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, const Para
             ctl.log_x[((size_t)path * ctl.cap + nlog) * PHC_CTL_NFEAT + i] = (float)feat[i];
           ctl.log_c[(size_t)path * ctl.cap + nlog] = cand;
         }
+        if (ctl.dc[cand] > Dk) {
+          series_orders<K>(P, w, Dk, ctl.dc[cand]);
+          Dk = ctl.dc[cand];
+          pfail = true;
+          r = 0.0;
+          fabry_radius<K>(P, w, Dk - 1 - par.fabry_offset, mr, &pfail, &r);
+        }
       }
+      const double pole = pfail ? par.maxstep : par.pbeta * r;
+      // the [L/PD] predictor of a series of Dk coefficients
+      const int pnum = par.pnum - (D - Dk);
       const int rej0 = rej, nnewt0 = nnewt;
       // ---- predictor coefficients and their error estimates ----
       if (l < n) {
         w.x0[l] = w.slots[l * D];
-        mr[l] = pade_with_error<K, PD>(w.slots + l * D, par.pnum, w.x0 + l);
+        mr[l] = pade_with_error<K, PD>(w.slots + l * D, pnum, w.x0 + l);
       }
       wsync();
       double nrm = 0.0;
@@ -1295,7 +1334,7 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, const Para
             reinterpret_cast<LDS double2 *>(w.u), reinterpret_cast<LDS double2 *>(w.u) + n * n, mr);
         const double sig1 = 1.0 / sqrt(lam);
         const double eta = (2.0 * sig1) / sqrt(hs2);
-        const double curv = kth_root((par.cbeta * eta) / nrm, D - 1);
+        const double curv = kth_root((par.cbeta * eta) / nrm, Dk - 1);
         // Minimum (three_way_minima.adb:49-62)
         if (pole < curv) {
           if (pole < par.maxstep) { step = pole; ++nbpole; }
@@ -1314,7 +1353,7 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, const Para
       // ---- Predictor_Feedback (standard_predictor_convolutions.adb:958-993) ----
       bool evaluated = false;
       for (;;) {
-        if (l < n) w.slots[l * D] = pade_value<K, PD>(w.slots + l * D, par.pnum, w.x0[l], step);
+        if (l < n) w.slots[l * D] = pade_value<K, PD>(w.slots + l * D, pnum, w.x0[l], step);
         wsync();
         set_weights<K>(P, w, tn);
         ++nfeval;
@@ -1348,7 +1387,7 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, const Para
           break;
         }
         tn = t + step;
-        if (l < n) w.slots[l * D] = pade_value<K, PD>(w.slots + l * D, par.pnum, w.x0[l], step);
+        if (l < n) w.slots[l * D] = pade_value<K, PD>(w.slots + l * D, pnum, w.x0[l], step);
         wsync();
         evaluated = false;
       }
@@ -1490,7 +1529,7 @@ __global__ void __launch_bounds__(64 * REFINE_MAX_WAVES) endgame_kernel(DevProgr
     const double lamJ =
         sigma_max_sq(n, [&](int a, int b) { return lead<K>(w.A[a * n + b]); }, U, U + n * n, Y);
     double hs2, rco = 0.0;
-    if (predictor_series<K>(P, w, &hs2)) {
+    if (predictor_series<K>(P, w, &hs2, D)) {
       const double lamI =
           sigma_max_sq(n, [&](int a, int b) { return lead<K>(w.A[a * n + b]); }, U, U + n * n, Y);
       rco = 1.0 / sqrt(lamJ * lamI);
@@ -1847,7 +1886,11 @@ int phc_track_from(phc_tracker *tr, int K, int npaths, const double *start, int 
   Ctl ctl;
   ctl.mode = 0; ctl.ncand = 1; ctl.cap = 0; ctl.w = ctl.mu = ctl.sd = nullptr; ctl.thresh = 0.0f;
   ctl.seed = 0; ctl.log_x = nullptr; ctl.log_c = ctl.log_y = ctl.log_n = nullptr; ctl.pcap = 0.0;
-  for (int c = 0; c < PHC_CTL_MAXCAND; ++c) ctl.mult[c] = 1.0;
+  for (int c = 0; c < PHC_CTL_MAXCAND; ++c) {
+    ctl.mult[c] = 1.0;
+    ctl.dc[c] = tr->P.D;
+  }
+  ctl.d0 = tr->P.D;
   if (ctlp && ctlp->mode != 0) {
     PHC_REQUIRE(ctlp->mode == 1 || ctlp->mode == 2, "phc_track: controller mode must be 0, 1 or 2");
     PHC_REQUIRE(ctlp->ncand >= 1 && ctlp->ncand <= PHC_CTL_MAXCAND, "phc_track: bad ncand");
@@ -1866,6 +1909,20 @@ int phc_track_from(phc_tracker *tr, int K, int npaths, const double *start, int 
     ctl.log_y = ctlp->log_y; ctl.log_n = ctlp->log_n;
     PHC_REQUIRE(ctlp->pole_cap >= 0.0, "phc_track: pole_cap must be >= 0");
     ctl.pcap = ctlp->pole_cap;
+    // series degrees: before the choice and per candidate (0: the program's)
+    const int deg = tr->P.D - 1;
+    auto usable = [&](int d) {
+      return d >= 1 && d <= deg && d - par.fabry_offset >= 1 && par.pnum - (deg - d) >= 1;
+    };
+    const int d0 = ctlp->deg0 ? ctlp->deg0 : deg;
+    PHC_REQUIRE(usable(d0), "phc_track: controller deg0 out of range (Fabry offset, Pade degrees)");
+    ctl.d0 = d0 + 1;
+    for (int c = 0; c < ctlp->ncand; ++c) {
+      const int dcand = ctlp->degree[c] ? ctlp->degree[c] : deg;
+      PHC_REQUIRE(usable(dcand) && dcand >= d0,
+                  "phc_track: candidate series degree out of range (deg0..program degree)");
+      ctl.dc[c] = dcand + 1;
+    }
   }
   if (tracker_enter(tr, stream) != 0) return -1;
   PHC_HIP(hipMemsetAsync(tr->counter, 0, sizeof(int), st));

@@ -130,6 +130,9 @@ class OracleController:
         c.mode, c.ncand = mode, ctl.C
         for i, m in enumerate(ctl.mults):
             c.mult[i] = m
+        c.deg0 = ctl.deg0
+        for i, d in enumerate(ctl.degrees):
+            c.degree[i] = d
         for name, arr in (("weights", ctl.w), ("mu", ctl.mu), ("sd", ctl.sd)):
             a = np.ascontiguousarray(arr, np.float32)
             self.keep.append(a)
@@ -184,7 +187,7 @@ def oracle_track(prog, starts_planes, K, params, nthreads=1, paths=None, control
 
 TRACE_FIELDS = ("t", "r", "pole", "curv", "eta", "nrm", "which", "rule", "step1", "step",
                 "nfb", "ncr", "mix", "nit", "accepted", "sigma1", "hs2", "t_after", "cand",
-                "unused")
+                "degree")
 
 
 def oracle_track_trace(prog, starts_planes, K, params, path=0, controller=None, cap=2000):

@@ -152,3 +152,57 @@ def test_trained_controller_gains_over_plain_rule():
         steps_learned += int(b["work"][0].sum())
     print(f"predictor steps plain {steps_plain} learned {steps_learned}", flush=True)
     assert steps_learned < 0.85 * steps_plain
+
+
+@pytest.mark.parametrize("K", [1, 2])
+def test_order_candidates_match_oracle(K):
+    """joint (multiplier, series degree) candidates, explore and learned
+    mode: the kernel's paths and step logs equal the oracle's bit for bit"""
+    from phcpack_amd.controller import StepController, StepLogs, order_candidates
+    from phcpack_amd.tracker import Tracker, TrackParams, TrackBuffers, points_to_planes
+    from phcpack_amd._lib import DeviceArray, check, load
+    hom, sols = _system(4, 2, 12)
+    tr = Tracker(hom, deg=8)
+    prm = TrackParams(tolend=1e-13 if K == 1 else 1e-28)
+    mults, degrees = order_candidates(low=5)
+    ctl = StepController(seed=4, threshold=0.5, mults=mults, degrees=degrees, deg0=5)
+    logs = StepLogs(len(sols), 256)
+    bufs = TrackBuffers(len(sols), 4, K)
+    d = DeviceArray.from_host(points_to_planes(sols))
+    tr.track_device(d, 1, len(sols), K, bufs, prm, controller=ctl.struct(2, logs, seed=13))
+    check(load().phc_synchronize())
+    g = bufs.to_host()
+    oc = OracleController(2, len(sols), ctl=ctl, seed=13, cap=256)
+    o = oracle_track(tr.prog, points_to_planes(sols), K, prm, controller=oc)
+    for k in ("endp", "status", "nsteps", "nrej", "work", "tend"):
+        assert np.array_equal(g[k], o[k]), k
+    gx, gc, gy = logs.to_host()
+    ox, oc_, oy = oc.logs()
+    assert np.array_equal(gx, ox) and np.array_equal(gc, oc_) and np.array_equal(gy, oy)
+    assert len(set(gc.tolist())) > 6
+    g = tr.track(sols, K=K, params=prm, controller=ctl.struct(1))
+    o = oracle_track(tr.prog, points_to_planes(sols), K, prm,
+                     controller=OracleController(1, len(sols), ctl=ctl))
+    for k in ("endp", "status", "nsteps", "nrej", "work", "tend"):
+        assert np.array_equal(g[k], o[k]), k
+
+
+def test_low_degree_candidate_is_the_rule_at_that_degree():
+    """one candidate (1, degree 6) after deg0 = 6 on a degree-8 program
+    tracks every path as the plain rule on the degree-6 program with the
+    [3/1] predictor, bit for bit, on the GPU"""
+    from phcpack_amd.controller import StepController, StepLogs
+    from phcpack_amd.tracker import Tracker, TrackParams, TrackBuffers, points_to_planes
+    from phcpack_amd._lib import DeviceArray, check, load
+    hom, sols = _system(4, 2, 14)
+    tr8, tr6 = Tracker(hom, deg=8), Tracker(hom, deg=6)
+    ctl = StepController(mults=(1.0,), degrees=(6,), deg0=6, pole_cap=0.0)
+    logs = StepLogs(len(sols), 8)
+    bufs = TrackBuffers(len(sols), 4, 2)
+    d = DeviceArray.from_host(points_to_planes(sols))
+    tr8.track_device(d, 1, len(sols), 2, bufs, TrackParams(), controller=ctl.struct(2, logs, seed=1))
+    check(load().phc_synchronize())
+    g = bufs.to_host()
+    ref = tr6.track(sols, K=2, params=TrackParams(pade_numdeg=3, pade_dendeg=1))
+    for k in ("endp", "status", "nsteps", "nrej", "work", "tend"):
+        assert np.array_equal(g[k], ref[k]), k
