@@ -315,37 +315,55 @@ def main():
     return run(args)
 
 
-def train_and_tune(trackers, starts, tune_tr, tune_sols, prm, ev, epochs=25):
-    """Explore-mode logs of the training systems, then two networks, one per
-    label (accepted at the first attempt; accepted within 2 Newton
-    iterations), each with its threshold and pole cap tuned by time on the
-    tuning system; the faster one is kept.  Returns (controller or None,
-    summary)."""
-    from phcpack_amd.controller import collect_training_logs, train_controller, tune_threshold
+def candidate_sets(orders=True):
+    """the candidate sets the controller is trained with: step multipliers
+    only (series degree 8), and, with orders, the joint step-size and
+    predictor-order candidates: every multiplier with degree 8 or 6, the
+    choice made after the series to degree 6"""
+    from phcpack_amd.controller import MULTS, order_candidates
+    sets = [("multiplier", MULTS, None, 0)]
+    if orders:
+        m, d = order_candidates(low=6)
+        sets.append(("multiplier+order", m, d, 6))
+    return sets
+
+
+def train_and_tune(trackers, starts, tune_tr, tune_sols, prm, ev, epochs=25, orders=True):
+    """For every candidate set: explore-mode logs of the training systems,
+    then two networks, one per label (accepted at the first attempt;
+    accepted within 2 Newton iterations), each with its threshold and pole
+    cap tuned by time on the tuning system; the fastest network is kept.
+    Returns (controller or None, summary)."""
+    from phcpack_amd.controller import (StepController, collect_training_logs, train_controller,
+                                        tune_threshold)
 
     def timer(fn):
         ev.start()
         fn()
         return ev.stop_ms()
 
-    logs = collect_training_logs(trackers, starts, prm, K=1, seed=7)
     best_ctl, best_ms, runs = None, None, []
-    for mn in (None, 2):
-        ctl, info = train_controller(trackers, starts, prm, K=1, seed=7, epochs=epochs,
-                                     max_newton=mn, logs=logs)
-        best, table = tune_threshold(ctl, tune_tr, tune_sols, prm, K=1, timer=timer)
-        ms = table[f"{best[0]}/{best[1]}"]["ms"] if best is not None else None
-        info.update(threshold=best[0] if best else None, pole_cap=best[1] if best else None,
-                    tuned_ms=ms, tune_table={str(k): v for k, v in table.items()})
-        runs.append(info)
-        if ms is not None and (best_ms is None or ms < best_ms):
-            best_ctl, best_ms = ctl, ms
+    for name, mults, degrees, deg0 in candidate_sets(orders):
+        explore = StepController(mults=mults, degrees=degrees, deg0=deg0)
+        logs = collect_training_logs(trackers, starts, prm, K=1, seed=7, controller=explore)
+        for mn in (None, 2):
+            ctl, info = train_controller(trackers, starts, prm, K=1, seed=7, epochs=epochs,
+                                         max_newton=mn, logs=logs, mults=mults, degrees=degrees,
+                                         deg0=deg0)
+            best, table = tune_threshold(ctl, tune_tr, tune_sols, prm, K=1, timer=timer)
+            ms = table[f"{best[0]}/{best[1]}"]["ms"] if best is not None else None
+            info.update(candidates=name, threshold=best[0] if best else None,
+                        pole_cap=best[1] if best else None, tuned_ms=ms,
+                        tune_table={str(k): v for k, v in table.items()})
+            runs.append(info)
+            if ms is not None and (best_ms is None or ms < best_ms):
+                best_ctl, best_ms = ctl, ms
     chosen = min((r for r in runs if r["tuned_ms"] is not None), key=lambda r: r["tuned_ms"],
                  default=None)
     summary = dict(chosen or runs[0])
-    summary["candidates"] = [{k: r[k] for k in ("max_newton", "accept_rate", "loss_last",
-                                                "threshold", "pole_cap", "tuned_ms")}
-                             for r in runs]
+    summary["networks"] = [{k: r[k] for k in ("candidates", "max_newton", "accept_rate",
+                                              "loss_last", "threshold", "pole_cap", "tuned_ms")}
+                           for r in runs]
     return best_ctl, summary
 
 
