@@ -676,6 +676,17 @@ def run_qr(args, n=1024, szt=128, K=4, seed=SEED):
     # and b padded with zero limbs, r = A x - b by the octo-double complex
     # GEMM, and x - x0 (exact in octo-double)
     od = octo_double_checks(lib, K, n, dA, dx, db, dx0)
+    # the real QR of the real part of A with the same plan (reported beside
+    # the metric, not part of it)
+    dAr = DeviceArray.from_host(np.ascontiguousarray(A[:K]))
+    dQr, dRr = DeviceArray((K, n, n)), DeviceArray((K, n, n))
+    qr.factor_real_device(dAr, dQr, dRr)
+    check(lib.phc_synchronize())
+    t_real = []
+    for _ in range(max(args.steps, 1)):
+        ev.start()
+        qr.factor_real_device(dAr, dQr, dRr)
+        t_real.append(ev.stop_ms())
     ms_qr, ms = float(np.mean(t_qr)), float(np.mean(t_all))
     executed = baqr_work(n, n, szt) * complex_mac(K)
     cm, ca = baqr_canonical(n)
@@ -688,6 +699,7 @@ def run_qr(args, n=1024, szt=128, K=4, seed=SEED):
         "config": {"workload": f"complex quad-double {n}x{n}, {n // szt} tiles of {szt} columns, "
                                "entries uniform in [-1,1], b = A x0", "seed": seed},
         "qr_ms": round(ms_qr, 3),
+        "real_qr_ms": round(float(np.mean(t_real)), 3),
         "tuning": {"gemm_block": QR_BLOCKS[best[1]], "split_target": best[2],
                    "qr_ms_by_block_and_split": {f"{QR_BLOCKS[b]}/{sp}": round(v, 2)
                                                 for (_, b, sp), v in sorted(table.items())}},

@@ -487,6 +487,20 @@ int phc_cmplx_blocked_houseqr(phc_baqr *plan, const double *A_dev, double *Q_dev
 int phc_cmplx_qhb(phc_baqr *plan, const double *Q_dev, const double *b_dev, double *y_dev,
                   void *stream);
 
+/* The real blocked QR with the same plan: A [K][nrows][ncols], Q
+ * [K][nrows][nrows], R [K][nrows][ncols] real limb-planar (K planes).
+ * CPU_dbl4_blocked_houseqr (Matrices/dbl4_baqr_host.cpp:1033), replaces
+ * GPU_dbl4_blocked_houseqr (Matrices/dbl4_baqr_kernels.cu:5746) and its dd
+ * and od twins: the real Householder vector of CPU_dbl4_factors_house
+ * (v0 = x0 - |x| or -sigma/(x0 + |x|), so R has a positive diagonal); the
+ * panel kernels run on the plan's complex workspace with zero imaginary
+ * parts, the matrix products on the real planes only.
+ * phc_real_qtb: y = Q^T b (b, y real, length nrows). */
+int phc_real_blocked_houseqr(phc_baqr *plan, const double *A_dev, double *Q_dev, double *R_dev,
+                             void *stream);
+int phc_real_qtb(phc_baqr *plan, const double *Q_dev, const double *b_dev, double *y_dev,
+                 void *stream);
+
 /* U x = b for upper triangular U (dim = szt*nbt): the diagonal tiles are
  * inverted, then the tiles are applied from the last one up.  As in the
  * reference, U's diagonal tiles are replaced by their inverses and b is

@@ -237,6 +237,39 @@ void GPU_cmplx8_product(
  * Matrices/dbl2_baqr_kernels.h:2540 and dbl4_baqr_kernels.h:3739:
  * A (nrows x ncols, nrows >= ncols = szt*nbt) = Q R with Q (nrows x nrows)
  * and R (nrows x ncols). */
+/* Matrices/dbl2_baqr_kernels.h:2479, dbl4_baqr_kernels.h:3668,
+ * dbl8_baqr_kernels.h:4787: the real blocked QR (phc_real_blocked_houseqr),
+ * R with a positive diagonal as the reference's real Householder vector
+ * gives; the timing arguments other than walltimesec are left unchanged. */
+void GPU_dbl2_blocked_houseqr(int nrows, int ncols, int szt, int nbt, double **Ahi, double **Alo,
+                              double **Qhi, double **Qlo, double **Rhi, double **Rlo,
+                              double *houselapms, double *RTvlapms, double *tileRlapms,
+                              double *vb2Wlapms, double *WYTlapms, double *QWYTlapms,
+                              double *Qaddlapms, double *YWTlapms, double *YWTClapms,
+                              double *Raddlapms, double *walltimesec, long long int *addcnt,
+                              long long int *mulcnt, long long int *divcnt,
+                              long long int *sqrtcnt, bool verbose = true);
+void GPU_dbl4_blocked_houseqr(int nrows, int ncols, int szt, int nbt, double **Ahihi,
+                              double **Alohi, double **Ahilo, double **Alolo, double **Qhihi,
+                              double **Qlohi, double **Qhilo, double **Qlolo, double **Rhihi,
+                              double **Rlohi, double **Rhilo, double **Rlolo,
+                              double *houselapms, double *RTvlapms, double *tileRlapms,
+                              double *vb2Wlapms, double *WYTlapms, double *QWYTlapms,
+                              double *Qaddlapms, double *YWTlapms, double *YWTClapms,
+                              double *Raddlapms, double *walltimesec, long long int *addcnt,
+                              long long int *mulcnt, long long int *divcnt,
+                              long long int *sqrtcnt, bool verbose = true);
+void GPU_dbl8_blocked_houseqr(
+    int nrows, int ncols, int szt, int nbt, double **Ahihihi, double **Alohihi, double **Ahilohi,
+    double **Alolohi, double **Ahihilo, double **Alohilo, double **Ahilolo, double **Alololo,
+    double **Qhihihi, double **Qlohihi, double **Qhilohi, double **Qlolohi, double **Qhihilo,
+    double **Qlohilo, double **Qhilolo, double **Qlololo, double **Rhihihi, double **Rlohihi,
+    double **Rhilohi, double **Rlolohi, double **Rhihilo, double **Rlohilo, double **Rhilolo,
+    double **Rlololo, double *houselapms, double *RTvlapms, double *tileRlapms,
+    double *vb2Wlapms, double *WYTlapms, double *QWYTlapms, double *Qaddlapms, double *YWTlapms,
+    double *YWTClapms, double *Raddlapms, double *walltimesec, long long int *addcnt,
+    long long int *mulcnt, long long int *divcnt, long long int *sqrtcnt, bool verbose = true);
+
 void GPU_cmplx2_blocked_houseqr(
     int nrows, int ncols, int szt, int nbt, double **Arehi, double **Arelo, double **Aimhi,
     double **Aimlo, double **Qrehi, double **Qrelo, double **Qimhi, double **Qimlo,

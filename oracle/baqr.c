@@ -17,6 +17,10 @@
  *   tile inverse, tiled solver  CPU_cmplx2_upper_inverse, CPU_cmplx2_upper_tiled_solver
  *                               (Matrices/dbl2_tabs_host.cpp:65-106, 289-513)
  *   Q^H b                       CPU_cmplx2_factors_qrbs (Matrices/dbl2_factorizations.cpp:842-879)
+ *   real blocked QR             CPU_dbl4_blocked_houseqr (Matrices/dbl4_baqr_host.cpp:1033-1205)
+ *     with CPU_dbl4_factors_house, CPU_dbl4_factors_leftRupdate
+ *     (Matrices/dbl4_factorizations.cpp:541-626, 788-832), CPU_dbl4_blocked_VB_to_W,
+ *     _leftRupdate, _rightQupdate (dbl4_baqr_host.cpp:13-86, 242-356, 586-742)
  *
  * Parity pinned by tests/test_oracle_baqr.py: bit-exact against the
  * reference functions compiled in oracle/_ref (K = 2, 4, 8), and Q^H Q = I,
@@ -515,4 +519,209 @@ void obq_matvec(int K, int m, int n, const double *A, const double *x, double *y
   free(a);
   free(xx);
   free(yy);
+}
+
+/* ---------------------------------------------------------------- real */
+
+/* real number i of an array: K limbs at a + i*K */
+#define RL(a, i) ((a) + (size_t)(i) * K)
+
+static void re_house(int K, int n, const double *x, double *v, double *beta) {
+  double sigma[MAXK], acc[MAXK], mu[MAXK], v0p2[MAXK];
+  int i;
+  omd_zero(K, sigma);
+  omd_from_d(K, 1.0, RL(v, 0));
+  for (i = 1; i < n; i++) {
+    omd_sqr(K, RL(x, i), acc);
+    omd_add(K, sigma, acc, sigma);
+    memcpy(RL(v, i), RL(x, i), sizeof(double) * K);
+  }
+  if (all_zero(K, sigma)) {
+    omd_zero(K, beta);
+    return;
+  }
+  omd_sqr(K, RL(x, 0), acc);
+  omd_add(K, acc, sigma, acc);
+  omd_sqrt(K, acc, mu);
+  if (RL(x, 0)[0] <= 0.0) {
+    omd_sub(K, RL(x, 0), mu, RL(v, 0));
+  } else {
+    omd_add(K, RL(x, 0), mu, acc);
+    omd_div(K, sigma, acc, RL(v, 0));
+    omd_neg(K, RL(v, 0), RL(v, 0));
+  }
+  omd_sqr(K, RL(v, 0), v0p2);
+  omd_add(K, sigma, v0p2, acc);
+  omd_div(K, v0p2, acc, beta);
+  omd_mul_d(K, beta, 2.0, beta);
+  for (i = 1; i < n; i++) omd_div(K, RL(v, i), RL(v, 0), RL(v, i));
+  omd_from_d(K, 1.0, RL(v, 0));
+}
+
+/* R (nrows x ldr) columns k..ncols-1 from row k: R -= v (beta v^T R) */
+static void re_left_r_update(int K, int nrows, int ncols, int ldr, int k, double *R,
+                             const double *v, const double *beta, double *w) {
+  double acc[MAXK];
+  int i, j;
+  for (j = k; j < ncols; j++) {
+    double *wj = RL(w, j - k);
+    omd_zero(K, wj);
+    for (i = k; i < nrows; i++) mac(K, RL(R, (size_t)i * ldr + j), RL(v, i - k), wj);
+    omd_mul(K, beta, wj, acc);
+    memcpy(wj, acc, sizeof(double) * K);
+  }
+  for (i = k; i < nrows; i++)
+    for (j = k; j < ncols; j++) msb(K, RL(v, i - k), RL(w, j - k), RL(R, (size_t)i * ldr + j));
+}
+
+static void re_vb_to_w(int K, int rowdim, int szt, const double *B, const double *V, double *W,
+                       double *p) {
+  double z[MAXK];
+  int i, j, k;
+  for (i = 0; i < rowdim; i++) {
+    omd_mul(K, B, RL(V, i), RL(W, i));
+    omd_neg(K, RL(W, i), RL(W, i));
+  }
+  for (j = 1; j < szt; j++) {
+    const double *vj = RL(V, (size_t)j * rowdim);
+    for (k = 0; k < j; k++) {
+      omd_zero(K, RL(p, k));
+      for (i = 0; i < rowdim; i++) mac(K, RL(V, (size_t)k * rowdim + i), RL(vj, i), RL(p, k));
+    }
+    for (i = 0; i < rowdim; i++) {
+      double *wji = RL(W, (size_t)j * rowdim + i);
+      omd_zero(K, z);
+      for (k = 0; k < j; k++) mac(K, RL(W, (size_t)k * rowdim + i), RL(p, k), z);
+      omd_add(K, z, RL(vj, i), z);
+      omd_mul(K, B + (size_t)j * K, z, wji);
+      omd_neg(K, wji, wji);
+    }
+  }
+}
+
+/* C = C + (Y W^T) C on rows rowoff.., columns coloff.. */
+static void re_blocked_left_r(int K, int nrows, int ncols, int szt, int idx, double *C,
+                              const double *Y, const double *W) {
+  const int rowoff = idx * szt, rowdim = nrows - rowoff, coloff = (idx + 1) * szt;
+  const int coldim = ncols - coloff;
+  double *ywt = malloc(sizeof(double) * K * (size_t)rowdim * rowdim);
+  double *prd = malloc(sizeof(double) * K * (size_t)rowdim * (coldim > 0 ? coldim : 1));
+  int i, j, k;
+  for (i = 0; i < rowdim; i++)
+    for (j = 0; j < rowdim; j++) {
+      double *e = RL(ywt, (size_t)i * rowdim + j);
+      omd_zero(K, e);
+      for (k = 0; k < szt; k++)
+        mac(K, RL(Y, (size_t)k * rowdim + i), RL(W, (size_t)k * rowdim + j), e);
+    }
+  for (i = 0; i < rowdim; i++)
+    for (j = 0; j < coldim; j++) {
+      double *e = RL(prd, (size_t)i * coldim + j);
+      omd_zero(K, e);
+      for (k = 0; k < rowdim; k++)
+        mac(K, RL(ywt, (size_t)i * rowdim + k), RL(C, (size_t)(rowoff + k) * ncols + coloff + j), e);
+    }
+  for (i = 0; i < rowdim; i++)
+    for (j = 0; j < coldim; j++) {
+      double *c = RL(C, (size_t)(rowoff + i) * ncols + coloff + j);
+      omd_add(K, c, RL(prd, (size_t)i * coldim + j), c);
+    }
+  free(ywt);
+  free(prd);
+}
+
+/* Q = Q + Q (W Y^T) on columns coloff.. */
+static void re_blocked_right_q(int K, int dim, int szt, int idx, double *Q, const double *Y,
+                               const double *W) {
+  const int coloff = idx * szt, rowdim = dim - coloff;
+  double *wyt = malloc(sizeof(double) * K * (size_t)rowdim * rowdim);
+  double *prd = malloc(sizeof(double) * K * (size_t)dim * rowdim);
+  int i, j, k;
+  for (i = 0; i < rowdim; i++)
+    for (j = 0; j < rowdim; j++) {
+      double *e = RL(wyt, (size_t)i * rowdim + j);
+      omd_zero(K, e);
+      for (k = 0; k < szt; k++)
+        mac(K, RL(W, (size_t)k * rowdim + i), RL(Y, (size_t)k * rowdim + j), e);
+    }
+  for (i = 0; i < dim; i++)
+    for (j = 0; j < rowdim; j++) {
+      double *e = RL(prd, (size_t)i * rowdim + j);
+      omd_zero(K, e);
+      for (k = 0; k < rowdim; k++)
+        mac(K, RL(Q, (size_t)i * dim + coloff + k), RL(wyt, (size_t)k * rowdim + j), e);
+    }
+  for (i = 0; i < dim; i++)
+    for (j = 0; j < rowdim; j++) {
+      double *q = RL(Q, (size_t)i * dim + coloff + j);
+      omd_add(K, q, RL(prd, (size_t)i * rowdim + j), q);
+    }
+  free(wyt);
+  free(prd);
+}
+
+static void re_blocked_houseqr(int K, int nrows, int ncols, int szt, int nbt, const double *A,
+                               double *Q, double *R) {
+  const size_t z = sizeof(double) * K;
+  double *x = malloc(z * nrows), *v = malloc(z * nrows), *w = malloc(z * (nrows + ncols));
+  double *B = malloc(z * szt);
+  double *Y = malloc(z * (size_t)szt * nrows), *W = malloc(z * (size_t)szt * nrows);
+  double *p = malloc(z * szt);
+  double beta[MAXK];
+  int i, j, k, L;
+  for (i = 0; i < nrows; i++) {
+    for (j = 0; j < nrows; j++) memset(RL(Q, (size_t)i * nrows + j), 0, z);
+    omd_from_d(K, 1.0, RL(Q, (size_t)i * nrows + i));
+    for (j = 0; j < ncols; j++) memcpy(RL(R, (size_t)i * ncols + j), RL(A, (size_t)i * ncols + j), z);
+  }
+  for (k = 0; k < nbt; k++) {
+    const int rowdim = nrows - k * szt;
+    for (L = 0; L < szt; L++) {
+      const int colidx = k * szt + L, endcol = (k + 1) * szt, nrowscol = nrows - colidx;
+      for (i = colidx; i < nrows; i++) memcpy(RL(x, i - colidx), RL(R, (size_t)i * ncols + colidx), z);
+      re_house(K, nrowscol, x, v, beta);
+      re_left_r_update(K, nrows, endcol, ncols, colidx, R, v, beta, w);
+      memcpy(B + (size_t)L * K, beta, z);
+      for (i = 0; i < L; i++) memset(RL(Y, (size_t)L * rowdim + i), 0, z);
+      for (i = 0; i < nrowscol; i++) memcpy(RL(Y, (size_t)L * rowdim + i + L), RL(v, i), z);
+    }
+    re_vb_to_w(K, rowdim, szt, B, Y, W, p);
+    if (k < nbt - 1) re_blocked_left_r(K, nrows, ncols, szt, k, R, Y, W);
+    re_blocked_right_q(K, nrows, szt, k, Q, Y, W);
+  }
+  free(x);
+  free(v);
+  free(w);
+  free(B);
+  free(Y);
+  free(W);
+  free(p);
+}
+
+static void re_from_planar(int K, size_t n, const double *p, double *a) {
+  size_t j;
+  int q;
+  for (j = 0; j < n; j++)
+    for (q = 0; q < K; q++) a[j * K + q] = p[q * n + j];
+}
+
+static void re_to_planar(int K, size_t n, const double *a, double *p) {
+  size_t j;
+  int q;
+  for (j = 0; j < n; j++)
+    for (q = 0; q < K; q++) p[q * n + j] = a[j * K + q];
+}
+
+void obq_real_blocked_houseqr(int K, int nrows, int ncols, int szt, int nbt, const double *A,
+                              double *Q, double *R) {
+  const size_t na = (size_t)nrows * ncols, nq = (size_t)nrows * nrows;
+  double *a = malloc(sizeof(double) * K * na), *q = malloc(sizeof(double) * K * nq);
+  double *r = malloc(sizeof(double) * K * na);
+  re_from_planar(K, na, A, a);
+  re_blocked_houseqr(K, nrows, ncols, szt, nbt, a, q, r);
+  re_to_planar(K, nq, q, Q);
+  re_to_planar(K, na, r, R);
+  free(a);
+  free(q);
+  free(r);
 }

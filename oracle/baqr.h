@@ -17,6 +17,9 @@ void obq_blocked_houseqr(int K, int nrows, int ncols, int szt, int nbt, const do
 void obq_upper_tiled_solver(int K, int dim, int szt, int nbt, double *U, double *b, double *x);
 /* y = Q^H b */
 void obq_qhb(int K, int n, const double *Q, const double *b, double *y);
+/* the real blocked QR: A, Q, R real limb-planar [K][rows][cols] */
+void obq_real_blocked_houseqr(int K, int nrows, int ncols, int szt, int nbt, const double *A,
+                              double *Q, double *R);
 /* y = A x (m x n) */
 void obq_matvec(int K, int m, int n, const double *A, const double *x, double *y);
 

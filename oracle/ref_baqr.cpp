@@ -1,4 +1,4 @@
-// Flat extern "C" wrappers around the reference's complex blocked
+// Flat extern "C" wrappers around the reference's complex and real blocked
 // Householder QR and tiled back substitution (host code, compiled in place
 // into oracle/_ref/libphcref.so).  TEST INFRASTRUCTURE ONLY.
 // Arrays are complex limb-planar [2K][rows][cols] as in oracle/baqr.h.
@@ -24,6 +24,7 @@ struct Planes {
   }
 };
 
+#define X2(a) a[0], a[1]
 #define X4(a) a[0], a[1], a[2], a[3]
 #define X8(a) a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7]
 #define X16(a) X8(a), a[8], a[9], a[10], a[11], a[12], a[13], a[14], a[15]
@@ -43,6 +44,22 @@ int ref_cmplx_blocked_houseqr(int K, int nrows, int ncols, int szt, int nbt, dou
     CPU_cmplx4_blocked_houseqr(nrows, ncols, szt, nbt, X8(a.p), X8(q.p), X8(r.p), &sec, false);
   else if (K == 8)
     CPU_cmplx8_blocked_houseqr(nrows, ncols, szt, nbt, X16(a.p), X16(q.p), X16(r.p), &sec, false);
+  else
+    return -1;
+  return 0;
+}
+
+// real A, Q, R limb-planar [K][rows][cols]
+int ref_real_blocked_houseqr(int K, int nrows, int ncols, int szt, int nbt, double *A, double *Q,
+                             double *R) {
+  Planes a(A, K, nrows, ncols), q(Q, K, nrows, nrows), r(R, K, nrows, ncols);
+  double sec = 0.0;
+  if (K == 2)
+    CPU_dbl2_blocked_houseqr(nrows, ncols, szt, nbt, X2(a.p), X2(q.p), X2(r.p), &sec, false);
+  else if (K == 4)
+    CPU_dbl4_blocked_houseqr(nrows, ncols, szt, nbt, X4(a.p), X4(q.p), X4(r.p), &sec, false);
+  else if (K == 8)
+    CPU_dbl8_blocked_houseqr(nrows, ncols, szt, nbt, X8(a.p), X8(q.p), X8(r.p), &sec, false);
   else
     return -1;
   return 0;

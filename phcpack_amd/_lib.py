@@ -134,6 +134,8 @@ SIGNATURES.update({
     "phc_baqr_set_tuning": (c_int, [c_void_p, c_int, c_int]),
     "phc_baqr_get_tuning": (c_int, [c_void_p, P_int, P_int]),
     "phc_cmplx_blocked_houseqr": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "phc_real_blocked_houseqr": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "phc_real_qtb": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "phc_cmplx_qhb": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "phc_cmplx_upper_tiled_solver": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p,
                                              c_void_p, c_void_p]),

@@ -1,5 +1,5 @@
-"""Host side of the complex blocked Householder QR and tiled back
-substitution on the GPU (K = 2 double-double, K = 4 quad-double).
+"""Host side of the complex and real blocked Householder QR and tiled back
+substitution on the GPU (K = 2 double-double, 4 quad-double, 8 octo-double).
 
 Mirrors CPU_cmplx4_blocked_houseqr (Matrices/dbl4_baqr_host.cpp:1206) and
 CPU_cmplx4_upper_tiled_solver (Matrices/dbl4_tabs_host.cpp:345): the same
@@ -45,6 +45,23 @@ class BlockedQR:
 
     def factor_device(self, dA, dQ, dR, stream=None):
         check(load().phc_cmplx_blocked_houseqr(self.handle, dA.ptr, dQ.ptr, dR.ptr, stream))
+
+    def factor_real_device(self, dA, dQ, dR, stream=None):
+        """the real QR: dA [K][m][n], dQ [K][m][m], dR [K][m][n]"""
+        check(load().phc_real_blocked_houseqr(self.handle, dA.ptr, dQ.ptr, dR.ptr, stream))
+
+    def factor_real(self, A):
+        """Q [K][m][m], R [K][m][n] of the real A [K][m][n]"""
+        K, m, n = self.K, self.nrows, self.ncols
+        dA = DeviceArray.from_host(np.ascontiguousarray(A, np.float64))
+        dQ, dR = DeviceArray((K, m, m)), DeviceArray((K, m, n))
+        self.factor_real_device(dA, dQ, dR)
+        check(load().phc_synchronize())
+        return dQ.to_host(), dR.to_host()
+
+    def qtb_device(self, dQ, db, dy, stream=None):
+        """y = Q^T b for the real Q (phc_real_qtb)"""
+        check(load().phc_real_qtb(self.handle, dQ.ptr, db.ptr, dy.ptr, stream))
 
     def qhb_device(self, dQ, db, dy, stream=None):
         """y = Q^H b (GPU_cmplx4_bals_qhb, Newton/dbl4_bals_kernels.cu:602)"""

@@ -53,6 +53,25 @@ struct Op {
   long s0, s1;
 };
 
+// A complex limb-planar array, or (RE, the real QR) an array of K real
+// planes read with zero imaginary parts and written as its real parts.
+template <int K, bool RE> struct Acc {
+  static __device__ __forceinline__ Z<K> load(const double *p, size_t ps, size_t i) {
+    if constexpr (RE) {
+      Z<K> z;
+      z.re = R<K>::load(p, ps, i);
+      z.im = R<K>::zero();
+      return z;
+    } else {
+      return C<K>::load(p, ps, i);
+    }
+  }
+  static __device__ __forceinline__ void store(double *p, size_t ps, size_t i, const Z<K> &z) {
+    if constexpr (RE) R<K>::store(p, ps, i, z.re);
+    else C<K>::store(p, ps, i, z);
+  }
+};
+
 // acc += op(a) op(b), op = conjugate when the flag is set
 template <int K, bool CA, bool CB>
 __device__ __forceinline__ void cmac(Z<K> &acc, const Z<K> &a, const Z<K> &b) {
@@ -85,11 +104,12 @@ inline int block_rb(int blk) { return (blk >= 3) ? 2 : 1; }
 // the sum starts from zero and is added to C at the end.  With a split of
 // the k range over gridDim.z slices, slice s writes its partial sums to
 // part[s][i][j] (plane stride pps) and csum_kernel adds the slices in order.
-template <int K, bool CA, bool CB, int RA, int RB>
+// RE: real operands (K planes, no conjugation), real accumulators
+template <int K, bool CA, bool CB, int RA, int RB, bool RE>
 __global__ void __launch_bounds__(256)
 cgemm_kernel(int M, int N, int KD, int kchunk, Op A, Op B, double *Cp, size_t cps, long c0,
              long c1, int accumulate, double *part, size_t pps) {
-  constexpr int TM = 16 * RA, TN = 16 * RB;
+  constexpr int TM = 16 * RA, TN = 16 * RB, PL = RE ? K : 2 * K;
   __shared__ double As[2 * K][TKD][TM];
   __shared__ double Bs[2 * K][TKD][TN];
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
@@ -110,7 +130,7 @@ cgemm_kernel(int M, int N, int KD, int kchunk, Op A, Op B, double *Cp, size_t cp
       const bool in = gi < M && gk < ke;
       const size_t o = in ? (size_t)(A.s0 * gi + A.s1 * gk) : 0;
 #pragma unroll
-      for (int q = 0; q < 2 * K; ++q) As[q][kk][ii] = in ? A.p[q * A.ps + o] : 0.0;
+      for (int q = 0; q < PL; ++q) As[q][kk][ii] = in ? A.p[q * A.ps + o] : 0.0;
     }
 #pragma unroll
     for (int e = t; e < TN * TKD; e += 256) {
@@ -120,20 +140,32 @@ cgemm_kernel(int M, int N, int KD, int kchunk, Op A, Op B, double *Cp, size_t cp
       const bool in = gj < N && gk < ke;
       const size_t o = in ? (size_t)(B.s0 * gk + B.s1 * gj) : 0;
 #pragma unroll
-      for (int q = 0; q < 2 * K; ++q) Bs[q][kk][jj] = in ? B.p[q * B.ps + o] : 0.0;
+      for (int q = 0; q < PL; ++q) Bs[q][kk][jj] = in ? B.p[q * B.ps + o] : 0.0;
     }
     __syncthreads();
     const int kmax = (ke - k0) < TKD ? (ke - k0) : TKD;
     for (int kk = 0; kk < kmax; ++kk) {
-      Z<K> av[RA], bv[RB];
+      if constexpr (RE) {
+        T<K> av[RA], bv[RB];
 #pragma unroll
-      for (int a = 0; a < RA; ++a) av[a] = C<K>::load(&As[0][0][0], TKD * TM, kk * TM + ty + 16 * a);
+        for (int a = 0; a < RA; ++a) av[a] = R<K>::load(&As[0][0][0], TKD * TM, kk * TM + ty + 16 * a);
 #pragma unroll
-      for (int b = 0; b < RB; ++b) bv[b] = C<K>::load(&Bs[0][0][0], TKD * TN, kk * TN + tx + 16 * b);
+        for (int b = 0; b < RB; ++b) bv[b] = R<K>::load(&Bs[0][0][0], TKD * TN, kk * TN + tx + 16 * b);
 #pragma unroll
-      for (int a = 0; a < RA; ++a)
+        for (int a = 0; a < RA; ++a)
 #pragma unroll
-        for (int b = 0; b < RB; ++b) cmac<K, CA, CB>(acc[a][b], av[a], bv[b]);
+          for (int b = 0; b < RB; ++b) acc[a][b].re = add(acc[a][b].re, mul(av[a], bv[b]));
+      } else {
+        Z<K> av[RA], bv[RB];
+#pragma unroll
+        for (int a = 0; a < RA; ++a) av[a] = C<K>::load(&As[0][0][0], TKD * TM, kk * TM + ty + 16 * a);
+#pragma unroll
+        for (int b = 0; b < RB; ++b) bv[b] = C<K>::load(&Bs[0][0][0], TKD * TN, kk * TN + tx + 16 * b);
+#pragma unroll
+        for (int a = 0; a < RA; ++a)
+#pragma unroll
+          for (int b = 0; b < RB; ++b) cmac<K, CA, CB>(acc[a][b], av[a], bv[b]);
+      }
     }
     __syncthreads();
   }
@@ -144,30 +176,34 @@ cgemm_kernel(int M, int N, int KD, int kchunk, Op A, Op B, double *Cp, size_t cp
       const int i = i0 + ty + 16 * a, j = j0 + tx + 16 * b;
       if (i < M && j < N) {
         if (gridDim.z > 1) {
-          C<K>::store(part, pps, (size_t)blockIdx.z * M * N + (size_t)i * N + j, acc[a][b]);
+          Acc<K, RE>::store(part, pps, (size_t)blockIdx.z * M * N + (size_t)i * N + j, acc[a][b]);
         } else {
           const size_t e = (size_t)(c0 * i + c1 * j);
           Z<K> v = acc[a][b];
-          if (accumulate) v = C<K>::add(C<K>::load(Cp, cps, e), v);
-          C<K>::store(Cp, cps, e, v);
+          if (accumulate) {
+            if constexpr (RE) v.re = add(R<K>::load(Cp, cps, e), v.re);
+            else v = C<K>::add(C<K>::load(Cp, cps, e), v);
+          }
+          Acc<K, RE>::store(Cp, cps, e, v);
         }
       }
     }
 }
 
 // C(i, j) = [C(i, j) +] sum_s part[s][i][j], slices in order
-template <int K>
+template <int K, bool RE>
 __global__ void csum_kernel(int M, int N, int S, const double *part, size_t pps, double *Cp,
                             size_t cps, long c0, long c1, int accumulate) {
+  typedef Acc<K, RE> A_;
   const size_t n = (size_t)M * N;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
        e += (size_t)gridDim.x * blockDim.x) {
-    Z<K> v = C<K>::load(part, pps, e);
-    for (int s = 1; s < S; ++s) v = C<K>::add(v, C<K>::load(part, pps, (size_t)s * n + e));
+    Z<K> v = A_::load(part, pps, e);
+    for (int s = 1; s < S; ++s) v = C<K>::add(v, A_::load(part, pps, (size_t)s * n + e));
     const size_t i = e / N, j = e % N;
     const size_t o = (size_t)(c0 * (long)i + c1 * (long)j);
-    if (accumulate) v = C<K>::add(C<K>::load(Cp, cps, o), v);
-    C<K>::store(Cp, cps, o, v);
+    if (accumulate) v = C<K>::add(A_::load(Cp, cps, o), v);
+    A_::store(Cp, cps, o, v);
   }
 }
 
@@ -181,13 +217,13 @@ inline int ksplit(int M, int N, int KD, size_t cap, int TM, int TN, int target) 
   return S;
 }
 
-template <int K, bool CA, bool CB>
+template <int K, bool CA, bool CB, bool RE>
 void launch_gemm(int blk, dim3 grid, hipStream_t st, int M, int N, int KD, int kchunk, Op A,
                  Op B, double *Cp, size_t cps, long c0, long c1, int acc, double *part,
                  size_t pps) {
 #define GEMM(RA_, RB_)                                                                           \
-  hipLaunchKernelGGL((cgemm_kernel<K, CA, CB, RA_, RB_>), grid, dim3(256), 0, st, M, N, KD, kchunk, \
-                     A, B, Cp, cps, c0, c1, acc, part, pps)
+  hipLaunchKernelGGL((cgemm_kernel<K, CA, CB, RA_, RB_, RE>), grid, dim3(256), 0, st, M, N, KD,   \
+                     kchunk, A, B, Cp, cps, c0, c1, acc, part, pps)
   switch (blk) {
     case 1: GEMM(1, 1); break;
     case 2: GEMM(2, 1); break;
@@ -197,7 +233,9 @@ void launch_gemm(int blk, dim3 grid, hipStream_t st, int M, int N, int KD, int k
 #undef GEMM
 }
 
-template <int K>
+// RE: real operands and result (K planes each); the conjugation flags are
+// then ignored
+template <int K, bool RE = false>
 int cgemm(bool ca, bool cb, int M, int N, int KD, Op A, Op B, double *Cp, size_t cps, long c0,
           long c1, bool accumulate, hipStream_t st, double *part = nullptr, size_t cap = 0,
           GemmCfg cfg = GemmCfg()) {
@@ -209,12 +247,13 @@ int cgemm(bool ca, bool cb, int M, int N, int KD, Op A, Op B, double *Cp, size_t
   dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, S);
   const int acc = accumulate ? 1 : 0;
   const size_t pps = cap;
-  if (!ca && !cb) launch_gemm<K, false, false>(blk, grid, st, M, N, KD, kchunk, A, B, Cp, cps, c0, c1, acc, part, pps);
-  else if (ca && !cb) launch_gemm<K, true, false>(blk, grid, st, M, N, KD, kchunk, A, B, Cp, cps, c0, c1, acc, part, pps);
-  else if (!ca && cb) launch_gemm<K, false, true>(blk, grid, st, M, N, KD, kchunk, A, B, Cp, cps, c0, c1, acc, part, pps);
-  else launch_gemm<K, true, true>(blk, grid, st, M, N, KD, kchunk, A, B, Cp, cps, c0, c1, acc, part, pps);
+  if (RE) launch_gemm<K, false, false, true>(blk, grid, st, M, N, KD, kchunk, A, B, Cp, cps, c0, c1, acc, part, pps);
+  else if (!ca && !cb) launch_gemm<K, false, false, false>(blk, grid, st, M, N, KD, kchunk, A, B, Cp, cps, c0, c1, acc, part, pps);
+  else if (ca && !cb) launch_gemm<K, true, false, false>(blk, grid, st, M, N, KD, kchunk, A, B, Cp, cps, c0, c1, acc, part, pps);
+  else if (!ca && cb) launch_gemm<K, false, true, false>(blk, grid, st, M, N, KD, kchunk, A, B, Cp, cps, c0, c1, acc, part, pps);
+  else launch_gemm<K, true, true, false>(blk, grid, st, M, N, KD, kchunk, A, B, Cp, cps, c0, c1, acc, part, pps);
   if (S > 1)
-    hipLaunchKernelGGL(csum_kernel<K>, dim3(phc::grid_for((size_t)M * N, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL((csum_kernel<K, RE>), dim3(phc::grid_for((size_t)M * N, 256)), dim3(256), 0, st,
                        M, N, S, part, pps, Cp, cps, c0, c1, acc);
   PHC_HIP(hipGetLastError());
   return 0;
@@ -309,12 +348,17 @@ constexpr int HW = 1024;  // lanes of the Householder kernel
 //   every lane: v_i = x_i * (1/v0) for its rows.
 // Sums are in tree order (the reference's is sequential): results agree to
 // a few units in the last limb.
-template <int K>
+// RE (real data, zero imaginary parts; CPU_dbl4_factors_house,
+// Matrices/dbl4_factorizations.cpp:541-626): mu = sqrt(x0^2 + sigma),
+// v0 = x0 - mu when the leading limb of x0 is <= 0, else -sigma/(x0 + mu),
+// beta = 2 v0^2 / (sigma + v0^2) on lane 0; every lane: v_i = x_i / v0.
+template <int K, bool RE>
 __global__ void __launch_bounds__(HW)
 house_kernel(const double *Pc, size_t pps, int m, int rowoff, int colidx, int nrows, int L,
              double *Y, size_t yps, int rowdim, double *Bv, int szt) {
   __shared__ T<K> part[HW / 64];
   __shared__ Z<K> sinv;
+  __shared__ T<K> sv0;
   __shared__ int sflag;
   const int t = threadIdx.x, w = t >> 6, n = nrows - colidx;
   const int nw = blockDim.x >> 6, stride = blockDim.x;
@@ -349,6 +393,21 @@ house_kernel(const double *Pc, size_t pps, int m, int rowoff, int colidx, int nr
     bool szero = true;
 #pragma unroll
     for (int q = 0; q < K; ++q) szero = szero && (sigma.x[q] == 0.0);
+    if constexpr (RE) {
+      if (t == 0) {
+        T<K> beta = R<K>::zero(), v0 = R<K>::from(1.0);
+        if (!szero) {
+          const T<K> mu = nl_sqrt<K>(nl_add<K>(sqr(x0.re), sigma));
+          if (R<K>::hi(x0.re) <= 0.0) v0 = nl_sub<K>(x0.re, mu);
+          else v0 = neg(nl_div<K>(sigma, nl_add<K>(x0.re, mu)));
+          const T<K> v0p2 = sqr(v0);
+          beta = mul_d(nl_div<K>(v0p2, nl_add<K>(sigma, v0p2)), 2.0);
+        }
+        R<K>::store(Bv, szt, L, beta);
+        sflag = szero ? 1 : 0;
+        sv0 = v0;
+      }
+    } else {
     // lane 0: nrm = sqrt(|x0|^2 + sigma), lane 1: r0 = sqrt(|x0|^2)
     T<K> rt = sq0;
     if (t == 0) rt = nl_add<K>(sq0, sigma);
@@ -392,15 +451,21 @@ house_kernel(const double *Pc, size_t pps, int m, int rowoff, int colidx, int nr
         sinv.im = q;
       }
     }
+    }
   }
   __syncthreads();
-  const Z<K> inv = sinv;
   const bool flag = sflag != 0;
   auto vof = [&](const Z<K> &xi) {
     if (flag) return xi;
     Z<K> v;
-    v.re = sub(mul(xi.re, inv.re), mul(xi.im, inv.im));
-    v.im = add(mul(xi.im, inv.re), mul(xi.re, inv.im));
+    if constexpr (RE) {
+      v.re = div(xi.re, sv0);
+      v.im = R<K>::zero();
+    } else {
+      const Z<K> inv = sinv;
+      v.re = sub(mul(xi.re, inv.re), mul(xi.im, inv.im));
+      v.im = add(mul(xi.im, inv.re), mul(xi.re, inv.im));
+    }
     return v;
   };
   double *yl = Y + (size_t)L * rowdim + L;  // v_i at yl[i]
@@ -462,7 +527,7 @@ column_upd_kernel(double *Pc, size_t pps, int m, int rowoff, int colidx, int nro
 
 // the panel columns rowoff..rowoff+szt-1 of R (rows rowoff..m-1) to and from
 // the column-major copy Pc
-template <int K>
+template <int K, bool RE>
 __global__ void panel_copy_kernel(double *Rm, size_t rps, int ld, int m, int rowoff, int szt,
                                   double *Pc, size_t pps, int to_panel) {
   const size_t rows = (size_t)(m - rowoff), tot = rows * szt;
@@ -471,9 +536,9 @@ __global__ void panel_copy_kernel(double *Rm, size_t rps, int ld, int m, int row
     const int c = (int)(e / rows), i = rowoff + (int)(e % rows);
     const size_t re = (size_t)i * ld + rowoff + c, pe = (size_t)c * m + i;
     if (to_panel)
-      C<K>::store(Pc, pps, pe, C<K>::load(Rm, rps, re));
+      C<K>::store(Pc, pps, pe, Acc<K, RE>::load(Rm, rps, re));
     else
-      C<K>::store(Rm, rps, re, C<K>::load(Pc, pps, pe));
+      Acc<K, RE>::store(Rm, rps, re, C<K>::load(Pc, pps, pe));
   }
 }
 
@@ -569,18 +634,27 @@ __global__ void copy_inverse_kernel(double *U, size_t ups, int dim, int szt, int
   }
 }
 
-template <int K>
+template <int K, bool RE>
 __global__ void init_qr_kernel(int nrows, int ncols, const double *A, double *Q, double *Rm) {
+  typedef Acc<K, RE> A_;
   const size_t nq = (size_t)nrows * nrows, nr = (size_t)nrows * ncols;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nq + nr;
        e += (size_t)gridDim.x * blockDim.x) {
     if (e < nq) {
       const size_t i = e / nrows, j = e % nrows;
-      C<K>::store(Q, nq, e, C<K>::from(i == j ? 1.0 : 0.0, 0.0));
+      A_::store(Q, nq, e, C<K>::from(i == j ? 1.0 : 0.0, 0.0));
     } else {
-      C<K>::store(Rm, nr, e - nq, C<K>::load(A, nr, e - nq));
+      A_::store(Rm, nr, e - nq, A_::load(A, nr, e - nq));
     }
   }
+}
+
+// zeros into n doubles (the imaginary planes of a workspace the real QR
+// reads as complex)
+__global__ void zero_kernel(double *p, size_t n) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x)
+    p[e] = 0.0;
 }
 
 }  // namespace
@@ -595,25 +669,32 @@ struct phc_baqr {
   hipGraphExec_t exec;
   const double *gA;
   double *gQ, *gR;
+  int greal;  // the captured sequence is the real QR's
 };
 
 namespace {
 
-template <int K> int run_qr(phc_baqr *p, const double *A, double *Q, double *R, hipStream_t st) {
+// RE: A, Q, R are real (K planes); the panel kernels run on the complex
+// workspace with zero imaginary parts, the products on the real planes
+template <int K, bool RE>
+int run_qr(phc_baqr *p, const double *A, double *Q, double *R, hipStream_t st) {
   const int m = p->nrows, n = p->ncols, szt = p->szt;
   const size_t qps = (size_t)m * m, rps = (size_t)m * n;
-  hipLaunchKernelGGL(init_qr_kernel<K>, dim3(phc::grid_for(qps + rps, 256)), dim3(256), 0, st, m,
+  hipLaunchKernelGGL((init_qr_kernel<K, RE>), dim3(phc::grid_for(qps + rps, 256)), dim3(256), 0, st, m,
                      n, A, Q, R);
+  if (RE)  // the Gram matrix is read as complex by the S factor
+    hipLaunchKernelGGL(zero_kernel, dim3(phc::grid_for((size_t)K * p->gps, 256)), dim3(256), 0, st,
+                       p->G + (size_t)K * p->gps, (size_t)K * p->gps);
   PHC_HIP(hipGetLastError());
   for (int k = 0; k < p->nbt; ++k) {
     const int rowoff = k * szt, rowdim = m - rowoff, endcol = (k + 1) * szt;
-    hipLaunchKernelGGL(panel_copy_kernel<K>, dim3(phc::grid_for((size_t)rowdim * szt, 256)),
+    hipLaunchKernelGGL((panel_copy_kernel<K, RE>), dim3(phc::grid_for((size_t)rowdim * szt, 256)),
                        dim3(256), 0, st, R, rps, n, m, rowoff, szt, p->Pc, p->pps, 1);
     for (int L = 0; L < szt; ++L) {
       const int colidx = rowoff + L, ncol = endcol - colidx;
       const int nr = m - colidx, ns = (nr + CW - 1) / CW;
       const int hwv = (nr - 1 + 63) / 64, hw = 64 * (hwv < 1 ? 1 : hwv < HW / 64 ? hwv : HW / 64);
-      hipLaunchKernelGGL(house_kernel<K>, dim3(1), dim3(hw), 0, st, p->Pc, p->pps, m, rowoff,
+      hipLaunchKernelGGL((house_kernel<K, RE>), dim3(1), dim3(hw), 0, st, p->Pc, p->pps, m, rowoff,
                          colidx, m, L, p->Y, p->yps, rowdim, p->B, szt);
       hipLaunchKernelGGL(column_dot_kernel<K>, dim3(ncol, ns), dim3(CW), 0, st, p->Pc, p->pps, m,
                          rowoff, colidx, m, p->Y, p->yps, rowdim, L, p->wpart, p->wpps, szt);
@@ -621,37 +702,37 @@ template <int K> int run_qr(phc_baqr *p, const double *A, double *Q, double *R, 
                          rowoff, colidx, m, p->Y, p->yps, rowdim, L, p->wpart, p->wpps, ns, p->B,
                          szt);
     }
-    hipLaunchKernelGGL(panel_copy_kernel<K>, dim3(phc::grid_for((size_t)rowdim * szt, 256)),
+    hipLaunchKernelGGL((panel_copy_kernel<K, RE>), dim3(phc::grid_for((size_t)rowdim * szt, 256)),
                        dim3(256), 0, st, R, rps, n, m, rowoff, szt, p->Pc, p->pps, 0);
     PHC_HIP(hipGetLastError());
     // Y, W stored as szt rows of length rowdim (row stride rowdim)
     const Op Yt{p->Y, p->yps, 1, rowdim};  // Yhat(i, k) = Y[k][i]
     const Op Yc{p->Y, p->yps, rowdim, 1};  // (k, i) -> Y[k][i]
     // G = Yhat^H Yhat
-    if (cgemm<K>(true, false, szt, szt, rowdim, Yc, Yt, p->G, p->gps, szt, 1, false, st, p->part,
+    if (cgemm<K, RE>(true, false, szt, szt, rowdim, Yc, Yt, p->G, p->gps, szt, 1, false, st, p->part,
                  p->pcap, p->cfg)) return -1;
     hipLaunchKernelGGL(s_factor_kernel<K>, dim3(szt), dim3(64), 0, st, p->G, p->gps, p->B, p->S,
                        p->gps, szt);
     PHC_HIP(hipGetLastError());
     // W = Yhat S, stored like Y
-    if (cgemm<K>(false, false, rowdim, szt, szt, Yt, Op{p->S, p->gps, szt, 1}, p->W, p->yps, 1,
+    if (cgemm<K, RE>(false, false, rowdim, szt, szt, Yt, Op{p->S, p->gps, szt, 1}, p->W, p->yps, 1,
                  rowdim, false, st, p->part, p->pcap, p->cfg)) return -1;
     const Op Wt{p->W, p->yps, 1, rowdim};  // What(i, k) = W[k][i]
     if (k < p->nbt - 1) {
       const int coloff = endcol, coldim = n - coloff;
       // T1 = What^H C with C = R[rowoff.., coloff..];  C += Yhat T1
       const Op Cm{R + (size_t)rowoff * n + coloff, rps, n, 1};
-      if (cgemm<K>(true, false, szt, coldim, rowdim, Op{p->W, p->yps, rowdim, 1}, Cm, p->T1,
+      if (cgemm<K, RE>(true, false, szt, coldim, rowdim, Op{p->W, p->yps, rowdim, 1}, Cm, p->T1,
                    p->t1ps, coldim, 1, false, st, p->part, p->pcap, p->cfg)) return -1;
-      if (cgemm<K>(false, false, rowdim, coldim, szt, Yt, Op{p->T1, p->t1ps, coldim, 1},
+      if (cgemm<K, RE>(false, false, rowdim, coldim, szt, Yt, Op{p->T1, p->t1ps, coldim, 1},
                    R + (size_t)rowoff * n + coloff, rps, n, 1, true, st, p->part, p->pcap, p->cfg))
         return -1;
     }
     // T2 = Q[:, rowoff..] What;  Q[:, rowoff..] += T2 Yhat^H
     const Op Qm{Q + rowoff, qps, m, 1};
-    if (cgemm<K>(false, false, m, szt, rowdim, Qm, Wt, p->T2, p->t2ps, szt, 1, false, st, p->part,
+    if (cgemm<K, RE>(false, false, m, szt, rowdim, Qm, Wt, p->T2, p->t2ps, szt, 1, false, st, p->part,
                  p->pcap, p->cfg)) return -1;
-    if (cgemm<K>(false, true, m, rowdim, szt, Op{p->T2, p->t2ps, szt, 1}, Yc, Q + rowoff, qps, m,
+    if (cgemm<K, RE>(false, true, m, rowdim, szt, Op{p->T2, p->t2ps, szt, 1}, Yc, Q + rowoff, qps, m,
                  1, true, st, p->part, p->pcap, p->cfg)) return -1;
   }
   return 0;
@@ -797,19 +878,25 @@ int phc_baqr_get_tuning(const phc_baqr *p, int *gemm_block, int *split_target) {
 // The ~4 launches per column are captured once into a HIP graph (per
 // (A, Q, R) pointers) and replayed: the panel kernels are short, so launch
 // gaps would otherwise dominate.
-int phc_cmplx_blocked_houseqr(phc_baqr *p, const double *A, double *Q, double *R, void *stream) {
-  PHC_REQUIRE(p && A && Q && R, "phc_cmplx_blocked_houseqr: null argument");
-  hipStream_t st = (hipStream_t)stream;
-  if (!p->exec || p->gA != A || p->gQ != Q || p->gR != R) {
+namespace {
+
+int blocked_houseqr(phc_baqr *p, const double *A, double *Q, double *R, hipStream_t st, int real) {
+  if (!p->exec || p->gA != A || p->gQ != Q || p->gR != R || p->greal != real) {
     if (p->exec) {
       PHC_HIP(hipGraphExecDestroy(p->exec));
       p->exec = nullptr;
     }
     hipGraph_t graph = nullptr;
     PHC_HIP(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
-    const int rc = (p->K == 2)   ? run_qr<2>(p, A, Q, R, p->cap)
-                   : (p->K == 4) ? run_qr<4>(p, A, Q, R, p->cap)
-                                 : run_qr<8>(p, A, Q, R, p->cap);
+    int rc;
+    if (real)
+      rc = (p->K == 2) ? run_qr<2, true>(p, A, Q, R, p->cap)
+           : (p->K == 4) ? run_qr<4, true>(p, A, Q, R, p->cap)
+                         : run_qr<8, true>(p, A, Q, R, p->cap);
+    else
+      rc = (p->K == 2) ? run_qr<2, false>(p, A, Q, R, p->cap)
+           : (p->K == 4) ? run_qr<4, false>(p, A, Q, R, p->cap)
+                         : run_qr<8, false>(p, A, Q, R, p->cap);
     const hipError_t e = hipStreamEndCapture(p->cap, &graph);
     if (rc) {
       if (graph) hipGraphDestroy(graph);
@@ -822,9 +909,36 @@ int phc_cmplx_blocked_houseqr(phc_baqr *p, const double *A, double *Q, double *R
     p->gA = A;
     p->gQ = Q;
     p->gR = R;
+    p->greal = real;
   }
   PHC_HIP(hipGraphLaunch(p->exec, st));
   return 0;
+}
+
+}  // namespace
+
+int phc_cmplx_blocked_houseqr(phc_baqr *p, const double *A, double *Q, double *R, void *stream) {
+  PHC_REQUIRE(p && A && Q && R, "phc_cmplx_blocked_houseqr: null argument");
+  return blocked_houseqr(p, A, Q, R, (hipStream_t)stream, 0);
+}
+
+int phc_real_blocked_houseqr(phc_baqr *p, const double *A, double *Q, double *R, void *stream) {
+  PHC_REQUIRE(p && A && Q && R, "phc_real_blocked_houseqr: null argument");
+  return blocked_houseqr(p, A, Q, R, (hipStream_t)stream, 1);
+}
+
+int phc_real_qtb(phc_baqr *p, const double *Q, const double *b, double *y, void *stream) {
+  PHC_REQUIRE(p && Q && b && y, "phc_real_qtb: null argument");
+  const int n = p->nrows;
+  hipStream_t st = (hipStream_t)stream;
+  const Op qa{Q, (size_t)n * n, 1, n}, bb{b, (size_t)n, 1, 0};
+  if (p->K == 8)
+    return cgemm<8, true>(true, false, n, 1, n, qa, bb, y, (size_t)n, 1, 0, false, st, p->part,
+                          p->pcap, p->cfg);
+  return p->K == 2 ? cgemm<2, true>(true, false, n, 1, n, qa, bb, y, (size_t)n, 1, 0, false, st,
+                                    p->part, p->pcap, p->cfg)
+                   : cgemm<4, true>(true, false, n, 1, n, qa, bb, y, (size_t)n, 1, 0, false, st,
+                                    p->part, p->pcap, p->cfg);
 }
 
 int phc_cmplx_qhb(phc_baqr *p, const double *Q, const double *b, double *y, void *stream) {

@@ -100,15 +100,17 @@ long long baqr_macs(long long m, long long n, long long szt) {
   return macs;
 }
 
+// real: K planes per matrix and phc_real_blocked_houseqr
 void blocked_qr(int K, int nrows, int ncols, int szt, int nbt, double **const *A,
                 double **const *Q, double **const *R, double *walltimesec, long long *addcnt,
-                long long *mulcnt, long long *divcnt, long long *sqrtcnt, const char *fn) {
+                long long *mulcnt, long long *divcnt, long long *sqrtcnt, const char *fn,
+                bool real = false) {
   const Clock::time_point t0 = Clock::now();
   if (szt * nbt != ncols) {
     std::fprintf(stderr, "%s: ncols must equal szt*nbt\n", fn);
     return;
   }
-  const int P = 2 * K;
+  const int P = real ? K : 2 * K;
   std::vector<double> hA, hQ, hR;
   pack(A, P, nrows, ncols, hA);
   phc_baqr *plan = nullptr;
@@ -117,15 +119,17 @@ void blocked_qr(int K, int nrows, int ncols, int szt, int nbt, double **const *A
     Buf dA(hA.size()), dQ((size_t)P * nrows * nrows), dR(hA.size());
     if (!dA.p || !dQ.p || !dR.p || !dA.up(hA)) {
       ok(-1, fn);
-    } else if (ok(phc_cmplx_blocked_houseqr(plan, dA.d(), dQ.d(), dR.d(), nullptr), fn) &&
+    } else if (ok(real ? phc_real_blocked_houseqr(plan, dA.d(), dQ.d(), dR.d(), nullptr)
+                       : phc_cmplx_blocked_houseqr(plan, dA.d(), dQ.d(), dR.d(), nullptr),
+                  fn) &&
                ok(phc_synchronize(), fn) && dQ.down(hQ) && dR.down(hR)) {
       unpack(hQ, P, nrows, nrows, Q);
       unpack(hR, P, nrows, ncols, R);
-      const long long macs = baqr_macs(nrows, ncols, szt);
-      if (addcnt) *addcnt += 4 * macs;
-      if (mulcnt) *mulcnt += 4 * macs;
-      if (divcnt) *divcnt += 4LL * ncols;
-      if (sqrtcnt) *sqrtcnt += 2LL * ncols;
+      const long long macs = baqr_macs(nrows, ncols, szt), per = real ? 1 : 4;
+      if (addcnt) *addcnt += per * macs;
+      if (mulcnt) *mulcnt += per * macs;
+      if (divcnt) *divcnt += (real ? 1LL : 4LL) * ncols;
+      if (sqrtcnt) *sqrtcnt += (real ? 1LL : 2LL) * ncols;
     }
   }
   phc_baqr_destroy(plan);
@@ -430,6 +434,50 @@ void GPU_cmplx4_product(double *xrehihi_h, double *xrelohi_h, double *xrehilo_h,
 }
 
 // ------------------------------------------------------------ blocked QR
+
+void GPU_dbl2_blocked_houseqr(int nrows, int ncols, int szt, int nbt, double **Ahi, double **Alo,
+                              double **Qhi, double **Qlo, double **Rhi, double **Rlo, double *,
+                              double *, double *, double *, double *, double *, double *,
+                              double *, double *, double *, double *walltimesec,
+                              long long int *addcnt, long long int *mulcnt,
+                              long long int *divcnt, long long int *sqrtcnt, bool) {
+  double **A[] = {Ahi, Alo};
+  double **Q[] = {Qhi, Qlo};
+  double **R[] = {Rhi, Rlo};
+  blocked_qr(2, nrows, ncols, szt, nbt, A, Q, R, walltimesec, addcnt, mulcnt, divcnt, sqrtcnt,
+             "GPU_dbl2_blocked_houseqr", true);
+}
+
+void GPU_dbl4_blocked_houseqr(int nrows, int ncols, int szt, int nbt, double **Ahihi,
+                              double **Alohi, double **Ahilo, double **Alolo, double **Qhihi,
+                              double **Qlohi, double **Qhilo, double **Qlolo, double **Rhihi,
+                              double **Rlohi, double **Rhilo, double **Rlolo, double *, double *,
+                              double *, double *, double *, double *, double *, double *,
+                              double *, double *, double *walltimesec, long long int *addcnt,
+                              long long int *mulcnt, long long int *divcnt,
+                              long long int *sqrtcnt, bool) {
+  double **A[] = {Ahihi, Alohi, Ahilo, Alolo};
+  double **Q[] = {Qhihi, Qlohi, Qhilo, Qlolo};
+  double **R[] = {Rhihi, Rlohi, Rhilo, Rlolo};
+  blocked_qr(4, nrows, ncols, szt, nbt, A, Q, R, walltimesec, addcnt, mulcnt, divcnt, sqrtcnt,
+             "GPU_dbl4_blocked_houseqr", true);
+}
+
+void GPU_dbl8_blocked_houseqr(
+    int nrows, int ncols, int szt, int nbt, double **Ahihihi, double **Alohihi, double **Ahilohi,
+    double **Alolohi, double **Ahihilo, double **Alohilo, double **Ahilolo, double **Alololo,
+    double **Qhihihi, double **Qlohihi, double **Qhilohi, double **Qlolohi, double **Qhihilo,
+    double **Qlohilo, double **Qhilolo, double **Qlololo, double **Rhihihi, double **Rlohihi,
+    double **Rhilohi, double **Rlolohi, double **Rhihilo, double **Rlohilo, double **Rhilolo,
+    double **Rlololo, double *, double *, double *, double *, double *, double *, double *,
+    double *, double *, double *, double *walltimesec, long long int *addcnt,
+    long long int *mulcnt, long long int *divcnt, long long int *sqrtcnt, bool) {
+  double **A[] = {Ahihihi, Alohihi, Ahilohi, Alolohi, Ahihilo, Alohilo, Ahilolo, Alololo};
+  double **Q[] = {Qhihihi, Qlohihi, Qhilohi, Qlolohi, Qhihilo, Qlohilo, Qhilolo, Qlololo};
+  double **R[] = {Rhihihi, Rlohihi, Rhilohi, Rlolohi, Rhihilo, Rlohilo, Rhilolo, Rlololo};
+  blocked_qr(8, nrows, ncols, szt, nbt, A, Q, R, walltimesec, addcnt, mulcnt, divcnt, sqrtcnt,
+             "GPU_dbl8_blocked_houseqr", true);
+}
 
 void GPU_cmplx2_blocked_houseqr(
     int nrows, int ncols, int szt, int nbt, double **Arehi, double **Arelo, double **Aimhi,

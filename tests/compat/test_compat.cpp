@@ -304,6 +304,43 @@ static void test_qr(int K, int m, int n, int szt) {
         tol * std::sqrt((double)m));
 }
 
+// the real QR through GPU_dbl{2,4,8}_blocked_houseqr against the oracle's
+// restatement of CPU_dbl{2,4,8}_blocked_houseqr
+static void test_real_qr(int K, int m, int n, int szt) {
+  Planes A(K, m, n), Qc(K, m, m), Rc(K, m, n), Qg(K, m, m), Rg(K, m, n);
+  A.randomize(K);
+  obq_real_blocked_houseqr(K, m, n, szt, n / szt, A.data(), Qc.data(), Rc.data());
+  double t[10] = {0}, wall = 0.0;
+  long long add = 0, mul = 0, div = 0, sq = 0;
+  if (K == 2)
+    GPU_dbl2_blocked_houseqr(m, n, szt, n / szt, A.p(0), A.p(1), Qg.p(0), Qg.p(1), Rg.p(0),
+                             Rg.p(1), &t[0], &t[1], &t[2], &t[3], &t[4], &t[5], &t[6], &t[7],
+                             &t[8], &t[9], &wall, &add, &mul, &div, &sq, false);
+  else if (K == 4)
+    GPU_dbl4_blocked_houseqr(m, n, szt, n / szt, A.p(0), A.p(1), A.p(2), A.p(3), Qg.p(0),
+                             Qg.p(1), Qg.p(2), Qg.p(3), Rg.p(0), Rg.p(1), Rg.p(2), Rg.p(3),
+                             &t[0], &t[1], &t[2], &t[3], &t[4], &t[5], &t[6], &t[7], &t[8],
+                             &t[9], &wall, &add, &mul, &div, &sq, false);
+  else
+    GPU_dbl8_blocked_houseqr(m, n, szt, n / szt, A.p(0), A.p(1), A.p(2), A.p(3), A.p(4), A.p(5),
+                             A.p(6), A.p(7), Qg.p(0), Qg.p(1), Qg.p(2), Qg.p(3), Qg.p(4),
+                             Qg.p(5), Qg.p(6), Qg.p(7), Rg.p(0), Rg.p(1), Rg.p(2), Rg.p(3),
+                             Rg.p(4), Rg.p(5), Rg.p(6), Rg.p(7), &t[0], &t[1], &t[2], &t[3],
+                             &t[4], &t[5], &t[6], &t[7], &t[8], &t[9], &wall, &add, &mul, &div,
+                             &sq, false);
+  if (!(wall > 0.0) || add <= 0 || mul <= 0) {
+    std::printf("GPU_dbl%d_blocked_houseqr: wall time or counters not set\n", K);
+    ++failures;
+  }
+  const double tol = 8.0 * m * std::pow(0x1p-52, K);
+  char what[96];
+  std::snprintf(what, sizeof what, "GPU_dbl%d_blocked_houseqr %dx%d szt %d: Q", K, m, n, szt);
+  check(Qg.diff(K, Qc) < tol, what, Qg.diff(K, Qc), tol);
+  std::snprintf(what, sizeof what, "GPU_dbl%d_blocked_houseqr %dx%d szt %d: R", K, m, n, szt);
+  check(Rg.diff(K, Rc) < tol * std::sqrt((double)m), what, Rg.diff(K, Rc),
+        tol * std::sqrt((double)m));
+}
+
 static void solve_gpu(int K, Planes &U, Planes &b, Planes &x, int szt) {
   const int n = U.rows;
   double t[4] = {0}, wall = 0.0, over[3] = {0};
@@ -526,6 +563,9 @@ int main() {
   test_qr(2, 96, 64, 32);
   test_qr(4, 48, 32, 8);
   test_qr(8, 32, 32, 8);
+  test_real_qr(2, 64, 64, 16);
+  test_real_qr(4, 96, 64, 32);
+  test_real_qr(8, 32, 32, 8);
   test_tiled(2, 64, 16);
   test_tiled(4, 96, 32);
   test_tiled(8, 32, 8);

@@ -522,6 +522,15 @@ def ref_blocked_houseqr(K, A, szt, nbt):
     return _cx_qr(ref(), "ref_cmplx_blocked_houseqr", K, A, szt, nbt)
 
 
+def oracle_real_blocked_houseqr(K, A, szt, nbt):
+    """real A [K][m][n] -> Q [K][m][m], R [K][m][n]"""
+    return _cx_qr(oracle(), "obq_real_blocked_houseqr", K, A, szt, nbt)
+
+
+def ref_real_blocked_houseqr(K, A, szt, nbt):
+    return _cx_qr(ref(), "ref_real_blocked_houseqr", K, A, szt, nbt)
+
+
 def _cx_tiled(lib, fn, K, U, b, szt, nbt):
     U, b = np.ascontiguousarray(U).copy(), np.ascontiguousarray(b).copy()
     x = np.zeros_like(b)

@@ -163,3 +163,54 @@ def test_tuned_configurations_match_oracle(K, shape):
     assert qr.tuning() == ((4, 1024) if K == 2 else (1, 1024))
     best, table = tune_qr(K, m, n, dA, szts=[szt, n], blocks=(1, 4), splits=(1024,), reps=1)
     assert len(table) == 4 and best in table and table[best] == min(table.values())
+
+
+@pytest.mark.parametrize("K,shape", [(K, s) for K in (2, 4) for s in [(32, 32, 8), (48, 32, 8),
+                                                                      (64, 64, 16), (40, 40, 40)]]
+                         + [(8, (32, 32, 8)), (2, (256, 128, 128))])
+def test_real_blocked_qr_matches_oracle(K, shape):
+    """the real QR (phc_real_blocked_houseqr) against the oracle's
+    restatement of CPU_dbl{2,4,8}_blocked_houseqr (bit-exact with the
+    reference): the tolerance of the complex QR (measured: at most 3.3 units
+    of eps^K for Q and 20 for R in dd); R's diagonal positive"""
+    from oraclelib import oracle_real_blocked_houseqr, rand_md
+    from phcpack_amd.baqr import BlockedQR
+    m, n, szt = shape
+    print(f"  real blocked qr K={K} {shape}", flush=True)
+    rng = np.random.default_rng(31 * m + n + K)
+    A = rand_md(rng, K, (m, n))
+    Q, R = BlockedQR(K, m, n, szt).factor_real(A)
+    Qo, Ro = oracle_real_blocked_houseqr(K, A, szt, n // szt)
+    dq, dr = np.abs(md_diff(K, Q, Qo)).max(), np.abs(md_diff(K, R, Ro)).max()
+    tol = QR_UNITS * m * ulps(K)
+    print(f"  max |Q - Qo| {dq / ulps(K):.1f} u  |R - Ro| {dr / ulps(K):.1f} u", flush=True)
+    assert dq < tol and dr < tol * np.sqrt(m)
+    # positive diagonal, except a last column with no row below its
+    # diagonal (sigma = 0: no reflection, as in the reference)
+    assert np.all(np.diag(R[0])[:min(n, m - 1)] > 0)
+
+
+def test_real_least_squares_recovers_planted_solution():
+    """A x0 = b with x0 planted: y = Q^T b (phc_real_qtb), then R x = y on
+    the host in double from the limb sums, close to x0"""
+    from oraclelib import rand_md
+    from phcpack_amd._lib import DeviceArray, check, load
+    from phcpack_amd.baqr import BlockedQR
+    K, n, szt = 4, 96, 32
+    rng = np.random.default_rng(5)
+    A = rand_md(rng, K, (n, n))
+    x0 = rng.uniform(-1, 1, n)
+    a = A.sum(axis=0)
+    b = np.zeros((K, n))
+    b[0] = a @ x0
+    qr = BlockedQR(K, n, n, szt)
+    dA = DeviceArray.from_host(np.ascontiguousarray(A))
+    dQ, dR = DeviceArray((K, n, n)), DeviceArray((K, n, n))
+    db, dy = DeviceArray.from_host(b), DeviceArray((K, n))
+    qr.factor_real_device(dA, dQ, dR)
+    qr.qtb_device(dQ, db, dy)
+    check(load().phc_synchronize())
+    q, r, y = dQ.to_host().sum(axis=0), dR.to_host().sum(axis=0), dy.to_host().sum(axis=0)
+    assert np.abs(q.T @ b[0] - y).max() < 1e-12
+    x = np.linalg.solve(np.triu(r), y)
+    assert np.abs(x - x0).max() < 1e-10

@@ -1,6 +1,6 @@
 """The reference-signature layer (include/phcmd_compat.h: GPU_*_product,
 GPU_*_poly_evaldiff with ConvolutionJobs/AdditionJobs schedules,
-GPU_cmplx{2,4,8}_blocked_houseqr, GPU_cmplx{2,4,8}_upper_tiled_solver,
+GPU_cmplx{2,4,8}_blocked_houseqr, GPU_dbl{2,4,8}_blocked_houseqr, GPU_cmplx{2,4,8}_upper_tiled_solver,
 GPU_dbl{2,4}_upper_tiled_solver, GPU_cmplx{2,4}_bals_qhb,
 GPU_dbl{,2,4}_bals_solve, GPU_dbl{,2,4}_linear_residue) against the CPU
 oracle, by the C++ program tests/compat/test_compat.cpp (built by the
@@ -28,5 +28,6 @@ def test_reference_signature_layer_matches_oracle():
     assert "0 failed" in r.stdout
     for name in ("GPU_dbl8_poly_evaldiff", "GPU_cmplx4_poly_evaldiff", "GPU_cmplx8_product",
                  "GPU_dbl4_bals_solve", "GPU_cmplx4_blocked_houseqr",
-                 "GPU_cmplx8_blocked_houseqr", "GPU_cmplx8_upper_tiled_solver"):
+                 "GPU_cmplx8_blocked_houseqr", "GPU_cmplx8_upper_tiled_solver",
+                 "GPU_dbl2_blocked_houseqr", "GPU_dbl8_blocked_houseqr"):
         assert name in r.stdout, f"{name} was not checked"
