@@ -411,6 +411,25 @@ int phc_linear_residue_batch(int K, int n, int degp1, int tailidx, int nbatch,
                              const double *mat_dev, const double *rhs_dev, const double *sol_dev,
                              double *res_dev, double *resmax_dev, void *stream);
 
+/* The complex twins (K = 2, 4; arrays complex limb-planar, 2K planes per
+ * array: K real, then K imaginary), bit-identical to CPU_cmplx2_factors_houseqr
+ * (Matrices/dbl2_factorizations.cpp:766), CPU_cmplx2_qrbs_solve and
+ * CPU_cmplx2_linear_residue (Newton/dbl2_bals_host.cpp) and their cmplx4 /
+ * cmplx8 twins: the skip tests on max(|re| + |im|) of the leading limbs
+ * (CPU_cmplx_onenorm), resmax [K][nbatch] = |re| + |im| of each limb of
+ * the entry of largest |re| + |im|.  K = 8 is refused (-2): the first
+ * device run of the complex octo-double QR stalled, cause not established;
+ * the CPU oracle covers the octo-double complex functions. */
+int phc_cmplx_houseqr_batch(int K, int n, int nbatch, const double *A_dev, double *Q_dev,
+                            double *R_dev, void *stream);
+int phc_cmplx_qrbs_solve_batch(int K, int n, int degp1, int tailidx, int nbatch,
+                               const double *mat_dev, double *rhs_dev, double *sol_dev,
+                               double *Q_dev, double *R_dev, int *flags_dev, void *stream);
+int phc_cmplx_linear_residue_batch(int K, int n, int degp1, int tailidx, int nbatch,
+                                   const double *mat_dev, const double *rhs_dev,
+                                   const double *sol_dev, double *res_dev, double *resmax_dev,
+                                   void *stream);
+
 /* the supports of a polynomial system for row Newton: polynomial i has
  * monomials monoptr[i]..monoptr[i+1]-1, monomial m the nvr[m] distinct
  * variables idx[idxptr[m]..idxptr[m+1]-1] (the nbr/nvr/idx arguments of
@@ -431,7 +450,8 @@ int phc_rowsys_create(phc_rowsys **sys, int dim, const int *monoptr, const int *
 int phc_rowsys_create_exp(phc_rowsys **sys, int dim, const int *monoptr, const int *nvr,
                           const int *idxptr, const int *idx, const int *exp);
 int phc_rowsys_destroy(phc_rowsys *sys);
-/* LDS bytes one system occupies at precision K and degree deg */
+/* LDS bytes one system occupies at degree deg with K doubles per number
+ * (pass 2K for the complex Newton) */
 int phc_rowsys_lds_bytes(const phc_rowsys *sys, int K, int deg, size_t *bytes);
 
 /* staggered row Newton on nsys systems with the same supports, one
@@ -449,6 +469,16 @@ int phc_row_newton(phc_rowsys *sys, int K, int deg, int nbsteps, int nsys, const
                    int cst_shared, const double *cff_dev, int cff_shared, double *x_dev,
                    int *steps_dev, int *wdeg_dev, int *tail_dev, int *up_dev, int *bs_dev,
                    double *history_dev, double *Q_dev, double *R_dev, void *stream);
+
+/* the complex staggered row Newton (K = 2, 4; a system of distinct
+ * variables): cst, cff, x, history, Q, R complex limb-planar (2K planes),
+ * each step as cmplx2_row_newton_qrstep (Newton/cmplx2_newton_method.cpp:565)
+ * in the loop of test_cmplx2_row_newton (:1759), bit-identical to it. */
+int phc_cmplx_row_newton(phc_rowsys *sys, int K, int deg, int nbsteps, int nsys,
+                         const double *cst_dev, int cst_shared, const double *cff_dev,
+                         int cff_shared, double *x_dev, int *steps_dev, int *wdeg_dev,
+                         int *tail_dev, int *up_dev, int *bs_dev, double *history_dev,
+                         double *Q_dev, double *R_dev, void *stream);
 
 /* ------------- complex blocked Householder QR, tiled back substitution ----
  * K = 2, 4 or 8.  Matrices are complex limb-planar [2K][rows][cols],

@@ -725,3 +725,18 @@ void obq_real_blocked_houseqr(int K, int nrows, int ncols, int szt, int nbt, con
   free(q);
   free(r);
 }
+
+/* ------------------------------------------- shared with oracle/newton.c */
+
+void obq_cx_house(int K, int n, const double *x, double *v, double *beta) {
+  cx_house(K, n, x, v, beta);
+}
+
+void obq_cx_left_r_update(int K, int nrows, int ncols, int ldr, int k, double *R,
+                          const double *v, const double *beta, double *w) {
+  cx_left_r_update(K, nrows, ncols, ldr, k, R, v, beta, w);
+}
+
+void obq_cx_backward(int K, int dim, int ld, const double *U, const double *b, double *x) {
+  cx_backward(K, dim, ld, U, b, x);
+}

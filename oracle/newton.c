@@ -28,6 +28,19 @@
  * dbl2_bals_host.cpp:43 (1e-28), dbl4_bals_host.cpp:51 (1e-56),
  * dbl8_bals_host.cpp:70 (1e-120).
  *
+ * Complex (K = 2, 4, 8; the operation order of the complex double-double
+ * host code):
+ *   QR with explicit Q     CPU_cmplx2_factors_houseqr (Matrices/dbl2_factorizations.cpp:766-822)
+ *                          with _house, _leftRupdate (baqr.c) and _rightQupdate (:661-722)
+ *   Q^H b, back subst.     CPU_cmplx2_factors_qrbs (:842-879), _backward (:142-200)
+ *   norm                   CPU_cmplx_onenorm (Newton/dbl_onenorms_host.cpp:19-28)
+ *   Toeplitz solve         CPU_cmplx2_qrbs_head, _tail, _solve (Newton/dbl2_bals_host.cpp)
+ *   residual               CPU_cmplx2_linear_residue (Newton/dbl2_bals_host.cpp)
+ *   Newton step            cmplx2_row_newton_qrstep host branch (Newton/cmplx2_newton_method.cpp:565-790),
+ *                          cmplx2_update_newton_qrstep (:163-328),
+ *                          cmplx2_update_series (Newton/dbl2_newton_testers.cpp:151-180)
+ *   staggered degrees      test_cmplx2_row_newton (Newton/cmplx2_newton_method.cpp:1759)
+ *
  * Parity pinned by tests/test_oracle_newton.py: bit-exact against the
  * reference functions compiled in oracle/_ref (QR, Toeplitz solve, residual,
  * whole staggered runs at K = 1, 2, 4, 8) and against committed fixtures of
@@ -40,6 +53,7 @@
 #include "md.h"
 #include "series.h"
 #include "newton.h"
+#include "cxlin.h"
 
 #define MAXK 8
 #define E(a, i) ((a) + (size_t)(i) * K)
@@ -443,6 +457,352 @@ int onw_row_newton(int K, int dim, int deg, int nbsteps, const int *monoptr, con
   to_planar(K, (size_t)dim * D, x, input);
   if (Q) to_planar(K, nn, q, Q);
   if (R) to_planar(K, nn, r, R);
+  free(x);
+  free(jac);
+  free(rhs);
+  free(sol);
+  free(q);
+  free(r);
+  free(w);
+  free(pin);
+  free(pcst);
+  free(pcff);
+  free(pout);
+  free(nv);
+  free(ip);
+  return stepcnt;
+}
+
+/* ================================================================ complex */
+
+/* complex number i: 2K doubles, K real limbs then K imaginary limbs */
+#define ZE(a, i) ((a) + (size_t)(i) * 2 * K)
+#define ZR(z) (z)
+#define ZI(z) ((z) + K)
+
+/* CPU_cmplx2_factors_rightQupdate: Q -= (beta Q v) v^H on columns k.. */
+static void cx_right_q_update(int K, int n, int k, double *Q, const double *v, const double *beta,
+                              double *w) {
+  double zre[MAXK], zim[MAXK], acc[MAXK];
+  int i, j;
+  for (i = 0; i < n; i++) {
+    double *wi = ZE(w, i);
+    omd_zero(K, ZR(wi));
+    omd_zero(K, ZI(wi));
+    for (j = k; j < n; j++) {
+      const double *q = ZE(Q, (size_t)i * n + j), *vj = ZE(v, j - k);
+      omd_mul(K, ZR(q), ZR(vj), zre);
+      omd_mul(K, ZI(q), ZI(vj), acc);
+      omd_sub(K, zre, acc, zre);
+      omd_mul(K, ZI(q), ZR(vj), zim);
+      omd_mul(K, ZR(q), ZI(vj), acc);
+      omd_add(K, zim, acc, zim);
+      omd_add(K, ZR(wi), zre, ZR(wi));
+      omd_add(K, ZI(wi), zim, ZI(wi));
+    }
+    omd_mul(K, beta, ZR(wi), ZR(wi));
+    omd_mul(K, beta, ZI(wi), ZI(wi));
+  }
+  for (i = 0; i < n; i++)
+    for (j = k; j < n; j++) {
+      double *q = ZE(Q, (size_t)i * n + j);
+      const double *wi = ZE(w, i), *vj = ZE(v, j - k);
+      omd_mul(K, ZR(wi), ZR(vj), zre);
+      omd_mul(K, ZI(wi), ZI(vj), acc);
+      omd_add(K, zre, acc, zre);
+      omd_mul(K, ZI(wi), ZR(vj), zim);
+      omd_mul(K, ZR(wi), ZI(vj), acc);
+      omd_sub(K, zim, acc, zim);
+      omd_sub(K, ZR(q), zre, ZR(q));
+      omd_sub(K, ZI(q), zim, ZI(q));
+    }
+}
+
+static void cx_houseqr(int K, int nrows, int ncols, const double *A, double *Q, double *R) {
+  const size_t z = sizeof(double) * 2 * K;
+  const int big = nrows > ncols ? nrows : ncols;
+  double *x = malloc(z * nrows), *v = malloc(z * nrows), *w = malloc(z * big);
+  double beta[MAXK];
+  int i, j, k;
+  for (i = 0; i < nrows; i++) {
+    for (j = 0; j < nrows; j++) memset(ZE(Q, (size_t)i * nrows + j), 0, z);
+    omd_from_d(K, 1.0, ZR(ZE(Q, (size_t)i * nrows + i)));
+    for (j = 0; j < ncols; j++) memcpy(ZE(R, (size_t)i * ncols + j), ZE(A, (size_t)i * ncols + j), z);
+  }
+  for (k = 0; k < ncols; k++) {
+    if (nrows - k > 0) {
+      for (i = k; i < nrows; i++) memcpy(ZE(x, i - k), ZE(R, (size_t)i * ncols + k), z);
+      obq_cx_house(K, nrows - k, x, v, beta);
+      obq_cx_left_r_update(K, nrows, ncols, ncols, k, R, v, beta, w);
+      cx_right_q_update(K, nrows, k, Q, v, beta, w);
+    }
+  }
+  free(x);
+  free(v);
+  free(w);
+}
+
+/* x = R^{-1} Q^H b */
+static void cx_qrbs(int K, int dim, const double *Q, const double *R, const double *b, double *x,
+                    double *wrk) {
+  double acc[MAXK];
+  int i, j;
+  for (i = 0; i < dim; i++) {
+    double *wi = ZE(wrk, i);
+    omd_zero(K, ZR(wi));
+    omd_zero(K, ZI(wi));
+    for (j = 0; j < dim; j++) {
+      const double *q = ZE(Q, (size_t)j * dim + i), *bj = ZE(b, j);
+      omd_mul(K, ZR(q), ZR(bj), acc);
+      omd_add(K, ZR(wi), acc, ZR(wi));
+      omd_mul(K, ZI(q), ZI(bj), acc);
+      omd_add(K, ZR(wi), acc, ZR(wi));
+      omd_mul(K, ZR(q), ZI(bj), acc);
+      omd_add(K, ZI(wi), acc, ZI(wi));
+      omd_mul(K, ZI(q), ZR(bj), acc);
+      omd_sub(K, ZI(wi), acc, ZI(wi));
+    }
+  }
+  obq_cx_backward(K, dim, dim, R, wrk, x);
+}
+
+/* max of |re| + |im| of the leading limbs (NaN only when v[0]'s is) */
+static double cx_maxnorm(int K, int dim, const double *v) {
+  double r = fabs(ZR(v)[0]) + fabs(ZI(v)[0]);
+  int i;
+  for (i = 1; i < dim; i++) {
+    const double a = fabs(ZR(ZE(v, i))[0]) + fabs(ZI(ZE(v, i))[0]);
+    if (a > r) r = a;
+  }
+  return r;
+}
+
+static void cx_qrbs_solve(int K, int dim, int degp1, int tailidx, const double *mat, double *rhs,
+                          double *sol, double *Q, double *R, double *wrk, int *zeroQ, int *noqr,
+                          int *upidx, int *bsidx, int *newtail) {
+  const double thr = onw_threshold(K);
+  const size_t nn = (size_t)dim * dim;
+  double acc[MAXK];
+  int i, j, k, L;
+  if (!(*noqr && !*zeroQ)) {
+    if (cx_maxnorm(K, dim, rhs) < thr) *noqr = *zeroQ ? 0 : 1;
+    if (!*noqr) {
+      cx_houseqr(K, dim, dim, mat, Q, R);
+      *zeroQ = 0;
+      cx_qrbs(K, dim, Q, R, rhs, sol, wrk);
+    }
+  }
+  if (degp1 > 1) {
+    int skipup = 0, skipbs = 0, firstbs = 1;
+    *newtail = degp1;
+    for (i = tailidx; i < degp1; i++) {
+      const double *xi = ZE(sol, (size_t)(i - 1) * dim);
+      if (cx_maxnorm(K, dim, xi) < thr) {
+        skipup++;
+      } else {
+        for (j = i; j < degp1; j++) {
+          const double *Aj = ZE(mat, (size_t)(j - i + 1) * nn);
+          double *wj = ZE(rhs, (size_t)j * dim);
+          for (k = 0; k < dim; k++)
+            for (L = 0; L < dim; L++) {
+              const double *a = ZE(Aj, (size_t)k * dim + L), *x = ZE(xi, L);
+              double *wk = ZE(wj, k);
+              omd_mul(K, ZR(a), ZR(x), acc);
+              omd_sub(K, ZR(wk), acc, ZR(wk));
+              omd_mul(K, ZI(a), ZI(x), acc);
+              omd_add(K, ZR(wk), acc, ZR(wk));
+              omd_mul(K, ZR(a), ZI(x), acc);
+              omd_sub(K, ZI(wk), acc, ZI(wk));
+              omd_mul(K, ZI(a), ZR(x), acc);
+              omd_sub(K, ZI(wk), acc, ZI(wk));
+            }
+        }
+      }
+      {
+        double *x = ZE(sol, (size_t)i * dim);
+        const double *b = ZE(rhs, (size_t)i * dim);
+        const double nrm = cx_maxnorm(K, dim, b);
+        if (nrm < thr || nrm > 1.0e+99) {
+          skipbs++;
+          memset(x, 0, sizeof(double) * 2 * K * dim);
+        } else {
+          if (firstbs) {
+            *newtail = i;
+            firstbs = 0;
+          }
+          cx_qrbs(K, dim, Q, R, b, x, wrk);
+        }
+      }
+    }
+    *upidx = skipup;
+    *bsidx = skipbs;
+  }
+}
+
+void onw_cx_houseqr(int K, int nrows, int ncols, const double *A, double *Q, double *R) {
+  const int P = 2 * K;
+  double *a = malloc(sizeof(double) * P * nrows * ncols);
+  double *q = malloc(sizeof(double) * P * nrows * nrows);
+  double *r = malloc(sizeof(double) * P * nrows * ncols);
+  from_planar(P, (size_t)nrows * ncols, A, a);
+  cx_houseqr(K, nrows, ncols, a, q, r);
+  to_planar(P, (size_t)nrows * nrows, q, Q);
+  to_planar(P, (size_t)nrows * ncols, r, R);
+  free(a);
+  free(q);
+  free(r);
+}
+
+void onw_cx_qrbs_solve(int K, int dim, int degp1, int tailidx, const double *mat, double *rhs,
+                       double *sol, double *Q, double *R, int *zeroQ, int *noqr, int *upidx,
+                       int *bsidx, int *newtail) {
+  const int P = 2 * K;
+  const size_t nn = (size_t)dim * dim, nv = (size_t)degp1 * dim;
+  double *m = malloc(sizeof(double) * P * nn * degp1);
+  double *b = malloc(sizeof(double) * P * nv), *x = malloc(sizeof(double) * P * nv);
+  double *q = malloc(sizeof(double) * P * nn), *r = malloc(sizeof(double) * P * nn);
+  double *w = malloc(sizeof(double) * P * dim);
+  from_planar(P, nn * degp1, mat, m);
+  from_planar(P, nv, rhs, b);
+  from_planar(P, nv, sol, x);
+  from_planar(P, nn, Q, q);
+  from_planar(P, nn, R, r);
+  cx_qrbs_solve(K, dim, degp1, tailidx, m, b, x, q, r, w, zeroQ, noqr, upidx, bsidx, newtail);
+  to_planar(P, nv, b, rhs);
+  to_planar(P, nv, x, sol);
+  to_planar(P, nn, q, Q);
+  to_planar(P, nn, r, R);
+  free(m);
+  free(b);
+  free(x);
+  free(q);
+  free(r);
+  free(w);
+}
+
+void onw_cx_linear_residue(int K, int dim, int degp1, int tailidx, const double *mat,
+                           const double *rhs, const double *sol, double *resvec, double *resmax) {
+  const int P = 2 * K;
+  const size_t nn = (size_t)dim * dim, nv = (size_t)degp1 * dim;
+  double *m = malloc(sizeof(double) * P * nn * degp1);
+  double *b = malloc(sizeof(double) * P * nv), *x = malloc(sizeof(double) * P * nv);
+  double *r = calloc((size_t)P * nv, sizeof(double));
+  double acc[MAXK];
+  int i, j, k, L, q;
+  from_planar(P, nn * degp1, mat, m);
+  from_planar(P, nv, rhs, b);
+  from_planar(P, nv, sol, x);
+  from_planar(P, nv, resvec, r);
+  for (q = 0; q < K; q++) resmax[q] = 0.0;
+  for (i = tailidx; i < degp1; i++) {
+    double *ri = ZE(r, (size_t)i * dim);
+    memcpy(ri, ZE(b, (size_t)i * dim), sizeof(double) * P * dim);
+    for (j = 0; j <= i - tailidx; j++) {
+      const double *Aj = ZE(m, (size_t)j * nn);
+      const double *xj = ZE(x, (size_t)(i - j) * dim);
+      for (k = 0; k < dim; k++)
+        for (L = 0; L < dim; L++) {
+          const double *a = ZE(Aj, (size_t)L * dim + k), *xk = ZE(xj, k);
+          double *rl = ZE(ri, L);
+          omd_mul(K, ZR(a), ZR(xk), acc);
+          omd_sub(K, ZR(rl), acc, ZR(rl));
+          omd_mul(K, ZI(a), ZI(xk), acc);
+          omd_add(K, ZR(rl), acc, ZR(rl));
+          omd_mul(K, ZR(a), ZI(xk), acc);
+          omd_sub(K, ZI(rl), acc, ZI(rl));
+          omd_mul(K, ZI(a), ZR(xk), acc);
+          omd_sub(K, ZI(rl), acc, ZI(rl));
+        }
+    }
+    for (j = 0; j < dim; j++) {
+      const double *rj = ZE(ri, j);
+      if (fabs(ZR(rj)[0]) + fabs(ZI(rj)[0]) > resmax[0])
+        for (q = 0; q < K; q++) resmax[q] = fabs(ZR(rj)[q]) + fabs(ZI(rj)[q]);
+    }
+  }
+  to_planar(P, nv, r, resvec);
+  free(m);
+  free(b);
+  free(x);
+  free(r);
+}
+
+int onw_cx_row_newton(int K, int dim, int deg, int nbsteps, const int *monoptr, const int *nvr,
+                      const int *idxptr, const int *idx, const double *cst, const double *cff,
+                      double *input, int *wdeg_out, int *tail_out, int *up_out, int *bs_out,
+                      double *history, double *Q, double *R) {
+  const int D = deg + 1, P = 2 * K;
+  const int nmono = monoptr[dim];
+  const size_t nn = (size_t)dim * dim;
+  double *x = malloc(sizeof(double) * P * dim * D);
+  double *jac = malloc(sizeof(double) * P * nn * D);
+  double *rhs = malloc(sizeof(double) * P * dim * D);
+  double *sol = malloc(sizeof(double) * P * dim * D);
+  double *q = calloc((size_t)P * nn, sizeof(double)), *r = calloc((size_t)P * nn, sizeof(double));
+  double *w = malloc(sizeof(double) * P * dim);
+  double *pin = malloc(sizeof(double) * P * dim * D);
+  double *pcst = malloc(sizeof(double) * P * D);
+  double *pcff = malloc(sizeof(double) * P * (nmono > 0 ? nmono : 1) * D);
+  double *pout = malloc(sizeof(double) * P * (dim + 1) * D);
+  int *nv = malloc(sizeof(int) * (nmono > 0 ? nmono : 1));
+  int *ip = malloc(sizeof(int) * (nmono > 0 ? nmono : 1));
+  int tailidx = 1, wrkdeg = 0, zeroQ = 1, noqr = 0, upidx = 0, bsidx = 0, step, stepcnt = 0;
+  int i, j, k, p;
+  from_planar(P, (size_t)dim * D, input, x);
+  for (step = 0; step < nbsteps; step++) {
+    const int Dw = wrkdeg + 1;
+    for (p = 0; p < P; p++)
+      for (i = 0; i < dim; i++)
+        for (k = 0; k < Dw; k++) pin[((size_t)p * dim + i) * Dw + k] = ZE(x, (size_t)i * D + k)[p];
+    for (i = 0; i < dim; i++) {
+      const int m0 = monoptr[i], nb = monoptr[i + 1] - monoptr[i];
+      for (p = 0; p < P; p++) {
+        for (k = 0; k < Dw; k++) pcst[(size_t)p * Dw + k] = cst[((size_t)p * dim + i) * D + k];
+        for (j = 0; j < nb; j++)
+          for (k = 0; k < Dw; k++)
+            pcff[((size_t)p * nb + j) * Dw + k] = cff[((size_t)p * nmono + m0 + j) * D + k];
+      }
+      for (j = 0; j < nb; j++) {
+        nv[j] = nvr[m0 + j];
+        ip[j] = idxptr[m0 + j];
+      }
+      osr_poly_evaldiff(K, 1, dim, nb, wrkdeg, nv, ip, idx, pcst, pcff, pin, pout);
+      for (k = 0; k < Dw; k++) {
+        double *dst = ZE(rhs, (size_t)k * dim + i);
+        for (p = 0; p < P; p++) dst[p] = -pout[((size_t)p * (dim + 1) + dim) * Dw + k];
+        for (j = 0; j < dim; j++) {
+          double *jd = ZE(jac, (size_t)k * nn + (size_t)i * dim + j);
+          for (p = 0; p < P; p++) jd[p] = pout[((size_t)p * (dim + 1) + j) * Dw + k];
+        }
+      }
+    }
+    memset(sol, 0, sizeof(double) * P * dim * Dw);
+    {
+      int newtail = tailidx;
+      cx_qrbs_solve(K, dim, Dw, tailidx, jac, rhs, sol, q, r, w, &zeroQ, &noqr, &upidx, &bsidx,
+                    &newtail);
+      tailidx = newtail;
+    }
+    for (k = tailidx - 1; k < Dw; k++)
+      for (i = 0; i < dim; i++) {
+        double *xi = ZE(x, (size_t)i * D + k);
+        const double *d = ZE(sol, (size_t)k * dim + i);
+        omd_add(K, ZR(xi), ZR(d), ZR(xi));
+        omd_add(K, ZI(xi), ZI(d), ZI(xi));
+      }
+    if (wdeg_out) wdeg_out[step] = wrkdeg;
+    if (tail_out) tail_out[step] = tailidx;
+    if (up_out) up_out[step] = upidx;
+    if (bs_out) bs_out[step] = bsidx;
+    if (history) to_planar(P, (size_t)dim * D, x, history + (size_t)step * P * dim * D);
+    stepcnt++;
+    if (tailidx >= deg) break;
+    wrkdeg = wrkdeg + 1 + wrkdeg / 2;
+    if (wrkdeg > deg) wrkdeg = deg;
+  }
+  to_planar(P, (size_t)dim * D, x, input);
+  if (Q) to_planar(P, nn, q, Q);
+  if (R) to_planar(P, nn, r, R);
   free(x);
   free(jac);
   free(rhs);

@@ -49,6 +49,20 @@ int onw_row_newton(int K, int dim, int deg, int nbsteps, const int *monoptr, con
                    const double *cff, double *input, int *wdeg, int *tail, int *up, int *bs,
                    double *history, double *Q, double *R);
 
+/* complex (K = 2, 4, 8), arrays complex limb-planar [2K][...] (K real
+ * planes, K imaginary planes); resmax [K] of |re| + |im| per limb */
+void onw_cx_houseqr(int K, int nrows, int ncols, const double *A, double *Q, double *R);
+void onw_cx_qrbs_solve(int K, int dim, int degp1, int tailidx, const double *mat, double *rhs,
+                       double *sol, double *Q, double *R, int *zeroQ, int *noqr, int *upidx,
+                       int *bsidx, int *newtail);
+void onw_cx_linear_residue(int K, int dim, int degp1, int tailidx, const double *mat,
+                           const double *rhs, const double *sol, double *resvec, double *resmax);
+/* the complex row Newton of distinct variables (no exponents) */
+int onw_cx_row_newton(int K, int dim, int deg, int nbsteps, const int *monoptr, const int *nvr,
+                      const int *idxptr, const int *idx, const double *cst, const double *cff,
+                      double *input, int *wdeg_out, int *tail_out, int *up_out, int *bs_out,
+                      double *history, double *Q, double *R);
+
 #ifdef __cplusplus
 }
 #endif

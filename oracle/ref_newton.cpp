@@ -102,6 +102,47 @@ DEFREF(4, dbl4, L4)
 DEFREF(8, dbl8, L8)
 #undef DEFREF
 
+// the complex functions (K = 2, 4, 8 limbs, 2K planes: real limbs, then
+// imaginary limbs; no octo-double complex polynomial evaluation)
+#define L16(a) L8(a), a[8], a[9], a[10], a[11], a[12], a[13], a[14], a[15]
+template <int K> struct CRef;
+#define DEFCREF(KK, PFX, LX, LK)                                                              \
+  template <> struct CRef<KK> {                                                               \
+    static void houseqr(int nr, int nc, double **A[], double **Q[], double **R[]) {           \
+      CPU_##PFX##_factors_houseqr(nr, nc, LX(A), LX(Q), LX(R));                               \
+    }                                                                                         \
+    static void solve(int dim, int degp1, int tail, double ***M[], double **B[],             \
+                      double **X[], double **Q[], double **R[], double *W[], bool *zq,        \
+                      bool *nq, int *up, int *bs, int *nt) {                                  \
+      CPU_##PFX##_qrbs_solve(dim, degp1, tail, LX(M), LX(B), LX(X), LX(Q), LX(R), LX(W), zq,  \
+                             nq, up, bs, nt, 0);                                              \
+    }                                                                                         \
+    static void residue(int dim, int degp1, int tail, double ***M[], double **B[],           \
+                        double **X[], double **Rv[], double *rm[]) {                          \
+      CPU_##PFX##_linear_residue(dim, degp1, tail, LX(M), LX(B), LX(X), LX(Rv), LK(rm), 0);   \
+    }                                                                                         \
+  };
+DEFCREF(2, cmplx2, L4, L2)
+DEFCREF(4, cmplx4, L8, L4)
+DEFCREF(8, cmplx8, L16, L8)
+#undef DEFCREF
+
+template <int K> struct CEval;
+template <> struct CEval<2> {
+  static void evaldiff(int dim, int nbr, int deg, int *nvr, int **idx, double *c[],
+                       double **f[], double **in[], double **out[]) {
+    double el = 0.0;
+    CPU_cmplx2_poly_evaldiff(dim, nbr, deg, nvr, idx, L4(c), L4(f), L4(in), L4(out), &el, 0);
+  }
+};
+template <> struct CEval<4> {
+  static void evaldiff(int dim, int nbr, int deg, int *nvr, int **idx, double *c[],
+                       double **f[], double **in[], double **out[]) {
+    double el = 0.0;
+    CPU_cmplx4_poly_evaldiff(dim, nbr, deg, nvr, idx, L8(c), L8(f), L8(in), L8(out), &el, 0);
+  }
+};
+
 // x += dx with the reference's in-place addition of each precision
 // (dbl2_update_series, Newton/dbl2_newton_testers.cpp:116-149, and its twins)
 template <int K> void inc(double *x[], const double *dx[]);
@@ -348,9 +389,175 @@ int row_newton_k(int dim, int deg, int nbsteps, const int *monoptr, const int *n
   return stepcnt;
 }
 
+template <int K>
+void cx_houseqr_k(int nrows, int ncols, double *A, double *Q, double *R) {
+  Mats a(A, 2 * K, 1, nrows, ncols), q(Q, 2 * K, 1, nrows, nrows), r(R, 2 * K, 1, nrows, ncols);
+  double **pa[2 * K], **pq[2 * K], **pr[2 * K];
+  for (int p = 0; p < 2 * K; ++p) { pa[p] = a.mat(p); pq[p] = q.mat(p); pr[p] = r.mat(p); }
+  CRef<K>::houseqr(nrows, ncols, pa, pq, pr);
+}
+
+template <int K>
+void cx_solve_k(int dim, int degp1, int tail, double *mat, double *rhs, double *sol, double *Q,
+                double *R, int *zeroQ, int *noqr, int *up, int *bs, int *nt) {
+  const int P = 2 * K;
+  Mats m(mat, P, degp1, dim, dim), b(rhs, P, 1, degp1, dim), x(sol, P, 1, degp1, dim);
+  Mats q(Q, P, 1, dim, dim), r(R, P, 1, dim, dim);
+  std::vector<double> wrk((size_t)P * dim);
+  double ***pm[P], **pb[P], **px[P], **pq[P], **pr[P], *pw[P];
+  for (int p = 0; p < P; ++p) {
+    pm[p] = m.ser(p); pb[p] = b.mat(p); px[p] = x.mat(p); pq[p] = q.mat(p); pr[p] = r.mat(p);
+    pw[p] = wrk.data() + (size_t)p * dim;
+  }
+  bool zq = *zeroQ != 0, nq = *noqr != 0;
+  CRef<K>::solve(dim, degp1, tail, pm, pb, px, pq, pr, pw, &zq, &nq, up, bs, nt);
+  *zeroQ = zq;
+  *noqr = nq;
+}
+
+template <int K>
+void cx_residue_k(int dim, int degp1, int tail, double *mat, double *rhs, double *sol,
+                  double *resvec, double *resmax) {
+  const int P = 2 * K;
+  Mats m(mat, P, degp1, dim, dim), b(rhs, P, 1, degp1, dim), x(sol, P, 1, degp1, dim);
+  Mats rv(resvec, P, 1, degp1, dim);
+  double ***pm[P], **pb[P], **px[P], **pr[P], *rm[K];
+  for (int p = 0; p < P; ++p) {
+    pm[p] = m.ser(p); pb[p] = b.mat(p); px[p] = x.mat(p); pr[p] = rv.mat(p);
+  }
+  for (int p = 0; p < K; ++p) rm[p] = resmax + p;
+  CRef<K>::residue(dim, degp1, tail, pm, pb, px, pr, rm);
+}
+
+// The host branch of the staggered complex row Newton test
+// (test_cmplx2_row_newton, Newton/cmplx2_newton_method.cpp:1759, with
+// cmplx2_row_newton_qrstep :565-790, cmplx2_update_newton_qrstep :163-328
+// and cmplx2_update_series, Newton/dbl2_newton_testers.cpp:151-180).
+template <int K>
+int cx_row_newton_k(int dim, int deg, int nbsteps, const int *monoptr, const int *nvr,
+                    const int *idxptr, const int *idxflat, double *cst, double *cff,
+                    double *input, int *wdeg, int *tails, int *ups, int *bss, double *history,
+                    double *Qo, double *Ro) {
+  const int D = deg + 1, nmono = monoptr[dim], P = 2 * K;
+  std::vector<int> nv(nvr, nvr + (nmono > 0 ? nmono : 1));
+  std::vector<int> idxc(idxflat, idxflat + idxptr[nmono]);
+  std::vector<int *> idx(nmono > 0 ? nmono : 1);
+  for (int m = 0; m < nmono; ++m) idx[m] = idxc.data() + idxptr[m];
+  Mats in(input, P, 1, dim, D);
+  Mats cf(cff, P, 1, nmono > 0 ? nmono : 1, D);
+  std::vector<double> out((size_t)P * dim * (dim + 1) * D);
+  Mats ou(out.data(), P, dim, dim + 1, D);
+  std::vector<double> jac((size_t)P * D * dim * dim), urhs((size_t)P * D * dim),
+      sol((size_t)P * D * dim), Q((size_t)P * dim * dim), R((size_t)P * dim * dim),
+      wrk((size_t)P * dim);
+  Mats jm(jac.data(), P, D, dim, dim), ub(urhs.data(), P, 1, D, dim), sx(sol.data(), P, 1, D, dim);
+  Mats qm(Q.data(), P, 1, dim, dim), rm(R.data(), P, 1, dim, dim);
+  double ***pm[P], **pb[P], **px[P], **pq[P], **pr[P], *pw[P], **pin[P];
+  for (int p = 0; p < P; ++p) {
+    pm[p] = jm.ser(p); pb[p] = ub.mat(p); px[p] = sx.mat(p); pq[p] = qm.mat(p);
+    pr[p] = rm.mat(p); pw[p] = wrk.data() + (size_t)p * dim; pin[p] = in.mat(p);
+  }
+  bool zeroQ = true, noqr = false;
+  int tailidx = 1, wrkdeg = 0, up = 0, bs = 0, stepcnt = 0;
+  for (int step = 0; step < nbsteps; ++step) {
+    const int degp1 = wrkdeg + 1;
+    for (int i = 0; i < dim; ++i) {
+      double *c[P], **f[P], **o[P];
+      for (int p = 0; p < P; ++p) {
+        c[p] = cst + ((size_t)p * dim + i) * D;
+        f[p] = cf.mat(p) + monoptr[i];
+        o[p] = ou.ser(p)[i];
+      }
+      CEval<K>::evaldiff(dim, monoptr[i + 1] - monoptr[i], wrkdeg, nv.data() + monoptr[i],
+                         idx.data() + monoptr[i], c, f, pin, o);
+    }
+    for (int p = 0; p < P; ++p)
+      for (int i = 0; i < dim; ++i)
+        for (int k = 0; k < degp1; ++k) {
+          urhs[((size_t)p * D + k) * dim + i] = -ou.ser(p)[i][dim][k];
+          sol[((size_t)p * D + k) * dim + i] = 0.0;
+          for (int j = 0; j < dim; ++j)
+            jac[(((size_t)p * D + k) * dim + i) * dim + j] = ou.ser(p)[i][j][k];
+        }
+    int newtail = tailidx;
+    CRef<K>::solve(dim, degp1, tailidx, pm, pb, px, pq, pr, pw, &zeroQ, &noqr, &up, &bs, &newtail);
+    tailidx = newtail;
+    for (int j = tailidx - 1; j < degp1; ++j)
+      for (int i = 0; i < dim; ++i) {
+        double *xr[K], *xi[K];
+        const double *dr[K], *di[K];
+        for (int p = 0; p < K; ++p) {
+          xr[p] = &pin[p][i][j];
+          xi[p] = &pin[K + p][i][j];
+          dr[p] = &px[p][j][i];
+          di[p] = &px[K + p][j][i];
+        }
+        inc<K>(xr, dr);
+        inc<K>(xi, di);
+      }
+    if (wdeg) wdeg[step] = wrkdeg;
+    if (tails) tails[step] = tailidx;
+    if (ups) ups[step] = up;
+    if (bss) bss[step] = bs;
+    if (history) std::memcpy(history + (size_t)step * P * dim * D, input, sizeof(double) * P * dim * D);
+    ++stepcnt;
+    if (tailidx >= deg) break;
+    wrkdeg = wrkdeg + 1 + wrkdeg / 2;
+    if (wrkdeg > deg) wrkdeg = deg;
+  }
+  if (Qo) std::memcpy(Qo, Q.data(), sizeof(double) * P * dim * dim);
+  if (Ro) std::memcpy(Ro, R.data(), sizeof(double) * P * dim * dim);
+  return stepcnt;
+}
+
 }  // namespace
 
 extern "C" {
+
+int ref_cx_houseqr(int K, int nrows, int ncols, double *A, double *Q, double *R) {
+  switch (K) {
+    case 2: cx_houseqr_k<2>(nrows, ncols, A, Q, R); return 0;
+    case 4: cx_houseqr_k<4>(nrows, ncols, A, Q, R); return 0;
+    case 8: cx_houseqr_k<8>(nrows, ncols, A, Q, R); return 0;
+  }
+  return -1;
+}
+
+int ref_cx_qrbs_solve(int K, int dim, int degp1, int tailidx, double *mat, double *rhs,
+                      double *sol, double *Q, double *R, int *zeroQ, int *noqr, int *upidx,
+                      int *bsidx, int *newtail) {
+  switch (K) {
+#define CASE(KK) \
+    case KK: cx_solve_k<KK>(dim, degp1, tailidx, mat, rhs, sol, Q, R, zeroQ, noqr, upidx, bsidx, newtail); return 0;
+    CASE(2) CASE(4) CASE(8)
+#undef CASE
+  }
+  return -1;
+}
+
+int ref_cx_linear_residue(int K, int dim, int degp1, int tailidx, double *mat, double *rhs,
+                          double *sol, double *resvec, double *resmax) {
+  switch (K) {
+#define CASE(KK) case KK: cx_residue_k<KK>(dim, degp1, tailidx, mat, rhs, sol, resvec, resmax); return 0;
+    CASE(2) CASE(4) CASE(8)
+#undef CASE
+  }
+  return -1;
+}
+
+int ref_cx_row_newton(int K, int dim, int deg, int nbsteps, const int *monoptr, const int *nvr,
+                      const int *idxptr, const int *idx, double *cst, double *cff, double *input,
+                      int *wdeg, int *tail, int *up, int *bs, double *history, double *Q,
+                      double *R) {
+  if (K == 2)
+    return cx_row_newton_k<2>(dim, deg, nbsteps, monoptr, nvr, idxptr, idx, cst, cff, input, wdeg,
+                              tail, up, bs, history, Q, R);
+  if (K == 4)
+    return cx_row_newton_k<4>(dim, deg, nbsteps, monoptr, nvr, idxptr, idx, cst, cff, input, wdeg,
+                              tail, up, bs, history, Q, R);
+  return -1;
+}
+
 
 int ref_houseqr(int K, int nrows, int ncols, double *A, double *Q, double *R) {
   switch (K) {

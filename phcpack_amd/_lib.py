@@ -129,6 +129,18 @@ SIGNATURES.update({
     "phc_row_newton": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p,
                                c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_void_p, c_void_p, c_void_p, c_void_p]),
+    "phc_cmplx_row_newton": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int,
+                                     c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p]),
+    "phc_cmplx_houseqr_batch": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                        c_void_p]),
+    "phc_cmplx_qrbs_solve_batch": (c_int, [c_int, c_int, c_int, c_int, c_int, c_void_p,
+                                           c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                           c_void_p]),
+    "phc_cmplx_linear_residue_batch": (c_int, [c_int, c_int, c_int, c_int, c_int, c_void_p,
+                                               c_void_p, c_void_p, c_void_p, c_void_p,
+                                               c_void_p]),
     "phc_baqr_create": (c_int, [ctypes.POINTER(c_void_p), c_int, c_int, c_int, c_int]),
     "phc_baqr_destroy": (c_int, [c_void_p]),
     "phc_baqr_set_tuning": (c_int, [c_void_p, c_int, c_int]),

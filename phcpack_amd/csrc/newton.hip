@@ -1,5 +1,9 @@
-// Linearized power-series Newton on real multiple-double series, batched:
-// one wavefront runs the whole staggered Newton method of one system in LDS.
+// Linearized power-series Newton on real and complex multiple-double series,
+// batched: one wavefront runs the whole staggered Newton method of one system
+// in LDS.  The complex twins (CX) follow CPU_cmplx2_factors_houseqr,
+// CPU_cmplx2_factors_qrbs, CPU_cmplx2_qrbs_head/tail/solve,
+// CPU_cmplx2_linear_residue and cmplx2_row_newton_qrstep
+// (Newton/cmplx2_newton_method.cpp:565) with their operation orders.
 //
 //   reference (host)                          here
 //   CPU_dbl2_poly_evaldiff per polynomial     eval_system (slots of lanes, one
@@ -52,13 +56,64 @@ template <int K> __host__ __device__ constexpr double threshold() {
   return K == 1 ? 1.0e-15 : K == 2 ? 1.0e-28 : K == 4 ? 1.0e-56 : 1.0e-120;
 }
 
-// a planar LDS (or global) array of m numbers with K limbs
-template <int K> struct Arr {
-  typedef typename RealT<K>::type T;
+// Real (CX = false) or complex (CX = true, K real planes then K imaginary
+// planes) multiple-double numbers: the operations the Newton method needs,
+// each in the order of the reference's host code of that type.
+template <int K, bool CX> struct NT;
+template <int K> struct NT<K, false> {
+  typedef typename RealT<K>::type V;
+  static constexpr int P = K;
+  __device__ static V ld(const double *b, size_t m, size_t j) { return R<K>::load(b, m, j); }
+  __device__ static void st(double *b, size_t m, size_t j, const V &v) { R<K>::store(b, m, j, v); }
+  __device__ static V zero() { return R<K>::zero(); }
+  __device__ static V one() { return R<K>::from(1.0); }
+  __device__ static V add(const V &a, const V &b) { return md::add(a, b); }
+  __device__ static V neg(const V &a) { return md::neg(a); }
+  __device__ static V mul(const V &a, const V &b) { return md::mul(a, b); }
+  __device__ static V mac(const V &c, const V &a, const V &b) { return md::add(c, md::mul(a, b)); }
+  // b - a x, the update of the tail solve and the residual
+  __device__ static V msub(const V &b, const V &a, const V &x) { return md::sub(b, md::mul(a, x)); }
+  // the magnitude CPU_dbl_onenorm takes: |leading limb|
+  __device__ static double mag(const V &a) { return fabs(R<K>::hi(a)); }
+};
+template <int K> struct NT<K, true> {
+  typedef cplx<K> V;
+  static constexpr int P = 2 * K;
+  __device__ static V ld(const double *b, size_t m, size_t j) { return C<K>::load(b, m, j); }
+  __device__ static void st(double *b, size_t m, size_t j, const V &v) { C<K>::store(b, m, j, v); }
+  __device__ static V zero() { return C<K>::zero(); }
+  __device__ static V one() { return C<K>::from(1.0, 0.0); }
+  __device__ static V add(const V &a, const V &b) { return C<K>::add(a, b); }
+  __device__ static V neg(const V &a) { return C<K>::neg(a); }
+  // the first term and the inner loop of CPU_cmplx2_product
+  __device__ static V mul(const V &a, const V &b) { return C<K>::mul(a, b); }
+  __device__ static V mac(const V &c, const V &a, const V &b) {
+    V z = c;
+    C<K>::mac(z, a, b);
+    return z;
+  }
+  // b - a x as CPU_cmplx2_qrbs_tail and CPU_cmplx2_linear_residue subtract
+  // it: re -= a.re x.re, re += a.im x.im, im -= a.re x.im, im -= a.im x.re
+  __device__ static V msub(const V &b, const V &a, const V &x) {
+    V z;
+    z.re = md::sub(b.re, md::mul(a.re, x.re));
+    z.re = md::add(z.re, md::mul(a.im, x.im));
+    z.im = md::sub(b.im, md::mul(a.re, x.im));
+    z.im = md::sub(z.im, md::mul(a.im, x.re));
+    return z;
+  }
+  // CPU_cmplx_onenorm's |re| + |im| of the leading limbs
+  __device__ static double mag(const V &a) { return fabs(R<K>::hi(a.re)) + fabs(R<K>::hi(a.im)); }
+};
+
+// a planar LDS (or global) array of m numbers with K limbs (2K planes when
+// complex)
+template <int K, bool CX = false> struct Arr {
+  typedef typename NT<K, CX>::V T;
   double *b;
   int m;
-  __device__ T ld(int j) const { return R<K>::load(b, m, j); }
-  __device__ void st(int j, const T &v) const { R<K>::store(b, m, j, v); }
+  __device__ T ld(int j) const { return NT<K, CX>::ld(b, m, j); }
+  __device__ void st(int j, const T &v) const { NT<K, CX>::st(b, m, j, v); }
 };
 
 __device__ __forceinline__ bool all_zero(double a) { return a == 0.0; }
@@ -72,23 +127,23 @@ __device__ __forceinline__ double limb(double a, int) { return a; }
 template <int K> __device__ __forceinline__ double limb(const mdr<K> &a, int i) { return a.x[i]; }
 
 // carve a planar array of m numbers out of LDS
-template <int K> __device__ Arr<K> carve(double *&p, int m) {
-  Arr<K> a{p, m};
-  p += (size_t)K * m;
+template <int K, bool CX = false> __device__ Arr<K, CX> carve(double *&p, int m) {
+  Arr<K, CX> a{p, m};
+  p += (size_t)NT<K, CX>::P * m;
   return a;
 }
 
 // largest |leading limb| of v[0..n-1], as CPU_dbl_onenorm computes it
 // (Newton/dbl_onenorms_host.cpp:9-17): NaN when v[0] is NaN, later NaNs
 // skipped.  Uniform over the wavefront; every lane must call it.
-template <int K> __device__ double maxnorm(const Arr<K> &v, int off, int n) {
+template <int K, bool CX = false> __device__ double maxnorm(const Arr<K, CX> &v, int off, int n) {
   const int l = lane_id();
   double r = 0.0, v0 = 0.0;
   for (int i = l; i < n; i += 64) {
-    const double a = fabs(R<K>::hi(v.ld(off + i)));
+    const double a = NT<K, CX>::mag(v.ld(off + i));
     if (!(a <= r)) r = (a != a) ? r : a;
   }
-  if (l == 0) v0 = fabs(R<K>::hi(v.ld(off)));
+  if (l == 0) v0 = NT<K, CX>::mag(v.ld(off));
   r = wave_fmax(r);
   v0 = __shfl(v0, 0, 64);
   return (v0 != v0) ? v0 : r;
@@ -96,9 +151,10 @@ template <int K> __device__ double maxnorm(const Arr<K> &v, int off, int n) {
 
 // ------------------------------------------------ Householder QR (square)
 
-// per-wave work arrays of the linear algebra
-template <int K> struct LinWork {
-  Arr<K> q, r, v, w1, w2, wrk, sc;  // sc[0] = beta, sc[1] = v0, sc[2] = 1 when sigma = 0
+// per-wave work arrays of the linear algebra; sc[0] = beta, sc[1] = v0
+// (complex: 1/v0), sc[2] = 1 when sigma = 0
+template <int K, bool CX = false> struct LinWork {
+  Arr<K, CX> q, r, v, w1, w2, wrk, sc;
 };
 
 // Q R = A for the n x n matrix given by the accessor a(r, c)
@@ -210,6 +266,216 @@ __device__ void qrbs(int n, const LinWork<K> &w, const Arr<K> &bv, int bo, const
   }
 }
 
+// Complex Q R = A (CPU_cmplx2_factors_houseqr with CPU_cmplx2_factors_house,
+// leftRupdate and rightQupdate, Matrices/dbl2_factorizations.cpp:441-533,
+// 567-627, 661-722, 766-822): lane 0 forms the Householder vector's scalars
+// (sigma = sum |x_i|^2, mu = sqrt(|x0|^2 + sigma) / |x0|, v0 = x0 - mu x0,
+// beta = 2 |v0|^2 / (sigma + |v0|^2), 1/v0 = conj(v0) / |v0|^2), the lanes
+// v_i = x_i / v0, then w1 = beta R^H v, w2 = beta Q v and the rank-one
+// updates R -= v w1^H, Q -= w2 v^H, every entry in the reference's order.
+template <int K, class A> __device__ void cx_houseqr(int n, const A &a, const LinWork<K, true> &w) {
+  typedef typename RealT<K>::type T;
+  typedef cplx<K> Z;
+  const int l = lane_id();
+  for (int t = l; t < n * n; t += 64) {
+    const int i = t / n, j = t % n;
+    w.q.st(t, C<K>::from(i == j ? 1.0 : 0.0, 0.0));
+    w.r.st(t, a(i, j));
+  }
+  wsync();
+  for (int k = 0; k < n; ++k) {
+    const int m = n - k;
+    if (l == 0) {
+      T sigma = R<K>::zero();
+      for (int i = 1; i < m; ++i) {
+        const Z x = w.r.ld((k + i) * n + k);
+        sigma = add(sigma, sqr(x.re));
+        sigma = add(sigma, sqr(x.im));
+      }
+      const bool zero = all_zero(sigma);
+      Z b;
+      b.re = R<K>::zero();
+      b.im = R<K>::from(zero ? 1.0 : 0.0);
+      w.sc.st(2, b);
+      if (zero) {
+        w.sc.st(0, C<K>::zero());
+        w.sc.st(1, C<K>::from(1.0, 0.0));
+      } else {
+        const Z x0 = w.r.ld(k * n + k);
+        T sqrx0 = add(sqr(x0.re), sqr(x0.im));
+        const T x0rad = R<K>::sqrt(sqrx0);
+        sqrx0 = add(sqrx0, sigma);
+        T mu = R<K>::sqrt(sqrx0);
+        Z v0;
+        if (all_zero(x0rad)) {
+          v0.re = neg(mu);
+          v0.im = R<K>::zero();
+        } else {
+          mu = div(mu, x0rad);
+          v0.re = sub(x0.re, mul(mu, x0.re));
+          v0.im = sub(x0.im, mul(mu, x0.im));
+        }
+        const T sqrv0 = add(sqr(v0.re), sqr(v0.im));
+        sigma = add(sigma, sqrv0);
+        T beta = div(sqrv0, sigma);
+        beta = mul_d(beta, 2.0);
+        Z inv;
+        inv.re = div(v0.re, sqrv0);
+        inv.im = neg(div(v0.im, sqrv0));
+        Z bz;
+        bz.re = beta;
+        bz.im = R<K>::zero();
+        w.sc.st(0, bz);
+        w.sc.st(1, inv);
+      }
+    }
+    wsync();
+    {
+      const Z inv = w.sc.ld(1);
+      const bool szero = R<K>::hi(w.sc.ld(2).im) != 0.0;
+      for (int i = l; i < m; i += 64) {
+        if (i == 0) {
+          w.v.st(0, C<K>::from(1.0, 0.0));
+        } else {
+          const Z x = w.r.ld((k + i) * n + k);
+          if (szero) {
+            w.v.st(i, x);
+          } else {
+            Z v;
+            v.re = mul(x.re, inv.re);
+            v.re = sub(v.re, mul(x.im, inv.im));
+            v.im = mul(x.im, inv.re);
+            v.im = add(v.im, mul(x.re, inv.im));
+            w.v.st(i, v);
+          }
+        }
+      }
+    }
+    wsync();
+    const T beta = w.sc.ld(0).re;
+    // w1[j-k] = beta * sum_i conj(R[i][j]) v[i-k]
+    for (int j = k + l; j < n; j += 64) {
+      Z s = C<K>::zero();
+      for (int i = k; i < n; ++i) {
+        const Z r = w.r.ld(i * n + j), v = w.v.ld(i - k);
+        T zre = mul(r.re, v.re);
+        zre = add(zre, mul(r.im, v.im));
+        T zim = mul(r.re, v.im);
+        zim = sub(zim, mul(r.im, v.re));
+        s.re = add(s.re, zre);
+        s.im = add(s.im, zim);
+      }
+      s.re = mul(beta, s.re);
+      s.im = mul(beta, s.im);
+      w.w1.st(j - k, s);
+    }
+    // w2[i] = beta * sum_{j>=k} Q[i][j] v[j-k]
+    for (int i = l; i < n; i += 64) {
+      Z s = C<K>::zero();
+      for (int j = k; j < n; ++j) {
+        const Z q = w.q.ld(i * n + j), v = w.v.ld(j - k);
+        T zre = mul(q.re, v.re);
+        zre = sub(zre, mul(q.im, v.im));
+        T zim = mul(q.im, v.re);
+        zim = add(zim, mul(q.re, v.im));
+        s.re = add(s.re, zre);
+        s.im = add(s.im, zim);
+      }
+      s.re = mul(beta, s.re);
+      s.im = mul(beta, s.im);
+      w.w2.st(i, s);
+    }
+    wsync();
+    // R[i][j] -= v[i-k] conj(w1[j-k]);  Q[i][j] -= w2[i] conj(v[j-k])
+    for (int t = l; t < m * m; t += 64) {
+      const int i = k + t / m, j = k + t % m;
+      const Z v = w.v.ld(i - k), x = w.w1.ld(j - k);
+      Z r = w.r.ld(i * n + j);
+      T zre = mul(v.re, x.re);
+      zre = add(zre, mul(v.im, x.im));
+      T zim = mul(v.im, x.re);
+      zim = sub(zim, mul(v.re, x.im));
+      r.re = sub(r.re, zre);
+      r.im = sub(r.im, zim);
+      w.r.st(i * n + j, r);
+    }
+    for (int t = l; t < n * m; t += 64) {
+      const int i = t / m, j = k + t % m;
+      const Z x = w.w2.ld(i), v = w.v.ld(j - k);
+      Z q = w.q.ld(i * n + j);
+      T zre = mul(x.re, v.re);
+      zre = add(zre, mul(x.im, v.im));
+      T zim = mul(x.im, v.re);
+      zim = sub(zim, mul(x.re, v.im));
+      q.re = sub(q.re, zre);
+      q.im = sub(q.im, zim);
+      w.q.st(i * n + j, q);
+    }
+    wsync();
+  }
+}
+
+// x = R^{-1} Q^H b (CPU_cmplx2_factors_qrbs and _backward, Matrices/
+// dbl2_factorizations.cpp:842-879, 142-200): b = bv[bo..], x = xv[xo..]
+template <int K>
+__device__ void cx_qrbs(int n, const LinWork<K, true> &w, const Arr<K, true> &bv, int bo,
+                        const Arr<K, true> &xv, int xo) {
+  typedef typename RealT<K>::type T;
+  typedef cplx<K> Z;
+  const int l = lane_id();
+  for (int i = l; i < n; i += 64) {
+    Z s = C<K>::zero();
+    for (int j = 0; j < n; ++j) {
+      const Z q = w.q.ld(j * n + i), b = bv.ld(bo + j);
+      s.re = add(s.re, mul(q.re, b.re));
+      s.re = add(s.re, mul(q.im, b.im));
+      s.im = add(s.im, mul(q.re, b.im));
+      s.im = sub(s.im, mul(q.im, b.re));
+    }
+    w.wrk.st(i, s);
+  }
+  wsync();
+  for (int i = l; i < n; i += 64) xv.st(xo + i, w.wrk.ld(i));
+  wsync();
+  for (int j = n - 1; j >= 0; --j) {
+    if (l == (j & 63)) {
+      const Z u = w.r.ld(j * n + j), x = xv.ld(xo + j);
+      const T den = add(mul(u.re, u.re), mul(u.im, u.im));
+      const T a1re = div(u.re, den), a1im = neg(div(u.im, den));
+      Z y;
+      y.re = sub(mul(x.re, a1re), mul(x.im, a1im));
+      y.im = add(mul(x.im, a1re), mul(x.re, a1im));
+      xv.st(xo + j, y);
+    }
+    wsync();
+    const Z xj = xv.ld(xo + j);
+    for (int i = l; i < j; i += 64) {
+      const Z u = w.r.ld(i * n + j);
+      Z y = xv.ld(xo + i);
+      const T a1re = mul(u.re, xj.re), a1im = mul(u.im, xj.im);
+      const T a2re = mul(u.im, xj.re), a2im = mul(u.re, xj.im);
+      y.re = sub(y.re, a1re);
+      y.re = add(y.re, a1im);
+      y.im = sub(y.im, a2re);
+      y.im = sub(y.im, a2im);
+      xv.st(xo + i, y);
+    }
+    wsync();
+  }
+}
+
+// the real or complex factorization and solve
+template <int K, bool CX, class A> __device__ void any_houseqr(int n, const A &a, const LinWork<K, CX> &w) {
+  if constexpr (CX) cx_houseqr<K>(n, a, w);
+  else houseqr<K>(n, a, w);
+}
+template <int K, bool CX>
+__device__ void any_qrbs(int n, const LinWork<K, CX> &w, const Arr<K, CX> &bv, int bo,
+                         const Arr<K, CX> &xv, int xo) {
+  if constexpr (CX) cx_qrbs<K>(n, w, bv, bo, xv, xo);
+  else qrbs<K>(n, w, bv, bo, xv, xo);
+}
+
 struct Flags {
   int zeroQ, noqr, up, bs, newtail;
 };
@@ -217,20 +483,20 @@ struct Flags {
 // lower block-triangular Toeplitz solve sum_{j<=k} A_j x_{k-j} = b_k
 // (CPU_dbl2_qrbs_solve with _head and _tail); mat(j, r, c) = A_j[r][c];
 // rhs and sol rows k at offsets k*n.  Uniform control flow.
-template <int K, class M>
-__device__ void qrbs_solve(int n, int degp1, int tailidx, const M &mat, const Arr<K> &rhs,
-                           const Arr<K> &sol, const LinWork<K> &w, Flags &f) {
-  typedef typename RealT<K>::type T;
+template <int K, bool CX, class M>
+__device__ void qrbs_solve(int n, int degp1, int tailidx, const M &mat, const Arr<K, CX> &rhs,
+                           const Arr<K, CX> &sol, const LinWork<K, CX> &w, Flags &f) {
+  typedef typename NT<K, CX>::V T;
   const int l = lane_id();
   const double thr = threshold<K>();
   if (!(f.noqr && !f.zeroQ)) {
-    const double nrm = maxnorm<K>(rhs, 0, n);
+    const double nrm = maxnorm<K, CX>(rhs, 0, n);
     const bool small = (K == 1) ? (nrm + 1.0 == 1.0) : (nrm < thr);
     if (small) f.noqr = f.zeroQ ? 0 : 1;
     if (!f.noqr) {
-      houseqr<K>(n, [&](int r, int c) { return mat(0, r, c); }, w);
+      any_houseqr<K, CX>(n, [&](int r, int c) { return mat(0, r, c); }, w);
       f.zeroQ = 0;
-      qrbs<K>(n, w, rhs, 0, sol, 0);
+      any_qrbs<K, CX>(n, w, rhs, 0, sol, 0);
     }
   }
   if (degp1 > 1) {
@@ -238,29 +504,29 @@ __device__ void qrbs_solve(int n, int degp1, int tailidx, const M &mat, const Ar
     bool firstbs = true;
     f.newtail = degp1;
     for (int i = tailidx; i < degp1; ++i) {
-      if (maxnorm<K>(sol, (i - 1) * n, n) < thr) {
+      if (maxnorm<K, CX>(sol, (i - 1) * n, n) < thr) {
         ++skipup;
       } else {
         const int npair = (degp1 - i) * n;
         for (int t = l; t < npair; t += 64) {
           const int j = i + t / n, k = t % n;
           T b = rhs.ld(j * n + k);
-          for (int L = 0; L < n; ++L) b = sub(b, mul(mat(j - i + 1, k, L), sol.ld((i - 1) * n + L)));
+          for (int L = 0; L < n; ++L) b = NT<K, CX>::msub(b, mat(j - i + 1, k, L), sol.ld((i - 1) * n + L));
           rhs.st(j * n + k, b);
         }
       }
       wsync();
-      const double nrm = maxnorm<K>(rhs, i * n, n);
+      const double nrm = maxnorm<K, CX>(rhs, i * n, n);
       if (nrm < thr || nrm > 1.0e+99) {
         ++skipbs;
-        for (int j = l; j < n; j += 64) sol.st(i * n + j, R<K>::zero());
+        for (int j = l; j < n; j += 64) sol.st(i * n + j, NT<K, CX>::zero());
         wsync();
       } else {
         if (firstbs) {
           f.newtail = i;
           firstbs = false;
         }
-        qrbs<K>(n, w, rhs, i * n, sol, i * n);
+        any_qrbs<K, CX>(n, w, rhs, i * n, sol, i * n);
       }
     }
     f.up = skipup;
@@ -287,10 +553,10 @@ template <int K> __device__ __forceinline__ mdr<K> times_exp(const mdr<K> &v, do
 
 // coefficient k of x*y: x_0 y_k, then += x_i y_{k-i} (CPU_dbl2_product,
 // Convolutions/dbl2_convolutions_host.cpp:7-26)
-template <int K, class FX, class FY>
-__device__ __forceinline__ typename RealT<K>::type cprod(const FX &x, const FY &y, int k) {
-  typename RealT<K>::type acc = mul(x(0), y(k));
-  for (int i = 1; i <= k; ++i) acc = add(acc, mul(x(i), y(k - i)));
+template <int K, bool CX, class FX, class FY>
+__device__ __forceinline__ typename NT<K, CX>::V cprod(const FX &x, const FY &y, int k) {
+  typename NT<K, CX>::V acc = NT<K, CX>::mul(x(0), y(k));
+  for (int i = 1; i <= k; ++i) acc = NT<K, CX>::mac(acc, x(i), y(k - i));
   return acc;
 }
 
@@ -305,12 +571,14 @@ __device__ __forceinline__ typename RealT<K>::type cprod(const FX &x, const FY &
 // factor at a time in the order of idx (the first loop of
 // CPU_dbl2_evaluate_monomials, Newton/dbl2_systems_host.cpp:18-45), and each
 // derivative contribution is multiplied by its exponent before it is added.
-template <int K>
+// (complex: the products of CPU_cmplx2_poly_evaldiff; no exponents)
+template <int K, bool CX>
 __device__ void eval_system(const RowSys &S, int D, int Dw, int Dp, const double *cst,
-                            size_t cstride, const double *cff, size_t fstride, const Arr<K> &X,
-                            const Arr<K> &OUT, const Arr<K> &FW, const Arr<K> &BW,
-                            const Arr<K> &CR, const Arr<K> &CF) {
-  typedef typename RealT<K>::type T;
+                            size_t cstride, const double *cff, size_t fstride,
+                            const Arr<K, CX> &X, const Arr<K, CX> &OUT, const Arr<K, CX> &FW,
+                            const Arr<K, CX> &BW, const Arr<K, CX> &CR, const Arr<K, CX> &CF) {
+  typedef NT<K, CX> N;
+  typedef typename N::V T;
   const int l = lane_id(), nslot = 64 / Dp, s = l / Dp, k = l % Dp;
   const int dim = S.dim, D1 = dim + 1;
   const int eo = s * S.nvmax * D;  // this slot's work rows
@@ -323,8 +591,8 @@ __device__ void eval_system(const RowSys &S, int D, int Dw, int Dp, const double
     const int p = r0 + s;
     const bool on = (p < dim) && (k < Dw);
     if (on) {
-      OUT.st((p * D1 + dim) * D + k, R<K>::load(cst, cstride, (size_t)p * D + k));
-      for (int j = 0; j < dim; ++j) OUT.st((p * D1 + j) * D + k, R<K>::zero());
+      OUT.st((p * D1 + dim) * D + k, N::ld(cst, cstride, (size_t)p * D + k));
+      for (int j = 0; j < dim; ++j) OUT.st((p * D1 + j) * D + k, N::zero());
     }
     const int nb = (p < dim) ? S.monoptr[p + 1] - S.monoptr[p] : 0;
     for (int mi = 0; mi < S.nbrmax; ++mi) {
@@ -332,57 +600,61 @@ __device__ void eval_system(const RowSys &S, int D, int Dw, int Dp, const double
       const int mono = act ? S.monoptr[p] + mi : 0;
       const int nv = act ? S.nvr[mono] : 0;
       const int *ix = S.idx + (act ? S.idxptr[mono] : 0);
-      const int *ex = (act && S.exp) ? S.exp + S.idxptr[mono] : nullptr;
+      const int *ex = (!CX && act && S.exp) ? S.exp + S.idxptr[mono] : nullptr;
       // v times the exponent of variable position j (positions with
       // exponent 1 are added as they are)
       auto addto = [&](int row, const T &v, int j) {
         const int o = (p * D1 + row) * D + k;
-        const int e = (ex && j >= 0) ? ex[j] : 1;
-        OUT.st(o, add(OUT.ld(o), e > 1 ? times_exp(v, (double)e) : v));
+        if constexpr (CX) {
+          OUT.st(o, N::add(OUT.ld(o), v));
+        } else {
+          const int e = (ex && j >= 0) ? ex[j] : 1;
+          OUT.st(o, add(OUT.ld(o), e > 1 ? times_exp(v, (double)e) : v));
+        }
       };
-      if (act) CF.st(co + k, R<K>::load(cff, fstride, (size_t)mono * D + k));
+      if (act) CF.st(co + k, N::ld(cff, fstride, (size_t)mono * D + k));
       wsync();
       if (ex) {
         for (int j = 0; j < nv; ++j)
           for (int e = 1; e < ex[j]; ++e) {
-            const T a = cprod<K>(xin(ix[j]), cf, k);
+            const T a = cprod<K, CX>(xin(ix[j]), cf, k);
             wsync();
             CF.st(co + k, a);
             wsync();
           }
       }
       if (nv == 1) {
-        const T f = cprod<K>(xin(ix[0]), cf, k);
+        const T f = cprod<K, CX>(xin(ix[0]), cf, k);
         addto(dim, f, -1);
         addto(ix[0], cf(k), 0);
       } else if (nv == 2) {
-        const T f0 = cprod<K>(cf, xin(ix[0]), k);
+        const T f0 = cprod<K, CX>(cf, xin(ix[0]), k);
         FW.st(eo + k, f0);
         addto(ix[1], f0, 1);
-        const T b0 = cprod<K>(cf, xin(ix[1]), k);
+        const T b0 = cprod<K, CX>(cf, xin(ix[1]), k);
         addto(ix[0], b0, 0);
         wsync();
-        const T f1 = cprod<K>(fw(0), xin(ix[1]), k);
+        const T f1 = cprod<K, CX>(fw(0), xin(ix[1]), k);
         addto(dim, f1, -1);
       } else if (nv > 2) {
         // forward and backward chains advance together
-        FW.st(eo + k, cprod<K>(cf, xin(ix[0]), k));
-        BW.st(eo + k, cprod<K>(xin(ix[nv - 1]), xin(ix[nv - 2]), k));
+        FW.st(eo + k, cprod<K, CX>(cf, xin(ix[0]), k));
+        BW.st(eo + k, cprod<K, CX>(xin(ix[nv - 1]), xin(ix[nv - 2]), k));
         wsync();
         for (int j = 1; j < nv; ++j) {
-          FW.st(eo + j * D + k, cprod<K>(fw(j - 1), xin(ix[j]), k));
-          if (j < nv - 2) BW.st(eo + j * D + k, cprod<K>(bw(j - 1), xin(ix[nv - 2 - j]), k));
+          FW.st(eo + j * D + k, cprod<K, CX>(fw(j - 1), xin(ix[j]), k));
+          if (j < nv - 2) BW.st(eo + j * D + k, cprod<K, CX>(bw(j - 1), xin(ix[nv - 2 - j]), k));
           wsync();
         }
-        const T blast = cprod<K>(bw(nv - 3), cf, k);
+        const T blast = cprod<K, CX>(bw(nv - 3), cf, k);
         wsync();
         BW.st(eo + (nv - 3) * D + k, blast);
         wsync();
         if (nv == 3) {
-          CR.st(eo + k, cprod<K>(fw(0), xin(ix[2]), k));
+          CR.st(eo + k, cprod<K, CX>(fw(0), xin(ix[2]), k));
         } else {
-          for (int j = 0; j < nv - 3; ++j) CR.st(eo + j * D + k, cprod<K>(fw(j), bw(nv - 4 - j), k));
-          CR.st(eo + (nv - 3) * D + k, cprod<K>(fw(nv - 3), xin(ix[nv - 1]), k));
+          for (int j = 0; j < nv - 3; ++j) CR.st(eo + j * D + k, cprod<K, CX>(fw(j), bw(nv - 4 - j), k));
+          CR.st(eo + (nv - 3) * D + k, cprod<K, CX>(fw(nv - 3), xin(ix[nv - 1]), k));
         }
         addto(dim, FW.ld(eo + (nv - 1) * D + k), -1);
         addto(ix[nv - 1], FW.ld(eo + (nv - 2) * D + k), nv - 1);
@@ -399,8 +671,9 @@ __device__ void eval_system(const RowSys &S, int D, int Dw, int Dp, const double
 
 struct NewtonLayout {
   int X, OUT, RHS, SOL, Q, R, V, W1, W2, WRK, SC, FW, BW, CR, CF;  // sizes in numbers
-  __host__ __device__ size_t bytes(int K) const {
-    return sizeof(double) * (size_t)K *
+  // P doubles per number: K (real) or 2K (complex)
+  __host__ __device__ size_t bytes(int P) const {
+    return sizeof(double) * (size_t)P *
            ((size_t)X + OUT + RHS + SOL + Q + R + V + W1 + W2 + WRK + SC + FW + BW + CR + CF);
   }
 };
@@ -420,41 +693,42 @@ __host__ __device__ inline NewtonLayout newton_layout(const RowSys &S, int D) {
   return L;
 }
 
-template <int K>
-__device__ LinWork<K> carve_lin(double *&p, int n) {
-  LinWork<K> w;
-  w.q = carve<K>(p, n * n); w.r = carve<K>(p, n * n); w.v = carve<K>(p, n);
-  w.w1 = carve<K>(p, n); w.w2 = carve<K>(p, n); w.wrk = carve<K>(p, n); w.sc = carve<K>(p, 3);
+template <int K, bool CX = false>
+__device__ LinWork<K, CX> carve_lin(double *&p, int n) {
+  LinWork<K, CX> w;
+  w.q = carve<K, CX>(p, n * n); w.r = carve<K, CX>(p, n * n); w.v = carve<K, CX>(p, n);
+  w.w1 = carve<K, CX>(p, n); w.w2 = carve<K, CX>(p, n); w.wrk = carve<K, CX>(p, n);
+  w.sc = carve<K, CX>(p, 3);
   return w;
 }
 
 // x [K][nsys][dim][D] in/out; cst [K][nc][dim][D], cff [K][nf][nmono][D] with
 // nc, nf = nsys or 1 (shared); logs [nsys][nbsteps]; hist
 // [nbsteps][K][nsys][dim][D]; Q, R [K][nsys][dim][dim].
-template <int K>
+template <int K, bool CX>
 __global__ void __launch_bounds__(512)
 row_newton_kernel(RowSys S, int deg, int nbsteps, int nsys, const double *cst, int cshared,
                   const double *cff, int fshared, double *x, int *steps_out, int *wdeg_out,
                   int *tail_out, int *up_out, int *bs_out, double *hist, double *Qo, double *Ro) {
-  typedef typename RealT<K>::type T;
+  typedef NT<K, CX> N;
   extern __shared__ double lds[];
   const int D = deg + 1, n = S.dim, Dp = pow2_at_least(D);
   const NewtonLayout L = newton_layout(S, D);
   const int wave = threadIdx.x >> 6, l = lane_id();
   const int sysid = blockIdx.x * (blockDim.x >> 6) + wave;
   if (sysid >= nsys) return;
-  double *p = lds + (size_t)wave * (L.bytes(K) / sizeof(double));
-  const Arr<K> X = carve<K>(p, L.X), OUT = carve<K>(p, L.OUT), RHS = carve<K>(p, L.RHS),
-               SOL = carve<K>(p, L.SOL);
-  const LinWork<K> w = carve_lin<K>(p, n);
-  const Arr<K> FW = carve<K>(p, L.FW), BW = carve<K>(p, L.BW), CR = carve<K>(p, L.CR),
-               CF = carve<K>(p, L.CF);
+  double *p = lds + (size_t)wave * (L.bytes(N::P) / sizeof(double));
+  const Arr<K, CX> X = carve<K, CX>(p, L.X), OUT = carve<K, CX>(p, L.OUT),
+                   RHS = carve<K, CX>(p, L.RHS), SOL = carve<K, CX>(p, L.SOL);
+  const LinWork<K, CX> w = carve_lin<K, CX>(p, n);
+  const Arr<K, CX> FW = carve<K, CX>(p, L.FW), BW = carve<K, CX>(p, L.BW),
+                   CR = carve<K, CX>(p, L.CR), CF = carve<K, CX>(p, L.CF);
   const size_t xs = (size_t)nsys * n * D, xb = (size_t)sysid * n * D;
   const size_t cs = (size_t)(cshared ? 1 : nsys) * n * D;
   const size_t fs = (size_t)(fshared ? 1 : nsys) * S.nmono * D;
   const double *cbase = cst + (cshared ? 0 : (size_t)sysid * n * D);
   const double *fbase = cff + (fshared ? 0 : (size_t)sysid * S.nmono * D);
-  for (int t = l; t < n * D; t += 64) X.st(t, R<K>::load(x, xs, xb + t));
+  for (int t = l; t < n * D; t += 64) X.st(t, N::ld(x, xs, xb + t));
   wsync();
   const int D1 = n + 1;
   auto jac = [&](int j, int r, int c) { return OUT.ld((r * D1 + c) * D + j); };
@@ -462,19 +736,19 @@ row_newton_kernel(RowSys S, int deg, int nbsteps, int nsys, const double *cst, i
   int tailidx = 1, wrkdeg = 0, stepcnt = 0;
   for (int step = 0; step < nbsteps; ++step) {
     const int Dw = wrkdeg + 1;
-    eval_system<K>(S, D, Dw, Dp, cbase, cs, fbase, fs, X, OUT, FW, BW, CR, CF);
+    eval_system<K, CX>(S, D, Dw, Dp, cbase, cs, fbase, fs, X, OUT, FW, BW, CR, CF);
     for (int t = l; t < Dw * n; t += 64) {
       const int kk = t / n, i = t % n;
-      RHS.st(t, neg(OUT.ld((i * D1 + n) * D + kk)));
-      SOL.st(t, R<K>::zero());
+      RHS.st(t, N::neg(OUT.ld((i * D1 + n) * D + kk)));
+      SOL.st(t, N::zero());
     }
     wsync();
     f.newtail = tailidx;
-    qrbs_solve<K>(n, Dw, tailidx, jac, RHS, SOL, w, f);
+    qrbs_solve<K, CX>(n, Dw, tailidx, jac, RHS, SOL, w, f);
     tailidx = f.newtail;
     for (int t = l; t < n * Dw; t += 64) {
       const int i = t / Dw, kk = t % Dw;
-      if (kk >= tailidx - 1) X.st(i * D + kk, add(X.ld(i * D + kk), SOL.ld(kk * n + i)));
+      if (kk >= tailidx - 1) X.st(i * D + kk, N::add(X.ld(i * D + kk), SOL.ld(kk * n + i)));
     }
     wsync();
     if (l == 0) {
@@ -485,79 +759,81 @@ row_newton_kernel(RowSys S, int deg, int nbsteps, int nsys, const double *cst, i
       if (bs_out) bs_out[o] = f.bs;
     }
     if (hist) {
-      double *h = hist + (size_t)step * K * xs;
-      for (int t = l; t < n * D; t += 64) R<K>::store(h, xs, xb + t, X.ld(t));
+      double *h = hist + (size_t)step * N::P * xs;
+      for (int t = l; t < n * D; t += 64) N::st(h, xs, xb + t, X.ld(t));
     }
     ++stepcnt;
     if (tailidx >= deg) break;
     wrkdeg = wrkdeg + 1 + wrkdeg / 2;
     if (wrkdeg > deg) wrkdeg = deg;
   }
-  for (int t = l; t < n * D; t += 64) R<K>::store(x, xs, xb + t, X.ld(t));
+  for (int t = l; t < n * D; t += 64) N::st(x, xs, xb + t, X.ld(t));
   const size_t qs = (size_t)nsys * n * n, qb = (size_t)sysid * n * n;
   for (int t = l; t < n * n; t += 64) {
-    if (Qo) R<K>::store(Qo, qs, qb + t, w.q.ld(t));
-    if (Ro) R<K>::store(Ro, qs, qb + t, w.r.ld(t));
+    if (Qo) N::st(Qo, qs, qb + t, w.q.ld(t));
+    if (Ro) N::st(Ro, qs, qb + t, w.r.ld(t));
   }
   if (l == 0 && steps_out) steps_out[sysid] = stepcnt;
 }
 
 // batched Householder QR of n x n matrices A [K][nb][n][n] -> Q, R
-template <int K>
+template <int K, bool CX>
 __global__ void __launch_bounds__(512)
 houseqr_kernel(int n, int nb, const double *A, double *Qo, double *Ro) {
+  typedef NT<K, CX> N;
   extern __shared__ double lds[];
   const int wave = threadIdx.x >> 6, l = lane_id();
   const int b = blockIdx.x * (blockDim.x >> 6) + wave;
   if (b >= nb) return;
-  const size_t per = (size_t)K * (2 * n * n + 4 * n + 3);
+  const size_t per = (size_t)N::P * (2 * n * n + 4 * n + 3);
   double *p = lds + (size_t)wave * per;
-  const LinWork<K> w = carve_lin<K>(p, n);
+  const LinWork<K, CX> w = carve_lin<K, CX>(p, n);
   const size_t s = (size_t)nb * n * n, o = (size_t)b * n * n;
-  houseqr<K>(n, [&](int r, int c) { return R<K>::load(A, s, o + (size_t)r * n + c); }, w);
+  any_houseqr<K, CX>(n, [&](int r, int c) { return N::ld(A, s, o + (size_t)r * n + c); }, w);
   for (int t = l; t < n * n; t += 64) {
-    R<K>::store(Qo, s, o + t, w.q.ld(t));
-    R<K>::store(Ro, s, o + t, w.r.ld(t));
+    N::st(Qo, s, o + t, w.q.ld(t));
+    N::st(Ro, s, o + t, w.r.ld(t));
   }
 }
 
 // batched Toeplitz solve: mat [K][nb][degp1][n][n], rhs/sol [K][nb][degp1][n]
 // (in/out), Q/R [K][nb][n][n] (in/out), flags [nb][5] (zeroQ, noqr, up, bs,
 // newtail; in/out)
-template <int K>
+template <int K, bool CX>
 __global__ void __launch_bounds__(512)
 qrbs_solve_kernel(int n, int degp1, int tailidx, int nb, const double *mat, double *rhs,
                   double *sol, double *Qio, double *Rio, int *flags) {
+  typedef NT<K, CX> N;
   extern __shared__ double lds[];
   const int wave = threadIdx.x >> 6, l = lane_id();
   const int b = blockIdx.x * (blockDim.x >> 6) + wave;
   if (b >= nb) return;
-  const size_t per = (size_t)K * (2 * n * n + 4 * n + 3 + 2 * degp1 * n);
+  const size_t per = (size_t)N::P * (2 * n * n + 4 * n + 3 + 2 * degp1 * n);
   double *p = lds + (size_t)wave * per;
-  const LinWork<K> w = carve_lin<K>(p, n);
-  const Arr<K> RHS = carve<K>(p, degp1 * n), SOL = carve<K>(p, degp1 * n);
+  const LinWork<K, CX> w = carve_lin<K, CX>(p, n);
+  const Arr<K, CX> RHS = carve<K, CX>(p, degp1 * n), SOL = carve<K, CX>(p, degp1 * n);
   const size_t vs = (size_t)nb * degp1 * n, vo = (size_t)b * degp1 * n;
   const size_t qs = (size_t)nb * n * n, qo = (size_t)b * n * n;
   const size_t ms = (size_t)nb * degp1 * n * n, mo = (size_t)b * degp1 * n * n;
   for (int t = l; t < degp1 * n; t += 64) {
-    RHS.st(t, R<K>::load(rhs, vs, vo + t));
-    SOL.st(t, R<K>::load(sol, vs, vo + t));
+    RHS.st(t, N::ld(rhs, vs, vo + t));
+    SOL.st(t, N::ld(sol, vs, vo + t));
   }
   for (int t = l; t < n * n; t += 64) {
-    w.q.st(t, R<K>::load(Qio, qs, qo + t));
-    w.r.st(t, R<K>::load(Rio, qs, qo + t));
+    w.q.st(t, N::ld(Qio, qs, qo + t));
+    w.r.st(t, N::ld(Rio, qs, qo + t));
   }
   wsync();
   Flags f{flags[b * 5], flags[b * 5 + 1], flags[b * 5 + 2], flags[b * 5 + 3], flags[b * 5 + 4]};
-  auto m = [&](int j, int r, int c) { return R<K>::load(mat, ms, mo + ((size_t)j * n + r) * n + c); };
-  qrbs_solve<K>(n, degp1, tailidx, m, RHS, SOL, w, f);
+  auto m = [&](int j, int r, int c) { return N::ld(mat, ms, mo + ((size_t)j * n + r) * n + c); };
+  qrbs_solve<K, CX>(n, degp1, tailidx, m, RHS, SOL, w, f);
   for (int t = l; t < degp1 * n; t += 64) {
-    R<K>::store(rhs, vs, vo + t, RHS.ld(t));
-    R<K>::store(sol, vs, vo + t, SOL.ld(t));
+    N::st(rhs, vs, vo + t, RHS.ld(t));
+    N::st(sol, vs, vo + t, SOL.ld(t));
   }
   for (int t = l; t < n * n; t += 64) {
-    R<K>::store(Qio, qs, qo + t, w.q.ld(t));
-    R<K>::store(Rio, qs, qo + t, w.r.ld(t));
+    N::st(Qio, qs, qo + t, w.q.ld(t));
+    N::st(Rio, qs, qo + t, w.r.ld(t));
   }
   if (l == 0) {
     flags[b * 5] = f.zeroQ; flags[b * 5 + 1] = f.noqr; flags[b * 5 + 2] = f.up;
@@ -569,11 +845,12 @@ qrbs_solve_kernel(int n, int degp1, int tailidx, int nb, const double *mat, doub
 // (CPU_dbl2_linear_residue, Newton/dbl2_bals_host.cpp:548-600); one lane per
 // residual entry; resmax [K][nb] = entry of largest |leading limb|, the first
 // such entry in the reference's scan order.
-template <int K>
+template <int K, bool CX>
 __global__ void __launch_bounds__(512)
 linear_residue_kernel(int n, int degp1, int tailidx, int nb, const double *mat,
                       const double *rhs, const double *sol, double *res, double *resmax) {
-  typedef typename RealT<K>::type T;
+  typedef NT<K, CX> N;
+  typedef typename N::V T;
   const int wave = threadIdx.x >> 6, l = lane_id();
   const int b = blockIdx.x * (blockDim.x >> 6) + wave;
   if (b >= nb) return;
@@ -583,13 +860,13 @@ linear_residue_kernel(int n, int degp1, int tailidx, int nb, const double *mat,
   int bidx = -1;
   for (int t = tailidx * n + l; t < degp1 * n; t += 64) {
     const int i = t / n, L = t % n;
-    T r = R<K>::load(rhs, vs, vo + t);
+    T r = N::ld(rhs, vs, vo + t);
     for (int j = 0; j <= i - tailidx; ++j)
       for (int k = 0; k < n; ++k)
-        r = sub(r, mul(R<K>::load(mat, ms, mo + ((size_t)j * n + L) * n + k),
-                       R<K>::load(sol, vs, vo + (size_t)(i - j) * n + k)));
-    R<K>::store(res, vs, vo + t, r);
-    const double a = fabs(R<K>::hi(r));
+        r = N::msub(r, N::ld(mat, ms, mo + ((size_t)j * n + L) * n + k),
+                    N::ld(sol, vs, vo + (size_t)(i - j) * n + k));
+    N::st(res, vs, vo + t, r);
+    const double a = N::mag(r);
     if (a > best) { best = a; bidx = t; }
   }
   // the scan keeps the first strictly larger entry: largest value, then the
@@ -600,10 +877,14 @@ linear_residue_kernel(int n, int degp1, int tailidx, int nb, const double *mat,
     if (ob > best || (ob == best && oi >= 0 && (bidx < 0 || oi < bidx))) { best = ob; bidx = oi; }
   }
   if (l == 0) {
-    T r = R<K>::zero();
-    if (bidx >= 0) r = R<K>::load(res, vs, vo + bidx);
+    T r = N::zero();
+    if (bidx >= 0) r = N::ld(res, vs, vo + bidx);
 #pragma unroll
-    for (int i = 0; i < K; ++i) resmax[(size_t)i * nb + b] = fabs(limb(r, i));
+    for (int i = 0; i < K; ++i) {
+      // complex: |re| + |im| limb by limb, as CPU_cmplx2_linear_residue
+      if constexpr (CX) resmax[(size_t)i * nb + b] = fabs(limb(r.re, i)) + fabs(limb(r.im, i));
+      else resmax[(size_t)i * nb + b] = fabs(limb(r, i));
+    }
   }
 }
 
@@ -702,17 +983,28 @@ int phc_rowsys_lds_bytes(const phc_rowsys *s, int K, int deg, size_t *bytes) {
   return 0;
 }
 
-int phc_row_newton(phc_rowsys *s, int K, int deg, int nbsteps, int nsys, const double *cst,
-                   int cst_shared, const double *cff, int cff_shared, double *x, int *steps,
-                   int *wdeg, int *tail, int *up, int *bs, double *history, double *Q, double *R,
-                   void *stream) {
-  PHC_REQUIRE(s, "phc_row_newton: null system");
-  PHC_REQUIRE(K == 1 || K == 2 || K == 4, "phc_row_newton: K must be 1, 2 or 4");
-  PHC_REQUIRE(deg >= 0 && deg <= 63, "phc_row_newton: deg must be in 0..63");
-  PHC_REQUIRE(nbsteps >= 0 && nsys >= 0, "phc_row_newton: negative size");
-  PHC_REQUIRE(cst && cff && x, "phc_row_newton: null coefficient or series array");
+}  // extern "C"
+
+namespace {
+
+// the staggered row Newton, real (P = K planes) or complex (2K)
+template <bool CX>
+int row_newton(phc_rowsys *s, int K, int deg, int nbsteps, int nsys, const double *cst,
+               int cst_shared, const double *cff, int cff_shared, double *x, int *steps,
+               int *wdeg, int *tail, int *up, int *bs, double *history, double *Q, double *R,
+               void *stream, const char *fn) {
+  PHC_REQUIRE(s, std::string(fn) + ": null system");
+  if (CX) {
+    PHC_REQUIRE(K == 2 || K == 4, std::string(fn) + ": K must be 2 or 4");
+    PHC_REQUIRE(s->S.exp == nullptr, std::string(fn) + ": exponents above 1 are real only");
+  } else {
+    PHC_REQUIRE(K == 1 || K == 2 || K == 4, std::string(fn) + ": K must be 1, 2 or 4");
+  }
+  PHC_REQUIRE(deg >= 0 && deg <= 63, std::string(fn) + ": deg must be in 0..63");
+  PHC_REQUIRE(nbsteps >= 0 && nsys >= 0, std::string(fn) + ": negative size");
+  PHC_REQUIRE(cst && cff && x, std::string(fn) + ": null coefficient or series array");
   if (nsys == 0 || nbsteps == 0) return 0;
-  const size_t per = newton_layout(s->S, deg + 1).bytes(K);
+  const size_t per = newton_layout(s->S, deg + 1).bytes(CX ? 2 * K : K);
   int wv = 0;
   if (waves_for(per, &wv) != 0) return -2;
   const int grid = (nsys + wv - 1) / wv;
@@ -720,13 +1012,13 @@ int phc_row_newton(phc_rowsys *s, int K, int deg, int nbsteps, int nsys, const d
   hipStream_t st = (hipStream_t)stream;
 #define LAUNCH(KK)                                                                            \
   do {                                                                                        \
-    PHC_HIP(hipFuncSetAttribute((const void *)row_newton_kernel<KK>,                         \
+    PHC_HIP(hipFuncSetAttribute((const void *)row_newton_kernel<KK, CX>,                     \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));       \
-    hipLaunchKernelGGL(row_newton_kernel<KK>, dim3(grid), dim3(64 * wv), lds, st, s->S, deg,  \
-                       nbsteps, nsys, cst, cst_shared, cff, cff_shared, x, steps, wdeg, tail, \
-                       up, bs, history, Q, R);                                                \
+    hipLaunchKernelGGL((row_newton_kernel<KK, CX>), dim3(grid), dim3(64 * wv), lds, st, s->S, \
+                       deg, nbsteps, nsys, cst, cst_shared, cff, cff_shared, x, steps, wdeg,  \
+                       tail, up, bs, history, Q, R);                                          \
   } while (0)
-  if (K == 1) LAUNCH(1);
+  if (K == 1 && !CX) LAUNCH(1);
   else if (K == 2) LAUNCH(2);
   else LAUNCH(4);
 #undef LAUNCH
@@ -734,82 +1026,154 @@ int phc_row_newton(phc_rowsys *s, int K, int deg, int nbsteps, int nsys, const d
   return 0;
 }
 
-int phc_houseqr_batch(int K, int n, int nbatch, const double *A, double *Q, double *R,
-                      void *stream) {
-  PHC_REQUIRE(K == 1 || K == 2 || K == 4, "phc_houseqr_batch: K must be 1, 2 or 4");
-  PHC_REQUIRE(n >= 1 && n <= 64 && nbatch >= 0, "phc_houseqr_batch: n must be in 1..64");
-  PHC_REQUIRE(A && Q && R, "phc_houseqr_batch: null array");
+template <bool CX>
+int houseqr_batch(int K, int n, int nbatch, const double *A, double *Q, double *R, void *stream,
+                  const char *fn) {
+  // complex octo-double (K = 8) is refused: its first device run stalled
+  // (a launch of houseqr_kernel<8, true> that did not return), cause not
+  // established
+  PHC_REQUIRE(CX ? (K == 2 || K == 4) : (K == 1 || K == 2 || K == 4),
+              std::string(fn) + (CX ? ": K must be 2 or 4" : ": K must be 1, 2 or 4"));
+  PHC_REQUIRE(n >= 1 && n <= 64 && nbatch >= 0, std::string(fn) + ": n must be in 1..64");
+  PHC_REQUIRE(A && Q && R, std::string(fn) + ": null array");
   if (nbatch == 0) return 0;
-  const size_t per = sizeof(double) * K * (2 * (size_t)n * n + 4 * n + 3);
+  const size_t per = sizeof(double) * (CX ? 2 * K : K) * (2 * (size_t)n * n + 4 * n + 3);
   int wv = 0;
   if (waves_for(per, &wv) != 0) return -2;
   const int grid = (nbatch + wv - 1) / wv;
   hipStream_t st = (hipStream_t)stream;
 #define LAUNCH(KK)                                                                            \
   do {                                                                                        \
-    PHC_HIP(hipFuncSetAttribute((const void *)houseqr_kernel<KK>,                            \
+    PHC_HIP(hipFuncSetAttribute((const void *)houseqr_kernel<KK, CX>,                        \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)(per * wv))); \
-    hipLaunchKernelGGL(houseqr_kernel<KK>, dim3(grid), dim3(64 * wv), per * wv, st, n, nbatch, \
-                       A, Q, R);                                                              \
+    hipLaunchKernelGGL((houseqr_kernel<KK, CX>), dim3(grid), dim3(64 * wv), per * wv, st, n,  \
+                       nbatch, A, Q, R);                                                      \
   } while (0)
-  if (K == 1) LAUNCH(1);
+  if (K == 1 && !CX) LAUNCH(1);
   else if (K == 2) LAUNCH(2);
   else LAUNCH(4);
 #undef LAUNCH
   PHC_HIP(hipGetLastError());
   return 0;
+}
+
+template <bool CX>
+int qrbs_solve_batch(int K, int n, int degp1, int tailidx, int nbatch, const double *mat,
+                     double *rhs, double *sol, double *Q, double *R, int *flags, void *stream,
+                     const char *fn) {
+  // complex octo-double (K = 8) is refused: its first device run stalled
+  // (a launch of houseqr_kernel<8, true> that did not return), cause not
+  // established
+  PHC_REQUIRE(CX ? (K == 2 || K == 4) : (K == 1 || K == 2 || K == 4),
+              std::string(fn) + (CX ? ": K must be 2 or 4" : ": K must be 1, 2 or 4"));
+  PHC_REQUIRE(n >= 1 && n <= 64 && degp1 >= 1 && degp1 <= 64 && nbatch >= 0,
+              std::string(fn) + ": n and degp1 must be in 1..64");
+  PHC_REQUIRE(tailidx >= 1, std::string(fn) + ": tailidx must be at least 1");
+  PHC_REQUIRE(mat && rhs && sol && Q && R && flags, std::string(fn) + ": null array");
+  if (nbatch == 0) return 0;
+  const size_t per =
+      sizeof(double) * (CX ? 2 * K : K) * (2 * (size_t)n * n + 4 * n + 3 + 2 * (size_t)degp1 * n);
+  int wv = 0;
+  if (waves_for(per, &wv) != 0) return -2;
+  const int grid = (nbatch + wv - 1) / wv;
+  hipStream_t st = (hipStream_t)stream;
+#define LAUNCH(KK)                                                                            \
+  do {                                                                                        \
+    PHC_HIP(hipFuncSetAttribute((const void *)qrbs_solve_kernel<KK, CX>,                     \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)(per * wv))); \
+    hipLaunchKernelGGL((qrbs_solve_kernel<KK, CX>), dim3(grid), dim3(64 * wv), per * wv, st,  \
+                       n, degp1, tailidx, nbatch, mat, rhs, sol, Q, R, flags);                \
+  } while (0)
+  if (K == 1 && !CX) LAUNCH(1);
+  else if (K == 2) LAUNCH(2);
+  else LAUNCH(4);
+#undef LAUNCH
+  PHC_HIP(hipGetLastError());
+  return 0;
+}
+
+template <bool CX>
+int linear_residue_batch(int K, int n, int degp1, int tailidx, int nbatch, const double *mat,
+                         const double *rhs, const double *sol, double *res, double *resmax,
+                         void *stream, const char *fn) {
+  // complex octo-double (K = 8) is refused: its first device run stalled
+  // (a launch of houseqr_kernel<8, true> that did not return), cause not
+  // established
+  PHC_REQUIRE(CX ? (K == 2 || K == 4) : (K == 1 || K == 2 || K == 4),
+              std::string(fn) + (CX ? ": K must be 2 or 4" : ": K must be 1, 2 or 4"));
+  PHC_REQUIRE(n >= 1 && degp1 >= 1 && nbatch >= 0 && tailidx >= 0,
+              std::string(fn) + ": bad sizes");
+  PHC_REQUIRE(mat && rhs && sol && res && resmax, std::string(fn) + ": null array");
+  if (nbatch == 0) return 0;
+  const int wv = 4, grid = (nbatch + wv - 1) / wv;
+  hipStream_t st = (hipStream_t)stream;
+#define LAUNCH(KK)                                                                            \
+  hipLaunchKernelGGL((linear_residue_kernel<KK, CX>), dim3(grid), dim3(64 * wv), 0, st, n,    \
+                     degp1, tailidx, nbatch, mat, rhs, sol, res, resmax)
+  if (K == 1 && !CX) LAUNCH(1);
+  else if (K == 2) LAUNCH(2);
+  else LAUNCH(4);
+#undef LAUNCH
+  PHC_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int phc_row_newton(phc_rowsys *s, int K, int deg, int nbsteps, int nsys, const double *cst,
+                   int cst_shared, const double *cff, int cff_shared, double *x, int *steps,
+                   int *wdeg, int *tail, int *up, int *bs, double *history, double *Q, double *R,
+                   void *stream) {
+  return row_newton<false>(s, K, deg, nbsteps, nsys, cst, cst_shared, cff, cff_shared, x, steps,
+                           wdeg, tail, up, bs, history, Q, R, stream, "phc_row_newton");
+}
+
+int phc_cmplx_row_newton(phc_rowsys *s, int K, int deg, int nbsteps, int nsys, const double *cst,
+                         int cst_shared, const double *cff, int cff_shared, double *x,
+                         int *steps, int *wdeg, int *tail, int *up, int *bs, double *history,
+                         double *Q, double *R, void *stream) {
+  return row_newton<true>(s, K, deg, nbsteps, nsys, cst, cst_shared, cff, cff_shared, x, steps,
+                          wdeg, tail, up, bs, history, Q, R, stream, "phc_cmplx_row_newton");
+}
+
+int phc_houseqr_batch(int K, int n, int nbatch, const double *A, double *Q, double *R,
+                      void *stream) {
+  return houseqr_batch<false>(K, n, nbatch, A, Q, R, stream, "phc_houseqr_batch");
+}
+
+int phc_cmplx_houseqr_batch(int K, int n, int nbatch, const double *A, double *Q, double *R,
+                            void *stream) {
+  return houseqr_batch<true>(K, n, nbatch, A, Q, R, stream, "phc_cmplx_houseqr_batch");
 }
 
 int phc_qrbs_solve_batch(int K, int n, int degp1, int tailidx, int nbatch, const double *mat,
                          double *rhs, double *sol, double *Q, double *R, int *flags,
                          void *stream) {
-  PHC_REQUIRE(K == 1 || K == 2 || K == 4, "phc_qrbs_solve_batch: K must be 1, 2 or 4");
-  PHC_REQUIRE(n >= 1 && n <= 64 && degp1 >= 1 && degp1 <= 64 && nbatch >= 0,
-              "phc_qrbs_solve_batch: n and degp1 must be in 1..64");
-  PHC_REQUIRE(tailidx >= 1, "phc_qrbs_solve_batch: tailidx must be at least 1");
-  PHC_REQUIRE(mat && rhs && sol && Q && R && flags, "phc_qrbs_solve_batch: null array");
-  if (nbatch == 0) return 0;
-  const size_t per = sizeof(double) * K * (2 * (size_t)n * n + 4 * n + 3 + 2 * (size_t)degp1 * n);
-  int wv = 0;
-  if (waves_for(per, &wv) != 0) return -2;
-  const int grid = (nbatch + wv - 1) / wv;
-  hipStream_t st = (hipStream_t)stream;
-#define LAUNCH(KK)                                                                            \
-  do {                                                                                        \
-    PHC_HIP(hipFuncSetAttribute((const void *)qrbs_solve_kernel<KK>,                         \
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)(per * wv))); \
-    hipLaunchKernelGGL(qrbs_solve_kernel<KK>, dim3(grid), dim3(64 * wv), per * wv, st, n,      \
-                       degp1, tailidx, nbatch, mat, rhs, sol, Q, R, flags);                   \
-  } while (0)
-  if (K == 1) LAUNCH(1);
-  else if (K == 2) LAUNCH(2);
-  else LAUNCH(4);
-#undef LAUNCH
-  PHC_HIP(hipGetLastError());
-  return 0;
+  return qrbs_solve_batch<false>(K, n, degp1, tailidx, nbatch, mat, rhs, sol, Q, R, flags,
+                                 stream, "phc_qrbs_solve_batch");
+}
+
+int phc_cmplx_qrbs_solve_batch(int K, int n, int degp1, int tailidx, int nbatch,
+                               const double *mat, double *rhs, double *sol, double *Q, double *R,
+                               int *flags, void *stream) {
+  return qrbs_solve_batch<true>(K, n, degp1, tailidx, nbatch, mat, rhs, sol, Q, R, flags, stream,
+                                "phc_cmplx_qrbs_solve_batch");
 }
 
 int phc_linear_residue_batch(int K, int n, int degp1, int tailidx, int nbatch,
                              const double *mat, const double *rhs, const double *sol,
                              double *res, double *resmax, void *stream) {
-  PHC_REQUIRE(K == 1 || K == 2 || K == 4, "phc_linear_residue_batch: K must be 1, 2 or 4");
-  PHC_REQUIRE(n >= 1 && degp1 >= 1 && nbatch >= 0 && tailidx >= 0,
-              "phc_linear_residue_batch: bad sizes");
-  PHC_REQUIRE(mat && rhs && sol && res && resmax, "phc_linear_residue_batch: null array");
-  if (nbatch == 0) return 0;
-  const int wv = 4, grid = (nbatch + wv - 1) / wv;
-  hipStream_t st = (hipStream_t)stream;
-  if (K == 1)
-    hipLaunchKernelGGL(linear_residue_kernel<1>, dim3(grid), dim3(64 * wv), 0, st, n, degp1,
-                       tailidx, nbatch, mat, rhs, sol, res, resmax);
-  else if (K == 2)
-    hipLaunchKernelGGL(linear_residue_kernel<2>, dim3(grid), dim3(64 * wv), 0, st, n, degp1,
-                       tailidx, nbatch, mat, rhs, sol, res, resmax);
-  else
-    hipLaunchKernelGGL(linear_residue_kernel<4>, dim3(grid), dim3(64 * wv), 0, st, n, degp1,
-                       tailidx, nbatch, mat, rhs, sol, res, resmax);
-  PHC_HIP(hipGetLastError());
-  return 0;
+  return linear_residue_batch<false>(K, n, degp1, tailidx, nbatch, mat, rhs, sol, res, resmax,
+                                     stream, "phc_linear_residue_batch");
+}
+
+int phc_cmplx_linear_residue_batch(int K, int n, int degp1, int tailidx, int nbatch,
+                                   const double *mat, const double *rhs, const double *sol,
+                                   double *res, double *resmax, void *stream) {
+  return linear_residue_batch<true>(K, n, degp1, tailidx, nbatch, mat, rhs, sol, res, resmax,
+                                    stream, "phc_cmplx_linear_residue_batch");
 }
 
 }  // extern "C"

@@ -10,7 +10,11 @@ Mirrors the reference's host interface for this path:
                  (:1447-1494); exponents > 1 by the common-factor trick of
                  CPU_dbl2_evaluate_monomials (Newton/dbl2_systems_host.cpp:12-111)
 for K = 1, 2, 4 limbs (dbl, dbl2, dbl4), batched: every array carries a batch
-index after the limb index.  Arrays are numpy float64 in the limb-planar
+index after the limb index.  With cplx=True the complex twins
+(CPU_cmplx2_factors_houseqr, CPU_cmplx2_qrbs_solve, CPU_cmplx2_linear_residue,
+cmplx2_row_newton_qrstep, Newton/cmplx2_newton_method.cpp:565) run on
+complex arrays of 2K planes (K real, then K imaginary), K = 2, 4 (8 for
+the QR, the solve and the residual).  Arrays are numpy float64 in the limb-planar
 layout of include/phcmd.h; the work runs on the GPU through libphcmd.so.
 """
 import ctypes
@@ -24,47 +28,49 @@ def _ip(a):
     return np.ascontiguousarray(a, np.int32)
 
 
-def houseqr(K, A):
-    """Q, R of every n x n matrix of A [K][nbatch][n][n]"""
+def houseqr(K, A, cplx=False):
+    """Q, R of every n x n matrix of A [P][nbatch][n][n] (P = K, or 2K complex)"""
     A = np.ascontiguousarray(A, np.float64)
     _, nb, n, _ = A.shape
     dA = DeviceArray.from_host(A)
     dQ, dR = DeviceArray(A.shape), DeviceArray(A.shape)
-    check(load().phc_houseqr_batch(K, n, nb, dA.ptr, dQ.ptr, dR.ptr, None))
+    fn = load().phc_cmplx_houseqr_batch if cplx else load().phc_houseqr_batch
+    check(fn(K, n, nb, dA.ptr, dQ.ptr, dR.ptr, None))
     return dQ.to_host(), dR.to_host()
 
 
-def qrbs_solve(K, mat, rhs, tailidx, sol=None, Q=None, R=None, zeroQ=1, noqr=0):
-    """block-Toeplitz solve of every batch entry: mat [K][nb][degp1][n][n],
-    rhs [K][nb][degp1][n].  Returns dict(rhs, sol, Q, R, zeroQ, noqr, up, bs,
-    newtail) with the flags as int arrays [nb]."""
+def qrbs_solve(K, mat, rhs, tailidx, sol=None, Q=None, R=None, zeroQ=1, noqr=0, cplx=False):
+    """block-Toeplitz solve of every batch entry: mat [P][nb][degp1][n][n],
+    rhs [P][nb][degp1][n] (P = K, or 2K complex).  Returns dict(rhs, sol,
+    Q, R, zeroQ, noqr, up, bs, newtail) with the flags as int arrays [nb]."""
     mat = np.ascontiguousarray(mat, np.float64)
-    _, nb, degp1, n, _ = mat.shape
-    vshape, qshape = (K, nb, degp1, n), (K, nb, n, n)
+    P, nb, degp1, n, _ = mat.shape
+    vshape, qshape = (P, nb, degp1, n), (P, nb, n, n)
     d = {k: DeviceArray.from_host(np.zeros(s) if v is None else np.ascontiguousarray(v, np.float64))
          for k, v, s in (("rhs", rhs, vshape), ("sol", sol, vshape), ("Q", Q, qshape),
                          ("R", R, qshape))}
     flags = np.zeros((nb, 5), np.int32)
     flags[:, 0], flags[:, 1], flags[:, 4] = zeroQ, noqr, tailidx
     dm, df = DeviceArray.from_host(mat), DeviceArray.from_host(flags)
-    check(load().phc_qrbs_solve_batch(K, n, degp1, tailidx, nb, dm.ptr, d["rhs"].ptr,
-                                      d["sol"].ptr, d["Q"].ptr, d["R"].ptr, df.ptr, None))
+    fn = load().phc_cmplx_qrbs_solve_batch if cplx else load().phc_qrbs_solve_batch
+    check(fn(K, n, degp1, tailidx, nb, dm.ptr, d["rhs"].ptr, d["sol"].ptr, d["Q"].ptr,
+             d["R"].ptr, df.ptr, None))
     f = df.to_host()
     out = {k: v.to_host() for k, v in d.items()}
     out.update(zeroQ=f[:, 0], noqr=f[:, 1], up=f[:, 2], bs=f[:, 3], newtail=f[:, 4])
     return out
 
 
-def linear_residue(K, mat, rhs, sol, tailidx):
-    """residual vectors [K][nb][degp1][n] and the largest entries [K][nb]"""
+def linear_residue(K, mat, rhs, sol, tailidx, cplx=False):
+    """residual vectors [P][nb][degp1][n] and the largest entries [K][nb]"""
     mat = np.ascontiguousarray(mat, np.float64)
-    _, nb, degp1, n, _ = mat.shape
+    P, nb, degp1, n, _ = mat.shape
     dm = DeviceArray.from_host(mat)
     db, dx = DeviceArray.from_host(np.ascontiguousarray(rhs)), DeviceArray.from_host(np.ascontiguousarray(sol))
-    dr = DeviceArray.zeros((K, nb, degp1, n))
+    dr = DeviceArray.zeros((P, nb, degp1, n))
     dmax = DeviceArray((K, nb))
-    check(load().phc_linear_residue_batch(K, n, degp1, tailidx, nb, dm.ptr, db.ptr, dx.ptr,
-                                          dr.ptr, dmax.ptr, None))
+    fn = load().phc_cmplx_linear_residue_batch if cplx else load().phc_linear_residue_batch
+    check(fn(K, n, degp1, tailidx, nb, dm.ptr, db.ptr, dx.ptr, dr.ptr, dmax.ptr, None))
     return dr.to_host(), dmax.to_host()
 
 
@@ -102,33 +108,34 @@ class RowSystem:
             pass
 
     def row_newton_device(self, K, deg, nbsteps, nsys, d_cst, cst_shared, d_cff, cff_shared, d_x,
-                          logs=None, stream=None):
+                          logs=None, stream=None, cplx=False):
         """device-resident run; logs: dict of DeviceArrays (steps, wdeg, tail,
         up, bs, history, Q, R) or None"""
         g = (lambda k: logs[k].ptr if logs and k in logs else None)
-        check(load().phc_row_newton(self.handle, K, deg, nbsteps, nsys, d_cst.ptr, int(cst_shared),
-                                    d_cff.ptr, int(cff_shared), d_x.ptr, g("steps"), g("wdeg"),
-                                    g("tail"), g("up"), g("bs"), g("history"), g("Q"), g("R"),
-                                    stream))
+        fn = load().phc_cmplx_row_newton if cplx else load().phc_row_newton
+        check(fn(self.handle, K, deg, nbsteps, nsys, d_cst.ptr, int(cst_shared), d_cff.ptr,
+                 int(cff_shared), d_x.ptr, g("steps"), g("wdeg"), g("tail"), g("up"), g("bs"),
+                 g("history"), g("Q"), g("R"), stream))
 
-    def row_newton(self, K, deg, nbsteps, cst, cff, x, history=False):
+    def row_newton(self, K, deg, nbsteps, cst, cff, x, history=False, cplx=False):
         """Staggered row Newton on a batch of systems with these supports.
 
-        cst [K][nc][dim][deg+1], cff [K][nf][nmono][deg+1] (nc, nf = 1 for
-        coefficients shared by the batch), x [K][nsys][dim][deg+1].
-        Returns dict(x, steps, wdeg, tail, up, bs, Q, R[, history])."""
+        cst [P][nc][dim][deg+1], cff [P][nf][nmono][deg+1] (nc, nf = 1 for
+        coefficients shared by the batch), x [P][nsys][dim][deg+1], P = K
+        (or 2K complex).  Returns dict(x, steps, wdeg, tail, up, bs, Q,
+        R[, history])."""
         x = np.ascontiguousarray(x, np.float64)
-        _, nsys, dim, D = x.shape
+        P, nsys, dim, D = x.shape
         cst, cff = np.ascontiguousarray(cst, np.float64), np.ascontiguousarray(cff, np.float64)
         dc, df, dx = DeviceArray.from_host(cst), DeviceArray.from_host(cff), DeviceArray.from_host(x)
         logs = {k: DeviceArray.zeros((nsys, nbsteps), np.int32) for k in ("wdeg", "tail", "up", "bs")}
         logs["steps"] = DeviceArray.zeros((nsys,), np.int32)
-        logs["Q"] = DeviceArray.zeros((K, nsys, dim, dim))
-        logs["R"] = DeviceArray.zeros((K, nsys, dim, dim))
+        logs["Q"] = DeviceArray.zeros((P, nsys, dim, dim))
+        logs["R"] = DeviceArray.zeros((P, nsys, dim, dim))
         if history:
-            logs["history"] = DeviceArray.zeros((nbsteps, K, nsys, dim, D))
+            logs["history"] = DeviceArray.zeros((nbsteps, P, nsys, dim, D))
         self.row_newton_device(K, D - 1, nbsteps, nsys, dc, cst.shape[1] == 1 and nsys > 1, df,
-                               cff.shape[1] == 1 and nsys > 1, dx, logs)
+                               cff.shape[1] == 1 and nsys > 1, dx, logs, cplx=cplx)
         check(load().phc_synchronize())
         out = {k: v.to_host() for k, v in logs.items()}
         out["x"] = dx.to_host()

@@ -574,3 +574,124 @@ def md_diff(K, a, b):
         oracle().omd_batch(1, K, n, pd(aa), pd(bb), pd(c))
         out[h] = c[0].reshape(a.shape[1:])
     return out
+
+
+# --------------------------------------------------- complex power-series Newton --
+
+def _cx_houseqr(lib, fn, K, A):
+    P, m, n = A.shape
+    Q, R = np.zeros((P, m, m)), np.zeros((P, m, n))
+    getattr(lib, fn)(K, m, n, pd(np.ascontiguousarray(A)), pd(Q), pd(R))
+    return Q, R
+
+
+def oracle_cx_houseqr(K, A):
+    return _cx_houseqr(oracle(), "onw_cx_houseqr", K, A)
+
+
+def ref_cx_houseqr(K, A):
+    return _cx_houseqr(ref(), "ref_cx_houseqr", K, A)
+
+
+def _cx_qrbs_solve(lib, fn, K, mat, rhs, tailidx, Q=None, R=None, zeroQ=1, noqr=0, sol=None):
+    P, degp1, dim, _ = mat.shape
+    rhs = np.ascontiguousarray(rhs).copy()
+    sol = np.zeros((P, degp1, dim)) if sol is None else sol.copy()
+    Q = np.zeros((P, dim, dim)) if Q is None else Q.copy()
+    R = np.zeros((P, dim, dim)) if R is None else R.copy()
+    z, nq = ctypes.c_int(zeroQ), ctypes.c_int(noqr)
+    up, bs, nt = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(tailidx)
+    getattr(lib, fn)(K, dim, degp1, tailidx, pd(np.ascontiguousarray(mat)), pd(rhs), pd(sol),
+                     pd(Q), pd(R), ctypes.byref(z), ctypes.byref(nq), ctypes.byref(up),
+                     ctypes.byref(bs), ctypes.byref(nt))
+    return dict(rhs=rhs, sol=sol, Q=Q, R=R, zeroQ=z.value, noqr=nq.value, up=up.value,
+                bs=bs.value, newtail=nt.value)
+
+
+def oracle_cx_qrbs_solve(K, mat, rhs, tailidx, **kw):
+    return _cx_qrbs_solve(oracle(), "onw_cx_qrbs_solve", K, mat, rhs, tailidx, **kw)
+
+
+def ref_cx_qrbs_solve(K, mat, rhs, tailidx, **kw):
+    return _cx_qrbs_solve(ref(), "ref_cx_qrbs_solve", K, mat, rhs, tailidx, **kw)
+
+
+def _cx_residue(lib, fn, K, mat, rhs, sol, tailidx):
+    P, degp1, dim, _ = mat.shape
+    rv = np.zeros((P, degp1, dim))
+    rm = np.zeros(K)
+    getattr(lib, fn)(K, dim, degp1, tailidx, pd(np.ascontiguousarray(mat)),
+                     pd(np.ascontiguousarray(rhs)), pd(np.ascontiguousarray(sol)), pd(rv), pd(rm))
+    return rv, rm
+
+
+def oracle_cx_linear_residue(K, mat, rhs, sol, tailidx):
+    return _cx_residue(oracle(), "onw_cx_linear_residue", K, mat, rhs, sol, tailidx)
+
+
+def ref_cx_linear_residue(K, mat, rhs, sol, tailidx):
+    return _cx_residue(ref(), "ref_cx_linear_residue", K, mat, rhs, sol, tailidx)
+
+
+def _cx_row_newton(lib, fn, K, sysm, start, nbsteps):
+    P, dim, D = start.shape
+    x = np.ascontiguousarray(start).copy()
+    wdeg, tail, up, bs = (np.zeros(nbsteps, np.int32) for _ in range(4))
+    hist = np.zeros((nbsteps, P, dim, D))
+    Q, R = np.zeros((P, dim, dim)), np.zeros((P, dim, dim))
+    idx = sysm.idx if sysm.idx.size else np.zeros(1, np.int32)
+    steps = getattr(lib, fn)(K, dim, sysm.deg, nbsteps, pi(sysm.monoptr), pi(sysm.nvr),
+                             pi(sysm.idxptr), pi(idx), pd(sysm.cst), pd(sysm.cff), pd(x),
+                             pi(wdeg), pi(tail), pi(up), pi(bs), pd(hist), pd(Q), pd(R))
+    return dict(x=x, steps=steps, wdeg=wdeg[:steps], tail=tail[:steps], up=up[:steps],
+                bs=bs[:steps], history=hist[:steps], Q=Q, R=R)
+
+
+def oracle_cx_row_newton(K, sysm, start, nbsteps):
+    return _cx_row_newton(oracle(), "onw_cx_row_newton", K, sysm, start, nbsteps)
+
+
+def ref_cx_row_newton(K, sysm, start, nbsteps):
+    return _cx_row_newton(ref(), "ref_cx_row_newton", K, sysm, start, nbsteps)
+
+
+def cx_poly_value(K, sysm, x):
+    """complex values [2K][dim][D] of every polynomial at the series x [2K][dim][D]"""
+    o = oracle()
+    dim, D = sysm.dim, sysm.deg + 1
+    val = np.zeros((2 * K, dim, D))
+    for i in range(dim):
+        m0, m1 = sysm.monoptr[i], sysm.monoptr[i + 1]
+        nv = np.ascontiguousarray(sysm.nvr[m0:m1])
+        ip = np.ascontiguousarray(sysm.idxptr[m0:m1])
+        cst = np.ascontiguousarray(sysm.cst[:, i])
+        cff = np.ascontiguousarray(sysm.cff[:, m0:m1])
+        out = np.zeros((2 * K, dim + 1, D))
+        idx = sysm.idx if sysm.idx.size else np.zeros(1, np.int32)
+        o.osr_poly_evaldiff_exp(K, 1, dim, int(m1 - m0), sysm.deg, pi(nv), pi(ip), pi(idx), None,
+                                pd(cst), pd(cff), pd(np.ascontiguousarray(x)), pd(out))
+        val[:, i] = out[:, dim]
+    return val
+
+
+def planted_cx_row_system(rng, K, dim, deg, nper=8, nvar=3, supports=None):
+    """the complex twin of planted_row_system: complex K-limb coefficient
+    series, the root x_i(t) = c_i exp(r_i t) with complex c_i (|c_i| in
+    [0.5, 1]) and r_i, constants set so that it is a root; start = the
+    leading coefficients plus 1e-10"""
+    D = deg + 1
+    monoptr, nvr, idxptr, idx = supports or random_supports(rng, dim, nper, nvar)
+    nmono = int(monoptr[-1])
+    cff = rand_cmd(rng, K, (nmono, D))
+    c = rng.uniform(0.5, 1.0, dim) * np.exp(2j * np.pi * rng.uniform(0, 1, dim))
+    r = rng.uniform(-1.0, 1.0, dim) + 1j * rng.uniform(-1.0, 1.0, dim)
+    fact = np.cumprod(np.concatenate([[1.0], np.arange(1, D)]))
+    z = c[:, None] * r[:, None] ** np.arange(D)[None, :] / fact[None, :]
+    root = np.zeros((2 * K, dim, D))
+    root[0], root[K] = z.real, z.imag
+    sysm = RowSystem(dim, deg, monoptr, nvr, idxptr, idx, np.zeros((2 * K, dim, D)), cff, None)
+    sysm.cst = np.ascontiguousarray(-cx_poly_value(K, sysm, root))
+    start = np.zeros((2 * K, dim, D))
+    start[0, :, 0] = root[0, :, 0] + 1e-10
+    start[K, :, 0] = root[K, :, 0] + 1e-10
+    return sysm, root, start

@@ -215,3 +215,122 @@ def test_row_newton_with_larger_exponents_matches_oracle():
                                               start[:, None])
         o = oracle_row_newton(2, sysm, start, 6)
         assert np.array_equal(out["x"][:, 0], o["x"]), nvar
+
+
+# ------------------------------------------------------------ complex
+
+# complex octo-double stays off the device: the first run of
+# test_cx_houseqr_batch_matches_oracle[1-8] stalled (killed for silence after
+# 180 s) and the cause is not established; the library refuses K = 8 for
+# the complex kernels, and these cases are skipped
+K8_STALL = "complex octo-double stalled on the device once; cause not established, K = 8 refused"
+CKS = (2, 4, pytest.param(8, marks=pytest.mark.skip(reason=K8_STALL)))
+
+
+@pytest.mark.parametrize("K", CKS)
+@pytest.mark.parametrize("n", [1, 5, 8, 13])
+def test_cx_houseqr_batch_matches_oracle(K, n):
+    from oraclelib import oracle_cx_houseqr, rand_cmd
+    from phcpack_amd.newton import houseqr
+    print(f"complex houseqr K={K} n={n}", flush=True)
+    rng = np.random.default_rng(20 * K + n)
+    nb = 4
+    A = rand_cmd(rng, K, (nb, n, n))
+    Q, R = houseqr(K, A, cplx=True)
+    for b in range(nb):
+        q, r = oracle_cx_houseqr(K, np.ascontiguousarray(A[:, b]))
+        assert np.array_equal(Q[:, b], q) and np.array_equal(R[:, b], r), b
+
+
+@pytest.mark.parametrize("K", CKS)
+@pytest.mark.parametrize("tailidx", [1, 2])
+def test_cx_qrbs_solve_batch_matches_oracle(K, tailidx):
+    from oraclelib import oracle_cx_qrbs_solve, rand_cmd
+    from phcpack_amd.newton import qrbs_solve
+    print(f"complex qrbs_solve K={K} tail={tailidx}", flush=True)
+    rng = np.random.default_rng(400 + K + tailidx)
+    nb, degp1, n = 4, 7, 6
+    mat = rand_cmd(rng, K, (nb, degp1, n, n))
+    rhs = rand_cmd(rng, K, (nb, degp1, n))
+    mat[:, 1, 1:] = 0.0     # system 1: block 3 of the right-hand side stays zero
+    rhs[:, 1, 3] = 0.0
+    g = qrbs_solve(K, mat, rhs, tailidx, cplx=True)
+    for b in range(nb):
+        o = oracle_cx_qrbs_solve(K, np.ascontiguousarray(mat[:, b]),
+                                 np.ascontiguousarray(rhs[:, b]), tailidx)
+        for k in ("rhs", "sol", "Q", "R"):
+            assert np.array_equal(g[k][:, b], o[k]), (b, k)
+        for k in ("zeroQ", "noqr", "up", "bs", "newtail"):
+            assert g[k][b] == o[k], (b, k)
+    assert g["bs"][1] >= 1 and g["up"][1] >= 1
+    rhs2 = rhs.copy()
+    rhs2[:, :, 0] = 0.0
+    g2 = qrbs_solve(K, mat, rhs2, tailidx, Q=g["Q"], R=g["R"], zeroQ=0, noqr=0, cplx=True)
+    for b in range(nb):
+        o2 = oracle_cx_qrbs_solve(K, np.ascontiguousarray(mat[:, b]),
+                                  np.ascontiguousarray(rhs2[:, b]), tailidx,
+                                  Q=np.ascontiguousarray(g["Q"][:, b]),
+                                  R=np.ascontiguousarray(g["R"][:, b]), zeroQ=0, noqr=0)
+        for k in ("rhs", "sol", "Q", "R"):
+            assert np.array_equal(g2[k][:, b], o2[k]), (b, k)
+        assert g2["noqr"][b] == 1 == o2["noqr"]
+
+
+@pytest.mark.parametrize("K", CKS)
+def test_cx_linear_residue_batch_matches_oracle(K):
+    from oraclelib import oracle_cx_linear_residue, rand_cmd
+    from phcpack_amd.newton import linear_residue
+    print(f"complex linear_residue K={K}", flush=True)
+    rng = np.random.default_rng(500 + K)
+    nb, degp1, n = 3, 6, 5
+    mat = rand_cmd(rng, K, (nb, degp1, n, n))
+    rhs = rand_cmd(rng, K, (nb, degp1, n))
+    sol = rand_cmd(rng, K, (nb, degp1, n))
+    for tail in (0, 2):
+        r, m = linear_residue(K, mat, rhs, sol, tail, cplx=True)
+        for b in range(nb):
+            ro, mo = oracle_cx_linear_residue(K, np.ascontiguousarray(mat[:, b]),
+                                              np.ascontiguousarray(rhs[:, b]),
+                                              np.ascontiguousarray(sol[:, b]), tail)
+            assert np.array_equal(r[:, b, tail:], ro[:, tail:]) and np.array_equal(m[:, b], mo)
+
+
+def test_cx_octo_double_is_refused():
+    """K = 8 complex is refused by the library, before any device work"""
+    from phcpack_amd._lib import PhcError
+    from phcpack_amd.newton import houseqr
+    with pytest.raises(PhcError):
+        houseqr(8, np.zeros((16, 1, 2, 2)), cplx=True)
+
+
+@pytest.mark.parametrize("K", (2, 4))
+@pytest.mark.parametrize("deg", [8, 15])
+def test_cx_row_newton_batch_matches_oracle(K, deg):
+    """the complex staggered row Newton on a batch of planted systems with
+    the same supports: every step, the series, Q and R bit-identical to the
+    oracle (itself bit-exact with the reference's complex host code), and
+    the planted roots reached"""
+    from oraclelib import oracle_cx_row_newton, planted_cx_row_system
+    print(f"complex row newton batch K={K} deg={deg}", flush=True)
+    rng = np.random.default_rng(600 + K + deg)
+    sup = random_supports(rng, 6, 6, 3)
+    nsys = 4
+    systems = [planted_cx_row_system(rng, K, 6, deg, supports=sup) for _ in range(nsys)]
+    cst = np.stack([s[0].cst for s in systems], axis=1)
+    cff = np.stack([s[0].cff for s in systems], axis=1)
+    x0 = np.stack([s[2] for s in systems], axis=1)
+    dev = _device_system(systems[0][0])
+    out = dev.row_newton(K, deg, 12, cst, cff, x0, history=True, cplx=True)
+    for b, (sysm, root, start) in enumerate(systems):
+        o = oracle_cx_row_newton(K, sysm, start, 12)
+        st = o["steps"]
+        assert out["steps"][b] == st, b
+        for k in ("wdeg", "tail", "up", "bs"):
+            assert np.array_equal(out[k][b, :st], o[k]), (b, k)
+        for s in range(st):
+            assert np.array_equal(out["history"][s, :, b], o["history"][s]), (b, s)
+        assert np.array_equal(out["x"][:, b], o["x"]), b
+        assert np.array_equal(out["Q"][:, b], o["Q"]) and np.array_equal(out["R"][:, b], o["R"])
+        err = np.abs(out["x"][:K, b].sum(axis=0) - root[:K].sum(axis=0)).max() + \
+            np.abs(out["x"][K:, b].sum(axis=0) - root[K:].sum(axis=0)).max()
+        assert err < 1e-12, (b, err)

@@ -315,3 +315,118 @@ def test_row_newton_with_exponents_converges_to_planted_root(K):
                                            exp=exp)
     out = oracle_row_newton(K, sysm, start, 12)
     assert np.abs(lead(out["x"]) - lead(root)).max() < {2: 1e-20, 4: 1e-40}[K]
+
+
+# ------------------------------------------------------------ complex
+
+CKS = (2, 4, 8)
+
+
+@needs_ref
+@pytest.mark.parametrize("K", CKS)
+@pytest.mark.parametrize("shape", [(8, 8), (9, 7), (1, 1), (5, 1)])
+def test_cx_houseqr_matches_reference(K, shape):
+    """CPU_cmplx{2,4,8}_factors_houseqr restated: bit-exact"""
+    from oraclelib import oracle_cx_houseqr, rand_cmd, ref_cx_houseqr
+    rng = np.random.default_rng(10 * K + shape[0])
+    A = rand_cmd(rng, K, shape)
+    q1, r1 = ref_cx_houseqr(K, A)
+    q2, r2 = oracle_cx_houseqr(K, A)
+    assert np.array_equal(q1, q2) and np.array_equal(r1, r2)
+
+
+@needs_ref
+@pytest.mark.parametrize("K", CKS)
+@pytest.mark.parametrize("tailidx", [1, 3])
+def test_cx_qrbs_solve_matches_reference(K, tailidx):
+    """CPU_cmplx{2,4,8}_qrbs_solve restated (head, tail, skip rules): a
+    zero block right-hand side and a zero solution block exercise the skips"""
+    from oraclelib import oracle_cx_qrbs_solve, rand_cmd, ref_cx_qrbs_solve
+    rng = np.random.default_rng(40 + K + tailidx)
+    dim, degp1 = 6, 7
+    mat = rand_cmd(rng, K, (degp1, dim, dim))
+    rhs = rand_cmd(rng, K, (degp1, dim))
+    rhs[:, 4] = 0.0
+    a = ref_cx_qrbs_solve(K, mat, rhs, tailidx)
+    b = oracle_cx_qrbs_solve(K, mat, rhs, tailidx)
+    for k in ("rhs", "sol", "Q", "R", "zeroQ", "noqr", "up", "bs", "newtail"):
+        assert np.array_equal(a[k], b[k]), k
+    # higher blocks zero and a zero block of the right-hand side: its
+    # solution block is skipped (bs), and then the update with it (up)
+    mz = mat.copy()
+    mz[:, 1:] = 0.0
+    rz = rhs.copy()
+    rz[:, 3] = 0.0
+    az = ref_cx_qrbs_solve(K, mz, rz, tailidx)
+    bz = oracle_cx_qrbs_solve(K, mz, rz, tailidx)
+    for k in ("rhs", "sol", "Q", "R", "zeroQ", "noqr", "up", "bs", "newtail"):
+        assert np.array_equal(az[k], bz[k]), k
+    assert az["bs"] >= 1 and az["up"] >= 1
+    # a second call reusing Q, R with a zero leading right-hand side (noqr)
+    rhs2 = rand_cmd(rng, K, (degp1, dim))
+    rhs2[:, 0] = 0.0
+    kw = dict(Q=a["Q"], R=a["R"], zeroQ=a["zeroQ"], noqr=a["noqr"])
+    a2 = ref_cx_qrbs_solve(K, mat, rhs2, tailidx, **kw)
+    b2 = oracle_cx_qrbs_solve(K, mat, rhs2, tailidx, **kw)
+    for k in ("rhs", "sol", "zeroQ", "noqr", "up", "bs", "newtail"):
+        assert np.array_equal(a2[k], b2[k]), k
+    assert a2["noqr"] == 1
+
+
+@needs_ref
+@pytest.mark.parametrize("K", CKS)
+def test_cx_linear_residue_matches_reference(K):
+    from oraclelib import oracle_cx_linear_residue, rand_cmd, ref_cx_linear_residue
+    rng = np.random.default_rng(70 + K)
+    dim, degp1 = 5, 6
+    mat = rand_cmd(rng, K, (degp1, dim, dim))
+    rhs = rand_cmd(rng, K, (degp1, dim))
+    sol = rand_cmd(rng, K, (degp1, dim))
+    for tail in (0, 2):
+        r1, m1 = ref_cx_linear_residue(K, mat, rhs, sol, tail)
+        r2, m2 = oracle_cx_linear_residue(K, mat, rhs, sol, tail)
+        assert np.array_equal(r1, r2) and np.array_equal(m1, m2)
+
+
+@needs_ref
+@pytest.mark.parametrize("K", (2, 4))
+def test_cx_row_newton_matches_reference(K):
+    """the staggered complex row Newton (test_cmplx{2,4}_row_newton's host
+    branch on the reference's functions) restated: every step bit-exact"""
+    from oraclelib import oracle_cx_row_newton, planted_cx_row_system, ref_cx_row_newton
+    rng = np.random.default_rng(90 + K)
+    sysm, root, start = planted_cx_row_system(rng, K, 6, 12, nper=4, nvar=3)
+    a = ref_cx_row_newton(K, sysm, start, 12)
+    b = oracle_cx_row_newton(K, sysm, start, 12)
+    assert a["steps"] == b["steps"]
+    for k in ("x", "wdeg", "tail", "up", "bs", "history", "Q", "R"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("K", (2, 4))
+def test_cx_row_newton_converges_to_planted_root(K):
+    """the complex row Newton from the perturbed leading coefficients
+    reaches the planted exponential-series root to working precision"""
+    from oraclelib import oracle_cx_row_newton, planted_cx_row_system
+    rng = np.random.default_rng(120 + K)
+    sysm, root, start = planted_cx_row_system(rng, K, 6, 12, nper=4, nvar=3)
+    out = oracle_cx_row_newton(K, sysm, start, 16)
+    err = np.abs(out["x"][:K].sum(axis=0) - root[:K].sum(axis=0)).max() + \
+        np.abs(out["x"][K:].sum(axis=0) - root[K:].sum(axis=0)).max()
+    assert err < 1e-12, err
+    assert out["tail"][-1] >= 12
+
+
+@pytest.mark.parametrize("K", (2, 4))
+def test_cx_houseqr_identities(K):
+    """Q^H Q = I and Q R = A in double from the limb sums; R upper triangular"""
+    from oraclelib import oracle_cx_houseqr, rand_cmd
+    rng = np.random.default_rng(5 + K)
+    A = rand_cmd(rng, K, (9, 9))
+    Q, R = oracle_cx_houseqr(K, A)
+    q = Q[:K].sum(axis=0) + 1j * Q[K:].sum(axis=0)
+    r = R[:K].sum(axis=0) + 1j * R[K:].sum(axis=0)
+    a = A[:K].sum(axis=0) + 1j * A[K:].sum(axis=0)
+    assert np.abs(q.conj().T @ q - np.eye(9)).max() < 1e-13
+    assert np.abs(q @ r - a).max() < 1e-13
+    assert np.abs(np.tril(r, -1)).max() < 1e-13
