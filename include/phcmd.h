@@ -470,6 +470,35 @@ int phc_row_newton(phc_rowsys *sys, int K, int deg, int nbsteps, int nsys, const
                    int *steps_dev, int *wdeg_dev, int *tail_dev, int *up_dev, int *bs_dev,
                    double *history_dev, double *Q_dev, double *R_dev, void *stream);
 
+/* A system in the column representation of dbl2_column_newton_qrstep
+ * (Newton/dbl2_newton_method.cpp:259-427): nbrcol columns of dim monomials,
+ * monomial m = c*dim + j (equation j of column c) with nvr[m] distinct
+ * variables (0: none), their indices idx[] concatenated in the order of m.
+ * nbrcol = 1 is a monomial system x^A_i = mb_i: nvr[i] >= 1, unit
+ * coefficients times the common factors of the exponents exps[] (parallel
+ * to idx, 1..64; NULL: all ones), evaluated by CPU_dbl2_evaluate_monomials
+ * (Newton/dbl2_systems_host.cpp:12-111) and linearized by
+ * dbl2_linearize_evaldiff_output (:443-523).  nbrcol > 1: coefficient
+ * series per monomial, no exponents, the constant -1 in the last equation,
+ * evaluated by CPU_dbl2_evaluate_columns (:232-329) with dbl2_define_rhs
+ * (:626-659).  Destroy with phc_rowsys_destroy. */
+int phc_colsys_create(phc_rowsys **sys, int dim, int nbrcol, const int *nvr, const int *idx,
+                      const int *exps);
+
+/* staggered column Newton (K = 1, 2, 4), bit-identical to the reference's
+ * host branch: the loop of test_dbl2_column_newton
+ * (Newton/dbl2_newton_method.cpp:1036-1287) over dbl2_column_newton_qrstep,
+ * one wavefront per system.  cff [K][nf][nbrcol*dim][deg+1] (nf = 1 when
+ * cff_shared, else nsys; ignored for monomial systems, whose coefficients
+ * are the unit series), mb [K][nm][dim][deg+1] the right-hand side (nm = 1
+ * when mb_shared, else nsys): each step solves J dx = -(f(x) - mb).  Other
+ * arguments and outputs as phc_row_newton. */
+int phc_column_newton(phc_rowsys *sys, int K, int deg, int nbsteps, int nsys,
+                      const double *cff_dev, int cff_shared, const double *mb_dev, int mb_shared,
+                      double *x_dev, int *steps_dev, int *wdeg_dev, int *tail_dev, int *up_dev,
+                      int *bs_dev, double *history_dev, double *Q_dev, double *R_dev,
+                      void *stream);
+
 /* the complex staggered row Newton (K = 2, 4; a system of distinct
  * variables): cst, cff, x, history, Q, R complex limb-planar (2K planes),
  * each step as cmplx2_row_newton_qrstep (Newton/cmplx2_newton_method.cpp:565)

@@ -23,6 +23,13 @@
  *                          dbl2_map_evaldiff_output (Newton/dbl2_systems_host.cpp:709-757),
  *                          dbl2_update_series (Newton/dbl2_newton_testers.cpp:116-149)
  *   staggered degrees      test_dbl2_row_newton loop (Newton/dbl2_newton_method.cpp:1447-1494)
+ *   column Newton          test_dbl2_column_newton loop (:1036-1287) over
+ *                          dbl2_column_newton_qrstep host branch (:259-427):
+ *                          CPU_dbl2_evaluate_monomials (Newton/dbl2_systems_host.cpp:12-111)
+ *                          with dbl2_linearize_evaldiff_output (:443-523), or
+ *                          CPU_dbl2_evaluate_columns (:232-329) with dbl2_define_rhs
+ *                          (:626-659); monomials by CPU_dbl2_evaldiff
+ *                          (Convolutions/dbl2_monomials_host.cpp:175-283)
  * Skip thresholds per precision: Newton/dbl_bals_host.cpp:199 (1e-15; its
  * head tests nrm + 1 == 1 instead, :44),
  * dbl2_bals_host.cpp:43 (1e-28), dbl4_bals_host.cpp:51 (1e-56),
@@ -41,10 +48,11 @@
  *                          cmplx2_update_series (Newton/dbl2_newton_testers.cpp:151-180)
  *   staggered degrees      test_cmplx2_row_newton (Newton/cmplx2_newton_method.cpp:1759)
  *
- * Parity pinned by tests/test_oracle_newton.py: bit-exact against the
- * reference functions compiled in oracle/_ref (QR, Toeplitz solve, residual,
- * whole staggered runs at K = 1, 2, 4, 8) and against committed fixtures of
- * the reference's own cyclic test system; QR identities Q^T Q = I, QR = A;
+ * Parity pinned by tests/test_oracle_newton.py and test_oracle_columns.py:
+ * bit-exact against the reference functions compiled in oracle/_ref (QR,
+ * Toeplitz solve, residual, whole staggered row and column runs at K = 1, 2,
+ * 4, 8) and against committed fixtures of the reference's own cyclic row
+ * system and its monomial, two-column and cyclic column systems; QR identities Q^T Q = I, QR = A;
  * a planted exponential-series solution recovered to working precision.
  */
 #include <stdlib.h>
@@ -470,6 +478,186 @@ int onw_row_newton(int K, int dim, int deg, int nbsteps, const int *monoptr, con
   free(pout);
   free(nv);
   free(ip);
+  return stepcnt;
+}
+
+/* ========================================================= column systems */
+
+/* z_k = a_0 b_k + sum_{i=1..k} a_i b_{k-i}, k = 0..deg, interleaved series
+ * (CPU_dbl2_product, Convolutions/dbl2_convolutions_host.cpp:7-26); z must
+ * not overlap a or b */
+static void iprod(int K, int deg, const double *a, const double *b, double *z) {
+  double t[MAXK];
+  int k, i;
+  for (k = 0; k <= deg; k++) {
+    double *acc = E(z, k);
+    omd_mul(K, E(a, 0), E(b, k), acc);
+    for (i = 1; i <= k; i++) {
+      omd_mul(K, E(a, i), E(b, k - i), t);
+      omd_add(K, acc, t, acc);
+    }
+  }
+}
+
+/* value and derivatives of the monomial cf * prod_j x_idx[j] at the series x
+ * (series i at x + i*D numbers) into out (series r at out + r*D, row dim the
+ * value, row idx[j] the derivative to x_idx[j]; other rows untouched), as
+ * CPU_dbl2_evaldiff (Convolutions/dbl2_monomials_host.cpp:175-283): one
+ * variable: x cf; two: (x_a x_b) cf, cf x_a, cf x_b; more: the forward,
+ * backward and cross products of CPU_dbl2_speel (:28-87).  fw, bw, cr hold
+ * nvr series of D numbers each. */
+static void mono_evaldiff(int K, int dim, int nvr, int deg, int D, const int *ix, const double *cf,
+                          const double *x, double *out, double *fw, double *bw, double *cr) {
+  const size_t sD = (size_t)D * K;
+#define XS(v) (x + (size_t)(v) * sD)
+#define OS(r) (out + (size_t)(r) * sD)
+#define WS(a, r) ((a) + (size_t)(r) * sD)
+  int k, j;
+  if (nvr == 1) {
+    iprod(K, deg, XS(ix[0]), cf, OS(dim));
+    for (k = 0; k <= deg; k++) omd_copy(K, E(cf, k), E(OS(ix[0]), k));
+  } else if (nvr == 2) {
+    iprod(K, deg, XS(ix[0]), XS(ix[1]), fw);
+    iprod(K, deg, fw, cf, OS(dim));
+    iprod(K, deg, cf, XS(ix[0]), OS(ix[1]));
+    iprod(K, deg, cf, XS(ix[1]), OS(ix[0]));
+  } else if (nvr > 2) {
+    iprod(K, deg, cf, XS(ix[0]), WS(fw, 0));
+    for (j = 1; j < nvr; j++) iprod(K, deg, WS(fw, j - 1), XS(ix[j]), WS(fw, j));
+    iprod(K, deg, XS(ix[nvr - 1]), XS(ix[nvr - 2]), WS(bw, 0));
+    for (j = 1; j < nvr - 2; j++) iprod(K, deg, WS(bw, j - 1), XS(ix[nvr - 2 - j]), WS(bw, j));
+    iprod(K, deg, WS(bw, nvr - 3), cf, WS(cr, 0));
+    memcpy(WS(bw, nvr - 3), WS(cr, 0), sizeof(double) * sD);
+    if (nvr == 3) {
+      iprod(K, deg, WS(fw, 0), XS(ix[2]), WS(cr, 0));
+    } else {
+      for (j = 0; j < nvr - 3; j++) iprod(K, deg, WS(fw, j), WS(bw, nvr - 4 - j), WS(cr, j));
+      iprod(K, deg, WS(fw, nvr - 3), XS(ix[nvr - 1]), WS(cr, nvr - 3));
+    }
+    for (k = 0; k <= deg; k++) {
+      omd_copy(K, E(WS(fw, nvr - 1), k), E(OS(dim), k));
+      omd_copy(K, E(WS(fw, nvr - 2), k), E(OS(ix[nvr - 1]), k));
+      omd_copy(K, E(WS(bw, nvr - 3), k), E(OS(ix[0]), k));
+      for (j = 1; j < nvr - 1; j++) omd_copy(K, E(WS(cr, j - 1), k), E(OS(ix[j]), k));
+    }
+  }
+#undef XS
+#undef OS
+#undef WS
+}
+
+int onw_column_newton(int K, int dim, int nbrcol, int deg, int nbsteps, const int *nvr,
+                      const int *idxptr, const int *idx, const int *exps, const double *cff,
+                      const double *mb, double *input, int *wdeg_out, int *tail_out,
+                      int *up_out, int *bs_out, double *history, double *Q, double *R) {
+  const int D = deg + 1;
+  const int nmono = nbrcol * dim;
+  const size_t nn = (size_t)dim * dim, sD = (size_t)D * K;
+  double *x = malloc(sizeof(double) * K * dim * D);
+  double *jac = malloc(sizeof(double) * K * nn * D);
+  double *rhs = malloc(sizeof(double) * K * dim * D);
+  double *sol = malloc(sizeof(double) * K * dim * D);
+  double *q = calloc((size_t)K * nn, sizeof(double)), *r = calloc((size_t)K * nn, sizeof(double));
+  double *w = malloc(sizeof(double) * K * dim);
+  double *fval = malloc(sizeof(double) * K * dim * D);
+  double *out = malloc(sizeof(double) * K * (dim + 1) * D);
+  double *fw = malloc(sizeof(double) * K * dim * D), *bw = malloc(sizeof(double) * K * dim * D);
+  double *cr = malloc(sizeof(double) * K * dim * D);
+  double *cf = malloc(sizeof(double) * K * D), *acc = malloc(sizeof(double) * K * D);
+  double *ci = malloc(sizeof(double) * K * (size_t)(nmono > 0 ? nmono : 1) * D);
+  double *mbi = malloc(sizeof(double) * K * dim * D);
+  int tailidx = 1, wrkdeg = 0, zeroQ = 1, noqr = 0, upidx = 0, bsidx = 0, step, stepcnt = 0;
+  int i, j, k, c, e;
+
+  from_planar(K, (size_t)dim * D, input, x);
+  from_planar(K, (size_t)dim * D, mb, mbi);
+  if (nbrcol > 1) from_planar(K, (size_t)nmono * D, cff, ci);
+  for (step = 0; step < nbsteps; step++) {
+    const int Dw = wrkdeg + 1;
+    memset(jac, 0, sizeof(double) * K * nn * Dw);
+    memset(fval, 0, sizeof(double) * K * dim * D);
+    if (nbrcol == 1) {
+      /* CPU_dbl2_evaluate_monomials (Newton/dbl2_systems_host.cpp:12-111)
+       * on unit coefficients (dbl2_unit_series_vector each step), then
+       * dbl2_linearize_evaldiff_output (:443-523) */
+      for (i = 0; i < dim; i++) {
+        const int *ix = idx + idxptr[i];
+        const int *ex = exps ? exps + idxptr[i] : NULL;
+        memset(cf, 0, sizeof(double) * K * Dw);
+        cf[0] = 1.0;
+        for (j = 0; ex && j < nvr[i]; j++)
+          for (e = 1; e < ex[j]; e++) {
+            iprod(K, wrkdeg, x + (size_t)ix[j] * sD, cf, acc);
+            memcpy(cf, acc, sizeof(double) * K * Dw);
+          }
+        mono_evaldiff(K, dim, nvr[i], wrkdeg, D, ix, cf, x, out, fw, bw, cr);
+        for (j = 0; ex && j < nvr[i]; j++)
+          if (ex[j] > 1)
+            for (k = 0; k < Dw; k++) {
+              double *o = E(out + (size_t)ix[j] * sD, k);
+              omd_mul_d(K, o, (double)ex[j], o);
+            }
+        for (k = 0; k < Dw; k++) {
+          omd_copy(K, E(out + (size_t)dim * sD, k), E(fval, (size_t)i * D + k));
+          for (j = 0; j < nvr[i]; j++)
+            omd_copy(K, E(out + (size_t)ix[j] * sD, k),
+                     E(jac, (size_t)k * nn + (size_t)i * dim + ix[j]));
+        }
+      }
+    } else {
+      /* CPU_dbl2_evaluate_columns (Newton/dbl2_systems_host.cpp:232-329):
+       * -1 is the constant of the last equation */
+      fval[(size_t)(dim - 1) * D * K] = -1.0;
+      for (c = 0; c < nbrcol; c++)
+        for (j = 0; j < dim; j++) {
+          const int m = c * dim + j;
+          const int *ix = idx + idxptr[m];
+          int v;
+          if (nvr[m] <= 0) continue;
+          mono_evaldiff(K, dim, nvr[m], wrkdeg, D, ix, ci + (size_t)m * sD, x, out, fw, bw, cr);
+          for (k = 0; k < Dw; k++) {
+            double *fv = E(fval, (size_t)j * D + k);
+            omd_add(K, fv, E(out + (size_t)dim * sD, k), fv);
+          }
+          for (v = 0; v < nvr[m]; v++)
+            for (k = 0; k < Dw; k++) {
+              double *jd = E(jac, (size_t)k * nn + (size_t)j * dim + ix[v]);
+              omd_add(K, jd, E(out + (size_t)ix[v] * sD, k), jd);
+            }
+        }
+    }
+    /* rhs[k][i] = -(funval[i][k] - mb[i][k]) (dbl2_define_rhs, :626-659) */
+    for (k = 0; k < Dw; k++)
+      for (i = 0; i < dim; i++) {
+        double *dst = E(rhs, (size_t)k * dim + i);
+        omd_sub(K, E(fval, (size_t)i * D + k), E(mbi, (size_t)i * D + k), dst);
+        omd_neg(K, dst, dst);
+      }
+    memset(sol, 0, sizeof(double) * K * dim * Dw);
+    {
+      int newtail = tailidx;
+      qrbs_solve(K, dim, Dw, tailidx, jac, rhs, sol, q, r, w, &zeroQ, &noqr, &upidx, &bsidx,
+                 &newtail);
+      tailidx = newtail;
+    }
+    for (k = tailidx - 1; k < Dw; k++)
+      for (i = 0; i < dim; i++)
+        omd_add(K, E(x, (size_t)i * D + k), E(sol, (size_t)k * dim + i), E(x, (size_t)i * D + k));
+    if (wdeg_out) wdeg_out[step] = wrkdeg;
+    if (tail_out) tail_out[step] = tailidx;
+    if (up_out) up_out[step] = upidx;
+    if (bs_out) bs_out[step] = bsidx;
+    if (history) to_planar(K, (size_t)dim * D, x, history + (size_t)step * K * dim * D);
+    stepcnt++;
+    if (tailidx >= deg) break;
+    wrkdeg = wrkdeg + 1 + wrkdeg / 2;
+    if (wrkdeg > deg) wrkdeg = deg;
+  }
+  to_planar(K, (size_t)dim * D, x, input);
+  if (Q) to_planar(K, nn, q, Q);
+  if (R) to_planar(K, nn, r, R);
+  free(x); free(jac); free(rhs); free(sol); free(q); free(r); free(w); free(fval); free(out);
+  free(fw); free(bw); free(cr); free(cf); free(acc); free(ci); free(mbi);
   return stepcnt;
 }
 

@@ -49,6 +49,20 @@ int onw_row_newton(int K, int dim, int deg, int nbsteps, const int *monoptr, con
                    const double *cff, double *input, int *wdeg, int *tail, int *up, int *bs,
                    double *history, double *Q, double *R);
 
+/* staggered column Newton (the loop of test_dbl2_column_newton with
+ * dbl2_column_newton_qrstep): nbrcol columns of dim monomials, monomial
+ * m = c*dim + j (equation j of column c) has nvr[m] variables (0: none)
+ * idx[idxptr[m] ..].  nbrcol = 1 is a monomial system x^A_i = mb_i with unit
+ * coefficients and exponents exps parallel to idx (NULL: all ones); for
+ * nbrcol > 1 the coefficients are cff [K][nbrcol*dim][deg+1], exps are
+ * ignored and the last equation has the constant -1.  The right-hand side
+ * of the linearized system is -(f - mb), mb [K][dim][deg+1].  Other
+ * arguments as onw_row_newton. */
+int onw_column_newton(int K, int dim, int nbrcol, int deg, int nbsteps, const int *nvr,
+                      const int *idxptr, const int *idx, const int *exps, const double *cff,
+                      const double *mb, double *input, int *wdeg, int *tail, int *up, int *bs,
+                      double *history, double *Q, double *R);
+
 /* complex (K = 2, 4, 8), arrays complex limb-planar [2K][...] (K real
  * planes, K imaginary planes); resmax [K] of |re| + |im| per limb */
 void onw_cx_houseqr(int K, int nrows, int ncols, const double *A, double *Q, double *R);

@@ -38,6 +38,12 @@
 #include "cyclic_indices.h"
 #include "dbl2_indexed_coefficients.h"
 #include "dbl2_monomial_systems.h"
+#include "dbl_systems_host.h"
+#include "dbl2_systems_host.h"
+#include "dbl4_systems_host.h"
+#include "dbl8_systems_host.h"
+#include "unimodular_matrices.h"
+#include "cyclic_columns.h"
 
 namespace {
 
@@ -94,6 +100,25 @@ template <int K> struct Ref;
     static void speel(int nvr, int deg, int *idx, double *c[], double **in[], double **f[],   \
                       double **b[], double **x[]) {                                           \
       CPU_##PFX##_speel(nvr, deg, idx, LX(c), LX(in), LX(f), LX(b), LX(x));                   \
+    }                                                                                         \
+    static void monomials(int dim, int deg, int *nvr, int **idx, int **exp, int *nbrfac,      \
+                          int **expfac, double **c[], double *acc[], double **in[],           \
+                          double ***out[]) {                                                  \
+      CPU_##PFX##_evaluate_monomials(dim, deg, nvr, idx, exp, nbrfac, expfac, LX(c), LX(acc), \
+                                     LX(in), LX(out), 0);                                     \
+    }                                                                                         \
+    static void linearize(int dim, int degp1, int *nvr, int **idx, double **mb[],            \
+                          double ***out[], double **f[], double **rhs[], double ***jac[]) {   \
+      PFX##_linearize_evaldiff_output(dim, degp1, nvr, idx, LX(mb), 1.0, LX(out), LX(f),      \
+                                      LX(rhs), LX(jac), 0);                                   \
+    }                                                                                         \
+    static void columns(int dim, int deg, int nbrcol, int **nvr, int ***idx, double ***c[],   \
+                        double **acc[], double **in[], double **f[], double ***jac[]) {       \
+      CPU_##PFX##_evaluate_columns(dim, deg, nbrcol, nvr, idx, LX(c), LX(acc), LX(in), LX(f), \
+                                   LX(jac), 0);                                               \
+    }                                                                                         \
+    static void define_rhs(int dim, int degp1, double **mb[], double **f[], double **rhs[]) { \
+      PFX##_define_rhs(dim, degp1, LX(mb), LX(f), LX(rhs), 0);                                \
     }                                                                                         \
   };
 DEFREF(1, dbl, L1)
@@ -361,6 +386,110 @@ int row_newton_k(int dim, int deg, int nbsteps, const int *monoptr, const int *n
           urhs[((size_t)p * D + k) * dim + i] = rhs[((size_t)p * D + k) * dim + i];
           sol[((size_t)p * D + k) * dim + i] = 0.0;
         }
+    int newtail = tailidx;
+    Ref<K>::solve(dim, degp1, tailidx, pm, pb, px, pq, pr, pw, &zeroQ, &noqr, &up, &bs, &newtail);
+    tailidx = newtail;
+    for (int j = tailidx - 1; j < degp1; ++j)
+      for (int i = 0; i < dim; ++i) {
+        double *x[K];
+        const double *dx[K];
+        for (int p = 0; p < K; ++p) {
+          x[p] = &pin[p][i][j];
+          dx[p] = &px[p][j][i];
+        }
+        inc<K>(x, dx);
+      }
+    if (wdeg) wdeg[step] = wrkdeg;
+    if (tails) tails[step] = tailidx;
+    if (ups) ups[step] = up;
+    if (bss) bss[step] = bs;
+    if (history) std::memcpy(history + (size_t)step * K * dim * D, input, sizeof(double) * K * dim * D);
+    ++stepcnt;
+    if (tailidx >= deg) break;
+    wrkdeg = wrkdeg + 1 + wrkdeg / 2;
+    if (wrkdeg > deg) wrkdeg = deg;
+  }
+  if (Qo) std::memcpy(Qo, Q.data(), sizeof(double) * K * dim * dim);
+  if (Ro) std::memcpy(Ro, R.data(), sizeof(double) * K * dim * dim);
+  return stepcnt;
+}
+
+// The host branch of the staggered column Newton test
+// (test_dbl2_column_newton, Newton/dbl2_newton_method.cpp:1036-1287, with
+// dbl2_column_newton_qrstep :259-427 and dbl2_update_newton_qrstep
+// :134-202): a monomial system (nbrcol = 1) by CPU_*_evaluate_monomials on
+// unit coefficients and *_linearize_evaldiff_output, else the columns by
+// CPU_*_evaluate_columns and *_define_rhs; then CPU_*_qrbs_solve and the
+// update of the series.  nvr[c*dim+j], idx from idxptr[c*dim+j], exps
+// parallel to idx (nbrcol = 1 only; NULL: all ones), cff [K][nbrcol*dim][D]
+// (nbrcol > 1), mb [K][dim][D], input [K][dim][D] in/out.
+template <int K>
+int column_newton_k(int dim, int nbrcol, int deg, int nbsteps, const int *nvrflat,
+                    const int *idxptr, const int *idxflat, const int *expflat, double *cff,
+                    double *mbp, double *input, int *wdeg, int *tails, int *ups, int *bss,
+                    double *history, double *Qo, double *Ro) {
+  const int D = deg + 1, nmono = nbrcol * dim;
+  std::vector<int> nvc(nvrflat, nvrflat + nmono), idxc(idxflat, idxflat + idxptr[nmono]);
+  std::vector<int> expc(idxptr[nmono] > 0 ? idxptr[nmono] : 1, 1), facc(expc.size(), 0);
+  if (expflat) std::copy(expflat, expflat + idxptr[nmono], expc.begin());
+  for (size_t j = 0; j < expc.size(); ++j) facc[j] = expc[j] > 1 ? expc[j] - 1 : 0;
+  std::vector<int *> nvcol(nbrcol), ixm(nmono), exm(nmono), fcm(nmono);
+  std::vector<int **> ixcol(nbrcol);
+  std::vector<int> nbrfac(dim, 0);
+  for (int m = 0; m < nmono; ++m) {
+    ixm[m] = idxc.data() + idxptr[m];
+    exm[m] = expc.data() + idxptr[m];
+    fcm[m] = facc.data() + idxptr[m];
+  }
+  for (int c = 0; c < nbrcol; ++c) {
+    nvcol[c] = nvc.data() + (size_t)c * dim;
+    ixcol[c] = ixm.data() + (size_t)c * dim;
+  }
+  for (int i = 0; nbrcol == 1 && i < dim; ++i)
+    for (int j = 0; j < nvc[i]; ++j) nbrfac[i] += facc[idxptr[i] + j] > 0;
+  Mats in(input, K, 1, dim, D), mb(mbp, K, 1, dim, D);
+  std::vector<double> cf1((size_t)K * dim * D), accb((size_t)K * (dim + 1) * D);
+  Mats unit(cf1.data(), K, 1, dim, D), acc(accb.data(), K, 1, dim + 1, D);
+  std::vector<double> zc((size_t)K * (nmono > 0 ? nmono : 1) * D, 0.0);
+  Mats cc(cff ? cff : zc.data(), K, nbrcol, dim, D);
+  std::vector<double> out((size_t)K * dim * (dim + 1) * D), fv((size_t)K * dim * D);
+  Mats ou(out.data(), K, dim, dim + 1, D), fm(fv.data(), K, 1, dim, D);
+  std::vector<double> jac((size_t)K * D * dim * dim), rhs((size_t)K * D * dim),
+      sol((size_t)K * D * dim), Q((size_t)K * dim * dim), R((size_t)K * dim * dim),
+      wrk((size_t)K * dim);
+  Mats jm(jac.data(), K, D, dim, dim), ub(rhs.data(), K, 1, D, dim), sx(sol.data(), K, 1, D, dim);
+  Mats qm(Q.data(), K, 1, dim, dim), rm(R.data(), K, 1, dim, dim);
+  double ***pm[K], **pb[K], **px[K], **pq[K], **pr[K], *pw[K], **pin[K], **pmb[K], **pfv[K];
+  double **pun[K], *pacc[K], ***pou[K], **pac[K], ***pcc[K];
+  for (int p = 0; p < K; ++p) {
+    pm[p] = jm.ser(p); pb[p] = ub.mat(p); px[p] = sx.mat(p); pq[p] = qm.mat(p);
+    pr[p] = rm.mat(p); pw[p] = wrk.data() + (size_t)p * dim; pin[p] = in.mat(p);
+    pmb[p] = mb.mat(p); pfv[p] = fm.mat(p); pun[p] = unit.mat(p); pacc[p] = acc.mat(p)[0];
+    pou[p] = ou.ser(p); pac[p] = acc.mat(p); pcc[p] = cc.ser(p);
+  }
+  bool zeroQ = true, noqr = false;
+  int tailidx = 1, wrkdeg = 0, up = 0, bs = 0, stepcnt = 0;
+  for (int step = 0; step < nbsteps; ++step) {
+    const int degp1 = wrkdeg + 1;
+    if (nbrcol == 1) {
+      // dbl2_unit_series_vector (Newton/dbl2_newton_testers.cpp:66-80)
+      for (int p = 0; p < K; ++p)
+        for (int i = 0; i < dim; ++i)
+          for (int k = 0; k <= wrkdeg; ++k) pun[p][i][k] = (p == 0 && k == 0) ? 1.0 : 0.0;
+      Ref<K>::monomials(dim, wrkdeg, nvc.data(), ixm.data(), exm.data(), nbrfac.data(),
+                        fcm.data(), pun, pacc, pin, pou);
+      for (int p = 0; p < K; ++p)
+        for (int k = 0; k < degp1; ++k)
+          for (int i = 0; i < dim; ++i)
+            for (int j = 0; j < dim; ++j) pm[p][k][i][j] = 0.0;
+      Ref<K>::linearize(dim, degp1, nvc.data(), ixm.data(), pmb, pou, pfv, pb, pm);
+    } else {
+      Ref<K>::columns(dim, wrkdeg, nbrcol, nvcol.data(), ixcol.data(), pcc, pac, pin, pfv, pm);
+      Ref<K>::define_rhs(dim, degp1, pmb, pfv, pb);
+    }
+    for (int p = 0; p < K; ++p)
+      for (int k = 0; k < degp1; ++k)
+        for (int i = 0; i < dim; ++i) px[p][k][i] = 0.0;
     int newtail = tailidx;
     Ref<K>::solve(dim, degp1, tailidx, pm, pb, px, pq, pr, pw, &zeroQ, &noqr, &up, &bs, &newtail);
     tailidx = newtail;
@@ -685,6 +814,120 @@ int ref_dbl2_cyclic_row_system(unsigned seed, int dim, int deg, int *monoptr, in
       }
   }
   return nmono;
+}
+
+int ref_column_newton(int K, int dim, int nbrcol, int deg, int nbsteps, const int *nvr,
+                      const int *idxptr, const int *idx, const int *exps, double *cff,
+                      double *mb, double *input, int *wdeg, int *tail, int *up, int *bs,
+                      double *history, double *Q, double *R) {
+  switch (K) {
+#define CASE(KK)                                                                              \
+    case KK:                                                                                  \
+      return column_newton_k<KK>(dim, nbrcol, deg, nbsteps, nvr, idxptr, idx, exps, cff, mb,  \
+                                 input, wdeg, tail, up, bs, history, Q, R);
+    CASE(1) CASE(2) CASE(4) CASE(8)
+#undef CASE
+  }
+  return -1;
+}
+
+// The reference's monomial test system (make_monomial_system,
+// Newton/unimodular_matrices.cpp:118-186, as dbl2_test_columns calls it,
+// Newton/test_dbl2_newton.cpp:108-118, positive values): after srand(seed)
+// the exponent matrix rowsA [dim][dim] (nbritr > 0: that many random
+// unimodular row operations with multipliers below size; -1: lower
+// triangular ones; -2: two-variable monomials), and per row its nonzero
+// entries as nvr[dim], idx and exp [dim*dim] (row i from i*dim).
+int ref_monomial_system(unsigned seed, int dim, int size, int nbritr, int *rowsA, int *nvr,
+                        int *idx, int *exp) {
+  std::vector<int *> A(dim), ix(dim), ex(dim), ef(dim);
+  std::vector<int> nv(dim), nf(dim);
+  srand(seed);
+  make_monomial_system(dim, size, 1, nbritr, nv.data(), ix.data(), ex.data(), nf.data(),
+                       ef.data(), A.data(), 0);
+  for (int i = 0; i < dim; ++i) {
+    nvr[i] = nv[i];
+    for (int j = 0; j < dim; ++j) rowsA[(size_t)i * dim + j] = A[i][j];
+    for (int j = 0; j < nv[i]; ++j) {
+      idx[(size_t)i * dim + j] = ix[i][j];
+      exp[(size_t)i * dim + j] = ex[i][j];
+    }
+  }
+  return 0;
+}
+
+// The reference's double-double column setup (dbl2_column_setup,
+// Newton/dbl2_newton_method.cpp:766-821, mode 1, after srand(seed)):
+// the exponential test solution (make_real2_exponentials), for nbrcol > 1
+// the random coefficients of make_real2_coefficients (drawn first, as
+// test_dbl2_column_newton :1057-1058 does) on the cyclic columns
+// (make_cyclic_variables/_columns, Newton/cyclic_columns.cpp:10-23), the
+// right-hand side mb as the system at the test solution
+// (evaluate_real2_monomials / evaluate_real2_columns) and the start series
+// (real2_start_series_vector).  nbrcol = 1 takes rowsA [dim][dim]; nbrcol = 2
+// takes, nbrcol > 2 writes (the cyclic columns) nvr [nbrcol*dim] and idx
+// [nbrcol*dim*dim] (monomial m from m*dim).
+// cff [2][nbrcol*dim][D], testsol, mb, start [2][dim][D].
+int ref_dbl2_column_setup(unsigned seed, int dim, int deg, int nbrcol, const int *rowsA,
+                          int *nvr, int *idx, double *cff, double *testsol, double *mb,
+                          double *start) {
+  const int D = deg + 1;
+  std::vector<int *> A(dim);
+  std::vector<int> Ab((size_t)dim * dim, 0);
+  for (int i = 0; i < dim; ++i) A[i] = Ab.data() + (size_t)i * dim;
+  if (rowsA) std::copy(rowsA, rowsA + (size_t)dim * dim, Ab.begin());
+  std::vector<int> nvb((size_t)nbrcol * dim, 0), ixb((size_t)nbrcol * dim * dim, 0);
+  std::vector<int *> nvc(nbrcol);
+  std::vector<int **> ixc(nbrcol);
+  std::vector<int *> ixm((size_t)nbrcol * dim);
+  for (int c = 0; c < nbrcol; ++c) {
+    nvc[c] = nvb.data() + (size_t)c * dim;
+    ixc[c] = ixm.data() + (size_t)c * dim;
+    for (int j = 0; j < dim; ++j) ixm[(size_t)c * dim + j] = ixb.data() + ((size_t)c * dim + j) * dim;
+  }
+  srand(seed);
+  std::vector<double> cb((size_t)2 * nbrcol * dim * D, 0.0);
+  Mats cm(cb.data(), 2, nbrcol, dim, D);
+  if (nbrcol != 1) make_real2_coefficients(nbrcol, dim, cm.ser(0), cm.ser(1));
+  if (nbrcol == 2) {  // the caller's two columns
+    std::copy(nvr, nvr + nvb.size(), nvb.begin());
+    std::copy(idx, idx + ixb.size(), ixb.begin());
+  } else if (nbrcol > 2) {
+    make_cyclic_variables(nbrcol, dim, nvc.data());
+    make_cyclic_columns(nbrcol, dim, nvc.data(), ixc.data());
+  }
+  std::vector<double *> shi(dim), slo(dim), mhi(dim), mlo(dim);
+  std::vector<double> sb((size_t)2 * dim * D), mbb((size_t)2 * dim * D);
+  for (int i = 0; i < dim; ++i) {
+    shi[i] = sb.data() + (size_t)i * D; slo[i] = sb.data() + ((size_t)dim + i) * D;
+    mhi[i] = mbb.data() + (size_t)i * D; mlo[i] = mbb.data() + ((size_t)dim + i) * D;
+  }
+  make_real2_exponentials(dim, deg, shi.data(), slo.data());
+  for (int i = 0; i < dim; ++i) {
+    mhi[i][0] = 1.0; mlo[i][0] = 0.0;
+    for (int k = 1; k < D; ++k) { mhi[i][k] = 0.0; mlo[i][k] = 0.0; }
+  }
+  if (nbrcol == 1)
+    evaluate_real2_monomials(dim, deg, A.data(), shi.data(), slo.data(), mhi.data(), mlo.data());
+  else
+    evaluate_real2_columns(dim, deg, nbrcol, nvc.data(), ixc.data(), A.data(), cm.ser(0),
+                           cm.ser(1), shi.data(), slo.data(), mhi.data(), mlo.data(), 0);
+  std::copy(sb.begin(), sb.end(), testsol);
+  std::copy(mbb.begin(), mbb.end(), mb);
+  if (cff) std::copy(cb.begin(), cb.end(), cff);
+  for (size_t t = 0; t < sb.size(); ++t) start[t] = 0.0;
+  for (int i = 0; i < dim; ++i) {
+    // real2_start_series_vector (Newton/dbl2_newton_testers.cpp:25-40)
+    double hi = shi[i][0], lo = slo[i][0];
+    ddf_inc(&hi, &lo, 1.0e-16, 0.0);
+    start[(size_t)i * D] = hi;
+    start[((size_t)dim + i) * D] = lo;
+  }
+  if (nbrcol > 2) {
+    for (size_t t = 0; t < nvb.size(); ++t) nvr[t] = nvb[t];
+    for (size_t t = 0; t < ixb.size(); ++t) idx[t] = ixb[t];
+  }
+  return 0;
 }
 
 }  // extern "C"

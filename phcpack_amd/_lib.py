@@ -125,6 +125,10 @@ SIGNATURES.update({
     "phc_rowsys_create_exp": (c_int, [ctypes.POINTER(c_void_p), c_int, P_int, P_int, P_int, P_int,
                                       P_int]),
     "phc_rowsys_destroy": (c_int, [c_void_p]),
+    "phc_colsys_create": (c_int, [ctypes.POINTER(c_void_p), c_int, c_int, P_int, P_int, P_int]),
+    "phc_column_newton": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int,
+                                  c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "phc_rowsys_lds_bytes": (c_int, [c_void_p, c_int, c_int, ctypes.POINTER(ctypes.c_size_t)]),
     "phc_row_newton": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p,
                                c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

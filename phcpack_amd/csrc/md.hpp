@@ -14,7 +14,9 @@
 //
 // two_prod uses a fused multiply-add.  The reference host code uses Dekker's
 // split; both return the exact product error for operands whose product
-// neither overflows nor underflows, so results are identical there.
+// neither overflows nor underflows, so results are identical there.  Below
+// |a b| = 2^-916 the error can fall into the subnormal range, where the two
+// round it differently (trailing limbs near 1e-140 and below then differ).
 // All code including this header must be compiled with -ffp-contract=off so
 // that the plain products of the reference (e.g. "p2 += a_hi*b_lo + a_lo*b_hi")
 // keep their two roundings.

@@ -541,6 +541,21 @@ struct RowSys {
   int dim, nmono, nvmax, nbrmax;
   const int *monoptr, *nvr, *idxptr, *idx;
   const int *exp;  // exponents parallel to idx, nullptr when all are 1
+  // A system in the column representation (the column Newton method),
+  // equation j holding monomial j of every column in column order: each
+  // monomial is evaluated as CPU_dbl2_evaldiff does (a two-variable
+  // monomial's value as (x_a x_b) c), the coefficient of row monomial m is
+  // coefficient series cmap[m] of the caller's ncff per system (unit series
+  // when the coefficients are null: a monomial system), the constant is -1
+  // in the last equation when negone and 0 elsewhere, and the right-hand
+  // side of the linearized system is -(f - mb).
+  int column, negone, ncff;
+  const int *cmap;
+  // a monomial system (one monomial per equation): its value and
+  // derivatives are stored, not added to zero, as
+  // dbl2_linearize_evaldiff_output copies them (Newton/dbl2_systems_host.cpp:
+  // 455-459, 506-513); an addition to zero renormalizes a quad double
+  int assign;
 };
 
 // v * e, the derivative of a monomial times its exponent, with the
@@ -591,7 +606,15 @@ __device__ void eval_system(const RowSys &S, int D, int Dw, int Dp, const double
     const int p = r0 + s;
     const bool on = (p < dim) && (k < Dw);
     if (on) {
-      OUT.st((p * D1 + dim) * D + k, N::ld(cst, cstride, (size_t)p * D + k));
+      if constexpr (CX) {
+        OUT.st((p * D1 + dim) * D + k, N::ld(cst, cstride, (size_t)p * D + k));
+      } else {
+        // column systems (cst null): 0, or -1 in the last equation
+        T c0 = N::zero();
+        if (cst) c0 = N::ld(cst, cstride, (size_t)p * D + k);
+        else if (S.negone && p == dim - 1 && k == 0) c0 = N::neg(N::one());
+        OUT.st((p * D1 + dim) * D + k, c0);
+      }
       for (int j = 0; j < dim; ++j) OUT.st((p * D1 + j) * D + k, N::zero());
     }
     const int nb = (p < dim) ? S.monoptr[p + 1] - S.monoptr[p] : 0;
@@ -609,10 +632,16 @@ __device__ void eval_system(const RowSys &S, int D, int Dw, int Dp, const double
           OUT.st(o, N::add(OUT.ld(o), v));
         } else {
           const int e = (ex && j >= 0) ? ex[j] : 1;
-          OUT.st(o, add(OUT.ld(o), e > 1 ? times_exp(v, (double)e) : v));
+          const T ve = e > 1 ? times_exp(v, (double)e) : v;
+          OUT.st(o, S.assign ? ve : add(OUT.ld(o), ve));
         }
       };
-      if (act) CF.st(co + k, N::ld(cff, fstride, (size_t)mono * D + k));
+      if constexpr (CX) {
+        if (act) CF.st(co + k, N::ld(cff, fstride, (size_t)mono * D + k));
+      } else if (act) {
+        const size_t cm = S.cmap ? (size_t)S.cmap[mono] : (size_t)mono;
+        CF.st(co + k, cff ? N::ld(cff, fstride, cm * D + k) : (k == 0 ? N::one() : N::zero()));
+      }
       wsync();
       if (ex) {
         for (int j = 0; j < nv; ++j)
@@ -628,14 +657,19 @@ __device__ void eval_system(const RowSys &S, int D, int Dw, int Dp, const double
         addto(dim, f, -1);
         addto(ix[0], cf(k), 0);
       } else if (nv == 2) {
+        // the value: (c x_a) x_b in a polynomial, (x_a x_b) c in a column
+        // system (real only)
+        bool col = false;
+        if constexpr (!CX) col = S.column != 0;
         const T f0 = cprod<K, CX>(cf, xin(ix[0]), k);
-        FW.st(eo + k, f0);
+        if (col) FW.st(eo + k, cprod<K, CX>(xin(ix[0]), xin(ix[1]), k));
+        else FW.st(eo + k, f0);
         addto(ix[1], f0, 1);
         const T b0 = cprod<K, CX>(cf, xin(ix[1]), k);
         addto(ix[0], b0, 0);
         wsync();
-        const T f1 = cprod<K, CX>(fw(0), xin(ix[1]), k);
-        addto(dim, f1, -1);
+        if (col) addto(dim, cprod<K, CX>(fw(0), cf, k), -1);
+        else addto(dim, cprod<K, CX>(fw(0), xin(ix[1]), k), -1);
       } else if (nv > 2) {
         // forward and backward chains advance together
         FW.st(eo + k, cprod<K, CX>(cf, xin(ix[0]), k));
@@ -702,14 +736,17 @@ __device__ LinWork<K, CX> carve_lin(double *&p, int n) {
   return w;
 }
 
-// x [K][nsys][dim][D] in/out; cst [K][nc][dim][D], cff [K][nf][nmono][D] with
+// x [K][nsys][dim][D] in/out; cst [K][nc][dim][D], cff [K][nf][ncff][D] with
 // nc, nf = nsys or 1 (shared); logs [nsys][nbsteps]; hist
-// [nbsteps][K][nsys][dim][D]; Q, R [K][nsys][dim][dim].
+// [nbsteps][K][nsys][dim][D]; Q, R [K][nsys][dim][dim].  Column systems:
+// cst null, cff null for unit coefficients, mb [K][nm][dim][D] (nm = nsys
+// or 1); rows: mb null.
 template <int K, bool CX>
 __global__ void __launch_bounds__(512)
 row_newton_kernel(RowSys S, int deg, int nbsteps, int nsys, const double *cst, int cshared,
-                  const double *cff, int fshared, double *x, int *steps_out, int *wdeg_out,
-                  int *tail_out, int *up_out, int *bs_out, double *hist, double *Qo, double *Ro) {
+                  const double *cff, int fshared, const double *mb, int mshared, double *x,
+                  int *steps_out, int *wdeg_out, int *tail_out, int *up_out, int *bs_out,
+                  double *hist, double *Qo, double *Ro) {
   typedef NT<K, CX> N;
   extern __shared__ double lds[];
   const int D = deg + 1, n = S.dim, Dp = pow2_at_least(D);
@@ -725,9 +762,10 @@ row_newton_kernel(RowSys S, int deg, int nbsteps, int nsys, const double *cst, i
                    CR = carve<K, CX>(p, L.CR), CF = carve<K, CX>(p, L.CF);
   const size_t xs = (size_t)nsys * n * D, xb = (size_t)sysid * n * D;
   const size_t cs = (size_t)(cshared ? 1 : nsys) * n * D;
-  const size_t fs = (size_t)(fshared ? 1 : nsys) * S.nmono * D;
-  const double *cbase = cst + (cshared ? 0 : (size_t)sysid * n * D);
-  const double *fbase = cff + (fshared ? 0 : (size_t)sysid * S.nmono * D);
+  const size_t fs = (size_t)(fshared ? 1 : nsys) * S.ncff * D;
+  const size_t ms = (size_t)(mshared ? 1 : nsys) * n * D, mo = mshared ? 0 : (size_t)sysid * n * D;
+  const double *cbase = cst ? cst + (cshared ? 0 : (size_t)sysid * n * D) : nullptr;
+  const double *fbase = cff ? cff + (fshared ? 0 : (size_t)sysid * S.ncff * D) : nullptr;
   for (int t = l; t < n * D; t += 64) X.st(t, N::ld(x, xs, xb + t));
   wsync();
   const int D1 = n + 1;
@@ -739,7 +777,13 @@ row_newton_kernel(RowSys S, int deg, int nbsteps, int nsys, const double *cst, i
     eval_system<K, CX>(S, D, Dw, Dp, cbase, cs, fbase, fs, X, OUT, FW, BW, CR, CF);
     for (int t = l; t < Dw * n; t += 64) {
       const int kk = t / n, i = t % n;
-      RHS.st(t, N::neg(OUT.ld((i * D1 + n) * D + kk)));
+      const typename N::V fv = OUT.ld((i * D1 + n) * D + kk);
+      if constexpr (CX) {
+        RHS.st(t, N::neg(fv));
+      } else {
+        // -(f - mb) as dbl2_define_rhs (Newton/dbl2_systems_host.cpp:626-659)
+        RHS.st(t, mb ? N::neg(md::sub(fv, N::ld(mb, ms, mo + (size_t)i * D + kk))) : N::neg(fv));
+      }
       SOL.st(t, N::zero());
     }
     wsync();
@@ -905,12 +949,15 @@ int waves_for(size_t per_wave, int *waves) {
 struct phc_rowsys {
   RowSys S;
   int *dmem;
+  int *cmem;   // column systems: the coefficient map
+  int nbrcol;  // column systems: the number of columns (0 for row systems)
 };
 
 extern "C" {
 
 int phc_rowsys_create_exp(phc_rowsys **out, int dim, const int *monoptr, const int *nvr,
                           const int *idxptr, const int *idx, const int *exp);
+int phc_rowsys_destroy(phc_rowsys *s);
 
 int phc_rowsys_create(phc_rowsys **out, int dim, const int *monoptr, const int *nvr,
                       const int *idxptr, const int *idx) {
@@ -961,6 +1008,66 @@ int phc_rowsys_create_exp(phc_rowsys **out, int dim, const int *monoptr, const i
   s->S.monoptr = p; s->S.nvr = p + dim + 1; s->S.idxptr = p + dim + 1 + nmono;
   s->S.idx = p + dim + 1 + 2 * nmono + 1;
   s->S.exp = any_exp ? s->S.idx + nidx : nullptr;
+  s->S.ncff = nmono;
+  *out = s;
+  return 0;
+}
+
+int phc_colsys_create(phc_rowsys **out, int dim, int nbrcol, const int *nvr, const int *idx,
+                      const int *exps) {
+  PHC_REQUIRE(out && nvr, "phc_colsys_create: null argument");
+  PHC_REQUIRE(dim >= 1 && dim <= 64, "phc_colsys_create: dim must be in 1..64");
+  PHC_REQUIRE(nbrcol >= 1 && nbrcol <= 64, "phc_colsys_create: nbrcol must be in 1..64");
+  PHC_REQUIRE(nbrcol == 1 || exps == nullptr,
+              "phc_colsys_create: exponents are for monomial systems (nbrcol = 1) only");
+  const int ncff = nbrcol * dim;
+  std::vector<int> iptr(ncff + 1, 0);
+  for (int m = 0; m < ncff; ++m) {
+    PHC_REQUIRE(nvr[m] >= (nbrcol == 1 ? 1 : 0) && nvr[m] <= dim,
+                "phc_colsys_create: monomial variable count out of range");
+    iptr[m + 1] = iptr[m] + nvr[m];
+  }
+  // equation j: monomial j of every column in column order (the order in
+  // which CPU_dbl2_evaluate_columns adds them, Newton/dbl2_systems_host.cpp:
+  // 291-323), empty monomials left out
+  std::vector<int> monoptr(dim + 1, 0), rnvr, ridxptr(1, 0), ridx, rexp, cmap;
+  for (int j = 0; j < dim; ++j) {
+    for (int c = 0; c < nbrcol; ++c) {
+      const int m = c * dim + j;
+      if (nvr[m] == 0) continue;
+      rnvr.push_back(nvr[m]);
+      cmap.push_back(m);
+      for (int k = iptr[m]; k < iptr[m + 1]; ++k) {
+        PHC_REQUIRE(idx, "phc_colsys_create: null index array");
+        ridx.push_back(idx[k]);
+        if (exps) rexp.push_back(exps[k]);
+      }
+      ridxptr.push_back((int)ridx.size());
+    }
+    monoptr[j + 1] = (int)rnvr.size();
+  }
+  for (size_t m = 0; m < rnvr.size(); ++m)
+    for (int a = ridxptr[m]; a < ridxptr[m + 1]; ++a)
+      for (int b = a + 1; b < ridxptr[m + 1]; ++b)
+        PHC_REQUIRE(ridx[a] != ridx[b], "phc_colsys_create: repeated variable in a monomial");
+  phc_rowsys *s = nullptr;
+  const int rc = phc_rowsys_create_exp(&s, dim, monoptr.data(), rnvr.data(), ridxptr.data(),
+                                       ridx.data(), exps ? rexp.data() : nullptr);
+  if (rc != 0) return rc;
+  int *dmap = nullptr;
+  if (!cmap.empty() && hipMalloc(&dmap, sizeof(int) * cmap.size()) != hipSuccess) {
+    phc_rowsys_destroy(s);
+    phc::set_error("phc_colsys_create: hipMalloc failed");
+    return -1;
+  }
+  if (dmap) PHC_HIP(hipMemcpy(dmap, cmap.data(), sizeof(int) * cmap.size(), hipMemcpyHostToDevice));
+  s->cmem = dmap;
+  s->nbrcol = nbrcol;
+  s->S.column = 1;
+  s->S.negone = nbrcol > 1;
+  s->S.assign = nbrcol == 1;
+  s->S.ncff = ncff;
+  s->S.cmap = dmap;
   *out = s;
   return 0;
 }
@@ -968,6 +1075,10 @@ int phc_rowsys_create_exp(phc_rowsys **out, int dim, const int *monoptr, const i
 int phc_rowsys_destroy(phc_rowsys *s) {
   if (!s) return 0;
   hipError_t e = hipFree(s->dmem);
+  if (s->cmem) {
+    const hipError_t e2 = hipFree(s->cmem);
+    if (e == hipSuccess) e = e2;
+  }
   delete s;
   if (e != hipSuccess) {
     phc::set_error("phc_rowsys_destroy: hipFree failed");
@@ -990,10 +1101,13 @@ namespace {
 // the staggered row Newton, real (P = K planes) or complex (2K)
 template <bool CX>
 int row_newton(phc_rowsys *s, int K, int deg, int nbsteps, int nsys, const double *cst,
-               int cst_shared, const double *cff, int cff_shared, double *x, int *steps,
-               int *wdeg, int *tail, int *up, int *bs, double *history, double *Q, double *R,
-               void *stream, const char *fn) {
+               int cst_shared, const double *cff, int cff_shared, const double *mb,
+               int mb_shared, double *x, int *steps, int *wdeg, int *tail, int *up, int *bs,
+               double *history, double *Q, double *R, void *stream, const char *fn) {
   PHC_REQUIRE(s, std::string(fn) + ": null system");
+  const bool column = s->S.column != 0;
+  PHC_REQUIRE(!column || mb, std::string(fn) + ": a column system goes to phc_column_newton "
+                                               "with its right-hand side");
   if (CX) {
     PHC_REQUIRE(K == 2 || K == 4, std::string(fn) + ": K must be 2 or 4");
     PHC_REQUIRE(s->S.exp == nullptr, std::string(fn) + ": exponents above 1 are real only");
@@ -1002,7 +1116,13 @@ int row_newton(phc_rowsys *s, int K, int deg, int nbsteps, int nsys, const doubl
   }
   PHC_REQUIRE(deg >= 0 && deg <= 63, std::string(fn) + ": deg must be in 0..63");
   PHC_REQUIRE(nbsteps >= 0 && nsys >= 0, std::string(fn) + ": negative size");
-  PHC_REQUIRE(cst && cff && x, std::string(fn) + ": null coefficient or series array");
+  if (column) {
+    PHC_REQUIRE(!CX, std::string(fn) + ": column systems are real");
+    PHC_REQUIRE(x && (cff || s->nbrcol == 1),
+                std::string(fn) + ": null coefficient or series array");
+  } else {
+    PHC_REQUIRE(cst && cff && x, std::string(fn) + ": null coefficient or series array");
+  }
   if (nsys == 0 || nbsteps == 0) return 0;
   const size_t per = newton_layout(s->S, deg + 1).bytes(CX ? 2 * K : K);
   int wv = 0;
@@ -1015,8 +1135,8 @@ int row_newton(phc_rowsys *s, int K, int deg, int nbsteps, int nsys, const doubl
     PHC_HIP(hipFuncSetAttribute((const void *)row_newton_kernel<KK, CX>,                     \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));       \
     hipLaunchKernelGGL((row_newton_kernel<KK, CX>), dim3(grid), dim3(64 * wv), lds, st, s->S, \
-                       deg, nbsteps, nsys, cst, cst_shared, cff, cff_shared, x, steps, wdeg,  \
-                       tail, up, bs, history, Q, R);                                          \
+                       deg, nbsteps, nsys, cst, cst_shared, cff, cff_shared, mb, mb_shared, \
+                       x, steps, wdeg, tail, up, bs, history, Q, R);                          \
   } while (0)
   if (K == 1 && !CX) LAUNCH(1);
   else if (K == 2) LAUNCH(2);
@@ -1126,16 +1246,31 @@ int phc_row_newton(phc_rowsys *s, int K, int deg, int nbsteps, int nsys, const d
                    int cst_shared, const double *cff, int cff_shared, double *x, int *steps,
                    int *wdeg, int *tail, int *up, int *bs, double *history, double *Q, double *R,
                    void *stream) {
-  return row_newton<false>(s, K, deg, nbsteps, nsys, cst, cst_shared, cff, cff_shared, x, steps,
-                           wdeg, tail, up, bs, history, Q, R, stream, "phc_row_newton");
+  return row_newton<false>(s, K, deg, nbsteps, nsys, cst, cst_shared, cff, cff_shared, nullptr,
+                           0, x, steps, wdeg, tail, up, bs, history, Q, R, stream,
+                           "phc_row_newton");
+}
+
+int phc_column_newton(phc_rowsys *s, int K, int deg, int nbsteps, int nsys, const double *cff,
+                      int cff_shared, const double *mb, int mb_shared, double *x, int *steps,
+                      int *wdeg, int *tail, int *up, int *bs, double *history, double *Q,
+                      double *R, void *stream) {
+  PHC_REQUIRE(s && s->S.column, "phc_column_newton: not a column system (phc_colsys_create)");
+  PHC_REQUIRE(mb, "phc_column_newton: null right-hand side");
+  // a monomial system's coefficients are the unit series
+  // (dbl2_unit_series_vector at each step, Newton/dbl2_newton_method.cpp:292)
+  return row_newton<false>(s, K, deg, nbsteps, nsys, nullptr, 0, s->nbrcol == 1 ? nullptr : cff,
+                           cff_shared, mb, mb_shared, x, steps, wdeg, tail, up, bs, history, Q, R,
+                           stream, "phc_column_newton");
 }
 
 int phc_cmplx_row_newton(phc_rowsys *s, int K, int deg, int nbsteps, int nsys, const double *cst,
                          int cst_shared, const double *cff, int cff_shared, double *x,
                          int *steps, int *wdeg, int *tail, int *up, int *bs, double *history,
                          double *Q, double *R, void *stream) {
-  return row_newton<true>(s, K, deg, nbsteps, nsys, cst, cst_shared, cff, cff_shared, x, steps,
-                          wdeg, tail, up, bs, history, Q, R, stream, "phc_cmplx_row_newton");
+  return row_newton<true>(s, K, deg, nbsteps, nsys, cst, cst_shared, cff, cff_shared, nullptr, 0,
+                          x, steps, wdeg, tail, up, bs, history, Q, R, stream,
+                          "phc_cmplx_row_newton");
 }
 
 int phc_houseqr_batch(int K, int n, int nbatch, const double *A, double *Q, double *R,

@@ -9,6 +9,10 @@ Mirrors the reference's host interface for this path:
                  iterated with staggered degrees as test_dbl2_row_newton
                  (:1447-1494); exponents > 1 by the common-factor trick of
                  CPU_dbl2_evaluate_monomials (Newton/dbl2_systems_host.cpp:12-111)
+  column_newton  dbl2_column_newton_qrstep (Newton/dbl2_newton_method.cpp:259)
+                 iterated as test_dbl2_column_newton (:1036-1287): monomial
+                 systems (CPU_dbl2_evaluate_monomials) and column systems
+                 (CPU_dbl2_evaluate_columns, :232-329)
 for K = 1, 2, 4 limbs (dbl, dbl2, dbl4), batched: every array carries a batch
 index after the limb index.  With cplx=True the complex twins
 (CPU_cmplx2_factors_houseqr, CPU_cmplx2_qrbs_solve, CPU_cmplx2_linear_residue,
@@ -136,6 +140,82 @@ class RowSystem:
             logs["history"] = DeviceArray.zeros((nbsteps, P, nsys, dim, D))
         self.row_newton_device(K, D - 1, nbsteps, nsys, dc, cst.shape[1] == 1 and nsys > 1, df,
                                cff.shape[1] == 1 and nsys > 1, dx, logs, cplx=cplx)
+        check(load().phc_synchronize())
+        out = {k: v.to_host() for k, v in logs.items()}
+        out["x"] = dx.to_host()
+        return out
+
+
+class ColumnSystem(RowSystem):
+    """A system in the column representation on the device (phc_colsys_create):
+    nbrcol columns of dim monomials, monomial m = c*dim + j (equation j of
+    column c) with nvr[m] variables (0: none), their indices idx
+    concatenated in the order of m.  nbrcol = 1 is the monomial system
+    x^A_i = mb_i (unit coefficients, exps parallel to idx); nbrcol > 1 has a
+    coefficient series per monomial and the constant -1 in the last
+    equation (CPU_dbl2_evaluate_columns, Newton/dbl2_systems_host.cpp:232)."""
+
+    def __init__(self, dim, nbrcol, nvr, idx, exps=None):
+        self.dim, self.nbrcol = dim, nbrcol
+        self.nvr, self.idx = _ip(nvr), _ip(idx if len(idx) else [0])
+        self.exp = None if exps is None else _ip(exps if len(exps) else [1])
+        h = ctypes.c_void_p()
+        check(load().phc_colsys_create(
+            ctypes.byref(h), dim, nbrcol, self.nvr.ctypes.data_as(P_int),
+            self.idx.ctypes.data_as(P_int),
+            None if self.exp is None else self.exp.ctypes.data_as(P_int)))
+        self.handle = h
+
+    @classmethod
+    def monomials(cls, rowsA):
+        """the monomial system of the exponent matrix rowsA (row i: the
+        nonzero entries in increasing column order, as make_monomial_system
+        lists them, Newton/unimodular_matrices.cpp:150-164)"""
+        A = np.asarray(rowsA, np.int64)
+        if A.ndim != 2 or A.shape[0] != A.shape[1] or (A < 0).any():
+            raise ValueError("rowsA must be a square matrix of nonnegative exponents")
+        nvr = (A != 0).sum(axis=1)
+        idx = np.concatenate([np.nonzero(r)[0] for r in A])
+        exps = np.concatenate([r[r != 0] for r in A])
+        return cls(A.shape[0], 1, nvr, idx, exps)
+
+    def row_newton_device(self, *args, **kw):
+        raise TypeError("a column system runs column_newton_device")
+
+    def column_newton_device(self, K, deg, nbsteps, nsys, d_cff, cff_shared, d_mb, mb_shared, d_x,
+                             logs=None, stream=None):
+        """device-resident run (d_cff None for a monomial system); logs as
+        RowSystem.row_newton_device"""
+        g = (lambda k: logs[k].ptr if logs and k in logs else None)
+        check(load().phc_column_newton(self.handle, K, deg, nbsteps, nsys,
+                                       None if d_cff is None else d_cff.ptr, int(cff_shared),
+                                       d_mb.ptr, int(mb_shared), d_x.ptr, g("steps"), g("wdeg"),
+                                       g("tail"), g("up"), g("bs"), g("history"), g("Q"), g("R"),
+                                       stream))
+
+    def column_newton(self, K, deg, nbsteps, mb, x, cff=None, history=False):
+        """Staggered column Newton on a batch of systems with this structure.
+
+        mb [K][nm][dim][deg+1] (nm = 1: shared), cff [K][nf][nbrcol*dim][deg+1]
+        (nf = 1: shared; None for a monomial system), x [K][nsys][dim][deg+1].
+        Returns dict(x, steps, wdeg, tail, up, bs, Q, R[, history])."""
+        x = np.ascontiguousarray(x, np.float64)
+        P, nsys, dim, D = x.shape
+        mb = np.ascontiguousarray(mb, np.float64)
+        dm, dx = DeviceArray.from_host(mb), DeviceArray.from_host(x)
+        df = None
+        if cff is not None and self.nbrcol > 1:
+            cff = np.ascontiguousarray(cff, np.float64)
+            df = DeviceArray.from_host(cff)
+        logs = {k: DeviceArray.zeros((nsys, nbsteps), np.int32) for k in ("wdeg", "tail", "up", "bs")}
+        logs["steps"] = DeviceArray.zeros((nsys,), np.int32)
+        logs["Q"] = DeviceArray.zeros((P, nsys, dim, dim))
+        logs["R"] = DeviceArray.zeros((P, nsys, dim, dim))
+        if history:
+            logs["history"] = DeviceArray.zeros((nbsteps, P, nsys, dim, D))
+        self.column_newton_device(K, D - 1, nbsteps, nsys, df,
+                                  df is not None and cff.shape[1] == 1 and nsys > 1, dm,
+                                  mb.shape[1] == 1 and nsys > 1, dx, logs)
         check(load().phc_synchronize())
         out = {k: v.to_host() for k, v in logs.items()}
         out["x"] = dx.to_host()

@@ -13,7 +13,18 @@ Inputs are seeded; outputs are whatever the reference functions return:
                  (CPU_dbl2_poly_evaldiff + CPU_dbl2_qrbs_solve driven as in
                  Newton/dbl2_newton_method.cpp:1447-1494), with the tail
                  indices, skip counters and the last Q and R.
-Run: python tests/golden/make_golden.py [newton]
+  columns.npz    the reference's dd column-Newton test systems
+                 (Newton/test_dbl2_newton.cpp:93-189, dbl2_column_setup,
+                 Newton/dbl2_newton_method.cpp:766-821): a monomial system
+                 of a random unimodular exponent matrix (make_monomial_system,
+                 seed 11, dimension 6, 10 row operations of multipliers below
+                 3), the two-column lower/upper triangular system of
+                 dimension 7 and the cyclic columns of dimension 6 with 6
+                 columns, degree 15, and the series after each of their 12
+                 staggered steps (CPU_dbl2_evaluate_monomials / _columns and
+                 CPU_dbl2_qrbs_solve driven as test_dbl2_column_newton,
+                 :1036-1287), with the tail indices, skip counters, Q and R.
+Run: python tests/golden/make_golden.py [newton|columns]
 """
 import os
 import sys
@@ -37,12 +48,50 @@ def newton():
                         tail=run["tail"], up=run["up"], bs=run["bs"], Q=run["Q"], R=run["R"])
 
 
+COLUMN_CASES = ("mono", "two", "cyclic")
+
+
+def column_cases():
+    """the three dd column systems of columns.npz (needs oracle/_ref)"""
+    from oraclelib import monomial_columns, ref_dbl2_column_setup, ref_monomial_system
+    A = ref_monomial_system(11, 6, 3, 10)
+    yield "mono", ref_dbl2_column_setup(3, 6, 15, 1, rowsA=A)
+    L = np.tril(np.ones((7, 7), np.int32))
+    nvr, idx, _ = monomial_columns(L)
+    ptr = np.concatenate([[0], np.cumsum(nvr)])
+    # column 1 holds the rows of L in reverse order (test_dbl2_newton.cpp:154-160)
+    nvr2 = np.concatenate([nvr, nvr[::-1]])
+    idx2 = np.concatenate([idx] + [idx[ptr[i]:ptr[i + 1]] for i in range(6, -1, -1)])
+    yield "two", ref_dbl2_column_setup(5, 7, 15, 2, nvr=nvr2, idx=idx2)
+    yield "cyclic", ref_dbl2_column_setup(7, 6, 15, 6)
+
+
+def columns():
+    from oraclelib import ref_column_newton
+    out = {}
+    for name, (csys, sol, start) in column_cases():
+        run = ref_column_newton(2, csys, start, 12)
+        out[name + "_meta"] = np.array([csys.dim, csys.deg, csys.nbrcol], np.int32)
+        out[name + "_nvr"], out[name + "_idx"] = csys.nvr, csys.idx
+        if csys.exps is not None:
+            out[name + "_exps"] = csys.exps
+        if csys.cff is not None:
+            out[name + "_cff"] = csys.cff
+        out[name + "_mb"], out[name + "_testsol"], out[name + "_start"] = csys.mb, sol, start
+        for k in ("history", "wdeg", "tail", "up", "bs", "Q", "R"):
+            out[name + "_" + k] = run[k]
+    np.savez_compressed(os.path.join(HERE, "columns.npz"), **out)
+
+
 def main():
     r = ref()
     if r is None:
         raise SystemExit("oracle/_ref/libphcref.so not built (needs the reference checkout)")
     if sys.argv[1:] == ["newton"]:
         newton()
+        return
+    if sys.argv[1:] == ["columns"]:
+        columns()
         return
     rng = np.random.default_rng(SEED)
     out = {}
@@ -88,6 +137,7 @@ def main():
             out["cst" + tag], out["cff" + tag], out["inp" + tag], out["out" + tag] = cst, cff, inp, o
     np.savez_compressed(os.path.join(HERE, "poly.npz"), **out)
     newton()
+    columns()
     print("golden vectors written to", HERE)
 
 

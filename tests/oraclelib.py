@@ -392,6 +392,125 @@ def ref_row_newton(K, sysm, start, nbsteps):
     return _row_newton(ref(), "ref_row_newton", K, sysm, start, nbsteps)
 
 
+class ColumnSystem:
+    """A real system in the column representation of the column Newton
+    method: nbrcol columns of dim monomials, monomial m = c*dim + j (equation
+    j of column c) with nvr[m] variables idx[idxptr[m]:idxptr[m+1]] (0: none);
+    nbrcol = 1 is a monomial system x^A_i = mb_i (unit coefficients, exps
+    parallel to idx); nbrcol > 1 has cff [K][nbrcol*dim][D] and the constant
+    -1 in the last equation; mb [K][dim][D] is the right-hand side."""
+
+    def __init__(self, dim, deg, nbrcol, nvr, idx, mb, cff=None, exps=None):
+        self.dim, self.deg, self.nbrcol = dim, deg, nbrcol
+        self.nvr = np.ascontiguousarray(nvr, np.int32)
+        self.idxptr = np.concatenate([[0], np.cumsum(self.nvr)]).astype(np.int32)
+        self.idx = np.ascontiguousarray(idx, np.int32)
+        self.mb = np.ascontiguousarray(mb, np.float64)
+        self.cff = None if cff is None else np.ascontiguousarray(cff, np.float64)
+        self.exps = None if exps is None else np.ascontiguousarray(exps, np.int32)
+        assert self.nvr.size == nbrcol * dim and self.idx.size == self.idxptr[-1]
+
+    def with_limbs(self, K):
+        """the same system with K limbs (limbs beyond the given ones zero)"""
+        def lk(a):
+            if a is None:
+                return None
+            out = np.zeros((K,) + a.shape[1:])
+            out[:min(K, a.shape[0])] = a[:K]
+            return out
+        return ColumnSystem(self.dim, self.deg, self.nbrcol, self.nvr, self.idx, lk(self.mb),
+                            lk(self.cff), self.exps)
+
+
+def monomial_columns(rowsA):
+    """nvr, idx, exps of the monomial system with exponent matrix rowsA:
+    row i's nonzero entries in increasing column order (the layout of
+    make_monomial_system, Newton/unimodular_matrices.cpp:150-164)"""
+    A = np.asarray(rowsA)
+    nvr = (A != 0).sum(axis=1).astype(np.int32)
+    idx = np.concatenate([np.nonzero(A[i])[0] for i in range(A.shape[0])]).astype(np.int32)
+    exps = np.concatenate([A[i][A[i] != 0] for i in range(A.shape[0])]).astype(np.int32)
+    return nvr, idx, exps
+
+
+def cyclic_columns(dim, nbrcol):
+    """nvr, idx of the cyclic columns: equation j < dim-1 of column c has the
+    j+1 variables (c + k) mod dim, the last equation only in column 0 with
+    all dim variables (make_cyclic_variables/_columns,
+    Newton/cyclic_columns.cpp:10-23)"""
+    nvr, idx = [], []
+    for c in range(nbrcol):
+        for j in range(dim):
+            n = j + 1 if j < dim - 1 else (dim if c == 0 else 0)
+            nvr.append(n)
+            idx.extend((c + k) % dim for k in range(n))
+    return np.array(nvr, np.int32), np.array(idx, np.int32)
+
+
+def ref_monomial_system(seed, dim, size, nbritr):
+    """rowsA [dim][dim] of the reference's make_monomial_system after
+    srand(seed) (needs oracle/_ref)"""
+    r = ref()
+    A = np.zeros((dim, dim), np.int32)
+    nvr = np.zeros(dim, np.int32)
+    idx = np.zeros(dim * dim, np.int32)
+    exp = np.zeros(dim * dim, np.int32)
+    r.ref_monomial_system(ctypes.c_uint(seed), dim, size, nbritr, pi(A), pi(nvr), pi(idx), pi(exp))
+    return A
+
+
+def ref_dbl2_column_setup(seed, dim, deg, nbrcol, rowsA=None, nvr=None, idx=None):
+    """the reference's dd column test setup (needs oracle/_ref): returns the
+    ColumnSystem (nbrcol = 1: the monomial system of rowsA; 2: the caller's
+    two columns; > 2: the cyclic columns), the test solution and the start"""
+    r = ref()
+    D = deg + 1
+    nv = np.zeros(nbrcol * dim, np.int32)
+    ix = np.zeros(nbrcol * dim * dim, np.int32)
+    if nbrcol == 2:
+        ptr = np.concatenate([[0], np.cumsum(nvr)])
+        for m in range(2 * dim):
+            nv[m] = nvr[m]
+            ix[m * dim:m * dim + nvr[m]] = idx[ptr[m]:ptr[m + 1]]
+    A = np.ascontiguousarray(rowsA if rowsA is not None else np.zeros((dim, dim)), np.int32)
+    cff = np.zeros((2, nbrcol * dim, D))
+    sol, mb, start = np.zeros((2, dim, D)), np.zeros((2, dim, D)), np.zeros((2, dim, D))
+    r.ref_dbl2_column_setup(ctypes.c_uint(seed), dim, deg, nbrcol, pi(A), pi(nv), pi(ix), pd(cff),
+                            pd(sol), pd(mb), pd(start))
+    if nbrcol == 1:
+        nvr_, idx_, exps = monomial_columns(A)
+        csys = ColumnSystem(dim, deg, 1, nvr_, idx_, mb, exps=exps)
+    else:
+        idx_ = np.concatenate([ix[m * dim:m * dim + nv[m]] for m in range(nbrcol * dim)])
+        csys = ColumnSystem(dim, deg, nbrcol, nv, idx_, mb, cff=cff)
+    return csys, sol, start
+
+
+def _column_newton(lib, fn, K, csys, start, nbsteps):
+    dim, D = csys.dim, csys.deg + 1
+    x = np.ascontiguousarray(start, np.float64).copy()
+    wdeg, tail = np.zeros(nbsteps, np.int32), np.zeros(nbsteps, np.int32)
+    up, bs = np.zeros(nbsteps, np.int32), np.zeros(nbsteps, np.int32)
+    hist = np.zeros((nbsteps, K, dim, D))
+    Q, R = np.zeros((K, dim, dim)), np.zeros((K, dim, dim))
+    idx = csys.idx if csys.idx.size else np.zeros(1, np.int32)
+    exps = None if csys.exps is None else pi(csys.exps)
+    cff = None if csys.cff is None else pd(csys.cff)
+    steps = getattr(lib, fn)(K, dim, csys.nbrcol, csys.deg, nbsteps, pi(csys.nvr),
+                             pi(csys.idxptr), pi(idx), exps, cff, pd(csys.mb), pd(x), pi(wdeg),
+                             pi(tail), pi(up), pi(bs), pd(hist), pd(Q), pd(R))
+    return dict(x=x, steps=steps, wdeg=wdeg[:steps], tail=tail[:steps], up=up[:steps],
+                bs=bs[:steps], history=hist[:steps], Q=Q, R=R)
+
+
+def oracle_column_newton(K, csys, start, nbsteps):
+    return _column_newton(oracle(), "onw_column_newton", K, csys, start, nbsteps)
+
+
+def ref_column_newton(K, csys, start, nbsteps):
+    return _column_newton(ref(), "ref_column_newton", K, csys, start, nbsteps)
+
+
 def _houseqr(lib, fn, K, A):
     K_, m, n = A.shape
     Q, R = np.zeros((K, m, m)), np.zeros((K, m, n))
@@ -503,6 +622,68 @@ def planted_row_system(rng, K, dim, deg, nper=8, nvar=3, supports=None, exp=None
     start = np.zeros((K, dim, D))
     start[0, :, 0] = root[0, :, 0] + 1e-10
     return sysm, root, start
+
+
+def unimodular_exponents(rng, dim, maxmul=2, nops=None):
+    """a random unimodular matrix of nonnegative exponents: the identity
+    after nops row operations row_a += m row_b (m in 1..maxmul), the
+    construction of make_unimodular_matrix (Newton/unimodular_matrices.cpp:
+    53-71) with Python's generator"""
+    A = np.eye(dim, dtype=np.int64)
+    for _ in range(dim if nops is None else nops):
+        a, b = rng.choice(dim, 2, replace=False)
+        A[a] += rng.integers(1, maxmul + 1) * A[b]
+    return A.astype(np.int32)
+
+
+def planted_column_system(rng, K, dim, deg, nbrcol, rowsA=None):
+    """a column system with a planted root: nbrcol = 1 the monomial system of
+    rowsA (default: unimodular_exponents), else the cyclic columns with
+    random constant coefficients (as make_real2_coefficients sets only the
+    leading coefficient); mb = the value of the system at the root
+    x_i(t) = c_i exp(r_i t), computed by the row evaluator of the oracle
+    (an independent code path).  Returns the system, the root and the
+    start (the root's leading coefficients plus 1e-10)."""
+    D = deg + 1
+    c = rng.uniform(0.5, 1.0, dim)
+    r = rng.uniform(-1.0, 1.0, dim)
+    fact = np.cumprod(np.concatenate([[1.0], np.arange(1, D)]))
+    root = np.zeros((K, dim, D))
+    root[0] = c[:, None] * r[:, None] ** np.arange(D)[None, :] / fact[None, :]
+    if nbrcol == 1:
+        A = unimodular_exponents(rng, dim) if rowsA is None else np.asarray(rowsA, np.int32)
+        nvr, idx, exps = monomial_columns(A)
+        rows = RowSystem(dim, deg, np.arange(dim + 1), nvr,
+                         np.concatenate([[0], np.cumsum(nvr)]), idx, np.zeros((K, dim, D)),
+                         np.concatenate([np.ones((1, dim, 1)), np.zeros((1, dim, deg))], axis=2)
+                         * (np.arange(K) == 0)[:, None, None], exps)
+        mb = poly_value(K, rows, root)
+        return ColumnSystem(dim, deg, 1, nvr, idx, mb, exps=exps), root, _start(root)
+    nvr, idx = cyclic_columns(dim, nbrcol)
+    cff = np.zeros((K, nbrcol * dim, D))
+    cff[:, :, 0] = rand_md(rng, K, (nbrcol * dim,))
+    ptr = np.concatenate([[0], np.cumsum(nvr)])
+    monoptr, rnvr, ridx, rcff = [0], [], [], []
+    for j in range(dim):
+        for col in range(nbrcol):
+            m = col * dim + j
+            if nvr[m]:
+                rnvr.append(nvr[m])
+                ridx.extend(idx[ptr[m]:ptr[m + 1]])
+                rcff.append(cff[:, m])
+        monoptr.append(len(rnvr))
+    cst = np.zeros((K, dim, D))
+    cst[0, dim - 1, 0] = -1.0
+    rows = RowSystem(dim, deg, monoptr, rnvr, np.concatenate([[0], np.cumsum(rnvr)]), ridx, cst,
+                     np.stack(rcff, axis=1))
+    mb = poly_value(K, rows, root)
+    return ColumnSystem(dim, deg, nbrcol, nvr, idx, mb, cff=cff), root, _start(root)
+
+
+def _start(root):
+    start = np.zeros_like(root)
+    start[0, :, 0] = root[0, :, 0] + 1e-10
+    return start
 
 
 # ------------------------------------------------------------ blocked QR --

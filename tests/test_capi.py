@@ -120,3 +120,13 @@ def test_invalid_arguments_are_refused_before_device_work():
                    ["degree"])
     mono = (ctypes.c_int * 2)(0, 1)
     assert refused(lib.phc_rowsys_create(ctypes.byref(h), 65, mono, nvr, mono, idx), ["dim"])
+    two = (ctypes.c_int * 2)(1, 1)
+    assert refused(lib.phc_colsys_create(ctypes.byref(h), 1, 2, two, mono, two), ["exponents"])
+    zero = (ctypes.c_int * 1)(0)
+    assert refused(lib.phc_colsys_create(ctypes.byref(h), 1, 1, zero, idx, None), ["count"])
+    rep = (ctypes.c_int * 3)(0, 0, 1)
+    two1 = (ctypes.c_int * 2)(2, 1)
+    assert refused(lib.phc_colsys_create(ctypes.byref(h), 2, 1, two1, rep, None), ["repeated"])
+    assert refused(lib.phc_column_newton(null, 2, 4, 1, 1, null, 0, null, 0, null, null, null,
+                                         null, null, null, null, null, null, null),
+                   ["not a column system"])
