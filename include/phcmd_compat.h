@@ -110,6 +110,125 @@ class AdditionJobs {
             bool verbose);
 };
 
+/* ---------------------------------------------- complex job planners ----
+ * The complex (vectorized) planners with the reference's public API
+ * (Polynomials/complexconv_job.h, complexconv_jobs.h, complexinc_job.h,
+ * complexinc_jobs.h, complexadd_job.h, complexadd_jobs.h): every real
+ * convolution splits into the four real convolutions re*re, im*im, re*im,
+ * im*re (output types 1-3 forward/backward/cross "re^a", 4-6 "re^b",
+ * 7-9 "im^a", 10-12 "im^b"; input types -1/-2 coefficient re/im, 0/4 input,
+ * 1-3/5-7 forward, backward, cross re/im), an increment job per real and
+ * imaginary part combines them, and every real addition job becomes one on
+ * the real and one on the imaginary parts, in the reference's layers and
+ * order.  GPU_cmplx*vectorized_poly_evaldiff below takes them by value. */
+class ComplexConvolutionJob {
+ public:
+  ComplexConvolutionJob(int monomialix, int input1tp, int input1ix, int input2tp, int input2ix,
+                        int outputtp, int outputix);
+  int get_monomial_index(void) const;
+  int get_first_type(void) const;
+  int get_first_input(void) const;
+  int get_second_type(void) const;
+  int get_second_input(void) const;
+  int get_output_type(void) const;
+  int get_output_index(void) const;
+  friend std::ostream &operator<<(std::ostream &os, const ComplexConvolutionJob &job);
+
+ private:
+  int monidx, inp1tp, inp1ix, inp2tp, inp2ix, outptp, outidx;
+};
+
+class ComplexConvolutionJobs {
+ public:
+  explicit ComplexConvolutionJobs(int dim);
+  void make(int nbr, int *nvr, int **idx, bool verbose);
+  int get_dimension(void) const;
+  int get_count(void) const;
+  int get_layer_count(int k) const;
+  int get_depth(void) const;
+  ComplexConvolutionJob get_job(int k, int i) const;
+  ~ComplexConvolutionJobs(void);
+
+ private:
+  int dimension, jobcount, laydepth;
+  std::vector<int> freqlaycnt;
+  std::vector<std::vector<ComplexConvolutionJob> > jobs;
+};
+
+class ComplexIncrementJob {
+ public:
+  /* kind 1 forward, 2 backward, 3 cross; isreal: the real (true) or the
+   * imaginary parts are combined */
+  ComplexIncrementJob(int monomix, int kind, int ix, bool isreal);
+  int get_monomial_index(void) const;
+  int get_increment_kind(void) const;
+  int get_increment_index(void) const;
+  bool get_isreal(void) const;
+  friend std::ostream &operator<<(std::ostream &os, const ComplexIncrementJob &job);
+
+ private:
+  int monidx, inkind, incidx;
+  bool rornot;
+};
+
+class ComplexIncrementJobs {
+ public:
+  ComplexIncrementJobs(ComplexConvolutionJobs cnvjobs, bool verbose);
+  int get_dimension(void) const;
+  int get_count(void) const;
+  int get_layer_count(int k) const;
+  int get_depth(void) const;
+  ComplexIncrementJob get_job(int k, int i) const;
+  ~ComplexIncrementJobs(void);
+
+ private:
+  int dimension, jobcount, laydepth;
+  std::vector<int> freqlaycnt;
+  std::vector<std::vector<ComplexIncrementJob> > jobs;
+  ComplexIncrementJob make_job(ComplexConvolutionJob cnvjob) const;
+};
+
+class ComplexAdditionJob {
+ public:
+  /* atp 1-3 forward/backward/cross real part, 4-6 imaginary part; itp as
+   * the output types of the convolution jobs (0 for a coefficient);
+   * monidx2 -1 or -2 for the constant (or a coefficient), real or
+   * imaginary part; ix2 -1 or -2 for the constant, real or imaginary */
+  ComplexAdditionJob(int atp, int itp, int monidx1, int monidx2, int ix1, int ix2);
+  int get_addition_type(void) const;
+  int get_increment_type(void) const;
+  int get_update_monomial(void) const;
+  int get_update_index(void) const;
+  int get_increment_monomial(void) const;
+  int get_increment_index(void) const;
+  friend std::ostream &operator<<(std::ostream &os, const ComplexAdditionJob &job);
+
+ private:
+  int adtype, intype, updmon, incmon, updidx, incidx;
+};
+
+class ComplexAdditionJobs {
+ public:
+  ComplexAdditionJobs(int dim, int nbr);
+  void make(int nbr, int *nvr, int **idx, bool verbose);
+  int get_number_of_variables(void) const;
+  int get_number_of_monomials(void) const;
+  int get_count(void) const;
+  int get_layer_count(int k) const;
+  int get_depth(void) const;
+  int get_differential_count(int k) const;
+  int get_differential_index(int k, int i) const;
+  int position(int n, int *idx, int k);
+  ComplexAdditionJob get_job(int k, int i) const;
+  ~ComplexAdditionJobs(void);
+
+ private:
+  int nbrvar, nbrmon, jobcount, laydepth;
+  std::vector<int> freqlaycnt, difcnt;
+  std::vector<std::vector<int> > difidx;
+  std::vector<std::vector<ComplexAdditionJob> > jobs;
+};
+
 /* ---------------------------------------- polynomial evaluation --------
  * Polynomials/dbl2_polynomials_kernels.h:812,870 and the dbl, dbl4, dbl8
  * twins: the value (output[dim]) and the dim derivatives (output[k]) of
@@ -179,6 +298,43 @@ void GPU_cmplx4_poly_evaldiff(
     double **outputimhihi, double **outputimlohi, double **outputimhilo,
     double **outputimlolo, ConvolutionJobs cnvjobs, AdditionJobs addjobs, double *cnvlapms,
     double *addlapms, double *elapsedms, double *walltimesec, int vrblvl);
+
+/* Polynomials/dbl_polynomials_kernels.h:735, dbl2_polynomials_kernels.h:960,
+ * dbl4_polynomials_kernels.h:1397: the complex evaluation with the
+ * vectorized schedules.  The reference's kernels split every complex
+ * product into real convolutions and agree with CPU_cmplx*_poly_evaldiff to
+ * rounding; this library returns CPU_cmplx*_poly_evaldiff's result bit for
+ * bit (the schedules are accepted and not needed); timings as above. */
+void GPU_cmplxvectorized_poly_evaldiff(int BS, int dim, int nbr, int deg, int *nvr, int **idx,
+                                       double *cstre, double *cstim, double **cffre,
+                                       double **cffim, double **inputre, double **inputim,
+                                       double **outputre, double **outputim,
+                                       ComplexConvolutionJobs cnvjobs,
+                                       ComplexIncrementJobs incjobs,
+                                       ComplexAdditionJobs addjobs, double *cnvlapms,
+                                       double *addlapms, double *elapsedms,
+                                       double *walltimesec, int vrblvl);
+void GPU_cmplx2vectorized_poly_evaldiff(
+    int BS, int dim, int nbr, int deg, int *nvr, int **idx, double *cstrehi, double *cstrelo,
+    double *cstimhi, double *cstimlo, double **cffrehi, double **cffrelo, double **cffimhi,
+    double **cffimlo, double **inputrehi, double **inputrelo, double **inputimhi,
+    double **inputimlo, double **outputrehi, double **outputrelo, double **outputimhi,
+    double **outputimlo, ComplexConvolutionJobs cnvjobs, ComplexIncrementJobs incjobs,
+    ComplexAdditionJobs addjobs, double *cnvlapms, double *addlapms, double *elapsedms,
+    double *walltimesec, int vrblvl);
+void GPU_cmplx4vectorized_poly_evaldiff(
+    int BS, int dim, int nbr, int deg, int *nvr, int **idx, double *cstrehihi,
+    double *cstrelohi, double *cstrehilo, double *cstrelolo, double *cstimhihi,
+    double *cstimlohi, double *cstimhilo, double *cstimlolo, double **cffrehihi,
+    double **cffrelohi, double **cffrehilo, double **cffrelolo, double **cffimhihi,
+    double **cffimlohi, double **cffimhilo, double **cffimlolo, double **inputrehihi,
+    double **inputrelohi, double **inputrehilo, double **inputrelolo, double **inputimhihi,
+    double **inputimlohi, double **inputimhilo, double **inputimlolo, double **outputrehihi,
+    double **outputrelohi, double **outputrehilo, double **outputrelolo,
+    double **outputimhihi, double **outputimlohi, double **outputimhilo,
+    double **outputimlolo, ComplexConvolutionJobs cnvjobs, ComplexIncrementJobs incjobs,
+    ComplexAdditionJobs addjobs, double *cnvlapms, double *addlapms, double *elapsedms,
+    double *walltimesec, int vrblvl);
 
 /* ------------------------------------------------ series products ------
  * Convolutions/dbl2_convolutions_kernels.h:121,145 and the dbl, dbl4, dbl8

@@ -865,3 +865,58 @@ void GPU_cmplx4_poly_evaldiff(
   poly_evaldiff(4, 1, dim, nbr, deg, nvr, idx, c, f, x, y, cnvlapms, addlapms, elapsedms,
                 walltimesec, "GPU_cmplx4_poly_evaldiff");
 }
+
+// the vectorized complex evaluation (Polynomials/dbl_polynomials_kernels.h:735,
+// dbl2_polynomials_kernels.h:960, dbl4_polynomials_kernels.h:1397): the same
+// value and derivatives, with CPU_cmplx*_poly_evaldiff's operation order
+void GPU_cmplxvectorized_poly_evaldiff(int, int dim, int nbr, int deg, int *nvr, int **idx,
+                                       double *cstre, double *cstim, double **cffre,
+                                       double **cffim, double **inputre, double **inputim,
+                                       double **outputre, double **outputim,
+                                       ComplexConvolutionJobs, ComplexIncrementJobs,
+                                       ComplexAdditionJobs, double *cnvlapms, double *addlapms,
+                                       double *elapsedms, double *walltimesec, int) {
+  double *c[] = {cstre, cstim};
+  double **f[] = {cffre, cffim}, **x[] = {inputre, inputim}, **y[] = {outputre, outputim};
+  poly_evaldiff(1, 1, dim, nbr, deg, nvr, idx, c, f, x, y, cnvlapms, addlapms, elapsedms,
+                walltimesec, "GPU_cmplxvectorized_poly_evaldiff");
+}
+
+void GPU_cmplx2vectorized_poly_evaldiff(
+    int, int dim, int nbr, int deg, int *nvr, int **idx, double *cstrehi, double *cstrelo,
+    double *cstimhi, double *cstimlo, double **cffrehi, double **cffrelo, double **cffimhi,
+    double **cffimlo, double **inputrehi, double **inputrelo, double **inputimhi,
+    double **inputimlo, double **outputrehi, double **outputrelo, double **outputimhi,
+    double **outputimlo, ComplexConvolutionJobs, ComplexIncrementJobs, ComplexAdditionJobs,
+    double *cnvlapms, double *addlapms, double *elapsedms, double *walltimesec, int) {
+  double *c[] = {cstrehi, cstrelo, cstimhi, cstimlo};
+  double **f[] = {cffrehi, cffrelo, cffimhi, cffimlo};
+  double **x[] = {inputrehi, inputrelo, inputimhi, inputimlo};
+  double **y[] = {outputrehi, outputrelo, outputimhi, outputimlo};
+  poly_evaldiff(2, 1, dim, nbr, deg, nvr, idx, c, f, x, y, cnvlapms, addlapms, elapsedms,
+                walltimesec, "GPU_cmplx2vectorized_poly_evaldiff");
+}
+
+void GPU_cmplx4vectorized_poly_evaldiff(
+    int, int dim, int nbr, int deg, int *nvr, int **idx, double *cstrehihi, double *cstrelohi,
+    double *cstrehilo, double *cstrelolo, double *cstimhihi, double *cstimlohi,
+    double *cstimhilo, double *cstimlolo, double **cffrehihi, double **cffrelohi,
+    double **cffrehilo, double **cffrelolo, double **cffimhihi, double **cffimlohi,
+    double **cffimhilo, double **cffimlolo, double **inputrehihi, double **inputrelohi,
+    double **inputrehilo, double **inputrelolo, double **inputimhihi, double **inputimlohi,
+    double **inputimhilo, double **inputimlolo, double **outputrehihi, double **outputrelohi,
+    double **outputrehilo, double **outputrelolo, double **outputimhihi, double **outputimlohi,
+    double **outputimhilo, double **outputimlolo, ComplexConvolutionJobs, ComplexIncrementJobs,
+    ComplexAdditionJobs, double *cnvlapms, double *addlapms, double *elapsedms,
+    double *walltimesec, int) {
+  double *c[] = {cstrehihi, cstrelohi, cstrehilo, cstrelolo,
+                 cstimhihi, cstimlohi, cstimhilo, cstimlolo};
+  double **f[] = {cffrehihi, cffrelohi, cffrehilo, cffrelolo,
+                  cffimhihi, cffimlohi, cffimhilo, cffimlolo};
+  double **x[] = {inputrehihi, inputrelohi, inputrehilo, inputrelolo,
+                  inputimhihi, inputimlohi, inputimhilo, inputimlolo};
+  double **y[] = {outputrehihi, outputrelohi, outputrehilo, outputrelolo,
+                  outputimhihi, outputimlohi, outputimhilo, outputimlolo};
+  poly_evaldiff(4, 1, dim, nbr, deg, nvr, idx, c, f, x, y, cnvlapms, addlapms, elapsedms,
+                walltimesec, "GPU_cmplx4vectorized_poly_evaldiff");
+}

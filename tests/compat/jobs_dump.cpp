@@ -1,5 +1,7 @@
 // Prints the convolution and addition job schedules of seeded random
-// supports through the public API of ConvolutionJobs and AdditionJobs.
+// supports through the public API of ConvolutionJobs and AdditionJobs, and
+// of their complex (vectorized) versions ComplexConvolutionJobs,
+// ComplexIncrementJobs and ComplexAdditionJobs.
 // Built twice: against this library's classes (include/phcmd_compat.h,
 // libphcmd.so) and, where the reference sources are present, against the
 // reference's own classes (Polynomials/convolution_jobs.h, addition_jobs.h,
@@ -15,6 +17,9 @@
 #ifdef REFERENCE_JOBS
 #include "convolution_jobs.h"
 #include "addition_jobs.h"
+#include "complexconv_jobs.h"
+#include "complexinc_jobs.h"
+#include "complexadd_jobs.h"
 #else
 #include "phcmd_compat.h"
 #endif
@@ -66,6 +71,54 @@ void dump(int dim, int nbr, std::vector<int> &nvr, std::vector<std::vector<int>>
     std::cout << " layer " << k << " count " << add.get_layer_count(k) << "\n";
     for (int i = 0; i < add.get_layer_count(k); ++i) {
       const AdditionJob j = add.get_job(k, i);
+      std::cout << "  " << j.get_addition_type() << " " << j.get_increment_type() << " "
+                << j.get_update_monomial() << " " << j.get_update_index() << " "
+                << j.get_increment_monomial() << " " << j.get_increment_index() << " | " << j
+                << "\n";
+    }
+  }
+  ComplexConvolutionJobs ccnv(dim);
+  ccnv.make(nbr, nvr.data(), idx.data(), false);
+  std::cout << "cplx conv dim " << ccnv.get_dimension() << " count " << ccnv.get_count()
+            << " depth " << ccnv.get_depth() << "\n";
+  for (int k = 0; k < ccnv.get_depth(); ++k) {
+    std::cout << " layer " << k << " count " << ccnv.get_layer_count(k) << "\n";
+    for (int i = 0; i < ccnv.get_layer_count(k); ++i) {
+      const ComplexConvolutionJob j = ccnv.get_job(k, i);
+      std::cout << "  " << j.get_monomial_index() << " " << j.get_first_type() << " "
+                << j.get_first_input() << " " << j.get_second_type() << " "
+                << j.get_second_input() << " " << j.get_output_type() << " "
+                << j.get_output_index() << " | " << j << "\n";
+    }
+  }
+  if (ccnv.get_depth() <= dim) {  // the reference's increment planner keeps dim layers
+    ComplexIncrementJobs cinc(ccnv, false);
+    std::cout << "cplx inc dim " << cinc.get_dimension() << " count " << cinc.get_count()
+              << " depth " << cinc.get_depth() << "\n";
+    for (int k = 0; k < cinc.get_depth(); ++k) {
+      std::cout << " layer " << k << " count " << cinc.get_layer_count(k) << "\n";
+      for (int i = 0; i < cinc.get_layer_count(k); ++i) {
+        const ComplexIncrementJob j = cinc.get_job(k, i);
+        std::cout << "  " << j.get_monomial_index() << " " << j.get_increment_kind() << " "
+                  << j.get_increment_index() << " " << j.get_isreal() << " | " << j << "\n";
+      }
+    }
+  }
+  ComplexAdditionJobs cadd(dim, nbr);
+  cadd.make(nbr, nvr.data(), idx.data(), false);
+  std::cout << "cplx add vars " << cadd.get_number_of_variables() << " mons "
+            << cadd.get_number_of_monomials() << " count " << cadd.get_count() << " depth "
+            << cadd.get_depth() << "\n";
+  for (int v = 0; v < dim; ++v) {
+    std::cout << " var " << v << " difcount " << cadd.get_differential_count(v) << " :";
+    for (int i = 0; i <= cadd.get_differential_count(v); ++i)
+      std::cout << " " << cadd.get_differential_index(v, i);
+    std::cout << "\n";
+  }
+  for (int k = 0; k < cadd.get_depth(); ++k) {
+    std::cout << " layer " << k << " count " << cadd.get_layer_count(k) << "\n";
+    for (int i = 0; i < cadd.get_layer_count(k); ++i) {
+      const ComplexAdditionJob j = cadd.get_job(k, i);
       std::cout << "  " << j.get_addition_type() << " " << j.get_increment_type() << " "
                 << j.get_update_monomial() << " " << j.get_update_index() << " "
                 << j.get_increment_monomial() << " " << j.get_increment_index() << " | " << j

@@ -256,6 +256,41 @@ static void test_poly_evaldiff(int K, int cplx, int dim, int nbr, int maxnvr, in
     std::printf("%s: timers not set\n", name);
     ++failures;
   }
+  if (!cplx) return;
+  // the vectorized complex evaluation, with the complex schedules
+  ComplexConvolutionJobs ccnv(dim);
+  ccnv.make(nbr, nvr.data(), idx.data(), false);
+  ComplexIncrementJobs cinc(ccnv, false);
+  ComplexAdditionJobs cadd(dim, nbr);
+  cadd.make(nbr, nvr.data(), idx.data(), false);
+  Planes outv(P, dim + 1, D);
+  for (int q = 0; q < P; ++q) y[q] = outv.p(q);
+  cnv = add = ela = wall = -1.0;
+  if (K == 1) {
+    GPU_cmplxvectorized_poly_evaldiff(D, dim, nbr, deg, nvr.data(), idx.data(), c[0], c[1], f[0],
+                                      f[1], x[0], x[1], y[0], y[1], ccnv, cinc, cadd, &cnv, &add,
+                                      &ela, &wall, 0);
+    name = "GPU_cmplxvectorized_poly_evaldiff";
+  } else if (K == 2) {
+    GPU_cmplx2vectorized_poly_evaldiff(D, dim, nbr, deg, nvr.data(), idx.data(), c[0], c[1], c[2],
+                                       c[3], f[0], f[1], f[2], f[3], x[0], x[1], x[2], x[3], y[0],
+                                       y[1], y[2], y[3], ccnv, cinc, cadd, &cnv, &add, &ela, &wall,
+                                       0);
+    name = "GPU_cmplx2vectorized_poly_evaldiff";
+  } else {
+    GPU_cmplx4vectorized_poly_evaldiff(
+        D, dim, nbr, deg, nvr.data(), idx.data(), c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7],
+        f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], x[0], x[1], x[2], x[3], x[4], x[5], x[6],
+        x[7], y[0], y[1], y[2], y[3], y[4], y[5], y[6], y[7], ccnv, cinc, cadd, &cnv, &add, &ela,
+        &wall, 0);
+    name = "GPU_cmplx4vectorized_poly_evaldiff";
+  }
+  std::snprintf(what, sizeof what, "%s dim %d nbr %d deg %d (bitwise)", name, dim, nbr, deg);
+  check(outv.same(outc), what, outv.diff(K, outc), 0.0);
+  if (!(cnv >= 0.0 && add == 0.0 && ela == cnv && wall > 0.0)) {
+    std::printf("%s: timers not set\n", name);
+    ++failures;
+  }
 }
 
 // ------------------------------------------------------------ QR, solver
