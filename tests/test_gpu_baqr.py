@@ -46,10 +46,12 @@ def test_blocked_qr_matches_oracle(K, shape):
     assert dq < tol and dr < tol * np.sqrt(m)
 
 
-@pytest.mark.parametrize("K,shape", [(2, (256, 128, 128)), (4, (128, 128, 64))])
+@pytest.mark.parametrize("K,shape", [(2, (256, 128, 128)), (4, (128, 128, 64)),
+                                     (4, (128, 128, 128))])
 def test_blocked_qr_split_sums_match_oracle(K, shape):
     """sizes where the Gram, W and Q-update products split their k range over
-    workgroups (k >= 128) and the panel spans up to 128 columns"""
+    workgroups (k >= 128) and the panel spans up to 128 columns (the last:
+    the quad-double tile of 128 columns of the 1024 x 1024 solve)"""
     test_blocked_qr_matches_oracle(K, shape)
 
 
