@@ -236,10 +236,12 @@ int phc_compiled_destroy(phc_compiled *compiled);
  * nfeat -> 64 (ReLU) -> 64 (ReLU) -> ncand logits (phc_mlp layout).
  * mode 0: plain rule (the controller is ignored);
  * mode 1: learned: the largest c whose logit >= logit_threshold (c = 0 if none);
- *         a path that does not finish (status other than success) is
- *         tracked again from its start with the plain rule in the same
- *         launch: its work counters add up both attempts, its other outputs
- *         are the second attempt's;
+ *         a path that does not finish (status other than success) at a
+ *         t below retry_below (0: 0.99) is tracked again from its start with
+ *         the plain rule in the same launch: its work counters add up both
+ *         attempts, its other outputs are the second attempt's; a path that
+ *         stops closer to t = 1 (a singular or diverging endpoint, which the
+ *         plain rule does not reach either) is not tracked again;
  * mode 2: explore: c drawn from a hash of (seed, path, step), and every step
  *         logged: features log_x[path][i][nfeat], candidate log_c[path][i],
  *         log_y[path][i] = the Newton iterations of the step if its first
@@ -272,6 +274,7 @@ typedef struct {
   double pole_cap;                /* 0: no cap */
   int deg0;                       /* series degree before the choice (0: the program's) */
   int degree[PHC_CTL_MAXCAND];    /* series degree of candidate c's step (0: the program's) */
+  double retry_below;             /* mode 1: retry only paths stopped at t < this (0: 0.99) */
 } phc_controller;
 
 typedef struct phc_tracker phc_tracker;

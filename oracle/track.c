@@ -974,9 +974,11 @@ static void track_path(const otr_program *P, work_t *w, int npaths, int path, co
   int st = ST_RUNNING, steps = 0, rej = 0, j, k, npred = 0, nnewt = 0, nfeval = 0, ncorr = 0;
   int nbpole = 0, nbhess = 0, nbmaxm = 0, nfirst = 0;
   int rejprev = 0, nlog = 0, attempt, pmode = mode;
-  /* a path the learned controller (mode 1) does not finish is tracked
-     again from its start with the plain rule; the work counters add up
-     both attempts, the other outputs are the last attempt's */
+  /* a path the learned controller (mode 1) does not finish, stopped at a
+     t below retry_below (0: 0.99), is tracked again from its start with
+     the plain rule; the work counters add up both attempts, the other
+     outputs are the last attempt's */
+  const double retry = (ctl && ctl->retry_below > 0.0) ? ctl->retry_below : 0.99;
   for (attempt = 0; attempt < 2; attempt++) {
     pmode = (attempt == 0) ? mode : 0;
     t = tstart ? tstart[path] : 0.0;
@@ -1128,7 +1130,7 @@ static void track_path(const otr_program *P, work_t *w, int npaths, int path, co
       nr = lu_newton_steps(P, w, prm->endsteps, prm->tolend, 1, 0, &dxl, &nnewt, &nfirst, NULL);
       st = (nr < 0) ? ST_SINGULAR : ST_SUCCESS;
     }
-    if (!(pmode == 1 && st != ST_SUCCESS)) break;
+    if (!(pmode == 1 && st != ST_SUCCESS && t < retry)) break;
   }
   store_point(P, w, endp, npaths, path);
   tend[path] = t;

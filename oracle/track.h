@@ -52,6 +52,7 @@ typedef struct {
   double pole_cap;
   int deg0;                 /* series degree before the choice (0: the program's) */
   int degree[OTR_MAXCAND];  /* series degree of candidate c's step (0: the program's) */
+  double retry_below;       /* mode 1: retry only paths stopped at t < this (0: 0.99) */
 } otr_controller;
 
 /* paths p0 <= p < p1 of npaths; arrays as in phc_track_from (tstart NULL:

@@ -84,7 +84,7 @@ class PhcController(ctypes.Structure):
         ("logit_threshold", ctypes.c_float), ("seed", ctypes.c_uint), ("log_capacity", c_int),
         ("log_x", c_void_p), ("log_c", c_void_p), ("log_y", c_void_p), ("log_n", c_void_p),
         ("pole_cap", c_double),
-        ("deg0", c_int), ("degree", c_int * CTL_MAXCAND),
+        ("deg0", c_int), ("degree", c_int * CTL_MAXCAND), ("retry_below", c_double),
     ]
 
 

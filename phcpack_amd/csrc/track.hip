@@ -107,6 +107,7 @@ struct Ctl {
   // candidate c (dc[c] >= d0), at most the program's deg + 1
   int d0;
   int dc[PHC_CTL_MAXCAND];
+  double retry;  // mode 1: paths stopped at t < retry are tracked again
 };
 
 // log2 of x > 0 from the binary exponent and the series
@@ -1265,9 +1266,10 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, const Para
     int st = ST_RUNNING, steps = 0, rej = 0, npred = 0, nnewt = 0, nfeval = 0, rejprev = 0;
     int ncorr = 0, nbpole = 0, nbhess = 0, nbmaxm = 0, nfirst = 0;
     int nlog = 0, pmode = ctl.mode;
-    // a path the learned controller (mode 1) does not finish is tracked
-    // again from its start with the plain rule; the work counters add up
-    // both attempts, the other outputs are the last attempt's
+    // a path the learned controller (mode 1) does not finish, stopped at a
+    // t below ctl.retry, is tracked again from its start with the plain
+    // rule; the work counters add up both attempts, the other outputs are
+    // the last attempt's
     for (int attempt = 0; attempt < 2; ++attempt) {
     pmode = (attempt == 0) ? ctl.mode : 0;
     t = tstart ? tstart[path] : 0.0;
@@ -1409,7 +1411,7 @@ __global__ void __launch_bounds__(64 * MW) track_kernel(DevProgram G, const Para
           lu_newton_steps<K>(P, w, par.endsteps, par.tolend, false, &dxl, &nnewt, &nfirst, nullptr);
       st = (nr < 0) ? ST_SINGULAR : ST_SUCCESS;
     }
-    if (!(pmode == 1 && st != ST_SUCCESS)) break;
+    if (!(pmode == 1 && st != ST_SUCCESS && t < ctl.retry)) break;
     }
     store_point<K>(P, w, endp, npaths, path);
     if (l == 0) {
@@ -1891,6 +1893,7 @@ int phc_track_from(phc_tracker *tr, int K, int npaths, const double *start, int 
     ctl.dc[c] = tr->P.D;
   }
   ctl.d0 = tr->P.D;
+  ctl.retry = 0.99;
   if (ctlp && ctlp->mode != 0) {
     PHC_REQUIRE(ctlp->mode == 1 || ctlp->mode == 2, "phc_track: controller mode must be 0, 1 or 2");
     PHC_REQUIRE(ctlp->ncand >= 1 && ctlp->ncand <= PHC_CTL_MAXCAND, "phc_track: bad ncand");
@@ -1909,6 +1912,8 @@ int phc_track_from(phc_tracker *tr, int K, int npaths, const double *start, int 
     ctl.log_y = ctlp->log_y; ctl.log_n = ctlp->log_n;
     PHC_REQUIRE(ctlp->pole_cap >= 0.0, "phc_track: pole_cap must be >= 0");
     ctl.pcap = ctlp->pole_cap;
+    PHC_REQUIRE(ctlp->retry_below >= 0.0, "phc_track: retry_below must be >= 0");
+    if (ctlp->retry_below > 0.0) ctl.retry = ctlp->retry_below;
     // series degrees: before the choice and per candidate (0: the program's)
     const int deg = tr->P.D - 1;
     auto usable = [&](int d) {

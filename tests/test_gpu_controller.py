@@ -123,6 +123,35 @@ def test_unfinished_learned_paths_fall_back_to_the_plain_rule(K):
     assert np.array_equal(g["work"][0], plain["work"][0] + prm.maxsteps)
 
 
+@pytest.mark.parametrize("K", [1, 2])
+def test_paths_stopped_late_are_not_tracked_again(K):
+    """The same quartered-step controller with retry_below = 0.05: the paths
+    stop (maxsteps) beyond t = 0.05, so none is tracked again: every status
+    is maxsteps, the predictor work is the first attempt's alone; GPU and
+    oracle agree bit for bit."""
+    from phcpack_amd.controller import StepController, nparams
+    from phcpack_amd.tracker import Tracker, TrackParams, points_to_planes
+    hom, sols = _system(4, 2, 10)
+    tr = Tracker(hom, deg=8)
+    base = TrackParams(tolend=1e-13 if K == 1 else 1e-28)
+    plain = tr.track(sols, K=K, params=base)
+    prm = TrackParams(tolend=base.tolend, maxsteps=int(1.5 * plain["nsteps"].max()) + 1)
+    mults = (1.0, 0.25)
+    w = np.zeros(nparams(C=len(mults)), np.float32)
+    w[-len(mults):] = 10.0
+    ctl = StepController(weights=w, threshold=0.5, mults=mults, pole_cap=0.0)
+    cs = ctl.struct(1)
+    cs.retry_below = 0.05
+    g = tr.track(sols, K=K, params=prm, controller=cs)
+    oc = OracleController(1, len(sols), ctl=ctl)
+    oc.struct.retry_below = 0.05
+    o = oracle_track(tr.prog, points_to_planes(sols), K, prm, controller=oc)
+    for k in ("endp", "status", "nsteps", "nrej", "work", "tend"):
+        assert np.array_equal(g[k], o[k]), k
+    assert np.all(g["status"] != 1) and np.all(g["tend"] >= 0.05) and np.all(g["tend"] < 1.0)
+    assert np.all(g["work"][0] == prm.maxsteps)
+
+
 def test_trained_controller_gains_over_plain_rule():
     """A/B on held-out systems: the controller trained (on the GPU, by the
     library's own kernels) on explore logs of other systems of the family
