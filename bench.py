@@ -847,6 +847,14 @@ def run_rankdef(args):
         "data": "synthetic", "config": {"workload": blk["workload"], "residual_tol": RES_TOL},
         "rankdef": blk, "roofline": None,
     }
+    head = blk.get("learned_mixed") or blk.get("mixed") or {}
+    if head.get("fp64_tflops") is not None:
+        # the FP64 tracking launch of the headline method, flops counted as for
+        # the 12-quadratics line (phcpack_amd/flops.py, FMA = 2)
+        line["roofline"] = {"bound": "fp64-valu", "achieved": head["fp64_tflops"], "peak": 78.6,
+                            "unit": "TFLOP/s", "frac": round(head["fp64_tflops"] / 78.6, 4),
+                            "traffic": None,
+                            "kernel": "track_kernel<1> (FP64 tracking, 16384 paths)"}
     print(json.dumps(line), flush=True)
     return 0
 
