@@ -935,6 +935,10 @@ def run_learned(args, n=8, degree=3, ntrain=10, neval=5, seed=SEED):
                      "lost_paths_vs_plain": lost})
         tot["t_plain"] += ms_p
         tot["t_learn"] += ms_l
+        # FP64 flops of the learned tracking launches (phcpack_amd/flops.py)
+        from phcpack_amd.flops import tracker_flops
+        tot["fl_learn"] = tot.get("fl_learn", 0.0) + tracker_flops(tr.prog, 1, ol["work"])
+        tot["tt_learn"] = tot.get("tt_learn", 0.0) + float(np.mean([r[0] for r in tl]))
         tot["good_plain"] += int(ok_p.sum())
         tot["good_learn"] += int(ok_l.sum())
         tot["lost"] += lost
@@ -964,6 +968,11 @@ def run_learned(args, n=8, degree=3, ntrain=10, neval=5, seed=SEED):
         "training": info,
         "roofline": None,
     }
+    if tot.get("tt_learn", 0.0) > 0.0:
+        ach = tot["fl_learn"] / (tot["tt_learn"] / 1e3) / 1e12
+        line["roofline"] = {"bound": "fp64-valu", "achieved": round(ach, 3), "peak": 78.6,
+                            "unit": "TFLOP/s", "frac": round(ach / 78.6, 4), "traffic": None,
+                            "kernel": "track_kernel<1> (FP64 tracking, learned controller)"}
     print(json.dumps(line), flush=True)
     return 0
 
